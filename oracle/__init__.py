@@ -1,0 +1,314 @@
+"""ctypes front-end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, where it is the checker.  The product package
+(piro_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+UW, DW, CD = 1, 2, 3
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "po_lap_norm": (C.c_float, [C.c_float, C.c_int]),
+        "po_grad_norm": (C.c_float, [C.c_float, C.c_int]),
+        "po_lap_nnz": (C.c_int64, [C.c_int, C.c_int, C.c_int]),
+        "po_lap_csr": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _i32p, _i32p, _f32p]),
+        "po_lap_csr_scan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _i32p, _i32p, _f32p]),
+        "po_row_scale": (None, [C.c_int, _i32p, _f32p, _f32p]),
+        "po_csr_to_coo_rows": (None, [C.c_int, _i32p, _i32p]),
+        "po_ht_table_size": (C.c_int, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "po_ht_build": (C.c_int, [C.c_int, _i32p, _i32p, _f32p, C.c_int, _i32p, _f32p]),
+        "po_ht_lookup": (C.c_float, [C.c_int, _i32p, _f32p, C.c_int]),
+        "po_grad_nnz": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int]),
+        "po_grad_csr": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _i32p, _i32p, _f32p]),
+        "po_geam3_count": (None, [C.c_int, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]),
+        "po_exclusive_scan": (None, [C.c_int, _i32p, _i32p]),
+        "po_geam3_fill": (None, [C.c_int, _i32p, _i32p, _f32p, _i32p, _i32p, _f32p, _i32p, _i32p, _f32p,
+                                 _f32p, _i32p, _i32p, _f32p]),
+        "po_geam2": (C.c_int64, [C.c_int, C.c_float, _i32p, _i32p, _f32p, C.c_float, _i32p, _i32p, _f32p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p]),
+        "po_face_count": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+        "po_face_indices": (C.c_int, [C.c_int, C.c_int, C.c_int, _i32p]),
+        "po_fe_csr_step": (None, [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f32p, C.c_float, _f32p,
+                                  C.c_void_p, _f32p]),
+        "po_fe_csr_step_f64": (None, [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f32p, C.c_double, _f64p,
+                                      C.c_void_p, _f64p]),
+        "po_fe_csr_run": (None, [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f32p, C.c_float, C.c_int, _f32p,
+                                 C.c_void_p, _f32p, _f32p]),
+        "po_fe_csr_run_f64": (None, [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f32p, C.c_double, C.c_int,
+                                     _f32p, C.c_void_p, _f64p, _f64p]),
+        "po_spmv_allrows_step": (None, [C.c_int, _i32p, _i32p, _f32p, C.c_float, _f32p, _f32p]),
+        "po_fe_ht_step": (None, [C.c_int, _i32p, _f32p, C.c_int, C.c_float, _f32p, _f32p]),
+        "po_jli": (C.c_int, [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f32p, C.c_float, _f32p, C.c_float,
+                             C.c_float, C.c_int, _f32p, _f32p, C.POINTER(C.c_float)]),
+        "po_ic_gaussian": (None, [C.c_int, C.c_int, C.c_int, _f32p, _f32p, _f32p, C.c_float, _f32p]),
+        "po_ic_tophat": (None, [C.c_int, C.c_int, C.c_int, _f32p, _f32p, C.c_float, _f32p]),
+        "po_hash_uniform": (C.c_float, [C.c_uint64, C.c_uint64]),
+        "po_alpha_uniform": (None, [C.c_int64, C.c_uint64, C.c_float, C.c_float, _f32p]),
+        "po_sum_f64": (C.c_double, [C.c_int64, _f32p]),
+        "po_sumsq_f64": (C.c_double, [C.c_int64, _f32p]),
+        "po_dot_f64": (C.c_double, [C.c_int64, _f32p, _f32p]),
+        "po_sum_numbers_f32": (C.c_float, [C.c_int64, _f32p]),
+        "po_lap2d_nnz": (C.c_int64, [C.c_int]),
+        "po_lap2d_csr": (C.c_int, [C.c_int, C.c_float, _i32p, _i32p, _f32p]),
+        "po_csr_to_dense_f64": (None, [C.c_int, C.c_int, _i32p, _i32p, _f32p, _f64p]),
+        "po_lu_factor_f64": (C.c_int, [C.c_int, _f64p, _i32p]),
+        "po_lu_solve_f64": (None, [C.c_int, _f64p, _i32p, C.c_int, _f64p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# --------------------------------------------------------------- wrappers
+
+class CSR:
+    """Plain host CSR triple (int32 row_ptr/col, float32 val)."""
+
+    def __init__(self, row_ptr, col, val):
+        self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float32)
+
+    @property
+    def n_rows(self):
+        return self.row_ptr.size - 1
+
+    @property
+    def nnz(self):
+        return self.col.size
+
+    def dense(self, n_cols=None):
+        n_cols = self.n_rows if n_cols is None else n_cols
+        d = np.zeros(self.n_rows * n_cols, dtype=np.float64)
+        lib.po_csr_to_dense_f64(self.n_rows, n_cols, self.row_ptr, self.col, self.val, d)
+        return d.reshape(self.n_rows, n_cols)
+
+
+def lap_norm(l0, n0):
+    return np.float32(lib.po_lap_norm(np.float32(l0), int(n0)))
+
+
+def grad_norm(l0, n0):
+    return np.float32(lib.po_grad_norm(np.float32(l0), int(n0)))
+
+
+def lap_csr(nx, ny, nz, scheme, norm, scan=False):
+    N = nx * ny * nz
+    nnz = lib.po_lap_nnz(nx, ny, nz)
+    rp = np.zeros(N + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float32)
+    fn = lib.po_lap_csr_scan if scan else lib.po_lap_csr
+    if fn(nx, ny, nz, scheme, np.float32(norm), rp, col, val) != 0:
+        raise ValueError("bad scheme")
+    return CSR(rp, col, val)
+
+
+def grad_csr(nx, ny, nz, scheme, d, norm):
+    N = nx * ny * nz
+    nnz = lib.po_grad_nnz(nx, ny, nz, d)
+    rp = np.zeros(N + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float32)
+    if lib.po_grad_csr(nx, ny, nz, scheme, d, np.float32(norm), rp, col, val) != 0:
+        raise ValueError("bad scheme/dir")
+    return CSR(rp, col, val)
+
+
+def row_scale(A: CSR, diag):
+    val = A.val.copy()
+    lib.po_row_scale(A.n_rows, A.row_ptr, val, np.ascontiguousarray(diag, np.float32))
+    return CSR(A.row_ptr, A.col, val)
+
+
+def coo_rows(A: CSR):
+    rows = np.zeros(A.nnz, np.int32)
+    lib.po_csr_to_coo_rows(A.n_rows, A.row_ptr, rows)
+    return rows
+
+
+def ht_table_size(sparsecount, N, a=3.3608, b=-0.5552, c=-0.0143, lf=0.7):
+    return lib.po_ht_table_size(int(sparsecount), int(N), np.float32(a), np.float32(b), np.float32(c),
+                                np.float32(lf))
+
+
+def ht_build(A: CSR, ts):
+    keys = np.full(ts, -1, np.int32)
+    vals = np.zeros(ts, np.float32)
+    if lib.po_ht_build(A.n_rows, A.row_ptr, A.col, A.val, ts, keys, vals) != 0:
+        raise RuntimeError("hash table full")
+    return keys, vals
+
+
+def geam3(Ga: CSR, Gb: CSR, Gc: CSR, v3):
+    n = Ga.n_rows
+    cnt = np.zeros(n, np.int32)
+    lib.po_geam3_count(n, Ga.row_ptr, Ga.col, Gb.row_ptr, Gb.col, Gc.row_ptr, Gc.col, cnt)
+    rp = np.zeros(n + 1, np.int32)
+    lib.po_exclusive_scan(n, cnt, rp)
+    nnz = int(rp[-1])
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float32)
+    lib.po_geam3_fill(n, Ga.row_ptr, Ga.col, Ga.val, Gb.row_ptr, Gb.col, Gb.val, Gc.row_ptr, Gc.col, Gc.val,
+                      np.ascontiguousarray(v3, np.float32), rp, col, val)
+    return CSR(rp, col, val)
+
+
+def geam2(alpha, A: CSR, beta, B: CSR):
+    n = A.n_rows
+    nnz = lib.po_geam2(n, np.float32(alpha), A.row_ptr, A.col, A.val, np.float32(beta), B.row_ptr, B.col,
+                       B.val, None, None, None)
+    if nnz < 0:
+        raise ValueError("unsorted rows")
+    rp = np.zeros(n + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float32)
+    lib.po_geam2(n, np.float32(alpha), A.row_ptr, A.col, A.val, np.float32(beta), B.row_ptr, B.col, B.val,
+                 _ptr(rp), _ptr(col), _ptr(val))
+    return CSR(rp, col, val)
+
+
+def face_indices(nx, ny, nz):
+    q = lib.po_face_count(nx, ny, nz)
+    idx = np.zeros(max(q, 1), np.int32)
+    got = lib.po_face_indices(nx, ny, nz, idx)
+    assert got == q
+    return idx[:q]
+
+
+def fe_csr_step(dims, A: CSR, dt, x, snap=None):
+    nx, ny, nz = dims
+    y = np.empty(nx * ny * nz, np.float32)
+    s = None if snap is None else np.ascontiguousarray(snap, np.float32)
+    lib.po_fe_csr_step(nx, ny, nz, A.row_ptr, A.col, A.val, np.float32(dt),
+                       np.ascontiguousarray(x, np.float32), _ptr(s), y)
+    return y
+
+
+def fe_csr_run(dims, A: CSR, dt, nsteps, x0, snap=None):
+    nx, ny, nz = dims
+    out = np.empty(nx * ny * nz, np.float32)
+    work = np.empty_like(out)
+    s = None if snap is None else np.ascontiguousarray(snap, np.float32)
+    lib.po_fe_csr_run(nx, ny, nz, A.row_ptr, A.col, A.val, np.float32(dt), int(nsteps),
+                      np.ascontiguousarray(x0, np.float32), _ptr(s), out, work)
+    return out
+
+
+def fe_csr_run_f64(dims, A: CSR, dt, nsteps, x0, snap=None):
+    nx, ny, nz = dims
+    out = np.empty(nx * ny * nz, np.float64)
+    work = np.empty_like(out)
+    s = None if snap is None else np.ascontiguousarray(snap, np.float32)
+    lib.po_fe_csr_run_f64(nx, ny, nz, A.row_ptr, A.col, A.val, float(dt), int(nsteps),
+                          np.ascontiguousarray(x0, np.float32), _ptr(s), out, work)
+    return out
+
+
+def spmv_allrows_step(A: CSR, dt, x):
+    y = np.empty(A.n_rows, np.float32)
+    lib.po_spmv_allrows_step(A.n_rows, A.row_ptr, A.col, A.val, np.float32(dt),
+                             np.ascontiguousarray(x, np.float32), y)
+    return y
+
+
+def fe_ht_step(N, keys, vals, dt, x):
+    y = np.empty(N, np.float32)
+    lib.po_fe_ht_step(N, keys, vals, keys.size, np.float32(dt), np.ascontiguousarray(x, np.float32), y)
+    return y
+
+
+def jli(dims, A: CSR, dt, b, urf=1.0, tol=1e-5, max_iter=1000):
+    nx, ny, nz = dims
+    N = nx * ny * nz
+    x = np.empty(N, np.float32)
+    work = np.empty(N, np.float32)
+    res = C.c_float(0)
+    it = lib.po_jli(nx, ny, nz, A.row_ptr, A.col, A.val, np.float32(dt), np.ascontiguousarray(b, np.float32),
+                    np.float32(urf), np.float32(tol), int(max_iter), x, work, C.byref(res))
+    return x, it, res.value
+
+
+def ic_gaussian(dims, l, mean, sigma, scale=1.0):
+    nx, ny, nz = dims
+    out = np.empty(nx * ny * nz, np.float32)
+    lib.po_ic_gaussian(nx, ny, nz, np.asarray(l, np.float32), np.asarray(mean, np.float32),
+                       np.asarray(sigma, np.float32), np.float32(scale), out)
+    return out
+
+
+def ic_tophat(dims, l, center, radius):
+    nx, ny, nz = dims
+    out = np.empty(nx * ny * nz, np.float32)
+    lib.po_ic_tophat(nx, ny, nz, np.asarray(l, np.float32), np.asarray(center, np.float32),
+                     np.float32(radius), out)
+    return out
+
+
+def alpha_uniform(n, seed, lo, hi):
+    out = np.empty(n, np.float32)
+    lib.po_alpha_uniform(int(n), int(seed), np.float32(lo), np.float32(hi), out)
+    return out
+
+
+def lap2d_csr(n, inv_h2):
+    nnz = lib.po_lap2d_nnz(n)
+    rp = np.zeros(n * n + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float32)
+    lib.po_lap2d_csr(n, np.float32(inv_h2), rp, col, val)
+    return CSR(rp, col, val)
+
+
+def lu_factor_f64(a):
+    a = np.ascontiguousarray(a, np.float64).copy()
+    n = a.shape[0]
+    piv = np.zeros(n, np.int32)
+    info = lib.po_lu_factor_f64(n, a, piv)
+    return a, piv, info
+
+
+def lu_solve_f64(lu, piv, b):
+    b2 = np.ascontiguousarray(b, np.float64).copy()
+    nrhs = 1 if b2.ndim == 1 else b2.shape[1]
+    lib.po_lu_solve_f64(lu.shape[0], np.ascontiguousarray(lu), piv, nrhs, b2)
+    return b2
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (d if d > 0 else 1.0))

@@ -1,0 +1,145 @@
+"""Freeze the reference's own host-side assembly output as fixtures.
+
+Runs oracle/_ref/piro_ref_harness (built from the PIRO sources by
+``make -C oracle ref``) on small grids and stores what it printed:
+
+* ``ref_assembly.npz``  -- full arrays for the small cases;
+* ``ref_digests.json``  -- sha256 of the raw arrays for the larger cases
+  (32^3 CSR Laplacian of every scheme), so the 2 MB arrays need not be
+  committed.
+
+Reference functions exercised (paths relative to the PIRO tree):
+  CSR::stencils::laplacian_{cd,uw,dw}   src/stencils.cpp:233-429
+  CSR::stencils::gradient_*             src/stencils.cpp:7-231
+  methods::sethash                      src/methods.cpp:74-118
+  calculategaussian                     src/ic.cpp:16-27
+
+Only this script runs the harness; tests read the frozen files.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "piro_ref_harness")
+
+LAP_GRIDS = [(3, 3, 3), (4, 4, 4), (5, 6, 7), (7, 5, 4), (9, 9, 9), (6, 3, 5)]
+GRAD_GRIDS = [(5, 6, 7), (4, 4, 4)]
+BIG_LAP = [(32, 32, 32)]
+HT_CASES = [  # (scheme, padded dims, l0): Test/LUDecomp.ini is n=3 3 3, l=5
+    (3, (5, 5, 5), 5.0),
+    (1, (5, 5, 5), 5.0),
+    (2, (6, 5, 4), 4.0),
+]
+GAUSS_CASES = [
+    ((8, 9, 10), (1.0, 1.2, 0.8), (0.5, 0.55, 0.4), (0.1, 0.2, 0.15)),
+    ((32, 32, 32), (30.0 / 31.0,) * 3, (15.0 / 31.0,) * 3, (0.1,) * 3),
+]
+
+
+def f32_hex(x: float) -> str:
+    return "%08x" % struct.unpack("<I", struct.pack("<f", np.float32(x)))[0]
+
+
+def lap_norm(l0: float, n0: int) -> np.float32:
+    # operatorlaplacian.cpp:33
+    h = np.float32(l0) / np.float32(n0)
+    return np.float32(float(h) ** 2)
+
+
+def grad_norm(l0: float, n0: int) -> np.float32:
+    # operatorgradient.cpp:25
+    return np.float32(2.0) * (np.float32(l0) / np.float32(n0))
+
+
+def ht_table_size(sparsecount: int, N: int) -> int:
+    # operatorlaplacian.cpp:184-185 with assets/Data/hashtable.ini
+    a, b, c, lf = np.float32(3.3608), np.float32(-0.5552), np.float32(-0.0143), np.float32(0.7)
+    load = np.float32(float(a) * (float(N) ** float(b)) + float(c))
+    return int(np.float32(sparsecount) / np.float32(load * lf))
+
+
+def read_arrays(path: str, dtypes):
+    out = []
+    with open(path, "rb") as f:
+        for dt in dtypes:
+            (n,) = struct.unpack("<q", f.read(8))
+            out.append(np.frombuffer(f.read(n * np.dtype(dt).itemsize), dtype=dt).copy())
+    return out
+
+
+def run(args, dtypes):
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "o.bin")
+        subprocess.run([HARNESS] + [str(a) for a in args] + [out], check=True)
+        return read_arrays(out, dtypes)
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def main() -> int:
+    if not os.path.exists(HARNESS):
+        print("build the harness first: make -C oracle ref", file=sys.stderr)
+        return 1
+    arrays = {}
+    meta = {"lap": [], "grad": [], "ht": [], "gauss": [], "lap_digest": []}
+    for scheme in (1, 2, 3):
+        for (nx, ny, nz) in LAP_GRIDS:
+            norm = lap_norm(1.0, nx - 2)
+            rp, col, val = run(["lap", scheme, nx, ny, nz, f32_hex(norm)], ["<i4", "<i4", "<f4"])
+            key = f"lap_s{scheme}_{nx}x{ny}x{nz}"
+            arrays[key + "_rp"], arrays[key + "_col"], arrays[key + "_val"] = rp, col, val
+            meta["lap"].append({"key": key, "scheme": scheme, "dims": [nx, ny, nz],
+                                "norm_hex": f32_hex(norm)})
+        for (nx, ny, nz) in BIG_LAP:
+            norm = lap_norm(30.0 / 31.0, nx - 2)
+            rp, col, val = run(["lap", scheme, nx, ny, nz, f32_hex(norm)], ["<i4", "<i4", "<f4"])
+            meta["lap_digest"].append({"scheme": scheme, "dims": [nx, ny, nz], "norm_hex": f32_hex(norm),
+                                       "rp": digest(rp), "col": digest(col), "val": digest(val),
+                                       "nnz": int(col.size)})
+        for d in range(3):
+            for (nx, ny, nz) in GRAD_GRIDS:
+                norm = grad_norm(1.0, nx - 2)
+                rp, col, val = run(["grad", scheme, d, nx, ny, nz, f32_hex(norm)], ["<i4", "<i4", "<f4"])
+                key = f"grad_s{scheme}_d{d}_{nx}x{ny}x{nz}"
+                arrays[key + "_rp"], arrays[key + "_col"], arrays[key + "_val"] = rp, col, val
+                meta["grad"].append({"key": key, "scheme": scheme, "dir": d, "dims": [nx, ny, nz],
+                                     "norm_hex": f32_hex(norm)})
+    for scheme, (nx, ny, nz), l0 in HT_CASES:
+        norm = lap_norm(l0, nx - 2)
+        N = nx * ny * nz
+        nnz = 7 * N - 2 * (ny * nz + nx * nz + nx * ny)
+        ts = ht_table_size(nnz, N)
+        keys, vals = run(["ht", scheme, nx, ny, nz, f32_hex(norm), ts], ["<i4", "<f4"])
+        key = f"ht_s{scheme}_{nx}x{ny}x{nz}"
+        arrays[key + "_keys"], arrays[key + "_vals"] = keys, vals
+        meta["ht"].append({"key": key, "scheme": scheme, "dims": [nx, ny, nz], "norm_hex": f32_hex(norm),
+                           "table_size": ts, "sparsecount": nnz, "l0": l0})
+    for i, (dims, l, mean, sigma) in enumerate(GAUSS_CASES):
+        (g,) = run(["gauss", *dims, *l, *mean, *sigma], ["<f4"])
+        key = f"gauss_{i}"
+        if g.size <= 4096:
+            arrays[key] = g
+        meta["gauss"].append({"key": key, "dims": list(dims), "l": list(l), "mean": list(mean),
+                              "sigma": list(sigma), "digest": digest(g),
+                              "sum_f64": float(np.sum(g.astype(np.float64)))})
+    np.savez_compressed(os.path.join(HERE, "ref_assembly.npz"), **arrays)
+    with open(os.path.join(HERE, "ref_digests.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    print(f"wrote {len(arrays)} arrays")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
