@@ -1,0 +1,177 @@
+"""Benchmark of the hot path: forward-Euler CSR heat stepping on a 256^3 grid.
+
+Workload (the reference's explicit-stepping configuration (a)): padded
+256x256x256 grid (N = 16,777,216 rows), 7-point central Laplacian assembled
+as CSR (117,047,296 non-zeros, int32 indices, FP32 values), per-cell
+diffusivity alpha ~ U[0.005, 0.02) from seed 20261203 applied as a row
+scaling, dt = 0.15*dx^2/0.02, Gaussian initial field (sigma 0.1, centre 0.5),
+zero Dirichlet faces.  One step = one launch of the fused SpMV + interior
+mask + face-BC kernel over the whole grid.  Inputs are resident in HBM
+before the timed region.
+
+Prints ONE JSON line.  `value` is cell-steps/s (padded rows x steps / s).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-steps/s + SpMV HBM GB/s (256³ heat); surrogate vs plain speedup @1e-3 L2"
+HBM_PEAK_GBS = 8000.0
+DIMS = (256, 256, 256)
+SEED = 20261203
+
+
+def algorithmic_bytes(n_rows: int, nnz: int) -> int:
+    # col + val per non-zero, row_ptr (N+1), x read once, y written once
+    return 8 * nnz + 4 * (n_rows + 1) + 4 * n_rows + 4 * n_rows
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "fe_csr_256cubed_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(budget_s: float = 15.0):
+    """The CPU oracle (C restatement of the reference's step) timed on this
+    host's cores on a bounded sample of the same 256^3 workload."""
+    import numpy as np
+    import oracle as po
+
+    nx, ny, nz = DIMS
+    norm = po.lap_norm(1.0, nx - 2)
+    A = po.lap_csr(nx, ny, nz, po.CD, norm)
+    alpha = po.alpha_uniform(A.n_rows, SEED, 0.005, 0.02)
+    A = po.row_scale(A, alpha)
+    dt = np.float32(np.float32(0.15) * norm / np.float32(0.02))
+    x = po.ic_gaussian(DIMS, np.float32([1, 1, 1]), np.float32([0.5] * 3), np.float32([0.1] * 3), 1.0)
+    y = np.empty_like(x)
+    threads = int(po.lib.po_omp_threads())
+    t0 = time.perf_counter()
+    po.lib.po_fe_csr_step(nx, ny, nz, A.row_ptr, A.col, A.val, dt, x, None, y)
+    t1 = time.perf_counter() - t0
+    steps = max(1, min(50, int(budget_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        po.lib.po_fe_csr_step(nx, ny, nz, A.row_ptr, A.col, A.val, dt, x, None, y)
+        x, y = y, x
+    el = time.perf_counter() - t0
+    return {
+        "value": A.n_rows * steps / el,
+        "unit": "cell-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} FE steps of the 256^3 CSR heat step (oracle/piro_oracle.c, OpenMP over rows, "
+                  f"{threads} threads), {el:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variant", type=int, default=-1, help="force an FE kernel variant (-1 = autotune)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are out of scope "
+                                   "(z-slab sharding is described in DESIGN.md)", "n_gpus": args.gpus}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    from piro_amd.core import CD, Context
+
+    ctx = Context(0)
+    nx, ny, nz = DIMS
+    norm = ctx.lap_norm(1.0, nx - 2)
+    A = ctx.laplacian_csr(nx, ny, nz, CD, norm)
+    alpha = ctx.alpha_uniform(A.n_rows, SEED, 0.005, 0.02)
+    ctx.row_scale(A, alpha)
+    dt = np.float32(np.float32(0.15) * np.float32(norm) / np.float32(0.02))
+    x = ctx.ic_gaussian(DIMS, [1.0] * 3, [0.5] * 3, [0.1] * 3, 1.0)
+    snap = ctx.apply_face_values(DIMS, x, [0.0] * 6)
+    y = torch.empty_like(x)
+    if args.variant >= 0:
+        ctx.set_variant(args.variant)
+        variant = args.variant
+    else:
+        variant, _ = ctx.autotune(DIMS, A, dt, x, snap, y)
+    vname = ctx.variants()[variant]
+
+    def step(a, b):
+        ctx.fe_step_csr(DIMS, A, dt, a, snap, b)
+
+    for _ in range(args.warmup):
+        step(x, y)
+        x, y = y, x
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t_host = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step(x, y)
+        x, y = y, x
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_host = time.perf_counter() - t_host
+    ms = e0.elapsed_time(e1)
+    ms_step = ms / max(args.steps, 1)
+    n_rows, nnz = A.n_rows, A.nnz
+    cell_steps = n_rows * args.steps / (ms / 1e3)
+    interior = (nx - 2) * (ny - 2) * (nz - 2)
+    byts = algorithmic_bytes(n_rows, nnz)
+    gbs = byts / (ms_step / 1e3) / 1e9
+    finite = bool(torch.isfinite(x).all().item())
+
+    base = None
+    if not args.no_cpu_baseline:
+        base = cpu_baseline()
+
+    out = {
+        "metric": METRIC,
+        "value": cell_steps,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded alpha field and Gaussian initial condition)",
+        "config": {"workload": "heat 256^3 FE CSR step (7-point cd Laplacian, alpha~U[0.005,0.02), "
+                               "alpha*dt/dx^2<=0.15, zero Dirichlet faces)",
+                   "grid": list(DIMS), "rows": n_rows, "nnz": nnz, "kernel_variant": vname},
+        "hbm_gbs": gbs,
+        "interior_cell_steps_per_s": interior * args.steps / (ms / 1e3),
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": load_traffic(),
+                     "algorithmic_bytes_per_launch": byts},
+        "cpu_baseline": base,
+        "field_finite": finite,
+        "host_wall_s": t_host,
+    }
+    print(json.dumps(out))
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,263 @@
+/*
+ * piro_hip.h -- C ABI of libpiro_hip.so, the MI355X (gfx950) implementation
+ * of PIRO's operator-and-solve hot path.
+ *
+ * Conventions
+ *   - Every function returns an int status (PIRO_OK = 0).  On failure
+ *     piro_last_error() returns a message for the calling thread.  Nothing
+ *     exits or throws across this boundary.
+ *   - Array arguments named d_* are DEVICE pointers (hipMalloc'd by the
+ *     caller, by piro_malloc, or torch tensors' data pointers); h_* are host
+ *     pointers.  Indices are int32 (as in the reference), fields FP32.
+ *   - Work is enqueued on the context's stream in order; call
+ *     piro_ctx_synchronize() before reading results on the host.  A context is
+ *     not thread-safe; use one per host thread.
+ *   - Grid dims nx, ny, nz are PADDED dims (the reference adds 2 per axis,
+ *     src/preprocess.cpp:72-76); row index i = x + y*nx + z*nx*ny.
+ *
+ * Each declaration names the reference interface (file:line, relative to the
+ * PIRO source tree) whose role it takes over.
+ */
+#ifndef PIRO_HIP_H
+#define PIRO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PIRO_OK = 0,
+    PIRO_EINVAL = 1,     /* bad argument / shape                             */
+    PIRO_EHIP = 2,       /* HIP runtime error                                */
+    PIRO_ENOMEM = 3,     /* allocation failed                                */
+    PIRO_EFULL = 4,      /* hash table full (src/methods.cpp:109-116)        */
+    PIRO_ENOTCONV = 5,   /* iteration did not converge                      */
+    PIRO_ESINGULAR = 6,  /* zero pivot                                       */
+    PIRO_EINTERNAL = 7
+};
+
+/* spatial schemes ([Schemes] Space, README.md:63-75) */
+enum { PIRO_UW = 1, PIRO_DW = 2, PIRO_CD = 3 };
+
+typedef struct piro_ctx piro_ctx;
+
+/* ---------------------------------------------------------------- runtime */
+
+/* Replaces Piro::opencl_init (src/opencl.cpp:67-138): one device, one
+ * in-order queue.  A fresh context owns a non-blocking HIP stream. */
+int piro_ctx_create(int device, piro_ctx **out);
+int piro_ctx_destroy(piro_ctx *ctx);
+/* Use an external hipStream_t (e.g. torch's current stream); NULL restores
+ * the context's own stream. */
+int piro_ctx_set_stream(piro_ctx *ctx, void *hip_stream);
+void *piro_ctx_get_stream(piro_ctx *ctx);
+/* Replaces the clFinish calls that follow every launch
+ * (src/kernelprocess.cpp:31 and passim). */
+int piro_ctx_synchronize(piro_ctx *ctx);
+const char *piro_last_error(void);
+int piro_version(void);
+
+/* Replaces clCreateBuffer / clEnqueueRead/WriteBuffer (copyCL,
+ * include/openclutilities.hpp:17-65). */
+int piro_malloc(piro_ctx *ctx, size_t bytes, void **d_out);
+int piro_free(piro_ctx *ctx, void *d_ptr);
+int piro_memcpy_h2d(piro_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int piro_memcpy_d2h(piro_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+int piro_memcpy_d2d(piro_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
+int piro_memset(piro_ctx *ctx, void *d_dst, int value, size_t bytes);
+
+/* ---------------------------------------------------------------- assembly */
+
+/* nnz of the 7-point Laplacian on padded dims (src/stencils.cpp:233-429:
+ * N + 2 * neighbour pairs, identical for cd/uw/dw). */
+int piro_lap_nnz(int nx, int ny, int nz, int64_t *nnz);
+
+/* Replaces Piro::CSR::laplacian + CSR::stencils::laplacian_{cd,uw,dw}
+ * (src/operatorlaplacian.cpp:14-63, src/stencils.cpp:233-429), assembled on
+ * the device: row_ptr[N+1], col[nnz], val[nnz], bit-identical to the
+ * reference including the boundary 10.0 rule (src/stencils.cpp:278-291).
+ * norm = (l0/float(n0))^2 as the reference computes it
+ * (src/operatorlaplacian.cpp:33); see piro_lap_norm. */
+int piro_assemble_laplacian_csr(piro_ctx *ctx, int nx, int ny, int nz,
+                                int scheme, float norm, int32_t *d_row_ptr,
+                                int32_t *d_col, float *d_val);
+/* Replaces Piro::COO::laplacian (src/operatorlaplacian.cpp:105-153,
+ * src/stencils.cpp:572-795): the same entries with explicit rows[nnz]. */
+int piro_assemble_laplacian_coo(piro_ctx *ctx, int nx, int ny, int nz,
+                                int scheme, float norm, int32_t *d_rows,
+                                int32_t *d_col, float *d_val);
+/* norm helpers (src/operatorlaplacian.cpp:33, src/operatorgradient.cpp:25) */
+float piro_lap_norm(float l0, int n0_interior);
+float piro_grad_norm(float l0, int n0_interior);
+
+/* Replaces Piro::CSR::gradient + gradient_{uw,dw,cd}_{x,y,z}
+ * (src/operatorgradient.cpp:13-117, src/stencils.cpp:7-231) for one
+ * direction dir (0=x,1=y,2=z).  nnz = 2N - 2N/n_dir. */
+int piro_grad_nnz(int nx, int ny, int nz, int dir, int64_t *nnz);
+int piro_assemble_gradient_csr(piro_ctx *ctx, int nx, int ny, int nz,
+                               int scheme, int dir, float norm,
+                               int32_t *d_row_ptr, int32_t *d_col,
+                               float *d_val);
+
+/* The intended semantics of kernelmethods::csrscale / SpMDSM
+ * (src/kernelcsrgeam.cpp:208-235, SpMDSM.cl:20-31), which the reference
+ * fails to enqueue: val[k] *= diag[row(k)]. */
+int piro_row_scale(piro_ctx *ctx, int n_rows, const int32_t *d_row_ptr,
+                   float *d_val, const float *d_diag);
+
+/* Hash-table format (src/operatorlaplacian.cpp:155-234,
+ * src/methods.cpp:74-118): table size from hashtable.ini's fit, then every
+ * CSR entry inserted in CSR order (key = row*N + col, linear probing,
+ * repeated keys overwrite).  The device insertion reproduces the sequential
+ * slot layout bit for bit.  d_keys/d_vals[ts] are initialised here.
+ * Only valid while N*N fits int32 (N <= 46340), as in the reference. */
+int piro_ht_table_size(int sparsecount, int n_rows, float a, float b, float c,
+                       float load_factor, int *ts_out);
+int piro_ht_build(piro_ctx *ctx, int n_rows, const int32_t *d_row_ptr,
+                  const int32_t *d_col, const float *d_val, int ts,
+                  int32_t *d_keys, float *d_vals);
+/* Unpack a hash table into CSR with each row's entries in ascending column
+ * order -- the order SpMVHT.cl:32-37 sums in.  nnz_out = occupied slots.
+ * d_row_ptr[N+1]; d_col/d_val need room for nnz_out (query with NULL). */
+int piro_ht_count(piro_ctx *ctx, int n_rows, const int32_t *d_keys, int ts,
+                  int64_t *nnz_out);
+int piro_ht_to_csr(piro_ctx *ctx, int n_rows, const int32_t *d_keys,
+                   const float *d_vals, int ts, int32_t *d_row_ptr,
+                   int32_t *d_col, float *d_val);
+/* COO rows -> CSR row_ptr (rows sorted, as the COO stencils emit them). */
+int piro_coo_row_ptr(piro_ctx *ctx, int n_rows, int64_t nnz,
+                     const int32_t *d_rows, int32_t *d_row_ptr);
+
+/* ------------------------------------------------------------------ geam */
+
+/* Replaces kernelmethods::csrgeam (src/kernelcsrgeam.cpp:9-86: csrgeam_1,
+ * N serial prefixsum launches, csrgeam_3).  D = diag(v_x)Gx + diag(v_y)Gy +
+ * diag(v_z)Gz as the reference's concatenation-merge (ties A before B before
+ * C, tails appended A->B->C, equal columns kept apart).  d_v3 = [vx|vy|vz]
+ * (3N).  Two phases: piro_geam3_rowptr fills d_row_ptr[N+1] (one device-wide
+ * scan) and returns nnz(D); piro_geam3_fill writes d_col/d_val. */
+int piro_geam3_rowptr(piro_ctx *ctx, int n_rows,
+                      const int32_t *d_rpa, const int32_t *d_ca,
+                      const int32_t *d_rpb, const int32_t *d_cb,
+                      const int32_t *d_rpc, const int32_t *d_cc,
+                      int32_t *d_row_ptr, int64_t *nnz_out);
+int piro_geam3_fill(piro_ctx *ctx, int n_rows,
+                    const int32_t *d_rpa, const int32_t *d_ca, const float *d_va,
+                    const int32_t *d_rpb, const int32_t *d_cb, const float *d_vb,
+                    const int32_t *d_rpc, const int32_t *d_cc, const float *d_vc,
+                    const float *d_v3, const int32_t *d_row_ptr,
+                    int32_t *d_col, float *d_val);
+/* Build-only: D = alpha*A + beta*B, a true sorted merge (equal columns summed
+ * as fl(fl(alpha*a)+fl(beta*b))); rows of A and B strictly increasing.  Used
+ * for I - dt*alpha*L.  Same two-phase protocol. */
+int piro_geam2_rowptr(piro_ctx *ctx, int n_rows,
+                      const int32_t *d_rpa, const int32_t *d_ca,
+                      const int32_t *d_rpb, const int32_t *d_cb,
+                      int32_t *d_row_ptr, int64_t *nnz_out);
+int piro_geam2_fill(piro_ctx *ctx, int n_rows, float alpha,
+                    const int32_t *d_rpa, const int32_t *d_ca, const float *d_va,
+                    float beta,
+                    const int32_t *d_rpb, const int32_t *d_cb, const float *d_vb,
+                    const int32_t *d_row_ptr, int32_t *d_col, float *d_val);
+
+/* -------------------------------------------------------- initial fields */
+
+/* Replaces Piro::initialcondition "Gaussian" (src/ic.cpp:16-27,44-61):
+ * out[i] = scale * g(idx*l/n_pad) on padded dims. */
+int piro_ic_gaussian(piro_ctx *ctx, int nx, int ny, int nz, const float *h_l3,
+                     const float *h_mean3, const float *h_sigma3, float scale,
+                     float *d_out);
+/* Build extension: top-hat blob (1 inside |c-center|<radius). */
+int piro_ic_tophat(piro_ctx *ctx, int nx, int ny, int nz, const float *h_l3,
+                   const float *h_center3, float radius, float *d_out);
+/* Synthetic diffusivity field: out[i] = lo + (hi-lo)*u(seed,i), u from a
+ * splitmix64 counter hash (the project's own generator). */
+int piro_alpha_uniform(piro_ctx *ctx, int64_t n, uint64_t seed, float lo,
+                       float hi, float *d_out);
+/* Replaces bc::setbc + the memD snapshot (src/bc.cpp:67-78, :20-36):
+ * snap = field with each of the 6 face sets (edges excluded,
+ * src/bc.cpp:115-146) overwritten by face_vals[0..5] (order: x=nx-1, x=0,
+ * y=ny-1, y=0, z=0, z=nz-1).  snap may alias field. */
+int piro_apply_face_values(piro_ctx *ctx, int nx, int ny, int nz,
+                           const float *d_field, const float *h_face_vals6,
+                           float *d_snap);
+
+/* ---------------------------------------------------------------- stepping */
+
+/* One forward-Euler step of the CSR path.  Replaces
+ * kernelsolve::CSR::TIMESCHEME_11 + SpMVCSR (src/kernelprocess.cpp:10-40,
+ * SpMVCSR.cl:1-40), the copy of scalarMatrix::Solve (src/process.cpp:234)
+ * and bc::opencl_setBC/setBC (src/bc.cpp:38-65), fused in one kernel:
+ *   interior rows: y = sum_k fl(fl(dt*val_k)*x[col_k]) (CSR order) + x[r]
+ *   face rows    : y = d_snap[r] (0 if d_snap == NULL)
+ *   edges/corners: y = 0
+ * d_x and d_y must not alias. */
+int piro_fe_step_csr(piro_ctx *ctx, int nx, int ny, int nz,
+                     const int32_t *d_row_ptr, const int32_t *d_col,
+                     const float *d_val, int64_t nnz, float dt,
+                     const float *d_x, const float *d_snap, float *d_y);
+/* nsteps of the above, ping-ponging d_x and d_work; the result is left in
+ * d_x.  Replaces the time loop of Piro::solve (src/solve.cpp:18-46). */
+int piro_fe_run_csr(piro_ctx *ctx, int nx, int ny, int nz,
+                    const int32_t *d_row_ptr, const int32_t *d_col,
+                    const float *d_val, int64_t nnz, float dt, int nsteps,
+                    float *d_x, float *d_work, const float *d_snap);
+/* Forward Euler over ALL rows, the COO/HT/DENSE semantics
+ * (src/kernelprocess.cpp:42-120: SpMV then addVectors):
+ * y = x + fl(dt * sum_k fl(val_k*x[col_k])).  Replaces SpMVCOO/SpMVHT +
+ * addVectors; feed it piro_coo_row_ptr or piro_ht_to_csr output. */
+int piro_fe_step_allrows(piro_ctx *ctx, int n_rows, const int32_t *d_row_ptr,
+                         const int32_t *d_col, const float *d_val, int64_t nnz,
+                         float dt, const float *d_x, float *d_y);
+/* Plain y = A x (FP32, CSR order). */
+int piro_spmv_csr(piro_ctx *ctx, int n_rows, const int32_t *d_row_ptr,
+                  const int32_t *d_col, const float *d_val, int64_t nnz,
+                  const float *d_x, float *d_y);
+/* Force a kernel variant for the FE step (-1 = autotune on first use);
+ * piro_fe_variant_count/name describe them. */
+int piro_fe_set_variant(piro_ctx *ctx, int variant);
+int piro_fe_get_variant(piro_ctx *ctx, int *variant);
+int piro_fe_variant_count(void);
+const char *piro_fe_variant_name(int variant);
+/* Time every variant on the given operands and keep the fastest. */
+int piro_fe_autotune(piro_ctx *ctx, int nx, int ny, int nz,
+                     const int32_t *d_row_ptr, const int32_t *d_col,
+                     const float *d_val, int64_t nnz, float dt,
+                     const float *d_x, const float *d_snap, float *d_y,
+                     int *best_variant, float *best_ms);
+
+/* -------------------------------------------------------------- reductions */
+
+/* FP64-accumulated device reductions (the build's replacement for the
+ * reference's unused residuals.cl:1-29 and the host sum_numbers,
+ * src/extras.cpp:24-30).  Results are written to host doubles. */
+int piro_reduce_sum(piro_ctx *ctx, int64_t n, const float *d_x, double *h_out);
+int piro_reduce_sumsq(piro_ctx *ctx, int64_t n, const float *d_x, double *h_out);
+int piro_reduce_dot(piro_ctx *ctx, int64_t n, const float *d_x,
+                    const float *d_y, double *h_out);
+int piro_reduce_absmax(piro_ctx *ctx, int64_t n, const float *d_x,
+                       double *h_out);
+
+/* ------------------------------------------------------------- JLI solve */
+
+/* Replaces kernelsolve::CSR::TIMESCHEME_12_40 (src/kernelprocess.cpp:
+ * 123-209): x0 = b; repeat { c = b - (interior ? x - dt*L x : 0);
+ * x += urf*c; res = sqrt(sum dx^2)/sqrt(N) } until (res-res_old)^2 < tol or
+ * max_iter.  One fused kernel per iteration; the residual is reduced on the
+ * device in FP64 (no per-iteration N-float read-back).  The stopping test is
+ * evaluated every `check_every` iterations from one 8-byte read-back.
+ * d_x (out) and d_work are N floats each.  *iters receives the iteration
+ * count, *res the last residual. */
+int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz,
+                   const int32_t *d_row_ptr, const int32_t *d_col,
+                   const float *d_val, int64_t nnz, float dt,
+                   const float *d_b, float urf, float tol, int max_iter,
+                   float *d_x, float *d_work, int *iters, float *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PIRO_HIP_H */

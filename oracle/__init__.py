@@ -76,6 +76,7 @@ def _load() -> C.CDLL:
         "po_csr_to_dense_f64": (None, [C.c_int, C.c_int, _i32p, _i32p, _f32p, _f64p]),
         "po_lu_factor_f64": (C.c_int, [C.c_int, _f64p, _i32p]),
         "po_lu_solve_f64": (None, [C.c_int, _f64p, _i32p, C.c_int, _f64p]),
+        "po_omp_threads": (C.c_int, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -819,3 +819,7 @@ void po_lu_solve_f64(int n, const double *lu, const int32_t *piv, int nrhs,
         for (int j = 0; j < nrhs; ++j) b[(int64_t)i * nrhs + j] /= d;
     }
 }
+
+/* number of OpenMP threads the parallel loops above use */
+#include <omp.h>
+int po_omp_threads(void) { return omp_get_max_threads(); }

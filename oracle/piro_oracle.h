@@ -111,6 +111,7 @@ int64_t po_lap2d_nnz(int n);
 void    po_csr_to_dense_f64(int n_rows, int n_cols, const int32_t *row_ptr,
                             const int32_t *col, const float *val, double *d);
 int     po_lu_factor_f64(int n, double *a, int32_t *piv);
+int     po_omp_threads(void);
 void    po_lu_solve_f64(int n, const double *lu, const int32_t *piv,
                         int nrhs, double *b);
 

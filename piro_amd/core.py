@@ -1,0 +1,252 @@
+"""Thin Python front-end over the C ABI, with torch tensors as device memory.
+
+torch is used only for allocation, copies and streams; every computation is a
+call into libpiro_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from ._lib import check, lib
+
+UW, DW, CD = 1, 2, 3
+
+
+def _p(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(a) for a in v])
+
+
+@dataclass
+class DevCSR:
+    row_ptr: torch.Tensor
+    col: torch.Tensor
+    val: torch.Tensor
+
+    @property
+    def n_rows(self) -> int:
+        return self.row_ptr.numel() - 1
+
+    @property
+    def nnz(self) -> int:
+        return self.col.numel()
+
+    def host(self):
+        return (self.row_ptr.cpu().numpy(), self.col.cpu().numpy(), self.val.cpu().numpy())
+
+
+class Context:
+    """One device, one in-order stream (torch's current stream by default)."""
+
+    def __init__(self, device: int = 0, use_torch_stream: bool = True):
+        self.device = torch.device("cuda", device)
+        h = C.c_void_p()
+        check(lib.piro_ctx_create(device, C.byref(h)))
+        self.h = h
+        if use_torch_stream:
+            check(lib.piro_ctx_set_stream(self.h, C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)))
+
+    def close(self):
+        if self.h:
+            check(lib.piro_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib.piro_ctx_synchronize(self.h))
+
+    # ------------------------------------------------------------ helpers
+    def empty(self, n, dtype=torch.float32):
+        return torch.empty(int(n), dtype=dtype, device=self.device)
+
+    # ----------------------------------------------------------- assembly
+    @staticmethod
+    def lap_nnz(nx, ny, nz) -> int:
+        v = C.c_int64()
+        check(lib.piro_lap_nnz(nx, ny, nz, C.byref(v)))
+        return v.value
+
+    @staticmethod
+    def lap_norm(l0, n0):
+        return lib.piro_lap_norm(float(l0), int(n0))
+
+    @staticmethod
+    def grad_norm(l0, n0):
+        return lib.piro_grad_norm(float(l0), int(n0))
+
+    def laplacian_csr(self, nx, ny, nz, scheme, norm) -> DevCSR:
+        N = nx * ny * nz
+        nnz = self.lap_nnz(nx, ny, nz)
+        A = DevCSR(self.empty(N + 1, torch.int32), self.empty(nnz, torch.int32), self.empty(nnz))
+        check(lib.piro_assemble_laplacian_csr(self.h, nx, ny, nz, scheme, float(norm), _p(A.row_ptr), _p(A.col),
+                                              _p(A.val)))
+        return A
+
+    def laplacian_coo(self, nx, ny, nz, scheme, norm):
+        nnz = self.lap_nnz(nx, ny, nz)
+        rows, col, val = self.empty(nnz, torch.int32), self.empty(nnz, torch.int32), self.empty(nnz)
+        check(lib.piro_assemble_laplacian_coo(self.h, nx, ny, nz, scheme, float(norm), _p(rows), _p(col), _p(val)))
+        return rows, col, val
+
+    def gradient_csr(self, nx, ny, nz, scheme, d, norm) -> DevCSR:
+        N = nx * ny * nz
+        v = C.c_int64()
+        check(lib.piro_grad_nnz(nx, ny, nz, d, C.byref(v)))
+        G = DevCSR(self.empty(N + 1, torch.int32), self.empty(v.value, torch.int32), self.empty(v.value))
+        check(lib.piro_assemble_gradient_csr(self.h, nx, ny, nz, scheme, d, float(norm), _p(G.row_ptr), _p(G.col),
+                                             _p(G.val)))
+        return G
+
+    def row_scale(self, A: DevCSR, diag):
+        check(lib.piro_row_scale(self.h, A.n_rows, _p(A.row_ptr), _p(A.val), _p(diag)))
+
+    @staticmethod
+    def ht_table_size(sparsecount, N, a=3.3608, b=-0.5552, c=-0.0143, lf=0.7) -> int:
+        v = C.c_int()
+        check(lib.piro_ht_table_size(int(sparsecount), int(N), a, b, c, lf, C.byref(v)))
+        return v.value
+
+    def ht_build(self, A: DevCSR, ts):
+        keys, vals = self.empty(ts, torch.int32), self.empty(ts)
+        check(lib.piro_ht_build(self.h, A.n_rows, _p(A.row_ptr), _p(A.col), _p(A.val), int(ts), _p(keys), _p(vals)))
+        return keys, vals
+
+    def ht_to_csr(self, N, keys, vals) -> DevCSR:
+        v = C.c_int64()
+        ts = keys.numel()
+        check(lib.piro_ht_count(self.h, N, _p(keys), ts, C.byref(v)))
+        A = DevCSR(self.empty(N + 1, torch.int32), self.empty(max(v.value, 1), torch.int32),
+                   self.empty(max(v.value, 1)))
+        check(lib.piro_ht_to_csr(self.h, N, _p(keys), _p(vals), ts, _p(A.row_ptr), _p(A.col), _p(A.val)))
+        A.col, A.val = A.col[:v.value], A.val[:v.value]
+        return A
+
+    def coo_to_csr(self, N, rows, col, val) -> DevCSR:
+        rp = self.empty(N + 1, torch.int32)
+        check(lib.piro_coo_row_ptr(self.h, N, rows.numel(), _p(rows), _p(rp)))
+        return DevCSR(rp, col, val)
+
+    def geam3(self, Ga: DevCSR, Gb: DevCSR, Gc: DevCSR, v3) -> DevCSR:
+        N = Ga.n_rows
+        rp = self.empty(N + 1, torch.int32)
+        v = C.c_int64()
+        check(lib.piro_geam3_rowptr(self.h, N, _p(Ga.row_ptr), _p(Ga.col), _p(Gb.row_ptr), _p(Gb.col),
+                                    _p(Gc.row_ptr), _p(Gc.col), _p(rp), C.byref(v)))
+        D = DevCSR(rp, self.empty(v.value, torch.int32), self.empty(v.value))
+        check(lib.piro_geam3_fill(self.h, N, _p(Ga.row_ptr), _p(Ga.col), _p(Ga.val), _p(Gb.row_ptr), _p(Gb.col),
+                                  _p(Gb.val), _p(Gc.row_ptr), _p(Gc.col), _p(Gc.val), _p(v3), _p(D.row_ptr),
+                                  _p(D.col), _p(D.val)))
+        return D
+
+    def geam2(self, alpha, A: DevCSR, beta, B: DevCSR) -> DevCSR:
+        N = A.n_rows
+        rp = self.empty(N + 1, torch.int32)
+        v = C.c_int64()
+        check(lib.piro_geam2_rowptr(self.h, N, _p(A.row_ptr), _p(A.col), _p(B.row_ptr), _p(B.col), _p(rp),
+                                    C.byref(v)))
+        D = DevCSR(rp, self.empty(v.value, torch.int32), self.empty(v.value))
+        check(lib.piro_geam2_fill(self.h, N, float(alpha), _p(A.row_ptr), _p(A.col), _p(A.val), float(beta),
+                                  _p(B.row_ptr), _p(B.col), _p(B.val), _p(D.row_ptr), _p(D.col), _p(D.val)))
+        return D
+
+    # ------------------------------------------------------------- fields
+    def ic_gaussian(self, dims, l, mean, sigma, scale=1.0):
+        nx, ny, nz = dims
+        out = self.empty(nx * ny * nz)
+        check(lib.piro_ic_gaussian(self.h, nx, ny, nz, _f3(l), _f3(mean), _f3(sigma), float(scale), _p(out)))
+        return out
+
+    def ic_tophat(self, dims, l, center, radius):
+        nx, ny, nz = dims
+        out = self.empty(nx * ny * nz)
+        check(lib.piro_ic_tophat(self.h, nx, ny, nz, _f3(l), _f3(center), float(radius), _p(out)))
+        return out
+
+    def alpha_uniform(self, n, seed, lo, hi):
+        out = self.empty(n)
+        check(lib.piro_alpha_uniform(self.h, int(n), int(seed), float(lo), float(hi), _p(out)))
+        return out
+
+    def apply_face_values(self, dims, field, vals6):
+        nx, ny, nz = dims
+        snap = self.empty(nx * ny * nz)
+        check(lib.piro_apply_face_values(self.h, nx, ny, nz, _p(field), (C.c_float * 6)(*map(float, vals6)),
+                                         _p(snap)))
+        return snap
+
+    # ----------------------------------------------------------- stepping
+    def fe_step_csr(self, dims, A: DevCSR, dt, x, snap=None, y=None):
+        nx, ny, nz = dims
+        y = self.empty(nx * ny * nz) if y is None else y
+        check(lib.piro_fe_step_csr(self.h, nx, ny, nz, _p(A.row_ptr), _p(A.col), _p(A.val), A.nnz, float(dt), _p(x),
+                                   _p(snap), _p(y)))
+        return y
+
+    def fe_run_csr(self, dims, A: DevCSR, dt, nsteps, x, work=None, snap=None):
+        """Advance x in place by nsteps."""
+        nx, ny, nz = dims
+        work = self.empty(nx * ny * nz) if work is None else work
+        check(lib.piro_fe_run_csr(self.h, nx, ny, nz, _p(A.row_ptr), _p(A.col), _p(A.val), A.nnz, float(dt),
+                                  int(nsteps), _p(x), _p(work), _p(snap)))
+        return x
+
+    def fe_step_allrows(self, A: DevCSR, dt, x, y=None):
+        y = self.empty(A.n_rows) if y is None else y
+        check(lib.piro_fe_step_allrows(self.h, A.n_rows, _p(A.row_ptr), _p(A.col), _p(A.val), A.nnz, float(dt),
+                                       _p(x), _p(y)))
+        return y
+
+    def spmv(self, A: DevCSR, x, y=None):
+        y = self.empty(A.n_rows) if y is None else y
+        check(lib.piro_spmv_csr(self.h, A.n_rows, _p(A.row_ptr), _p(A.col), _p(A.val), A.nnz, _p(x), _p(y)))
+        return y
+
+    def set_variant(self, v: int):
+        check(lib.piro_fe_set_variant(self.h, int(v)))
+
+    @staticmethod
+    def variants():
+        return [lib.piro_fe_variant_name(i).decode() for i in range(lib.piro_fe_variant_count())]
+
+    def autotune(self, dims, A: DevCSR, dt, x, snap, y):
+        nx, ny, nz = dims
+        bv, bm = C.c_int(), C.c_float()
+        check(lib.piro_fe_autotune(self.h, nx, ny, nz, _p(A.row_ptr), _p(A.col), _p(A.val), A.nnz, float(dt), _p(x),
+                                   _p(snap), _p(y), C.byref(bv), C.byref(bm)))
+        return bv.value, bm.value
+
+    # --------------------------------------------------------- reductions
+    def reduce(self, op, x, y=None) -> float:
+        out = C.c_double()
+        n = x.numel()
+        if op == "sum":
+            check(lib.piro_reduce_sum(self.h, n, _p(x), C.byref(out)))
+        elif op == "sumsq":
+            check(lib.piro_reduce_sumsq(self.h, n, _p(x), C.byref(out)))
+        elif op == "dot":
+            check(lib.piro_reduce_dot(self.h, n, _p(x), _p(y), C.byref(out)))
+        elif op == "absmax":
+            check(lib.piro_reduce_absmax(self.h, n, _p(x), C.byref(out)))
+        else:
+            raise ValueError(op)
+        return out.value
+
+    # ---------------------------------------------------------------- JLI
+    def jli_solve(self, dims, A: DevCSR, dt, b, urf=1.0, tol=1e-5, max_iter=1000):
+        nx, ny, nz = dims
+        x, work = self.empty(b.numel()), self.empty(b.numel())
+        it, res = C.c_int(), C.c_float()
+        check(lib.piro_jli_solve(self.h, nx, ny, nz, _p(A.row_ptr), _p(A.col), _p(A.val), A.nnz, float(dt), _p(b),
+                                 float(urf), float(tol), int(max_iter), _p(x), _p(work), C.byref(it), C.byref(res)))
+        return x, it.value, res.value

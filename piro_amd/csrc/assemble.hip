@@ -1,0 +1,594 @@
+// assemble.hip -- device assembly of PIRO's operators and initial fields.
+//
+// The reference assembles on the host with push_back and an O(n^2 N)
+// std::find for its boundary rule (src/stencils.cpp:233-429).  Here every
+// row is independent: its row_ptr comes from a closed form of the row
+// lengths and the boundary rule from the position of the first occurrence
+// of each boundary column, so one thread writes one row and the whole 256^3
+// operator (1.1 GB) is written once, coalesced by row.
+#include "device_utils.hpp"
+
+#include <cmath>
+
+using namespace piro;
+
+namespace {
+
+struct Dims {
+    int nx, ny, nz;
+    int64_t sxy;
+};
+
+__device__ __forceinline__ bool is_bnd(int x, int y, int z, const Dims &d)
+{
+    // src/bc.cpp:169-178
+    return x == 0 || x == d.nx - 1 || y == 0 || y == d.ny - 1 || z == 0 || z == d.nz - 1;
+}
+
+// number of CSR entries in rows [0, i) of the 7-point operator: 7 per row
+// minus the missing neighbours (per axis: 2 per full line/plane, plus the
+// partial line/plane before row i).
+__device__ __forceinline__ int64_t lap_before(int x, int y, int z, int64_t i, const Dims &d)
+{
+    int64_t sx = 2 * ((int64_t)z * d.ny + y) + (x > 0);
+    int my = (y == 0) + (y == d.ny - 1);
+    int64_t sy = 2 * (int64_t)d.nx * z + (int64_t)d.nx * (y > 0) + (int64_t)x * my;
+    int mz = (z == 0) + (z == d.nz - 1);
+    int64_t sz = d.sxy * (z > 0) + ((int64_t)y * d.nx + x) * mz;
+    return 7 * i - sx - sy - sz;
+}
+
+// Row entries in the reference order [self, x-, x+, y-, y+, z-, z+]
+// (src/stencils.cpp:239-276); uw replaces "+" neighbours by self
+// (:305-342), dw replaces "-" neighbours by self (:371-408).  The boundary
+// rule (src/stencils.cpp:278-291) sets the FIRST entry holding each boundary
+// column to 10.0f; in closed form (cd/dw): the z+ entry whose target is a
+// boundary cell, the y+ entry of a z=0 row, the x+ entry of a z=0,y=0 row,
+// and row 0's self entry; (uw): the self entry of every boundary row.
+template <bool COO>
+__global__ __launch_bounds__(256) void k_lap_csr(Dims d, int scheme, float vself, float vnb,
+                                                 int32_t *__restrict__ row_ptr_or_rows,
+                                                 int32_t *__restrict__ col, float *__restrict__ val)
+{
+    int64_t N = d.sxy * d.nz;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int x = (int)(i % d.nx);
+    int64_t t = i / d.nx;
+    int y = (int)(t % d.ny);
+    int z = (int)(t / d.ny);
+    int64_t k = lap_before(x, y, z, i, d);
+    if (!COO) {
+        row_ptr_or_rows[i] = (int32_t)k;
+        if (i == N - 1) row_ptr_or_rows[N] = (int32_t)(7 * N - 2 * ((int64_t)d.ny * d.nz + (int64_t)d.nx * d.nz + d.sxy));
+    }
+    const bool uw = scheme == PIRO_UW, dw = scheme == PIRO_DW;
+    const bool bnd = is_bnd(x, y, z, d);
+    int32_t ii = (int32_t)i;
+    auto emit = [&](int32_t c, float v) {
+        col[k] = c;
+        val[k] = v;
+        if (COO) row_ptr_or_rows[k] = ii;
+        ++k;
+    };
+    // self
+    emit(ii, (uw ? bnd : (i == 0)) ? 10.0f : vself);
+    if (x > 0) emit(dw ? ii : ii - 1, vnb);
+    if (x < d.nx - 1) emit(uw ? ii : ii + 1, (!uw && z == 0 && y == 0) ? 10.0f : vnb);
+    if (y > 0) emit(dw ? ii : ii - d.nx, vnb);
+    if (y < d.ny - 1) emit(uw ? ii : ii + d.nx, (!uw && z == 0) ? 10.0f : vnb);
+    if (z > 0) emit(dw ? ii : (int32_t)(i - d.sxy), vnb);
+    if (z < d.nz - 1) emit(uw ? ii : (int32_t)(i + d.sxy), (!uw && is_bnd(x, y, z + 1, d)) ? 10.0f : vnb);
+}
+
+// src/stencils.cpp:7-231: per row (lower, -1/norm) if d>0, (upper, +1/norm)
+// if d<n_d-1; uw: upper -> self; dw: lower -> self.
+__global__ __launch_bounds__(256) void k_grad_csr(Dims d, int scheme, int dir, float vlo, float vhi,
+                                                  int32_t *__restrict__ row_ptr, int32_t *__restrict__ col,
+                                                  float *__restrict__ val)
+{
+    int64_t N = d.sxy * d.nz;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int x = (int)(i % d.nx);
+    int64_t t = i / d.nx;
+    int y = (int)(t % d.ny);
+    int z = (int)(t / d.ny);
+    int64_t miss;
+    int c, nd;
+    int64_t stride;
+    if (dir == 0) {
+        miss = 2 * ((int64_t)z * d.ny + y) + (x > 0);
+        c = x; nd = d.nx; stride = 1;
+    } else if (dir == 1) {
+        int my = (y == 0) + (y == d.ny - 1);
+        miss = 2 * (int64_t)d.nx * z + (int64_t)d.nx * (y > 0) + (int64_t)x * my;
+        c = y; nd = d.ny; stride = d.nx;
+    } else {
+        int mz = (z == 0) + (z == d.nz - 1);
+        miss = d.sxy * (z > 0) + ((int64_t)y * d.nx + x) * mz;
+        c = z; nd = d.nz; stride = d.sxy;
+    }
+    int64_t k = 2 * i - miss;
+    row_ptr[i] = (int32_t)k;
+    if (i == N - 1) row_ptr[N] = (int32_t)(2 * N - 2 * (N / nd));
+    if (c > 0) {
+        col[k] = (int32_t)(scheme == PIRO_DW ? i : i - stride);
+        val[k] = vlo;
+        ++k;
+    }
+    if (c < nd - 1) {
+        col[k] = (int32_t)(scheme == PIRO_UW ? i : i + stride);
+        val[k] = vhi;
+    }
+}
+
+// SpMDSM.cl:20-31 intended semantics, one thread per row.
+__global__ __launch_bounds__(256) void k_row_scale(int n_rows, const int32_t *__restrict__ rp,
+                                                   float *__restrict__ val, const float *__restrict__ diag)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    float s = diag[r];
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) val[k] = val[k] * s;
+}
+
+// src/ic.cpp:16-27,50-58 on padded coordinates idx*l/n_pad.
+__global__ __launch_bounds__(256) void k_ic_gauss(Dims d, float l0, float l1, float l2, float m0, float m1,
+                                                  float m2, float s0, float s1, float s2, float norm,
+                                                  float scale, float *__restrict__ out)
+{
+    int64_t N = d.sxy * d.nz;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int x = (int)(i % d.nx);
+    int64_t t = i / d.nx;
+    int y = (int)(t % d.ny);
+    int z = (int)(t / d.ny);
+    float cx = __fdiv_rn(__fmul_rn((float)x, l0), (float)d.nx);
+    float cy = __fdiv_rn(__fmul_rn((float)y, l1), (float)d.ny);
+    float cz = __fdiv_rn(__fmul_rn((float)z, l2), (float)d.nz);
+    float gx = __fdiv_rn(cx - m0, s0);
+    float gy = __fdiv_rn(cy - m1, s1);
+    float gz = __fdiv_rn(cz - m2, s2);
+    float q = __fadd_rn(__fadd_rn(__fmul_rn(gx, gx), __fmul_rn(gy, gy)), __fmul_rn(gz, gz));
+    float e = __fmul_rn(-0.5f, q);
+    out[i] = __fmul_rn(scale, __fmul_rn(norm, expf(e)));
+}
+
+__global__ __launch_bounds__(256) void k_ic_tophat(Dims d, float l0, float l1, float l2, float c0, float c1,
+                                                   float c2, float r2, float *__restrict__ out)
+{
+    int64_t N = d.sxy * d.nz;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int x = (int)(i % d.nx);
+    int64_t t = i / d.nx;
+    int y = (int)(t % d.ny);
+    int z = (int)(t / d.ny);
+    float dx = __fdiv_rn(__fmul_rn((float)x, l0), (float)d.nx) - c0;
+    float dy = __fdiv_rn(__fmul_rn((float)y, l1), (float)d.ny) - c1;
+    float dz = __fdiv_rn(__fmul_rn((float)z, l2), (float)d.nz) - c2;
+    float d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+    out[i] = d2 < r2 ? 1.0f : 0.0f;
+}
+
+__device__ __forceinline__ uint64_t smix(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_alpha_uniform(int64_t n, uint64_t seed, float lo, float w,
+                                                       float *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t h = smix(seed ^ smix((uint64_t)i));
+    float u = __fmul_rn((float)(h >> 40), 1.0f / 16777216.0f);
+    out[i] = __fadd_rn(lo, __fmul_rn(w, u));
+}
+
+// setbc (src/bc.cpp:67-78) over the face lists of src/bc.cpp:115-146.
+__global__ __launch_bounds__(256) void k_face_values(Dims d, const float *__restrict__ field, float f0, float f1,
+                                                     float f2, float f3, float f4, float f5,
+                                                     float *__restrict__ snap)
+{
+    int64_t N = d.sxy * d.nz;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int x = (int)(i % d.nx);
+    int64_t t = i / d.nx;
+    int y = (int)(t % d.ny);
+    int z = (int)(t / d.ny);
+    bool xi = x != 0 && x != d.nx - 1, yi = y != 0 && y != d.ny - 1, zi = z != 0 && z != d.nz - 1;
+    float v = field[i];
+    if (y == 0 && xi && zi) v = f3;
+    else if (y == d.ny - 1 && xi && zi) v = f2;
+    if (z == 0 && xi && yi) v = f4;
+    else if (z == d.nz - 1 && xi && yi) v = f5;
+    if (x == 0 && yi && zi) v = f1;
+    else if (x == d.nx - 1 && yi && zi) v = f0;
+    snap[i] = v;
+}
+
+// ------------------------------------------------------------ hash table
+
+// methods::sethash (src/methods.cpp:74-118) applied to every CSR entry in
+// CSR order.  Linear-probing layout depends on insertion order, so one lane
+// performs the insertions in exactly the reference's order; the table is
+// sparse (load ~0.2 at N=125 and falling with N), so a probe usually ends
+// at the home slot.  The other lanes of the block stage the next chunk of
+// (key, value) pairs in LDS.
+constexpr int HT_CHUNK = 1024;
+
+__global__ __launch_bounds__(256) void k_ht_build(int n_rows, const int32_t *__restrict__ rp,
+                                                  const int32_t *__restrict__ col,
+                                                  const float *__restrict__ val, int ts,
+                                                  int32_t *__restrict__ keys, float *__restrict__ hv,
+                                                  int *__restrict__ status)
+{
+    __shared__ int32_t s_key[HT_CHUNK];
+    __shared__ float s_val[HT_CHUNK];
+    __shared__ int s_fail;
+    if (threadIdx.x == 0) s_fail = 0;
+    const int64_t nnz = rp[n_rows];
+    for (int64_t base = 0; base < nnz; base += HT_CHUNK) {
+        __syncthreads();
+        int64_t cnt = nnz - base < HT_CHUNK ? nnz - base : HT_CHUNK;
+        for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+            int64_t k = base + j;
+            // row of entry k: binary search of row_ptr
+            int lo = 0, hi = n_rows;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (rp[mid] <= k) lo = mid; else hi = mid;
+            }
+            s_key[j] = lo * n_rows + col[k];
+            s_val[j] = val[k];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && !s_fail) {
+            for (int j = 0; j < cnt; ++j) {
+                int key = s_key[j];
+                int h = key % ts, attempts = 0;
+                bool done = false;
+                while (attempts < ts) {
+                    int kh = keys[h];
+                    if (kh == key || kh == -1) {
+                        keys[h] = key;
+                        hv[h] = s_val[j];
+                        done = true;
+                        break;
+                    }
+                    h = h + 1 == ts ? 0 : h + 1;
+                    ++attempts;
+                }
+                if (!done) { s_fail = 1; break; }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *status = s_fail;
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_ht_row_count(int n_rows, const int32_t *__restrict__ keys, int ts,
+                                                      int32_t *__restrict__ count)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ts) return;
+    int k = keys[i];
+    if (k >= 0) atomicAdd(&count[k / n_rows], 1);
+}
+
+__global__ __launch_bounds__(256) void k_ht_scatter(int n_rows, const int32_t *__restrict__ keys,
+                                                    const float *__restrict__ hv, int ts,
+                                                    const int32_t *__restrict__ rp, int32_t *__restrict__ cursor,
+                                                    int32_t *__restrict__ col, float *__restrict__ val)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ts) return;
+    int k = keys[i];
+    if (k < 0) return;
+    int r = k / n_rows;
+    int pos = rp[r] + atomicAdd(&cursor[r], 1);
+    col[pos] = k % n_rows;
+    val[pos] = hv[i];
+}
+
+// keys are unique, so sorting each row by column gives a deterministic CSR
+__global__ __launch_bounds__(256) void k_sort_rows(int n_rows, const int32_t *__restrict__ rp,
+                                                   int32_t *__restrict__ col, float *__restrict__ val)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    int b = rp[r], e = rp[r + 1];
+    for (int i = b + 1; i < e; ++i) {
+        int c = col[i];
+        float v = val[i];
+        int j = i - 1;
+        while (j >= b && col[j] > c) {
+            col[j + 1] = col[j];
+            val[j + 1] = val[j];
+            --j;
+        }
+        col[j + 1] = c;
+        val[j + 1] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_coo_count(int64_t nnz, const int32_t *__restrict__ rows,
+                                                   int32_t *__restrict__ count)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    // rows are sorted: the last entry of each row's run records the count
+    int r = rows[k];
+    if (k + 1 == nnz || rows[k + 1] != r) {
+        int64_t b = k;
+        // find the first entry of the run by binary search over [0, k]
+        int64_t lo = 0, hi = k;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (rows[mid] < r) lo = mid + 1; else hi = mid;
+        }
+        b = lo;
+        count[r] = (int32_t)(k - b + 1);
+    }
+}
+
+Dims mkdims(int nx, int ny, int nz)
+{
+    PIRO_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "grid dims must be >= 1");
+    int64_t N = (int64_t)nx * ny * nz;
+    PIRO_REQUIRE(N < 0x7fffffffLL, "grid exceeds int32 row indices");
+    return Dims{nx, ny, nz, (int64_t)nx * ny};
+}
+
+int64_t lap_nnz(int nx, int ny, int nz)
+{
+    int64_t N = (int64_t)nx * ny * nz;
+    return 7 * N - 2 * ((int64_t)ny * nz + (int64_t)nx * nz + (int64_t)nx * ny);
+}
+
+}  // namespace
+
+extern "C" {
+
+int piro_lap_nnz(int nx, int ny, int nz, int64_t *nnz)
+{
+    PIRO_API_BEGIN
+    PIRO_REQUIRE(nnz, "nnz is null");
+    mkdims(nx, ny, nz);
+    *nnz = lap_nnz(nx, ny, nz);
+    PIRO_API_END
+}
+
+int piro_assemble_laplacian_csr(piro_ctx *ctx, int nx, int ny, int nz, int scheme, float norm,
+                                int32_t *d_row_ptr, int32_t *d_col, float *d_val)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    Dims d = mkdims(nx, ny, nz);
+    PIRO_REQUIRE(scheme == PIRO_UW || scheme == PIRO_DW || scheme == PIRO_CD, "scheme must be 1, 2 or 3");
+    PIRO_REQUIRE(lap_nnz(nx, ny, nz) < 0x7fffffffLL, "nnz exceeds int32");
+    PIRO_REQUIRE(d_row_ptr && d_col && d_val, "null output");
+    // float(-6.0/double(norm)), float(1.0/double(norm)) as push_back stores them
+    float vself = (float)(-6.0 / (double)norm), vnb = (float)(1.0 / (double)norm);
+    int64_t N = d.sxy * nz;
+    k_lap_csr<false><<<grid_for(N, 256), 256, 0, ctx->stream>>>(d, scheme, vself, vnb, d_row_ptr, d_col, d_val);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_assemble_laplacian_coo(piro_ctx *ctx, int nx, int ny, int nz, int scheme, float norm, int32_t *d_rows,
+                                int32_t *d_col, float *d_val)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    Dims d = mkdims(nx, ny, nz);
+    PIRO_REQUIRE(scheme == PIRO_UW || scheme == PIRO_DW || scheme == PIRO_CD, "scheme must be 1, 2 or 3");
+    PIRO_REQUIRE(lap_nnz(nx, ny, nz) < 0x7fffffffLL, "nnz exceeds int32");
+    PIRO_REQUIRE(d_rows && d_col && d_val, "null output");
+    float vself = (float)(-6.0 / (double)norm), vnb = (float)(1.0 / (double)norm);
+    int64_t N = d.sxy * nz;
+    k_lap_csr<true><<<grid_for(N, 256), 256, 0, ctx->stream>>>(d, scheme, vself, vnb, d_rows, d_col, d_val);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_grad_nnz(int nx, int ny, int nz, int dir, int64_t *nnz)
+{
+    PIRO_API_BEGIN
+    PIRO_REQUIRE(nnz, "nnz is null");
+    PIRO_REQUIRE(dir >= 0 && dir <= 2, "dir must be 0, 1 or 2");
+    mkdims(nx, ny, nz);
+    int64_t N = (int64_t)nx * ny * nz;
+    int nd = dir == 0 ? nx : (dir == 1 ? ny : nz);
+    *nnz = 2 * N - 2 * (N / nd);
+    PIRO_API_END
+}
+
+int piro_assemble_gradient_csr(piro_ctx *ctx, int nx, int ny, int nz, int scheme, int dir, float norm,
+                               int32_t *d_row_ptr, int32_t *d_col, float *d_val)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    Dims d = mkdims(nx, ny, nz);
+    PIRO_REQUIRE(scheme == PIRO_UW || scheme == PIRO_DW || scheme == PIRO_CD, "scheme must be 1, 2 or 3");
+    PIRO_REQUIRE(dir >= 0 && dir <= 2, "dir must be 0, 1 or 2");
+    PIRO_REQUIRE(d_row_ptr && d_col && d_val, "null output");
+    // -1/norm and 1/norm are float divisions in the reference (int / float)
+    float vlo = -1.0f / norm, vhi = 1.0f / norm;
+    int64_t N = d.sxy * nz;
+    k_grad_csr<<<grid_for(N, 256), 256, 0, ctx->stream>>>(d, scheme, dir, vlo, vhi, d_row_ptr, d_col, d_val);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_row_scale(piro_ctx *ctx, int n_rows, const int32_t *d_row_ptr, float *d_val, const float *d_diag)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n_rows >= 0, "n_rows < 0");
+    PIRO_REQUIRE(d_row_ptr && d_val && d_diag, "null pointer");
+    if (n_rows == 0) return PIRO_OK;
+    k_row_scale<<<grid_for(n_rows, 256), 256, 0, ctx->stream>>>(n_rows, d_row_ptr, d_val, d_diag);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_ic_gaussian(piro_ctx *ctx, int nx, int ny, int nz, const float *h_l3, const float *h_mean3,
+                     const float *h_sigma3, float scale, float *d_out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    Dims d = mkdims(nx, ny, nz);
+    PIRO_REQUIRE(h_l3 && h_mean3 && h_sigma3 && d_out, "null pointer");
+    // float normalization = 1.0f / (std::pow(2.0f * M_PI, 1.5f) * s0*s1*s2),
+    // evaluated in double (M_PI is double), src/ic.cpp:24
+    double den = std::pow(2.0f * 3.14159265358979323846, (double)1.5f) * (double)h_sigma3[0] *
+                 (double)h_sigma3[1] * (double)h_sigma3[2];
+    float norm = (float)(1.0f / den);
+    int64_t N = d.sxy * nz;
+    k_ic_gauss<<<grid_for(N, 256), 256, 0, ctx->stream>>>(d, h_l3[0], h_l3[1], h_l3[2], h_mean3[0], h_mean3[1],
+                                                          h_mean3[2], h_sigma3[0], h_sigma3[1], h_sigma3[2], norm,
+                                                          scale, d_out);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_ic_tophat(piro_ctx *ctx, int nx, int ny, int nz, const float *h_l3, const float *h_center3, float radius,
+                   float *d_out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    Dims d = mkdims(nx, ny, nz);
+    PIRO_REQUIRE(h_l3 && h_center3 && d_out, "null pointer");
+    int64_t N = d.sxy * nz;
+    k_ic_tophat<<<grid_for(N, 256), 256, 0, ctx->stream>>>(d, h_l3[0], h_l3[1], h_l3[2], h_center3[0],
+                                                           h_center3[1], h_center3[2], radius * radius, d_out);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_alpha_uniform(piro_ctx *ctx, int64_t n, uint64_t seed, float lo, float hi, float *d_out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n >= 0 && d_out, "bad arguments");
+    if (n == 0) return PIRO_OK;
+    k_alpha_uniform<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, seed, lo, hi - lo, d_out);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_apply_face_values(piro_ctx *ctx, int nx, int ny, int nz, const float *d_field, const float *h_face_vals6,
+                           float *d_snap)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    Dims d = mkdims(nx, ny, nz);
+    PIRO_REQUIRE(d_field && h_face_vals6 && d_snap, "null pointer");
+    const float *f = h_face_vals6;
+    int64_t N = d.sxy * nz;
+    k_face_values<<<grid_for(N, 256), 256, 0, ctx->stream>>>(d, d_field, f[0], f[1], f[2], f[3], f[4], f[5],
+                                                             d_snap);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_ht_table_size(int sparsecount, int n_rows, float a, float b, float c, float load_factor, int *ts_out)
+{
+    PIRO_API_BEGIN
+    PIRO_REQUIRE(ts_out, "ts_out is null");
+    // src/operatorlaplacian.cpp:184-185
+    float load = (float)((double)a * std::pow((double)n_rows, (double)b) + (double)c);
+    float ts = (float)sparsecount / (load * load_factor);
+    PIRO_REQUIRE(load > 0.0f && ts >= 1.0f && ts < 2147483647.0f,
+                 "hash-table size fit is out of range for this N (load = a*N^b + c <= 0 "
+                 "beyond N ~ 18652 with the shipped hashtable.ini)");
+    *ts_out = (int)ts;
+    PIRO_API_END
+}
+
+int piro_ht_build(piro_ctx *ctx, int n_rows, const int32_t *d_row_ptr, const int32_t *d_col, const float *d_val,
+                  int ts, int32_t *d_keys, float *d_vals)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n_rows > 0 && n_rows <= 46340, "hash keys row*N+col need N <= 46340 (int32)");
+    PIRO_REQUIRE(ts > 0, "table size must be positive");
+    PIRO_REQUIRE(d_row_ptr && d_col && d_val && d_keys && d_vals, "null pointer");
+    k_fill_i32<<<grid_for(ts, 256), 256, 0, ctx->stream>>>(d_keys, ts, -1);
+    check_launch();
+    PIRO_CHECK_HIP(hipMemsetAsync(d_vals, 0, (size_t)ts * sizeof(float), ctx->stream));
+    int *status = (int *)ctx->scratch[0].get(sizeof(int));
+    k_ht_build<<<1, 256, 0, ctx->stream>>>(n_rows, d_row_ptr, d_col, d_val, ts, d_keys, d_vals, status);
+    check_launch();
+    int h = 0;
+    PIRO_CHECK_HIP(hipMemcpyAsync(&h, status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    if (h) throw Error(PIRO_EFULL, "hash table full; reduce LoadFactor (src/methods.cpp:109-116)");
+    PIRO_API_END
+}
+
+int piro_ht_count(piro_ctx *ctx, int n_rows, const int32_t *d_keys, int ts, int64_t *nnz_out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n_rows > 0 && ts > 0 && d_keys && nnz_out, "bad arguments");
+    int32_t *count = (int32_t *)ctx->scratch[1].get((size_t)(n_rows + 1) * sizeof(int32_t) * 2);
+    PIRO_CHECK_HIP(hipMemsetAsync(count, 0, (size_t)n_rows * sizeof(int32_t), ctx->stream));
+    k_ht_row_count<<<grid_for(ts, 256), 256, 0, ctx->stream>>>(n_rows, d_keys, ts, count);
+    check_launch();
+    int32_t *tmp_rp = count + n_rows + 1;
+    *nnz_out = exclusive_scan_i32(ctx, 2, count, n_rows, tmp_rp);
+    PIRO_API_END
+}
+
+int piro_ht_to_csr(piro_ctx *ctx, int n_rows, const int32_t *d_keys, const float *d_vals, int ts,
+                   int32_t *d_row_ptr, int32_t *d_col, float *d_val)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n_rows > 0 && ts > 0 && d_keys && d_vals && d_row_ptr && d_col && d_val, "bad arguments");
+    int32_t *count = (int32_t *)ctx->scratch[1].get((size_t)(n_rows + 1) * sizeof(int32_t) * 2);
+    PIRO_CHECK_HIP(hipMemsetAsync(count, 0, (size_t)n_rows * sizeof(int32_t), ctx->stream));
+    k_ht_row_count<<<grid_for(ts, 256), 256, 0, ctx->stream>>>(n_rows, d_keys, ts, count);
+    check_launch();
+    exclusive_scan_i32(ctx, 2, count, n_rows, d_row_ptr);
+    PIRO_CHECK_HIP(hipMemsetAsync(count, 0, (size_t)n_rows * sizeof(int32_t), ctx->stream));
+    k_ht_scatter<<<grid_for(ts, 256), 256, 0, ctx->stream>>>(n_rows, d_keys, d_vals, ts, d_row_ptr, count, d_col,
+                                                             d_val);
+    check_launch();
+    k_sort_rows<<<grid_for(n_rows, 256), 256, 0, ctx->stream>>>(n_rows, d_row_ptr, d_col, d_val);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_coo_row_ptr(piro_ctx *ctx, int n_rows, int64_t nnz, const int32_t *d_rows, int32_t *d_row_ptr)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n_rows > 0 && nnz >= 0 && d_rows && d_row_ptr, "bad arguments");
+    int32_t *count = (int32_t *)ctx->scratch[1].get((size_t)n_rows * sizeof(int32_t));
+    PIRO_CHECK_HIP(hipMemsetAsync(count, 0, (size_t)n_rows * sizeof(int32_t), ctx->stream));
+    if (nnz > 0) {
+        k_coo_count<<<grid_for(nnz, 256), 256, 0, ctx->stream>>>(nnz, d_rows, count);
+        check_launch();
+    }
+    int64_t tot = exclusive_scan_i32(ctx, 2, count, n_rows, d_row_ptr);
+    PIRO_REQUIRE(tot == nnz, "COO rows must be sorted and within [0, n_rows)");
+    PIRO_API_END
+}
+
+}  // extern "C"

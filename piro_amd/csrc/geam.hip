@@ -1,0 +1,171 @@
+// geam.hip -- sparse matrix add/merge.
+//
+// Replaces kernelmethods::csrgeam (src/kernelcsrgeam.cpp:9-86): csrgeam_1
+// (count), N serial single-row prefixsum launches with a clFinish each
+// (O(N^2) work), csrgeam_3 (fill).  Here: count -> one device-wide scan ->
+// fill, one lane per row.  The reference's merge never merges equal columns
+// (csrgeam_1.cl:34-51: the colA==min test fires first), so its count is
+// exactly nnzA+nnzB+nnzC per row; the fill reproduces its tie order (A before
+// B before C) and its tail order (A, then B, then C, csrgeam_3.cl:138-155).
+//
+// geam2 is the build's own true merge D = alpha*A + beta*B (sorted union,
+// equal columns summed), used to form I - dt*alpha*L.
+#include "device_utils.hpp"
+
+using namespace piro;
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_geam3_count(int n, const int32_t *__restrict__ rpa,
+                                                     const int32_t *__restrict__ rpb,
+                                                     const int32_t *__restrict__ rpc, int32_t *__restrict__ cnt)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    cnt[r] = (rpa[r + 1] - rpa[r]) + (rpb[r + 1] - rpb[r]) + (rpc[r + 1] - rpc[r]);
+}
+
+// csrgeam_3.cl:88-156
+__global__ __launch_bounds__(256) void k_geam3_fill(int n, const int32_t *__restrict__ rpa,
+                                                    const int32_t *__restrict__ ca, const float *__restrict__ va,
+                                                    const int32_t *__restrict__ rpb,
+                                                    const int32_t *__restrict__ cb, const float *__restrict__ vb,
+                                                    const int32_t *__restrict__ rpc,
+                                                    const int32_t *__restrict__ cc, const float *__restrict__ vc,
+                                                    const float *__restrict__ v3, const int32_t *__restrict__ rpd,
+                                                    int32_t *__restrict__ cd, float *__restrict__ vd)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int i = rpa[r], ie = rpa[r + 1], j = rpb[r], je = rpb[r + 1], k = rpc[r], ke = rpc[r + 1];
+    int d = rpd[r];
+    const float sa = v3[r], sb = v3[(int64_t)n + r], sc = v3[2 * (int64_t)n + r];
+    while (i < ie && j < je && k < ke) {
+        int a = ca[i], b = cb[j], c = cc[k];
+        int mn = min(a, min(c, b));
+        if (a == mn) { cd[d] = a; vd[d] = va[i] * sa; ++i; }
+        else if (b == mn) { cd[d] = b; vd[d] = vb[j] * sb; ++j; }
+        else { cd[d] = c; vd[d] = vc[k] * sc; ++k; }
+        ++d;
+    }
+    for (; i < ie; ++i, ++d) { cd[d] = ca[i]; vd[d] = va[i] * sa; }
+    for (; j < je; ++j, ++d) { cd[d] = cb[j]; vd[d] = vb[j] * sb; }
+    for (; k < ke; ++k, ++d) { cd[d] = cc[k]; vd[d] = vc[k] * sc; }
+}
+
+__global__ __launch_bounds__(256) void k_geam2_count(int n, const int32_t *__restrict__ rpa,
+                                                     const int32_t *__restrict__ ca,
+                                                     const int32_t *__restrict__ rpb,
+                                                     const int32_t *__restrict__ cb, int32_t *__restrict__ cnt,
+                                                     int *__restrict__ bad)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int i = rpa[r], ie = rpa[r + 1], j = rpb[r], je = rpb[r + 1], m = 0;
+    for (int t = i + 1; t < ie; ++t)
+        if (ca[t] <= ca[t - 1]) *bad = 1;
+    for (int t = j + 1; t < je; ++t)
+        if (cb[t] <= cb[t - 1]) *bad = 1;
+    while (i < ie || j < je) {
+        if (j >= je || (i < ie && ca[i] < cb[j])) ++i;
+        else if (i >= ie || cb[j] < ca[i]) ++j;
+        else { ++i; ++j; }
+        ++m;
+    }
+    cnt[r] = m;
+}
+
+__global__ __launch_bounds__(256) void k_geam2_fill(int n, float alpha, const int32_t *__restrict__ rpa,
+                                                    const int32_t *__restrict__ ca, const float *__restrict__ va,
+                                                    float beta, const int32_t *__restrict__ rpb,
+                                                    const int32_t *__restrict__ cb, const float *__restrict__ vb,
+                                                    const int32_t *__restrict__ rpd, int32_t *__restrict__ cd,
+                                                    float *__restrict__ vd)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int i = rpa[r], ie = rpa[r + 1], j = rpb[r], je = rpb[r + 1], d = rpd[r];
+    while (i < ie || j < je) {
+        if (j >= je || (i < ie && ca[i] < cb[j])) { cd[d] = ca[i]; vd[d] = alpha * va[i]; ++i; }
+        else if (i >= ie || cb[j] < ca[i]) { cd[d] = cb[j]; vd[d] = beta * vb[j]; ++j; }
+        else {
+            float ta = alpha * va[i], tb = beta * vb[j];
+            cd[d] = ca[i];
+            vd[d] = ta + tb;
+            ++i; ++j;
+        }
+        ++d;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int piro_geam3_rowptr(piro_ctx *ctx, int n_rows, const int32_t *d_rpa, const int32_t *d_ca, const int32_t *d_rpb,
+                      const int32_t *d_cb, const int32_t *d_rpc, const int32_t *d_cc, int32_t *d_row_ptr,
+                      int64_t *nnz_out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n_rows > 0 && d_rpa && d_rpb && d_rpc && d_row_ptr, "bad arguments");
+    (void)d_ca; (void)d_cb; (void)d_cc;
+    int32_t *cnt = (int32_t *)ctx->scratch[1].get((size_t)n_rows * sizeof(int32_t));
+    k_geam3_count<<<grid_for(n_rows, 256), 256, 0, ctx->stream>>>(n_rows, d_rpa, d_rpb, d_rpc, cnt);
+    check_launch();
+    int64_t nnz = exclusive_scan_i32(ctx, 2, cnt, n_rows, d_row_ptr);
+    if (nnz_out) *nnz_out = nnz;
+    PIRO_API_END
+}
+
+int piro_geam3_fill(piro_ctx *ctx, int n_rows, const int32_t *d_rpa, const int32_t *d_ca, const float *d_va,
+                    const int32_t *d_rpb, const int32_t *d_cb, const float *d_vb, const int32_t *d_rpc,
+                    const int32_t *d_cc, const float *d_vc, const float *d_v3, const int32_t *d_row_ptr,
+                    int32_t *d_col, float *d_val)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n_rows > 0 && d_rpa && d_ca && d_va && d_rpb && d_cb && d_vb && d_rpc && d_cc && d_vc && d_v3 &&
+                     d_row_ptr && d_col && d_val,
+                 "bad arguments");
+    k_geam3_fill<<<grid_for(n_rows, 256), 256, 0, ctx->stream>>>(n_rows, d_rpa, d_ca, d_va, d_rpb, d_cb, d_vb, d_rpc,
+                                                                 d_cc, d_vc, d_v3, d_row_ptr, d_col, d_val);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_geam2_rowptr(piro_ctx *ctx, int n_rows, const int32_t *d_rpa, const int32_t *d_ca, const int32_t *d_rpb,
+                      const int32_t *d_cb, int32_t *d_row_ptr, int64_t *nnz_out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n_rows > 0 && d_rpa && d_ca && d_rpb && d_cb && d_row_ptr, "bad arguments");
+    int32_t *cnt = (int32_t *)ctx->scratch[1].get((size_t)n_rows * sizeof(int32_t) + 64);
+    int *bad = (int *)(cnt + n_rows + 1);
+    PIRO_CHECK_HIP(hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
+    k_geam2_count<<<grid_for(n_rows, 256), 256, 0, ctx->stream>>>(n_rows, d_rpa, d_ca, d_rpb, d_cb, cnt, bad);
+    check_launch();
+    int hbad = 0;
+    PIRO_CHECK_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    PIRO_REQUIRE(!hbad, "geam2 needs rows with strictly increasing columns");
+    int64_t nnz = exclusive_scan_i32(ctx, 2, cnt, n_rows, d_row_ptr);
+    if (nnz_out) *nnz_out = nnz;
+    PIRO_API_END
+}
+
+int piro_geam2_fill(piro_ctx *ctx, int n_rows, float alpha, const int32_t *d_rpa, const int32_t *d_ca,
+                    const float *d_va, float beta, const int32_t *d_rpb, const int32_t *d_cb, const float *d_vb,
+                    const int32_t *d_row_ptr, int32_t *d_col, float *d_val)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n_rows > 0 && d_rpa && d_ca && d_va && d_rpb && d_cb && d_vb && d_row_ptr && d_col && d_val,
+                 "bad arguments");
+    k_geam2_fill<<<grid_for(n_rows, 256), 256, 0, ctx->stream>>>(n_rows, alpha, d_rpa, d_ca, d_va, beta, d_rpb,
+                                                                 d_cb, d_vb, d_row_ptr, d_col, d_val);
+    check_launch();
+    PIRO_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+// jli.hip -- backward-Euler Richardson ("JLI") solve on the device.
+//
+// The reference (src/kernelprocess.cpp:123-209) launches SpMVCSR_I1,
+// subtractVectors and addVectors_URF with a clFinish after each, then reads
+// the whole N-float update vector back to the host to sum its squares, every
+// iteration.  Here one fused kernel per iteration computes
+//   c = b - (interior ? x - sum fl(fl(dt*v)*x[col]) : 0);  x' = x + c*urf
+// and accumulates sum (x'-x)^2 in FP64 on the device; a one-thread kernel
+// evaluates the reference's stopping rule res = sqrt(s)/sqrt(N),
+// (res - res_old)^2 < tol and raises a device flag that turns the remaining
+// queued iterations into no-ops.  The host looks at 8 bytes every
+// `JLI_BATCH` iterations instead of N floats every iteration.
+#include "device_utils.hpp"
+
+#include <cmath>
+
+using namespace piro;
+
+namespace piro {
+void jli_iteration(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *rp, const int32_t *col, const float *val,
+                   int64_t nnz, float dt, const float *x, const float *b, float urf, float *x_new, double *acc,
+                   const int *done);
+}
+
+namespace {
+
+constexpr int JLI_BATCH = 16;
+
+struct JliState {
+    double acc;       // sum of squared updates of the current iteration
+    float res_old;
+    float last_res;
+    int done;
+    int iters;        // iterations executed (including the converging one)
+    int final_buf;    // 0: result in x, 1: result in work
+};
+
+__global__ void k_jli_init(JliState *st)
+{
+    st->acc = 0.0;
+    st->res_old = 0.0f;
+    st->last_res = 0.0f;
+    st->done = 0;
+    st->iters = 0;
+    st->final_buf = 0;
+}
+
+// src/kernelprocess.cpp:197-204 on the FP64 device sum
+__global__ void k_jli_check(JliState *st, float sqrt_n, float tol, int it, int buf_written)
+{
+    if (st->done) return;
+    float res = sqrtf((float)st->acc) / sqrt_n;
+    float d = (res - st->res_old) * (res - st->res_old);
+    st->last_res = res;
+    st->iters = it + 1;
+    st->final_buf = buf_written;
+    if (d < tol) st->done = 1;
+    st->res_old = res;
+    st->acc = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_copy(int64_t n, const float *__restrict__ a, float *__restrict__ b)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+}  // namespace
+
+extern "C" int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row_ptr, const int32_t *d_col,
+                              const float *d_val, int64_t nnz, float dt, const float *d_b, float urf, float tol,
+                              int max_iter, float *d_x, float *d_work, int *iters, float *res)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && (int64_t)nx * ny * nz < 0x7fffffffLL, "bad grid");
+    PIRO_REQUIRE(d_b && d_x && d_work && d_x != d_work && d_x != d_b && d_work != d_b, "bad buffers");
+    PIRO_REQUIRE(max_iter >= 0, "max_iter < 0");
+    const int64_t N = (int64_t)nx * ny * nz;
+    JliState *st = (JliState *)ctx->scratch[0].get(sizeof(JliState));
+    k_jli_init<<<1, 1, 0, ctx->stream>>>(st);
+    check_launch();
+    // x0 = b (src/kernelprocess.cpp:146)
+    k_copy<<<grid_for(N, 256), 256, 0, ctx->stream>>>(N, d_b, d_x);
+    check_launch();
+    const float sqrt_n = sqrtf((float)N);
+    float *bufs[2] = {d_x, d_work};
+    JliState h{};
+    for (int it = 0; it < max_iter; ++it) {
+        const float *src = bufs[it & 1];
+        float *dst = bufs[(it + 1) & 1];
+        jli_iteration(ctx, nx, ny, nz, d_row_ptr, d_col, d_val, nnz, dt, src, d_b, urf, dst, &st->acc,
+                      &st->done);
+        k_jli_check<<<1, 1, 0, ctx->stream>>>(st, sqrt_n, tol, it, (it + 1) & 1);
+        check_launch();
+        if ((it + 1) % JLI_BATCH == 0 || it + 1 == max_iter) {
+            PIRO_CHECK_HIP(hipMemcpyAsync(&h, st, sizeof(JliState), hipMemcpyDeviceToHost, ctx->stream));
+            PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+            if (h.done) break;
+        }
+    }
+    if (max_iter == 0) {
+        h.iters = 0;
+        h.final_buf = 0;
+    }
+    if (h.final_buf == 1) {
+        k_copy<<<grid_for(N, 256), 256, 0, ctx->stream>>>(N, d_work, d_x);
+        check_launch();
+    }
+    PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    if (iters) *iters = h.iters;
+    if (res) *res = h.last_res;
+    PIRO_API_END
+}

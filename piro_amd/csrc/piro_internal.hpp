@@ -1,0 +1,110 @@
+// piro_internal.hpp -- shared plumbing of libpiro_hip (gfx950 only).
+//
+// Every extern "C" entry point goes through PIRO_API_BEGIN/END: errors are
+// turned into a status code plus a thread-local message (piro_last_error),
+// never an exception or exit across the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/piro_hip.h"
+
+namespace piro {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define PIRO_CHECK_HIP(expr)                                                   \
+    do {                                                                       \
+        hipError_t _e = (expr);                                                \
+        if (_e != hipSuccess)                                                  \
+            throw ::piro::Error(PIRO_EHIP, std::string(#expr) + ": " +         \
+                                               hipGetErrorString(_e));         \
+    } while (0)
+
+#define PIRO_REQUIRE(cond, msg)                                                \
+    do {                                                                       \
+        if (!(cond)) throw ::piro::Error(PIRO_EINVAL, (msg));                  \
+    } while (0)
+
+#define PIRO_API_BEGIN try {
+#define PIRO_API_END                                                           \
+    return PIRO_OK;                                                            \
+    }                                                                          \
+    catch (const ::piro::Error &e) {                                           \
+        ::piro::set_last_error(e.what());                                      \
+        return e.code;                                                         \
+    }                                                                          \
+    catch (const std::exception &e) {                                          \
+        ::piro::set_last_error(e.what());                                      \
+        return PIRO_EINTERNAL;                                                 \
+    }                                                                          \
+    catch (...) {                                                              \
+        ::piro::set_last_error("unknown error");                               \
+        return PIRO_EINTERNAL;                                                 \
+    }
+
+// Growable device scratch owned by a context (reduction partials, geam
+// counts, LU panels ...).  Reallocation only grows.
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void *get(size_t need) {
+        if (need > bytes) {
+            if (ptr) PIRO_CHECK_HIP(hipFree(ptr));
+            ptr = nullptr;
+            bytes = 0;
+            size_t n = need < (1u << 20) ? (1u << 20) : need;
+            PIRO_CHECK_HIP(hipMalloc(&ptr, n));
+            bytes = n;
+        }
+        return ptr;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+}  // namespace piro
+
+struct piro_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;   // null = legacy default stream
+    bool own_stream = false;
+    int num_cus = 256;
+    int num_xcds = 8;
+    piro::Scratch scratch[4];
+    // autotuned FE-step choice keyed by the last matrix shape
+    int fe_variant = -1;
+    int64_t fe_tuned_rows = -1, fe_tuned_nnz = -1;
+};
+
+namespace piro {
+
+inline piro_ctx *ctx_of(piro_ctx *c) {
+    if (!c) throw Error(PIRO_EINVAL, "null context");
+    PIRO_CHECK_HIP(hipSetDevice(c->device));
+    return c;
+}
+
+inline unsigned grid_for(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 0x7fffffff) throw Error(PIRO_EINVAL, "grid too large");
+    return (unsigned)g;
+}
+
+inline void check_launch() { PIRO_CHECK_HIP(hipGetLastError()); }
+
+}  // namespace piro

@@ -1,0 +1,476 @@
+// spmv.hip -- CSR SpMV time stepping on gfx950.
+//
+// The reference runs one work-item per row with scalar strided loads
+// (SpMVCSR.cl:1-40), a clFinish, a buffer copy and a separate setBC launch
+// per step (src/kernelprocess.cpp:10-40, src/process.cpp:234, src/bc.cpp:
+// 38-65).  Here one kernel does the whole step:
+//
+//   * a workgroup owns RB consecutive rows; their non-zeros are one
+//     contiguous range of col/val, streamed with 16-byte non-temporal loads
+//     (every lane busy, no strided access) while the x gathers go through the
+//     cache hierarchy (the 7-point neighbours of a row block sit in three
+//     short runs of x);
+//   * each lane forms the products fl(fl(dt*v)*x[c]) of its 4 entries and
+//     stores them to LDS;
+//   * each lane then sums its own row's products from LDS in CSR order, so
+//     the rounding sequence is exactly the reference's sequential loop and
+//     every variant (and the CPU oracle) produces identical bits;
+//   * the interior mask, the face-BC rewrite and the edge zeroing are applied
+//     in the same pass, and y is written once, coalesced.
+//
+// Variants differ only in rows per workgroup, the load path and the block ->
+// row mapping (XCD-contiguous or the hardware's round robin); all compute
+// the same bits, and an autotuner picks the fastest for the operand shape.
+#include "device_utils.hpp"
+
+#include <vector>
+
+using namespace piro;
+
+namespace {
+
+typedef int int4v __attribute__((ext_vector_type(4)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+enum Mode { FE_INTERIOR = 0, ALLROWS = 1, SPMV = 2, JLI = 3 };
+
+// exact n / d for 0 <= n < 2^31, 1 <= d < 2^31 (round-up multiplier)
+struct FastDiv {
+    uint64_t m;
+    uint32_t d;
+    int s;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t div) : d(div)
+    {
+        s = 0;
+        while ((1ull << s) < div) ++s;
+        m = (((unsigned __int128)1 << (32 + s)) + div - 1) / div;  // <= 2^33
+    }
+    // n < 2^31 and m <= 2^33 keep n*m below 2^64
+    __device__ __forceinline__ uint32_t div(uint32_t n) const { return (uint32_t)(((uint64_t)n * m) >> (32 + s)); }
+};
+
+struct Grid3 {
+    FastDiv fx, fxy;
+    int nx, ny, nz;
+};
+
+// 0 interior, 1 face (exactly one coordinate on the boundary), 2 edge/corner
+__device__ __forceinline__ int row_class(uint32_t r, const Grid3 &g)
+{
+    uint32_t z = g.fxy.div(r);
+    uint32_t rem = r - z * (uint32_t)g.fxy.d;
+    uint32_t y = g.fx.div(rem);
+    uint32_t x = rem - y * (uint32_t)g.nx;
+    int bx = (x == 0) | (x == (uint32_t)g.nx - 1);
+    int by = (y == 0) | (y == (uint32_t)g.ny - 1);
+    int bz = (z == 0) | (z == (uint32_t)g.nz - 1);
+    int nb = bx + by + bz;
+    return nb == 0 ? 0 : (nb == 1 ? 1 : 2);
+}
+
+struct Params {
+    const int32_t *rp;
+    const int32_t *col;
+    const float *val;
+    int64_t nnz;
+    int n_rows;
+    float dt;
+    const float *x;
+    const float *snap;   // FE: face values (nullable); JLI: b
+    float *y;            // FE/SPMV/ALLROWS: output; JLI: x_new
+    double *acc;         // JLI: sum of squared updates (atomic)
+    const int *done;     // JLI: converged flag; set -> the kernel is a no-op
+    float urf;
+    Grid3 g;
+    int n_blocks;        // logical blocks
+    int per_xcd;         // XCD remap: logical blocks per XCD slot (0 = off)
+};
+
+template <int MODE>
+__device__ __forceinline__ float product(float v, float xc, float dt)
+{
+    if (MODE == FE_INTERIOR || MODE == JLI) {
+        float t = dt * v;     // (deltat * value_A) ...
+        return t * xc;        // ... * B[col_A]   (SpMVCSR.cl:25)
+    } else {
+        return v * xc;        // values_A[ind] * B[columns_A[ind]] (SpMVCOO.cl:20)
+    }
+}
+
+// epilogue of one row given its product sum; returns the JLI squared update
+template <int MODE>
+__device__ __forceinline__ double finish_row(const Params &p, int r, float acc, bool have_acc)
+{
+    if (MODE == FE_INTERIOR) {
+        int cls = row_class((uint32_t)r, p.g);
+        float out;
+        if (cls == 0) out = acc + p.x[r];                       // SpMVCSR.cl:34
+        else if (cls == 1) out = p.snap ? p.snap[r] : 0.0f;     // setBC.cl:7
+        else out = 0.0f;                                        // zeroed partD
+        p.y[r] = out;
+        return 0.0;
+    } else if (MODE == ALLROWS) {
+        float c = p.dt * acc;                                   // SpMVCOO.cl:24
+        p.y[r] = p.x[r] + c;                                    // addVectors.cl:7
+        return 0.0;
+    } else if (MODE == SPMV) {
+        p.y[r] = acc;
+        return 0.0;
+    } else {  // JLI, src/kernelprocess.cpp:156-189
+        int cls = row_class((uint32_t)r, p.g);
+        float xr = p.x[r];
+        float ci = cls == 0 ? xr - acc : 0.0f;                  // SpMVCSR_I1.cl:32
+        float c = p.snap[r] - ci;                               // subtractVectors.cl:7
+        float inc = c * p.urf;
+        float xn = xr + inc;                                    // addVectors_URF.cl:8
+        float d = xn - xr;                                      // addVectors_URF.cl:9
+        p.y[r] = xn;
+        (void)have_acc;
+        return (double)d * (double)d;
+    }
+}
+
+__device__ __forceinline__ int logical_block(const Params &p)
+{
+    int b = blockIdx.x;
+    if (p.per_xcd > 0) b = (b & 7) * p.per_xcd + (b >> 3);
+    return b;
+}
+
+// ------------------------------------------------ LDS-staged stream kernel
+
+template <int RB, int MODE, bool VEC>
+__global__ __launch_bounds__(256) void k_stream(Params p)
+{
+    constexpr int CAP = RB * 8 + 8;  // products staged per block
+    __shared__ int32_t s_rp[RB + 1];
+    __shared__ __attribute__((aligned(16))) float s_p[CAP];
+
+    const int b = logical_block(p);
+    if (b >= p.n_blocks) return;
+    if (MODE == JLI && *p.done) return;
+    const int r0 = b * RB;
+    const int nr = min(RB, p.n_rows - r0);
+    for (int j = threadIdx.x; j <= nr; j += 256) s_rp[j] = __builtin_nontemporal_load(p.rp + r0 + j);
+    __syncthreads();
+    const int k0 = s_rp[0], k1 = s_rp[nr];
+    const int ka = k0 & ~3;
+    double jli_sq = 0.0;
+    if (k1 - ka <= CAP - 4) {
+        if (VEC) {
+            const int q0 = ka >> 2, q1 = (k1 + 3) >> 2;
+            for (int q = q0 + threadIdx.x; q < q1; q += 256) {
+                const int kb = q << 2;
+                int4v c;
+                float4v v;
+                if ((int64_t)kb + 3 < p.nnz) {
+                    c = __builtin_nontemporal_load((const int4v *)(p.col + kb));
+                    v = __builtin_nontemporal_load((const float4v *)(p.val + kb));
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        bool ok = (int64_t)kb + e < p.nnz;
+                        c[e] = ok ? p.col[kb + e] : 0;
+                        v[e] = ok ? p.val[kb + e] : 0.0f;
+                    }
+                }
+                float4v out;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int k = kb + e;
+                    out[e] = (k >= k0 && k < k1) ? product<MODE>(v[e], p.x[c[e]], p.dt) : 0.0f;
+                }
+                *(float4v *)(s_p + (kb - ka)) = out;
+            }
+        } else {
+            for (int k = k0 + threadIdx.x; k < k1; k += 256) {
+                int c = __builtin_nontemporal_load(p.col + k);
+                float v = __builtin_nontemporal_load(p.val + k);
+                s_p[k - ka] = product<MODE>(v, p.x[c], p.dt);
+            }
+        }
+        __syncthreads();
+        for (int rr = threadIdx.x; rr < nr; rr += 256) {
+            const int b0 = s_rp[rr] - ka, b1 = s_rp[rr + 1] - ka;
+            float acc = 0.0f;
+            for (int k = b0; k < b1; ++k) acc = acc + s_p[k];
+            jli_sq += finish_row<MODE>(p, r0 + rr, acc, true);
+        }
+    } else {
+        // long rows: each lane walks its own row straight from memory
+        for (int rr = threadIdx.x; rr < nr; rr += 256) {
+            float acc = 0.0f;
+            for (int k = s_rp[rr]; k < s_rp[rr + 1]; ++k)
+                acc = acc + product<MODE>(p.val[k], p.x[p.col[k]], p.dt);
+            jli_sq += finish_row<MODE>(p, r0 + rr, acc, true);
+        }
+    }
+    if (MODE == JLI) {
+        double s = block_sum_f64<256>(jli_sq);
+        if (threadIdx.x == 0) atomicAdd(p.acc, s);
+    }
+}
+
+// ------------------------------------------------------- row per lane
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_row_lane(Params p)
+{
+    const int b = logical_block(p);
+    if (b >= p.n_blocks) return;
+    if (MODE == JLI && *p.done) return;
+    const int r = b * 256 + threadIdx.x;
+    double jli_sq = 0.0;
+    if (r < p.n_rows) {
+        const int kb = __builtin_nontemporal_load(p.rp + r), ke = __builtin_nontemporal_load(p.rp + r + 1);
+        float acc = 0.0f;
+        for (int k = kb; k < ke; ++k)
+            acc = acc + product<MODE>(__builtin_nontemporal_load(p.val + k), p.x[__builtin_nontemporal_load(p.col + k)],
+                                      p.dt);
+        jli_sq = finish_row<MODE>(p, r, acc, true);
+    }
+    if (MODE == JLI) {
+        double s = block_sum_f64<256>(jli_sq);
+        if (threadIdx.x == 0) atomicAdd(p.acc, s);
+    }
+}
+
+// ----------------------------------------------------------- dispatch
+
+struct Variant {
+    const char *name;
+    int rows_per_block;   // 0 = row-per-lane
+    bool vec;
+    bool xcd;
+};
+
+const Variant kVariants[] = {
+    {"stream_rb256_vec", 256, true, false},
+    {"stream_rb512_vec", 512, true, false},
+    {"stream_rb1024_vec", 1024, true, false},
+    {"stream_rb256_vec_xcd", 256, true, true},
+    {"stream_rb512_vec_xcd", 512, true, true},
+    {"stream_rb1024_vec_xcd", 1024, true, true},
+    {"stream_rb512_scalar", 512, false, false},
+    {"row_per_lane", 0, false, false},
+    {"row_per_lane_xcd", 0, false, true},
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+template <int MODE>
+void launch_mode(const Variant &v, Params p, hipStream_t s)
+{
+    int rpb = v.rows_per_block ? v.rows_per_block : 256;
+    int nb = (int)((p.n_rows + (int64_t)rpb - 1) / rpb);
+    if (nb < 1) nb = 1;
+    p.n_blocks = nb;
+    int grid = nb;
+    p.per_xcd = 0;
+    if (v.xcd) {
+        int per = (nb + 7) / 8;
+        p.per_xcd = per;
+        grid = per * 8;
+    }
+    switch (v.rows_per_block) {
+    case 256:
+        if (v.vec) k_stream<256, MODE, true><<<grid, 256, 0, s>>>(p);
+        else k_stream<256, MODE, false><<<grid, 256, 0, s>>>(p);
+        break;
+    case 512:
+        if (v.vec) k_stream<512, MODE, true><<<grid, 256, 0, s>>>(p);
+        else k_stream<512, MODE, false><<<grid, 256, 0, s>>>(p);
+        break;
+    case 1024:
+        if (v.vec) k_stream<1024, MODE, true><<<grid, 256, 0, s>>>(p);
+        else k_stream<1024, MODE, false><<<grid, 256, 0, s>>>(p);
+        break;
+    default:
+        k_row_lane<MODE><<<grid, 256, 0, s>>>(p);
+        break;
+    }
+    check_launch();
+}
+
+void launch(int mode, int variant, const Params &p, hipStream_t s)
+{
+    const Variant &v = kVariants[variant];
+    switch (mode) {
+    case FE_INTERIOR: launch_mode<FE_INTERIOR>(v, p, s); break;
+    case ALLROWS: launch_mode<ALLROWS>(v, p, s); break;
+    case SPMV: launch_mode<SPMV>(v, p, s); break;
+    default: launch_mode<JLI>(v, p, s); break;
+    }
+}
+
+bool vec_ok(const void *col, const void *val)
+{
+    return (((uintptr_t)col | (uintptr_t)val) & 15) == 0;
+}
+
+Params make_params(int nx, int ny, int nz, int n_rows, const int32_t *rp, const int32_t *col, const float *val,
+                   int64_t nnz, float dt, const float *x, const float *snap, float *y)
+{
+    PIRO_REQUIRE(rp && col && val && x && y, "null operand");
+    PIRO_REQUIRE(nnz >= 0 && nnz < 0x7fffffffLL, "nnz must fit int32");
+    PIRO_REQUIRE(n_rows > 0, "n_rows must be positive");
+    PIRO_REQUIRE((const void *)x != (const void *)y, "x and y must not alias");
+    Params p{};
+    p.rp = rp; p.col = col; p.val = val; p.nnz = nnz; p.n_rows = n_rows; p.dt = dt;
+    p.x = x; p.snap = snap; p.y = y; p.acc = nullptr; p.done = nullptr; p.urf = 1.0f;
+    p.g.nx = nx; p.g.ny = ny; p.g.nz = nz;
+    p.g.fx = FastDiv((uint32_t)nx);
+    p.g.fxy = FastDiv((uint32_t)((int64_t)nx * ny));
+    return p;
+}
+
+int pick_variant(piro_ctx *ctx, const Params &p)
+{
+    int v = ctx->fe_variant >= 0 ? ctx->fe_variant : 1;
+    if (kVariants[v].vec && !vec_ok(p.col, p.val)) v = 6;
+    return v;
+}
+
+void check_grid(int nx, int ny, int nz)
+{
+    PIRO_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "grid dims must be >= 1");
+    PIRO_REQUIRE((int64_t)nx * ny * nz < 0x7fffffffLL, "grid exceeds int32 rows");
+}
+
+}  // namespace
+
+namespace piro {
+// used by jli.hip
+void jli_iteration(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *rp, const int32_t *col, const float *val,
+                   int64_t nnz, float dt, const float *x, const float *b, float urf, float *x_new, double *acc,
+                   const int *done)
+{
+    Params p = make_params(nx, ny, nz, (int)((int64_t)nx * ny * nz), rp, col, val, nnz, dt, x, b, x_new);
+    p.acc = acc;
+    p.done = done;
+    p.urf = urf;
+    launch(JLI, pick_variant(ctx, p), p, ctx->stream);
+}
+}  // namespace piro
+
+extern "C" {
+
+int piro_fe_variant_count(void) { return kNumVariants; }
+
+const char *piro_fe_variant_name(int variant)
+{
+    return (variant >= 0 && variant < kNumVariants) ? kVariants[variant].name : "";
+}
+
+int piro_fe_set_variant(piro_ctx *ctx, int variant)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(variant >= -1 && variant < kNumVariants, "no such variant");
+    ctx->fe_variant = variant;
+    PIRO_API_END
+}
+
+int piro_fe_get_variant(piro_ctx *ctx, int *variant)
+{
+    PIRO_API_BEGIN
+    PIRO_REQUIRE(ctx && variant, "null pointer");
+    *variant = ctx->fe_variant;
+    PIRO_API_END
+}
+
+int piro_fe_step_csr(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row_ptr, const int32_t *d_col,
+                     const float *d_val, int64_t nnz, float dt, const float *d_x, const float *d_snap, float *d_y)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    check_grid(nx, ny, nz);
+    Params p = make_params(nx, ny, nz, (int)((int64_t)nx * ny * nz), d_row_ptr, d_col, d_val, nnz, dt, d_x, d_snap,
+                           d_y);
+    launch(FE_INTERIOR, pick_variant(ctx, p), p, ctx->stream);
+    PIRO_API_END
+}
+
+int piro_fe_run_csr(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row_ptr, const int32_t *d_col,
+                    const float *d_val, int64_t nnz, float dt, int nsteps, float *d_x, float *d_work,
+                    const float *d_snap)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    check_grid(nx, ny, nz);
+    PIRO_REQUIRE(nsteps >= 0, "nsteps < 0");
+    PIRO_REQUIRE(d_work && d_work != d_x, "work buffer must be distinct from x");
+    int n_rows = (int)((int64_t)nx * ny * nz);
+    Params p = make_params(nx, ny, nz, n_rows, d_row_ptr, d_col, d_val, nnz, dt, d_x, d_snap, d_work);
+    int v = pick_variant(ctx, p);
+    float *a = d_x, *b = d_work;
+    for (int s = 0; s < nsteps; ++s) {
+        p.x = a;
+        p.y = b;
+        launch(FE_INTERIOR, v, p, ctx->stream);
+        float *t = a; a = b; b = t;
+    }
+    if (a != d_x)
+        PIRO_CHECK_HIP(hipMemcpyAsync(d_x, a, (size_t)n_rows * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    PIRO_API_END
+}
+
+int piro_fe_step_allrows(piro_ctx *ctx, int n_rows, const int32_t *d_row_ptr, const int32_t *d_col,
+                         const float *d_val, int64_t nnz, float dt, const float *d_x, float *d_y)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    Params p = make_params(n_rows, 1, 1, n_rows, d_row_ptr, d_col, d_val, nnz, dt, d_x, nullptr, d_y);
+    launch(ALLROWS, pick_variant(ctx, p), p, ctx->stream);
+    PIRO_API_END
+}
+
+int piro_spmv_csr(piro_ctx *ctx, int n_rows, const int32_t *d_row_ptr, const int32_t *d_col, const float *d_val,
+                  int64_t nnz, const float *d_x, float *d_y)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    Params p = make_params(n_rows, 1, 1, n_rows, d_row_ptr, d_col, d_val, nnz, 1.0f, d_x, nullptr, d_y);
+    launch(SPMV, pick_variant(ctx, p), p, ctx->stream);
+    PIRO_API_END
+}
+
+int piro_fe_autotune(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row_ptr, const int32_t *d_col,
+                     const float *d_val, int64_t nnz, float dt, const float *d_x, const float *d_snap, float *d_y,
+                     int *best_variant, float *best_ms)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    check_grid(nx, ny, nz);
+    Params p = make_params(nx, ny, nz, (int)((int64_t)nx * ny * nz), d_row_ptr, d_col, d_val, nnz, dt, d_x, d_snap,
+                           d_y);
+    hipEvent_t e0, e1;
+    PIRO_CHECK_HIP(hipEventCreate(&e0));
+    PIRO_CHECK_HIP(hipEventCreate(&e1));
+    int best = -1;
+    float best_t = 1e30f;
+    const int reps = 5;
+    for (int v = 0; v < kNumVariants; ++v) {
+        if (kVariants[v].vec && !vec_ok(d_col, d_val)) continue;
+        launch(FE_INTERIOR, v, p, ctx->stream);  // warm-up
+        PIRO_CHECK_HIP(hipEventRecord(e0, ctx->stream));
+        for (int i = 0; i < reps; ++i) launch(FE_INTERIOR, v, p, ctx->stream);
+        PIRO_CHECK_HIP(hipEventRecord(e1, ctx->stream));
+        PIRO_CHECK_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        PIRO_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= reps;
+        if (ms < best_t) { best_t = ms; best = v; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    PIRO_REQUIRE(best >= 0, "no variant applicable");
+    ctx->fe_variant = best;
+    ctx->fe_tuned_rows = (int64_t)nx * ny * nz;
+    ctx->fe_tuned_nnz = nnz;
+    if (best_variant) *best_variant = best;
+    if (best_ms) *best_ms = best_t;
+    PIRO_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,304 @@
+"""GPU parity tests of the core path: device assembly, the fused FE step,
+reductions, JLI, HT/COO formats and geam, each against the CPU oracle.
+
+Bar: bit-exact for indices and for every FP32 result whose rounding sequence
+the oracle restates (assembly, FE step, geam, HT); FP64-accumulated
+reductions to 1e-12 relative; fields additionally within 1e-5 relative L2,
+with the error against the FP64 oracle printed.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from piro_amd.core import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def dev(a, ctx):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(ctx.device)
+
+
+LAP_CASES = [((3, 3, 3), 1), ((5, 6, 7), 1), ((5, 6, 7), 2), ((5, 6, 7), 3), ((34, 3, 4), 3), ((7, 5, 4), 2),
+             ((32, 32, 32), 3), ((32, 32, 32), 1), ((32, 32, 32), 2), ((1, 4, 5), 3), ((2, 2, 2), 2)]
+
+
+@pytest.mark.parametrize("dims,scheme", LAP_CASES)
+def test_laplacian_csr_bit_exact(ctx, dims, scheme):
+    print(f"case lap dims={dims} scheme={scheme}", flush=True)
+    norm = po.lap_norm(1.0, max(dims[0] - 2, 1))
+    A = ctx.laplacian_csr(*dims, scheme, norm)
+    R = po.lap_csr(*dims, scheme, norm)
+    rp, col, val = A.host()
+    np.testing.assert_array_equal(rp, R.row_ptr)
+    np.testing.assert_array_equal(col, R.col)
+    assert val.tobytes() == R.val.tobytes()
+
+
+@pytest.mark.parametrize("dims,scheme", [((5, 6, 7), 1), ((9, 9, 9), 3), ((6, 3, 5), 2)])
+def test_laplacian_coo_bit_exact(ctx, dims, scheme):
+    norm = po.lap_norm(1.0, dims[0] - 2)
+    rows, col, val = ctx.laplacian_coo(*dims, scheme, norm)
+    R = po.lap_csr(*dims, scheme, norm)
+    np.testing.assert_array_equal(host(rows), po.coo_rows(R))
+    np.testing.assert_array_equal(host(col), R.col)
+    assert host(val).tobytes() == R.val.tobytes()
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_gradient_bit_exact(ctx, scheme, d):
+    dims = (5, 6, 7)
+    g = po.grad_norm(1.0, 3)
+    G = ctx.gradient_csr(*dims, scheme, d, g)
+    R = po.grad_csr(*dims, scheme, d, g)
+    rp, col, val = G.host()
+    np.testing.assert_array_equal(rp, R.row_ptr)
+    np.testing.assert_array_equal(col, R.col)
+    assert val.tobytes() == R.val.tobytes()
+
+
+def test_large_grid_assembly_sampled(ctx):
+    # 128^3 (N=2.1M, nnz=14.6M): full comparison with the oracle
+    dims = (128, 128, 128)
+    norm = po.lap_norm(1.0, 126)
+    A = ctx.laplacian_csr(*dims, 3, norm)
+    R = po.lap_csr(*dims, 3, norm)
+    rp, col, val = A.host()
+    np.testing.assert_array_equal(rp, R.row_ptr)
+    np.testing.assert_array_equal(col, R.col)
+    assert val.tobytes() == R.val.tobytes()
+
+
+def test_row_scale_bit_exact(ctx):
+    dims = (8, 9, 10)
+    norm = po.lap_norm(1.0, 6)
+    A = ctx.laplacian_csr(*dims, 3, norm)
+    alpha = ctx.alpha_uniform(A.n_rows, 7, 0.005, 0.02)
+    ctx.row_scale(A, alpha)
+    R = po.row_scale(po.lap_csr(*dims, 3, norm), host(alpha))
+    assert host(A.val).tobytes() == R.val.tobytes()
+
+
+def test_alpha_generator_bit_exact(ctx):
+    a = ctx.alpha_uniform(100003, 20261203, 0.005, 0.02)
+    assert host(a).tobytes() == po.alpha_uniform(100003, 20261203, 0.005, 0.02).tobytes()
+
+
+def test_gaussian_ic_within_ulps(ctx):
+    dims = (32, 32, 32)
+    l = [30 / 31] * 3
+    g = host(ctx.ic_gaussian(dims, l, [15 / 31] * 3, [0.1] * 3, 1.0))
+    r = po.ic_gaussian(dims, np.float32(l), np.float32([15 / 31] * 3), np.float32([0.1] * 3), 1.0)
+    # device expf vs glibc expf: a couple of ulps at most
+    ulp = np.spacing(np.abs(r)).astype(np.float64)
+    assert np.all(np.abs(g.astype(np.float64) - r) <= 4 * ulp + 1e-30)
+
+
+def test_face_values(ctx):
+    dims = (6, 7, 8)
+    f = ctx.alpha_uniform(6 * 7 * 8, 3, -1, 1)
+    vals = [1, 2, 3, 4, 5, 6]
+    snap = host(ctx.apply_face_values(dims, f, vals))
+    fh = host(f)
+    idx = po.face_indices(*dims)
+    nx, ny, nz = dims
+    counts = [(ny - 2) * (nz - 2)] * 2 + [(nx - 2) * (nz - 2)] * 2 + [(nx - 2) * (ny - 2)] * 2
+    off = 0
+    for lst, c in enumerate(counts):
+        assert np.all(snap[idx[off:off + c]] == vals[lst])
+        off += c
+    other = np.setdiff1d(np.arange(nx * ny * nz), idx)
+    np.testing.assert_array_equal(snap[other], fh[other])
+
+
+def _c1_setup(ctx):
+    # 32^3 heat: n=30 interior, l=30/31 -> dx=1/31, alpha=0.01, r=0.1
+    dims = (32, 32, 32)
+    l0 = 30.0 / 31.0
+    norm = po.lap_norm(l0, 30)
+    alpha = np.float32(0.01)
+    dt = np.float32(np.float32(0.1) * norm / alpha)
+    A = ctx.laplacian_csr(*dims, 3, norm)
+    a = torch.full((A.n_rows,), float(alpha), dtype=torch.float32, device=ctx.device)
+    ctx.row_scale(A, a)
+    x0 = ctx.ic_gaussian(dims, [l0] * 3, [l0 / 2] * 3, [0.1] * 3, 1.0)
+    snap = ctx.apply_face_values(dims, x0, [0.0] * 6)
+    return dims, A, dt, x0, snap
+
+
+@pytest.mark.parametrize("variant", list(range(9)))
+def test_fe_c1_100_steps_all_variants_bit_exact(ctx, variant):
+    print(f"case fe c1 variant={variant}", flush=True)
+    dims, A, dt, x0, snap = _c1_setup(ctx)
+    ctx.set_variant(variant)
+    try:
+        x = x0.clone()
+        ctx.fe_run_csr(dims, A, dt, 100, x, snap=snap)
+        got = host(x)
+    finally:
+        ctx.set_variant(-1)
+    R = po.CSR(*A.host())
+    ref = po.fe_csr_run(dims, R, dt, 100, host(x0), host(snap))
+    ref64 = po.fe_csr_run_f64(dims, R, float(dt), 100, host(x0), host(snap))
+    assert got.tobytes() == ref.tobytes()
+    err32 = po.rel_l2(got, ref)
+    err64 = po.rel_l2(got, ref64)
+    print(f"C1 variant {variant}: rel L2 vs FP32 oracle {err32:.2e}, vs FP64 oracle {err64:.2e}")
+    assert err32 <= 1e-5 and err64 < 1e-5
+
+
+def test_fe_snapshot_faces_and_random_inputs(ctx):
+    dims = (13, 17, 11)
+    N = 13 * 17 * 11
+    norm = po.lap_norm(1.0, 11)
+    A = ctx.laplacian_csr(*dims, 2, norm)
+    x = ctx.alpha_uniform(N, 11, -1, 1)
+    snap = ctx.alpha_uniform(N, 12, -1, 1)
+    R = po.CSR(*A.host())
+    for v in range(9):
+        ctx.set_variant(v)
+        y = host(ctx.fe_step_csr(dims, A, 1e-3, x, snap))
+        assert y.tobytes() == po.fe_csr_step(dims, R, np.float32(1e-3), host(x), host(snap)).tobytes()
+        y0 = host(ctx.fe_step_csr(dims, A, 1e-3, x, None))
+        assert y0.tobytes() == po.fe_csr_step(dims, R, np.float32(1e-3), host(x), None).tobytes()
+    ctx.set_variant(-1)
+
+
+def test_fe_256cubed_two_steps_bit_exact(ctx):
+    dims = (256, 256, 256)
+    norm = po.lap_norm(1.0, 254)
+    A = ctx.laplacian_csr(*dims, 3, norm)
+    alpha = ctx.alpha_uniform(A.n_rows, 20261203, 0.005, 0.02)
+    ctx.row_scale(A, alpha)
+    dt = np.float32(np.float32(0.15) * norm / np.float32(0.02))
+    x0 = ctx.ic_gaussian(dims, [1.0] * 3, [0.5] * 3, [0.1] * 3, 1.0)
+    snap = ctx.apply_face_values(dims, x0, [0.0] * 6)
+    best, ms = ctx.autotune(dims, A, dt, x0, snap, torch.empty_like(x0))
+    print(f"256^3 autotune: variant {ctx.variants()[best]} {ms * 1e3:.1f} us/step")
+    x = x0.clone()
+    ctx.fe_run_csr(dims, A, dt, 2, x, snap=snap)
+    got = host(x)
+    R = po.CSR(*A.host())
+    ref = po.fe_csr_run(dims, R, dt, 2, host(x0), host(snap))
+    assert got.tobytes() == ref.tobytes()
+
+
+def test_allrows_coo_and_ht(ctx):
+    dims = (5, 5, 5)
+    N = 125
+    norm = po.lap_norm(5.0, 3)
+    for scheme in (1, 2, 3):
+        A = ctx.laplacian_csr(*dims, scheme, norm)
+        R = po.CSR(*A.host())
+        x = ctx.alpha_uniform(N, 5, -1, 1)
+        dt = np.float32(1e-3)
+        # COO
+        rows, col, val = ctx.laplacian_coo(*dims, scheme, norm)
+        Acoo = ctx.coo_to_csr(N, rows, col, val)
+        y = host(ctx.fe_step_allrows(Acoo, dt, x))
+        assert y.tobytes() == po.spmv_allrows_step(R, dt, host(x)).tobytes()
+        # HT: bit-exact slot layout, then the column-ordered step
+        ts = ctx.ht_table_size(A.nnz, N)
+        assert ts == po.ht_table_size(A.nnz, N)
+        keys, vals = ctx.ht_build(A, ts)
+        rk, rv = po.ht_build(R, ts)
+        np.testing.assert_array_equal(host(keys), rk)
+        assert host(vals).tobytes() == rv.tobytes()
+        Aht = ctx.ht_to_csr(N, keys, vals)
+        yh = host(ctx.fe_step_allrows(Aht, dt, x))
+        assert yh.tobytes() == po.fe_ht_step(N, rk, rv, dt, host(x)).tobytes()
+
+
+def test_ht_larger_table(ctx):
+    dims = (12, 11, 10)
+    N = 1320
+    norm = po.lap_norm(1.0, 10)
+    A = ctx.laplacian_csr(*dims, 3, norm)
+    R = po.CSR(*A.host())
+    ts = po.ht_table_size(A.nnz, N)
+    keys, vals = ctx.ht_build(A, ts)
+    rk, rv = po.ht_build(R, ts)
+    np.testing.assert_array_equal(host(keys), rk)
+    assert host(vals).tobytes() == rv.tobytes()
+
+
+def test_reductions(ctx):
+    rng = np.random.default_rng(0)
+    for n in (1, 7, 1000, 1 << 20, 3_000_001):
+        a = (rng.standard_normal(n) * 10).astype(np.float32)
+        b = rng.standard_normal(n).astype(np.float32)
+        da, db = dev(a, ctx), dev(b, ctx)
+        assert ctx.reduce("sum", da) == pytest.approx(po.lib.po_sum_f64(n, a), rel=1e-12, abs=1e-9)
+        assert ctx.reduce("sumsq", da) == pytest.approx(po.lib.po_sumsq_f64(n, a), rel=1e-12)
+        assert ctx.reduce("dot", da, db) == pytest.approx(po.lib.po_dot_f64(n, a, b), rel=1e-12, abs=1e-9)
+        assert ctx.reduce("absmax", da) == float(np.abs(a).max())
+    # unaligned view exercises the scalar path
+    a = rng.standard_normal(1001).astype(np.float32)
+    da = dev(a, ctx)[1:]
+    assert ctx.reduce("sum", da) == pytest.approx(po.lib.po_sum_f64(1000, np.ascontiguousarray(a[1:])), rel=1e-12)
+
+
+def test_geam3_concat_bit_exact(ctx):
+    dims = (6, 7, 8)
+    N = 6 * 7 * 8
+    g = po.grad_norm(1.0, 4)
+    for scheme in (1, 2, 3):
+        G = [ctx.gradient_csr(*dims, scheme, d, g) for d in range(3)]
+        v3 = ctx.alpha_uniform(3 * N, 9, -1, 1)
+        D = ctx.geam3(*G, v3)
+        R = po.geam3(*[po.CSR(*x.host()) for x in G], host(v3))
+        rp, col, val = D.host()
+        np.testing.assert_array_equal(rp, R.row_ptr)
+        np.testing.assert_array_equal(col, R.col)
+        assert val.tobytes() == R.val.tobytes()
+
+
+def test_geam2_true_merge_bit_exact(ctx):
+    n = 64
+    inv_h2 = np.float32(65.0 * 65.0)
+    L = po.lap2d_csr(n, inv_h2)
+    I = po.CSR(np.arange(n * n + 1, dtype=np.int32), np.arange(n * n, dtype=np.int32), np.ones(n * n, np.float32))
+    from piro_amd.core import DevCSR
+    dL = DevCSR(dev(L.row_ptr, ctx), dev(L.col, ctx), dev(L.val, ctx))
+    dI = DevCSR(dev(I.row_ptr, ctx), dev(I.col, ctx), dev(I.val, ctx))
+    beta = np.float32(-2.0 / float(inv_h2))
+    D = ctx.geam2(1.0, dI, beta, dL)
+    R = po.geam2(1.0, I, beta, L)
+    rp, col, val = D.host()
+    np.testing.assert_array_equal(rp, R.row_ptr)
+    np.testing.assert_array_equal(col, R.col)
+    assert val.tobytes() == R.val.tobytes()
+
+
+def test_jli_matches_oracle(ctx):
+    dims = (10, 10, 12)
+    N = 1200
+    norm = po.lap_norm(1.0, 8)
+    A = ctx.laplacian_csr(*dims, 2, norm)
+    dt = np.float32(0.05 * float(norm))
+    # boundary values of b are ~1e-6 of the peak: the boundary rows of x grow
+    # by b each iteration (x += urf*b there), so a one-iteration difference in
+    # the stopping test must not dominate the comparison
+    b = ctx.ic_gaussian(dims, [1.0] * 3, [0.5] * 3, [0.1] * 3, 1.0)
+    x, it, res = ctx.jli_solve(dims, A, dt, b, 1.0, 1e-10, 400)
+    R = po.CSR(*A.host())
+    xr, itr, resr = po.jli(dims, R, dt, host(b), 1.0, 1e-10, 400)
+    print(f"JLI: device {it} iterations res {res:.3e}; oracle {itr} iterations res {resr:.3e}")
+    # the device sums the residual in FP64, the reference in FP32: the
+    # stopping iteration may differ by one
+    assert abs(it - itr) <= 1
+    assert po.rel_l2(host(x), xr) < 1e-5
