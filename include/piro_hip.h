@@ -257,6 +257,78 @@ int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz,
                    const float *d_b, float urf, float tol, int max_iter,
                    float *d_x, float *d_work, int *iters, float *res);
 
+
+/* -------------------------------------------------- mixed-precision LU */
+
+/* Densify CSR into a row-major FP32 matrix (d_dense[r*ld + c]); duplicate
+ * columns are summed.  Used to form the dense backward-Euler system. */
+int piro_csr_to_dense(piro_ctx *ctx, int n_rows, int n_cols,
+                      const int32_t *d_row_ptr, const int32_t *d_col,
+                      const float *d_val, float *d_dense, int ld);
+/* P A = L U in FP32 with partial pivoting (pivot = first row of maximal
+ * magnitude, magnitudes compared in FP64), in place on the row-major n x n
+ * matrix d_a; d_ipiv[k] (0-based) is the row exchanged with row k at step k
+ * (LAPACK getrf convention).  The trailing updates run on MFMA
+ * (v_mfma_f32_16x16x4f32).  Replaces the reference's hash-table elimination
+ * matrix_operations::HT::lu_decomposition (src/matrixoperations.cpp:10-104,
+ * lu_decompHT.cl:22-62), which factors without pivoting and never solves.
+ * *info = 0, or k+1 for an exactly zero pivot at step k (PIRO_ESINGULAR). */
+int piro_lu_factor_f32(piro_ctx *ctx, int n, float *d_a, int lda,
+                       int32_t *d_ipiv, int *info);
+/* Solve with the FP32 factors, in place on d_b (row-major n x nrhs).
+ * Takes the role of the never-enqueued forward/backward_substitution_csr
+ * kernels (forward_substitution_csr.cl, backward_substitution_csr.cl). */
+int piro_lu_solve_f32(piro_ctx *ctx, int n, const float *d_lu, int lda,
+                      const int32_t *d_ipiv, int nrhs, float *d_b, int ldb);
+/* Mixed-precision solve: x0 from the FP32 factors, then repeat
+ * r = b - A x (FP64 accumulation over A's FP32 entries), d = LU^-1 r (FP32),
+ * x += d (FP64), until max_c ||r_c||_2/||b_c||_2 <= tol or max_sweeps
+ * corrections.  d_a is the original matrix, d_b/d_x row-major FP64
+ * n x nrhs (nrhs <= 32).  h_relres[nrhs] receives the final relative
+ * residuals; returns PIRO_ENOTCONV if tol was not reached. */
+int piro_lu_solve_refine(piro_ctx *ctx, int n, const float *d_a, int lda,
+                         const float *d_lu, int ldlu, const int32_t *d_ipiv,
+                         int nrhs, const double *d_b, int ldb, double *d_x,
+                         int ldx, double tol, int max_sweeps, int *sweeps,
+                         double *h_relres);
+
+/* ------------------------------------------------------- batched stepping */
+
+/* No reference counterpart (PIRO advances one grid per process): many
+ * independent heat instances stepped by ONE persistent launch per step.
+ * Instance i has padded dims h_dims[3i..3i+2] (nz == 1: the build's 2-D
+ * 5-point operator [self,x-,x+,y-,y+], -4/h^2, 1/h^2; nz >= 3: the
+ * reference's 3-D cd operator with its boundary rule), h = 1/(nx-2), its
+ * diffusivity field d_alpha (one value per row of the batch, rows ordered
+ * instance by instance) and its step h_dt[i].  Interior rows advance as the
+ * single-grid forward Euler; boundary rows are 0 (zero Dirichlet data).
+ * Results are bit-identical to stepping each instance alone for every
+ * tiling. */
+typedef struct piro_batch piro_batch;
+int piro_batch_create(piro_ctx *ctx, int n_inst, const int *h_dims,
+                      const float *d_alpha, const float *h_dt,
+                      piro_batch **out);
+int piro_batch_destroy(piro_batch *b);
+/* total rows, nnz and the n_inst+1 row offsets of the instances */
+int piro_batch_info(piro_batch *b, int64_t *total_rows, int64_t *nnz,
+                    int64_t *h_row_offsets);
+/* the assembled block-diagonal operator (values pre-multiplied by dt_i) */
+int piro_batch_operator(piro_batch *b, const int32_t **d_row_ptr,
+                        const int32_t **d_col, const float **d_val);
+/* rows per work unit (128/256/512/1024) and persistent grid size
+ * (0 = 8 workgroups per CU) */
+int piro_batch_set_tiling(piro_batch *b, int rows_per_unit, int grid);
+int piro_batch_get_tiling(piro_batch *b, int *rows_per_unit, int *grid);
+/* time every tiling on these buffers and keep the fastest */
+int piro_batch_autotune(piro_batch *b, float *d_x, float *d_work,
+                        int *best_rows_per_unit, int *best_grid,
+                        float *best_ms);
+int piro_batch_step(piro_batch *b, const float *d_x, float *d_y);
+/* nsteps steps, result in d_x; use_graph != 0 captures the steps in a
+ * hipGraph (re-used while buffers, step count and tiling are unchanged) */
+int piro_batch_run(piro_batch *b, float *d_x, float *d_work, int nsteps,
+                   int use_graph);
+
 #ifdef __cplusplus
 }
 #endif

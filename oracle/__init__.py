@@ -77,6 +77,9 @@ def _load() -> C.CDLL:
         "po_lu_factor_f64": (C.c_int, [C.c_int, _f64p, _i32p]),
         "po_lu_solve_f64": (None, [C.c_int, _f64p, _i32p, C.c_int, _f64p]),
         "po_omp_threads": (C.c_int, []),
+        "po_lap2d_padded_nnz": (C.c_int64, [C.c_int, C.c_int]),
+        "po_lap2d_padded_csr": (C.c_int, [C.c_int, C.c_int, C.c_float, _i32p, _i32p, _f32p]),
+        "po_fe_csr_step2d": (None, [C.c_int, C.c_int, _i32p, _i32p, _f32p, C.c_float, _f32p, _f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -291,6 +294,24 @@ def lap2d_csr(n, inv_h2):
     val = np.zeros(nnz, np.float32)
     lib.po_lap2d_csr(n, np.float32(inv_h2), rp, col, val)
     return CSR(rp, col, val)
+
+
+def lap2d_padded_csr(nx, ny, norm):
+    nnz = lib.po_lap2d_padded_nnz(nx, ny)
+    rp = np.zeros(nx * ny + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float32)
+    lib.po_lap2d_padded_csr(nx, ny, np.float32(norm), rp, col, val)
+    return CSR(rp, col, val)
+
+
+def fe_csr_run2d(nx, ny, A: CSR, dt, nsteps, x0):
+    x = np.ascontiguousarray(x0, np.float32).copy()
+    y = np.empty_like(x)
+    for _ in range(nsteps):
+        lib.po_fe_csr_step2d(nx, ny, A.row_ptr, A.col, A.val, np.float32(dt), x, y)
+        x, y = y, x
+    return x
 
 
 def lu_factor_f64(a):

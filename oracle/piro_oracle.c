@@ -823,3 +823,57 @@ void po_lu_solve_f64(int n, const double *lu, const int32_t *piv, int nrhs,
 /* number of OpenMP threads the parallel loops above use */
 #include <omp.h>
 int po_omp_threads(void) { return omp_get_max_threads(); }
+
+/* ---------------------------------------- 2-D padded operator (batched) */
+
+/* Build-only operator of the batched stepper (PIRO is 3-D only): padded
+ * nx x ny grid, rows [self, x-, x+, y-, y+] (each neighbour if in-grid),
+ * values float(-4.0/double(norm)) and float(1.0/double(norm)); no boundary
+ * rule. */
+int64_t po_lap2d_padded_nnz(int nx, int ny)
+{
+    return 5 * (int64_t)nx * ny - 2 * ((int64_t)nx + ny);
+}
+
+int po_lap2d_padded_csr(int nx, int ny, float norm, int32_t *row_ptr,
+                        int32_t *col, float *val)
+{
+    float vs = (float)(-4.0 / (double)norm), vn = (float)(1.0 / (double)norm);
+    int64_t k = 0;
+    row_ptr[0] = 0;
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            int i = x + y * nx;
+            col[k] = i; val[k++] = vs;
+            if (x > 0)      { col[k] = i - 1;  val[k++] = vn; }
+            if (x < nx - 1) { col[k] = i + 1;  val[k++] = vn; }
+            if (y > 0)      { col[k] = i - nx; val[k++] = vn; }
+            if (y < ny - 1) { col[k] = i + nx; val[k++] = vn; }
+            row_ptr[i + 1] = (int32_t)k;
+        }
+    return 0;
+}
+
+/* Forward Euler of the 2-D batched instances: interior rows
+ * y = sum fl(fl(dt*v)*x[c]) + x[r] (CSR order), every boundary row 0 (zero
+ * Dirichlet data). */
+void po_fe_csr_step2d(int nx, int ny, const int32_t *row_ptr,
+                      const int32_t *col, const float *val, float dt,
+                      const float *x, float *y)
+{
+    for (int yy = 0; yy < ny; ++yy)
+        for (int xx = 0; xx < nx; ++xx) {
+            int64_t r = xx + (int64_t)yy * nx;
+            if (xx > 0 && xx < nx - 1 && yy > 0 && yy < ny - 1) {
+                float acc = 0.0f;
+                for (int32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+                    float t = dt * val[k];
+                    float p = t * x[col[k]];
+                    acc = acc + p;
+                }
+                y[r] = acc + x[r];
+            } else {
+                y[r] = 0.0f;
+            }
+        }
+}

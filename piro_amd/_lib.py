@@ -76,7 +76,20 @@ _SIGS = {
     "piro_reduce_sumsq": [vp, i64, vp, P_f64],
     "piro_reduce_dot": [vp, i64, vp, vp, P_f64],
     "piro_reduce_absmax": [vp, i64, vp, P_f64],
+    "piro_csr_to_dense": [vp, i32, i32, vp, vp, vp, vp, i32],
+    "piro_lu_factor_f32": [vp, i32, vp, i32, vp, P_i32],
+    "piro_lu_solve_f32": [vp, i32, vp, i32, vp, i32, vp, i32],
+    "piro_lu_solve_refine": [vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, f64, i32, P_i32, P_f64],
     "piro_jli_solve": [vp, i32, i32, i32, vp, vp, vp, i64, f32, vp, f32, f32, i32, vp, vp, P_i32, P_f32],
+    "piro_batch_create": [vp, i32, P_i32, vp, P_f32, C.POINTER(vp)],
+    "piro_batch_destroy": [vp],
+    "piro_batch_info": [vp, P_i64, P_i64, P_i64],
+    "piro_batch_operator": [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)],
+    "piro_batch_set_tiling": [vp, i32, i32],
+    "piro_batch_get_tiling": [vp, P_i32, P_i32],
+    "piro_batch_autotune": [vp, vp, vp, P_i32, P_i32, P_f32],
+    "piro_batch_step": [vp, vp, vp],
+    "piro_batch_run": [vp, vp, vp, i32, i32],
 }
 
 for _name, _args in _SIGS.items():

@@ -242,6 +242,42 @@ class Context:
             raise ValueError(op)
         return out.value
 
+    # ----------------------------------------------------------------- LU
+    def csr_to_dense(self, A: DevCSR, n_cols=None):
+        n_cols = A.n_rows if n_cols is None else n_cols
+        d = torch.empty((A.n_rows, n_cols), dtype=torch.float32, device=self.device)
+        check(lib.piro_csr_to_dense(self.h, A.n_rows, n_cols, _p(A.row_ptr), _p(A.col), _p(A.val), _p(d), n_cols))
+        return d
+
+    def lu_factor(self, a):
+        """In-place FP32 LU with partial pivoting of a row-major square tensor."""
+        n = a.shape[0]
+        ipiv = torch.empty(n, dtype=torch.int32, device=self.device)
+        info = C.c_int()
+        check(lib.piro_lu_factor_f32(self.h, n, _p(a), a.stride(0), _p(ipiv), C.byref(info)))
+        return ipiv
+
+    def lu_solve_f32(self, lu, ipiv, b):
+        n = lu.shape[0]
+        b2 = b.reshape(n, -1)
+        check(lib.piro_lu_solve_f32(self.h, n, _p(lu), lu.stride(0), _p(ipiv), b2.shape[1], _p(b2), b2.stride(0)))
+        return b
+
+    def lu_solve_refine(self, a, lu, ipiv, b64, tol=1e-10, max_sweeps=10):
+        n = a.shape[0]
+        b2 = b64.reshape(n, -1).contiguous()
+        nrhs = b2.shape[1]
+        x = torch.empty_like(b2)
+        sweeps = C.c_int()
+        rel = (C.c_double * nrhs)()
+        check(lib.piro_lu_solve_refine(self.h, n, _p(a), a.stride(0), _p(lu), lu.stride(0), _p(ipiv), nrhs, _p(b2),
+                                       nrhs, _p(x), nrhs, float(tol), int(max_sweeps), C.byref(sweeps), rel))
+        return x.reshape(b64.shape), sweeps.value, list(rel)
+
+    # ------------------------------------------------------------- batch
+    def batch(self, dims_list, alpha, dts):
+        return Batch(self, dims_list, alpha, dts)
+
     # ---------------------------------------------------------------- JLI
     def jli_solve(self, dims, A: DevCSR, dt, b, urf=1.0, tol=1e-5, max_iter=1000):
         nx, ny, nz = dims
@@ -250,3 +286,57 @@ class Context:
         check(lib.piro_jli_solve(self.h, nx, ny, nz, _p(A.row_ptr), _p(A.col), _p(A.val), A.nnz, float(dt), _p(b),
                                  float(urf), float(tol), int(max_iter), _p(x), _p(work), C.byref(it), C.byref(res)))
         return x, it.value, res.value
+
+
+class Batch:
+    """Many independent heat instances stepped by one launch per step."""
+
+    def __init__(self, ctx: Context, dims_list, alpha, dts):
+        self.ctx = ctx
+        n = len(dims_list)
+        flat = (C.c_int * (3 * n))(*[int(v) for d in dims_list for v in d])
+        dt = (C.c_float * n)(*[float(v) for v in dts])
+        h = C.c_void_p()
+        check(lib.piro_batch_create(ctx.h, n, flat, _p(alpha), dt, C.byref(h)))
+        self.h = h
+        tot, nnz = C.c_int64(), C.c_int64()
+        offs = (C.c_int64 * (n + 1))()
+        check(lib.piro_batch_info(self.h, C.byref(tot), C.byref(nnz), offs))
+        self.total_rows, self.nnz = tot.value, nnz.value
+        self.offsets = list(offs)
+
+    def close(self):
+        if self.h:
+            check(lib.piro_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def operator(self):
+        rp, col, val = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib.piro_batch_operator(self.h, C.byref(rp), C.byref(col), C.byref(val)))
+        return rp.value, col.value, val.value
+
+    def set_tiling(self, rows_per_unit, grid=0):
+        check(lib.piro_batch_set_tiling(self.h, int(rows_per_unit), int(grid)))
+
+    def tiling(self):
+        a, b = C.c_int(), C.c_int()
+        check(lib.piro_batch_get_tiling(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, x, work):
+        rb, g, ms = C.c_int(), C.c_int(), C.c_float()
+        check(lib.piro_batch_autotune(self.h, _p(x), _p(work), C.byref(rb), C.byref(g), C.byref(ms)))
+        return rb.value, g.value, ms.value
+
+    def step(self, x, y):
+        check(lib.piro_batch_step(self.h, _p(x), _p(y)))
+
+    def run(self, x, work, nsteps, use_graph=True):
+        check(lib.piro_batch_run(self.h, _p(x), _p(work), int(nsteps), 1 if use_graph else 0))
+        return x

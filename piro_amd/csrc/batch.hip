@@ -1,0 +1,467 @@
+// batch.hip -- many independent heat instances advanced by one launch per step.
+//
+// No reference counterpart (PIRO steps one grid per process).  A batch is the
+// block-diagonal CSR of all instances (2-D 5-point or the reference's 3-D
+// 7-point operator, each row-scaled by its diffusivity and pre-multiplied by
+// its own dt, which leaves every product fl(fl(dt*v)*x) unchanged), cut into
+// work units of RB consecutive rows that never straddle two instances.  One
+// persistent launch per step walks the unit list with a static stride, so a
+// captured hipGraph of N steps replays without any counter to reset.  Each
+// unit is processed like the single-grid kernel (non-zeros streamed into LDS,
+// per-row sequential sums), so every instance's field is bit-identical to
+// stepping it alone, whatever RB and the grid size the autotuner picks.
+// Boundary rows are zero (zero Dirichlet data); interior rows get
+// y = sum + x.
+#include "device_utils.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+using namespace piro;
+
+struct InstDesc {
+    int64_t row0;   // first global row
+    int nx, ny, nz; // padded dims (nz == 1: 2-D)
+    uint64_t mx, mxy; // round-up multipliers for / nx and / (nx*ny)
+    int sx, sxy;
+};
+
+struct Unit {
+    int64_t row0;
+    int n_rows;
+    int inst;
+};
+
+struct piro_batch {
+    piro_ctx *ctx = nullptr;
+    int n_inst = 0;
+    int64_t total_rows = 0, nnz = 0;
+    std::vector<int64_t> row_off;  // n_inst + 1
+    std::vector<InstDesc> h_inst;
+    InstDesc *d_inst = nullptr;
+    int32_t *rp = nullptr, *col = nullptr;
+    float *val = nullptr;
+    // work units for the current tiling
+    int rb = 0;
+    Unit *d_units = nullptr;
+    int n_units = 0;
+    int grid = 0;
+    // captured graph
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    const float *g_x = nullptr;
+    float *g_w = nullptr;
+    int g_steps = -1, g_rb = 0, g_grid = 0;
+};
+
+namespace {
+
+inline void divmagic(uint32_t d, uint64_t &m, int &s)
+{
+    s = 0;
+    while ((1ull << s) < d) ++s;
+    m = (((unsigned __int128)1 << (32 + s)) + d - 1) / d;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, uint64_t m, int s) { return (uint32_t)(((uint64_t)n * m) >> (32 + s)); }
+
+// ---------------------------------------------------------------- assembly
+
+// per-instance operator rows, dt folded in: val = fl(dt * fl(v * alpha_row))
+__global__ __launch_bounds__(256) void k_batch_assemble(int n_inst, const InstDesc *__restrict__ inst,
+                                                        const int64_t *__restrict__ nnz_off,
+                                                        const float *__restrict__ alpha, const float *__restrict__ dt,
+                                                        const float *__restrict__ vself, const float *__restrict__ vnb,
+                                                        int64_t total_rows, int32_t *__restrict__ rp,
+                                                        int32_t *__restrict__ col, float *__restrict__ val)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total_rows) return;
+    // instance of row g: binary search over row0
+    int lo = 0, hi = n_inst;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (inst[mid].row0 <= g) lo = mid; else hi = mid;
+    }
+    const InstDesc d = inst[lo];
+    const int64_t i = g - d.row0;
+    const int x = (int)(i % d.nx);
+    const int64_t tq = i / d.nx;
+    const int y = (int)(tq % d.ny);
+    const int z = (int)(tq / d.ny);
+    const float a = alpha[g], h = dt[lo], vs = vself[lo], vn = vnb[lo];
+    const int64_t sxy = (int64_t)d.nx * d.ny;
+    int64_t k;
+    if (d.nz == 1) {
+        // rows before (x, y) of a 5-point operator: 5 per row minus missing
+        int64_t sx = 2 * (int64_t)y + (x > 0);
+        int my = (y == 0) + (y == d.ny - 1);
+        int64_t sy = (int64_t)d.nx * (y > 0) + (int64_t)x * my;
+        k = 5 * i - sx - sy;
+    } else {
+        int64_t sx = 2 * ((int64_t)z * d.ny + y) + (x > 0);
+        int my = (y == 0) + (y == d.ny - 1);
+        int64_t sy = 2 * (int64_t)d.nx * z + (int64_t)d.nx * (y > 0) + (int64_t)x * my;
+        int mz = (z == 0) + (z == d.nz - 1);
+        int64_t sz = sxy * (z > 0) + ((int64_t)y * d.nx + x) * mz;
+        k = 7 * i - sx - sy - sz;
+    }
+    k += nnz_off[lo];
+    rp[g] = (int32_t)k;
+    if (g == total_rows - 1) rp[total_rows] = (int32_t)nnz_off[n_inst];
+    const int32_t gi = (int32_t)g;
+    auto emit = [&](int32_t c, float v) {
+        col[k] = c;
+        float sv = v * a;
+        val[k] = h * sv;
+        ++k;
+    };
+    if (d.nz == 1) {
+        // build-only 2-D operator: [self, x-, x+, y-, y+], -4/h^2 and 1/h^2
+        emit(gi, vs);
+        if (x > 0) emit(gi - 1, vn);
+        if (x < d.nx - 1) emit(gi + 1, vn);
+        if (y > 0) emit(gi - d.nx, vn);
+        if (y < d.ny - 1) emit(gi + d.nx, vn);
+    } else {
+        // the reference's cd operator including its boundary 10.0 rule
+        // (src/stencils.cpp:233-297)
+        auto bnd = [&](int xx, int yy, int zz) {
+            return xx == 0 || xx == d.nx - 1 || yy == 0 || yy == d.ny - 1 || zz == 0 || zz == d.nz - 1;
+        };
+        emit(gi, i == 0 ? 10.0f : vs);
+        if (x > 0) emit(gi - 1, vn);
+        if (x < d.nx - 1) emit(gi + 1, (z == 0 && y == 0) ? 10.0f : vn);
+        if (y > 0) emit(gi - d.nx, vn);
+        if (y < d.ny - 1) emit(gi + d.nx, z == 0 ? 10.0f : vn);
+        if (z > 0) emit((int32_t)(g - sxy), vn);
+        if (z < d.nz - 1) emit((int32_t)(g + sxy), bnd(x, y, z + 1) ? 10.0f : vn);
+    }
+}
+
+// ------------------------------------------------------------------ step
+
+template <int RB>
+__global__ __launch_bounds__(256) void k_batch_step(const Unit *__restrict__ units, int n_units,
+                                                    const InstDesc *__restrict__ inst,
+                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                    const float *__restrict__ val, const float *__restrict__ x,
+                                                    float *__restrict__ y)
+{
+    constexpr int CAP = RB * 8 + 8;
+    __shared__ int32_t s_rp[RB + 1];
+    __shared__ float s_p[CAP];
+    for (int u = blockIdx.x; u < n_units; u += gridDim.x) {
+        const Unit un = units[u];
+        const InstDesc d = inst[un.inst];
+        __syncthreads();  // LDS reuse across units
+        for (int j = threadIdx.x; j <= un.n_rows; j += 256) s_rp[j] = rp[un.row0 + j];
+        __syncthreads();
+        const int k0 = s_rp[0], k1 = s_rp[un.n_rows];
+        const bool staged = k1 - k0 <= CAP;
+        if (staged) {
+            for (int k = k0 + threadIdx.x; k < k1; k += 256) {
+                const int c = __builtin_nontemporal_load(col + k);
+                const float v = __builtin_nontemporal_load(val + k);
+                s_p[k - k0] = v * x[c];
+            }
+        }
+        __syncthreads();
+        for (int rr = threadIdx.x; rr < un.n_rows; rr += 256) {
+            const int64_t g = un.row0 + rr;
+            const uint32_t li = (uint32_t)(g - d.row0);
+            const uint32_t zz = fdiv(li, d.mxy, d.sxy);
+            const uint32_t rem = li - zz * (uint32_t)(d.nx * d.ny);
+            const uint32_t yy = fdiv(rem, d.mx, d.sx);
+            const uint32_t xx = rem - yy * (uint32_t)d.nx;
+            const bool interior = xx > 0 && xx < (uint32_t)d.nx - 1 && yy > 0 && yy < (uint32_t)d.ny - 1 &&
+                                  (d.nz == 1 || (zz > 0 && zz < (uint32_t)d.nz - 1));
+            float out = 0.0f;
+            if (interior) {
+                float acc = 0.0f;
+                if (staged) {
+                    for (int k = s_rp[rr] - k0; k < s_rp[rr + 1] - k0; ++k) acc = acc + s_p[k];
+                } else {
+                    for (int k = s_rp[rr]; k < s_rp[rr + 1]; ++k) acc = acc + val[k] * x[col[k]];
+                }
+                out = acc + x[g];
+            }
+            y[g] = out;
+        }
+    }
+}
+
+void build_units(piro_batch *b, int rb)
+{
+    std::vector<Unit> u;
+    for (int i = 0; i < b->n_inst; ++i) {
+        int64_t r0 = b->row_off[i], r1 = b->row_off[i + 1];
+        for (int64_t r = r0; r < r1; r += rb) u.push_back(Unit{r, (int)std::min<int64_t>(rb, r1 - r), i});
+    }
+    if (b->d_units) PIRO_CHECK_HIP(hipFree(b->d_units));
+    PIRO_CHECK_HIP(hipMalloc(&b->d_units, u.size() * sizeof(Unit)));
+    PIRO_CHECK_HIP(hipMemcpy(b->d_units, u.data(), u.size() * sizeof(Unit), hipMemcpyHostToDevice));
+    b->n_units = (int)u.size();
+    b->rb = rb;
+}
+
+void launch_step(piro_batch *b, const float *x, float *y, hipStream_t s)
+{
+    const int g = b->grid > 0 ? b->grid : std::min(b->n_units, b->ctx->num_cus * 8);
+    switch (b->rb) {
+    case 128: k_batch_step<128><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y); break;
+    case 256: k_batch_step<256><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y); break;
+    case 512: k_batch_step<512><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y); break;
+    default: k_batch_step<1024><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y); break;
+    }
+    check_launch();
+}
+
+void drop_graph(piro_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->g_steps = -1;
+}
+
+piro_batch *bat(piro_batch *b)
+{
+    if (!b) throw Error(PIRO_EINVAL, "null batch");
+    ctx_of(b->ctx);
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int piro_batch_create(piro_ctx *ctx, int n_inst, const int *h_dims, const float *d_alpha, const float *h_dt,
+                      piro_batch **out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n_inst > 0 && h_dims && d_alpha && h_dt && out, "bad arguments");
+    piro_batch *b = new piro_batch();
+    try {
+        b->ctx = ctx;
+        b->n_inst = n_inst;
+        b->row_off.assign(n_inst + 1, 0);
+        std::vector<int64_t> nnz_off(n_inst + 1, 0);
+        std::vector<float> vself(n_inst), vnb(n_inst), dt(h_dt, h_dt + n_inst);
+        b->h_inst.resize(n_inst);
+        for (int i = 0; i < n_inst; ++i) {
+            int nx = h_dims[3 * i], ny = h_dims[3 * i + 1], nz = h_dims[3 * i + 2];
+            PIRO_REQUIRE(nx >= 3 && ny >= 3 && nz >= 1 && nz != 2, "instance dims must be >= 3 (nz == 1 for 2-D)");
+            int64_t N = (int64_t)nx * ny * nz;
+            int64_t nnz = nz == 1 ? 5 * N - 2 * ((int64_t)nx + ny)
+                                  : 7 * N - 2 * ((int64_t)ny * nz + (int64_t)nx * nz + (int64_t)nx * ny);
+            b->row_off[i + 1] = b->row_off[i] + N;
+            nnz_off[i + 1] = nnz_off[i] + nnz;
+            // spacing h = 1/(nx-2) (unit length, the reference's dx = l/n)
+            float hh = 1.0f / (float)(nx - 2);
+            float norm = (float)std::pow((double)hh, 2.0);
+            vself[i] = (float)((nz == 1 ? -4.0 : -6.0) / (double)norm);
+            vnb[i] = (float)(1.0 / (double)norm);
+            InstDesc &d = b->h_inst[i];
+            d.row0 = b->row_off[i];
+            d.nx = nx; d.ny = ny; d.nz = nz;
+            divmagic((uint32_t)nx, d.mx, d.sx);
+            divmagic((uint32_t)((int64_t)nx * ny), d.mxy, d.sxy);
+        }
+        b->total_rows = b->row_off[n_inst];
+        b->nnz = nnz_off[n_inst];
+        PIRO_REQUIRE(b->total_rows < 0x7fffffffLL && b->nnz < 0x7fffffffLL, "batch exceeds int32 indexing");
+        PIRO_CHECK_HIP(hipMalloc(&b->d_inst, n_inst * sizeof(InstDesc)));
+        PIRO_CHECK_HIP(hipMemcpy(b->d_inst, b->h_inst.data(), n_inst * sizeof(InstDesc), hipMemcpyHostToDevice));
+        PIRO_CHECK_HIP(hipMalloc(&b->rp, (b->total_rows + 1) * sizeof(int32_t)));
+        PIRO_CHECK_HIP(hipMalloc(&b->col, std::max<int64_t>(b->nnz, 1) * sizeof(int32_t)));
+        PIRO_CHECK_HIP(hipMalloc(&b->val, std::max<int64_t>(b->nnz, 1) * sizeof(float)));
+        int64_t *d_nnz_off;
+        float *d_small;
+        PIRO_CHECK_HIP(hipMalloc(&d_nnz_off, (n_inst + 1) * sizeof(int64_t)));
+        PIRO_CHECK_HIP(hipMalloc(&d_small, 3 * n_inst * sizeof(float)));
+        PIRO_CHECK_HIP(hipMemcpy(d_nnz_off, nnz_off.data(), (n_inst + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+        PIRO_CHECK_HIP(hipMemcpy(d_small, dt.data(), n_inst * sizeof(float), hipMemcpyHostToDevice));
+        PIRO_CHECK_HIP(hipMemcpy(d_small + n_inst, vself.data(), n_inst * sizeof(float), hipMemcpyHostToDevice));
+        PIRO_CHECK_HIP(hipMemcpy(d_small + 2 * n_inst, vnb.data(), n_inst * sizeof(float), hipMemcpyHostToDevice));
+        k_batch_assemble<<<grid_for(b->total_rows, 256), 256, 0, ctx->stream>>>(
+            n_inst, b->d_inst, d_nnz_off, d_alpha, d_small, d_small + n_inst, d_small + 2 * n_inst, b->total_rows,
+            b->rp, b->col, b->val);
+        check_launch();
+        PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+        PIRO_CHECK_HIP(hipFree(d_nnz_off));
+        PIRO_CHECK_HIP(hipFree(d_small));
+        build_units(b, 256);
+    } catch (...) {
+        if (b->d_inst) (void)hipFree(b->d_inst);
+        if (b->rp) (void)hipFree(b->rp);
+        if (b->col) (void)hipFree(b->col);
+        if (b->val) (void)hipFree(b->val);
+        if (b->d_units) (void)hipFree(b->d_units);
+        delete b;
+        throw;
+    }
+    *out = b;
+    PIRO_API_END
+}
+
+int piro_batch_destroy(piro_batch *b)
+{
+    PIRO_API_BEGIN
+    if (!b) return PIRO_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    drop_graph(b);
+    (void)hipFree(b->d_inst);
+    (void)hipFree(b->rp);
+    (void)hipFree(b->col);
+    (void)hipFree(b->val);
+    (void)hipFree(b->d_units);
+    delete b;
+    PIRO_API_END
+}
+
+int piro_batch_info(piro_batch *b, int64_t *total_rows, int64_t *nnz, int64_t *h_row_offsets)
+{
+    PIRO_API_BEGIN
+    bat(b);
+    if (total_rows) *total_rows = b->total_rows;
+    if (nnz) *nnz = b->nnz;
+    if (h_row_offsets)
+        for (int i = 0; i <= b->n_inst; ++i) h_row_offsets[i] = b->row_off[i];
+    PIRO_API_END
+}
+
+int piro_batch_operator(piro_batch *b, const int32_t **d_row_ptr, const int32_t **d_col, const float **d_val)
+{
+    PIRO_API_BEGIN
+    bat(b);
+    if (d_row_ptr) *d_row_ptr = b->rp;
+    if (d_col) *d_col = b->col;
+    if (d_val) *d_val = b->val;
+    PIRO_API_END
+}
+
+int piro_batch_set_tiling(piro_batch *b, int rows_per_unit, int grid)
+{
+    PIRO_API_BEGIN
+    bat(b);
+    PIRO_REQUIRE(rows_per_unit == 128 || rows_per_unit == 256 || rows_per_unit == 512 || rows_per_unit == 1024,
+                 "rows_per_unit must be 128, 256, 512 or 1024");
+    PIRO_REQUIRE(grid >= 0, "grid < 0");
+    if (rows_per_unit != b->rb) build_units(b, rows_per_unit);
+    b->grid = grid;
+    drop_graph(b);
+    PIRO_API_END
+}
+
+int piro_batch_get_tiling(piro_batch *b, int *rows_per_unit, int *grid)
+{
+    PIRO_API_BEGIN
+    bat(b);
+    if (rows_per_unit) *rows_per_unit = b->rb;
+    if (grid) *grid = b->grid > 0 ? b->grid : std::min(b->n_units, b->ctx->num_cus * 8);
+    PIRO_API_END
+}
+
+int piro_batch_step(piro_batch *b, const float *d_x, float *d_y)
+{
+    PIRO_API_BEGIN
+    bat(b);
+    PIRO_REQUIRE(d_x && d_y && d_x != d_y, "bad buffers");
+    launch_step(b, d_x, d_y, b->ctx->stream);
+    PIRO_API_END
+}
+
+int piro_batch_run(piro_batch *b, float *d_x, float *d_work, int nsteps, int use_graph)
+{
+    PIRO_API_BEGIN
+    bat(b);
+    PIRO_REQUIRE(d_x && d_work && d_x != d_work && nsteps >= 0, "bad arguments");
+    hipStream_t s = b->ctx->stream;
+    if (!use_graph) {
+        float *a = d_x, *w = d_work;
+        for (int i = 0; i < nsteps; ++i) {
+            launch_step(b, a, w, s);
+            std::swap(a, w);
+        }
+        if (a != d_x)
+            PIRO_CHECK_HIP(hipMemcpyAsync(d_x, a, b->total_rows * sizeof(float), hipMemcpyDeviceToDevice, s));
+        return PIRO_OK;
+    }
+    const int g = b->grid > 0 ? b->grid : std::min(b->n_units, b->ctx->num_cus * 8);
+    if (!(b->gexec && b->g_x == d_x && b->g_w == d_work && b->g_steps == nsteps && b->g_rb == b->rb &&
+          b->g_grid == g)) {
+        drop_graph(b);
+        hipStream_t cs;
+        PIRO_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        PIRO_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        float *a = d_x, *w = d_work;
+        for (int i = 0; i < nsteps; ++i) {
+            launch_step(b, a, w, cs);
+            std::swap(a, w);
+        }
+        if (a != d_x) {
+            // odd step count: one more kernel-free copy node would be a memcpy;
+            // keep the graph kernel-only and copy outside it
+        }
+        hipError_t e = hipStreamEndCapture(cs, &b->graph);
+        (void)hipStreamDestroy(cs);
+        PIRO_CHECK_HIP(e);
+        PIRO_CHECK_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        b->g_x = d_x;
+        b->g_w = d_work;
+        b->g_steps = nsteps;
+        b->g_rb = b->rb;
+        b->g_grid = g;
+    }
+    PIRO_CHECK_HIP(hipGraphLaunch(b->gexec, s));
+    if (nsteps % 2 == 1)
+        PIRO_CHECK_HIP(hipMemcpyAsync(d_x, d_work, b->total_rows * sizeof(float), hipMemcpyDeviceToDevice, s));
+    PIRO_API_END
+}
+
+int piro_batch_autotune(piro_batch *b, float *d_x, float *d_work, int *best_rows_per_unit, int *best_grid,
+                        float *best_ms)
+{
+    PIRO_API_BEGIN
+    bat(b);
+    PIRO_REQUIRE(d_x && d_work && d_x != d_work, "bad buffers");
+    hipStream_t s = b->ctx->stream;
+    hipEvent_t e0, e1;
+    PIRO_CHECK_HIP(hipEventCreate(&e0));
+    PIRO_CHECK_HIP(hipEventCreate(&e1));
+    int best_rb = 256, best_g = 0;
+    float best_t = 1e30f;
+    const int rbs[4] = {128, 256, 512, 1024};
+    const int per_cu[4] = {2, 4, 8, 16};
+    for (int rb : rbs) {
+        build_units(b, rb);
+        for (int pc : per_cu) {
+            b->grid = std::min(b->n_units, b->ctx->num_cus * pc);
+            launch_step(b, d_x, d_work, s);
+            PIRO_CHECK_HIP(hipEventRecord(e0, s));
+            for (int i = 0; i < 4; ++i) launch_step(b, d_x, d_work, s);
+            PIRO_CHECK_HIP(hipEventRecord(e1, s));
+            PIRO_CHECK_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            PIRO_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms /= 4;
+            if (ms < best_t) { best_t = ms; best_rb = rb; best_g = b->grid; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    build_units(b, best_rb);
+    b->grid = best_g;
+    drop_graph(b);
+    if (best_rows_per_unit) *best_rows_per_unit = best_rb;
+    if (best_grid) *best_grid = best_g;
+    if (best_ms) *best_ms = best_t;
+    PIRO_API_END
+}
+
+}  // extern "C"

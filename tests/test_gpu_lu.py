@@ -1,0 +1,145 @@
+"""GPU tests of the mixed-precision LU (FP32 factor on MFMA + FP64 refinement).
+
+Pinned against the FP64 LAPACK-convention oracle (oracle.lu_factor_f64,
+itself pinned to scipy.linalg.lu_factor) and numpy's FP64 solve.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from piro_amd.core import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def dev(a, ctx):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(ctx.device)
+
+
+def perm_from_ipiv(ipiv, n):
+    p = np.arange(n)
+    for k, q in enumerate(ipiv):
+        p[k], p[q] = p[q], p[k]
+    return p
+
+
+@pytest.mark.parametrize("n", [16, 48, 100, 257, 1000])
+def test_lu_factor_reconstructs(ctx, n):
+    print(f"case lu n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n)).astype(np.float32)
+    d = dev(a, ctx)
+    ipiv = ctx.lu_factor(d)
+    torch.cuda.synchronize()
+    lu = d.cpu().numpy().astype(np.float64)
+    piv = ipiv.cpu().numpy()
+    L = np.tril(lu, -1) + np.eye(n)
+    U = np.triu(lu)
+    P = perm_from_ipiv(piv, n)
+    err = np.linalg.norm(a[P].astype(np.float64) - L @ U) / np.linalg.norm(a)
+    assert err < 1e-5 * max(1.0, np.sqrt(n) / 10)
+    # partial pivoting bounds |l_ij| <= 1
+    assert np.abs(np.tril(lu, -1)).max() <= 1.0 + 1e-6
+    # pivot sequence equals the FP64 oracle's on this (well-separated) input
+    _, piv64, info = po.lu_factor_f64(a.astype(np.float64))
+    assert info == 0
+    agree = np.mean(piv == piv64)
+    assert agree > 0.95
+
+
+@pytest.mark.parametrize("n", [48, 160])
+def test_mfma_update_layout_exact(ctx, n):
+    # A = L0 U0 with L0 unit lower (entries in {0, +-1/4, +-1/2}) and U0
+    # upper with small integers: no row exchange is chosen (|l| < 1) and every
+    # intermediate of the elimination is exactly representable, so the
+    # factors must come back bit-exact -- this pins the MFMA operand/result
+    # lane layout of the trailing update
+    rng = np.random.default_rng(n)
+    L0 = np.tril(rng.choice([0.0, 0.25, -0.25, 0.5, -0.5], (n, n)), -1) + np.eye(n)
+    U0 = np.triu(rng.integers(-3, 4, (n, n)).astype(np.float64), 1) + np.diag(rng.integers(4, 8, n).astype(np.float64))
+    a = (L0 @ U0).astype(np.float32)
+    assert np.array_equal(a.astype(np.float64), L0 @ U0)
+    d = dev(a, ctx)
+    ipiv = ctx.lu_factor(d)
+    torch.cuda.synchronize()
+    lu = d.cpu().numpy().astype(np.float64)
+    np.testing.assert_array_equal(ipiv.cpu().numpy(), np.arange(n))
+    np.testing.assert_array_equal(np.triu(lu), U0)
+    np.testing.assert_array_equal(np.tril(lu, -1), np.tril(L0, -1))
+
+
+def test_lu_solve_f32_matches_numpy(ctx):
+    n = 300
+    rng = np.random.default_rng(3)
+    a = (rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)).astype(np.float32)
+    b = rng.standard_normal((n, 5)).astype(np.float32)
+    d = dev(a, ctx)
+    ipiv = ctx.lu_factor(d)
+    db = dev(b, ctx)
+    ctx.lu_solve_f32(d, ipiv, db)
+    torch.cuda.synchronize()
+    x = db.cpu().numpy()
+    xr = np.linalg.solve(a.astype(np.float64), b.astype(np.float64))
+    assert po.rel_l2(x, xr) < 1e-4
+
+
+def c3_system(n_side=64, alpha=0.05, r=2.0):
+    """A = I - dt*alpha*L on the n_side^2 interior grid (dx = 1/(n_side+1)),
+    built with the oracle's 2-D Laplacian and true-merge geam."""
+    h = 1.0 / (n_side + 1)
+    inv_h2 = np.float32(1.0 / (h * h))
+    L = po.lap2d_csr(n_side, inv_h2)
+    N = n_side * n_side
+    I = po.CSR(np.arange(N + 1, dtype=np.int32), np.arange(N, dtype=np.int32), np.ones(N, np.float32))
+    dt = np.float32(r * h * h / alpha)
+    beta = np.float32(-(dt * np.float32(alpha)))
+    return po.geam2(1.0, I, beta, L), L, I, beta
+
+
+def c3_rhs(n_side, nrhs, seed=2026):
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (n_side + 1)
+    xs = (np.arange(n_side) + 1) * h
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    out = np.zeros((n_side * n_side, nrhs))
+    for c in range(nrhs):
+        f = np.zeros_like(X)
+        for _ in range(8):
+            cx, cy = rng.uniform(0, 1, 2)
+            w = rng.uniform(0.05, 0.2)
+            f += np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * w * w))
+        out[:, c] = f.ravel()
+    return out
+
+
+@pytest.mark.parametrize("nrhs", [1, 32])
+def test_c3_mixed_precision_refinement(ctx, nrhs):
+    print(f"case c3 nrhs={nrhs}", flush=True)
+    A, L, I, beta = c3_system()
+    n = A.n_rows
+    from piro_amd.core import DevCSR
+    dA = DevCSR(dev(A.row_ptr, ctx), dev(A.col, ctx), dev(A.val, ctx))
+    dense = ctx.csr_to_dense(dA)
+    lu = dense.clone()
+    ipiv = ctx.lu_factor(lu)
+    b = c3_rhs(64, nrhs)
+    x, sweeps, rel = ctx.lu_solve_refine(dense, lu, ipiv, dev(b, ctx), tol=1e-10, max_sweeps=10)
+    torch.cuda.synchronize()
+    xh = x.cpu().numpy()
+    Ad = A.dense()
+    xr = np.linalg.solve(Ad, b)
+    res = np.linalg.norm(b - Ad @ xh, axis=0) / np.linalg.norm(b, axis=0)
+    print(f"C3 nrhs={nrhs}: {sweeps} refinement sweeps, max rel residual {res.max():.2e}, "
+          f"rel error vs FP64 solve {po.rel_l2(xh, xr):.2e}")
+    assert res.max() <= 1e-10
+    assert max(rel) <= 1e-10
+    assert po.rel_l2(xh, xr) < 1e-9
+    assert sweeps <= 4
