@@ -329,6 +329,41 @@ int piro_batch_step(piro_batch *b, const float *d_x, float *d_y);
 int piro_batch_run(piro_batch *b, float *d_x, float *d_work, int nsteps,
                    int use_graph);
 
+/* ------------------------------------------------------ learned surrogate */
+
+/* No reference counterpart (PIRO has no learned part).  A 6-layer 3x3x3
+ * residual CNN advancing a heat field k forward-Euler steps at a time:
+ * inputs u/scale (FP32) and (alpha-0.0125)/0.0075; L1 2->32 and L6 32->1 on
+ * VALU, L2-L5 32->32 on BF16 MFMA with FP32 accumulation, ReLU between
+ * layers; output u + scale*delta on interior cells, 0 on boundary cells.
+ * Fields are [n][D][H][W] FP32 (n instances of one shape, zero padding at
+ * instance boundaries).  Parameters: flat FP32 vector of
+ * piro_surrogate_num_params() values: W1[27][32][2], b1[32], then for
+ * L2..L5 W[27][32 co][32 ci], b[32], then W6[27][32], b6[1]; tap index
+ * (dz+1)*9 + (dy+1)*3 + (dx+1).  The output layer is zero-initialised, so an
+ * untrained surrogate is the identity. */
+typedef struct piro_surrogate piro_surrogate;
+int piro_surrogate_num_params(void);
+int piro_surrogate_create(piro_ctx *ctx, uint64_t seed, piro_surrogate **out);
+int piro_surrogate_destroy(piro_surrogate *s);
+int piro_surrogate_get_params(piro_surrogate *s, float *h_params);
+int piro_surrogate_set_params(piro_surrogate *s, const float *h_params);
+int piro_surrogate_forward(piro_surrogate *s, int n, int D, int H, int W,
+                           const float *d_u, const float *d_alpha, float scale,
+                           float *d_u_out);
+/* Forward + backward of the loss mean_interior((delta - target)^2) where
+ * target = (u_after_k_steps - u)/scale is supplied in d_target; FP32
+ * gradients to h_grads (may be NULL). */
+int piro_surrogate_gradients(piro_surrogate *s, int n, int D, int H, int W,
+                             const float *d_u, const float *d_alpha,
+                             const float *d_target, float scale,
+                             float *h_grads, double *loss);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on that loss. */
+int piro_surrogate_train_step(piro_surrogate *s, int n, int D, int H, int W,
+                              const float *d_u, const float *d_alpha,
+                              const float *d_target, float scale, float lr,
+                              double *loss);
+
 #ifdef __cplusplus
 }
 #endif

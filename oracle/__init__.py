@@ -79,6 +79,8 @@ def _load() -> C.CDLL:
         "po_omp_threads": (C.c_int, []),
         "po_lap2d_padded_nnz": (C.c_int64, [C.c_int, C.c_int]),
         "po_lap2d_padded_csr": (C.c_int, [C.c_int, C.c_int, C.c_float, _i32p, _i32p, _f32p]),
+        "po_conv3d_fwd_f64": (None, [C.c_int] * 6 + [_f64p, _f64p, C.c_void_p, _f64p]),
+        "po_conv3d_bwd_f64": (None, [C.c_int] * 6 + [_f64p, _f64p, _f64p, C.c_void_p, _f64p, C.c_void_p]),
         "po_fe_csr_step2d": (None, [C.c_int, C.c_int, _i32p, _i32p, _f32p, C.c_float, _f32p, _f32p]),
     }
     for name, (res, args) in sig.items():
@@ -312,6 +314,31 @@ def fe_csr_run2d(nx, ny, A: CSR, dt, nsteps, x0):
         lib.po_fe_csr_step2d(nx, ny, A.row_ptr, A.col, A.val, np.float32(dt), x, y)
         x, y = y, x
     return x
+
+
+def conv3d_fwd_f64(x, wt, bias=None):
+    """x[n,d,h,w,ci], wt[27,co,ci], bias[co] -> y[n,d,h,w,co] (FP64)."""
+    x = np.ascontiguousarray(x, np.float64)
+    wt = np.ascontiguousarray(wt, np.float64)
+    n, d, h, w, ci = x.shape
+    co = wt.shape[1]
+    y = np.empty((n, d, h, w, co), np.float64)
+    b = None if bias is None else np.ascontiguousarray(bias, np.float64)
+    lib.po_conv3d_fwd_f64(n, d, h, w, ci, co, x, wt, _ptr(b), y)
+    return y
+
+
+def conv3d_bwd_f64(x, wt, dy, need_dx=True):
+    x = np.ascontiguousarray(x, np.float64)
+    wt = np.ascontiguousarray(wt, np.float64)
+    dy = np.ascontiguousarray(dy, np.float64)
+    n, d, h, w, ci = x.shape
+    co = wt.shape[1]
+    dx = np.empty_like(x) if need_dx else None
+    dw = np.empty_like(wt)
+    db = np.empty(co, np.float64)
+    lib.po_conv3d_bwd_f64(n, d, h, w, ci, co, x, wt, dy, _ptr(dx), dw, _ptr(db))
+    return dx, dw, db
 
 
 def lu_factor_f64(a):

@@ -877,3 +877,77 @@ void po_fe_csr_step2d(int nx, int ny, const int32_t *row_ptr,
             }
         }
 }
+
+/* ------------------------------------------------ surrogate conv3d (FP64) */
+
+/* Build-only (no reference counterpart): 3x3x3 'same' convolution with zero
+ * padding on channels-last fields x[n][d][h][w][ci], weights
+ * wt[tap][co][ci] with tap = (dz+1)*9 + (dy+1)*3 + (dx+1), bias[co]:
+ *   y[v][co] = bias[co] + sum_tap sum_ci wt[tap][co][ci] * x[v+tap][ci]
+ * The FP64 yardstick of the device's BF16/FP32 kernels. */
+void po_conv3d_fwd_f64(int n, int d, int h, int w, int ci, int co,
+                       const double *x, const double *wt, const double *bias,
+                       double *y)
+{
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int b = 0; b < n; ++b)
+        for (int z = 0; z < d; ++z)
+            for (int yy = 0; yy < h; ++yy)
+                for (int xx = 0; xx < w; ++xx) {
+                    double *out = y + ((((int64_t)b * d + z) * h + yy) * w + xx) * co;
+                    for (int o = 0; o < co; ++o) out[o] = bias ? bias[o] : 0.0;
+                    for (int t = 0; t < 27; ++t) {
+                        int zz = z + t / 9 - 1, y2 = yy + (t / 3) % 3 - 1,
+                            x2 = xx + t % 3 - 1;
+                        if (zz < 0 || zz >= d || y2 < 0 || y2 >= h || x2 < 0 ||
+                            x2 >= w)
+                            continue;
+                        const double *in =
+                            x + ((((int64_t)b * d + zz) * h + y2) * w + x2) * ci;
+                        for (int o = 0; o < co; ++o) {
+                            const double *wr = wt + ((int64_t)t * co + o) * ci;
+                            double s = 0.0;
+                            for (int c = 0; c < ci; ++c) s += wr[c] * in[c];
+                            out[o] += s;
+                        }
+                    }
+                }
+}
+
+/* Gradients of po_conv3d_fwd_f64 given dy: dx (may be NULL), dw[tap][co][ci]
+ * and db[co] (may be NULL). */
+void po_conv3d_bwd_f64(int n, int d, int h, int w, int ci, int co,
+                       const double *x, const double *wt, const double *dy,
+                       double *dx, double *dw, double *db)
+{
+    int64_t nv = (int64_t)n * d * h * w;
+    if (dx) memset(dx, 0, (size_t)nv * ci * sizeof(double));
+    memset(dw, 0, (size_t)27 * co * ci * sizeof(double));
+    if (db) memset(db, 0, (size_t)co * sizeof(double));
+    for (int b = 0; b < n; ++b)
+        for (int z = 0; z < d; ++z)
+            for (int yy = 0; yy < h; ++yy)
+                for (int xx = 0; xx < w; ++xx) {
+                    int64_t v = (((int64_t)b * d + z) * h + yy) * w + xx;
+                    const double *g = dy + v * co;
+                    if (db)
+                        for (int o = 0; o < co; ++o) db[o] += g[o];
+                    for (int t = 0; t < 27; ++t) {
+                        int zz = z + t / 9 - 1, y2 = yy + (t / 3) % 3 - 1,
+                            x2 = xx + t % 3 - 1;
+                        if (zz < 0 || zz >= d || y2 < 0 || y2 >= h || x2 < 0 ||
+                            x2 >= w)
+                            continue;
+                        int64_t u = (((int64_t)b * d + zz) * h + y2) * w + x2;
+                        const double *in = x + u * ci;
+                        for (int o = 0; o < co; ++o) {
+                            double *dwr = dw + ((int64_t)t * co + o) * ci;
+                            const double *wr = wt + ((int64_t)t * co + o) * ci;
+                            for (int c = 0; c < ci; ++c) {
+                                dwr[c] += g[o] * in[c];
+                                if (dx) dx[u * ci + c] += g[o] * wr[c];
+                            }
+                        }
+                    }
+                }
+}

@@ -115,6 +115,12 @@ int     po_omp_threads(void);
 int64_t po_lap2d_padded_nnz(int nx, int ny);
 int     po_lap2d_padded_csr(int nx, int ny, float norm, int32_t *row_ptr,
                             int32_t *col, float *val);
+void    po_conv3d_fwd_f64(int n, int d, int h, int w, int ci, int co,
+                          const double *x, const double *wt,
+                          const double *bias, double *y);
+void    po_conv3d_bwd_f64(int n, int d, int h, int w, int ci, int co,
+                          const double *x, const double *wt, const double *dy,
+                          double *dx, double *dw, double *db);
 void    po_fe_csr_step2d(int nx, int ny, const int32_t *row_ptr,
                          const int32_t *col, const float *val, float dt,
                          const float *x, float *y);

@@ -90,6 +90,13 @@ _SIGS = {
     "piro_batch_autotune": [vp, vp, vp, P_i32, P_i32, P_f32],
     "piro_batch_step": [vp, vp, vp],
     "piro_batch_run": [vp, vp, vp, i32, i32],
+    "piro_surrogate_create": [vp, u64, C.POINTER(vp)],
+    "piro_surrogate_destroy": [vp],
+    "piro_surrogate_get_params": [vp, P_f32],
+    "piro_surrogate_set_params": [vp, P_f32],
+    "piro_surrogate_forward": [vp, i32, i32, i32, i32, vp, vp, f32, vp],
+    "piro_surrogate_gradients": [vp, i32, i32, i32, i32, vp, vp, vp, f32, P_f32, P_f64],
+    "piro_surrogate_train_step": [vp, i32, i32, i32, i32, vp, vp, vp, f32, f32, P_f64],
 }
 
 for _name, _args in _SIGS.items():
@@ -108,6 +115,8 @@ lib.piro_grad_norm.restype = C.c_float
 lib.piro_grad_norm.argtypes = [C.c_float, C.c_int]
 lib.piro_fe_variant_count.restype = C.c_int
 lib.piro_fe_variant_count.argtypes = []
+lib.piro_surrogate_num_params.restype = C.c_int
+lib.piro_surrogate_num_params.argtypes = []
 lib.piro_fe_variant_name.restype = C.c_char_p
 lib.piro_fe_variant_name.argtypes = [C.c_int]
 

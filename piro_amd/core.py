@@ -340,3 +340,59 @@ class Batch:
     def run(self, x, work, nsteps, use_graph=True):
         check(lib.piro_batch_run(self.h, _p(x), _p(work), int(nsteps), 1 if use_graph else 0))
         return x
+
+
+class Surrogate:
+    """The learned k-step propagator (see include/piro_hip.h)."""
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib.piro_surrogate_create(ctx.h, int(seed), C.byref(h)))
+        self.h = h
+        self.n_params = lib.piro_surrogate_num_params()
+
+    def close(self):
+        if self.h:
+            check(lib.piro_surrogate_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        import numpy as np
+        p = np.empty(self.n_params, np.float32)
+        check(lib.piro_surrogate_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+        return p
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        check(lib.piro_surrogate_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def forward(self, shape, u, alpha, scale, out=None):
+        n, D, H, W = shape
+        out = torch.empty_like(u) if out is None else out
+        check(lib.piro_surrogate_forward(self.h, n, D, H, W, _p(u), _p(alpha), float(scale), _p(out)))
+        return out
+
+    def gradients(self, shape, u, alpha, target, scale):
+        import numpy as np
+        n, D, H, W = shape
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_double()
+        check(lib.piro_surrogate_gradients(self.h, n, D, H, W, _p(u), _p(alpha), _p(target), float(scale),
+                                           g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, shape, u, alpha, target, scale, lr):
+        n, D, H, W = shape
+        loss = C.c_double()
+        check(lib.piro_surrogate_train_step(self.h, n, D, H, W, _p(u), _p(alpha), _p(target), float(scale),
+                                            float(lr), C.byref(loss)))
+        return loss.value

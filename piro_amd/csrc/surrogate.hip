@@ -1,0 +1,792 @@
+// surrogate.hip -- learned k-step heat propagator: a 6-layer 3x3x3 residual
+// CNN trained and evaluated by the project's own kernels (no reference
+// counterpart; PIRO has no learned part).
+//
+//   in  : u/s (FP32) and the normalised diffusivity (alpha-0.0125)/0.0075
+//   L1  : conv 2->32 + ReLU     (VALU, FP32 inputs and weights, BF16 output)
+//   L2-5: conv 32->32 + ReLU    (MFMA v_mfma_f32_16x16x32_bf16: BF16 weights
+//                                and activations, FP32 accumulation)
+//   L6  : conv 32->1            (VALU, FP32 accumulation)
+//   out : u + s*delta on interior cells, 0 on the boundary (zero Dirichlet)
+//
+// Fields are channels-last [n][D][H][W][C]; zero padding at every instance
+// boundary.  The input layer reads u in FP32 because the network's job is a
+// small difference of neighbouring values, which BF16 inputs would drown.
+//
+// MFMA operand layout of v_mfma_f32_16x16x32_bf16 (wave64): lane l supplies
+// A[i = l%16][k = 8*(l/16) .. +7] and B[k = 8*(l/16) .. +7][j = l%16]; it
+// receives D[i = 4*(l/16) + r][j = l%16], r = 0..3.
+//
+// Backward: dgrad of L2-5 is the forward MFMA kernel run on the output
+// gradient with the flipped, transposed weights and the ReLU mask of the
+// layer input fused in its epilogue; wgrad of L2-5 is an MFMA GEMM over
+// voxels with the operands transposed through LDS; L1/L6 gradients on VALU.
+// Per-workgroup weight-gradient partials are summed in a fixed order, so a
+// training step is deterministic.
+#include "device_utils.hpp"
+
+#include <cmath>
+#include <random>
+#include <vector>
+
+using namespace piro;
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int CH = 32;
+constexpr int NT = 27;
+// parameter offsets in the flat FP32 vector
+constexpr int W1_OFF = 0;                    // [27][32][2]
+constexpr int B1_OFF = W1_OFF + NT * CH * 2; // [32]
+constexpr int WM_OFF = B1_OFF + CH;          // 4 x ([27][32][32] + [32])
+constexpr int WM_SZ = NT * CH * CH + CH;
+constexpr int W6_OFF = WM_OFF + 4 * WM_SZ;   // [27][32]
+constexpr int B6_OFF = W6_OFF + NT * CH;     // [1]
+constexpr int N_PARAMS = B6_OFF + 1;
+
+constexpr float ALPHA_MU = 0.0125f, ALPHA_INV = 1.0f / 0.0075f;
+
+struct Shape {
+    int n, D, H, W;
+    __host__ __device__ int64_t nv() const { return (int64_t)n * D * H * W; }
+};
+
+__device__ __forceinline__ bool voxel_coords(int64_t v, const Shape &s, int &b, int &z, int &y, int &x)
+{
+    x = (int)(v % s.W);
+    int64_t t = v / s.W;
+    y = (int)(t % s.H);
+    t /= s.H;
+    z = (int)(t % s.D);
+    b = (int)(t / s.D);
+    return b < s.n;
+}
+
+__device__ __forceinline__ int64_t vidx(int b, int z, int y, int x, const Shape &s)
+{
+    return (((int64_t)b * s.D + z) * s.H + y) * s.W + x;
+}
+
+// ------------------------------------------------------------------ L1
+
+__global__ __launch_bounds__(256) void k_l1_fwd(Shape s, const float *__restrict__ u, const float *__restrict__ al,
+                                                float inv_scale, const float *__restrict__ w1,
+                                                const float *__restrict__ b1, bf16 *__restrict__ x1)
+{
+    __shared__ float sw[NT * CH * 2 + CH];
+    for (int i = threadIdx.x; i < NT * CH * 2 + CH; i += 256) sw[i] = i < NT * CH * 2 ? w1[i] : b1[i - NT * CH * 2];
+    __syncthreads();
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int b, z, y, x;
+    if (v >= s.nv() || !voxel_coords(v, s, b, z, y, x)) return;
+    float acc[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) acc[c] = sw[NT * CH * 2 + c];
+    for (int t = 0; t < NT; ++t) {
+        const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
+        if (zz < 0 || zz >= s.D || yy < 0 || yy >= s.H || xx < 0 || xx >= s.W) continue;
+        const int64_t q = vidx(b, zz, yy, xx, s);
+        const float in0 = u[q] * inv_scale, in1 = (al[q] - ALPHA_MU) * ALPHA_INV;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] += sw[(t * CH + c) * 2] * in0 + sw[(t * CH + c) * 2 + 1] * in1;
+    }
+    bf16x8 o[4];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) o[c / 8][c % 8] = (bf16)fmaxf(acc[c], 0.0f);
+    bf16x8 *dst = (bf16x8 *)(x1 + v * CH);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) dst[g] = o[g];
+}
+
+// dW1[t][c][k] = sum_v dz[v][c] * in_k[v+t]; db1[c] = sum_v dz[v][c]
+// one block per chunk of voxels; partials[block][NT*CH*2 + CH]
+__global__ __launch_bounds__(256) void k_l1_wgrad(Shape s, const float *__restrict__ u,
+                                                  const float *__restrict__ al, float inv_scale,
+                                                  const bf16 *__restrict__ dz, int64_t chunk,
+                                                  float *__restrict__ partial)
+{
+    // thread layout: 32 channels x 8 voxel lanes
+    const int c = threadIdx.x & 31, lane8 = threadIdx.x >> 5;
+    float g[NT * 2];
+#pragma unroll
+    for (int i = 0; i < NT * 2; ++i) g[i] = 0.0f;
+    float gb = 0.0f;
+    const int64_t v0 = (int64_t)blockIdx.x * chunk, v1 = min(v0 + chunk, s.nv());
+    for (int64_t v = v0 + lane8; v < v1; v += 8) {
+        int b, z, y, x;
+        voxel_coords(v, s, b, z, y, x);
+        const float d = (float)dz[v * CH + c];
+        if (d == 0.0f) continue;
+        gb += d;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
+            if (zz < 0 || zz >= s.D || yy < 0 || yy >= s.H || xx < 0 || xx >= s.W) continue;
+            const int64_t q = vidx(b, zz, yy, xx, s);
+            g[2 * t] += d * (u[q] * inv_scale);
+            g[2 * t + 1] += d * ((al[q] - ALPHA_MU) * ALPHA_INV);
+        }
+    }
+    __shared__ float red[8][CH];
+    float *out = partial + (int64_t)blockIdx.x * (NT * CH * 2 + CH);
+    for (int i = 0; i <= NT * 2; ++i) {
+        red[lane8][c] = i < NT * 2 ? g[i] : gb;
+        __syncthreads();
+        if (lane8 == 0) {
+            float ssum = 0.0f;
+            for (int q = 0; q < 8; ++q) ssum += red[q][c];
+            if (i < NT * 2) out[((i / 2) * CH + c) * 2 + (i % 2)] = ssum;
+            else out[NT * CH * 2 + c] = ssum;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ L6
+
+// training: delta[v] = b6 + sum; inference: out[v] = interior ? u + s*delta : 0
+template <bool INFER>
+__global__ __launch_bounds__(256) void k_l6_fwd(Shape s, const bf16 *__restrict__ x5, const float *__restrict__ w6,
+                                                const float *__restrict__ b6, const float *__restrict__ u, float scale,
+                                                float *__restrict__ out)
+{
+    __shared__ float sw[NT * CH];
+    for (int i = threadIdx.x; i < NT * CH; i += 256) sw[i] = w6[i];
+    __syncthreads();
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int b, z, y, x;
+    if (v >= s.nv() || !voxel_coords(v, s, b, z, y, x)) return;
+    float acc = b6[0];
+    for (int t = 0; t < NT; ++t) {
+        const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
+        if (zz < 0 || zz >= s.D || yy < 0 || yy >= s.H || xx < 0 || xx >= s.W) continue;
+        const bf16x8 *src = (const bf16x8 *)(x5 + vidx(b, zz, yy, xx, s) * CH);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            bf16x8 q = src[g];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc += sw[t * CH + g * 8 + e] * (float)q[e];
+        }
+    }
+    if (INFER) {
+        const bool interior = z > 0 && z < s.D - 1 && y > 0 && y < s.H - 1 && x > 0 && x < s.W - 1;
+        out[v] = interior ? u[v] + scale * acc : 0.0f;
+    } else {
+        out[v] = acc;
+    }
+}
+
+// loss on interior cells: d[v] = 2 (delta - target)/n_int; per-block SSE
+__global__ __launch_bounds__(256) void k_mse(Shape s, const float *__restrict__ delta,
+                                             const float *__restrict__ target, float inv_n, float *__restrict__ dd,
+                                             double *__restrict__ sse_part)
+{
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int b, z, y, x;
+    double e2 = 0.0;
+    if (v < s.nv() && voxel_coords(v, s, b, z, y, x)) {
+        const bool interior = z > 0 && z < s.D - 1 && y > 0 && y < s.H - 1 && x > 0 && x < s.W - 1;
+        float e = interior ? delta[v] - target[v] : 0.0f;
+        dd[v] = 2.0f * e * inv_n;
+        e2 = (double)e * (double)e;
+    }
+    double t = block_sum_f64<256>(e2);
+    if (threadIdx.x == 0) sse_part[blockIdx.x] = t;
+}
+
+// dX5[v][c] = (sum_t w6[t][c] * dd[v - off(t)]) * (x5[v][c] > 0)
+__global__ __launch_bounds__(256) void k_l6_dgrad(Shape s, const float *__restrict__ dd, const float *__restrict__ w6,
+                                                  const bf16 *__restrict__ x5, bf16 *__restrict__ dx5)
+{
+    __shared__ float sw[NT * CH];
+    for (int i = threadIdx.x; i < NT * CH; i += 256) sw[i] = w6[i];
+    __syncthreads();
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int b, z, y, x;
+    if (v >= s.nv() || !voxel_coords(v, s, b, z, y, x)) return;
+    float acc[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) acc[c] = 0.0f;
+    for (int t = 0; t < NT; ++t) {
+        const int zz = z - (t / 9 - 1), yy = y - ((t / 3) % 3 - 1), xx = x - (t % 3 - 1);
+        if (zz < 0 || zz >= s.D || yy < 0 || yy >= s.H || xx < 0 || xx >= s.W) continue;
+        const float d = dd[vidx(b, zz, yy, xx, s)];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] += sw[t * CH + c] * d;
+    }
+    const bf16x8 *m = (const bf16x8 *)(x5 + v * CH);
+    bf16x8 *dst = (bf16x8 *)(dx5 + v * CH);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        bf16x8 mm = m[g], o;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) o[e] = (bf16)((float)mm[e] > 0.0f ? acc[g * 8 + e] : 0.0f);
+        dst[g] = o;
+    }
+}
+
+// dW6[t][c] = sum_v dd[v] x5[v+off(t)][c]; db6 = sum dd; partial[block][NT*CH+1]
+__global__ __launch_bounds__(256) void k_l6_wgrad(Shape s, const float *__restrict__ dd,
+                                                  const bf16 *__restrict__ x5, int64_t chunk,
+                                                  float *__restrict__ partial)
+{
+    const int c = threadIdx.x & 31, lane8 = threadIdx.x >> 5;
+    float g[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) g[t] = 0.0f;
+    float gb = 0.0f;
+    const int64_t v0 = (int64_t)blockIdx.x * chunk, v1 = min(v0 + chunk, s.nv());
+    for (int64_t v = v0 + lane8; v < v1; v += 8) {
+        int b, z, y, x;
+        voxel_coords(v, s, b, z, y, x);
+        const float d = dd[v];
+        if (d == 0.0f) continue;
+        gb += d;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
+            if (zz < 0 || zz >= s.D || yy < 0 || yy >= s.H || xx < 0 || xx >= s.W) continue;
+            g[t] += d * (float)x5[vidx(b, zz, yy, xx, s) * CH + c];
+        }
+    }
+    __shared__ float red[8][CH];
+    float *out = partial + (int64_t)blockIdx.x * (NT * CH + 1);
+    for (int t = 0; t <= NT; ++t) {
+        red[lane8][c] = t < NT ? g[t] : gb;
+        __syncthreads();
+        if (lane8 == 0) {
+            float ssum = 0.0f;
+            for (int q = 0; q < 8; ++q) ssum += red[q][c];
+            if (t < NT) out[t * CH + c] = ssum;
+        }
+        if (t == NT && threadIdx.x == 0) {
+            // every channel lane accumulated the same sum of d: take channel 0's
+            float ssum = 0.0f;
+            for (int q = 0; q < 8; ++q) ssum += red[q][0];
+            out[NT * CH] = ssum;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------- 32 -> 32 MFMA conv
+
+constexpr int BZ = 4, BY = 8, BX = 8;          // output brick (one z-slice per wave)
+constexpr int HZ = BZ + 2, HY = BY + 2, HX = BX + 2;
+constexpr int HALO = HZ * HY * HX;             // 600 voxels
+
+// MODE 0: y = relu(conv(x, w) + bias); MODE 1 (dgrad): y = conv(x, w) * (mask > 0)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_conv32(Shape s, const bf16 *__restrict__ x, const bf16 *__restrict__ w,
+                                                const float *__restrict__ bias, const bf16 *__restrict__ mask,
+                                                bf16 *__restrict__ y)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sx[HALO * CH];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int li = lane & 15, kg = lane >> 4;
+    // B fragments of all taps, both output-channel tiles, in registers
+    bf16x8 wb[NT][2];
+#pragma unroll
+    for (int tp = 0; tp < NT; ++tp)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+            wb[tp][nt] = *(const bf16x8 *)(w + ((int64_t)tp * CH + nt * 16 + li) * CH + kg * 8);
+    float bsum[2] = {0.0f, 0.0f};
+    if (MODE == 0) {
+        bsum[0] = bias[li];
+        bsum[1] = bias[16 + li];
+    }
+    const int nbz = (s.D + BZ - 1) / BZ, nby = (s.H + BY - 1) / BY, nbx = (s.W + BX - 1) / BX;
+    const int64_t nbricks = (int64_t)s.n * nbz * nby * nbx;
+    for (int64_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
+        int64_t q = br;
+        const int bx = (int)(q % nbx); q /= nbx;
+        const int by = (int)(q % nby); q /= nby;
+        const int bz = (int)(q % nbz);
+        const int bn = (int)(q / nbz);
+        const int z0 = bz * BZ, y0 = by * BY, x0 = bx * BX;
+        __syncthreads();
+        // halo brick -> LDS, 16 bytes per (voxel, channel group)
+        for (int e = t; e < HALO * 4; e += 256) {
+            const int hv = e >> 2, g = e & 3;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int zz = z0 + hz - 1, yy = y0 + hy - 1, xx = x0 + hx - 1;
+            bf16x8 val = {};
+            if (zz >= 0 && zz < s.D && yy >= 0 && yy < s.H && xx >= 0 && xx < s.W)
+                val = *(const bf16x8 *)(x + vidx(bn, zz, yy, xx, s) * CH + g * 8);
+            *(bf16x8 *)(sx + hv * CH + g * 8) = val;
+        }
+        __syncthreads();
+        f32x4 acc[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) acc[m][nt] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int tp = 0; tp < NT; ++tp) {
+            const int tz = tp / 9, ty = (tp / 3) % 3, tx = tp % 3;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                // A row i = li: voxel (y = 2m + li/8, x = li%8) of slice wv
+                const int hv = ((wv + tz) * HY + (2 * m + (li >> 3) + ty)) * HX + ((li & 7) + tx);
+                const bf16x8 a = *(const bf16x8 *)(sx + hv * CH + kg * 8);
+                acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb[tp][0], acc[m][0], 0, 0, 0);
+                acc[m][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb[tp][1], acc[m][1], 0, 0, 0);
+            }
+        }
+        // epilogue: D[i = 4*kg + r][j = li] of tile (m, nt)
+        const int zz = z0 + wv;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 4 * kg + r;
+                const int yy = y0 + 2 * m + (i >> 3), xx = x0 + (i & 7);
+                if (zz >= s.D || yy >= s.H || xx >= s.W) continue;
+                const int64_t vo = vidx(bn, zz, yy, xx, s) * CH;
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt) {
+                    const int c = nt * 16 + li;
+                    float o = acc[m][nt][r];
+                    if (MODE == 0) o = fmaxf(o + bsum[nt], 0.0f);
+                    else o = (float)mask[vo + c] > 0.0f ? o : 0.0f;
+                    y[vo + c] = (bf16)o;
+                }
+            }
+    }
+}
+
+// -------------------------------------------------- 32 -> 32 MFMA wgrad
+
+// dW[t][co][ci] = sum_v dz[v][co] x[v + off(t)][ci], db[co] = sum_v dz[v][co]
+// brick of 4x8x8 voxels; dz transposed in LDS [co][256]; x halo transposed
+// in LDS as three x-shifted copies [s][ci][6][10][8] so that every operand
+// fragment is one aligned 16-byte read.  Wave w owns taps w, w+4, ...
+constexpr int WG_TAPS = 7;  // ceil(27/4)
+
+__global__ __launch_bounds__(256) void k_wgrad32(Shape s, const bf16 *__restrict__ x, const bf16 *__restrict__ dz,
+                                                 float *__restrict__ partial)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sdz[CH * BZ * BY * BX];
+    __shared__ __attribute__((aligned(16))) bf16 sxs[3 * CH * HZ * HY * BX];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int li = lane & 15, kg = lane >> 4;
+    f32x4 acc[WG_TAPS][2][2];
+#pragma unroll
+    for (int a = 0; a < WG_TAPS; ++a)
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int n2 = 0; n2 < 2; ++n2) acc[a][m][n2] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    float dbias = 0.0f;  // thread t < 32: channel t
+    const int nbz = (s.D + BZ - 1) / BZ, nby = (s.H + BY - 1) / BY, nbx = (s.W + BX - 1) / BX;
+    const int64_t nbricks = (int64_t)s.n * nbz * nby * nbx;
+    for (int64_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
+        int64_t q = br;
+        const int bx = (int)(q % nbx); q /= nbx;
+        const int by = (int)(q % nby); q /= nby;
+        const int bz = (int)(q % nbz);
+        const int bn = (int)(q / nbz);
+        const int z0 = bz * BZ, y0 = by * BY, x0 = bx * BX;
+        __syncthreads();
+        // dz brick, transposed: sdz[c][v], v = (z*8 + y)*8 + x
+        {
+            const int v = t;  // 256 voxels, one per thread
+            const int zz = z0 + v / 64, yy = y0 + (v / 8) % 8, xx = x0 + v % 8;
+            const bool in = zz < s.D && yy < s.H && xx < s.W;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                bf16x8 val = {};
+                if (in) val = *(const bf16x8 *)(dz + vidx(bn, zz, yy, xx, s) * CH + g * 8);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) sdz[(g * 8 + e) * 256 + v] = val[e];
+            }
+        }
+        // x halo, transposed and x-shifted: sxs[sh][c][hz][hy][j] = X(hx = j + sh)
+        for (int e = t; e < HALO * 4; e += 256) {
+            const int hv = e >> 2, g = e & 3;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int zz = z0 + hz - 1, yy = y0 + hy - 1, xx = x0 + hx - 1;
+            bf16x8 val = {};
+            if (zz >= 0 && zz < s.D && yy >= 0 && yy < s.H && xx >= 0 && xx < s.W)
+                val = *(const bf16x8 *)(x + vidx(bn, zz, yy, xx, s) * CH + g * 8);
+#pragma unroll
+            for (int sh = 0; sh < 3; ++sh) {
+                const int j = hx - sh;
+                if (j < 0 || j >= BX) continue;
+#pragma unroll
+                for (int ee = 0; ee < 8; ++ee)
+                    sxs[(((sh * CH + g * 8 + ee) * HZ + hz) * HY + hy) * BX + j] = val[ee];
+            }
+        }
+        __syncthreads();
+        if (t < CH)
+            for (int v = 0; v < 256; ++v) dbias += (float)sdz[t * 256 + v];
+        // K loop: 8 steps of 32 voxels (4 x-rows of 8)
+        for (int ks = 0; ks < 8; ++ks) {
+            const int row = ks * 4 + kg;          // x-row index 0..31 = z*8 + y
+            const int oz = row / 8, oy = row % 8;
+            bf16x8 af[2];
+#pragma unroll
+            for (int m = 0; m < 2; ++m) af[m] = *(const bf16x8 *)(sdz + (m * 16 + li) * 256 + row * 8);
+#pragma unroll
+            for (int a = 0; a < WG_TAPS; ++a) {
+                const int tp = wv + 4 * a;
+                if (tp >= NT) break;
+                const int tz = tp / 9, ty = (tp / 3) % 3, tx = tp % 3;
+#pragma unroll
+                for (int n2 = 0; n2 < 2; ++n2) {
+                    const bf16x8 bfr = *(const bf16x8 *)(sxs + (((tx * CH + n2 * 16 + li) * HZ + oz + tz) * HY + oy + ty) *
+                                                                   BX);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                        acc[a][m][n2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bfr, acc[a][m][n2], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // partial[block][NT*CH*CH + CH]: D[i = co (4*kg + r) ][j = ci (li)]
+    float *out = partial + (int64_t)blockIdx.x * (NT * CH * CH + CH);
+#pragma unroll
+    for (int a = 0; a < WG_TAPS; ++a) {
+        const int tp = wv + 4 * a;
+        if (tp >= NT) break;
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int n2 = 0; n2 < 2; ++n2)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int co = m * 16 + 4 * kg + r, ci = n2 * 16 + li;
+                    out[((int64_t)tp * CH + co) * CH + ci] = acc[a][m][n2][r];
+                }
+    }
+    if (t < CH) out[NT * CH * CH + t] = dbias;
+}
+
+// out[j] = sum_b partial[b][j] (fixed order)
+__global__ __launch_bounds__(256) void k_sum_partials(const float *__restrict__ partial, int nb, int len,
+                                                      float *__restrict__ out)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= len) return;
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b) s += partial[(int64_t)b * len + j];
+    out[j] = s;
+}
+
+// Adam on the flat FP32 parameters, then refresh the BF16 copies:
+// wf[l][t][co][ci] (forward) and wd[l][t'][ci][co] = w[26-t'][co][ci] (dgrad)
+__global__ __launch_bounds__(256) void k_adam(int n, float *__restrict__ p, const float *__restrict__ g,
+                                              float *__restrict__ m, float *__restrict__ v, float lr, float b1,
+                                              float b2, float eps, float c1, float c2)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ __launch_bounds__(256) void k_pack(const float *__restrict__ p, bf16 *__restrict__ wf,
+                                              bf16 *__restrict__ wd)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= 4 * NT * CH * CH) return;
+    const int l = i / (NT * CH * CH), r = i % (NT * CH * CH);
+    const int tp = r / (CH * CH), co = (r / CH) % CH, ci = r % CH;
+    const float w = p[WM_OFF + l * WM_SZ + r];
+    wf[i] = (bf16)w;
+    wd[(int64_t)l * NT * CH * CH + ((int64_t)(NT - 1 - tp) * CH + ci) * CH + co] = (bf16)w;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+
+struct piro_surrogate {
+    piro_ctx *ctx = nullptr;
+    float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    bf16 *wf = nullptr, *wd = nullptr;
+    int64_t adam_t = 0;
+    // activations / gradients (grown on demand)
+    int64_t cap_nv = 0;
+    bf16 *act = nullptr;     // 5 x [nv][32]
+    bf16 *gbuf = nullptr;    // 2 x [nv][32]
+    float *delta = nullptr, *dd = nullptr;
+    float *partial = nullptr;
+    int64_t cap_partial = 0;
+    double *sse = nullptr;
+    int64_t cap_sse = 0;
+};
+
+namespace {
+
+piro_surrogate *sur(piro_surrogate *s)
+{
+    if (!s) throw Error(PIRO_EINVAL, "null surrogate");
+    ctx_of(s->ctx);
+    return s;
+}
+
+template <typename T>
+void grow(T *&p, int64_t &cap, int64_t need)
+{
+    if (need <= cap) return;
+    if (p) PIRO_CHECK_HIP(hipFree(p));
+    p = nullptr;
+    hipError_t e = hipMalloc(&p, (size_t)need * sizeof(T));
+    if (e != hipSuccess) throw Error(PIRO_ENOMEM, "surrogate workspace allocation failed");
+    cap = need;
+}
+
+void ensure_acts(piro_surrogate *s, int64_t nv)
+{
+    if (nv <= s->cap_nv) return;
+    if (s->act) PIRO_CHECK_HIP(hipFree(s->act));
+    if (s->gbuf) PIRO_CHECK_HIP(hipFree(s->gbuf));
+    if (s->delta) PIRO_CHECK_HIP(hipFree(s->delta));
+    if (s->dd) PIRO_CHECK_HIP(hipFree(s->dd));
+    s->act = nullptr; s->gbuf = nullptr; s->delta = nullptr; s->dd = nullptr;
+    hipError_t e1 = hipMalloc(&s->act, (size_t)nv * CH * 5 * sizeof(bf16));
+    hipError_t e2 = hipMalloc(&s->gbuf, (size_t)nv * CH * 2 * sizeof(bf16));
+    hipError_t e3 = hipMalloc(&s->delta, (size_t)nv * sizeof(float));
+    hipError_t e4 = hipMalloc(&s->dd, (size_t)nv * sizeof(float));
+    if (e1 || e2 || e3 || e4) throw Error(PIRO_ENOMEM, "surrogate activation allocation failed");
+    s->cap_nv = nv;
+}
+
+int conv_grid(piro_surrogate *s) { return s->ctx->num_cus; }
+
+void repack(piro_surrogate *s)
+{
+    k_pack<<<grid_for(4 * NT * CH * CH, 256), 256, 0, s->ctx->stream>>>(s->params, s->wf, s->wd);
+    check_launch();
+}
+
+// forward through L5; L6 by the caller
+void forward_body(piro_surrogate *s, const Shape &sh, const float *u, const float *al, float inv_scale)
+{
+    hipStream_t st = s->ctx->stream;
+    const int64_t nv = sh.nv();
+    bf16 *x[5];
+    for (int l = 0; l < 5; ++l) x[l] = s->act + (int64_t)l * nv * CH;
+    k_l1_fwd<<<grid_for(nv, 256), 256, 0, st>>>(sh, u, al, inv_scale, s->params + W1_OFF, s->params + B1_OFF, x[0]);
+    check_launch();
+    for (int l = 0; l < 4; ++l) {
+        k_conv32<0><<<conv_grid(s), 256, 0, st>>>(sh, x[l], s->wf + (int64_t)l * NT * CH * CH,
+                                                   s->params + WM_OFF + l * WM_SZ + NT * CH * CH, nullptr, x[l + 1]);
+        check_launch();
+    }
+}
+
+Shape mkshape(int n, int D, int H, int W)
+{
+    PIRO_REQUIRE(n > 0 && D >= 1 && H >= 1 && W >= 1, "bad shape");
+    Shape s{n, D, H, W};
+    PIRO_REQUIRE(s.nv() < (1LL << 31) / CH, "shape too large");
+    return s;
+}
+
+// forward + backward; gradients into s->grads; returns the mean squared error
+double forward_backward(piro_surrogate *s, const Shape &sh, const float *u, const float *al, const float *target,
+                        float scale)
+{
+    hipStream_t st = s->ctx->stream;
+    const int64_t nv = sh.nv();
+    ensure_acts(s, nv);
+    const float inv_scale = 1.0f / scale;
+    forward_body(s, sh, u, al, inv_scale);
+    bf16 *x[5];
+    for (int l = 0; l < 5; ++l) x[l] = s->act + (int64_t)l * nv * CH;
+    k_l6_fwd<false><<<grid_for(nv, 256), 256, 0, st>>>(sh, x[4], s->params + W6_OFF, s->params + B6_OFF, nullptr,
+                                                       1.0f, s->delta);
+    check_launch();
+    const int64_t n_int = (int64_t)sh.n * std::max(sh.D - 2, 0) * std::max(sh.H - 2, 0) * std::max(sh.W - 2, 0);
+    PIRO_REQUIRE(n_int > 0, "instances need interior cells");
+    const int nb = (int)grid_for(nv, 256);
+    grow(s->sse, s->cap_sse, nb);
+    k_mse<<<nb, 256, 0, st>>>(sh, s->delta, target, 1.0f / (float)n_int, s->dd, s->sse);
+    check_launch();
+    // partial buffers: sized for the largest user
+    const int gparts = conv_grid(s);
+    const int64_t chunk = std::max<int64_t>(4096, (nv + 1023) / 1024);
+    const int nchunks = (int)((nv + chunk - 1) / chunk);
+    grow(s->partial, s->cap_partial,
+         std::max<int64_t>((int64_t)gparts * (NT * CH * CH + CH), (int64_t)nchunks * (NT * CH * 2 + CH)));
+    // L6
+    k_l6_wgrad<<<nchunks, 256, 0, st>>>(sh, s->dd, x[4], chunk, s->partial);
+    check_launch();
+    k_sum_partials<<<grid_for(NT * CH + 1, 256), 256, 0, st>>>(s->partial, nchunks, NT * CH + 1, s->grads + W6_OFF);
+    check_launch();
+    bf16 *g0 = s->gbuf, *g1 = s->gbuf + nv * CH;
+    k_l6_dgrad<<<grid_for(nv, 256), 256, 0, st>>>(sh, s->dd, s->params + W6_OFF, x[4], g0);
+    check_launch();
+    // L5..L2 (layer l maps x[l-1] -> x[l], l = 4..1 in x indices)
+    for (int l = 3; l >= 0; --l) {
+        // g0 = dL/d(pre-activation of x[l+1]) (masked)
+        k_wgrad32<<<gparts, 256, 0, st>>>(sh, x[l], g0, s->partial);
+        check_launch();
+        k_sum_partials<<<grid_for(NT * CH * CH + CH, 256), 256, 0, st>>>(s->partial, gparts, NT * CH * CH + CH,
+                                                                          s->grads + WM_OFF + l * WM_SZ);
+        check_launch();
+        k_conv32<1><<<conv_grid(s), 256, 0, st>>>(sh, g0, s->wd + (int64_t)l * NT * CH * CH, nullptr, x[l], g1);
+        check_launch();
+        std::swap(g0, g1);
+    }
+    // L1
+    k_l1_wgrad<<<nchunks, 256, 0, st>>>(sh, u, al, inv_scale, g0, chunk, s->partial);
+    check_launch();
+    k_sum_partials<<<grid_for(NT * CH * 2 + CH, 256), 256, 0, st>>>(s->partial, nchunks, NT * CH * 2 + CH,
+                                                                     s->grads + W1_OFF);
+    check_launch();
+    std::vector<double> h(nb);
+    PIRO_CHECK_HIP(hipMemcpyAsync(h.data(), s->sse, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    PIRO_CHECK_HIP(hipStreamSynchronize(st));
+    double sse = 0.0;
+    for (double v : h) sse += v;
+    return sse / (double)n_int;
+}
+
+}  // namespace
+
+extern "C" {
+
+int piro_surrogate_num_params(void) { return N_PARAMS; }
+
+int piro_surrogate_create(piro_ctx *ctx, uint64_t seed, piro_surrogate **out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(out, "out is null");
+    piro_surrogate *s = new piro_surrogate();
+    s->ctx = ctx;
+    try {
+        PIRO_CHECK_HIP(hipMalloc(&s->params, N_PARAMS * sizeof(float)));
+        PIRO_CHECK_HIP(hipMalloc(&s->grads, N_PARAMS * sizeof(float)));
+        PIRO_CHECK_HIP(hipMalloc(&s->adam_m, N_PARAMS * sizeof(float)));
+        PIRO_CHECK_HIP(hipMalloc(&s->adam_v, N_PARAMS * sizeof(float)));
+        PIRO_CHECK_HIP(hipMalloc(&s->wf, 4 * NT * CH * CH * sizeof(bf16)));
+        PIRO_CHECK_HIP(hipMalloc(&s->wd, 4 * NT * CH * CH * sizeof(bf16)));
+        // He-style uniform init; the output layer starts at zero so the
+        // untrained surrogate is the identity map
+        std::vector<float> p(N_PARAMS, 0.0f);
+        std::mt19937_64 rng(seed);
+        auto fill = [&](int off, int count, int fan_in) {
+            float lim = std::sqrt(6.0f / (float)fan_in);
+            std::uniform_real_distribution<float> U(-lim, lim);
+            for (int i = 0; i < count; ++i) p[off + i] = U(rng);
+        };
+        fill(W1_OFF, NT * CH * 2, NT * 2);
+        for (int l = 0; l < 4; ++l) fill(WM_OFF + l * WM_SZ, NT * CH * CH, NT * CH);
+        PIRO_CHECK_HIP(hipMemcpy(s->params, p.data(), N_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+        PIRO_CHECK_HIP(hipMemset(s->adam_m, 0, N_PARAMS * sizeof(float)));
+        PIRO_CHECK_HIP(hipMemset(s->adam_v, 0, N_PARAMS * sizeof(float)));
+        PIRO_CHECK_HIP(hipMemset(s->grads, 0, N_PARAMS * sizeof(float)));
+        repack(s);
+        PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        (void)hipFree(s->params); (void)hipFree(s->grads); (void)hipFree(s->adam_m); (void)hipFree(s->adam_v);
+        (void)hipFree(s->wf); (void)hipFree(s->wd);
+        delete s;
+        throw;
+    }
+    *out = s;
+    PIRO_API_END
+}
+
+int piro_surrogate_destroy(piro_surrogate *s)
+{
+    PIRO_API_BEGIN
+    if (!s) return PIRO_OK;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    for (void *p : {(void *)s->params, (void *)s->grads, (void *)s->adam_m, (void *)s->adam_v, (void *)s->wf,
+                    (void *)s->wd, (void *)s->act, (void *)s->gbuf, (void *)s->delta, (void *)s->dd,
+                    (void *)s->partial, (void *)s->sse})
+        if (p) (void)hipFree(p);
+    delete s;
+    PIRO_API_END
+}
+
+int piro_surrogate_get_params(piro_surrogate *s, float *h_params)
+{
+    PIRO_API_BEGIN
+    sur(s);
+    PIRO_REQUIRE(h_params, "null");
+    PIRO_CHECK_HIP(hipMemcpyAsync(h_params, s->params, N_PARAMS * sizeof(float), hipMemcpyDeviceToHost, s->ctx->stream));
+    PIRO_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    PIRO_API_END
+}
+
+int piro_surrogate_set_params(piro_surrogate *s, const float *h_params)
+{
+    PIRO_API_BEGIN
+    sur(s);
+    PIRO_REQUIRE(h_params, "null");
+    PIRO_CHECK_HIP(hipMemcpyAsync(s->params, h_params, N_PARAMS * sizeof(float), hipMemcpyHostToDevice, s->ctx->stream));
+    repack(s);
+    PIRO_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    PIRO_API_END
+}
+
+int piro_surrogate_forward(piro_surrogate *s, int n, int D, int H, int W, const float *d_u, const float *d_alpha,
+                           float scale, float *d_u_out)
+{
+    PIRO_API_BEGIN
+    sur(s);
+    PIRO_REQUIRE(d_u && d_alpha && d_u_out && d_u != d_u_out, "bad buffers");
+    PIRO_REQUIRE(scale > 0.0f && std::isfinite(scale), "scale must be positive and finite");
+    Shape sh = mkshape(n, D, H, W);
+    ensure_acts(s, sh.nv());
+    forward_body(s, sh, d_u, d_alpha, 1.0f / scale);
+    k_l6_fwd<true><<<grid_for(sh.nv(), 256), 256, 0, s->ctx->stream>>>(
+        sh, s->act + 4 * sh.nv() * CH, s->params + W6_OFF, s->params + B6_OFF, d_u, scale, d_u_out);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_surrogate_gradients(piro_surrogate *s, int n, int D, int H, int W, const float *d_u, const float *d_alpha,
+                             const float *d_target, float scale, float *h_grads, double *loss)
+{
+    PIRO_API_BEGIN
+    sur(s);
+    PIRO_REQUIRE(d_u && d_alpha && d_target, "bad buffers");
+    PIRO_REQUIRE(scale > 0.0f, "scale must be positive");
+    double l = forward_backward(s, mkshape(n, D, H, W), d_u, d_alpha, d_target, scale);
+    if (h_grads) {
+        PIRO_CHECK_HIP(hipMemcpyAsync(h_grads, s->grads, N_PARAMS * sizeof(float), hipMemcpyDeviceToHost,
+                                      s->ctx->stream));
+        PIRO_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    }
+    if (loss) *loss = l;
+    PIRO_API_END
+}
+
+int piro_surrogate_train_step(piro_surrogate *s, int n, int D, int H, int W, const float *d_u, const float *d_alpha,
+                              const float *d_target, float scale, float lr, double *loss)
+{
+    PIRO_API_BEGIN
+    sur(s);
+    PIRO_REQUIRE(d_u && d_alpha && d_target, "bad buffers");
+    PIRO_REQUIRE(scale > 0.0f && lr > 0.0f, "scale and lr must be positive");
+    double l = forward_backward(s, mkshape(n, D, H, W), d_u, d_alpha, d_target, scale);
+    s->adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - (float)std::pow(b1, (double)s->adam_t), c2 = 1.0f - (float)std::pow(b2, (double)s->adam_t);
+    k_adam<<<grid_for(N_PARAMS, 256), 256, 0, s->ctx->stream>>>(N_PARAMS, s->params, s->grads, s->adam_m, s->adam_v,
+                                                                lr, b1, b2, 1e-8f, c1, c2);
+    check_launch();
+    repack(s);
+    if (loss) *loss = l;
+    PIRO_API_END
+}
+
+}  // extern "C"

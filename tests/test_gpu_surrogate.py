@@ -1,0 +1,204 @@
+"""GPU tests of the learned surrogate: forward and gradients of the project's
+own BF16-MFMA kernels against an FP64 CPU oracle of the same network (the
+oracle's conv3d is pinned by brute force and finite differences in
+tests/test_oracle_surrogate.py), plus a training run that must cut the loss."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as po
+
+pytestmark = pytest.mark.gpu
+
+NT, CH = 27, 32
+
+
+def bf16(a):
+    """round FP32 values to BF16 (nearest even) and back, as the device stores them"""
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def unpack(p):
+    o = 0
+    w1 = p[o:o + NT * CH * 2].reshape(NT, CH, 2); o += NT * CH * 2
+    b1 = p[o:o + CH]; o += CH
+    mids = []
+    for _ in range(4):
+        w = p[o:o + NT * CH * CH].reshape(NT, CH, CH); o += NT * CH * CH
+        b = p[o:o + CH]; o += CH
+        mids.append((w, b))
+    w6 = p[o:o + NT * CH].reshape(NT, 1, CH); o += NT * CH
+    b6 = p[o:o + 1]; o += 1
+    assert o == p.size
+    return w1, b1, mids, w6, b6
+
+
+def ref_forward(p, u, al, scale):
+    """FP64 network with the device's BF16 rounding points; returns inputs,
+    activations and delta (shape [n,D,H,W])"""
+    w1, b1, mids, w6, b6 = unpack(p.astype(np.float64))
+    in0 = (u.astype(np.float32) * np.float32(1.0 / scale)).astype(np.float64)
+    in1 = ((al.astype(np.float32) - np.float32(0.0125)) * np.float32(1.0 / 0.0075)).astype(np.float64)
+    x_in = np.stack([in0, in1], -1)
+    acts = []
+    z = po.conv3d_fwd_f64(x_in, w1, b1)
+    x = bf16(np.maximum(z, 0).astype(np.float32)).astype(np.float64)
+    acts.append(x)
+    for w, b in mids:
+        z = po.conv3d_fwd_f64(x, bf16(w.astype(np.float32)).astype(np.float64), b)
+        x = bf16(np.maximum(z, 0).astype(np.float32)).astype(np.float64)
+        acts.append(x)
+    delta = po.conv3d_fwd_f64(x, w6, b6)[..., 0]
+    return x_in, acts, delta
+
+
+def interior_mask(shape):
+    n, D, H, W = shape
+    m = np.zeros(shape, bool)
+    m[:, 1:-1, 1:-1, 1:-1] = True
+    return m
+
+
+def ref_grads(p, u, al, target, scale):
+    w1, b1, mids, w6, b6 = unpack(p.astype(np.float64))
+    x_in, acts, delta = ref_forward(p, u, al, scale)
+    m = interior_mask(u.shape)
+    n_int = m.sum()
+    dd = np.where(m, 2.0 * (delta - target) / n_int, 0.0)
+    loss = float(np.sum(np.where(m, (delta - target) ** 2, 0.0)) / n_int)
+    g = {}
+    dx, g["w6"], g["b6"] = po.conv3d_bwd_f64(acts[4], w6, dd[..., None])
+    dz = dx * (acts[4] > 0)
+    for l in range(3, -1, -1):
+        w, b = mids[l]
+        wb = bf16(w.astype(np.float32)).astype(np.float64)
+        dx, g[f"w{l + 2}"], g[f"b{l + 2}"] = po.conv3d_bwd_f64(acts[l], wb, dz)
+        dz = dx * (acts[l] > 0)
+    _, g["w1"], g["b1"] = po.conv3d_bwd_f64(x_in, w1, dz, need_dx=False)
+    flat = np.concatenate([g["w1"].ravel(), g["b1"]] +
+                          sum([[g[f"w{l}"].ravel(), g[f"b{l}"]] for l in range(2, 6)], []) +
+                          [g["w6"].ravel(), g["b6"]])
+    return flat, loss, delta
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from piro_amd.core import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def random_params(sur, seed):
+    rng = np.random.default_rng(seed)
+    p = sur.get_params()
+    # non-zero output layer so every gradient path is exercised
+    o = p.size - (NT * CH + 1)
+    p[o:] = rng.uniform(-0.05, 0.05, NT * CH + 1).astype(np.float32)
+    p[: NT * CH * 2 + CH] += rng.uniform(-0.05, 0.05, NT * CH * 2 + CH).astype(np.float32)
+    return p
+
+
+def fields(shape, seed):
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-1, 1, shape).astype(np.float32)
+    al = rng.uniform(0.005, 0.02, shape).astype(np.float32)
+    return u, al
+
+
+@pytest.mark.parametrize("shape", [(1, 8, 8, 8), (2, 5, 9, 10), (1, 12, 16, 20)])
+def test_forward_matches_oracle(ctx, shape):
+    print(f"case surrogate fwd {shape}", flush=True)
+    from piro_amd.core import Surrogate
+    sur = Surrogate(ctx, seed=3)
+    p = random_params(sur, 4)
+    sur.set_params(p)
+    u, al = fields(shape, 5)
+    scale = 1.7
+    du = torch.from_numpy(u.ravel()).to(ctx.device)
+    da = torch.from_numpy(al.ravel()).to(ctx.device)
+    out = sur.forward(shape, du, da, scale)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().reshape(shape)
+    _, _, delta = ref_forward(p, u, al, scale)
+    m = interior_mask(shape)
+    assert np.all(got[~m] == 0)
+    dg = (got[m].astype(np.float64) - u[m]) / scale
+    err = np.linalg.norm(dg - delta[m]) / np.linalg.norm(delta[m])
+    print(f"surrogate forward {shape}: rel L2 of delta vs FP64 oracle {err:.2e}")
+    assert err < 5e-3
+    sur.close()
+
+
+def test_untrained_surrogate_is_identity_on_interior(ctx):
+    from piro_amd.core import Surrogate
+    sur = Surrogate(ctx, seed=9)
+    shape = (1, 6, 7, 8)
+    u, al = fields(shape, 1)
+    out = sur.forward(shape, torch.from_numpy(u.ravel()).to(ctx.device), torch.from_numpy(al.ravel()).to(ctx.device),
+                      1.0)
+    got = out.cpu().numpy().reshape(shape)
+    m = interior_mask(shape)
+    np.testing.assert_array_equal(got[m], u[m])
+    sur.close()
+
+
+@pytest.mark.parametrize("shape", [(1, 8, 8, 8), (2, 6, 9, 11)])
+def test_gradients_match_oracle(ctx, shape):
+    print(f"case surrogate grad {shape}", flush=True)
+    from piro_amd.core import Surrogate
+    sur = Surrogate(ctx, seed=11)
+    p = random_params(sur, 12)
+    sur.set_params(p)
+    u, al = fields(shape, 13)
+    rng = np.random.default_rng(14)
+    target = rng.uniform(-0.1, 0.1, shape).astype(np.float32)
+    scale = 1.3
+    g, loss = sur.gradients(shape, torch.from_numpy(u.ravel()).to(ctx.device),
+                            torch.from_numpy(al.ravel()).to(ctx.device),
+                            torch.from_numpy(target.ravel()).to(ctx.device), scale)
+    gr, lr, _ = ref_grads(p, u, al, target.astype(np.float64), scale)
+    assert loss == pytest.approx(lr, rel=1e-2)
+    # per parameter block (BF16 activation gradients: a few 1e-3 per layer)
+    offs = [0, NT * CH * 2, NT * CH * 2 + CH]
+    for _ in range(4):
+        offs += [offs[-1] + NT * CH * CH, offs[-1] + NT * CH * CH + CH]
+    offs += [offs[-1] + NT * CH, offs[-1] + NT * CH + 1]
+    names = ["w1", "b1"] + sum([[f"w{l}", f"b{l}"] for l in range(2, 6)], []) + ["w6", "b6"]
+    for i, nm in enumerate(names):
+        a, b = g[offs[i]:offs[i + 1]].astype(np.float64), gr[offs[i]:offs[i + 1]]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"grad {nm}: rel L2 {rel:.2e}")
+        assert rel < 3e-2, nm
+    sur.close()
+
+
+def test_training_reduces_loss(ctx):
+    from piro_amd.core import Surrogate
+    sur = Surrogate(ctx, seed=21)
+    shape = (4, 12, 12, 12)
+    n, D, H, W = shape
+    rng = np.random.default_rng(22)
+    # targets: 4 FE steps of the 7-point operator per instance (r = 0.1)
+    us, ts, als = [], [], []
+    for i in range(n):
+        z, y, x = np.meshgrid(*[np.linspace(0, 1, s) for s in (D, H, W)], indexing="ij")
+        c = rng.uniform(0.3, 0.7, 3)
+        u = np.exp(-((z - c[0]) ** 2 + (y - c[1]) ** 2 + (x - c[2]) ** 2) / 0.05).astype(np.float32)
+        a = np.full((D, H, W), 0.0125, np.float32)
+        A = po.lap_csr(W, H, D, po.CD, po.lap_norm(1.0, W - 2))
+        A = po.row_scale(A, a.ravel())
+        dt = np.float32(0.1 * float(po.lap_norm(1.0, W - 2)) / 0.0125)
+        v = po.fe_csr_run((W, H, D), A, dt, 4, u.ravel(), None).reshape(D, H, W)
+        us.append(u); ts.append(v - u); als.append(a)
+    u = np.stack(us); t = np.stack(ts); a = np.stack(als)
+    du = torch.from_numpy(u.ravel()).to(ctx.device)
+    da = torch.from_numpy(a.ravel()).to(ctx.device)
+    dt_ = torch.from_numpy(t.ravel().astype(np.float32)).to(ctx.device)
+    losses = [sur.train_step(shape, du, da, dt_, 1.0, 3e-3) for _ in range(150)]
+    print(f"surrogate training: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    assert losses[-1] < 0.2 * losses[0]
+    sur.close()
