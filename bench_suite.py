@@ -1,0 +1,289 @@
+"""Benchmarks of the other workloads of the path (one JSON line each).
+
+  python bench_suite.py lu          # 4096^2 backward-Euler system: FP32-MFMA LU
+                                    #   + FP64 refinement vs FP64 LU (torch/rocSOLVER,
+                                    #   timed in a separate process)
+  python bench_suite.py batch       # 512 uneven 2-D instances, 200 steps, hipGraph
+  python bench_suite.py advection   # 256^3 upwind advection via geam3, 1000 steps
+  python bench_suite.py surrogate   # train on solver trajectories, 128^3 run vs plain
+  python bench_suite.py jli         # device JLI backward Euler
+
+bench.py remains the headline (256^3 heat) measurement.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from piro_amd.core import CD, UW, Batch, Context, DevCSR, Surrogate  # noqa: E402
+
+HBM = 8000.0
+MFMA_F32 = 157.3e12
+MFMA_BF16 = 2.5e15
+
+
+def timed(ctx, fn, reps=1):
+    """seconds per call, HIP events on the library's stream"""
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(reps):
+        fn()
+    return ctx.timer_stop() / reps / 1e3
+
+
+# ----------------------------------------------------------------------- LU
+
+def c3_rhs(n_side, nrhs, seed=2026):
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (n_side + 1)
+    xs = (np.arange(n_side) + 1) * h
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    out = np.zeros((n_side * n_side, nrhs))
+    for c in range(nrhs):
+        f = np.zeros_like(X)
+        for _ in range(8):
+            cx, cy = rng.uniform(0, 1, 2)
+            w = rng.uniform(0.05, 0.2)
+            f += np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * w * w))
+        out[:, c] = f.ravel()
+    return out
+
+
+FP64_SCRIPT = r"""
+import json, sys, numpy as np, torch
+a = torch.from_numpy(np.load(sys.argv[1])).cuda().double()
+b = torch.from_numpy(np.load(sys.argv[2])).cuda().double()
+def run():
+    LU, piv = torch.linalg.lu_factor(a)
+    return torch.linalg.lu_solve(LU, piv, b)
+run(); torch.cuda.synchronize()
+e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+e0.record()
+for _ in range(3):
+    x = run()
+e1.record(); torch.cuda.synchronize()
+rel = (torch.linalg.vector_norm(b - a @ x, dim=0) / torch.linalg.vector_norm(b, dim=0)).max().item()
+print(json.dumps({"seconds": e0.elapsed_time(e1) / 3 / 1e3, "rel": rel}))
+"""
+
+
+def bench_lu(args):
+    ctx = Context(0)
+    side, alpha, r = 64, 0.05, 2.0
+    h = 1.0 / (side + 1)
+    inv_h2 = np.float32(1.0 / (h * h))
+    L = ctx.laplacian2d_csr(side, inv_h2)
+    N = side * side
+    I = DevCSR(ctx.from_host(np.arange(N + 1, dtype=np.int32)), ctx.from_host(np.arange(N, dtype=np.int32)),
+               ctx.full(N, 1.0))
+    dt = np.float32(r * h * h / alpha)
+    A = ctx.geam2(1.0, I, np.float32(-(dt * np.float32(alpha))), L)
+    dense = ctx.csr_to_dense(A)
+    out = []
+    for nrhs in (1, 32):
+        bh = c3_rhs(side, nrhs)
+        b = ctx.from_host(bh.astype(np.float64))
+        lu = dense.clone()
+        ipiv = ctx.lu_factor(lu, N)  # warm-up
+        t_fac = timed(ctx, lambda: (lu.copy_(dense), ctx.lu_factor(lu, N)))
+        t_copy = timed(ctx, lambda: lu.copy_(dense))
+        t_fac -= t_copy
+        res = {}
+
+        def solve():
+            res["x"] = ctx.lu_solve_refine(dense, lu, N, ipiv, b, tol=1e-10, max_sweeps=10)
+
+        solve()
+        t_sol = timed(ctx, solve)
+        x, sweeps, rel = res["x"]
+        with tempfile.TemporaryDirectory() as td:
+            fa, fb = os.path.join(td, "a.npy"), os.path.join(td, "b.npy")
+            np.save(fa, dense.host().reshape(N, N))
+            np.save(fb, bh)
+            r64 = subprocess.run([sys.executable, "-c", FP64_SCRIPT, fa, fb], capture_output=True, text=True)
+        f64 = json.loads(r64.stdout.strip().splitlines()[-1]) if r64.returncode == 0 else {"seconds": None,
+                                                                                           "rel": None}
+        flop = 2.0 / 3.0 * N ** 3
+        tot = t_fac + t_sol
+        out.append({"metric": "backward-Euler LU time-to-1e-10 (4096^2)", "nrhs": nrhs,
+                    "mixed_factor_s": t_fac, "mixed_solve_refine_s": t_sol, "mixed_total_s": tot,
+                    "refinement_sweeps": sweeps, "mixed_max_rel_residual": max(rel),
+                    "fp64_torch_total_s": f64["seconds"], "fp64_max_rel_residual": f64["rel"],
+                    "speedup_vs_fp64": (f64["seconds"] / tot) if f64["seconds"] else None,
+                    "factor_tflops": flop / t_fac / 1e12, "factor_frac_of_fp32_mfma_peak": flop / t_fac / MFMA_F32})
+    for o in out:
+        print(json.dumps(o), flush=True)
+    ctx.close()
+
+
+# -------------------------------------------------------------------- batch
+
+def bench_batch(args):
+    ctx = Context(0)
+    rng = np.random.default_rng(4)
+    n_inst = 512
+    sides = np.exp(rng.uniform(np.log(32), np.log(256), n_inst)).astype(int)
+    alphas = np.exp(rng.uniform(np.log(1e-3), np.log(1e-1), n_inst)).astype(np.float32)
+    dims = [(int(s), int(s), 1) for s in sides]
+    a_parts, dts, x_parts = [], [], []
+    for (nx, ny, _), a in zip(dims, alphas):
+        a_parts.append(np.full(nx * ny, a, np.float32))
+        hh = np.float32(1.0) / np.float32(nx - 2)
+        dts.append(np.float32(np.float32(0.2) * np.float32(float(hh) ** 2) / a))
+        X, Y = np.meshgrid(np.linspace(0, 1, nx), np.linspace(0, 1, ny), indexing="xy")
+        f = np.zeros_like(X)
+        for _ in range(int(rng.integers(4, 17))):
+            c = rng.uniform(0.1, 0.9, 2)
+            w = rng.uniform(0.03, 0.2)
+            f += np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2) / (2 * w * w))
+        x_parts.append(f.astype(np.float32).ravel())
+    alpha_h = np.concatenate(a_parts)
+    x0h = np.concatenate(x_parts)
+    alpha = ctx.from_host(alpha_h)
+    b = Batch(ctx, dims, alpha, dts)
+    x = ctx.from_host(x0h)
+    w = ctx.empty_like(x)
+    rb, g, tune_ms = b.autotune(x.clone(), w)
+    steps = 200
+    x.upload(x0h)
+    b.run(x, w, steps, use_graph=True)  # capture + warm
+    x.upload(x0h)
+    t = timed(ctx, lambda: b.run(x, w, steps, use_graph=True))
+    byts = 8 * b.nnz + 12 * b.total_rows
+    got = x.host()
+    ok = True
+    for i in rng.choice(n_inst, 8, replace=False):
+        s, e = b.offsets[i], b.offsets[i + 1]
+        single = Batch(ctx, [dims[i]], ctx.from_host(alpha_h[s:e]), [dts[i]])
+        xs = ctx.from_host(x0h[s:e])
+        ws = ctx.empty_like(xs)
+        single.run(xs, ws, steps, use_graph=False)
+        ok = ok and (xs.host().tobytes() == got[s:e].tobytes())
+        single.close()
+    print(json.dumps({"metric": "batched 2-D heat, 512 uneven instances, 200 steps under hipGraph",
+                      "rows": b.total_rows, "nnz": b.nnz, "seconds": t, "us_per_step": t / steps * 1e6,
+                      "cell_steps_per_s": b.total_rows * steps / t, "hbm_gbs": byts * steps / t / 1e9,
+                      "roofline_frac": byts * steps / t / 1e9 / HBM, "tiling_rows_per_unit": rb,
+                      "tiling_grid": g, "bit_exact_vs_single_instance": ok}), flush=True)
+    b.close()
+    ctx.close()
+
+
+# ---------------------------------------------------------------- advection
+
+def bench_advection(args):
+    ctx = Context(0)
+    side = args.side
+    dims = (side, side, side)
+    N = side ** 3
+    g = ctx.grad_norm(1.0, side - 2)
+    G = [ctx.gradient_csr(side, side, side, UW, d, g) for d in range(3)]
+    # velocity in the reference's sign convention (dU/dt = +sum v_d G_d U with
+    # backward differences): v = -(1, 0.5, 0.25) is upwind advection
+    vel = (-1.0, -0.5, -0.25)
+    v3 = ctx.from_host(np.concatenate([np.full(N, v, np.float32) for v in vel]))
+    t0 = time.perf_counter()
+    D = ctx.geam3(G[0], G[1], G[2], v3)
+    ctx.sync()
+    t_geam = time.perf_counter() - t0
+    dx = 1.0 / (side - 2)
+    dt = np.float32(0.5 * 2 * dx / sum(abs(v) for v in vel))
+    x0 = ctx.ic_tophat(dims, [1.0] * 3, [0.5] * 3, 0.15)
+    snap = ctx.apply_face_values(dims, x0, [0.0] * 6)
+    ctx.autotune(dims, D, dt, x0, snap, ctx.empty_like(x0))
+    x = x0.clone()
+    w = ctx.empty_like(x)
+    steps = args.steps or 1000
+    t = timed(ctx, lambda: ctx.fe_run_csr(dims, D, dt, steps, x, w, snap))
+    mass0 = ctx.reduce("sum", x0)
+    mass1 = ctx.reduce("sum", x)
+    byts = 8 * D.nnz + 12 * N
+    print(json.dumps({"metric": f"advection {side}^3 upwind via geam3 concatenation, FE", "nnz": D.nnz,
+                      "geam_seconds": t_geam, "steps": steps, "us_per_step": t / steps * 1e6,
+                      "cell_steps_per_s": N * steps / t, "hbm_gbs": byts * steps / t / 1e9,
+                      "roofline_frac": byts * steps / t / 1e9 / HBM, "mass_ratio": mass1 / mass0,
+                      "max_after": ctx.reduce("absmax", x)}), flush=True)
+    ctx.close()
+
+
+# ---------------------------------------------------------------- surrogate
+
+def bench_surrogate(args):
+    from piro_amd import learned
+    ctx = Context(0)
+    sur = Surrogate(ctx, seed=5)
+    log = learned.train(ctx, sur, n_traj=args.traj, per_batch=50, side=args.train_side, segments=4, k=16,
+                        passes=args.passes, log_every=50)
+    run = learned.HeatRun(ctx, side=128)
+    steps = 1600
+    run.plain(160)  # warm-up
+    ref, t_plain = run.plain(steps)
+    run.accelerated(sur, 160)  # warm-up
+    acc, res = run.accelerated(sur, steps)
+    err = ctx.rel_l2(acc, ref)
+    shape = (1, 128, 128, 128)
+    y = ctx.empty_like(run.u0)
+    t_call = timed(ctx, lambda: sur.forward(shape, run.u0, run.alpha, 1.0, out=y), reps=10)
+    flop = 226368 * 128 ** 3
+    print(json.dumps({"metric": "surrogate-accelerated 128^3 heat, 1600 steps (k=16, check every 10th call)",
+                      "train_trajectories": log.trajectories, "train_seconds": log.seconds,
+                      "train_loss_first": log.losses[0], "train_loss_last": log.losses[-1],
+                      "plain_seconds": t_plain, "accelerated_seconds": res.seconds,
+                      "speedup": t_plain / res.seconds, "rel_l2_vs_plain": err,
+                      "checks": res.checks, "rejected": res.rejected,
+                      "check_errors": [float(f"{e:.3e}") for e in res.check_errors],
+                      "surrogate_call_ms": t_call * 1e3, "surrogate_tflops": flop / t_call / 1e12,
+                      "surrogate_frac_bf16_peak": flop / t_call / MFMA_BF16,
+                      "plain_16_steps_ms": t_plain / 100 * 1e3}), flush=True)
+    sur.close()
+    ctx.close()
+
+
+# ---------------------------------------------------------------------- JLI
+
+def bench_jli(args):
+    ctx = Context(0)
+    side = args.side
+    dims = (side, side, side)
+    norm = ctx.lap_norm(1.0, side - 2)
+    A = ctx.laplacian_csr(side, side, side, 2, norm)
+    dt = np.float32(0.05 * norm)
+    b = ctx.ic_gaussian(dims, [1.0] * 3, [0.5] * 3, [0.1] * 3, 1.0)
+    ctx.jli_solve(dims, A, dt, b, 1.0, 1e-12, 50)
+    out = {}
+
+    def run():
+        out["r"] = ctx.jli_solve(dims, A, dt, b, 1.0, 1e-12, 200)
+
+    t = timed(ctx, run)
+    x, it, res = out["r"]
+    print(json.dumps({"metric": f"device JLI backward Euler {side}^3", "iterations": it, "residual": res,
+                      "seconds": t, "us_per_iteration": t / max(it, 1) * 1e6}), flush=True)
+    ctx.close()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["lu", "batch", "advection", "surrogate", "jli"])
+    ap.add_argument("--side", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=0)
+    ap.add_argument("--traj", type=int, default=2000)
+    ap.add_argument("--train-side", type=int, default=32)
+    ap.add_argument("--passes", type=int, default=3)
+    args = ap.parse_args()
+    {"lu": bench_lu, "batch": bench_batch, "advection": bench_advection, "surrogate": bench_surrogate,
+     "jli": bench_jli}[args.what](args)
+
+
+if __name__ == "__main__":
+    main()

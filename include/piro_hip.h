@@ -59,6 +59,12 @@ void *piro_ctx_get_stream(piro_ctx *ctx);
 int piro_ctx_synchronize(piro_ctx *ctx);
 const char *piro_last_error(void);
 int piro_version(void);
+/* number of visible HIP devices (0 when there is no GPU; never fails) */
+int piro_device_count(int *count);
+/* HIP-event timing on the context's stream: start, then stop waits for the
+ * stop event and returns the elapsed milliseconds */
+int piro_timer_start(piro_ctx *ctx);
+int piro_timer_stop(piro_ctx *ctx, float *ms);
 
 /* Replaces clCreateBuffer / clEnqueueRead/WriteBuffer (copyCL,
  * include/openclutilities.hpp:17-65). */
@@ -101,6 +107,14 @@ int piro_assemble_gradient_csr(piro_ctx *ctx, int nx, int ny, int nz,
                                int scheme, int dir, float norm,
                                int32_t *d_row_ptr, int32_t *d_col,
                                float *d_val);
+
+/* Build-only (PIRO is 3-D): 5-point Laplacian on an n x n grid of unknowns
+ * with zero Dirichlet values eliminated (the backward-Euler system's L):
+ * rows sorted by column [i-n, i-1, i, i+1, i+n], values inv_h2 and
+ * -4*inv_h2; nnz = 5n^2 - 4n. */
+int piro_assemble_laplacian2d_csr(piro_ctx *ctx, int n, float inv_h2,
+                                  int32_t *d_row_ptr, int32_t *d_col,
+                                  float *d_val);
 
 /* The intended semantics of kernelmethods::csrscale / SpMDSM
  * (src/kernelcsrgeam.cpp:208-235, SpMDSM.cl:20-31), which the reference
@@ -184,6 +198,10 @@ int piro_alpha_uniform(piro_ctx *ctx, int64_t n, uint64_t seed, float lo,
 int piro_apply_face_values(piro_ctx *ctx, int nx, int ny, int nz,
                            const float *d_field, const float *h_face_vals6,
                            float *d_snap);
+/* Replaces bc::opencl_setBC + setBC.cl (src/bc.cpp:38-65, setBC.cl:1-9):
+ * d_field[f] = d_snap[f] for every face cell f (edges excluded). */
+int piro_set_faces(piro_ctx *ctx, int nx, int ny, int nz, const float *d_snap,
+                   float *d_field);
 
 /* ---------------------------------------------------------------- stepping */
 
@@ -240,6 +258,14 @@ int piro_reduce_dot(piro_ctx *ctx, int64_t n, const float *d_x,
                     const float *d_y, double *h_out);
 int piro_reduce_absmax(piro_ctx *ctx, int64_t n, const float *d_x,
                        double *h_out);
+/* d_out = fl(fl(a*x) + fl(b*y)) elementwise; a = 1, b = +-1 are the
+ * reference's addVectors / subtractVectors (addVectors.cl:1-9,
+ * subtractVectors.cl:1-9).  d_out may alias d_x or d_y. */
+int piro_vec_axpby(piro_ctx *ctx, int64_t n, float a, const float *d_x,
+                   float b, const float *d_y, float *d_out);
+/* sum (x_i - y_i)^2 in FP64 (relative-L2 checks without a read-back) */
+int piro_reduce_diff_sumsq(piro_ctx *ctx, int64_t n, const float *d_x,
+                           const float *d_y, double *h_out);
 
 /* ------------------------------------------------------------- JLI solve */
 
@@ -363,6 +389,21 @@ int piro_surrogate_train_step(piro_surrogate *s, int n, int D, int H, int W,
                               const float *d_u, const float *d_alpha,
                               const float *d_target, float scale, float lr,
                               double *loss);
+
+/* ------------------------------------------------------------ case driver */
+
+/* Replaces `PIRO <case.ini> 0` (src/main.cpp:11-32 -> Piro::preprocess,
+ * src/preprocess.cpp:20-208, and Piro::solve, src/solve.cpp:11-48): reads
+ * <assets_dir>/<case_name> plus setup.ini (BCs), solver.ini (JLI),
+ * IC/distribution.ini and Data/hashtable.ini from assets_dir, runs the time
+ * loop on the device and, when out_dir is given, writes Paraview files to
+ * <out_dir>/<casename>/mesh/ every Save steps (nothing is ever deleted, unlike
+ * restart=1, src/init.cpp:14-16).  max_steps >= 0 caps the steps;
+ * alpha_row_scaled = 0 reproduces the shipped CSR operator without the
+ * diffusivity.  h_field (optional, field_len >= N) receives field slot 1. */
+int piro_case_run(const char *assets_dir, const char *case_name,
+                  const char *out_dir, int max_steps, int alpha_row_scaled,
+                  int *steps_done, float *h_field, int64_t field_len);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,166 @@
+// piro_solver.hpp -- C++ drop-in for PIRO's solver API on the MI355X path.
+//
+// Mirrors the reference's operator layer so that its time loop
+// (src/solve.cpp:18-46) reads the same:
+//
+//     piro::process solver(case_);
+//     piro::scalarMatrix UEqn(solver.ddt_r("T") = solver.laplacian("T", "Alpha"));
+//     UEqn.Solve(time);
+//
+//     piro::scalarMatrix VEqn(solver.ddt_r("U") = solver.vector("v") & solver.gradient("U"));
+//
+// Reference interfaces replaced (file:line, PIRO tree):
+//   Piro::preprocess                 src/preprocess.cpp:20-208
+//   Piro::solve                      src/solve.cpp:11-48
+//   process::ddt_r/laplacian/gradient/vector   src/process.cpp:52-166,
+//                                    include/process.hpp:47-81
+//   CLBuffer::operator= / operator&  src/operatoroverload.cpp:20-123,
+//                                    include/operatoroverload.hpp:15,19
+//   scalarMatrix::Solve              src/process.cpp:217-249
+//   paraview::writevth/writevti      src/paraview.cpp:12-137
+//
+// Device work goes through the C ABI of include/piro_hip.h.  Errors are
+// reported as piro::SolverError exceptions inside C++ and as status codes
+// through piro_case_run (piro_hip.h).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "piro_hip.h"
+
+namespace piro {
+
+struct SolverError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// [section] key = value reader with the reference's parsing rules
+// (src/fileutilities.cpp:124-175): trims spaces, skips ';'/'#' comments.
+class IniReader {
+  public:
+    explicit IniReader(const std::string &path);
+    std::string get(const std::string &section, const std::string &key,
+                    const std::string &def = "default_value") const;
+    bool ok() const { return ok_; }
+
+  private:
+    std::map<std::string, std::map<std::string, std::string>> data_;
+    bool ok_ = false;
+};
+
+class Case;
+
+// A device field slot (the reference's CLBuffer with its `ind`).
+struct Field {
+    Case *c = nullptr;
+    int slot = -1;
+    float *d = nullptr;   // device data (N or 3N floats)
+    // CLBuffer::operator=: advance one time step of `op` from this field and
+    // return the new state (not yet committed; scalarMatrix::Solve commits)
+    Field operator=(const struct Operator &op);
+};
+
+// An assembled operator (CSR on the device; the HT/COO formats are kept in
+// their own layout and unpacked to the CSR the step kernels read).
+struct Operator {
+    Case *c = nullptr;
+    int format = 0;              // 0 CSR, 2 HT, 3 COO
+    int32_t *rp = nullptr, *col = nullptr;
+    float *val = nullptr;
+    int64_t nnz = 0;
+    int32_t *ht_keys = nullptr;  // HT only
+    float *ht_vals = nullptr;
+    int ht_size = 0;
+};
+
+struct VectorRef {
+    Case *c = nullptr;
+    int slot = -1;
+};
+
+Operator operator&(const VectorRef &v, const Operator &grad);
+
+class process {
+  public:
+    explicit process(Case &c) : c_(&c) {}
+    Field ddt_r(const std::string &var);
+    Operator laplacian(const std::string &var1, const std::string &var2);
+    Operator gradient(const std::string &var);
+    VectorRef vector(const std::string &var);
+
+  private:
+    Case *c_;
+};
+
+class scalarMatrix {
+  public:
+    explicit scalarMatrix(Field f) : f_(f) {}
+    void Solve(float currenttime);
+
+  private:
+    Field f_;
+};
+
+struct CaseOptions {
+    std::string out_dir;        // where <casename>/mesh/... is written ("" = none)
+    bool alpha_row_scaled = true;  // false: reproduce the shipped (alpha-free) operator
+    int max_steps = -1;         // cap on time steps (-1: until TotalTime)
+};
+
+// The case: parsed .ini files, device fields, lazily assembled operators.
+class Case {
+  public:
+    Case(const std::string &assets_dir, const std::string &case_name, const CaseOptions &opt);
+    ~Case();
+    Case(const Case &) = delete;
+    Case &operator=(const Case &) = delete;
+
+    // time loop of src/solve.cpp:18-46; returns the number of steps taken
+    int solve();
+
+    // accessors used by the operator layer
+    piro_ctx *ctx() const { return ctx_; }
+    int n[3] = {0, 0, 0};       // padded dims
+    int64_t N = 0;
+    float l[3] = {0, 0, 0};
+    float timestep = 0, totaltime = 0;
+    int save = 1, simindex = 0, time_scheme = 11, space_scheme = 3, solver_scheme = 0, data_type = 0;
+    std::string casename;
+    std::vector<std::string> names;   // slot names (constants, scalars, vectors)
+    std::vector<int> types;           // 0 constant, 1 scalar, 2 vector
+    std::vector<float *> fields;      // device data per slot
+    float *snapshot = nullptr;        // BC snapshot (field slot 1 with face values)
+    float *work = nullptr;            // step output buffer
+    float *work2 = nullptr;
+    bool has_bc = false;
+    float jli_tol = 1e-5f, jli_urf = 1.0f;
+    int jli_maxiter = 1000;
+    float ht_a = 0, ht_b = 0, ht_c = 0, ht_lf = 0.7f;
+    int ts = 0;
+    CaseOptions opt;
+    int match(const std::string &var) const;  // matchscalartovar: 0 if absent
+    Operator lap, grad_x, grad_y, grad_z, rhs;
+    bool lap_init = false, grad_init = false, dot_init = false, lu_init = false;
+    float *lu = nullptr, *lu_a = nullptr;
+    int32_t *lu_piv = nullptr;
+    void export_paraview(int step);
+    std::vector<float> download(int slot) const;
+    void *alloc(size_t bytes);  // device memory owned by the case
+
+  private:
+    piro_ctx *ctx_ = nullptr;
+    std::vector<void *> owned_;
+    void init_fields(const IniReader &reader, const std::string &assets_dir);
+    void init_bc(const std::string &assets_dir);
+    friend struct Field;
+};
+
+// paraview text of one level (src/paraview.cpp:12-103): interior cells only
+std::string writevti(const Case &c, const std::vector<std::vector<float>> &host_fields);
+std::string writevth(const Case &c, int step);
+
+}  // namespace piro

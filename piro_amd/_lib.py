@@ -40,6 +40,9 @@ _SIGS = {
     "piro_ctx_destroy": [vp],
     "piro_ctx_set_stream": [vp, vp],
     "piro_ctx_synchronize": [vp],
+    "piro_device_count": [P_i32],
+    "piro_timer_start": [vp],
+    "piro_timer_stop": [vp, P_f32],
     "piro_malloc": [vp, C.c_size_t, C.POINTER(vp)],
     "piro_free": [vp, vp],
     "piro_memcpy_h2d": [vp, vp, vp, C.c_size_t],
@@ -52,6 +55,7 @@ _SIGS = {
     "piro_grad_nnz": [i32, i32, i32, i32, P_i64],
     "piro_assemble_gradient_csr": [vp, i32, i32, i32, i32, i32, f32, vp, vp, vp],
     "piro_row_scale": [vp, i32, vp, vp, vp],
+    "piro_assemble_laplacian2d_csr": [vp, i32, f32, vp, vp, vp],
     "piro_ht_table_size": [i32, i32, f32, f32, f32, f32, P_i32],
     "piro_ht_build": [vp, i32, vp, vp, vp, i32, vp, vp],
     "piro_ht_count": [vp, i32, vp, i32, P_i64],
@@ -65,6 +69,7 @@ _SIGS = {
     "piro_ic_tophat": [vp, i32, i32, i32, P_f32, P_f32, f32, vp],
     "piro_alpha_uniform": [vp, i64, u64, f32, f32, vp],
     "piro_apply_face_values": [vp, i32, i32, i32, vp, P_f32, vp],
+    "piro_set_faces": [vp, i32, i32, i32, vp, vp],
     "piro_fe_step_csr": [vp, i32, i32, i32, vp, vp, vp, i64, f32, vp, vp, vp],
     "piro_fe_run_csr": [vp, i32, i32, i32, vp, vp, vp, i64, f32, i32, vp, vp, vp],
     "piro_fe_step_allrows": [vp, i32, vp, vp, vp, i64, f32, vp, vp],
@@ -76,6 +81,8 @@ _SIGS = {
     "piro_reduce_sumsq": [vp, i64, vp, P_f64],
     "piro_reduce_dot": [vp, i64, vp, vp, P_f64],
     "piro_reduce_absmax": [vp, i64, vp, P_f64],
+    "piro_reduce_diff_sumsq": [vp, i64, vp, vp, P_f64],
+    "piro_vec_axpby": [vp, i64, f32, vp, f32, vp, vp],
     "piro_csr_to_dense": [vp, i32, i32, vp, vp, vp, vp, i32],
     "piro_lu_factor_f32": [vp, i32, vp, i32, vp, P_i32],
     "piro_lu_solve_f32": [vp, i32, vp, i32, vp, i32, vp, i32],
@@ -90,6 +97,7 @@ _SIGS = {
     "piro_batch_autotune": [vp, vp, vp, P_i32, P_i32, P_f32],
     "piro_batch_step": [vp, vp, vp],
     "piro_batch_run": [vp, vp, vp, i32, i32],
+    "piro_case_run": [C.c_char_p, C.c_char_p, C.c_char_p, i32, i32, P_i32, P_f32, i64],
     "piro_surrogate_create": [vp, u64, C.POINTER(vp)],
     "piro_surrogate_destroy": [vp],
     "piro_surrogate_get_params": [vp, P_f32],

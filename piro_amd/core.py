@@ -1,18 +1,99 @@
-"""Thin Python front-end over the C ABI, with torch tensors as device memory.
+"""Thin Python front-end over the C ABI (include/piro_hip.h).
 
-torch is used only for allocation, copies and streams; every computation is a
-call into libpiro_hip.so.
+Device memory is owned by the library's own allocator (DevArray); host data
+are numpy arrays.  Nothing here computes: every operation is a call into
+libpiro_hip.so.  (torch is deliberately not used on the device: its wheel
+bundles a second HIP runtime, and two runtimes in one process do not share
+devices.)
 """
 from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass
 
-import torch
+import numpy as np
 
 from ._lib import check, lib
 
 UW, DW, CD = 1, 2, 3
+
+
+def device_count() -> int:
+    n = C.c_int()
+    check(lib.piro_device_count(C.byref(n)))
+    return n.value
+
+
+class DevArray:
+    """A 1-D device array allocated by the library (piro_malloc)."""
+
+    def __init__(self, ctx: "Context", n: int, dtype=np.float32, _ptr=None, _base=None):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self._base = _base
+        if _ptr is not None:
+            self.ptr = _ptr
+            self._own = False
+        else:
+            p = C.c_void_p()
+            check(lib.piro_malloc(ctx.h, max(self.nbytes, 1), C.byref(p)))
+            self.ptr = p.value
+            self._own = True
+
+    @property
+    def nbytes(self) -> int:
+        return self.n * self.dtype.itemsize
+
+    def numel(self) -> int:
+        return self.n
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def free(self):
+        if getattr(self, "_own", False) and self.ptr:
+            try:
+                if self.ctx.h:
+                    lib.piro_free(self.ctx.h, C.c_void_p(self.ptr))
+            except Exception:
+                pass
+            self.ptr = None
+            self._own = False
+
+    def __del__(self):
+        self.free()
+
+    def host(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        if self.n:
+            check(lib.piro_memcpy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes))
+        return out
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype).ravel()
+        assert a.size == self.n, (a.size, self.n)
+        if self.n:
+            check(lib.piro_memcpy_h2d(self.ctx.h, C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), self.nbytes))
+        return self
+
+    def copy_(self, other: "DevArray"):
+        assert other.nbytes == self.nbytes
+        if self.n:
+            check(lib.piro_memcpy_d2d(self.ctx.h, C.c_void_p(self.ptr), C.c_void_p(other.ptr), self.nbytes))
+        return self
+
+    def clone(self) -> "DevArray":
+        return DevArray(self.ctx, self.n, self.dtype).copy_(self)
+
+    def view(self, start: int, stop: int) -> "DevArray":
+        """a non-owning window [start, stop) (keeps self alive)"""
+        return DevArray(self.ctx, stop - start, self.dtype, _ptr=self.ptr + start * self.dtype.itemsize, _base=self)
+
+    def zero_(self):
+        if self.n:
+            check(lib.piro_memset(self.ctx.h, C.c_void_p(self.ptr), 0, self.nbytes))
+        return self
 
 
 def _p(t):
@@ -25,9 +106,9 @@ def _f3(v):
 
 @dataclass
 class DevCSR:
-    row_ptr: torch.Tensor
-    col: torch.Tensor
-    val: torch.Tensor
+    row_ptr: DevArray
+    col: DevArray
+    val: DevArray
 
     @property
     def n_rows(self) -> int:
@@ -38,19 +119,16 @@ class DevCSR:
         return self.col.numel()
 
     def host(self):
-        return (self.row_ptr.cpu().numpy(), self.col.cpu().numpy(), self.val.cpu().numpy())
+        return (self.row_ptr.host(), self.col.host(), self.val.host())
 
 
 class Context:
-    """One device, one in-order stream (torch's current stream by default)."""
+    """One device, one in-order HIP stream owned by the library."""
 
-    def __init__(self, device: int = 0, use_torch_stream: bool = True):
-        self.device = torch.device("cuda", device)
+    def __init__(self, device: int = 0):
         h = C.c_void_p()
         check(lib.piro_ctx_create(device, C.byref(h)))
         self.h = h
-        if use_torch_stream:
-            check(lib.piro_ctx_set_stream(self.h, C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)))
 
     def close(self):
         if self.h:
@@ -66,9 +144,29 @@ class Context:
     def sync(self):
         check(lib.piro_ctx_synchronize(self.h))
 
-    # ------------------------------------------------------------ helpers
-    def empty(self, n, dtype=torch.float32):
-        return torch.empty(int(n), dtype=dtype, device=self.device)
+    # ------------------------------------------------------------ memory
+    def empty(self, n, dtype=np.float32) -> DevArray:
+        return DevArray(self, n, dtype)
+
+    def empty_like(self, a: DevArray) -> DevArray:
+        return DevArray(self, a.n, a.dtype)
+
+    def from_host(self, a, dtype=None) -> DevArray:
+        a = np.ascontiguousarray(a)
+        dt = a.dtype if dtype is None else np.dtype(dtype)
+        return DevArray(self, a.size, dt).upload(a.astype(dt, copy=False).ravel())
+
+    def full(self, n, value, dtype=np.float32) -> DevArray:
+        return self.from_host(np.full(int(n), value, dtype))
+
+    # ------------------------------------------------------------ timing
+    def timer_start(self):
+        check(lib.piro_timer_start(self.h))
+
+    def timer_stop(self) -> float:
+        ms = C.c_float()
+        check(lib.piro_timer_stop(self.h, C.byref(ms)))
+        return ms.value
 
     # ----------------------------------------------------------- assembly
     @staticmethod
@@ -88,14 +186,21 @@ class Context:
     def laplacian_csr(self, nx, ny, nz, scheme, norm) -> DevCSR:
         N = nx * ny * nz
         nnz = self.lap_nnz(nx, ny, nz)
-        A = DevCSR(self.empty(N + 1, torch.int32), self.empty(nnz, torch.int32), self.empty(nnz))
+        A = DevCSR(self.empty(N + 1, np.int32), self.empty(nnz, np.int32), self.empty(nnz))
         check(lib.piro_assemble_laplacian_csr(self.h, nx, ny, nz, scheme, float(norm), _p(A.row_ptr), _p(A.col),
                                               _p(A.val)))
         return A
 
+    def laplacian2d_csr(self, n, inv_h2) -> DevCSR:
+        N = n * n
+        nnz = 5 * N - 4 * n
+        A = DevCSR(self.empty(N + 1, np.int32), self.empty(nnz, np.int32), self.empty(nnz))
+        check(lib.piro_assemble_laplacian2d_csr(self.h, n, float(inv_h2), _p(A.row_ptr), _p(A.col), _p(A.val)))
+        return A
+
     def laplacian_coo(self, nx, ny, nz, scheme, norm):
         nnz = self.lap_nnz(nx, ny, nz)
-        rows, col, val = self.empty(nnz, torch.int32), self.empty(nnz, torch.int32), self.empty(nnz)
+        rows, col, val = self.empty(nnz, np.int32), self.empty(nnz, np.int32), self.empty(nnz)
         check(lib.piro_assemble_laplacian_coo(self.h, nx, ny, nz, scheme, float(norm), _p(rows), _p(col), _p(val)))
         return rows, col, val
 
@@ -103,12 +208,15 @@ class Context:
         N = nx * ny * nz
         v = C.c_int64()
         check(lib.piro_grad_nnz(nx, ny, nz, d, C.byref(v)))
-        G = DevCSR(self.empty(N + 1, torch.int32), self.empty(v.value, torch.int32), self.empty(v.value))
+        G = DevCSR(self.empty(N + 1, np.int32), self.empty(v.value, np.int32), self.empty(v.value))
         check(lib.piro_assemble_gradient_csr(self.h, nx, ny, nz, scheme, d, float(norm), _p(G.row_ptr), _p(G.col),
                                              _p(G.val)))
         return G
 
-    def row_scale(self, A: DevCSR, diag):
+    def csr_from_host(self, row_ptr, col, val) -> DevCSR:
+        return DevCSR(self.from_host(row_ptr, np.int32), self.from_host(col, np.int32), self.from_host(val, np.float32))
+
+    def row_scale(self, A: DevCSR, diag: DevArray):
         check(lib.piro_row_scale(self.h, A.n_rows, _p(A.row_ptr), _p(A.val), _p(diag)))
 
     @staticmethod
@@ -118,7 +226,7 @@ class Context:
         return v.value
 
     def ht_build(self, A: DevCSR, ts):
-        keys, vals = self.empty(ts, torch.int32), self.empty(ts)
+        keys, vals = self.empty(ts, np.int32), self.empty(ts)
         check(lib.piro_ht_build(self.h, A.n_rows, _p(A.row_ptr), _p(A.col), _p(A.val), int(ts), _p(keys), _p(vals)))
         return keys, vals
 
@@ -126,24 +234,24 @@ class Context:
         v = C.c_int64()
         ts = keys.numel()
         check(lib.piro_ht_count(self.h, N, _p(keys), ts, C.byref(v)))
-        A = DevCSR(self.empty(N + 1, torch.int32), self.empty(max(v.value, 1), torch.int32),
-                   self.empty(max(v.value, 1)))
+        m = max(v.value, 1)
+        A = DevCSR(self.empty(N + 1, np.int32), self.empty(m, np.int32), self.empty(m))
         check(lib.piro_ht_to_csr(self.h, N, _p(keys), _p(vals), ts, _p(A.row_ptr), _p(A.col), _p(A.val)))
-        A.col, A.val = A.col[:v.value], A.val[:v.value]
+        A.col, A.val = A.col.view(0, v.value), A.val.view(0, v.value)
         return A
 
     def coo_to_csr(self, N, rows, col, val) -> DevCSR:
-        rp = self.empty(N + 1, torch.int32)
+        rp = self.empty(N + 1, np.int32)
         check(lib.piro_coo_row_ptr(self.h, N, rows.numel(), _p(rows), _p(rp)))
         return DevCSR(rp, col, val)
 
     def geam3(self, Ga: DevCSR, Gb: DevCSR, Gc: DevCSR, v3) -> DevCSR:
         N = Ga.n_rows
-        rp = self.empty(N + 1, torch.int32)
+        rp = self.empty(N + 1, np.int32)
         v = C.c_int64()
         check(lib.piro_geam3_rowptr(self.h, N, _p(Ga.row_ptr), _p(Ga.col), _p(Gb.row_ptr), _p(Gb.col),
                                     _p(Gc.row_ptr), _p(Gc.col), _p(rp), C.byref(v)))
-        D = DevCSR(rp, self.empty(v.value, torch.int32), self.empty(v.value))
+        D = DevCSR(rp, self.empty(v.value, np.int32), self.empty(v.value))
         check(lib.piro_geam3_fill(self.h, N, _p(Ga.row_ptr), _p(Ga.col), _p(Ga.val), _p(Gb.row_ptr), _p(Gb.col),
                                   _p(Gb.val), _p(Gc.row_ptr), _p(Gc.col), _p(Gc.val), _p(v3), _p(D.row_ptr),
                                   _p(D.col), _p(D.val)))
@@ -151,11 +259,11 @@ class Context:
 
     def geam2(self, alpha, A: DevCSR, beta, B: DevCSR) -> DevCSR:
         N = A.n_rows
-        rp = self.empty(N + 1, torch.int32)
+        rp = self.empty(N + 1, np.int32)
         v = C.c_int64()
         check(lib.piro_geam2_rowptr(self.h, N, _p(A.row_ptr), _p(A.col), _p(B.row_ptr), _p(B.col), _p(rp),
                                     C.byref(v)))
-        D = DevCSR(rp, self.empty(v.value, torch.int32), self.empty(v.value))
+        D = DevCSR(rp, self.empty(v.value, np.int32), self.empty(v.value))
         check(lib.piro_geam2_fill(self.h, N, float(alpha), _p(A.row_ptr), _p(A.col), _p(A.val), float(beta),
                                   _p(B.row_ptr), _p(B.col), _p(B.val), _p(D.row_ptr), _p(D.col), _p(D.val)))
         return D
@@ -184,6 +292,10 @@ class Context:
         check(lib.piro_apply_face_values(self.h, nx, ny, nz, _p(field), (C.c_float * 6)(*map(float, vals6)),
                                          _p(snap)))
         return snap
+
+    def set_faces(self, dims, snap, field):
+        nx, ny, nz = dims
+        check(lib.piro_set_faces(self.h, nx, ny, nz, _p(snap), _p(field)))
 
     # ----------------------------------------------------------- stepping
     def fe_step_csr(self, dims, A: DevCSR, dt, x, snap=None, y=None):
@@ -238,41 +350,50 @@ class Context:
             check(lib.piro_reduce_dot(self.h, n, _p(x), _p(y), C.byref(out)))
         elif op == "absmax":
             check(lib.piro_reduce_absmax(self.h, n, _p(x), C.byref(out)))
+        elif op == "diffsq":
+            check(lib.piro_reduce_diff_sumsq(self.h, n, _p(x), _p(y), C.byref(out)))
         else:
             raise ValueError(op)
         return out.value
 
+    def axpby(self, a, x: DevArray, b, y: DevArray, out: DevArray = None) -> DevArray:
+        out = self.empty_like(x) if out is None else out
+        check(lib.piro_vec_axpby(self.h, x.numel(), float(a), _p(x), float(b), _p(y), _p(out)))
+        return out
+
+    def rel_l2(self, a: DevArray, b: DevArray) -> float:
+        """||a - b|| / ||b|| on the device (FP64 sums)"""
+        d = self.reduce("diffsq", a, b)
+        nb = self.reduce("sumsq", b)
+        return (d / nb) ** 0.5 if nb > 0 else d ** 0.5
+
     # ----------------------------------------------------------------- LU
     def csr_to_dense(self, A: DevCSR, n_cols=None):
         n_cols = A.n_rows if n_cols is None else n_cols
-        d = torch.empty((A.n_rows, n_cols), dtype=torch.float32, device=self.device)
+        d = self.empty(A.n_rows * n_cols)
         check(lib.piro_csr_to_dense(self.h, A.n_rows, n_cols, _p(A.row_ptr), _p(A.col), _p(A.val), _p(d), n_cols))
         return d
 
-    def lu_factor(self, a):
-        """In-place FP32 LU with partial pivoting of a row-major square tensor."""
-        n = a.shape[0]
-        ipiv = torch.empty(n, dtype=torch.int32, device=self.device)
+    def lu_factor(self, a: DevArray, n: int):
+        """In-place FP32 LU with partial pivoting of a row-major n x n array."""
+        ipiv = self.empty(n, np.int32)
         info = C.c_int()
-        check(lib.piro_lu_factor_f32(self.h, n, _p(a), a.stride(0), _p(ipiv), C.byref(info)))
+        check(lib.piro_lu_factor_f32(self.h, n, _p(a), n, _p(ipiv), C.byref(info)))
         return ipiv
 
-    def lu_solve_f32(self, lu, ipiv, b):
-        n = lu.shape[0]
-        b2 = b.reshape(n, -1)
-        check(lib.piro_lu_solve_f32(self.h, n, _p(lu), lu.stride(0), _p(ipiv), b2.shape[1], _p(b2), b2.stride(0)))
+    def lu_solve_f32(self, lu: DevArray, n: int, ipiv, b: DevArray):
+        nrhs = b.numel() // n
+        check(lib.piro_lu_solve_f32(self.h, n, _p(lu), n, _p(ipiv), nrhs, _p(b), nrhs))
         return b
 
-    def lu_solve_refine(self, a, lu, ipiv, b64, tol=1e-10, max_sweeps=10):
-        n = a.shape[0]
-        b2 = b64.reshape(n, -1).contiguous()
-        nrhs = b2.shape[1]
-        x = torch.empty_like(b2)
+    def lu_solve_refine(self, a, lu, n, ipiv, b64: DevArray, tol=1e-10, max_sweeps=10):
+        nrhs = b64.numel() // n
+        x = self.empty(n * nrhs, np.float64)
         sweeps = C.c_int()
         rel = (C.c_double * nrhs)()
-        check(lib.piro_lu_solve_refine(self.h, n, _p(a), a.stride(0), _p(lu), lu.stride(0), _p(ipiv), nrhs, _p(b2),
-                                       nrhs, _p(x), nrhs, float(tol), int(max_sweeps), C.byref(sweeps), rel))
-        return x.reshape(b64.shape), sweeps.value, list(rel)
+        check(lib.piro_lu_solve_refine(self.h, n, _p(a), n, _p(lu), n, _p(ipiv), nrhs, _p(b64), nrhs, _p(x), nrhs,
+                                       float(tol), int(max_sweeps), C.byref(sweeps), rel))
+        return x, sweeps.value, list(rel)
 
     # ------------------------------------------------------------- batch
     def batch(self, dims_list, alpha, dts):
@@ -291,7 +412,7 @@ class Context:
 class Batch:
     """Many independent heat instances stepped by one launch per step."""
 
-    def __init__(self, ctx: Context, dims_list, alpha, dts):
+    def __init__(self, ctx: Context, dims_list, alpha: DevArray, dts):
         self.ctx = ctx
         n = len(dims_list)
         flat = (C.c_int * (3 * n))(*[int(v) for d in dims_list for v in d])
@@ -315,11 +436,6 @@ class Batch:
             self.close()
         except Exception:
             pass
-
-    def operator(self):
-        rp, col, val = C.c_void_p(), C.c_void_p(), C.c_void_p()
-        check(lib.piro_batch_operator(self.h, C.byref(rp), C.byref(col), C.byref(val)))
-        return rp.value, col.value, val.value
 
     def set_tiling(self, rows_per_unit, grid=0):
         check(lib.piro_batch_set_tiling(self.h, int(rows_per_unit), int(grid)))
@@ -364,25 +480,22 @@ class Surrogate:
             pass
 
     def get_params(self):
-        import numpy as np
         p = np.empty(self.n_params, np.float32)
         check(lib.piro_surrogate_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
         return p
 
     def set_params(self, p):
-        import numpy as np
         p = np.ascontiguousarray(p, np.float32)
         assert p.size == self.n_params
         check(lib.piro_surrogate_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
 
     def forward(self, shape, u, alpha, scale, out=None):
         n, D, H, W = shape
-        out = torch.empty_like(u) if out is None else out
+        out = self.ctx.empty_like(u) if out is None else out
         check(lib.piro_surrogate_forward(self.h, n, D, H, W, _p(u), _p(alpha), float(scale), _p(out)))
         return out
 
     def gradients(self, shape, u, alpha, target, scale):
-        import numpy as np
         n, D, H, W = shape
         g = np.empty(self.n_params, np.float32)
         loss = C.c_double()
@@ -396,3 +509,16 @@ class Surrogate:
         check(lib.piro_surrogate_train_step(self.h, n, D, H, W, _p(u), _p(alpha), _p(target), float(scale),
                                             float(lr), C.byref(loss)))
         return loss.value
+
+
+def case_run(assets_dir, case_name, out_dir=None, max_steps=-1, alpha_row_scaled=True, n_cells=None):
+    """Run a reference-format case (piro_case_run); returns (steps, field slot 1)."""
+    steps = C.c_int()
+    buf = None
+    if n_cells:
+        buf = np.empty(int(n_cells), np.float32)
+    check(lib.piro_case_run(assets_dir.encode(), case_name.encode(), out_dir.encode() if out_dir else None,
+                            int(max_steps), 1 if alpha_row_scaled else 0, C.byref(steps),
+                            buf.ctypes.data_as(C.POINTER(C.c_float)) if buf is not None else None,
+                            int(n_cells or 0)))
+    return steps.value, buf

@@ -123,6 +123,30 @@ __global__ __launch_bounds__(256) void k_grad_csr(Dims d, int scheme, int dir, f
     }
 }
 
+// Build-only 2-D operator for the backward-Euler system: n x n unknowns,
+// zero Dirichlet values eliminated, rows sorted by column
+// [i-n, i-1, i, i+1, i+n]; values inv_h2 and -4*inv_h2.
+__global__ __launch_bounds__(256) void k_lap2d(int n, float inv_h2, int32_t *__restrict__ rp,
+                                               int32_t *__restrict__ col, float *__restrict__ val)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = (int64_t)n * n;
+    if (i >= N) return;
+    const int x = (int)(i % n), y = (int)(i / n);
+    // rows before i: 5 each minus the missing neighbours
+    const int64_t miss = 2 * (int64_t)y + (x > 0) + (int64_t)n * (y > 0) + (int64_t)x * ((y == 0) + (y == n - 1));
+    int64_t k = 5 * i - miss;
+    rp[i] = (int32_t)k;
+    if (i == N - 1) rp[N] = (int32_t)(5 * N - 4 * (int64_t)n);
+    const float vd = -4.0f * inv_h2;
+    const int32_t ii = (int32_t)i;
+    if (y > 0) { col[k] = ii - n; val[k++] = inv_h2; }
+    if (x > 0) { col[k] = ii - 1; val[k++] = inv_h2; }
+    col[k] = ii; val[k++] = vd;
+    if (x < n - 1) { col[k] = ii + 1; val[k++] = inv_h2; }
+    if (y < n - 1) { col[k] = ii + n; val[k] = inv_h2; }
+}
+
 // SpMDSM.cl:20-31 intended semantics, one thread per row.
 __global__ __launch_bounds__(256) void k_row_scale(int n_rows, const int32_t *__restrict__ rp,
                                                    float *__restrict__ val, const float *__restrict__ diag)
@@ -212,6 +236,20 @@ __global__ __launch_bounds__(256) void k_face_values(Dims d, const float *__rest
     if (x == 0 && yi && zi) v = f1;
     else if (x == d.nx - 1 && yi && zi) v = f0;
     snap[i] = v;
+}
+
+// setBC.cl:1-9 over the face lists: field[f] = snap[f]
+__global__ __launch_bounds__(256) void k_set_faces(Dims d, const float *__restrict__ snap, float *__restrict__ field)
+{
+    int64_t N = d.sxy * d.nz;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int x = (int)(i % d.nx);
+    int64_t t = i / d.nx;
+    int y = (int)(t % d.ny);
+    int z = (int)(t / d.ny);
+    int nb = (x == 0 || x == d.nx - 1) + (y == 0 || y == d.ny - 1) + (z == 0 || z == d.nz - 1);
+    if (nb == 1) field[i] = snap[i];
 }
 
 // ------------------------------------------------------------ hash table
@@ -434,6 +472,18 @@ int piro_assemble_gradient_csr(piro_ctx *ctx, int nx, int ny, int nz, int scheme
     PIRO_API_END
 }
 
+int piro_assemble_laplacian2d_csr(piro_ctx *ctx, int n, float inv_h2, int32_t *d_row_ptr, int32_t *d_col,
+                                  float *d_val)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n >= 2 && (int64_t)n * n * 5 < 0x7fffffffLL, "bad n");
+    PIRO_REQUIRE(d_row_ptr && d_col && d_val, "null output");
+    k_lap2d<<<grid_for((int64_t)n * n, 256), 256, 0, ctx->stream>>>(n, inv_h2, d_row_ptr, d_col, d_val);
+    check_launch();
+    PIRO_API_END
+}
+
 int piro_row_scale(piro_ctx *ctx, int n_rows, const int32_t *d_row_ptr, float *d_val, const float *d_diag)
 {
     PIRO_API_BEGIN
@@ -502,6 +552,17 @@ int piro_apply_face_values(piro_ctx *ctx, int nx, int ny, int nz, const float *d
     int64_t N = d.sxy * nz;
     k_face_values<<<grid_for(N, 256), 256, 0, ctx->stream>>>(d, d_field, f[0], f[1], f[2], f[3], f[4], f[5],
                                                              d_snap);
+    check_launch();
+    PIRO_API_END
+}
+
+int piro_set_faces(piro_ctx *ctx, int nx, int ny, int nz, const float *d_snap, float *d_field)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    Dims d = mkdims(nx, ny, nz);
+    PIRO_REQUIRE(d_snap && d_field, "null pointer");
+    k_set_faces<<<grid_for(d.sxy * nz, 256), 256, 0, ctx->stream>>>(d, d_snap, d_field);
     check_launch();
     PIRO_API_END
 }

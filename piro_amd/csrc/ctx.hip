@@ -43,6 +43,8 @@ int piro_ctx_destroy(piro_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     for (auto &s : ctx->scratch) s.release();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     PIRO_API_END
@@ -73,6 +75,39 @@ int piro_ctx_synchronize(piro_ctx *ctx)
     PIRO_API_BEGIN
     ctx_of(ctx);
     PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    PIRO_API_END
+}
+
+int piro_device_count(int *count)
+{
+    PIRO_API_BEGIN
+    PIRO_REQUIRE(count, "count is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = e == hipSuccess ? n : 0;
+    PIRO_API_END
+}
+
+int piro_timer_start(piro_ctx *ctx)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    if (!ctx->ev0) PIRO_CHECK_HIP(hipEventCreate(&ctx->ev0));
+    if (!ctx->ev1) PIRO_CHECK_HIP(hipEventCreate(&ctx->ev1));
+    PIRO_CHECK_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    PIRO_API_END
+}
+
+int piro_timer_stop(piro_ctx *ctx, float *ms)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(ctx->ev0 && ctx->ev1, "piro_timer_start was not called");
+    PIRO_CHECK_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    PIRO_CHECK_HIP(hipEventSynchronize(ctx->ev1));
+    float t = 0.0f;
+    PIRO_CHECK_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    if (ms) *ms = t;
     PIRO_API_END
 }
 

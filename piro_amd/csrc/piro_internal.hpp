@@ -85,6 +85,7 @@ struct piro_ctx {
     int num_cus = 256;
     int num_xcds = 8;
     piro::Scratch scratch[4];
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // piro_timer_start/stop
     // autotuned FE-step choice keyed by the last matrix shape
     int fe_variant = -1;
     int64_t fe_tuned_rows = -1, fe_tuned_nnz = -1;

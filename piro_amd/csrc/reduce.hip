@@ -18,7 +18,7 @@ namespace {
 constexpr int RBLOCK = 256;
 constexpr int RGRID_MAX = 2048;
 
-enum Op { SUM = 0, SUMSQ = 1, DOT = 2, ABSMAX = 3 };
+enum Op { SUM = 0, SUMSQ = 1, DOT = 2, ABSMAX = 3, DIFFSQ = 4 };
 
 typedef float float4v __attribute__((ext_vector_type(4)));
 
@@ -28,6 +28,7 @@ __device__ __forceinline__ double term(float a, float b)
     if (OP == SUM) return (double)a;
     if (OP == SUMSQ) return (double)a * (double)a;
     if (OP == DOT) return (double)a * (double)b;
+    if (OP == DIFFSQ) return ((double)a - (double)b) * ((double)a - (double)b);
     return fabs((double)a);
 }
 
@@ -66,21 +67,21 @@ __global__ __launch_bounds__(RBLOCK) void k_reduce_pass1(int64_t n, const float 
     double s = 0.0;
     const int64_t n4 = n >> 2;
     const int64_t stride = (int64_t)gridDim.x * RBLOCK;
-    const bool vec = ((((uintptr_t)x) | (OP == DOT ? (uintptr_t)y : 0)) & 15) == 0;
+    const bool vec = ((((uintptr_t)x) | ((OP == DOT || OP == DIFFSQ) ? (uintptr_t)y : 0)) & 15) == 0;
     if (vec) {
         for (int64_t i = (int64_t)blockIdx.x * RBLOCK + threadIdx.x; i < n4; i += stride) {
             float4v a = __builtin_nontemporal_load((const float4v *)x + i);
-            float4v b = OP == DOT ? __builtin_nontemporal_load((const float4v *)y + i) : a;
+            float4v b = (OP == DOT || OP == DIFFSQ) ? __builtin_nontemporal_load((const float4v *)y + i) : a;
             s = combine<OP>(s, term<OP>(a[0], b[0]));
             s = combine<OP>(s, term<OP>(a[1], b[1]));
             s = combine<OP>(s, term<OP>(a[2], b[2]));
             s = combine<OP>(s, term<OP>(a[3], b[3]));
         }
         for (int64_t i = (n4 << 2) + (int64_t)blockIdx.x * RBLOCK + threadIdx.x; i < n; i += stride)
-            s = combine<OP>(s, term<OP>(x[i], OP == DOT ? y[i] : x[i]));
+            s = combine<OP>(s, term<OP>(x[i], (OP == DOT || OP == DIFFSQ) ? y[i] : x[i]));
     } else {
         for (int64_t i = (int64_t)blockIdx.x * RBLOCK + threadIdx.x; i < n; i += stride)
-            s = combine<OP>(s, term<OP>(x[i], OP == DOT ? y[i] : x[i]));
+            s = combine<OP>(s, term<OP>(x[i], (OP == DOT || OP == DIFFSQ) ? y[i] : x[i]));
     }
     s = block_combine<OP>(s);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(RBLOCK) void k_reduce_pass2(int nparts, double *__r
 template <int OP>
 double run(piro_ctx *ctx, int64_t n, const float *x, const float *y)
 {
-    PIRO_REQUIRE(n >= 0 && x && (OP != DOT || y), "bad reduction arguments");
+    PIRO_REQUIRE(n >= 0 && x && ((OP != DOT && OP != DIFFSQ) || y), "bad reduction arguments");
     if (n == 0) return 0.0;
     double *part = (double *)ctx->scratch[3].get((RGRID_MAX + 1) * sizeof(double));
     int64_t want = (n / 4 + RBLOCK - 1) / RBLOCK;
@@ -113,9 +114,32 @@ double run(piro_ctx *ctx, int64_t n, const float *x, const float *y)
     return h;
 }
 
+// out = fl(fl(a*x) + fl(b*y)) elementwise (addVectors/subtractVectors of the
+// reference, addVectors.cl:1-9 and subtractVectors.cl:1-9, are a = 1,
+// b = +-1; both products are then exact)
+__global__ __launch_bounds__(256) void k_axpby(int64_t n, float a, const float *x, float b, const float *y,
+                                               float *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float p = a * x[i], q = b * y[i];
+    out[i] = p + q;
+}
+
 }  // namespace
 
 extern "C" {
+
+int piro_vec_axpby(piro_ctx *ctx, int64_t n, float a, const float *d_x, float b, const float *d_y, float *d_out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n >= 0 && d_x && d_y && d_out, "bad arguments");
+    if (n == 0) return PIRO_OK;
+    k_axpby<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, a, d_x, b, d_y, d_out);
+    check_launch();
+    PIRO_API_END
+}
 
 int piro_reduce_sum(piro_ctx *ctx, int64_t n, const float *d_x, double *h_out)
 {
@@ -141,6 +165,15 @@ int piro_reduce_dot(piro_ctx *ctx, int64_t n, const float *d_x, const float *d_y
     ctx_of(ctx);
     PIRO_REQUIRE(h_out, "h_out is null");
     *h_out = run<DOT>(ctx, n, d_x, d_y);
+    PIRO_API_END
+}
+
+int piro_reduce_diff_sumsq(piro_ctx *ctx, int64_t n, const float *d_x, const float *d_y, double *h_out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(h_out, "h_out is null");
+    *h_out = run<DIFFSQ>(ctx, n, d_x, d_y);
     PIRO_API_END
 }
 

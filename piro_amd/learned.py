@@ -1,0 +1,187 @@
+"""The learned k-step heat propagator around the C-ABI surrogate kernels.
+
+* training data come from the batched stepper (many independent 3-D
+  instances, each with its own smooth diffusivity field) -- the solver is the
+  only data source;
+* training runs the project's own forward/backward/Adam kernels;
+* the accelerated run advances k forward-Euler steps per surrogate call and
+  re-runs every `check_every`-th call with the plain CSR stepper, rejecting
+  (and replacing) the surrogate result when its relative L2 error exceeds
+  `tol` -- the plain stepper stays the fallback and the judge.
+
+Index-space dynamics depend only on r = alpha*dt/h^2, so instances of any size
+with dt = r_max*h^2/alpha_max reproduce the per-cell map of the large run.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .core import CD, Batch, Context, DevArray, Surrogate
+
+ALPHA_LO, ALPHA_HI = 0.005, 0.02
+R_MAX = 0.15
+
+
+def _grid(dims):
+    nx, ny, nz = dims
+    return np.meshgrid(np.arange(nz, dtype=np.float64), np.arange(ny, dtype=np.float64),
+                       np.arange(nx, dtype=np.float64), indexing="ij")
+
+
+def smooth_alpha(dims, rng, n_gauss=8, width_cells=(13.0, 38.0)):
+    """sum of n_gauss Gaussians (seeded centres inside the box, widths given in
+    cells) mapped linearly onto [0.005, 0.02]; dims = (nx, ny, nz).  At 128^3
+    the default widths are 0.1-0.3 of the box."""
+    nx, ny, nz = dims
+    z, y, x = _grid(dims)
+    f = np.zeros((nz, ny, nx))
+    for _ in range(n_gauss):
+        c = rng.uniform(0, 1, 3) * np.array([nx - 1, ny - 1, nz - 1])
+        w = rng.uniform(*width_cells)
+        amp = rng.uniform(0.5, 1.0)
+        f += amp * np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / (2 * w * w))
+    f = (f - f.min()) / max(f.max() - f.min(), 1e-12)
+    return (ALPHA_LO + (ALPHA_HI - ALPHA_LO) * f).astype(np.float32).ravel()
+
+
+def blob_ic(dims, rng, n_blobs=None, width_cells=(3.0, 14.0)):
+    """1-4 Gaussian blobs (widths in cells), peak normalised to 1, zero on the
+    boundary cells"""
+    nx, ny, nz = dims
+    n_blobs = int(rng.integers(1, 5)) if n_blobs is None else n_blobs
+    z, y, x = _grid(dims)
+    f = np.zeros((nz, ny, nx))
+    for _ in range(n_blobs):
+        c = rng.uniform(0.2, 0.8, 3) * np.array([nx - 1, ny - 1, nz - 1])
+        w = rng.uniform(*width_cells)
+        f += rng.uniform(0.3, 1.0) * np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / (2 * w * w))
+    f /= max(np.abs(f).max(), 1e-12)
+    f[0, :, :] = f[-1, :, :] = 0
+    f[:, 0, :] = f[:, -1, :] = 0
+    f[:, :, 0] = f[:, :, -1] = 0
+    return f.astype(np.float32).ravel()
+
+
+def dt_for(nx):
+    h = np.float32(1.0) / np.float32(nx - 2)
+    norm = np.float32(float(h) ** 2)
+    return np.float32(np.float32(R_MAX) * norm / np.float32(ALPHA_HI))
+
+
+@dataclass
+class TrainLog:
+    losses: list = field(default_factory=list)
+    trajectories: int = 0
+    seconds: float = 0.0
+
+
+def train(ctx: Context, sur: Surrogate, n_traj=2000, per_batch=50, side=24, segments=4, k=16, passes=2,
+          lr=2e-3, lr_final=2e-4, seed=1234, log_every=0) -> TrainLog:
+    """Generate n_traj trajectories with the batched stepper and train on
+    every (u_t, u_{t+k}) pair as it is produced."""
+    rng = np.random.default_rng(seed)
+    dims = (side, side, side)
+    log = TrainLog()
+    n_batches = max(1, n_traj // per_batch)
+    total_steps = n_batches * segments * passes
+    step = 0
+    t0 = time.perf_counter()
+    dt = dt_for(side)
+    shape = (per_batch, side, side, side)
+    for _ in range(n_batches):
+        alpha = np.concatenate([smooth_alpha(dims, rng) for _ in range(per_batch)])
+        u0 = np.concatenate([blob_ic(dims, rng) for _ in range(per_batch)])
+        da = ctx.from_host(alpha)
+        b = Batch(ctx, [dims] * per_batch, da, [dt] * per_batch)
+        x = ctx.from_host(u0)
+        w = ctx.empty_like(x)
+        prev = ctx.empty_like(x)
+        target = ctx.empty_like(x)
+        for _ in range(segments):
+            prev.copy_(x)
+            b.run(x, w, k, use_graph=True)
+            ctx.axpby(1.0, x, -1.0, prev, target)
+            for _ in range(passes):
+                frac = step / max(total_steps - 1, 1)
+                cur_lr = lr * (lr_final / lr) ** frac
+                loss = sur.train_step(shape, prev, da, target, 1.0, cur_lr)
+                log.losses.append(loss)
+                step += 1
+                if log_every and step % log_every == 0:
+                    print(f"train step {step}/{total_steps} loss {loss:.3e}", flush=True)
+        log.trajectories += per_batch
+        b.close()
+    ctx.sync()
+    log.seconds = time.perf_counter() - t0
+    return log
+
+
+@dataclass
+class RunResult:
+    seconds: float
+    calls: int = 0
+    checks: int = 0
+    rejected: int = 0
+    check_errors: list = field(default_factory=list)
+
+
+class HeatRun:
+    """128^3-style run: plain stepper and surrogate-accelerated stepper."""
+
+    def __init__(self, ctx: Context, side=128, seed=77):
+        self.ctx = ctx
+        self.dims = (side, side, side)
+        rng = np.random.default_rng(seed)
+        self.alpha = ctx.from_host(smooth_alpha(self.dims, rng))
+        self.A = ctx.laplacian_csr(side, side, side, CD, ctx.lap_norm(1.0, side - 2))
+        ctx.row_scale(self.A, self.alpha)
+        self.dt = dt_for(side)
+        self.u0 = ctx.ic_gaussian(self.dims, [1.0] * 3, [0.5] * 3, [0.1] * 3, 1.0)
+        # zero Dirichlet faces from step 1 on, as the reference's setBC does
+        self.snap = ctx.empty_like(self.u0).zero_()
+
+    def plain(self, steps, x: DevArray = None):
+        x = self.u0.clone() if x is None else x
+        w = self.ctx.empty_like(x)
+        self.ctx.sync()
+        t0 = time.perf_counter()
+        self.ctx.fe_run_csr(self.dims, self.A, self.dt, steps, x, w, self.snap)
+        self.ctx.sync()
+        return x, time.perf_counter() - t0
+
+    def accelerated(self, sur: Surrogate, steps, k=16, check_every=10, tol=1e-3):
+        side = self.dims[0]
+        shape = (1, side, side, side)
+        ctx = self.ctx
+        x = self.u0.clone()
+        y = ctx.empty_like(x)
+        w = ctx.empty_like(x)
+        ref = ctx.empty_like(x)
+        res = RunResult(0.0)
+        calls = steps // k
+        ctx.sync()
+        t0 = time.perf_counter()
+        for c in range(calls):
+            scale = ctx.reduce("absmax", x)
+            scale = scale if scale > 0 else 1.0
+            sur.forward(shape, x, self.alpha, scale, out=y)
+            res.calls += 1
+            if (c + 1) % check_every == 0:
+                ref.copy_(x)
+                ctx.fe_run_csr(self.dims, self.A, self.dt, k, ref, w, self.snap)
+                err = ctx.rel_l2(y, ref)
+                res.checks += 1
+                res.check_errors.append(err)
+                if err > tol:
+                    res.rejected += 1
+                    y.copy_(ref)
+            x, y = y, x
+        rem = steps - calls * k
+        if rem:
+            ctx.fe_run_csr(self.dims, self.A, self.dt, rem, x, w, self.snap)
+        ctx.sync()
+        res.seconds = time.perf_counter() - t0
+        return x, res

@@ -3,7 +3,6 @@ the CPU oracle's single-instance run, for every tiling and with/without the
 captured hipGraph; a 3-D instance bit-identical to the single-grid FE path."""
 import numpy as np
 import pytest
-import torch
 
 import oracle as po
 
@@ -61,14 +60,13 @@ def test_batch_matches_single_instances(ctx, tiling):
     print(f"case batch tiling={tiling}", flush=True)
     dims, alphas = make_instances(12, 1)
     alpha, dts, x0 = instance_inputs(ctx, dims, alphas, 2)
-    b = ctx.batch(dims, torch.from_numpy(alpha).to(ctx.device), dts)
+    b = ctx.batch(dims, ctx.from_host(alpha), dts)
     b.set_tiling(*tiling)
-    x = torch.from_numpy(x0).to(ctx.device)
-    w = torch.empty_like(x)
+    x = ctx.from_host(x0)
+    w = ctx.empty_like(x)
     steps = 9
     b.run(x, w, steps, use_graph=False)
-    torch.cuda.synchronize()
-    got = x.cpu().numpy()
+    got = x.host()
     for i, (d, a) in enumerate(zip(dims, alphas)):
         s, e = b.offsets[i], b.offsets[i + 1]
         ref = oracle_instance(d, a, dts[i], x0[s:e], steps)
@@ -79,20 +77,17 @@ def test_batch_matches_single_instances(ctx, tiling):
 def test_batch_graph_replays_equal_eager(ctx):
     dims, alphas = make_instances(20, 3)
     alpha, dts, x0 = instance_inputs(ctx, dims, alphas, 4)
-    b = ctx.batch(dims, torch.from_numpy(alpha).to(ctx.device), dts)
-    xe = torch.from_numpy(x0).to(ctx.device)
-    we = torch.empty_like(xe)
+    b = ctx.batch(dims, ctx.from_host(alpha), dts)
+    xe = ctx.from_host(x0)
+    we = ctx.empty_like(xe)
     b.run(xe, we, 10, use_graph=False)
-    torch.cuda.synchronize()
-    eager = xe.cpu().numpy()
-    xg = torch.from_numpy(x0).to(ctx.device)
-    wg = torch.empty_like(xg)
+    eager = xe.host()
+    xg = ctx.from_host(x0)
+    wg = ctx.empty_like(xg)
     for rep in range(3):
-        xg.copy_(torch.from_numpy(x0).to(ctx.device))
-        torch.cuda.synchronize()
+        xg.upload(x0)
         b.run(xg, wg, 10, use_graph=True)
-        torch.cuda.synchronize()
-        assert xg.cpu().numpy().tobytes() == eager.tobytes(), f"replay {rep}"
+        assert xg.host().tobytes() == eager.tobytes(), f"replay {rep}"
     b.close()
 
 
@@ -100,31 +95,29 @@ def test_batch_3d_instance_equals_single_grid_path(ctx):
     dims = [(12, 11, 10)]
     a = np.float32(0.02)
     alpha, dts, x0 = instance_inputs(ctx, dims, [a], 5, r=0.15)
-    b = ctx.batch(dims, torch.from_numpy(alpha).to(ctx.device), dts)
-    x = torch.from_numpy(x0).to(ctx.device)
-    w = torch.empty_like(x)
+    b = ctx.batch(dims, ctx.from_host(alpha), dts)
+    x = ctx.from_host(x0)
+    w = ctx.empty_like(x)
     b.run(x, w, 6, use_graph=True)
     # the single-grid path: assembled operator, row scaled, zero faces
     A = ctx.laplacian_csr(12, 11, 10, 3, ctx.lap_norm(1.0, 10))
-    ctx.row_scale(A, torch.from_numpy(alpha).to(ctx.device))
-    xs = torch.from_numpy(x0).to(ctx.device)
+    ctx.row_scale(A, ctx.from_host(alpha))
+    xs = ctx.from_host(x0)
     ctx.fe_run_csr((12, 11, 10), A, dts[0], 6, xs, snap=None)
-    torch.cuda.synchronize()
-    assert x.cpu().numpy().tobytes() == xs.cpu().numpy().tobytes()
+    assert x.host().tobytes() == xs.host().tobytes()
     b.close()
 
 
 def test_batch_autotune_keeps_results(ctx):
     dims, alphas = make_instances(30, 6, 16, 64, with3d=False)
     alpha, dts, x0 = instance_inputs(ctx, dims, alphas, 7)
-    b = ctx.batch(dims, torch.from_numpy(alpha).to(ctx.device), dts)
-    x = torch.from_numpy(x0).to(ctx.device)
-    w = torch.empty_like(x)
+    b = ctx.batch(dims, ctx.from_host(alpha), dts)
+    x = ctx.from_host(x0)
+    w = ctx.empty_like(x)
     rb, g, ms = b.autotune(x.clone(), w)
     print(f"batch autotune: rows/unit {rb}, grid {g}, {ms * 1e3:.1f} us/step")
     b.run(x, w, 5, use_graph=True)
-    torch.cuda.synchronize()
-    got = x.cpu().numpy()
+    got = x.host()
     for i, (d, a) in enumerate(zip(dims, alphas)):
         s, e = b.offsets[i], b.offsets[i + 1]
         assert got[s:e].tobytes() == oracle_instance(d, a, dts[i], x0[s:e], 5).tobytes()

@@ -5,7 +5,6 @@ itself pinned to scipy.linalg.lu_factor) and numpy's FP64 solve.
 """
 import numpy as np
 import pytest
-import torch
 
 import oracle as po
 
@@ -21,7 +20,7 @@ def ctx():
 
 
 def dev(a, ctx):
-    return torch.from_numpy(np.ascontiguousarray(a)).to(ctx.device)
+    return ctx.from_host(np.ascontiguousarray(a))
 
 
 def perm_from_ipiv(ipiv, n):
@@ -37,10 +36,9 @@ def test_lu_factor_reconstructs(ctx, n):
     rng = np.random.default_rng(n)
     a = rng.standard_normal((n, n)).astype(np.float32)
     d = dev(a, ctx)
-    ipiv = ctx.lu_factor(d)
-    torch.cuda.synchronize()
-    lu = d.cpu().numpy().astype(np.float64)
-    piv = ipiv.cpu().numpy()
+    ipiv = ctx.lu_factor(d, n)
+    lu = d.host().reshape(n, n).astype(np.float64)
+    piv = ipiv.host()
     L = np.tril(lu, -1) + np.eye(n)
     U = np.triu(lu)
     P = perm_from_ipiv(piv, n)
@@ -68,10 +66,9 @@ def test_mfma_update_layout_exact(ctx, n):
     a = (L0 @ U0).astype(np.float32)
     assert np.array_equal(a.astype(np.float64), L0 @ U0)
     d = dev(a, ctx)
-    ipiv = ctx.lu_factor(d)
-    torch.cuda.synchronize()
-    lu = d.cpu().numpy().astype(np.float64)
-    np.testing.assert_array_equal(ipiv.cpu().numpy(), np.arange(n))
+    ipiv = ctx.lu_factor(d, n)
+    lu = d.host().reshape(n, n).astype(np.float64)
+    np.testing.assert_array_equal(ipiv.host(), np.arange(n))
     np.testing.assert_array_equal(np.triu(lu), U0)
     np.testing.assert_array_equal(np.tril(lu, -1), np.tril(L0, -1))
 
@@ -82,11 +79,10 @@ def test_lu_solve_f32_matches_numpy(ctx):
     a = (rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)).astype(np.float32)
     b = rng.standard_normal((n, 5)).astype(np.float32)
     d = dev(a, ctx)
-    ipiv = ctx.lu_factor(d)
+    ipiv = ctx.lu_factor(d, n)
     db = dev(b, ctx)
-    ctx.lu_solve_f32(d, ipiv, db)
-    torch.cuda.synchronize()
-    x = db.cpu().numpy()
+    ctx.lu_solve_f32(d, n, ipiv, db)
+    x = db.host().reshape(n, 5)
     xr = np.linalg.solve(a.astype(np.float64), b.astype(np.float64))
     assert po.rel_l2(x, xr) < 1e-4
 
@@ -129,11 +125,10 @@ def test_c3_mixed_precision_refinement(ctx, nrhs):
     dA = DevCSR(dev(A.row_ptr, ctx), dev(A.col, ctx), dev(A.val, ctx))
     dense = ctx.csr_to_dense(dA)
     lu = dense.clone()
-    ipiv = ctx.lu_factor(lu)
+    ipiv = ctx.lu_factor(lu, n)
     b = c3_rhs(64, nrhs)
-    x, sweeps, rel = ctx.lu_solve_refine(dense, lu, ipiv, dev(b, ctx), tol=1e-10, max_sweeps=10)
-    torch.cuda.synchronize()
-    xh = x.cpu().numpy()
+    x, sweeps, rel = ctx.lu_solve_refine(dense, lu, n, ipiv, dev(b, ctx), tol=1e-10, max_sweeps=10)
+    xh = x.host().reshape(n, nrhs)
     Ad = A.dense()
     xr = np.linalg.solve(Ad, b)
     res = np.linalg.norm(b - Ad @ xh, axis=0) / np.linalg.norm(b, axis=0)

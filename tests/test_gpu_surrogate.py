@@ -4,7 +4,6 @@ oracle's conv3d is pinned by brute force and finite differences in
 tests/test_oracle_surrogate.py), plus a training run that must cut the loss."""
 import numpy as np
 import pytest
-import torch
 
 import oracle as po
 
@@ -118,11 +117,10 @@ def test_forward_matches_oracle(ctx, shape):
     sur.set_params(p)
     u, al = fields(shape, 5)
     scale = 1.7
-    du = torch.from_numpy(u.ravel()).to(ctx.device)
-    da = torch.from_numpy(al.ravel()).to(ctx.device)
+    du = ctx.from_host(u.ravel())
+    da = ctx.from_host(al.ravel())
     out = sur.forward(shape, du, da, scale)
-    torch.cuda.synchronize()
-    got = out.cpu().numpy().reshape(shape)
+    got = out.host().reshape(shape)
     _, _, delta = ref_forward(p, u, al, scale)
     m = interior_mask(shape)
     assert np.all(got[~m] == 0)
@@ -138,9 +136,9 @@ def test_untrained_surrogate_is_identity_on_interior(ctx):
     sur = Surrogate(ctx, seed=9)
     shape = (1, 6, 7, 8)
     u, al = fields(shape, 1)
-    out = sur.forward(shape, torch.from_numpy(u.ravel()).to(ctx.device), torch.from_numpy(al.ravel()).to(ctx.device),
+    out = sur.forward(shape, ctx.from_host(u.ravel()), ctx.from_host(al.ravel()),
                       1.0)
-    got = out.cpu().numpy().reshape(shape)
+    got = out.host().reshape(shape)
     m = interior_mask(shape)
     np.testing.assert_array_equal(got[m], u[m])
     sur.close()
@@ -157,9 +155,9 @@ def test_gradients_match_oracle(ctx, shape):
     rng = np.random.default_rng(14)
     target = rng.uniform(-0.1, 0.1, shape).astype(np.float32)
     scale = 1.3
-    g, loss = sur.gradients(shape, torch.from_numpy(u.ravel()).to(ctx.device),
-                            torch.from_numpy(al.ravel()).to(ctx.device),
-                            torch.from_numpy(target.ravel()).to(ctx.device), scale)
+    g, loss = sur.gradients(shape, ctx.from_host(u.ravel()),
+                            ctx.from_host(al.ravel()),
+                            ctx.from_host(target.ravel()), scale)
     gr, lr, _ = ref_grads(p, u, al, target.astype(np.float64), scale)
     assert loss == pytest.approx(lr, rel=1e-2)
     # per parameter block (BF16 activation gradients: a few 1e-3 per layer)
@@ -195,9 +193,9 @@ def test_training_reduces_loss(ctx):
         v = po.fe_csr_run((W, H, D), A, dt, 4, u.ravel(), None).reshape(D, H, W)
         us.append(u); ts.append(v - u); als.append(a)
     u = np.stack(us); t = np.stack(ts); a = np.stack(als)
-    du = torch.from_numpy(u.ravel()).to(ctx.device)
-    da = torch.from_numpy(a.ravel()).to(ctx.device)
-    dt_ = torch.from_numpy(t.ravel().astype(np.float32)).to(ctx.device)
+    du = ctx.from_host(u.ravel())
+    da = ctx.from_host(a.ravel())
+    dt_ = ctx.from_host(t.ravel().astype(np.float32))
     losses = [sur.train_step(shape, du, da, dt_, 1.0, 3e-3) for _ in range(150)]
     print(f"surrogate training: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     assert losses[-1] < 0.2 * losses[0]
