@@ -27,6 +27,7 @@ METRIC = "cell-steps/s + SpMV HBM GB/s (256³ heat); surrogate vs plain speedup 
 HBM_PEAK_GBS = 8000.0
 DIMS = (256, 256, 256)
 SEED = 20261203
+COPY_PROBE_FLOATS = 64 << 20   # 256 MiB read + 256 MiB written per launch
 
 
 def algorithmic_bytes(n_rows: int, nnz: int) -> int:
@@ -34,13 +35,18 @@ def algorithmic_bytes(n_rows: int, nnz: int) -> int:
     return 8 * nnz + 4 * (n_rows + 1) + 4 * n_rows + 4 * n_rows
 
 
-def load_traffic():
+def load_traffic(variant_name):
+    """HBM bytes per launch of the FE kernel measured with rocprofv3 counters
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), for
+    the kernel variant that was measured; None when no measurement of this
+    variant is on file."""
     path = os.path.join(ROOT, "profiles", "fe_csr_256cubed_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get("hbm_bytes_per_launch")
+    rec = d.get("variants", {}).get(variant_name)
+    return None if rec is None else rec.get("hbm_bytes_per_launch")
 
 
 def cpu_baseline(budget_s: float = 15.0):
@@ -84,6 +90,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=-1, help="force an FE kernel variant (-1 = autotune)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed region, run the library's streaming copy on a 256 MiB vector 3 times "
+                         "(the known-size kernel that HBM counter readings are calibrated against)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are out of scope "
@@ -134,6 +143,14 @@ def main() -> int:
     byts = algorithmic_bytes(n_rows, nnz)
     gbs = byts / (ms_step / 1e3) / 1e9
     finite = bool(np.isfinite(ctx.reduce("sumsq", x)))
+    if args.copy_probe:
+        src = ctx.full(COPY_PROBE_FLOATS, 1.0)
+        dst = ctx.empty_like(src)
+        for _ in range(3):
+            ctx.copy(src, dst)
+        ctx.sync()
+        src.free()
+        dst.free()
 
     base = None
     if not args.no_cpu_baseline:
@@ -158,7 +175,7 @@ def main() -> int:
         "hbm_gbs": gbs,
         "interior_cell_steps_per_s": interior * args.steps / (ms / 1e3),
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": load_traffic(),
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": load_traffic(vname),
                      "algorithmic_bytes_per_launch": byts},
         "cpu_baseline": base,
         "field_finite": finite,

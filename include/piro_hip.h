@@ -263,6 +263,12 @@ int piro_reduce_absmax(piro_ctx *ctx, int64_t n, const float *d_x,
  * subtractVectors.cl:1-9).  d_out may alias d_x or d_y. */
 int piro_vec_axpby(piro_ctx *ctx, int64_t n, float a, const float *d_x,
                    float b, const float *d_y, float *d_out);
+/* d_dst = d_src with a plain streaming kernel (16-byte accesses when both
+ * pointers are 16-byte aligned): the reference's per-step buffer copy
+ * (clEnqueueCopyBuffer, src/process.cpp:234).  Also the known-size kernel
+ * that HBM counter readings are calibrated against.  The two ranges must
+ * not overlap (equal pointers are a no-op). */
+int piro_vec_copy(piro_ctx *ctx, int64_t n, const float *d_src, float *d_dst);
 /* sum (x_i - y_i)^2 in FP64 (relative-L2 checks without a read-back) */
 int piro_reduce_diff_sumsq(piro_ctx *ctx, int64_t n, const float *d_x,
                            const float *d_y, double *h_out);

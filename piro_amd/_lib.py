@@ -83,6 +83,7 @@ _SIGS = {
     "piro_reduce_absmax": [vp, i64, vp, P_f64],
     "piro_reduce_diff_sumsq": [vp, i64, vp, vp, P_f64],
     "piro_vec_axpby": [vp, i64, f32, vp, f32, vp, vp],
+    "piro_vec_copy": [vp, i64, vp, vp],
     "piro_csr_to_dense": [vp, i32, i32, vp, vp, vp, vp, i32],
     "piro_lu_factor_f32": [vp, i32, vp, i32, vp, P_i32],
     "piro_lu_solve_f32": [vp, i32, vp, i32, vp, i32, vp, i32],

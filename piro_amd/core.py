@@ -356,6 +356,13 @@ class Context:
             raise ValueError(op)
         return out.value
 
+    def copy(self, src: DevArray, dst: DevArray = None) -> DevArray:
+        """dst = src with the library's streaming copy kernel"""
+        dst = self.empty_like(src) if dst is None else dst
+        assert dst.numel() >= src.numel()
+        check(lib.piro_vec_copy(self.h, src.numel(), _p(src), _p(dst)))
+        return dst
+
     def axpby(self, a, x: DevArray, b, y: DevArray, out: DevArray = None) -> DevArray:
         out = self.empty_like(x) if out is None else out
         check(lib.piro_vec_axpby(self.h, x.numel(), float(a), _p(x), float(b), _p(y), _p(out)))

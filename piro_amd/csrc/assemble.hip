@@ -83,29 +83,35 @@ __global__ __launch_bounds__(256) void k_lap_csr(Dims d, int scheme, float vself
 
 // src/stencils.cpp:7-231: per row (lower, -1/norm) if d>0, (upper, +1/norm)
 // if d<n_d-1; uw: upper -> self; dw: lower -> self.
-__global__ __launch_bounds__(256) void k_grad_csr(Dims d, int scheme, int dir, float vlo, float vhi,
+template <int DIR>
+__global__ __launch_bounds__(256) void k_grad_csr(Dims d, int scheme, float vlo, float vhi,
                                                   int32_t *__restrict__ row_ptr, int32_t *__restrict__ col,
                                                   float *__restrict__ val)
 {
-    int64_t N = d.sxy * d.nz;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // The direction is a template parameter: a run-time if/else chain over it
+    // was mis-structurized by the gfx950 back end (the last arm read an
+    // undefined register for the coordinate).
+    const int64_t N = d.sxy * d.nz;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    int x = (int)(i % d.nx);
-    int64_t t = i / d.nx;
-    int y = (int)(t % d.ny);
-    int z = (int)(t / d.ny);
+    const int x = (int)(i % d.nx);
+    const int64_t t = i / d.nx;
+    const int y = (int)(t % d.ny);
+    const int z = (int)(t / d.ny);
+    // entries in rows [0, i): 2 per row minus one per row on either end of the
+    // differentiated axis
     int64_t miss;
     int c, nd;
     int64_t stride;
-    if (dir == 0) {
+    if constexpr (DIR == 0) {
         miss = 2 * ((int64_t)z * d.ny + y) + (x > 0);
         c = x; nd = d.nx; stride = 1;
-    } else if (dir == 1) {
-        int my = (y == 0) + (y == d.ny - 1);
+    } else if constexpr (DIR == 1) {
+        const int my = (y == 0) + (y == d.ny - 1);
         miss = 2 * (int64_t)d.nx * z + (int64_t)d.nx * (y > 0) + (int64_t)x * my;
         c = y; nd = d.ny; stride = d.nx;
     } else {
-        int mz = (z == 0) + (z == d.nz - 1);
+        const int mz = (z == 0) + (z == d.nz - 1);
         miss = d.sxy * (z > 0) + ((int64_t)y * d.nx + x) * mz;
         c = z; nd = d.nz; stride = d.sxy;
     }
@@ -467,7 +473,13 @@ int piro_assemble_gradient_csr(piro_ctx *ctx, int nx, int ny, int nz, int scheme
     // -1/norm and 1/norm are float divisions in the reference (int / float)
     float vlo = -1.0f / norm, vhi = 1.0f / norm;
     int64_t N = d.sxy * nz;
-    k_grad_csr<<<grid_for(N, 256), 256, 0, ctx->stream>>>(d, scheme, dir, vlo, vhi, d_row_ptr, d_col, d_val);
+    const dim3 g = grid_for(N, 256);
+    if (dir == 0)
+        k_grad_csr<0><<<g, 256, 0, ctx->stream>>>(d, scheme, vlo, vhi, d_row_ptr, d_col, d_val);
+    else if (dir == 1)
+        k_grad_csr<1><<<g, 256, 0, ctx->stream>>>(d, scheme, vlo, vhi, d_row_ptr, d_col, d_val);
+    else
+        k_grad_csr<2><<<g, 256, 0, ctx->stream>>>(d, scheme, vlo, vhi, d_row_ptr, d_col, d_val);
     check_launch();
     PIRO_API_END
 }

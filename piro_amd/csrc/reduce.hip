@@ -126,9 +126,49 @@ __global__ __launch_bounds__(256) void k_axpby(int64_t n, float a, const float *
     out[i] = p + q;
 }
 
+// 16-byte copy; n4 = number of float4 groups, the tail is copied elementwise
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_copy4(int64_t n4, const f32x4 *__restrict__ src, f32x4 *__restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+__global__ __launch_bounds__(256) void k_copy1(int64_t n, const float *__restrict__ src, float *__restrict__ dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+int piro_vec_copy(piro_ctx *ctx, int64_t n, const float *d_src, float *d_dst)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n >= 0 && d_src && d_dst, "bad arguments");
+    if (n == 0 || d_src == d_dst) return PIRO_OK;
+    const bool aligned = (((uintptr_t)d_src | (uintptr_t)d_dst) & 15) == 0;
+    if (aligned && n >= 4) {
+        const int64_t n4 = n / 4;
+        const int64_t blocks = (n4 + 255) / 256;
+        const unsigned grid = (unsigned)(blocks < (int64_t)ctx->num_cus * 16 ? blocks : (int64_t)ctx->num_cus * 16);
+        k_copy4<<<grid, 256, 0, ctx->stream>>>(n4, (const f32x4 *)d_src, (f32x4 *)d_dst);
+        check_launch();
+        if (n4 * 4 < n) {
+            k_copy1<<<1, 256, 0, ctx->stream>>>(n - n4 * 4, d_src + n4 * 4, d_dst + n4 * 4);
+            check_launch();
+        }
+    } else {
+        k_copy1<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, d_src, d_dst);
+        check_launch();
+    }
+    PIRO_API_END
+}
 
 int piro_vec_axpby(piro_ctx *ctx, int64_t n, float a, const float *d_x, float b, const float *d_y, float *d_out)
 {

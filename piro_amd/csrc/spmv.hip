@@ -212,6 +212,159 @@ __global__ __launch_bounds__(256) void k_stream(Params p)
     }
 }
 
+// --------------------------------------------- prefetching stream kernel
+//
+// Same products, LDS staging and per-row sums as k_stream, with the memory
+// round trips of a block cut to one before the non-zero stream starts:
+//   * the block's non-zero range [k0, k1) comes from two uniform loads;
+//   * every lane issues all its 16-byte col/val loads (QPL quads) together,
+//     along with its rows' row_ptr pair, x[r] and (face rows) the snapshot
+//     value, before the first gather;
+//   * one barrier separates the LDS products from the row sums.
+// A block whose range exceeds the staged capacity walks its rows from memory.
+
+// quad j of lane t is q0 + t + 256*j.  Fast path (every quad up to the last
+// one holding an entry below k1 lies wholly inside [0, nnz)): straight-line
+// code in which a lane past that last quad re-reads it (the same address as
+// another lane of the wave, so no extra traffic) and re-stores the same
+// products to the same LDS slots; all 2*QPL 16-byte loads of a lane are
+// issued together, then the per-row loads, then the 4*QPL gathers.
+template <int QPL>
+struct Quads {
+    int4v c[QPL];
+    float4v v[QPL];
+    int kb[QPL];
+};
+
+template <int QPL>
+__device__ __forceinline__ void load_quads(const Params &p, int q0, int k1, Quads<QPL> &qd)
+{
+    const int qlast = ((k1 + 3) >> 2) - 1;
+#pragma unroll
+    for (int j = 0; j < QPL; ++j) {
+        qd.kb[j] = min(q0 + (int)threadIdx.x + 256 * j, qlast) << 2;
+        qd.c[j] = __builtin_nontemporal_load((const int4v *)(p.col + qd.kb[j]));
+        qd.v[j] = __builtin_nontemporal_load((const float4v *)(p.val + qd.kb[j]));
+    }
+}
+
+template <int MODE, int QPL>
+__device__ __forceinline__ void store_products(const Params &p, int ka, const Quads<QPL> &qd, float *s_p)
+{
+#pragma unroll
+    for (int j = 0; j < QPL; ++j) {
+        float4v out;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) out[e] = product<MODE>(qd.v[j][e], p.x[qd.c[j][e]], p.dt);
+        *(float4v *)(s_p + (qd.kb[j] - ka)) = out;
+    }
+}
+
+// the block holding the end of the arrays: element-wise guarded loads
+template <int MODE, int QPL>
+__device__ __forceinline__ void stage_tail(const Params &p, int q0, int ka, int k1, float *s_p)
+{
+#pragma unroll
+    for (int j = 0; j < QPL; ++j) {
+        const int kb = (q0 + (int)threadIdx.x + 256 * j) << 2;
+        if (kb < k1) {
+            float4v out;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const bool ok = (int64_t)kb + e < p.nnz;
+                const int cc = ok ? p.col[kb + e] : 0;
+                const float vv = ok ? p.val[kb + e] : 0.0f;
+                out[e] = product<MODE>(vv, p.x[cc], p.dt);
+            }
+            *(float4v *)(s_p + (kb - ka)) = out;
+        }
+    }
+}
+
+template <int RB, int MODE, int QPL>
+__global__ __launch_bounds__(256) void k_pf(Params p)
+{
+    constexpr int RPL = RB / 256;      // rows per lane
+    constexpr int CAP = QPL * 256 * 4; // products staged per block
+    __shared__ __attribute__((aligned(16))) float s_p[CAP];
+
+    const int b = logical_block(p);
+    if (b >= p.n_blocks) return;
+    if (MODE == JLI && *p.done) return;
+    const int t = threadIdx.x;
+    const int r0 = b * RB;
+    const int nr = min(RB, p.n_rows - r0);
+    const int k0 = p.rp[r0], k1 = p.rp[r0 + nr];
+    const int ka = k0 & ~3;
+    const bool staged = k1 - ka <= CAP;
+    // fast: at least one entry from ka on, and the quads up to the one
+    // holding entry k1 - 1 lie inside the arrays
+    const bool fast = staged && k1 > ka && (int64_t)((k1 + 3) & ~3) <= p.nnz;
+
+    Quads<QPL> qd;
+    if (fast) load_quads<QPL>(p, ka >> 2, k1, qd);
+
+    // per-row operands
+    int rb[RPL], re[RPL], cls[RPL];
+    float xr[RPL], sv[RPL];
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) {
+        const int rr = t + 256 * j;
+        const int r = r0 + min(rr, nr - 1);
+        rb[j] = p.rp[r];
+        re[j] = p.rp[r + 1];
+        cls[j] = (MODE == FE_INTERIOR || MODE == JLI) ? row_class((uint32_t)r, p.g) : 0;
+        xr[j] = (MODE == SPMV) ? 0.0f : p.x[r];
+        sv[j] = 0.0f;
+        if (MODE == JLI) sv[j] = p.snap[r];
+        if (MODE == FE_INTERIOR) {
+            // face rows read the snapshot; other rows re-read x[r] (a cache
+            // hit) instead of branching around the load
+            const bool face = cls[j] == 1 && p.snap;
+            const float f = *(face ? p.snap + r : p.x + r);
+            sv[j] = face ? f : 0.0f;
+        }
+    }
+
+    if (fast) store_products<MODE, QPL>(p, ka, qd, s_p);
+    else if (staged) stage_tail<MODE, QPL>(p, ka >> 2, ka, k1, s_p);
+    __syncthreads();
+
+    double jli_sq = 0.0;
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) {
+        const int rr = t + 256 * j;
+        if (rr >= nr) continue;
+        const int r = r0 + rr;
+        float acc = 0.0f;
+        if (staged) {
+            for (int k = rb[j] - ka; k < re[j] - ka; ++k) acc = acc + s_p[k];
+        } else {
+            for (int k = rb[j]; k < re[j]; ++k) acc = acc + product<MODE>(p.val[k], p.x[p.col[k]], p.dt);
+        }
+        if (MODE == FE_INTERIOR) {
+            p.y[r] = cls[j] == 0 ? acc + xr[j] : (cls[j] == 1 ? sv[j] : 0.0f);
+        } else if (MODE == ALLROWS) {
+            const float cc = p.dt * acc;
+            p.y[r] = xr[j] + cc;
+        } else if (MODE == SPMV) {
+            p.y[r] = acc;
+        } else {
+            const float ci = cls[j] == 0 ? xr[j] - acc : 0.0f;
+            const float cc = sv[j] - ci;
+            const float inc = cc * p.urf;
+            const float xn = xr[j] + inc;
+            const float d = xn - xr[j];
+            p.y[r] = xn;
+            jli_sq += (double)d * (double)d;
+        }
+    }
+    if (MODE == JLI) {
+        double s = block_sum_f64<256>(jli_sq);
+        if (threadIdx.x == 0) atomicAdd(p.acc, s);
+    }
+}
+
 // ------------------------------------------------------- row per lane
 
 template <int MODE>
@@ -243,18 +396,25 @@ struct Variant {
     int rows_per_block;   // 0 = row-per-lane
     bool vec;
     bool xcd;
+    bool pf;              // prefetching stream kernel
 };
 
 const Variant kVariants[] = {
-    {"stream_rb256_vec", 256, true, false},
-    {"stream_rb512_vec", 512, true, false},
-    {"stream_rb1024_vec", 1024, true, false},
-    {"stream_rb256_vec_xcd", 256, true, true},
-    {"stream_rb512_vec_xcd", 512, true, true},
-    {"stream_rb1024_vec_xcd", 1024, true, true},
-    {"stream_rb512_scalar", 512, false, false},
-    {"row_per_lane", 0, false, false},
-    {"row_per_lane_xcd", 0, false, true},
+    {"stream_rb256_vec", 256, true, false, false},
+    {"stream_rb512_vec", 512, true, false, false},
+    {"stream_rb1024_vec", 1024, true, false, false},
+    {"stream_rb256_vec_xcd", 256, true, true, false},
+    {"stream_rb512_vec_xcd", 512, true, true, false},
+    {"stream_rb1024_vec_xcd", 1024, true, true, false},
+    {"stream_rb512_scalar", 512, false, false, false},
+    {"row_per_lane", 0, false, false, false},
+    {"row_per_lane_xcd", 0, false, true, false},
+    {"pf_rb256", 256, true, false, true},
+    {"pf_rb512", 512, true, false, true},
+    {"pf_rb1024", 1024, true, false, true},
+    {"pf_rb256_xcd", 256, true, true, true},
+    {"pf_rb512_xcd", 512, true, true, true},
+    {"pf_rb1024_xcd", 1024, true, true, true},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -271,6 +431,17 @@ void launch_mode(const Variant &v, Params p, hipStream_t s)
         int per = (nb + 7) / 8;
         p.per_xcd = per;
         grid = per * 8;
+    }
+    if (v.pf) {
+        // quads per lane: 7 entries per row of the 7-point operator plus the
+        // alignment slack; wider rows take the unstaged path
+        switch (v.rows_per_block) {
+        case 256: k_pf<256, MODE, 2><<<grid, 256, 0, s>>>(p); break;
+        case 512: k_pf<512, MODE, 4><<<grid, 256, 0, s>>>(p); break;
+        default: k_pf<1024, MODE, 8><<<grid, 256, 0, s>>>(p); break;
+        }
+        check_launch();
+        return;
     }
     switch (v.rows_per_block) {
     case 256:

@@ -14,6 +14,14 @@ import oracle as po
 pytestmark = pytest.mark.gpu
 
 
+def _n_variants():
+    from piro_amd._lib import lib
+    return int(lib.piro_fe_variant_count())
+
+
+N_VARIANTS = _n_variants()
+
+
 @pytest.fixture(scope="module")
 def ctx():
     from piro_amd.core import Context
@@ -138,7 +146,7 @@ def _c1_setup(ctx):
     return dims, A, dt, x0, snap
 
 
-@pytest.mark.parametrize("variant", list(range(9)))
+@pytest.mark.parametrize("variant", list(range(N_VARIANTS)))
 def test_fe_c1_100_steps_all_variants_bit_exact(ctx, variant):
     print(f"case fe c1 variant={variant}", flush=True)
     dims, A, dt, x0, snap = _c1_setup(ctx)
@@ -167,13 +175,79 @@ def test_fe_snapshot_faces_and_random_inputs(ctx):
     x = ctx.alpha_uniform(N, 11, -1, 1)
     snap = ctx.alpha_uniform(N, 12, -1, 1)
     R = po.CSR(*A.host())
-    for v in range(9):
+    for v in range(N_VARIANTS):
         ctx.set_variant(v)
         y = host(ctx.fe_step_csr(dims, A, 1e-3, x, snap))
         assert y.tobytes() == po.fe_csr_step(dims, R, np.float32(1e-3), host(x), host(snap)).tobytes()
         y0 = host(ctx.fe_step_csr(dims, A, 1e-3, x, None))
         assert y0.tobytes() == po.fe_csr_step(dims, R, np.float32(1e-3), host(x), None).tobytes()
     ctx.set_variant(-1)
+
+
+def _random_csr(rng, n_rows, n_cols, long_rows=(), long_len=0):
+    lens = rng.integers(0, 12, n_rows)
+    lens[1000:1700] = 0   # whole row blocks without entries
+    for r in long_rows:
+        lens[r] = long_len
+    rp = np.zeros(n_rows + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    col = rng.integers(0, n_cols, int(rp[-1])).astype(np.int32)
+    val = rng.uniform(-1, 1, int(rp[-1])).astype(np.float32)
+    return po.CSR(rp, col, val)
+
+
+@pytest.mark.parametrize("long_len", [0, 9000])
+def test_generic_csr_modes_all_variants(ctx, long_len):
+    # arbitrary CSR (unsorted, ragged, empty rows; with long_len a few rows
+    # longer than any staged block, which takes the unstaged path)
+    rng = np.random.default_rng(5 + long_len)
+    n = 5000
+    R = _random_csr(rng, n, n, long_rows=(0, 777, 4999) if long_len else (), long_len=long_len)
+    A = ctx.csr_from_host(R.row_ptr, R.col, R.val)
+    xh = rng.uniform(-1, 1, n).astype(np.float32)
+    x = dev(xh, ctx)
+    dt = np.float32(0.01)
+    ref_all = po.spmv_allrows_step(R, dt, xh)
+    ref_mv = ref_mv_from(R, xh)
+    for v in range(N_VARIANTS):
+        print(f"case generic csr long_len={long_len} variant={v}", flush=True)
+        ctx.set_variant(v)
+        try:
+            got = host(ctx.fe_step_allrows(A, dt, x))
+            mv = host(ctx.spmv(A, x))
+        finally:
+            ctx.set_variant(-1)
+        assert got.tobytes() == ref_all.tobytes(), v
+        assert mv.tobytes() == ref_mv.tobytes(), v
+
+
+def ref_mv_from(R, xh):
+    # sequential per-row sum of fl(v * x[c]), the order of every variant
+    y = np.zeros(R.n_rows, np.float32)
+    for r in range(R.n_rows):
+        acc = np.float32(0)
+        for k in range(R.row_ptr[r], R.row_ptr[r + 1]):
+            acc = np.float32(acc + np.float32(R.val[k] * xh[R.col[k]]))
+        y[r] = acc
+    return y
+
+
+def test_jli_identical_across_variants(ctx):
+    dims = (24, 20, 18)
+    norm = po.lap_norm(1.0, 22)
+    A = ctx.laplacian_csr(*dims, 3, norm)
+    b = ctx.ic_gaussian(dims, [1.0] * 3, [0.5] * 3, [0.15] * 3, 1.0)
+    dt = np.float32(0.2 * float(norm))
+    outs = []
+    for v in range(N_VARIANTS):
+        print(f"case jli variant={v}", flush=True)
+        ctx.set_variant(v)
+        try:
+            x, it, res = ctx.jli_solve(dims, A, dt, b, tol=1e-6, max_iter=400)
+        finally:
+            ctx.set_variant(-1)
+        outs.append((host(x).tobytes(), it))
+    assert all(o == outs[0] for o in outs)
 
 
 def test_fe_256cubed_two_steps_bit_exact(ctx):
@@ -312,3 +386,15 @@ def test_laplacian2d_bit_exact(ctx, n):
     np.testing.assert_array_equal(rp, R.row_ptr)
     np.testing.assert_array_equal(col, R.col)
     assert val.tobytes() == R.val.tobytes()
+
+
+@pytest.mark.parametrize("n,offset", [(1, 0), (4, 0), (1003, 0), (1 << 20, 0), (4099, 1), (4099, 3)])
+def test_vec_copy(ctx, n, offset):
+    print(f"case copy n={n} offset={offset}", flush=True)
+    rng = np.random.default_rng(n)
+    src = dev(rng.standard_normal(n + offset).astype(np.float32), ctx)
+    dst = ctx.full(n + offset + 5, -7.0)
+    ctx.copy(src.view(offset, offset + n), dst.view(offset, offset + n))
+    got = host(dst)
+    assert got[offset:offset + n].tobytes() == host(src)[offset:offset + n].tobytes()
+    assert np.all(got[:offset] == -7.0) and np.all(got[offset + n:] == -7.0)
