@@ -6,9 +6,11 @@
  *   - Every function returns an int status (PIRO_OK = 0).  On failure
  *     piro_last_error() returns a message for the calling thread.  Nothing
  *     exits or throws across this boundary.
- *   - Array arguments named d_* are DEVICE pointers (hipMalloc'd by the
- *     caller, by piro_malloc, or torch tensors' data pointers); h_* are host
- *     pointers.  Indices are int32 (as in the reference), fields FP32.
+ *   - Array arguments named d_* are DEVICE pointers (from piro_malloc or
+ *     from hipMalloc of the same HIP runtime this library links, /opt/rocm;
+ *     a second runtime in the process, such as the one a PyTorch wheel
+ *     bundles, does not share devices with it); h_* are host pointers.
+ *     Indices are int32 (as in the reference), fields FP32.
  *   - Work is enqueued on the context's stream in order; call
  *     piro_ctx_synchronize() before reading results on the host.  A context is
  *     not thread-safe; use one per host thread.
@@ -340,6 +342,7 @@ typedef struct piro_batch piro_batch;
 int piro_batch_create(piro_ctx *ctx, int n_inst, const int *h_dims,
                       const float *d_alpha, const float *h_dt,
                       piro_batch **out);
+/* destroy every batch before the context it was created on */
 int piro_batch_destroy(piro_batch *b);
 /* total rows, nnz and the n_inst+1 row offsets of the instances */
 int piro_batch_info(piro_batch *b, int64_t *total_rows, int64_t *nnz,
@@ -377,6 +380,7 @@ int piro_batch_run(piro_batch *b, float *d_x, float *d_work, int nsteps,
 typedef struct piro_surrogate piro_surrogate;
 int piro_surrogate_num_params(void);
 int piro_surrogate_create(piro_ctx *ctx, uint64_t seed, piro_surrogate **out);
+/* destroy every surrogate before the context it was created on */
 int piro_surrogate_destroy(piro_surrogate *s);
 int piro_surrogate_get_params(piro_surrogate *s, float *h_params);
 int piro_surrogate_set_params(piro_surrogate *s, const float *h_params);

@@ -9,6 +9,7 @@ devices.)
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -129,9 +130,16 @@ class Context:
         h = C.c_void_p()
         check(lib.piro_ctx_create(device, C.byref(h)))
         self.h = h
+        # batches and surrogates hold the C context: they are destroyed first
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, child):
+        self._children.add(child)
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                child.close()
             check(lib.piro_ctx_destroy(self.h))
             self.h = None
 
@@ -427,6 +435,7 @@ class Batch:
         h = C.c_void_p()
         check(lib.piro_batch_create(ctx.h, n, flat, _p(alpha), dt, C.byref(h)))
         self.h = h
+        ctx._adopt(self)
         tot, nnz = C.c_int64(), C.c_int64()
         offs = (C.c_int64 * (n + 1))()
         check(lib.piro_batch_info(self.h, C.byref(tot), C.byref(nnz), offs))
@@ -434,9 +443,11 @@ class Batch:
         self.offsets = list(offs)
 
     def close(self):
-        if self.h:
+        # the context closes its children before itself, so a live handle
+        # always has a live context here
+        if self.h and self.ctx.h:
             check(lib.piro_batch_destroy(self.h))
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -473,12 +484,15 @@ class Surrogate:
         h = C.c_void_p()
         check(lib.piro_surrogate_create(ctx.h, int(seed), C.byref(h)))
         self.h = h
+        ctx._adopt(self)
         self.n_params = lib.piro_surrogate_num_params()
 
     def close(self):
-        if self.h:
+        # the context closes its children before itself, so a live handle
+        # always has a live context here
+        if self.h and self.ctx.h:
             check(lib.piro_surrogate_destroy(self.h))
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -69,13 +69,15 @@ def ref_grads(p, u, al, target, scale):
     dd = np.where(m, 2.0 * (delta - target) / n_int, 0.0)
     loss = float(np.sum(np.where(m, (delta - target) ** 2, 0.0)) / n_int)
     g = {}
+    # the device stores each ReLU-masked upstream gradient in BF16 before the
+    # next layer's data and weight gradients use it
     dx, g["w6"], g["b6"] = po.conv3d_bwd_f64(acts[4], w6, dd[..., None])
-    dz = dx * (acts[4] > 0)
+    dz = bf16((dx * (acts[4] > 0)).astype(np.float32)).astype(np.float64)
     for l in range(3, -1, -1):
         w, b = mids[l]
         wb = bf16(w.astype(np.float32)).astype(np.float64)
         dx, g[f"w{l + 2}"], g[f"b{l + 2}"] = po.conv3d_bwd_f64(acts[l], wb, dz)
-        dz = dx * (acts[l] > 0)
+        dz = bf16((dx * (acts[l] > 0)).astype(np.float32)).astype(np.float64)
     _, g["w1"], g["b1"] = po.conv3d_bwd_f64(x_in, w1, dz, need_dx=False)
     flat = np.concatenate([g["w1"].ravel(), g["b1"]] +
                           sum([[g[f"w{l}"].ravel(), g[f"b{l}"]] for l in range(2, 6)], []) +
@@ -160,18 +162,21 @@ def test_gradients_match_oracle(ctx, shape):
                             ctx.from_host(target.ravel()), scale)
     gr, lr, _ = ref_grads(p, u, al, target.astype(np.float64), scale)
     assert loss == pytest.approx(lr, rel=1e-2)
-    # per parameter block (BF16 activation gradients: a few 1e-3 per layer)
+    # per parameter block; the reference rounds at the device's BF16 points,
+    # so what remains is FP32 accumulation order and rare rounding flips
     offs = [0, NT * CH * 2, NT * CH * 2 + CH]
     for _ in range(4):
         offs += [offs[-1] + NT * CH * CH, offs[-1] + NT * CH * CH + CH]
     offs += [offs[-1] + NT * CH, offs[-1] + NT * CH + 1]
     names = ["w1", "b1"] + sum([[f"w{l}", f"b{l}"] for l in range(2, 6)], []) + ["w6", "b6"]
+    rels = {}
     for i, nm in enumerate(names):
         a, b = g[offs[i]:offs[i + 1]].astype(np.float64), gr[offs[i]:offs[i + 1]]
-        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
-        print(f"grad {nm}: rel L2 {rel:.2e}")
-        assert rel < 3e-2, nm
+        rels[nm] = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+    print("grad rel L2: " + ", ".join(f"{k} {v:.2e}" for k, v in rels.items()))
     sur.close()
+    bad = {k: v for k, v in rels.items() if not v < 5e-3}
+    assert not bad, bad
 
 
 def test_training_reduces_loss(ctx):
@@ -196,7 +201,8 @@ def test_training_reduces_loss(ctx):
     du = ctx.from_host(u.ravel())
     da = ctx.from_host(a.ravel())
     dt_ = ctx.from_host(t.ravel().astype(np.float32))
-    losses = [sur.train_step(shape, du, da, dt_, 1.0, 3e-3) for _ in range(150)]
-    print(f"surrogate training: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
-    assert losses[-1] < 0.2 * losses[0]
+    losses = [sur.train_step(shape, du, da, dt_, 1.0, 3e-3) for _ in range(400)]
     sur.close()
+    print("surrogate training loss: " + ", ".join(f"{i}: {losses[i]:.3e}" for i in (0, 50, 100, 150, 200, 300, 399)))
+    assert np.all(np.isfinite(losses))
+    assert np.mean(losses[-10:]) < 0.5 * losses[0]
