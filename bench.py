@@ -27,7 +27,7 @@ METRIC = "cell-steps/s + SpMV HBM GB/s (256³ heat); surrogate vs plain speedup 
 HBM_PEAK_GBS = 8000.0
 DIMS = (256, 256, 256)
 SEED = 20261203
-COPY_PROBE_FLOATS = 64 << 20   # 256 MiB read + 256 MiB written per launch
+COPY_PROBE_FLOATS = 64 << 20   # 256 MiB read + 256 MiB written per copy launch
 
 
 def algorithmic_bytes(n_rows: int, nnz: int) -> int:
@@ -90,9 +90,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=-1, help="force an FE kernel variant (-1 = autotune)")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="after the timed region, run the library's streaming copy on a 256 MiB vector 3 times "
-                         "(the known-size kernel that HBM counter readings are calibrated against)")
+    ap.add_argument("--copy-reps", type=int, default=10,
+                    help="after the timed region, time the library's streaming copy on a 256 MiB vector this many "
+                         "times (the box's practical HBM rate, and the known-size kernel that HBM counter "
+                         "readings are calibrated against); 0 skips it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are out of scope "
@@ -143,12 +144,17 @@ def main() -> int:
     byts = algorithmic_bytes(n_rows, nnz)
     gbs = byts / (ms_step / 1e3) / 1e9
     finite = bool(np.isfinite(ctx.reduce("sumsq", x)))
-    if args.copy_probe:
+    copy_gbs = None
+    if args.copy_reps > 0:
         src = ctx.full(COPY_PROBE_FLOATS, 1.0)
         dst = ctx.empty_like(src)
-        for _ in range(3):
-            ctx.copy(src, dst)
+        ctx.copy(src, dst)
         ctx.sync()
+        ctx.timer_start()
+        for _ in range(args.copy_reps):
+            ctx.copy(src, dst)
+        cms = ctx.timer_stop()
+        copy_gbs = 8.0 * COPY_PROBE_FLOATS * args.copy_reps / (cms / 1e3) / 1e9
         src.free()
         dst.free()
 
@@ -176,7 +182,7 @@ def main() -> int:
         "interior_cell_steps_per_s": interior * args.steps / (ms / 1e3),
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS, "traffic": load_traffic(vname),
-                     "algorithmic_bytes_per_launch": byts},
+                     "algorithmic_bytes_per_launch": byts, "measured_copy_gbs": copy_gbs},
         "cpu_baseline": base,
         "field_finite": finite,
         "host_wall_s": t_host,

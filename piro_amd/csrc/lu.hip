@@ -624,6 +624,7 @@ int piro_lu_solve_refine(piro_ctx *ctx, int n, const float *d_a, int lda, const 
             double rr = std::sqrt(h[2 * c]), bb = std::sqrt(h[2 * c + 1]);
             double rel = bb > 0 ? rr / bb : rr;
             if (h_relres) h_relres[c] = rel;
+            if (!std::isfinite(rel)) throw Error(PIRO_ENOTCONV, "non-finite residual in refinement");
             worst = std::max(worst, rel);
         }
         if (worst <= tol || sw >= max_sweeps) break;

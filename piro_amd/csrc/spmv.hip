@@ -85,6 +85,7 @@ struct Params {
     Grid3 g;
     int n_blocks;        // logical blocks
     int per_xcd;         // XCD remap: logical blocks per XCD slot (0 = off)
+    int chunk;           // chunked XCD remap: runs of `chunk` logical blocks per XCD (0 = off)
 };
 
 template <int MODE>
@@ -131,10 +132,21 @@ __device__ __forceinline__ double finish_row(const Params &p, int r, float acc, 
     }
 }
 
+// Workgroups are dealt to the 8 XCDs round robin (workgroup b runs on XCD
+// b % 8).  per_xcd: XCD j takes one contiguous range of logical blocks.
+// chunk C: XCD j takes logical runs [(8q + j) C, (8q + j + 1) C), so the
+// device still streams the arrays front to back while blocks that share x
+// lines (row neighbours, and plane neighbours when C divides the blocks per
+// plane / 8) sit behind the same L2.
 __device__ __forceinline__ int logical_block(const Params &p)
 {
     int b = blockIdx.x;
-    if (p.per_xcd > 0) b = (b & 7) * p.per_xcd + (b >> 3);
+    if (p.per_xcd > 0) {
+        b = (b & 7) * p.per_xcd + (b >> 3);
+    } else if (p.chunk > 0) {
+        const int k = b >> 3;
+        b = ((k / p.chunk) * 8 + (b & 7)) * p.chunk + k % p.chunk;
+    }
     return b;
 }
 
@@ -397,9 +409,10 @@ struct Variant {
     bool vec;
     bool xcd;
     bool pf;              // prefetching stream kernel
+    int chunk;            // chunked XCD remap (0 = off)
 };
 
-const Variant kVariants[] = {
+constexpr Variant kVariants[] = {
     {"stream_rb256_vec", 256, true, false, false},
     {"stream_rb512_vec", 512, true, false, false},
     {"stream_rb1024_vec", 1024, true, false, false},
@@ -415,8 +428,17 @@ const Variant kVariants[] = {
     {"pf_rb256_xcd", 256, true, true, true},
     {"pf_rb512_xcd", 512, true, true, true},
     {"pf_rb1024_xcd", 1024, true, true, true},
+    {"pf_rb256_c8", 256, true, false, true, 8},
+    {"pf_rb256_c32", 256, true, false, true, 32},
+    {"pf_rb512_c16", 512, true, false, true, 16},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+constexpr int kDefaultVariant = 9;   // pf_rb256
+constexpr int kScalarVariant = 6;    // stream_rb512_scalar: no 16-byte alignment needed
+static_assert(kVariants[kDefaultVariant].pf && kVariants[kDefaultVariant].rows_per_block == 256 &&
+                  !kVariants[kDefaultVariant].xcd && kVariants[kDefaultVariant].chunk == 0,
+              "default variant");
+static_assert(!kVariants[kScalarVariant].vec, "scalar fallback variant");
 
 template <int MODE>
 void launch_mode(const Variant &v, Params p, hipStream_t s)
@@ -427,10 +449,15 @@ void launch_mode(const Variant &v, Params p, hipStream_t s)
     p.n_blocks = nb;
     int grid = nb;
     p.per_xcd = 0;
+    p.chunk = 0;
     if (v.xcd) {
         int per = (nb + 7) / 8;
         p.per_xcd = per;
         grid = per * 8;
+    } else if (v.chunk > 0) {
+        const int64_t span = 8LL * v.chunk;
+        p.chunk = v.chunk;
+        grid = (int)((nb + span - 1) / span * span);
     }
     if (v.pf) {
         // quads per lane: 7 entries per row of the 7-point operator plus the
@@ -497,8 +524,9 @@ Params make_params(int nx, int ny, int nz, int n_rows, const int32_t *rp, const 
 
 int pick_variant(piro_ctx *ctx, const Params &p)
 {
-    int v = ctx->fe_variant >= 0 ? ctx->fe_variant : 1;
-    if (kVariants[v].vec && !vec_ok(p.col, p.val)) v = 6;
+    // untuned default: pf_rb256, the fastest measured on 7-point operators
+    int v = ctx->fe_variant >= 0 ? ctx->fe_variant : kDefaultVariant;
+    if (kVariants[v].vec && !vec_ok(p.col, p.val)) v = kScalarVariant;
     return v;
 }
 

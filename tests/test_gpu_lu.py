@@ -138,3 +138,19 @@ def test_c3_mixed_precision_refinement(ctx, nrhs):
     assert max(rel) <= 1e-10
     assert po.rel_l2(xh, xr) < 1e-9
     assert sweeps <= 4
+
+
+def test_refine_rejects_non_finite_residual(ctx):
+    # "factors" that are A itself (never factored): the FP32 substitution with
+    # A's -2 sub-diagonals as a unit-lower L overflows, and the refinement must
+    # report it instead of treating the NaN residual as converged
+    print("case refine non-finite", flush=True)
+    from piro_amd._lib import PiroError
+    from piro_amd.core import DevCSR
+    A, L, I, beta = c3_system()
+    n = A.n_rows
+    dense = ctx.csr_to_dense(DevCSR(dev(A.row_ptr, ctx), dev(A.col, ctx), dev(A.val, ctx)))
+    ipiv = dev(np.arange(n, dtype=np.int32), ctx)
+    b = c3_rhs(64, 1)
+    with pytest.raises(PiroError, match="non-finite"):
+        ctx.lu_solve_refine(dense, dense.clone(), n, ipiv, dev(b, ctx), tol=1e-10, max_sweeps=3)

@@ -85,6 +85,46 @@ def ref_grads(p, u, al, target, scale):
     return flat, loss, delta
 
 
+def param_blocks():
+    """name -> [start, stop) in the flat parameter vector"""
+    offs = [0, NT * CH * 2, NT * CH * 2 + CH]
+    for _ in range(4):
+        offs += [offs[-1] + NT * CH * CH, offs[-1] + NT * CH * CH + CH]
+    offs += [offs[-1] + NT * CH, offs[-1] + NT * CH + 1]
+    names = ["w1", "b1"] + sum([[f"w{l}", f"b{l}"] for l in range(2, 6)], []) + ["w6", "b6"]
+    return {nm: (offs[i], offs[i + 1]) for i, nm in enumerate(names)}
+
+
+def dyadic_network(rng):
+    """Sparse parameters for which every value of the forward and backward
+    passes is exactly representable: two +-1 weights per output channel
+    (mid layers: from two distinct input channels, so each input channel
+    feeds at most two outputs; mostly +1 so that activations stay alive),
+    biases 0 or 1, eight output weights +-1/8."""
+    blk = param_blocks()
+    p = np.zeros(max(b for _, b in blk.values()), np.float32)
+    sign = lambda: rng.choice([1.0, 1.0, 1.0, -1.0])
+    w1 = np.zeros((NT, CH, 2), np.float32)
+    for co in range(CH):
+        for tp in rng.choice(NT, 2, replace=False):
+            w1[tp, co, 0] = sign()
+    p[slice(*blk["w1"])] = w1.ravel()
+    p[slice(*blk["b1"])] = rng.choice([0.0, 1.0], CH)
+    for l in range(2, 6):
+        w = np.zeros((NT, CH, CH), np.float32)
+        p1, p2 = rng.permutation(CH), rng.permutation(CH)
+        for co in range(CH):
+            for ci in {p1[co], p2[co]}:
+                w[rng.integers(NT), co, ci] = sign()
+        p[slice(*blk[f"w{l}"])] = w.ravel()
+        p[slice(*blk[f"b{l}"])] = rng.choice([0.0, 1.0], CH)
+    w6 = np.zeros((NT, CH), np.float32)
+    for ci in rng.choice(CH, 8, replace=False):
+        w6[rng.integers(NT), ci] = rng.choice([-0.125, 0.125])
+    p[slice(*blk["w6"])] = w6.ravel()
+    return p
+
+
 @pytest.fixture(scope="module")
 def ctx():
     from piro_amd.core import Context
@@ -162,21 +202,58 @@ def test_gradients_match_oracle(ctx, shape):
                             ctx.from_host(target.ravel()), scale)
     gr, lr, _ = ref_grads(p, u, al, target.astype(np.float64), scale)
     assert loss == pytest.approx(lr, rel=1e-2)
-    # per parameter block; the reference rounds at the device's BF16 points,
-    # so what remains is FP32 accumulation order and rare rounding flips
-    offs = [0, NT * CH * 2, NT * CH * 2 + CH]
-    for _ in range(4):
-        offs += [offs[-1] + NT * CH * CH, offs[-1] + NT * CH * CH + CH]
-    offs += [offs[-1] + NT * CH, offs[-1] + NT * CH + 1]
-    names = ["w1", "b1"] + sum([[f"w{l}", f"b{l}"] for l in range(2, 6)], []) + ["w6", "b6"]
-    rels = {}
-    for i, nm in enumerate(names):
-        a, b = g[offs[i]:offs[i + 1]].astype(np.float64), gr[offs[i]:offs[i + 1]]
+    # Per parameter block.  The reference rounds at the device's BF16 points,
+    # but a BF16 pipeline is chaotic: perturbing the FP64 pipeline's own conv
+    # outputs by 1e-6..1e-5 relative (FP32-accumulation scale) moves its
+    # gradients by 0.03-4% (w6 ... w1) on these shapes, the size of the
+    # differences seen here.  So this test bounds them at the level that
+    # separates rounding from a layout or indexing error (which gives O(1));
+    # test_gradients_exact_on_dyadic_network pins the kernels exactly.
+    rels, coss = {}, {}
+    for nm, (o0, o1) in param_blocks().items():
+        a, b = g[o0:o1].astype(np.float64), gr[o0:o1]
         rels[nm] = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        coss[nm] = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-300))
     print("grad rel L2: " + ", ".join(f"{k} {v:.2e}" for k, v in rels.items()))
     sur.close()
-    bad = {k: v for k, v in rels.items() if not v < 5e-3}
+    bad = {k: (rels[k], coss[k]) for k in rels if not (rels[k] < 0.1 and coss[k] > 0.995)}
     assert not bad, bad
+
+
+def test_gradients_exact_on_dyadic_network(ctx):
+    # integer inputs, alpha = 0.0125 (normalised input 0), scale 1, 4^3
+    # interior cells (the loss scale 2/64 is a power of two) and targets
+    # within +-1/8 of the output: every activation is a small integer, every
+    # upstream gradient a small multiple of 2^-11 -- all exact in BF16 -- and
+    # every sum exact in FP32.  Device and FP64 reference must agree exactly.
+    from piro_amd.core import Surrogate
+    shape = (1, 6, 6, 6)
+    for seed in range(3):
+        print(f"case surrogate exact seed={seed}", flush=True)
+        rng = np.random.default_rng(100 + seed)
+        p = dyadic_network(rng)
+        u = rng.integers(0, 3, shape).astype(np.float32)
+        al = np.full(shape, 0.0125, np.float32)
+        _, acts, delta = ref_forward(p, u, al, 1.0)
+        assert max(float(a.max()) for a in acts) < 256          # integers exact in BF16
+        target = (delta + rng.choice([-0.125, 0.0, 0.125], shape)).astype(np.float32)
+        assert np.array_equal(target.astype(np.float64) - delta, np.round((target - delta) * 8) / 8)
+        sur = Surrogate(ctx, seed=1)
+        sur.set_params(p)
+        out = sur.forward(shape, ctx.from_host(u.ravel()), ctx.from_host(al.ravel()), 1.0).host().reshape(shape)
+        m = interior_mask(shape)
+        np.testing.assert_array_equal(out[m].astype(np.float64), (u.astype(np.float64) + delta)[m])
+        g, loss = sur.gradients(shape, ctx.from_host(u.ravel()), ctx.from_host(al.ravel()),
+                                ctx.from_host(target.ravel()), 1.0)
+        sur.close()
+        gr, lr, _ = ref_grads(p, u, al, target.astype(np.float64), 1.0)
+        assert loss == pytest.approx(lr, rel=1e-12, abs=1e-15)
+        for nm, (o0, o1) in param_blocks().items():
+            a, b = g[o0:o1].astype(np.float64), gr[o0:o1]
+            assert np.array_equal(a, b), (seed, nm, np.abs(a - b).max(), np.abs(b).max())
+        for nm, (o0, o1) in param_blocks().items():
+            if nm != "b6":
+                assert np.abs(gr[o0:o1]).max() > 0, (seed, nm)   # every block is exercised
 
 
 def test_training_reduces_loss(ctx):
