@@ -10,12 +10,21 @@
 //   Piro::CSR::stencils::gradient_{cd,uw,dw}_*  src/stencils.cpp:7-231
 //   Piro::methods::sethash                      src/methods.cpp:74-118
 //   Piro::calculategaussian                     src/ic.cpp:16-27
+//   Piro::bc::prepbc + Piro::paraview::writevth src/bc.cpp:80-105,
+//     (+ writevti, writefile)                    src/paraview.cpp:12-137,
+//                                                src/fileutilities.cpp:24-40
 //
 // Usage:
 //   piro_ref_harness lap  <scheme 1|2|3> nx ny nz norm_hex out.bin
 //   piro_ref_harness grad <scheme> <dir 0|1|2> nx ny nz norm_hex out.bin
 //   piro_ref_harness ht   <scheme> nx ny nz norm_hex table_size out.bin
 //   piro_ref_harness gauss nx ny nz lx ly lz mx my mz sx sy sz out.bin
+//   piro_ref_harness vtk  nx ny nz n_scal n_vec step in.bin out_dir
+//     in.bin: n_scal fields of nx*ny*nz floats, then n_vec fields of
+//     3*nx*ny*nz floats (raw little-endian); fields are named S0.., V0..
+//     after one constant "Alpha"; writes out_dir/case/mesh/mesh_<step>.vth
+//     and out_dir/case/mesh/level/<step>_level_0.vti with the reference's
+//     own writer
 // norm_hex is the IEEE-754 bit pattern of the float norm (so no decimal
 // round trip can perturb it).
 #include <cstdint>
@@ -25,8 +34,13 @@
 #include <string>
 #include <vector>
 
+#include <filesystem>
+
+#include <bc.hpp>
 #include <datatypes.hpp>
+#include <fileutilities.hpp>
 #include <ic.hpp>
+#include <postprocess.hpp>
 #include <methods.hpp>
 #include <stencils.hpp>
 
@@ -130,6 +144,50 @@ int main(int argc, char **argv)
         FILE *f = fopen(argv[14], "wb");
         put(f, out);
         fclose(f);
+        return 0;
+    }
+    if (cmd == "vtk" && argc == 10) {
+        std::vector<uint> n = {(uint)atoi(argv[2]), (uint)atoi(argv[3]), (uint)atoi(argv[4])};
+        const int n_scal = atoi(argv[5]), n_vec = atoi(argv[6]), step = atoi(argv[7]);
+        const size_t N = (size_t)n[0] * n[1] * n[2];
+        Piro::MeshParams &MP = Piro::MeshParams::getInstance();
+        Piro::SolveParams &SP = Piro::SolveParams::getInstance();
+        MP.setvalue(Piro::MeshParams::num_cells, n);
+        MP.setvalue(Piro::MeshParams::VECTORNUM, n_vec);
+        MP.setvalue(Piro::MeshParams::LEVELS, 1);
+        MP.setvalue(Piro::MeshParams::INDEX, std::vector<int>{1});
+        SP.setvalue(Piro::SolveParams::CASENAME, std::string("case"));
+        Piro::AMR amr;
+        for (int d = 0; d < 3; ++d) {
+            amr.WholeExtent[2 * d] = 0;
+            amr.WholeExtent[2 * d + 1] = (int)n[d];
+            amr.Origin[d] = 0;
+            amr.Spacing[d] = 1;
+        }
+        FILE *f = fopen(argv[8], "rb");
+        if (!f) return 3;
+        Piro::CellData cd;
+        cd.Scalars = "Alpha";
+        cd.type = 0;
+        cd.values.assign(N, 0.1f);
+        amr.CD.push_back(cd);
+        for (int i = 0; i < n_scal + n_vec; ++i) {
+            const bool vec = i >= n_scal;
+            cd.Scalars = vec ? "V" + std::to_string(i - n_scal) : "S" + std::to_string(i);
+            cd.type = vec ? 2 : 1;
+            cd.values.assign(vec ? 3 * N : N, 0.0f);
+            if (fread(cd.values.data(), sizeof(float), cd.values.size(), f) != cd.values.size()) return 3;
+            amr.CD.push_back(cd);
+        }
+        fclose(f);
+        // the three operator slots preprocess appends (src/preprocess.cpp:197-202)
+        Piro::CellData op;
+        for (int i = 0; i < 3; ++i) amr.CD.push_back(op);
+        MP.setvalue(Piro::MeshParams::AMR, std::vector<Piro::AMR>{amr});
+        Piro::bc::prepbc();
+        Piro::file_utilities::current_path = argv[9];
+        std::filesystem::create_directories(std::string(argv[9]) + "/case/mesh/level");
+        Piro::paraview::writevth(step);
         return 0;
     }
     fprintf(stderr, "bad arguments\n");
