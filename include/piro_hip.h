@@ -415,6 +415,21 @@ int piro_case_run(const char *assets_dir, const char *case_name,
                   const char *out_dir, int max_steps, int alpha_row_scaled,
                   int *steps_done, float *h_field, int64_t field_len);
 
+/* Replaces Piro::post::export_paraview -> paraview::writevth/writevti
+ * (src/postprocess.cpp:6-8, src/paraview.cpp:12-137) for host data: writes
+ * <out_dir>/<case_name>/mesh/mesh_<step>.vth and
+ * <out_dir>/<case_name>/mesh/level/<step>_level_0.vti, byte-compatible with
+ * the reference (interior cells, 6-significant-digit text, "Float64" label,
+ * one level and one block).  h_dims3 are padded dims; h_origin3/h_spacing3
+ * the integer [Mesh] o and s (NULL: 0 0 0 and 1 1 1).  Field k has name
+ * names[k] and type types[k]: 0 constant (skipped), 1 scalar (N floats),
+ * 2 vector (3N floats, written as <name>_x/_y/_z).  Host-only: no GPU. */
+int piro_paraview_write(const char *out_dir, const char *case_name, int step,
+                        const int *h_dims3, const int *h_origin3,
+                        const int *h_spacing3, int n_fields,
+                        const char *const *names, const int *types,
+                        const float *const *h_fields);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,32 @@ class IniReader {
     bool ok_ = false;
 };
 
+// ------------------------------------------------------------ Paraview
+// src/paraview.cpp:12-137 with one level and one block ([Mesh] levels = 1,
+// index = 1; the reference's writer indexes past its cell data otherwise).
+
+struct VtkGrid {
+    int n[3] = {0, 0, 0};        // padded dims
+    int origin[3] = {0, 0, 0};   // [Mesh] o, truncated to int as the reference's AMR struct stores it
+    int spacing[3] = {1, 1, 1};  // [Mesh] s, likewise
+};
+struct VtkField {
+    std::string name;
+    int type = 1;                // 0 constant (never written), 1 scalar (N values), 2 vector (3N values)
+    const float *data = nullptr; // host
+};
+// the .vti text: interior cells only, each scalar and each vector component
+// as one ascii "Float64" DataArray of the values printed as std::ostream
+// prints a float
+std::string writevti(const VtkGrid &g, const std::vector<VtkField> &fields);
+// the .vth index pointing at level/<step>_level_0.vti
+std::string writevth(const VtkGrid &g, int step);
+// writes <out_dir>/<casename>/mesh/mesh_<step>.vth and
+// <out_dir>/<casename>/mesh/level/<step>_level_0.vti (each followed by a
+// newline, src/fileutilities.cpp:24-40), creating the folders
+void write_paraview(const std::string &out_dir, const std::string &casename, int step, const VtkGrid &g,
+                    const std::vector<VtkField> &fields);
+
 class Case;
 
 // A device field slot (the reference's CLBuffer with its `ind`).
@@ -127,12 +153,18 @@ class Case {
     int n[3] = {0, 0, 0};       // padded dims
     int64_t N = 0;
     float l[3] = {0, 0, 0};
+    VtkGrid grid;               // dims, [Mesh] o and s for the Paraview files
     float timestep = 0, totaltime = 0;
     int save = 1, simindex = 0, time_scheme = 11, space_scheme = 3, solver_scheme = 0, data_type = 0;
     std::string casename;
     std::vector<std::string> names;   // slot names (constants, scalars, vectors)
     std::vector<int> types;           // 0 constant, 1 scalar, 2 vector
     std::vector<float *> fields;      // device data per slot
+    // host values per slot as the reference's CellData keeps them: the
+    // initial condition with the BC face values written into the slot each
+    // BC property names; the Paraview files print slot 1 from the device
+    // and every other slot from here
+    std::vector<std::vector<float>> host_values;
     float *snapshot = nullptr;        // BC snapshot (field slot 1 with face values)
     float *work = nullptr;            // step output buffer
     float *work2 = nullptr;
@@ -159,8 +191,5 @@ class Case {
     friend struct Field;
 };
 
-// paraview text of one level (src/paraview.cpp:12-103): interior cells only
-std::string writevti(const Case &c, const std::vector<std::vector<float>> &host_fields);
-std::string writevth(const Case &c, int step);
 
 }  // namespace piro

@@ -99,6 +99,8 @@ _SIGS = {
     "piro_batch_step": [vp, vp, vp],
     "piro_batch_run": [vp, vp, vp, i32, i32],
     "piro_case_run": [C.c_char_p, C.c_char_p, C.c_char_p, i32, i32, P_i32, P_f32, i64],
+    "piro_paraview_write": [C.c_char_p, C.c_char_p, i32, P_i32, P_i32, P_i32, i32, C.POINTER(C.c_char_p), P_i32,
+                            C.POINTER(vp)],
     "piro_surrogate_create": [vp, u64, C.POINTER(vp)],
     "piro_surrogate_destroy": [vp],
     "piro_surrogate_get_params": [vp, P_f32],

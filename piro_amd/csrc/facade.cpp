@@ -169,6 +169,15 @@ Case::Case(const std::string &assets_dir, const std::string &case_name, const Ca
         l[d] = ll[d];
     }
     N = (int64_t)n[0] * n[1] * n[2];
+    // [Mesh] o and s land in the AMR struct's int fields (src/preprocess.cpp:95-101)
+    auto oo = to_vec<float>(reader.get("Mesh", "o", "0 0 0"));
+    auto ss = to_vec<float>(reader.get("Mesh", "s", "1 1 1"));
+    if (oo.size() != 3 || ss.size() != 3) throw SolverError("[Mesh] o and s need 3 values");
+    for (int d = 0; d < 3; ++d) {
+        grid.n[d] = n[d];
+        grid.origin[d] = (int)oo[d];
+        grid.spacing[d] = (int)ss[d];
+    }
     if (solver_scheme == 40) {
         IniReader rs(assets_dir + "/solver.ini");
         jli_tol = stof_req(rs, "JLI", "tolerance");
@@ -277,6 +286,7 @@ void Case::init_fields(const IniReader &reader, const std::string &assets_dir)
         std::vector<float> h(N, i < (int)cvals.size() ? cvals[i] : 0.0f);
         ck(piro_memcpy_h2d(ctx_, d, h.data(), (size_t)N * sizeof(float)));
         fields.push_back(d);
+        host_values.push_back(std::move(h));
     }
     for (int i = 0; i < n_scal; ++i) {
         names.push_back(i < (int)snames.size() ? snames[i] : "");
@@ -285,6 +295,7 @@ void Case::init_fields(const IniReader &reader, const std::string &assets_dir)
         float *d = (float *)alloc((size_t)N * 3 * sizeof(float));
         ck(piro_memcpy_h2d(ctx_, d, h.data(), (size_t)N * sizeof(float)));  // first N values
         fields.push_back(d);
+        host_values.push_back(std::move(h));
     }
     for (int i = 0; i < n_vec; ++i) {
         names.push_back(i < (int)vnames.size() ? vnames[i] : "");
@@ -294,6 +305,7 @@ void Case::init_fields(const IniReader &reader, const std::string &assets_dir)
         float *d = (float *)alloc((size_t)N * 3 * sizeof(float));
         ck(piro_memcpy_h2d(ctx_, d, h.data(), (size_t)N * 3 * sizeof(float)));
         fields.push_back(d);
+        host_values.push_back(std::move(h));
     }
     if (fields.size() < 2) throw SolverError("the case needs at least one constant and one scalar");
 }
@@ -309,12 +321,16 @@ void Case::init_bc(const std::string &assets_dir)
     auto props = split(r.get("BC", "property"), ' ');
     auto vals = to_vec<float>(r.get("BC", "values"));
     if (props.size() < 6 || vals.size() < 6) throw SolverError("setup.ini [BC] needs 6 properties and values");
-    std::vector<float> snap(N);
-    ck(piro_memcpy_d2h(ctx_, snap.data(), fields[1], (size_t)N * sizeof(float)));
+    // bc::setbc writes each face's value into the host values of the slot
+    // its property names (slot 0 when the name is unknown); the snapshot is
+    // slot 1's host values afterwards (bc::opencl_initBC)
     auto lists = face_lists(n);
-    for (int f = 0; f < 6; ++f)
-        if (match(props[f]) == 1)
-            for (int idx : lists[f]) snap[idx] = vals[f];
+    for (int f = 0; f < 6; ++f) {
+        std::vector<float> &hv = host_values[match(props[f])];
+        for (int idx : lists[f])
+            if (idx < (int64_t)hv.size()) hv[idx] = vals[f];
+    }
+    std::vector<float> snap(host_values[1].begin(), host_values[1].begin() + N);
     snapshot = (float *)alloc((size_t)N * sizeof(float));
     ck(piro_memcpy_h2d(ctx_, snapshot, snap.data(), (size_t)N * sizeof(float)));
     has_bc = true;
@@ -545,39 +561,41 @@ void scalarMatrix::Solve(float currenttime)
 
 // ------------------------------------------------------------- Paraview
 
-std::string writevti(const Case &c, const std::vector<std::vector<float>> &hf)
+std::string writevti(const VtkGrid &g, const std::vector<VtkField> &fields)
 {
-    // src/paraview.cpp:12-103 (extents of the interior; Origin/Spacing are
-    // ints in the reference's AMR struct)
-    const int *n = c.n;
-    std::string ext = "0 " + std::to_string(n[0] - 2) + " 0 " + std::to_string(n[1] - 2) + " 0 " +
-                      std::to_string(n[2] - 2);
+    // src/paraview.cpp:12-103: extents of the interior, int origin/spacing,
+    // boundary cells erased (bc::prepbc's list), "Float64" ascii arrays
+    const int *n = g.n;
+    const std::string ext = "0 " + std::to_string(n[0] - 2) + " 0 " + std::to_string(n[1] - 2) + " 0 " +
+                            std::to_string(n[2] - 2);
+    auto three = [](const int *v) {
+        return std::to_string(v[0]) + " " + std::to_string(v[1]) + " " + std::to_string(v[2]);
+    };
     std::string s = "<VTKFile type=\"ImageData\" version=\"1.0\" byte_order=\"LittleEndian\">\n";
-    s += "<ImageData WholeExtent=\"" + ext + "\" Origin=\"0 0 0\" Spacing=\"1 1 1\">\n";
+    s += "<ImageData WholeExtent=\"" + ext + "\" Origin=\"" + three(g.origin) + "\" Spacing=\"" + three(g.spacing) +
+         "\">\n";
     s += "<Piece Extent=\"" + ext + "\">\n";
     s += "<PointData Scalars=\"ScalarValues\">\n</PointData>\n<CellData>\n";
-    auto interior = [&](const std::vector<float> &v, int64_t off) {
+    const int64_t N = (int64_t)n[0] * n[1] * n[2];
+    auto interior = [&](const float *v) {
         std::string out;
-        for (int z = 0; z < n[2]; ++z)
-            for (int y = 0; y < n[1]; ++y)
-                for (int x = 0; x < n[0]; ++x) {
-                    if (x == 0 || x == n[0] - 1 || y == 0 || y == n[1] - 1 || z == 0 || z == n[2] - 1) continue;
+        for (int z = 1; z < n[2] - 1; ++z)
+            for (int y = 1; y < n[1] - 1; ++y)
+                for (int x = 1; x < n[0] - 1; ++x) {
                     if (!out.empty()) out += ' ';
-                    out += fmt_float(v[off + x + (int64_t)y * n[0] + (int64_t)z * n[0] * n[1]]);
+                    out += fmt_float(v[x + (int64_t)y * n[0] + (int64_t)z * n[0] * n[1]]);
                 }
         return out;
     };
-    int n_vec = 0;
-    for (int t : c.types) n_vec += t == 2;
-    for (size_t k = 0; k < c.types.size(); ++k) {
-        if (c.types[k] == 1) {
-            s += "<DataArray type=\"Float64\" Name=\"" + c.names[k] + "\" format=\"ascii\">\n";
-            s += interior(hf[k], 0) + "</DataArray>\n";
-        } else if (c.types[k] == 2 && n_vec > 0) {
+    for (const VtkField &f : fields) {
+        if (f.type == 1) {
+            s += "<DataArray type=\"Float64\" Name=\"" + f.name + "\" format=\"ascii\">\n";
+            s += interior(f.data) + "</DataArray>\n";
+        } else if (f.type == 2) {
             const char *suf[3] = {"_x", "_y", "_z"};
             for (int d = 0; d < 3; ++d) {
-                s += "<DataArray type=\"Float64\" Name=\"" + c.names[k] + suf[d] + "\" format=\"ascii\">\n";
-                s += interior(hf[k], (int64_t)d * c.N) + "</DataArray>\n";
+                s += "<DataArray type=\"Float64\" Name=\"" + f.name + suf[d] + "\" format=\"ascii\">\n";
+                s += interior(f.data + d * N) + "</DataArray>\n";
             }
         }
     }
@@ -585,32 +603,51 @@ std::string writevti(const Case &c, const std::vector<std::vector<float>> &hf)
     return s;
 }
 
-std::string writevth(const Case &c, int step)
+static std::string step_tag(int step)
+{
+    // 5-digit zero padding (src/paraview.cpp:108-109)
+    std::string ts = std::to_string(step);
+    return std::string(ts.size() < 5 ? 5 - ts.size() : 0, '0') + ts;
+}
+
+std::string writevth(const VtkGrid &g, int step)
 {
     // src/paraview.cpp:105-137 (one level, one block)
-    std::string ts = std::to_string(step);
-    ts = std::string(ts.size() < 5 ? 5 - ts.size() : 0, '0') + ts;
+    const std::string ts = step_tag(step);
     std::string s = "<?xml version=\"1.0\"?>\n";
     s += "<VTKFile type=\"vtkOverlappingAMR\" version=\"1.1\" byte_order=\"LittleEndian\" header_type=\"UInt32\">\n";
     s += "<vtkOverlappingAMR origin=\"0 0 0\" grid_description=\"XYZ\">\n";
     s += "<Block level=\"0\" spacing=\"1 1 1\">\n";
-    s += "<DataSet index=\"0\" amr_box=\"0 " + std::to_string(c.n[0] - 2) + " 0 " + std::to_string(c.n[1] - 2) +
-         " 0 " + std::to_string(c.n[2] - 2) + "\" file=\"level/" + ts + "_level_0.vti\">\n";
+    s += "<DataSet index=\"0\" amr_box=\"0 " + std::to_string(g.n[0] - 2) + " 0 " + std::to_string(g.n[1] - 2) +
+         " 0 " + std::to_string(g.n[2] - 2) + "\" file=\"level/" + ts + "_level_0.vti\">\n";
     s += "</DataSet>\n</Block>\n</vtkOverlappingAMR>\n</VTKFile>\n";
     return s;
 }
 
+void write_paraview(const std::string &out_dir, const std::string &casename, int step, const VtkGrid &g,
+                    const std::vector<VtkField> &fields)
+{
+    const std::string base = out_dir + "/" + casename + "/mesh/";
+    std::filesystem::create_directories(base + "level");
+    const std::string ts = step_tag(step);
+    std::ofstream vti(base + "level/" + ts + "_level_0.vti");
+    vti << writevti(g, fields) << "\n";
+    std::ofstream vth(base + "mesh_" + ts + ".vth");
+    vth << writevth(g, step) << "\n";
+    if (!vti || !vth) throw SolverError("cannot write Paraview files under " + base);
+}
+
 void Case::export_paraview(int step)
 {
-    std::vector<std::vector<float>> hf(fields.size());
-    for (size_t k = 0; k < fields.size(); ++k)
-        if (types[k] != 0) hf[k] = download((int)k);
-    std::string ts = std::to_string(step);
-    ts = std::string(ts.size() < 5 ? 5 - ts.size() : 0, '0') + ts;
-    const std::string base = opt.out_dir + "/" + casename + "/mesh/";
-    // writefile appends a newline (src/fileutilities.cpp:24-40)
-    std::ofstream(base + "level/" + ts + "_level_0.vti") << writevti(*this, hf) << "\n";
-    std::ofstream(base + "mesh_" + ts + ".vth") << writevth(*this, step) << "\n";
+    // scalarMatrix::Solve reads slot 1 back before export_paraview
+    // (src/process.cpp:238-247); the other slots print their host values
+    const std::vector<float> slot1 = download(1);
+    std::vector<VtkField> vf;
+    for (size_t k = 0; k < fields.size(); ++k) {
+        if (types[k] == 0) continue;
+        vf.push_back(VtkField{names[k], types[k], k == 1 ? slot1.data() : host_values[k].data()});
+    }
+    write_paraview(opt.out_dir, casename, step, grid, vf);
 }
 
 }  // namespace piro
@@ -638,5 +675,30 @@ extern "C" int piro_case_run(const char *assets_dir, const char *case_name, cons
     } catch (const piro::SolverError &e) {
         throw piro::Error(PIRO_EINVAL, e.what());
     }
+    PIRO_API_END
+}
+
+extern "C" int piro_paraview_write(const char *out_dir, const char *case_name, int step, const int *h_dims3,
+                                   const int *h_origin3, const int *h_spacing3, int n_fields,
+                                   const char *const *names, const int *types, const float *const *h_fields)
+{
+    PIRO_API_BEGIN
+    PIRO_REQUIRE(out_dir && case_name && h_dims3 && n_fields >= 0, "null argument");
+    PIRO_REQUIRE(n_fields == 0 || (names && types && h_fields), "null field arrays");
+    PIRO_REQUIRE(step >= 0, "step < 0");
+    piro::VtkGrid g;
+    for (int d = 0; d < 3; ++d) {
+        PIRO_REQUIRE(h_dims3[d] >= 3, "padded dims must be >= 3");
+        g.n[d] = h_dims3[d];
+        g.origin[d] = h_origin3 ? h_origin3[d] : 0;
+        g.spacing[d] = h_spacing3 ? h_spacing3[d] : 1;
+    }
+    std::vector<piro::VtkField> vf;
+    for (int k = 0; k < n_fields; ++k) {
+        PIRO_REQUIRE(types[k] >= 0 && types[k] <= 2, "field type must be 0, 1 or 2");
+        PIRO_REQUIRE(types[k] == 0 || (names[k] && h_fields[k]), "null field");
+        if (types[k] != 0) vf.push_back(piro::VtkField{names[k], types[k], h_fields[k]});
+    }
+    piro::write_paraview(out_dir, case_name, step, g, vf);
     PIRO_API_END
 }

@@ -13,6 +13,10 @@ Reference functions exercised (paths relative to the PIRO tree):
   CSR::stencils::gradient_*             src/stencils.cpp:7-231
   methods::sethash                      src/methods.cpp:74-118
   calculategaussian                     src/ic.cpp:16-27
+  bc::prepbc + paraview::writevth/vti   src/bc.cpp:80-105, src/paraview.cpp:12-137
+
+* ``ref_vtk/<case>/``    -- the reference writer's .vth/.vti files for small
+  fields, with the fields themselves (``fields.npz``).
 
 Only this script runs the harness; tests read the frozen files.
 """
@@ -44,6 +48,32 @@ GAUSS_CASES = [
     ((8, 9, 10), (1.0, 1.2, 0.8), (0.5, 0.55, 0.4), (0.1, 0.2, 0.15)),
     ((32, 32, 32), (30.0 / 31.0,) * 3, (15.0 / 31.0,) * 3, (0.1,) * 3),
 ]
+
+
+# (padded dims, scalars, vectors, step): interior values include signed
+# zeros, infinities, NaNs of both signs, subnormals and wide exponents
+VTK_CASES = [
+    ("mixed_5x4x6", (5, 4, 6), 2, 1, 7),
+    ("single_3x3x3", (3, 3, 3), 1, 0, 12345),
+    ("vectors_4x5x3", (4, 5, 3), 1, 2, 0),
+]
+
+
+def vtk_fields(dims, n_scal, n_vec, seed):
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims
+    N = nx * ny * nz
+    interior = [x + y * nx + z * nx * ny for z in range(1, nz - 1) for y in range(1, ny - 1) for x in range(1, nx - 1)]
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, 1e-40, -3e-39, 1.0 / 3, 123456789.0,
+                        1e30, -2.5e-12, 100000.0, 999999.0, 1e6, 0.1], np.float32)
+    out = []
+    for k in range(n_scal + n_vec):
+        m = 3 * N if k >= n_scal else N
+        v = (rng.standard_normal(m) * 10.0 ** rng.integers(-9, 10, m)).astype(np.float32)
+        for j, idx in enumerate(interior[: len(special)]):
+            v[idx + (N if k >= n_scal else 0)] = special[(j + k) % len(special)]
+        out.append(v)
+    return out
 
 
 def f32_hex(x: float) -> str:
@@ -134,6 +164,25 @@ def main() -> int:
         meta["gauss"].append({"key": key, "dims": list(dims), "l": list(l), "mean": list(mean),
                               "sigma": list(sigma), "digest": digest(g),
                               "sum_f64": float(np.sum(g.astype(np.float64)))})
+    for i, (key, dims, n_scal, n_vec, step) in enumerate(VTK_CASES):
+        fields = vtk_fields(dims, n_scal, n_vec, 500 + i)
+        outdir = os.path.join(HERE, "ref_vtk", key)
+        os.makedirs(outdir, exist_ok=True)
+        with tempfile.TemporaryDirectory() as td:
+            inp = os.path.join(td, "in.bin")
+            np.concatenate(fields).astype("<f4").tofile(inp)
+            subprocess.run([HARNESS, "vtk", *map(str, dims), str(n_scal), str(n_vec), str(step), inp, td],
+                           check=True, capture_output=True)
+            tag = "%05d" % step
+            for src, dst in ((f"case/mesh/mesh_{tag}.vth", f"mesh_{tag}.vth"),
+                             (f"case/mesh/level/{tag}_level_0.vti", f"{tag}_level_0.vti")):
+                with open(os.path.join(td, src), "rb") as fi, open(os.path.join(outdir, dst), "wb") as fo:
+                    fo.write(fi.read())
+        np.savez_compressed(os.path.join(outdir, "fields.npz"),
+                            **{f"f{k}": f for k, f in enumerate(fields)})
+        meta.setdefault("vtk", []).append({"key": key, "dims": list(dims), "n_scal": n_scal, "n_vec": n_vec,
+                                           "step": step, "names": ["Alpha"] + [f"S{k}" for k in range(n_scal)]
+                                           + [f"V{k}" for k in range(n_vec)]})
     np.savez_compressed(os.path.join(HERE, "ref_assembly.npz"), **arrays)
     with open(os.path.join(HERE, "ref_digests.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
