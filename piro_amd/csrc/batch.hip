@@ -192,6 +192,101 @@ __global__ __launch_bounds__(256) void k_batch_step(const Unit *__restrict__ uni
     }
 }
 
+// ------------------------------------------------- prefetching step (256/512)
+//
+// The same per-unit arithmetic with the single-grid prefetching kernel's
+// memory schedule: the unit's non-zero range from two uniform loads, all of
+// a lane's 16-byte col/val loads issued together, then its rows' row_ptr
+// pair and x[r], then the gathers; products in LDS, one barrier per unit
+// (two LDS buffers alternate between consecutive units of a workgroup).
+
+typedef int int4v __attribute__((ext_vector_type(4)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+template <int RB, int QPL>
+__global__ __launch_bounds__(256) void k_batch_pf(const Unit *__restrict__ units, int n_units,
+                                                  const InstDesc *__restrict__ inst,
+                                                  const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                  const float *__restrict__ val, int64_t nnz,
+                                                  const float *__restrict__ x, float *__restrict__ y)
+{
+    constexpr int RPL = RB / 256;
+    constexpr int CAP = QPL * 1024;
+    __shared__ __attribute__((aligned(16))) float s_p[2][CAP];
+    const int t = threadIdx.x;
+    int buf = 0;
+    for (int u = blockIdx.x; u < n_units; u += gridDim.x, buf ^= 1) {
+        const Unit un = units[u];
+        const InstDesc d = inst[un.inst];
+        const int64_t r0 = un.row0;
+        const int nr = un.n_rows;
+        const int k0 = rp[r0], k1 = rp[r0 + nr];
+        const int ka = k0 & ~3;
+        const bool staged = k1 - ka <= CAP;
+        const bool fast = staged && k1 > ka && (int64_t)((k1 + 3) & ~3) <= nnz;
+        float *sp = s_p[buf];
+        int4v c[QPL];
+        float4v v[QPL];
+        int kb[QPL];
+        if (fast) {
+            // a lane past the unit's last quad re-reads it (same address as
+            // another lane, no extra traffic) and re-stores the same products
+            const int qlast = ((k1 + 3) >> 2) - 1;
+#pragma unroll
+            for (int j = 0; j < QPL; ++j) {
+                kb[j] = min((ka >> 2) + t + 256 * j, qlast) << 2;
+                c[j] = __builtin_nontemporal_load((const int4v *)(col + kb[j]));
+                v[j] = __builtin_nontemporal_load((const float4v *)(val + kb[j]));
+            }
+        }
+        int rb[RPL], re[RPL];
+        float xr[RPL];
+        bool interior[RPL];
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+            const int64_t g = r0 + min(t + 256 * j, nr - 1);
+            rb[j] = rp[g];
+            re[j] = rp[g + 1];
+            xr[j] = x[g];
+            const uint32_t li = (uint32_t)(g - d.row0);
+            const uint32_t zz = fdiv(li, d.mxy, d.sxy);
+            const uint32_t rem = li - zz * (uint32_t)(d.nx * d.ny);
+            const uint32_t yy = fdiv(rem, d.mx, d.sx);
+            const uint32_t xx = rem - yy * (uint32_t)d.nx;
+            interior[j] = xx > 0 && xx < (uint32_t)d.nx - 1 && yy > 0 && yy < (uint32_t)d.ny - 1 &&
+                          (d.nz == 1 || (zz > 0 && zz < (uint32_t)d.nz - 1));
+        }
+        if (fast) {
+#pragma unroll
+            for (int j = 0; j < QPL; ++j) {
+                float4v o;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) o[e] = v[j][e] * x[c[j][e]];
+                *(float4v *)(sp + (kb[j] - ka)) = o;
+            }
+        } else if (staged) {
+            for (int k = k0 + t; k < k1; k += 256) sp[k - ka] = val[k] * x[col[k]];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+            const int rr = t + 256 * j;
+            if (rr >= nr) continue;
+            float out = 0.0f;
+            if (interior[j]) {
+                float acc = 0.0f;
+                if (staged) {
+                    for (int k = rb[j] - ka; k < re[j] - ka; ++k) acc = acc + sp[k];
+                } else {
+                    for (int k = rb[j]; k < re[j]; ++k) acc = acc + val[k] * x[col[k]];
+                }
+                out = acc + xr[j];
+            }
+            y[r0 + rr] = out;
+        }
+    }
+}
+
 void build_units(piro_batch *b, int rb)
 {
     std::vector<Unit> u;
@@ -211,8 +306,13 @@ void launch_step(piro_batch *b, const float *x, float *y, hipStream_t s)
     const int g = b->grid > 0 ? b->grid : std::min(b->n_units, b->ctx->num_cus * 8);
     switch (b->rb) {
     case 128: k_batch_step<128><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y); break;
-    case 256: k_batch_step<256><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y); break;
-    case 512: k_batch_step<512><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y); break;
+    // 7 entries per row at most, plus the quad alignment slack
+    case 256:
+        k_batch_pf<256, 2><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x, y);
+        break;
+    case 512:
+        k_batch_pf<512, 4><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x, y);
+        break;
     default: k_batch_step<1024><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y); break;
     }
     check_launch();
@@ -437,11 +537,11 @@ int piro_batch_autotune(piro_batch *b, float *d_x, float *d_work, int *best_rows
     int best_rb = 256, best_g = 0;
     float best_t = 1e30f;
     const int rbs[4] = {128, 256, 512, 1024};
-    const int per_cu[4] = {2, 4, 8, 16};
+    const int per_cu[6] = {2, 4, 8, 16, 64, 0};  // 0: one workgroup per unit
     for (int rb : rbs) {
         build_units(b, rb);
         for (int pc : per_cu) {
-            b->grid = std::min(b->n_units, b->ctx->num_cus * pc);
+            b->grid = pc ? std::min(b->n_units, b->ctx->num_cus * pc) : b->n_units;
             launch_step(b, d_x, d_work, s);
             PIRO_CHECK_HIP(hipEventRecord(e0, s));
             for (int i = 0; i < 4; ++i) launch_step(b, d_x, d_work, s);

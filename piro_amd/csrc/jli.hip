@@ -5,11 +5,13 @@
 // the whole N-float update vector back to the host to sum its squares, every
 // iteration.  Here one fused kernel per iteration computes
 //   c = b - (interior ? x - sum fl(fl(dt*v)*x[col]) : 0);  x' = x + c*urf
-// and accumulates sum (x'-x)^2 in FP64 on the device; a one-thread kernel
-// evaluates the reference's stopping rule res = sqrt(s)/sqrt(N),
-// (res - res_old)^2 < tol and raises a device flag that turns the remaining
-// queued iterations into no-ops.  The host looks at 8 bytes every
-// `JLI_BATCH` iterations instead of N floats every iteration.
+// and writes each workgroup's FP64 sum of (x'-x)^2 to its own slot; a
+// one-workgroup kernel adds the slots in a fixed order (so the residual is
+// reproducible run to run), evaluates the reference's stopping rule
+// res = sqrt(s)/sqrt(N), (res - res_old)^2 < tol, and raises a device flag
+// that turns the iterations still queued into no-ops.  The host reads 8
+// bytes after iterations 1, 2, 4, 8, 16 and then every 16, instead of N
+// floats every iteration.
 #include "device_utils.hpp"
 
 #include <cmath>
@@ -17,17 +19,22 @@
 using namespace piro;
 
 namespace piro {
-void jli_iteration(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *rp, const int32_t *col, const float *val,
-                   int64_t nnz, float dt, const float *x, const float *b, float urf, float *x_new, double *acc,
-                   const int *done);
+int jli_iteration(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *rp, const int32_t *col, const float *val,
+                  int64_t nnz, float dt, const float *x, const float *b, float urf, float *x_new, double *acc,
+                  const int *done);
 }
 
 namespace {
 
 constexpr int JLI_BATCH = 16;
 
+// host checks after iterations 1, 2, 4, 8, 16, then every JLI_BATCH
+inline bool check_after(int done_iters)
+{
+    return done_iters <= JLI_BATCH ? (done_iters & (done_iters - 1)) == 0 : done_iters % JLI_BATCH == 0;
+}
+
 struct JliState {
-    double acc;       // sum of squared updates of the current iteration
     float res_old;
     float last_res;
     int done;
@@ -37,7 +44,6 @@ struct JliState {
 
 __global__ void k_jli_init(JliState *st)
 {
-    st->acc = 0.0;
     st->res_old = 0.0f;
     st->last_res = 0.0f;
     st->done = 0;
@@ -45,18 +51,24 @@ __global__ void k_jli_init(JliState *st)
     st->final_buf = 0;
 }
 
-// src/kernelprocess.cpp:197-204 on the FP64 device sum
-__global__ void k_jli_check(JliState *st, float sqrt_n, float tol, int it, int buf_written)
+// src/kernelprocess.cpp:197-204 on the FP64 device sum of the n_part
+// per-workgroup partials, added in a fixed order
+__global__ __launch_bounds__(256) void k_jli_check(JliState *st, const double *__restrict__ part, int n_part,
+                                                   float sqrt_n, float tol, int it, int buf_written)
 {
     if (st->done) return;
-    float res = sqrtf((float)st->acc) / sqrt_n;
-    float d = (res - st->res_old) * (res - st->res_old);
-    st->last_res = res;
-    st->iters = it + 1;
-    st->final_buf = buf_written;
-    if (d < tol) st->done = 1;
-    st->res_old = res;
-    st->acc = 0.0;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n_part; i += 256) s += part[i];
+    s = block_sum_f64<256>(s);
+    if (threadIdx.x == 0) {
+        float res = sqrtf((float)s) / sqrt_n;
+        float d = (res - st->res_old) * (res - st->res_old);
+        st->last_res = res;
+        st->iters = it + 1;
+        st->final_buf = buf_written;
+        if (d < tol) st->done = 1;
+        st->res_old = res;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_copy(int64_t n, const float *__restrict__ a, float *__restrict__ b)
@@ -78,6 +90,7 @@ extern "C" int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32
     PIRO_REQUIRE(max_iter >= 0, "max_iter < 0");
     const int64_t N = (int64_t)nx * ny * nz;
     JliState *st = (JliState *)ctx->scratch[0].get(sizeof(JliState));
+    double *part = (double *)ctx->scratch[2].get((size_t)(N / 256 + 2) * sizeof(double));
     k_jli_init<<<1, 1, 0, ctx->stream>>>(st);
     check_launch();
     // x0 = b (src/kernelprocess.cpp:146)
@@ -89,11 +102,11 @@ extern "C" int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32
     for (int it = 0; it < max_iter; ++it) {
         const float *src = bufs[it & 1];
         float *dst = bufs[(it + 1) & 1];
-        jli_iteration(ctx, nx, ny, nz, d_row_ptr, d_col, d_val, nnz, dt, src, d_b, urf, dst, &st->acc,
-                      &st->done);
-        k_jli_check<<<1, 1, 0, ctx->stream>>>(st, sqrt_n, tol, it, (it + 1) & 1);
+        const int n_part = jli_iteration(ctx, nx, ny, nz, d_row_ptr, d_col, d_val, nnz, dt, src, d_b, urf, dst,
+                                         part, &st->done);
+        k_jli_check<<<1, 256, 0, ctx->stream>>>(st, part, n_part, sqrt_n, tol, it, (it + 1) & 1);
         check_launch();
-        if ((it + 1) % JLI_BATCH == 0 || it + 1 == max_iter) {
+        if (check_after(it + 1) || it + 1 == max_iter) {
             PIRO_CHECK_HIP(hipMemcpyAsync(&h, st, sizeof(JliState), hipMemcpyDeviceToHost, ctx->stream));
             PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
             if (h.done) break;

@@ -79,7 +79,7 @@ struct Params {
     const float *x;
     const float *snap;   // FE: face values (nullable); JLI: b
     float *y;            // FE/SPMV/ALLROWS: output; JLI: x_new
-    double *acc;         // JLI: sum of squared updates (atomic)
+    double *acc;         // JLI: acc[b] = sum of squared updates of logical block b
     const int *done;     // JLI: converged flag; set -> the kernel is a no-op
     float urf;
     Grid3 g;
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_stream(Params p)
     }
     if (MODE == JLI) {
         double s = block_sum_f64<256>(jli_sq);
-        if (threadIdx.x == 0) atomicAdd(p.acc, s);
+        if (threadIdx.x == 0) p.acc[b] = s;
     }
 }
 
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(256) void k_pf(Params p)
     }
     if (MODE == JLI) {
         double s = block_sum_f64<256>(jli_sq);
-        if (threadIdx.x == 0) atomicAdd(p.acc, s);
+        if (threadIdx.x == 0) p.acc[b] = s;
     }
 }
 
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void k_row_lane(Params p)
     }
     if (MODE == JLI) {
         double s = block_sum_f64<256>(jli_sq);
-        if (threadIdx.x == 0) atomicAdd(p.acc, s);
+        if (threadIdx.x == 0) p.acc[b] = s;
     }
 }
 
@@ -539,16 +539,20 @@ void check_grid(int nx, int ny, int nz)
 }  // namespace
 
 namespace piro {
-// used by jli.hip
-void jli_iteration(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *rp, const int32_t *col, const float *val,
-                   int64_t nnz, float dt, const float *x, const float *b, float urf, float *x_new, double *acc,
-                   const int *done)
+// used by jli.hip: one iteration; acc receives one partial sum per logical
+// block and the number of blocks is returned (at most n_rows / 256 + 1)
+int jli_iteration(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *rp, const int32_t *col, const float *val,
+                  int64_t nnz, float dt, const float *x, const float *b, float urf, float *x_new, double *acc,
+                  const int *done)
 {
     Params p = make_params(nx, ny, nz, (int)((int64_t)nx * ny * nz), rp, col, val, nnz, dt, x, b, x_new);
     p.acc = acc;
     p.done = done;
     p.urf = urf;
-    launch(JLI, pick_variant(ctx, p), p, ctx->stream);
+    const int v = pick_variant(ctx, p);
+    launch(JLI, v, p, ctx->stream);
+    const int rpb = kVariants[v].rows_per_block ? kVariants[v].rows_per_block : 256;
+    return (int)((p.n_rows + (int64_t)rpb - 1) / rpb);
 }
 }  // namespace piro
 

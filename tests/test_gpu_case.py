@@ -29,7 +29,8 @@ def make_assets(tmp_path, case, edits=None, setup_edits=None):
     """copy the fixture assets and apply `key = value` edits to the case file
     (and to setup.ini)"""
     d = tmp_path / "assets"
-    shutil.copytree(CASES, d)
+    # copyfile: new files with default permissions (the fixtures may be read-only)
+    shutil.copytree(CASES, d, copy_function=shutil.copyfile)
     for fname, ed in ((case, edits), ("setup.ini", setup_edits)):
         if not ed:
             continue
