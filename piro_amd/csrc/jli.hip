@@ -5,9 +5,10 @@
 // the whole N-float update vector back to the host to sum its squares, every
 // iteration.  Here one fused kernel per iteration computes
 //   c = b - (interior ? x - sum fl(fl(dt*v)*x[col]) : 0);  x' = x + c*urf
-// and writes each workgroup's FP64 sum of (x'-x)^2 to its own slot; a
-// one-workgroup kernel adds the slots in a fixed order (so the residual is
-// reproducible run to run), evaluates the reference's stopping rule
+// and writes each workgroup's FP64 sum of (x'-x)^2 to its own slot; two
+// reduction launches add the slots in a fixed order (so the residual is
+// reproducible run to run), and the second evaluates the reference's
+// stopping rule
 // res = sqrt(s)/sqrt(N), (res - res_old)^2 < tol, and raises a device flag
 // that turns the iterations still queued into no-ops.  The host reads 8
 // bytes after iterations 1, 2, 4, 8, 16 and then every 16, instead of N
@@ -51,14 +52,28 @@ __global__ void k_jli_init(JliState *st)
     st->final_buf = 0;
 }
 
-// src/kernelprocess.cpp:197-204 on the FP64 device sum of the n_part
-// per-workgroup partials, added in a fixed order
-__global__ __launch_bounds__(256) void k_jli_check(JliState *st, const double *__restrict__ part, int n_part,
-                                                   float sqrt_n, float tol, int it, int buf_written)
+constexpr int JLI_STAGE = 256;  // workgroups of the first reduction stage
+
+// first stage: workgroup g adds partials [g*chunk, (g+1)*chunk) in a fixed
+// order
+__global__ __launch_bounds__(256) void k_jli_stage(const JliState *st, const double *__restrict__ part, int n_part,
+                                                   int chunk, double *__restrict__ stage)
 {
     if (st->done) return;
+    const int lo = blockIdx.x * chunk, hi = min(n_part, lo + chunk);
     double s = 0.0;
-    for (int i = threadIdx.x; i < n_part; i += 256) s += part[i];
+    for (int i = lo + threadIdx.x; i < hi; i += 256) s += part[i];
+    s = block_sum_f64<256>(s);
+    if (threadIdx.x == 0) stage[blockIdx.x] = s;
+}
+
+// src/kernelprocess.cpp:197-204 on the FP64 device sum of the first
+// stage's JLI_STAGE values
+__global__ __launch_bounds__(256) void k_jli_check(JliState *st, const double *__restrict__ stage, float sqrt_n,
+                                                   float tol, int it, int buf_written)
+{
+    if (st->done) return;
+    double s = threadIdx.x < JLI_STAGE ? stage[threadIdx.x] : 0.0;
     s = block_sum_f64<256>(s);
     if (threadIdx.x == 0) {
         float res = sqrtf((float)s) / sqrt_n;
@@ -90,7 +105,8 @@ extern "C" int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32
     PIRO_REQUIRE(max_iter >= 0, "max_iter < 0");
     const int64_t N = (int64_t)nx * ny * nz;
     JliState *st = (JliState *)ctx->scratch[0].get(sizeof(JliState));
-    double *part = (double *)ctx->scratch[2].get((size_t)(N / 256 + 2) * sizeof(double));
+    double *part = (double *)ctx->scratch[2].get((size_t)(N / 256 + 2 + JLI_STAGE) * sizeof(double));
+    double *stage = part + (N / 256 + 2);
     k_jli_init<<<1, 1, 0, ctx->stream>>>(st);
     check_launch();
     // x0 = b (src/kernelprocess.cpp:146)
@@ -104,7 +120,10 @@ extern "C" int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32
         float *dst = bufs[(it + 1) & 1];
         const int n_part = jli_iteration(ctx, nx, ny, nz, d_row_ptr, d_col, d_val, nnz, dt, src, d_b, urf, dst,
                                          part, &st->done);
-        k_jli_check<<<1, 256, 0, ctx->stream>>>(st, part, n_part, sqrt_n, tol, it, (it + 1) & 1);
+        const int chunk = (n_part + JLI_STAGE - 1) / JLI_STAGE;
+        k_jli_stage<<<JLI_STAGE, 256, 0, ctx->stream>>>(st, part, n_part, chunk, stage);
+        check_launch();
+        k_jli_check<<<1, 256, 0, ctx->stream>>>(st, stage, sqrt_n, tol, it, (it + 1) & 1);
         check_launch();
         if (check_after(it + 1) || it + 1 == max_iter) {
             PIRO_CHECK_HIP(hipMemcpyAsync(&h, st, sizeof(JliState), hipMemcpyDeviceToHost, ctx->stream));

@@ -31,6 +31,7 @@ using namespace piro;
 namespace {
 
 constexpr int PB = 16;  // panel width
+constexpr int NB = 64;  // factorisation block: 4 panels, then one K = 64 update
 
 // ------------------------------------------------------------- panel
 
@@ -173,6 +174,136 @@ __global__ __launch_bounds__(256) void k_panel(float *__restrict__ a, int lda, i
     }
 }
 
+// ------------------------------------------- 1024-lane panel (m <= 4096)
+//
+// Thread t owns panel rows k + t + 1024 r (R <= 4 rows, 16 columns in
+// registers), so each SIMD runs four waves and the column steps' latencies
+// overlap.  Per column: the pivot is the first row of largest magnitude
+// (LAPACK isamax; FP32 magnitudes compare exactly), found by a wave
+// reduction and a redundant 16-way reduction in every thread after one
+// barrier; the pivot row and the row it replaces go through LDS, which also
+// broadcasts the pivot row to the update -- two barriers per column.
+
+struct Panel1kShared {
+    float mag[16];
+    int row[16];
+    float swp[2][PB];
+};
+
+template <int R, int J>
+__device__ __forceinline__ void panel1k_step(float (&p)[R][PB], Panel1kShared &sh, int t, int n, int k, int jb,
+                                             int32_t *__restrict__ ipiv, int *__restrict__ info)
+{
+    if (J >= jb) return;  // uniform: jb < 16 only for the last panel
+    const int lane = t & 63, w = t >> 6, gj = k + J;
+    float best = -1.0f;
+    int brow = 0x7fffffff;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int gi = k + t + 1024 * r;
+        const float m = (gi >= gj && gi < n) ? fabsf(p[r][J]) : -1.0f;
+        if (m > best) { best = m; brow = gi; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ob = __shfl_xor(best, off, 64);
+        const int orow = __shfl_xor(brow, off, 64);
+        if (ob > best || (ob == best && orow < brow)) { best = ob; brow = orow; }
+    }
+    if (lane == 0) { sh.mag[w] = best; sh.row[w] = brow; }
+    __syncthreads();
+    float b = sh.mag[0];
+    int piv = sh.row[0];
+#pragma unroll
+    for (int q = 1; q < 16; ++q)
+        if (sh.mag[q] > b || (sh.mag[q] == b && sh.row[q] < piv)) { b = sh.mag[q]; piv = sh.row[q]; }
+    if (t == 0) {
+        ipiv[gj] = piv;
+        if (b == 0.0f && *info == 0) *info = gj + 1;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int gi = k + t + 1024 * r;
+        if (gi == piv)
+#pragma unroll
+            for (int c = 0; c < PB; ++c) sh.swp[0][c] = p[r][c];
+        if (gi == gj)
+#pragma unroll
+            for (int c = 0; c < PB; ++c) sh.swp[1][c] = p[r][c];
+    }
+    __syncthreads();
+    if (piv != gj) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int gi = k + t + 1024 * r;
+            if (gi == gj)
+#pragma unroll
+                for (int c = 0; c < PB; ++c) p[r][c] = sh.swp[0][c];
+            if (gi == piv)
+#pragma unroll
+                for (int c = 0; c < PB; ++c) p[r][c] = sh.swp[1][c];
+        }
+    }
+    const float d = sh.swp[0][J];
+    if (d != 0.0f) {
+        float u[PB];
+#pragma unroll
+        for (int c = J + 1; c < PB; ++c) u[c] = sh.swp[0][c];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int gi = k + t + 1024 * r;
+            if (gi > gj && gi < n) {
+                const float l = p[r][J] / d;
+                p[r][J] = l;
+#pragma unroll
+                for (int c = J + 1; c < PB; ++c) p[r][c] = p[r][c] - l * u[c];
+            }
+        }
+    }
+}
+
+template <int R, int J>
+struct Panel1kLoop {
+    static __device__ __forceinline__ void run(float (&p)[R][PB], Panel1kShared &sh, int t, int n, int k, int jb,
+                                               int32_t *ipiv, int *info)
+    {
+        panel1k_step<R, J>(p, sh, t, n, k, jb, ipiv, info);
+        Panel1kLoop<R, J + 1>::run(p, sh, t, n, k, jb, ipiv, info);
+    }
+};
+
+template <int R>
+struct Panel1kLoop<R, PB> {
+    static __device__ __forceinline__ void run(float (&)[R][PB], Panel1kShared &, int, int, int, int, int32_t *,
+                                               int *)
+    {
+    }
+};
+
+template <int R>
+__global__ __launch_bounds__(1024) void k_panel1k(float *__restrict__ a, int lda, int n, int k, int jb,
+                                                  int32_t *__restrict__ ipiv, int *__restrict__ info)
+{
+    __shared__ Panel1kShared sh;
+    const int t = threadIdx.x;
+    float p[R][PB];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int gi = k + t + 1024 * r;
+#pragma unroll
+        for (int c = 0; c < PB; ++c) p[r][c] = (gi < n && c < jb) ? a[(int64_t)gi * lda + k + c] : 0.0f;
+    }
+    Panel1kLoop<R, 0>::run(p, sh, t, n, k, jb, ipiv, info);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int gi = k + t + 1024 * r;
+        if (gi < n)
+#pragma unroll
+            for (int c = 0; c < PB; ++c)
+                if (c < jb) a[(int64_t)gi * lda + k + c] = p[r][c];
+    }
+}
+
 // generic (slow) panel for n beyond the register path: same arithmetic,
 // panel in global memory
 __global__ __launch_bounds__(256) void k_panel_global(float *__restrict__ a, int lda, int n, int k, int jb,
@@ -230,38 +361,40 @@ __global__ __launch_bounds__(256) void k_panel_global(float *__restrict__ a, int
 
 // ----------------------------------------------------- swap + U12 TRSM
 
-// columns [c0, n): apply the panel's exchanges, then U12 = L11^-1 A12
-__global__ __launch_bounds__(256) void k_swap_trsm(float *__restrict__ a, int lda, int n, int k, int jb, int c0,
+// columns [c0, c1): apply the exchanges of steps k .. k+jb-1 (jb <= W), then
+// U12 = L11^-1 A12 with L11 the unit lower jb x jb block at (k, k)
+template <int W>
+__global__ __launch_bounds__(256) void k_swap_trsm(float *__restrict__ a, int lda, int k, int jb, int c0, int c1,
                                                    const int32_t *__restrict__ ipiv)
 {
-    __shared__ float s_l[PB][PB];
-    __shared__ int s_p[PB];
+    __shared__ float s_l[W][W + 1];
+    __shared__ int s_p[W];
     const int t = threadIdx.x;
-    if (t < PB * PB) {
-        int i = t / PB, j = t % PB;
+    for (int e = t; e < W * W; e += 256) {
+        const int i = e / W, j = e % W;
         s_l[i][j] = (i < jb && j < jb) ? a[(int64_t)(k + i) * lda + k + j] : 0.0f;
     }
-    if (t < PB) s_p[t] = t < jb ? ipiv[k + t] : k + t;
+    if (t < W) s_p[t] = t < jb ? ipiv[k + t] : k + t;
     __syncthreads();
     const int c = c0 + blockIdx.x * 256 + t;
-    if (c >= n) return;
+    if (c >= c1) return;
     for (int j = 0; j < jb; ++j) {
         const int pj = s_p[j];
         if (pj != k + j) {
-            float x = a[(int64_t)(k + j) * lda + c];
+            const float x = a[(int64_t)(k + j) * lda + c];
             a[(int64_t)(k + j) * lda + c] = a[(int64_t)pj * lda + c];
             a[(int64_t)pj * lda + c] = x;
         }
     }
-    float u[PB];
+    float u[W];
 #pragma unroll
-    for (int i = 0; i < PB; ++i) u[i] = i < jb ? a[(int64_t)(k + i) * lda + c] : 0.0f;
+    for (int i = 0; i < W; ++i) u[i] = i < jb ? a[(int64_t)(k + i) * lda + c] : 0.0f;
 #pragma unroll
-    for (int i = 1; i < PB; ++i)
+    for (int i = 1; i < W; ++i)
 #pragma unroll
         for (int j = 0; j < i; ++j) u[i] = u[i] - s_l[i][j] * u[j];
 #pragma unroll
-    for (int i = 0; i < PB; ++i)
+    for (int i = 0; i < W; ++i)
         if (i < jb) a[(int64_t)(k + i) * lda + c] = u[i];
 }
 
@@ -269,27 +402,29 @@ __global__ __launch_bounds__(256) void k_swap_trsm(float *__restrict__ a, int ld
 
 typedef float float4v __attribute__((ext_vector_type(4)));
 
-// C[r0.., c0..] -= L21 (rows r0.., cols k..k+16) x U12 (rows k..k+16, cols c0..)
-// block tile 128 x 128, 4 waves in a 2 x 2 arrangement, 64 x 64 per wave
-// (4 x 4 MFMA 16x16 tiles), K = 16 in 4 MFMA k-steps of 4.
+// A[r0..r1, c0..c1) -= L21 (rows r0..r1, cols k..k+jb) x U12 (rows
+// k..k+jb, cols c0..c1), jb <= KW; the row bound r1 is passed as n.  Block tile 128 x 128, 4 waves in a 2 x 2
+// arrangement, 64 x 64 per wave (4 x 4 MFMA 16x16 tiles), K in steps of 4.
 // v_mfma_f32_16x16x4f32 lane layout: A[i][k]: i = lane%16, k = lane/16;
 // B[k][j]: j = lane%16, k = lane/16; D[i][j]: j = lane%16, i = 4*(lane/16)+r.
-__global__ __launch_bounds__(256) void k_update_mfma(float *__restrict__ a, int lda, int n, int k, int jb)
+template <int KW>
+__global__ __launch_bounds__(256) void k_update_mfma(float *__restrict__ a, int lda, int n, int k, int jb, int r0g,
+                                                     int c0g, int c1)
 {
-    __shared__ float s_a[128][PB + 1];
-    __shared__ float s_b[PB][128 + 4];
-    const int r0 = k + jb + blockIdx.y * 128, c0 = k + jb + blockIdx.x * 128;
+    __shared__ float s_a[128][KW + 1];
+    __shared__ float s_b[KW][128 + 4];
+    const int r0 = r0g + blockIdx.y * 128, c0 = c0g + blockIdx.x * 128;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    // stage L21 tile (negated) and U12 tile
-    for (int e = t; e < 128 * PB; e += 256) {
-        int i = e / PB, kk = e % PB;
-        int gi = r0 + i;
+    // stage the L21 tile (negated) and the U12 tile
+    for (int e = t; e < 128 * KW; e += 256) {
+        const int i = e / KW, kk = e % KW;
+        const int gi = r0 + i;
         s_a[i][kk] = (gi < n && kk < jb) ? -a[(int64_t)gi * lda + k + kk] : 0.0f;
     }
-    for (int e = t; e < PB * 128; e += 256) {
-        int kk = e / 128, j = e % 128;
-        int gj = c0 + j;
-        s_b[kk][j] = (gj < n && kk < jb) ? a[(int64_t)(k + kk) * lda + gj] : 0.0f;
+    for (int e = t; e < KW * 128; e += 256) {
+        const int kk = e / 128, j = e % 128;
+        const int gj = c0 + j;
+        s_b[kk][j] = (gj < c1 && kk < jb) ? a[(int64_t)(k + kk) * lda + gj] : 0.0f;
     }
     __syncthreads();
     const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
@@ -303,11 +438,11 @@ __global__ __launch_bounds__(256) void k_update_mfma(float *__restrict__ a, int 
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int grow = r0 + wr + ti * 16 + 4 * lk + r;
-                acc[ti][tj][r] = (grow < n && gcol < n) ? a[(int64_t)grow * lda + gcol] : 0.0f;
+                acc[ti][tj][r] = (grow < n && gcol < c1) ? a[(int64_t)grow * lda + gcol] : 0.0f;
             }
         }
-#pragma unroll
-    for (int ks = 0; ks < PB; ks += 4) {
+    for (int ks = 0; ks < KW; ks += 4) {
+        if (ks >= jb) break;  // uniform; staged entries past jb are 0 anyway
         float fa[4], fb[4];
 #pragma unroll
         for (int ti = 0; ti < 4; ++ti) fa[ti] = s_a[wr + ti * 16 + li][ks + lk];
@@ -327,19 +462,37 @@ __global__ __launch_bounds__(256) void k_update_mfma(float *__restrict__ a, int 
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int grow = r0 + wr + ti * 16 + 4 * lk + r;
-                if (grow < n && gcol < n) a[(int64_t)grow * lda + gcol] = acc[ti][tj][r];
+                if (grow < n && gcol < c1) a[(int64_t)grow * lda + gcol] = acc[ti][tj][r];
             }
         }
 }
 
-// exchanges of panel steps >= the column's own panel, applied to the L part
-// (columns [0, k_end) of each panel) after the factorisation
+// within one NB-column block: the exchanges of the block's later panels
+// applied to its earlier panels' columns, so that the block's L11 is in
+// final row order before the block-level TRSM
+__global__ __launch_bounds__(256) void k_swap_block_left(float *__restrict__ a, int lda, int kb, int nbk,
+                                                         const int32_t *__restrict__ ipiv)
+{
+    const int c = kb + blockIdx.x * 256 + threadIdx.x;
+    if (c >= kb + nbk) return;
+    for (int j = kb + ((c - kb) / PB + 1) * PB; j < kb + nbk; ++j) {
+        const int pj = ipiv[j];
+        if (pj != j) {
+            const float x = a[(int64_t)j * lda + c];
+            a[(int64_t)j * lda + c] = a[(int64_t)pj * lda + c];
+            a[(int64_t)pj * lda + c] = x;
+        }
+    }
+}
+
+// exchanges of the blocks after a column's own block, applied to the L part
+// after the factorisation
 __global__ __launch_bounds__(256) void k_swap_left(float *__restrict__ a, int lda, int n,
                                                    const int32_t *__restrict__ ipiv)
 {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
-    const int start = (c / PB + 1) * PB;
+    const int start = (c / NB + 1) * NB;
     for (int j = start; j < n; ++j) {
         const int pj = ipiv[j];
         if (pj != j) {
@@ -348,80 +501,6 @@ __global__ __launch_bounds__(256) void k_swap_left(float *__restrict__ a, int ld
             a[(int64_t)pj * lda + c] = x;
         }
     }
-}
-
-// ----------------------------------------------------------------- solve
-
-// One workgroup per right-hand side: b <- U^-1 L^-1 P b in FP32, x in LDS.
-// Blocked by 64 rows: the block's right-hand side is reduced against the
-// solved part by all 1024 lanes (16 lanes per row), then one wave solves the
-// 64 x 64 triangle.
-constexpr int SB = 64;
-
-__global__ __launch_bounds__(1024) void k_solve_rhs(const float *__restrict__ lu, int lda, int n,
-                                                    const int32_t *__restrict__ ipiv, float *__restrict__ b,
-                                                    int ldb)
-{
-    extern __shared__ float s_x[];  // n floats
-    __shared__ float s_part[SB][17];
-    const int t = threadIdx.x, rhs = blockIdx.x;
-    for (int i = t; i < n; i += 1024) s_x[i] = b[(int64_t)i * ldb + rhs];
-    __syncthreads();
-    if (t == 0)
-        for (int j = 0; j < n; ++j) {
-            int pj = ipiv[j];
-            if (pj != j) { float x = s_x[j]; s_x[j] = s_x[pj]; s_x[pj] = x; }
-        }
-    __syncthreads();
-    const int rr = t >> 4, q = t & 15;  // row within block, lane within row
-    // forward (unit lower)
-    for (int i0 = 0; i0 < n; i0 += SB) {
-        const int i = i0 + rr;
-        float s = 0.0f;
-        if (i < n)
-            for (int j = q; j < i0; j += 16) s += lu[(int64_t)i * lda + j] * s_x[j];
-        s_part[rr][q] = s;
-        __syncthreads();
-        if (t < 64) {
-            const int ii = i0 + t;
-            float acc = 0.0f;
-            for (int qq = 0; qq < 16; ++qq) acc += s_part[t][qq];
-            float v = ii < n ? s_x[ii] - acc : 0.0f;
-            // 64 x 64 unit-lower triangle, one row per lane
-            for (int j = 0; j < SB && i0 + j < n; ++j) {
-                float xj = __shfl(v, j, 64);
-                if (t > j && ii < n) v -= lu[(int64_t)ii * lda + i0 + j] * xj;
-            }
-            if (ii < n) s_x[ii] = v;
-        }
-        __syncthreads();
-    }
-    // backward (upper with diagonal)
-    const int nblk = (n + SB - 1) / SB;
-    for (int bi = nblk - 1; bi >= 0; --bi) {
-        const int i0 = bi * SB, i1 = min(n, i0 + SB);
-        const int i = i0 + rr;
-        float s = 0.0f;
-        if (i < n)
-            for (int j = i1 + q; j < n; j += 16) s += lu[(int64_t)i * lda + j] * s_x[j];
-        s_part[rr][q] = s;
-        __syncthreads();
-        if (t < 64) {
-            const int ii = i0 + t;
-            float acc = 0.0f;
-            for (int qq = 0; qq < 16; ++qq) acc += s_part[t][qq];
-            float v = ii < n ? s_x[ii] - acc : 0.0f;
-            for (int j = i1 - 1 - i0; j >= 0; --j) {
-                // lane j finalises x_j, then broadcasts it
-                if (t == j) v = v / lu[(int64_t)ii * lda + ii];
-                float xj = __shfl(v, j, 64);
-                if (t < j && ii < n) v -= lu[(int64_t)ii * lda + i0 + j] * xj;
-            }
-            if (ii < n) s_x[ii] = v;
-        }
-        __syncthreads();
-    }
-    for (int i = t; i < n; i += 1024) b[(int64_t)i * ldb + rhs] = s_x[i];
 }
 
 // ------------------------------------------------------------ refinement
@@ -512,37 +591,159 @@ __global__ __launch_bounds__(256) void k_csr_to_dense(int n_rows, int n_cols, co
         for (int k = rp[r]; k < rp[r + 1]; ++k) d[(int64_t)r * ld + col[k]] += val[k];
 }
 
-void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
+// ------------------------------------------------- blocked triangular solve
+//
+// One launch per 64-row block step over all right-hand sides (<= 32 per
+// launch).  Every workgroup first solves the block's 64 x 64 triangle for the
+// block's right-hand-side rows (redundantly: the triangle is 16 KB, read
+// from cache) with one wave per right-hand side and lane shuffles, then
+// subtracts the block's contribution from its own 64 rows further down
+// (forward) or up (backward).  Workgroup 0 writes the block's solution.
+
+constexpr int TB = 64;
+
+// P b: the factorisation's exchanges in order, one lane per right-hand side
+__global__ __launch_bounds__(256) void k_perm_rhs(int n, const int32_t *__restrict__ ipiv, float *__restrict__ b,
+                                                  int ldb, int nrhs)
 {
-    hipStream_t s = ctx->stream;
-    for (int k = 0; k < n; k += PB) {
-        const int jb = min(PB, n - k), m = n - k;
-        const int R = (m + 255) / 256;
-        if (R <= 1) k_panel<1><<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
-        else if (R <= 2) k_panel<2><<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
-        else if (R <= 4) k_panel<4><<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
-        else if (R <= 8) k_panel<8><<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
-        else if (R <= 16) k_panel<16><<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
-        else k_panel_global<<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
-        check_launch();
-        const int c0 = k + jb;
-        if (c0 < n) {
-            k_swap_trsm<<<grid_for(n - c0, 256), 256, 0, s>>>(a, lda, n, k, jb, c0, ipiv);
-            check_launch();
-            dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
-            k_update_mfma<<<g, 256, 0, s>>>(a, lda, n, k, jb);
-            check_launch();
+    for (int c = threadIdx.x; c < nrhs; c += 256)
+        for (int j = 0; j < n; ++j) {
+            const int pj = ipiv[j];
+            if (pj != j) {
+                const float x = b[(int64_t)j * ldb + c];
+                b[(int64_t)j * ldb + c] = b[(int64_t)pj * ldb + c];
+                b[(int64_t)pj * ldb + c] = x;
+            }
         }
+}
+
+// LOWER: unit lower triangle, rows below are updated; else upper triangle
+// with its diagonal, rows above are updated
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_step(const float *__restrict__ lu, int lda, int n, int i0,
+                                                   float *__restrict__ b, int ldb, int nrhs)
+{
+    __shared__ float s_d[TB][TB + 1];
+    __shared__ float s_x[TB][33];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int bs = min(TB, n - i0);
+    for (int e = t; e < TB * TB; e += 256) {
+        const int i = e / TB, j = e % TB;
+        s_d[i][j] = (i < bs && j < bs) ? lu[(int64_t)(i0 + i) * lda + i0 + j] : 0.0f;
     }
-    k_swap_left<<<grid_for(n, 256), 256, 0, s>>>(a, lda, n, ipiv);
-    check_launch();
+    __syncthreads();
+    for (int c = w; c < nrhs; c += 4) {
+        float v = lane < bs ? b[(int64_t)(i0 + lane) * ldb + c] : 0.0f;
+        if (LOWER) {
+            for (int j = 0; j < bs; ++j) {
+                const float xj = __shfl(v, j, 64);
+                if (lane > j) v -= s_d[lane][j] * xj;
+            }
+        } else {
+            for (int j = bs - 1; j >= 0; --j) {
+                if (lane == j) v = v / s_d[j][j];
+                const float xj = __shfl(v, j, 64);
+                if (lane < j) v -= s_d[lane][j] * xj;
+            }
+        }
+        s_x[lane][c] = v;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0)
+        for (int e = t; e < bs * nrhs; e += 256) {
+            const int i = e / nrhs, c = e % nrhs;
+            b[(int64_t)(i0 + i) * ldb + c] = s_x[i][c];
+        }
+    // this workgroup's 64 rows outside the block: 4 lanes per row
+    const int rr = t >> 2, q = t & 3;
+    const int r = LOWER ? i0 + bs + blockIdx.x * TB + rr : blockIdx.x * TB + rr;
+    const bool live = LOWER ? r < n : r < i0;
+    if (!live) return;
+    const float *row = lu + (int64_t)r * lda + i0;
+    for (int c = q; c < nrhs; c += 4) {
+        float acc = 0.0f;
+        for (int j = 0; j < bs; ++j) acc += row[j] * s_x[j][c];
+        b[(int64_t)r * ldb + c] -= acc;
+    }
 }
 
 void solve_f32(piro_ctx *ctx, int n, const float *lu, int lda, const int32_t *ipiv, int nrhs, float *b, int ldb)
 {
-    size_t shm = (size_t)n * sizeof(float);
-    PIRO_REQUIRE(shm <= 96 * 1024, "LU solve keeps x in LDS: n <= 24576");
-    k_solve_rhs<<<nrhs, 1024, shm, ctx->stream>>>(lu, lda, n, ipiv, b, ldb);
+    hipStream_t s = ctx->stream;
+    k_perm_rhs<<<1, 256, 0, s>>>(n, ipiv, b, ldb, nrhs);
+    check_launch();
+    for (int c0 = 0; c0 < nrhs; c0 += 32) {
+        const int nc = min(32, nrhs - c0);
+        float *bc = b + c0;
+        for (int i0 = 0; i0 < n; i0 += TB) {
+            const int bs = min(TB, n - i0);
+            const int below = n - i0 - bs;
+            k_trsv_step<true><<<max(1, (below + TB - 1) / TB), 256, 0, s>>>(lu, lda, n, i0, bc, ldb, nc);
+            check_launch();
+        }
+        for (int i0 = (n - 1) / TB * TB; i0 >= 0; i0 -= TB) {
+            k_trsv_step<false><<<max(1, (i0 + TB - 1) / TB), 256, 0, s>>>(lu, lda, n, i0, bc, ldb, nc);
+            check_launch();
+        }
+    }
+}
+
+void panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int32_t *ipiv, int *d_info)
+{
+    hipStream_t s = ctx->stream;
+    const int m = n - k;
+    if (m <= 1024) k_panel1k<1><<<1, 1024, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
+    else if (m <= 2048) k_panel1k<2><<<1, 1024, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
+    else if (m <= 4096) k_panel1k<4><<<1, 1024, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
+    else if (m <= 2048 * 4) k_panel<8><<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
+    else if (m <= 4096 * 4) k_panel<16><<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
+    else k_panel_global<<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
+    check_launch();
+}
+
+void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
+{
+    hipStream_t s = ctx->stream;
+    for (int kb = 0; kb < n; kb += NB) {
+        const int nbk = min(NB, n - kb);
+        // panels of the block; each updates only the block's columns
+        for (int k = kb; k < kb + nbk; k += PB) {
+            const int jb = min(PB, kb + nbk - k);
+            panel(ctx, n, a, lda, k, jb, ipiv, d_info);
+            const int c0 = k + jb, c1 = kb + nbk;
+            if (c0 < c1) {
+                k_swap_trsm<PB><<<grid_for(c1 - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, c1, ipiv);
+                check_launch();
+                dim3 g((c1 - c0 + 127) / 128, (n - c0 + 127) / 128);
+                k_update_mfma<PB><<<g, 256, 0, s>>>(a, lda, n, k, jb, c0, c0, c1);
+                check_launch();
+            }
+        }
+        if (nbk > PB) {
+            k_swap_block_left<<<grid_for(nbk, 256), 256, 0, s>>>(a, lda, kb, nbk, ipiv);
+            check_launch();
+        }
+        // trailing columns: each panel's exchanges and U rows in turn, with
+        // the K = 16 updates confined to the block's own rows; then one
+        // K = nbk update of every row below the block
+        const int c0 = kb + nbk;
+        if (c0 < n) {
+            for (int k = kb; k < kb + nbk; k += PB) {
+                const int jb = min(PB, kb + nbk - k);
+                k_swap_trsm<PB><<<grid_for(n - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, n, ipiv);
+                check_launch();
+                if (k + jb < c0) {
+                    dim3 g((n - c0 + 127) / 128, (c0 - (k + jb) + 127) / 128);
+                    k_update_mfma<PB><<<g, 256, 0, s>>>(a, lda, c0, k, jb, k + jb, c0, n);
+                    check_launch();
+                }
+            }
+            dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
+            k_update_mfma<NB><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n);
+            check_launch();
+        }
+    }
+    k_swap_left<<<grid_for(n, 256), 256, 0, s>>>(a, lda, n, ipiv);
     check_launch();
 }
 

@@ -378,180 +378,6 @@ __global__ __launch_bounds__(256) void k_pf(Params p)
     }
 }
 
-// ------------------------------------- persistent, software-pipelined stream
-//
-// A fixed grid (a multiple of 8 workgroups).  Workgroup w on XCD j = w % 8
-// takes the row blocks that the chunked schedule gives XCD j, in order,
-// striding by the number of workgroups on that XCD.  While a workgroup forms
-// the products of one row block, the 16-byte col/val loads of its next row
-// block are already in flight (a second register set), and the nnz range of
-// the block after that is read one step earlier still, so a block costs one
-// memory latency instead of three.  Per-block arithmetic, and therefore
-// every bit of the result, is that of k_pf.
-
-// per-row operands of one row block (as k_pf loads them)
-template <int RPL>
-struct RowOps {
-    int rb[RPL], re[RPL], cls[RPL];
-    float xr[RPL], sv[RPL];
-};
-
-template <int RB, int MODE>
-__device__ __forceinline__ void pp_rows(const Params &p, int r0, int nr, RowOps<RB / 256> &ro)
-{
-#pragma unroll
-    for (int j = 0; j < RB / 256; ++j) {
-        const int r = r0 + min((int)threadIdx.x + 256 * j, nr - 1);
-        ro.rb[j] = p.rp[r];
-        ro.re[j] = p.rp[r + 1];
-        ro.cls[j] = (MODE == FE_INTERIOR || MODE == JLI) ? row_class((uint32_t)r, p.g) : 0;
-        ro.xr[j] = (MODE == SPMV) ? 0.0f : p.x[r];
-        ro.sv[j] = 0.0f;
-        if (MODE == JLI) ro.sv[j] = p.snap[r];
-        if (MODE == FE_INTERIOR) {
-            const bool face = ro.cls[j] == 1 && p.snap;
-            const float f = *(face ? p.snap + r : p.x + r);
-            ro.sv[j] = face ? f : 0.0f;
-        }
-    }
-}
-
-// x values of a lane's quads, loaded into registers
-template <int QPL>
-__device__ __forceinline__ void pp_gather(const Params &p, const Quads<QPL> &qd, float4v (&xg)[QPL])
-{
-#pragma unroll
-    for (int j = 0; j < QPL; ++j)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) xg[j][e] = p.x[qd.c[j][e]];
-}
-
-// products to LDS (fast path from the gathered registers, else the tail
-// path), one barrier, row sums, epilogue
-template <int RB, int MODE, int QPL>
-__device__ __forceinline__ void pp_finish(const Params &p, int b, int k0, int k1, bool fast, const Quads<QPL> &qd,
-                                          const float4v (&xg)[QPL], const RowOps<RB / 256> &ro, float *s_p)
-{
-    constexpr int RPL = RB / 256;
-    constexpr int CAP = QPL * 256 * 4;
-    const int r0 = b * RB;
-    const int nr = min(RB, p.n_rows - r0);
-    const int ka = k0 & ~3;
-    const bool staged = k1 - ka <= CAP;
-    if (fast) {
-#pragma unroll
-        for (int j = 0; j < QPL; ++j) {
-            float4v out;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) out[e] = product<MODE>(qd.v[j][e], xg[j][e], p.dt);
-            *(float4v *)(s_p + (qd.kb[j] - ka)) = out;
-        }
-    } else if (staged) {
-        stage_tail<MODE, QPL>(p, ka >> 2, ka, k1, s_p);
-    }
-    __syncthreads();
-    double jli_sq = 0.0;
-#pragma unroll
-    for (int j = 0; j < RPL; ++j) {
-        const int rr = threadIdx.x + 256 * j;
-        if (rr >= nr) continue;
-        const int r = r0 + rr;
-        float acc = 0.0f;
-        if (staged) {
-            for (int k = ro.rb[j] - ka; k < ro.re[j] - ka; ++k) acc = acc + s_p[k];
-        } else {
-            for (int k = ro.rb[j]; k < ro.re[j]; ++k) acc = acc + product<MODE>(p.val[k], p.x[p.col[k]], p.dt);
-        }
-        if (MODE == FE_INTERIOR) {
-            p.y[r] = ro.cls[j] == 0 ? acc + ro.xr[j] : (ro.cls[j] == 1 ? ro.sv[j] : 0.0f);
-        } else if (MODE == ALLROWS) {
-            const float cc = p.dt * acc;
-            p.y[r] = ro.xr[j] + cc;
-        } else if (MODE == SPMV) {
-            p.y[r] = acc;
-        } else {
-            const float ci = ro.cls[j] == 0 ? ro.xr[j] - acc : 0.0f;
-            const float cc = ro.sv[j] - ci;
-            const float inc = cc * p.urf;
-            const float xn = ro.xr[j] + inc;
-            const float d = xn - ro.xr[j];
-            p.y[r] = xn;
-            jli_sq += (double)d * (double)d;
-        }
-    }
-    if (MODE == JLI) {
-        double s = block_sum_f64<256>(jli_sq);
-        if (threadIdx.x == 0) p.acc[b] = s;
-    }
-}
-
-// quads of a row block of nnz range [k0, k1) into qd; a block off the fast
-// path (or no block) loads quad 0 instead, which is then ignored, so the
-// loads are issued unconditionally (requires nnz >= 4)
-template <int QPL>
-__device__ __forceinline__ void pp_load(const Params &p, int k0, int k1, bool fast, Quads<QPL> &qd)
-{
-    load_quads<QPL>(p, fast ? (k0 & ~3) >> 2 : 0, fast ? k1 : 4, qd);
-}
-
-template <int RB, int MODE, int QPL>
-__global__ __launch_bounds__(256) void k_pp(Params p)
-{
-    constexpr int CAP = QPL * 256 * 4;
-    __shared__ __attribute__((aligned(16))) float s_p[2][CAP];
-    if (MODE == JLI && *p.done) return;
-    const int nb = p.n_blocks, C = p.chunk;
-    const int xcd = blockIdx.x & 7, slots = gridDim.x >> 3;
-    const int lane = threadIdx.x & 63;
-    // the k-th row block of this XCD in the chunked schedule (increasing in k)
-    auto block_of = [&](int k) { return ((k / C) * 8 + xcd) * C + k % C; };
-    auto fast_of = [&](int k0, int k1) {
-        const int ka = k0 & ~3;
-        return k1 - ka <= CAP && k1 > ka && (int64_t)((k1 + 3) & ~3) <= p.nnz;
-    };
-    for (int kbase = blockIdx.x >> 3;; kbase += 64 * slots) {
-        // nnz ranges of this workgroup's next 64 row blocks, one per lane
-        const int bl = block_of(kbase + lane * slots);
-        int k0v = 0, k1v = 0;
-        if (bl < nb) {
-            k0v = p.rp[bl * RB];
-            k1v = p.rp[min(bl * RB + RB, p.n_rows)];
-        }
-        const int cnt = __popcll(__ballot(bl < nb));  // a prefix of the lanes
-        if (cnt == 0) return;
-        auto item = [&](int j, int &b, int &k0, int &k1, bool &fast) {
-            b = block_of(kbase + j * slots);
-            k0 = __builtin_amdgcn_readlane(k0v, j);
-            k1 = __builtin_amdgcn_readlane(k1v, j);
-            fast = fast_of(k0, k1);
-        };
-        Quads<QPL> qa, qb;
-        float4v xg[QPL];
-        RowOps<RB / 256> ro;
-        int ba, a0, a1, bb, b0, b1;
-        bool fa, fb;
-        item(0, ba, a0, a1, fa);
-        pp_load<QPL>(p, a0, a1, fa, qa);
-        // per block: its row loads and gathers, then the next block's
-        // col/val loads, then its products, sums and stores -- the next
-        // block's loads are in flight while this block finishes
-        for (int j = 0; j < cnt; j += 2) {
-            pp_rows<RB, MODE>(p, ba * RB, min(RB, p.n_rows - ba * RB), ro);
-            if (fa) pp_gather<QPL>(p, qa, xg);
-            item(min(j + 1, cnt - 1), bb, b0, b1, fb);
-            pp_load<QPL>(p, b0, b1, fb, qb);
-            pp_finish<RB, MODE, QPL>(p, ba, a0, a1, fa, qa, xg, ro, s_p[0]);
-            if (j + 1 >= cnt) break;
-            pp_rows<RB, MODE>(p, bb * RB, min(RB, p.n_rows - bb * RB), ro);
-            if (fb) pp_gather<QPL>(p, qb, xg);
-            item(min(j + 2, cnt - 1), ba, a0, a1, fa);
-            pp_load<QPL>(p, a0, a1, fa, qa);
-            pp_finish<RB, MODE, QPL>(p, bb, b0, b1, fb, qb, xg, ro, s_p[1]);
-        }
-        if (cnt < 64) return;
-    }
-}
-
 // ------------------------------------------------------- row per lane
 
 template <int MODE>
@@ -585,7 +411,6 @@ struct Variant {
     bool xcd;
     bool pf;              // prefetching stream kernel
     int chunk;            // chunked XCD remap (0 = off)
-    int persist;          // > 0: persistent pipelined kernel, this many workgroups per CU
 };
 
 constexpr Variant kVariants[] = {
@@ -607,9 +432,6 @@ constexpr Variant kVariants[] = {
     {"pf_rb256_c8", 256, true, false, true, 8},
     {"pf_rb256_c32", 256, true, false, true, 32},
     {"pf_rb512_c16", 512, true, false, true, 16},
-    {"pp_rb256_c32_w4", 256, true, false, true, 32, 4},
-    {"pp_rb256_c32_w8", 256, true, false, true, 32, 8},
-    {"pp_rb256_c8_w8", 256, true, false, true, 8, 8},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 constexpr int kDefaultVariant = 9;   // pf_rb256
@@ -620,7 +442,7 @@ static_assert(kVariants[kDefaultVariant].pf && kVariants[kDefaultVariant].rows_p
 static_assert(!kVariants[kScalarVariant].vec, "scalar fallback variant");
 
 template <int MODE>
-void launch_mode(const Variant &v, Params p, hipStream_t s, int num_cus)
+void launch_mode(const Variant &v, Params p, hipStream_t s)
 {
     int rpb = v.rows_per_block ? v.rows_per_block : 256;
     int nb = (int)((p.n_rows + (int64_t)rpb - 1) / rpb);
@@ -637,16 +459,6 @@ void launch_mode(const Variant &v, Params p, hipStream_t s, int num_cus)
         const int64_t span = 8LL * v.chunk;
         p.chunk = v.chunk;
         grid = (int)((nb + span - 1) / span * span);
-    }
-    if (v.persist > 0 && p.nnz >= 4) {  // (its placeholder loads read quad 0)
-        // a multiple of 8 workgroups (each XCD walks its own share of the
-        // chunked schedule), no more than there are row blocks to start on
-        const int want = num_cus * v.persist;
-        const int cap = (int)((nb + 7) / 8 * 8);
-        grid = std::max(8, std::min(want, cap) / 8 * 8);
-        k_pp<256, MODE, 2><<<grid, 256, 0, s>>>(p);
-        check_launch();
-        return;
     }
     if (v.pf) {
         // quads per lane: 7 entries per row of the 7-point operator plus the
@@ -684,10 +496,10 @@ void launch(int mode, int variant, const Params &p, piro_ctx *ctx)
     const Variant &v = kVariants[variant];
     hipStream_t s = ctx->stream;
     switch (mode) {
-    case FE_INTERIOR: launch_mode<FE_INTERIOR>(v, p, s, ctx->num_cus); break;
-    case ALLROWS: launch_mode<ALLROWS>(v, p, s, ctx->num_cus); break;
-    case SPMV: launch_mode<SPMV>(v, p, s, ctx->num_cus); break;
-    default: launch_mode<JLI>(v, p, s, ctx->num_cus); break;
+    case FE_INTERIOR: launch_mode<FE_INTERIOR>(v, p, s); break;
+    case ALLROWS: launch_mode<ALLROWS>(v, p, s); break;
+    case SPMV: launch_mode<SPMV>(v, p, s); break;
+    default: launch_mode<JLI>(v, p, s); break;
     }
 }
 

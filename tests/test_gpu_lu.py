@@ -30,7 +30,10 @@ def perm_from_ipiv(ipiv, n):
     return p
 
 
-@pytest.mark.parametrize("n", [16, 48, 100, 257, 1000])
+# sizes cover every panel path (1024-lane panels with 1, 2 and 4 rows per
+# lane, the 256-lane panel above 4096 rows), partial 16-column panels and
+# partial 64-column blocks
+@pytest.mark.parametrize("n", [16, 48, 100, 257, 1000, 2100, 4160])
 def test_lu_factor_reconstructs(ctx, n):
     print(f"case lu n={n}", flush=True)
     rng = np.random.default_rng(n)
@@ -53,7 +56,7 @@ def test_lu_factor_reconstructs(ctx, n):
     assert agree > 0.95
 
 
-@pytest.mark.parametrize("n", [48, 160])
+@pytest.mark.parametrize("n", [48, 160, 300])
 def test_mfma_update_layout_exact(ctx, n):
     # A = L0 U0 with L0 unit lower (entries in {0, +-1/4, +-1/2}) and U0
     # upper with small integers: no row exchange is chosen (|l| < 1) and every
