@@ -185,10 +185,42 @@ __global__ __launch_bounds__(256) void k_panel(float *__restrict__ a, int lda, i
 // broadcasts the pivot row to the update -- two barriers per column.
 
 struct Panel1kShared {
-    float mag[16];
+    int mag[16];   // candidate magnitudes as float bits (>= 0), -1 = none
     int row[16];
     float swp[2][PB];
 };
+
+// (magnitude bits, row) pair a beats b: larger magnitude, then smaller row
+__device__ __forceinline__ bool pivot_better(int am, int ar, int bm, int br)
+{
+    return am > bm || (am == bm && ar < br);
+}
+
+// wave-wide best (magnitude, row): DPP within each row of 16 lanes (xor 1,
+// xor 2, rotate 4, rotate 8), then the four rows' results through readlane;
+// the result is uniform across the wave
+__device__ __forceinline__ void wave_pivot(int &m, int &r)
+{
+#define PIRO_DPP_STEP(ctrl)                                                   \
+    {                                                                         \
+        const int om = __builtin_amdgcn_update_dpp(m, m, ctrl, 0xF, 0xF, false); \
+        const int orr = __builtin_amdgcn_update_dpp(r, r, ctrl, 0xF, 0xF, false); \
+        if (pivot_better(om, orr, m, r)) { m = om; r = orr; }                 \
+    }
+    PIRO_DPP_STEP(0xB1)   // quad_perm [1,0,3,2]
+    PIRO_DPP_STEP(0x4E)   // quad_perm [2,3,0,1]
+    PIRO_DPP_STEP(0x124)  // row_ror:4
+    PIRO_DPP_STEP(0x128)  // row_ror:8
+#undef PIRO_DPP_STEP
+    int bm = __builtin_amdgcn_readlane(m, 0), br = __builtin_amdgcn_readlane(r, 0);
+#pragma unroll
+    for (int q = 16; q < 64; q += 16) {
+        const int qm = __builtin_amdgcn_readlane(m, q), qr = __builtin_amdgcn_readlane(r, q);
+        if (pivot_better(qm, qr, bm, br)) { bm = qm; br = qr; }
+    }
+    m = bm;
+    r = br;
+}
 
 template <int R, int J>
 __device__ __forceinline__ void panel1k_step(float (&p)[R][PB], Panel1kShared &sh, int t, int n, int k, int jb,
@@ -196,30 +228,24 @@ __device__ __forceinline__ void panel1k_step(float (&p)[R][PB], Panel1kShared &s
 {
     if (J >= jb) return;  // uniform: jb < 16 only for the last panel
     const int lane = t & 63, w = t >> 6, gj = k + J;
-    float best = -1.0f;
-    int brow = 0x7fffffff;
+    // magnitudes compare as the bit patterns of non-negative floats
+    int best = -1, brow = 0x7fffffff;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int gi = k + t + 1024 * r;
-        const float m = (gi >= gj && gi < n) ? fabsf(p[r][J]) : -1.0f;
+        const int m = (gi >= gj && gi < n) ? __float_as_int(fabsf(p[r][J])) : -1;
         if (m > best) { best = m; brow = gi; }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const float ob = __shfl_xor(best, off, 64);
-        const int orow = __shfl_xor(brow, off, 64);
-        if (ob > best || (ob == best && orow < brow)) { best = ob; brow = orow; }
-    }
+    wave_pivot(best, brow);
     if (lane == 0) { sh.mag[w] = best; sh.row[w] = brow; }
     __syncthreads();
-    float b = sh.mag[0];
-    int piv = sh.row[0];
+    int b = sh.mag[0], piv = sh.row[0];
 #pragma unroll
     for (int q = 1; q < 16; ++q)
-        if (sh.mag[q] > b || (sh.mag[q] == b && sh.row[q] < piv)) { b = sh.mag[q]; piv = sh.row[q]; }
+        if (pivot_better(sh.mag[q], sh.row[q], b, piv)) { b = sh.mag[q]; piv = sh.row[q]; }
     if (t == 0) {
         ipiv[gj] = piv;
-        if (b == 0.0f && *info == 0) *info = gj + 1;
+        if (b == 0 && *info == 0) *info = gj + 1;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -361,9 +387,25 @@ __global__ __launch_bounds__(256) void k_panel_global(float *__restrict__ a, int
 
 // ----------------------------------------------------- swap + U12 TRSM
 
-// columns [c0, c1): apply the exchanges of steps k .. k+jb-1 (jb <= W), then
-// U12 = L11^-1 A12 with L11 the unit lower jb x jb block at (k, k)
-template <int W>
+// columns [c0, c1): the exchanges of steps k0 .. k1-1, in order
+__global__ __launch_bounds__(256) void k_laswp(float *__restrict__ a, int lda, int k0, int k1, int c0, int c1,
+                                               const int32_t *__restrict__ ipiv)
+{
+    const int c = c0 + blockIdx.x * 256 + threadIdx.x;
+    if (c >= c1) return;
+    for (int j = k0; j < k1; ++j) {
+        const int pj = ipiv[j];
+        if (pj != j) {
+            const float x = a[(int64_t)j * lda + c];
+            a[(int64_t)j * lda + c] = a[(int64_t)pj * lda + c];
+            a[(int64_t)pj * lda + c] = x;
+        }
+    }
+}
+
+// columns [c0, c1): apply the exchanges of steps k .. k+jb-1 (jb <= W) if
+// SWAP, then U12 = L11^-1 A12 with L11 the unit lower jb x jb block at (k, k)
+template <int W, bool SWAP>
 __global__ __launch_bounds__(256) void k_swap_trsm(float *__restrict__ a, int lda, int k, int jb, int c0, int c1,
                                                    const int32_t *__restrict__ ipiv)
 {
@@ -378,7 +420,7 @@ __global__ __launch_bounds__(256) void k_swap_trsm(float *__restrict__ a, int ld
     __syncthreads();
     const int c = c0 + blockIdx.x * 256 + t;
     if (c >= c1) return;
-    for (int j = 0; j < jb; ++j) {
+    for (int j = 0; SWAP && j < jb; ++j) {
         const int pj = s_p[j];
         if (pj != k + j) {
             const float x = a[(int64_t)(k + j) * lda + c];
@@ -618,10 +660,17 @@ __global__ __launch_bounds__(256) void k_perm_rhs(int n, const int32_t *__restri
 }
 
 // LOWER: unit lower triangle, rows below are updated; else upper triangle
-// with its diagonal, rows above are updated
+// with its diagonal, rows above are updated.  The triangle is solved with
+// one wave per right-hand side, x_j broadcast through readlane; the rows to
+// update are staged in LDS (64 x 64) once for all right-hand sides.
+//
+// Buffers: the block's right-hand side is read from `in`, the rows outside
+// the block are updated in place in `in`, and the block's solution goes to
+// `out` -- a different buffer, because every workgroup of the launch reads
+// the block's rows of `in`.
 template <bool LOWER>
 __global__ __launch_bounds__(256) void k_trsv_step(const float *__restrict__ lu, int lda, int n, int i0,
-                                                   float *__restrict__ b, int ldb, int nrhs)
+                                                   float *in, int ldin, float *out, int ldout, int nrhs)
 {
     __shared__ float s_d[TB][TB + 1];
     __shared__ float s_x[TB][33];
@@ -633,16 +682,16 @@ __global__ __launch_bounds__(256) void k_trsv_step(const float *__restrict__ lu,
     }
     __syncthreads();
     for (int c = w; c < nrhs; c += 4) {
-        float v = lane < bs ? b[(int64_t)(i0 + lane) * ldb + c] : 0.0f;
+        float v = lane < bs ? in[(int64_t)(i0 + lane) * ldin + c] : 0.0f;
         if (LOWER) {
             for (int j = 0; j < bs; ++j) {
-                const float xj = __shfl(v, j, 64);
+                const float xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
                 if (lane > j) v -= s_d[lane][j] * xj;
             }
         } else {
             for (int j = bs - 1; j >= 0; --j) {
                 if (lane == j) v = v / s_d[j][j];
-                const float xj = __shfl(v, j, 64);
+                const float xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
                 if (lane < j) v -= s_d[lane][j] * xj;
             }
         }
@@ -652,37 +701,58 @@ __global__ __launch_bounds__(256) void k_trsv_step(const float *__restrict__ lu,
     if (blockIdx.x == 0)
         for (int e = t; e < bs * nrhs; e += 256) {
             const int i = e / nrhs, c = e % nrhs;
-            b[(int64_t)(i0 + i) * ldb + c] = s_x[i][c];
+            out[(int64_t)(i0 + i) * ldout + c] = s_x[i][c];
         }
-    // this workgroup's 64 rows outside the block: 4 lanes per row
+    // this workgroup's 64 rows outside the block, staged over s_d
+    const int rbase = LOWER ? i0 + bs + blockIdx.x * TB : blockIdx.x * TB;
+    const int rend = LOWER ? n : i0;
+    const int nrow = min(TB, rend - rbase);
+    if (nrow <= 0) return;
+    __syncthreads();
+    for (int e = t; e < TB * TB; e += 256) {
+        const int i = e / TB, j = e % TB;
+        s_d[i][j] = (i < nrow && j < bs) ? lu[(int64_t)(rbase + i) * lda + i0 + j] : 0.0f;
+    }
+    __syncthreads();
+    // 4 lanes per row, each taking every 4th right-hand side
     const int rr = t >> 2, q = t & 3;
-    const int r = LOWER ? i0 + bs + blockIdx.x * TB + rr : blockIdx.x * TB + rr;
-    const bool live = LOWER ? r < n : r < i0;
-    if (!live) return;
-    const float *row = lu + (int64_t)r * lda + i0;
-    for (int c = q; c < nrhs; c += 4) {
-        float acc = 0.0f;
-        for (int j = 0; j < bs; ++j) acc += row[j] * s_x[j][c];
-        b[(int64_t)r * ldb + c] -= acc;
+    if (rr >= nrow) return;
+    float acc[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) acc[m] = 0.0f;
+    for (int j = 0; j < bs; ++j) {
+        const float lj = s_d[rr][j];
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+            if (q + 4 * m < nrhs) acc[m] += lj * s_x[j][q + 4 * m];
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int c = q + 4 * m;
+        if (c < nrhs) in[(int64_t)(rbase + rr) * ldin + c] -= acc[m];
     }
 }
 
 void solve_f32(piro_ctx *ctx, int n, const float *lu, int lda, const int32_t *ipiv, int nrhs, float *b, int ldb)
 {
     hipStream_t s = ctx->stream;
+    // forward solution of up to 32 right-hand sides (n x 32)
+    float *y = (float *)ctx->scratch[3].get((size_t)n * 32 * sizeof(float));
     k_perm_rhs<<<1, 256, 0, s>>>(n, ipiv, b, ldb, nrhs);
     check_launch();
     for (int c0 = 0; c0 < nrhs; c0 += 32) {
         const int nc = min(32, nrhs - c0);
         float *bc = b + c0;
+        // forward: b (rows below updated in place) -> y
         for (int i0 = 0; i0 < n; i0 += TB) {
             const int bs = min(TB, n - i0);
             const int below = n - i0 - bs;
-            k_trsv_step<true><<<max(1, (below + TB - 1) / TB), 256, 0, s>>>(lu, lda, n, i0, bc, ldb, nc);
+            k_trsv_step<true><<<max(1, (below + TB - 1) / TB), 256, 0, s>>>(lu, lda, n, i0, bc, ldb, y, 32, nc);
             check_launch();
         }
+        // backward: y (rows above updated in place) -> b
         for (int i0 = (n - 1) / TB * TB; i0 >= 0; i0 -= TB) {
-            k_trsv_step<false><<<max(1, (i0 + TB - 1) / TB), 256, 0, s>>>(lu, lda, n, i0, bc, ldb, nc);
+            k_trsv_step<false><<<max(1, (i0 + TB - 1) / TB), 256, 0, s>>>(lu, lda, n, i0, y, 32, bc, ldb, nc);
             check_launch();
         }
     }
@@ -712,7 +782,7 @@ void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
             panel(ctx, n, a, lda, k, jb, ipiv, d_info);
             const int c0 = k + jb, c1 = kb + nbk;
             if (c0 < c1) {
-                k_swap_trsm<PB><<<grid_for(c1 - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, c1, ipiv);
+                k_swap_trsm<PB, true><<<grid_for(c1 - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, c1, ipiv);
                 check_launch();
                 dim3 g((c1 - c0 + 127) / 128, (n - c0 + 127) / 128);
                 k_update_mfma<PB><<<g, 256, 0, s>>>(a, lda, n, k, jb, c0, c0, c1);
@@ -723,14 +793,18 @@ void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
             k_swap_block_left<<<grid_for(nbk, 256), 256, 0, s>>>(a, lda, kb, nbk, ipiv);
             check_launch();
         }
-        // trailing columns: each panel's exchanges and U rows in turn, with
-        // the K = 16 updates confined to the block's own rows; then one
-        // K = nbk update of every row below the block
+        // trailing columns: all of the block's exchanges first (rows below
+        // the block carry deferred updates, so no exchange may come after
+        // an update), then each panel's U rows in turn with the K = 16
+        // updates confined to the block's own rows, then one K = nbk update
+        // of every row below the block
         const int c0 = kb + nbk;
         if (c0 < n) {
+            k_laswp<<<grid_for(n - c0, 256), 256, 0, s>>>(a, lda, kb, kb + nbk, c0, n, ipiv);
+            check_launch();
             for (int k = kb; k < kb + nbk; k += PB) {
                 const int jb = min(PB, kb + nbk - k);
-                k_swap_trsm<PB><<<grid_for(n - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, n, ipiv);
+                k_swap_trsm<PB, false><<<grid_for(n - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, n, ipiv);
                 check_launch();
                 if (k + jb < c0) {
                     dim3 g((n - c0 + 127) / 128, (c0 - (k + jb) + 127) / 128);
