@@ -8,17 +8,21 @@
 // solved to FP64 accuracy by iterative refinement.
 //
 // Layout: row-major, leading dimension lda (row swaps are contiguous runs).
-// Factorisation, right-looking with 16-column panels:
-//   panel  -- one workgroup (4 waves, one per SIMD, so each wave may use the
-//             whole register file) holds the m x 16 panel in registers:
-//             pivot search (magnitudes compared in FP64), row exchange
-//             through LDS, scaling and the rank-1 updates;
-//   swap+trsm -- the panel's 16 row exchanges applied to the trailing
-//             columns, then U12 = L11^-1 A12;
-//   update -- A22 -= L21 U12 on MFMA (v_mfma_f32_16x16x4f32, FP32 in, FP32
-//             accumulate), one 128 x 128 tile per workgroup.
-// The exchanges are applied to the already-factored L columns once, at the
-// end.  Solves: one workgroup per right-hand side, blocked substitution.
+// Factorisation, right-looking, 64-column blocks of four 16-column panels:
+//   panel  -- one 1024-lane workgroup holds the m x 16 panel in registers
+//             (m <= 4096; taller panels work in global memory): pivot search
+//             with DPP wave reductions, row exchange through LDS, scaling and
+//             the rank-1 updates;
+//   in-block -- each panel's exchanges and U12 = L11^-1 A12 for the block's
+//             later columns, then a K = 16 MFMA update of those columns only;
+//   block  -- the block's 64 exchanges applied to the trailing columns
+//             (LASWP), the deferred K = 16 updates of the block's rows, the
+//             block TRSM, then one K = 64 MFMA update (v_mfma_f32_16x16x4f32,
+//             FP32 in, FP32 accumulate) of the trailing matrix, one 128 x 128
+//             tile per workgroup.
+// The exchanges of later blocks reach the already-factored L columns once, at
+// the end.  Solves: P b, then blocked forward and backward substitution in
+// 64-row steps over up to 32 right-hand sides per launch.
 // Refinement: r = b - A x with FP64 accumulation over the FP32 entries of A,
 // correction d = U^-1 L^-1 P r in FP32, x += d in FP64.
 #include "device_utils.hpp"
@@ -34,145 +38,6 @@ constexpr int PB = 16;  // panel width
 constexpr int NB = 64;  // factorisation block: 4 panels, then one K = 64 update
 
 // ------------------------------------------------------------- panel
-
-struct PanelShared {
-    double mag[4];
-    int row[4];
-    float swp[2][PB];
-    float prow[PB];
-    int piv;
-};
-
-// One elimination step (column J of the panel).  J is a template parameter
-// so that every index into p[][] is a compile-time constant and the panel
-// stays in registers.
-template <int R, int J>
-__device__ __forceinline__ void panel_step(float (&p)[R][PB], PanelShared &sh, int t, int n, int k, int jb,
-                                           int32_t *__restrict__ ipiv, int *__restrict__ info)
-{
-    if (J >= jb) return;  // uniform: jb < 16 only for the last panel
-    const int lane = t & 63, w = t >> 6, gj = k + J;
-    // pivot search over rows >= gj: first row of maximal |a| (isamax)
-    double best = -1.0;
-    int brow = 0x7fffffff;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int gi = k + t + 256 * r;
-        double m = (gi >= gj && gi < n) ? fabs((double)p[r][J]) : -1.0;
-        if (m > best) { best = m; brow = gi; }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        double ob = __shfl_xor(best, off, 64);
-        int orow = __shfl_xor(brow, off, 64);
-        if (ob > best || (ob == best && orow < brow)) { best = ob; brow = orow; }
-    }
-    if (lane == 0) { sh.mag[w] = best; sh.row[w] = brow; }
-    __syncthreads();
-    if (t == 0) {
-        double b = sh.mag[0];
-        int br = sh.row[0];
-        for (int q = 1; q < 4; ++q)
-            if (sh.mag[q] > b || (sh.mag[q] == b && sh.row[q] < br)) { b = sh.mag[q]; br = sh.row[q]; }
-        sh.piv = br;
-        ipiv[gj] = br;
-        if (b == 0.0 && *info == 0) *info = gj + 1;
-    }
-    __syncthreads();
-    const int piv = sh.piv;
-    // exchange rows gj and piv of the panel through LDS
-    if (piv != gj) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int gi = k + t + 256 * r;
-            if (gi == piv)
-#pragma unroll
-                for (int c = 0; c < PB; ++c) sh.swp[0][c] = p[r][c];
-            if (gi == gj)
-#pragma unroll
-                for (int c = 0; c < PB; ++c) sh.swp[1][c] = p[r][c];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int gi = k + t + 256 * r;
-            if (gi == piv)
-#pragma unroll
-                for (int c = 0; c < PB; ++c) p[r][c] = sh.swp[1][c];
-            if (gi == gj)
-#pragma unroll
-                for (int c = 0; c < PB; ++c) p[r][c] = sh.swp[0][c];
-        }
-    }
-    // broadcast the pivot row (columns J..15 are all the update needs)
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-        if (k + t + 256 * r == gj)
-#pragma unroll
-            for (int c = J; c < PB; ++c) sh.prow[c] = p[r][c];
-    __syncthreads();
-    const float d = sh.prow[J];
-    if (d != 0.0f) {
-        float u[PB];
-#pragma unroll
-        for (int c = J + 1; c < PB; ++c) u[c] = sh.prow[c];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int gi = k + t + 256 * r;
-            if (gi > gj && gi < n) {
-                const float l = p[r][J] / d;
-                p[r][J] = l;
-#pragma unroll
-                for (int c = J + 1; c < PB; ++c) p[r][c] = p[r][c] - l * u[c];
-            }
-        }
-    }
-    __syncthreads();
-}
-
-template <int R, int J>
-struct PanelLoop {
-    static __device__ __forceinline__ void run(float (&p)[R][PB], PanelShared &sh, int t, int n, int k, int jb,
-                                               int32_t *ipiv, int *info)
-    {
-        panel_step<R, J>(p, sh, t, n, k, jb, ipiv, info);
-        PanelLoop<R, J + 1>::run(p, sh, t, n, k, jb, ipiv, info);
-    }
-};
-
-template <int R>
-struct PanelLoop<R, PB> {
-    static __device__ __forceinline__ void run(float (&)[R][PB], PanelShared &, int, int, int, int, int32_t *,
-                                               int *)
-    {
-    }
-};
-
-// one workgroup of 4 waves (one per SIMD, so each wave may use the whole
-// register file); thread t owns panel rows k + t + 256 r
-template <int R>
-__global__ __launch_bounds__(256) void k_panel(float *__restrict__ a, int lda, int n, int k, int jb,
-                                               int32_t *__restrict__ ipiv, int *__restrict__ info)
-{
-    __shared__ PanelShared sh;
-    const int t = threadIdx.x;
-    float p[R][PB];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int gi = k + t + 256 * r;
-#pragma unroll
-        for (int c = 0; c < PB; ++c) p[r][c] = (gi < n && c < jb) ? a[(int64_t)gi * lda + k + c] : 0.0f;
-    }
-    PanelLoop<R, 0>::run(p, sh, t, n, k, jb, ipiv, info);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int gi = k + t + 256 * r;
-        if (gi < n)
-#pragma unroll
-            for (int c = 0; c < PB; ++c)
-                if (c < jb) a[(int64_t)gi * lda + k + c] = p[r][c];
-    }
-}
 
 // ------------------------------------------- 1024-lane panel (m <= 4096)
 //
@@ -575,6 +440,57 @@ __global__ __launch_bounds__(256) void k_residual(const float *__restrict__ A, i
     }
 }
 
+// the same for 5..32 right-hand sides: workgroup = 16 rows x 32 right-hand
+// sides, A and x staged through LDS in 64-column chunks; thread (row t >> 4,
+// right-hand sides 2 (t & 15), +1); sums in column order
+__global__ __launch_bounds__(256) void k_residual_tile(const float *__restrict__ A, int lda, int n, int nrhs,
+                                                       const double *__restrict__ b, int ldb,
+                                                       const double *__restrict__ x, int ldx,
+                                                       double *__restrict__ r, float *__restrict__ r32)
+{
+    __shared__ float s_a[16][68];
+    __shared__ double2 s_x[64][16];
+    const int t = threadIdx.x, rr = t >> 4, cp = t & 15;
+    const int r0 = blockIdx.x * 16;
+    double a0 = 0.0, a1 = 0.0;
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        __syncthreads();
+        {
+            const int i = t >> 4, j = 4 * (t & 15);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                s_a[i][j + q] = (r0 + i < n && k0 + j + q < n) ? A[(int64_t)(r0 + i) * lda + k0 + j + q] : 0.0f;
+        }
+        for (int e = t; e < 64 * 16; e += 256) {
+            const int k = e >> 4, c = 2 * (e & 15);
+            const bool kin = k0 + k < n;
+            const double *src = x + (int64_t)(k0 + k) * ldx;
+            s_x[k][e & 15] = make_double2((kin && c < nrhs) ? src[c] : 0.0, (kin && c + 1 < nrhs) ? src[c + 1] : 0.0);
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int k = 0; k < 64; ++k) {
+            const double av = (double)s_a[rr][k];
+            const double2 xv = s_x[k][cp];
+            a0 += av * xv.x;
+            a1 += av * xv.y;
+        }
+    }
+    const int row = r0 + rr;
+    if (row >= n) return;
+    const int c = 2 * cp;
+    if (c < nrhs) {
+        const double rv = b[(int64_t)row * ldb + c] - a0;
+        r[(int64_t)row * nrhs + c] = rv;
+        r32[(int64_t)row * nrhs + c] = (float)rv;
+    }
+    if (c + 1 < nrhs) {
+        const double rv = b[(int64_t)row * ldb + c + 1] - a1;
+        r[(int64_t)row * nrhs + c + 1] = rv;
+        r32[(int64_t)row * nrhs + c + 1] = (float)rv;
+    }
+}
+
 // per right-hand-side sums of squares of r and b (column sums)
 __global__ __launch_bounds__(256) void k_colnorms(int n, int nrhs, const double *__restrict__ r,
                                                   const double *__restrict__ b, int ldb, double *__restrict__ out)
@@ -635,138 +551,316 @@ __global__ __launch_bounds__(256) void k_csr_to_dense(int n_rows, int n_cols, co
 
 // ------------------------------------------------- blocked triangular solve
 //
-// One launch per 64-row block step over all right-hand sides (<= 32 per
-// launch).  Every workgroup first solves the block's 64 x 64 triangle for the
-// block's right-hand-side rows (redundantly: the triangle is 16 KB, read
-// from cache) with one wave per right-hand side and lane shuffles, then
-// subtracts the block's contribution from its own 64 rows further down
-// (forward) or up (backward).  Workgroup 0 writes the block's solution.
+// P A = L U is solved as L y = P b, U x = y over 64-row blocks, up to 32
+// right-hand sides per pass:
+//   * the row permutation P is formed once per call on the host from the
+//     exchange sequence (it is inherently sequential) and applied as a
+//     gather when the forward pass reads b;
+//   * the inverses of the 64 x 64 diagonal blocks of L and U are formed in
+//     one launch (one workgroup per block), so a block's solve is a product;
+//   * each triangle is one launch with a workgroup per row block.  Blocks
+//     take tickets in solve order; the block of ticket k subtracts the
+//     contributions of the k finished blocks as they appear (a per-wave
+//     count of finished blocks), multiplies by the diagonal inverse, writes
+//     its rows and advances the count.  A block only waits on tickets issued
+//     before its own, so every wait ends; the wait is also bounded and
+//     reports an error instead of spinning forever.
 
 constexpr int TB = 64;
+constexpr int TRSV_SPIN_LIMIT = 1 << 22;
 
-// P b: the factorisation's exchanges in order, one lane per right-hand side
-__global__ __launch_bounds__(256) void k_perm_rhs(int n, const int32_t *__restrict__ ipiv, float *__restrict__ b,
-                                                  int ldb, int nrhs)
-{
-    for (int c = threadIdx.x; c < nrhs; c += 256)
-        for (int j = 0; j < n; ++j) {
-            const int pj = ipiv[j];
-            if (pj != j) {
-                const float x = b[(int64_t)j * ldb + c];
-                b[(int64_t)j * ldb + c] = b[(int64_t)pj * ldb + c];
-                b[(int64_t)pj * ldb + c] = x;
-            }
-        }
-}
-
-// LOWER: unit lower triangle, rows below are updated; else upper triangle
-// with its diagonal, rows above are updated.  The triangle is solved with
-// one wave per right-hand side, x_j broadcast through readlane; the rows to
-// update are staged in LDS (64 x 64) once for all right-hand sides.
-//
-// Buffers: the block's right-hand side is read from `in`, the rows outside
-// the block are updated in place in `in`, and the block's solution goes to
-// `out` -- a different buffer, because every workgroup of the launch reads
-// the block's rows of `in`.
-template <bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_step(const float *__restrict__ lu, int lda, int n, int i0,
-                                                   float *in, int ldin, float *out, int ldout, int nrhs)
+// inverses of the diagonal blocks: blockIdx.x = 2 b + u; u = 0: L (unit
+// lower), u = 1: U (upper with its diagonal).  Output row-major 64 x 64 per
+// block, zero outside the block's bs x bs part.  Lane = row; each wave forms
+// 16 columns, 4 at a time, with the x_j broadcast through readlane.
+__global__ __launch_bounds__(256) void k_diag_inv(const float *__restrict__ lu, int lda, int n,
+                                                  float *__restrict__ linv, float *__restrict__ uinv)
 {
     __shared__ float s_d[TB][TB + 1];
-    __shared__ float s_x[TB][33];
+    const int b = blockIdx.x >> 1;
+    const bool upper = blockIdx.x & 1;
+    const int i0 = b * TB, bs = min(TB, n - i0);
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int bs = min(TB, n - i0);
     for (int e = t; e < TB * TB; e += 256) {
-        const int i = e / TB, j = e % TB;
+        const int i = e >> 6, j = e & 63;
         s_d[i][j] = (i < bs && j < bs) ? lu[(int64_t)(i0 + i) * lda + i0 + j] : 0.0f;
     }
     __syncthreads();
-    for (int c = w; c < nrhs; c += 4) {
-        float v = lane < bs ? in[(int64_t)(i0 + lane) * ldin + c] : 0.0f;
-        if (LOWER) {
-            for (int j = 0; j < bs; ++j) {
-                const float xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-                if (lane > j) v -= s_d[lane][j] * xj;
+    float *out = (upper ? uinv : linv) + (int64_t)b * TB * TB;
+    const float dr = (lane < bs && upper) ? 1.0f / s_d[lane][lane] : 1.0f;
+    for (int cg = w * 16; cg < w * 16 + 16; cg += 4) {
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = (lane == cg + q && lane < bs) ? 1.0f : 0.0f;
+        if (!upper) {
+            for (int j = cg; j < bs; ++j) {
+                const float l = lane > j ? s_d[lane][j] : 0.0f;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[q]), j));
+                    v[q] -= l * xj;
+                }
             }
         } else {
-            for (int j = bs - 1; j >= 0; --j) {
-                if (lane == j) v = v / s_d[j][j];
-                const float xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-                if (lane < j) v -= s_d[lane][j] * xj;
+            for (int j = min(cg + 3, bs - 1); j >= 0; --j) {
+                const float u = lane < j ? s_d[lane][j] : 0.0f;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (lane == j) v[q] *= dr;
+                    const float xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[q]), j));
+                    v[q] -= u * xj;
+                }
             }
         }
-        s_x[lane][c] = v;
-    }
-    __syncthreads();
-    if (blockIdx.x == 0)
-        for (int e = t; e < bs * nrhs; e += 256) {
-            const int i = e / nrhs, c = e % nrhs;
-            out[(int64_t)(i0 + i) * ldout + c] = s_x[i][c];
-        }
-    // this workgroup's 64 rows outside the block, staged over s_d
-    const int rbase = LOWER ? i0 + bs + blockIdx.x * TB : blockIdx.x * TB;
-    const int rend = LOWER ? n : i0;
-    const int nrow = min(TB, rend - rbase);
-    if (nrow <= 0) return;
-    __syncthreads();
-    for (int e = t; e < TB * TB; e += 256) {
-        const int i = e / TB, j = e % TB;
-        s_d[i][j] = (i < nrow && j < bs) ? lu[(int64_t)(rbase + i) * lda + i0 + j] : 0.0f;
-    }
-    __syncthreads();
-    // 4 lanes per row, each taking every 4th right-hand side
-    const int rr = t >> 2, q = t & 3;
-    if (rr >= nrow) return;
-    float acc[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = 0.0f;
-    for (int j = 0; j < bs; ++j) {
-        const float lj = s_d[rr][j];
-#pragma unroll
-        for (int m = 0; m < 8; ++m)
-            if (q + 4 * m < nrhs) acc[m] += lj * s_x[j][q + 4 * m];
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        const int c = q + 4 * m;
-        if (c < nrhs) in[(int64_t)(rbase + rr) * ldin + c] -= acc[m];
+        for (int q = 0; q < 4; ++q) out[lane * TB + cg + q] = v[q];
     }
 }
 
-void solve_f32(piro_ctx *ctx, int n, const float *lu, int lda, const int32_t *ipiv, int nrhs, float *b, int ldb)
+struct TrsvArgs {
+    const float *lu;
+    int lda, n, nb;
+    const float *dinv;      // nb diagonal-block inverses
+    const int32_t *perm;    // forward pass: b row of solution row i; else null
+    const float *in;        // right-hand side rows
+    int ldin;
+    float *out;             // solution rows (read back by later blocks)
+    int ldout, nrhs;
+    int *done;              // per wave slot: blocks finished, in solve order; zero at launch
+    int *ticket;            // zero at launch
+    int *err;               // set when a wait exceeds its bound
+};
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// wait until *done >= need (every lane reads the same word); returns the
+// value seen, or need after raising *err when the bound is exceeded
+__device__ __forceinline__ int wait_done(int *done, int need, int *err)
 {
-    hipStream_t s = ctx->stream;
-    // forward solution of up to 32 right-hand sides (n x 32)
-    float *y = (float *)ctx->scratch[3].get((size_t)n * 32 * sizeof(float));
-    k_perm_rhs<<<1, 256, 0, s>>>(n, ipiv, b, ldb, nrhs);
-    check_launch();
-    for (int c0 = 0; c0 < nrhs; c0 += 32) {
-        const int nc = min(32, nrhs - c0);
-        float *bc = b + c0;
-        // forward: b (rows below updated in place) -> y
-        for (int i0 = 0; i0 < n; i0 += TB) {
-            const int bs = min(TB, n - i0);
-            const int below = n - i0 - bs;
-            k_trsv_step<true><<<max(1, (below + TB - 1) / TB), 256, 0, s>>>(lu, lda, n, i0, bc, ldb, y, 32, nc);
-            check_launch();
+    int spins = 0, d;
+    while ((d = __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
+        if (++spins > TRSV_SPIN_LIMIT) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            d = need;
+            break;
         }
-        // backward: y (rows above updated in place) -> b
-        for (int i0 = (n - 1) / TB * TB; i0 >= 0; i0 -= TB) {
-            k_trsv_step<false><<<max(1, (i0 + TB - 1) / TB), 256, 0, s>>>(lu, lda, n, i0, y, 32, bc, ldb, nc);
-            check_launch();
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return d;
+}
+
+// row `row` of lu, columns j0 .. j0+63 (zero from js on, and for dead rows)
+template <bool VEC>
+__device__ __forceinline__ void load_row(f4 (&r)[16], const float *__restrict__ lu, int lda, int row, bool live,
+                                         int j0, int js)
+{
+    const float *src = lu + (int64_t)row * lda + j0;
+    if (VEC && js == TB) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) r[q] = live ? *(const f4 *)(src + 4 * q) : f4{0.0f, 0.0f, 0.0f, 0.0f};
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) r[q][e] = (live && 4 * q + e < js) ? src[4 * q + e] : 0.0f;
+    }
+}
+
+// One workgroup per 64-row block, blocks in ticket order.  Right-hand sides
+// are independent, so each wave solves its own NRW of them with no barrier
+// after the start: it counts finished blocks in its own `done` word, keeps
+// its row of the next off-diagonal block in registers (lane = row; loaded
+// while the wave waits for that block) and takes x_j by readlane (NRW = 1)
+// or from a wave-private LDS slice (NRW = 8).
+template <bool LOWER, int NRW, bool VEC>
+__global__ __launch_bounds__(256) void k_trsv(TrsvArgs p)
+{
+    __shared__ float s_dinv[TB][TB + 1];
+    __shared__ float s_x[4][TB][8];
+    __shared__ int s_tk;
+    const int t = threadIdx.x, lane = t & 63, g = t >> 6;
+    if (t == 0) s_tk = atomicAdd(p.ticket, 1);
+    __syncthreads();
+    const int tk = s_tk;
+    const int b = LOWER ? tk : p.nb - 1 - tk;
+    const int i0 = b * TB, bs = min(TB, p.n - i0);
+    for (int e = t; e < TB * TB; e += 256) s_dinv[e >> 6][e & 63] = p.dinv[(int64_t)b * TB * TB + e];
+    __syncthreads();
+    const int c0 = g * NRW;
+    if (c0 >= p.nrhs) return;
+    int *done = p.done + g;
+    const bool live = lane < bs;
+    const int row = i0 + lane;
+    float acc[NRW];
+    {
+        const int64_t src = live ? (LOWER ? p.perm[row] : row) : 0;
+#pragma unroll
+        for (int q = 0; q < NRW; ++q) acc[q] = (live && c0 + q < p.nrhs) ? p.in[src * p.ldin + c0 + q] : 0.0f;
+    }
+    f4 cur[16];
+    if (tk > 0) {
+        const int j = LOWER ? 0 : p.nb - 1;
+        load_row<VEC>(cur, p.lu, p.lda, row, live, j * TB, min(TB, p.n - j * TB));
+    }
+    int avail = 0;
+    for (int s = 0; s < tk; ++s) {
+        const int j = LOWER ? s : p.nb - 1 - s;
+        const int j0 = j * TB, js = min(TB, p.n - j0);
+        if (s >= avail) avail = wait_done(done, s + 1, p.err);
+        float xv[NRW];
+#pragma unroll
+        for (int q = 0; q < NRW; ++q)
+            xv[q] = (lane < js && c0 + q < p.nrhs) ? p.out[(int64_t)(j0 + lane) * p.ldout + c0 + q] : 0.0f;
+        if constexpr (NRW == 1) {
+#pragma unroll
+            for (int jj = 0; jj < TB; ++jj) {
+                const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv[0]), jj));
+                acc[0] -= cur[jj >> 2][jj & 3] * x;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NRW; ++q) s_x[g][lane][q] = xv[q];
+#pragma unroll
+            for (int jj = 0; jj < TB; ++jj) {
+                const float l = cur[jj >> 2][jj & 3];
+#pragma unroll
+                for (int q = 0; q < NRW; ++q) acc[q] -= l * s_x[g][jj][q];
+                // keep the LDS reads next to their use (hoisting them all
+                // spills): the accumulators pass through an empty asm that
+                // later memory accesses may not cross
+                if ((jj & 3) == 3)
+                    __asm__ volatile("" : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]),
+                                     "+v"(acc[5]), "+v"(acc[6]), "+v"(acc[7])::"memory");
+            }
+        }
+        // the next block's row is in flight while this wave waits for it
+        if (s + 1 < tk) {
+            const int jn = LOWER ? s + 1 : p.nb - 2 - s;
+            load_row<VEC>(cur, p.lu, p.lda, row, live, jn * TB, min(TB, p.n - jn * TB));
         }
     }
+    // x_b = D_b^-1 acc
+    float x[NRW];
+#pragma unroll
+    for (int q = 0; q < NRW; ++q) x[q] = 0.0f;
+    if constexpr (NRW == 1) {
+#pragma unroll 8
+        for (int jj = 0; jj < TB; ++jj)
+            x[0] += s_dinv[lane][jj] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc[0]), jj));
+    } else {
+#pragma unroll
+        for (int q = 0; q < NRW; ++q) s_x[g][lane][q] = acc[q];
+#pragma unroll 8
+        for (int jj = 0; jj < TB; ++jj) {
+            const float d = s_dinv[lane][jj];
+#pragma unroll
+            for (int q = 0; q < NRW; ++q) x[q] += d * s_x[g][jj][q];
+        }
+    }
+    if (live)
+#pragma unroll
+        for (int q = 0; q < NRW; ++q)
+            if (c0 + q < p.nrhs) p.out[(int64_t)row * p.ldout + c0 + q] = x[q];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(done, tk + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(256) void k_zero_i32(int *__restrict__ p, int count)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) p[i] = 0;
+}
+
+// state of the solves of one call: permutation, diagonal inverses, the
+// forward solution and the progress counters of both passes
+struct SolvePlan {
+    int n = 0, nb = 0;
+    int32_t *perm = nullptr;
+    float *linv = nullptr, *uinv = nullptr, *y = nullptr;
+    int *sync = nullptr;  // [fwd done 4][fwd ticket][bwd done 4][bwd ticket][err]
+};
+
+SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const int32_t *d_ipiv)
+{
+    hipStream_t s = ctx->stream;
+    SolvePlan pl;
+    pl.n = n;
+    pl.nb = (n + TB - 1) / TB;
+    const size_t blk = (size_t)pl.nb * TB * TB;
+    const size_t bytes = (size_t)n * 32 * 4 + 2 * blk * 4 + ((size_t)n + 16) * 4 + 256;
+    char *ws = (char *)ctx->scratch[3].get(bytes);
+    pl.y = (float *)ws;
+    pl.linv = pl.y + (size_t)n * 32;
+    pl.uinv = pl.linv + blk;
+    pl.perm = (int32_t *)(pl.uinv + blk);
+    pl.sync = pl.perm + n;
+    // P from the exchange sequence: row i of P b is row perm[i] of b
+    std::vector<int32_t> piv(n), perm(n);
+    PIRO_CHECK_HIP(hipMemcpyAsync(piv.data(), d_ipiv, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    PIRO_CHECK_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int i = 0; i < n; ++i) {
+        const int q = piv[i];
+        if (q < i || q >= n) throw Error(PIRO_EINVAL, "pivot index out of range");
+        std::swap(perm[i], perm[q]);
+    }
+    PIRO_CHECK_HIP(hipMemcpyAsync(pl.perm, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+    PIRO_CHECK_HIP(hipStreamSynchronize(s));
+    k_diag_inv<<<2 * pl.nb, 256, 0, s>>>(lu, lda, n, pl.linv, pl.uinv);
+    check_launch();
+    k_zero_i32<<<1, 256, 0, s>>>(pl.sync + 10, 1);
+    check_launch();
+    return pl;
+}
+
+template <bool LOWER>
+void launch_trsv(hipStream_t s, const TrsvArgs &a, bool vec)
+{
+    if (a.nrhs <= 4) {
+        if (vec) k_trsv<LOWER, 1, true><<<a.nb, 256, 0, s>>>(a);
+        else k_trsv<LOWER, 1, false><<<a.nb, 256, 0, s>>>(a);
+    } else {
+        if (vec) k_trsv<LOWER, 8, true><<<a.nb, 256, 0, s>>>(a);
+        else k_trsv<LOWER, 8, false><<<a.nb, 256, 0, s>>>(a);
+    }
+    check_launch();
+}
+
+// solve in place: b (n x nrhs, ldb) := U^-1 L^-1 P b
+void solve_f32(piro_ctx *ctx, const SolvePlan &pl, const float *lu, int lda, int nrhs, float *b, int ldb)
+{
+    hipStream_t s = ctx->stream;
+    const int nb = pl.nb;
+    const bool vec = (lda % 4 == 0) && ((uintptr_t)lu % 16 == 0);
+    int *fwd = pl.sync, *bwd = pl.sync + 5, *err = pl.sync + 10;
+    for (int c0 = 0; c0 < nrhs; c0 += 32) {
+        const int nc = min(32, nrhs - c0);
+        k_zero_i32<<<1, 256, 0, s>>>(pl.sync, 10);
+        check_launch();
+        launch_trsv<true>(s, TrsvArgs{lu, lda, pl.n, nb, pl.linv, pl.perm, b + c0, ldb, pl.y, 32, nc, fwd, fwd + 4, err},
+                          vec);
+        launch_trsv<false>(s, TrsvArgs{lu, lda, pl.n, nb, pl.uinv, nullptr, pl.y, 32, b + c0, ldb, nc, bwd, bwd + 4, err},
+                           vec);
+    }
+}
+
+// 0 if no wait of the plan's solves ran past its bound (synchronises)
+int solve_status(piro_ctx *ctx, const SolvePlan &pl)
+{
+    int h = 0;
+    PIRO_CHECK_HIP(hipMemcpyAsync(&h, pl.sync + 10, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return h;
 }
 
 void panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int32_t *ipiv, int *d_info)
 {
     hipStream_t s = ctx->stream;
     const int m = n - k;
+    // register panels hold 1024 x R rows; taller panels work in global memory
     if (m <= 1024) k_panel1k<1><<<1, 1024, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
     else if (m <= 2048) k_panel1k<2><<<1, 1024, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
     else if (m <= 4096) k_panel1k<4><<<1, 1024, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
-    else if (m <= 2048 * 4) k_panel<8><<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
-    else if (m <= 4096 * 4) k_panel<16><<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
     else k_panel_global<<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
     check_launch();
 }
@@ -859,7 +953,9 @@ int piro_lu_solve_f32(piro_ctx *ctx, int n, const float *d_lu, int lda, const in
     PIRO_API_BEGIN
     ctx_of(ctx);
     PIRO_REQUIRE(n > 0 && lda >= n && nrhs > 0 && ldb >= nrhs && d_lu && d_ipiv && d_b, "bad arguments");
-    solve_f32(ctx, n, d_lu, lda, d_ipiv, nrhs, d_b, ldb);
+    const SolvePlan pl = prepare_solve(ctx, n, d_lu, lda, d_ipiv);
+    solve_f32(ctx, pl, d_lu, lda, nrhs, d_b, ldb);
+    if (solve_status(ctx, pl)) throw Error(PIRO_EINTERNAL, "triangular solve: a block wait exceeded its bound");
     PIRO_API_END
 }
 
@@ -878,22 +974,26 @@ int piro_lu_solve_refine(piro_ctx *ctx, int n, const float *d_a, int lda, const 
     double *r = (double *)ws;
     float *r32 = (float *)(ws + nn * 8);
     double *norms = (double *)(ws + nn * 12 + ((16 - (nn * 12) % 16) % 16));
+    const SolvePlan pl = prepare_solve(ctx, n, d_lu, ldlu, d_ipiv);
     // x0 = LU^-1 b in FP32
     k_b_to_f32<<<grid_for(nn, 256), 256, 0, s>>>(n, nrhs, d_b, ldb, r32);
     check_launch();
-    solve_f32(ctx, n, d_lu, ldlu, d_ipiv, nrhs, r32, nrhs);
+    solve_f32(ctx, pl, d_lu, ldlu, nrhs, r32, nrhs);
     k_f32_to_x<<<grid_for(nn, 256), 256, 0, s>>>(n, nrhs, r32, d_x, ldx);
     check_launch();
     std::vector<double> h(2 * nrhs);
     double worst = 0.0;
     int sw = 0;
     for (;;) {
-        k_residual<32><<<(n + 3) / 4, 256, 0, s>>>(d_a, lda, n, nrhs, d_b, ldb, d_x, ldx, r, r32);
+        if (nrhs <= 4)
+            k_residual<4><<<(n + 3) / 4, 256, 0, s>>>(d_a, lda, n, nrhs, d_b, ldb, d_x, ldx, r, r32);
+        else
+            k_residual_tile<<<(n + 15) / 16, 256, 0, s>>>(d_a, lda, n, nrhs, d_b, ldb, d_x, ldx, r, r32);
         check_launch();
         k_colnorms<<<nrhs, 256, 0, s>>>(n, nrhs, r, d_b, ldb, norms);
         check_launch();
         PIRO_CHECK_HIP(hipMemcpyAsync(h.data(), norms, 2 * nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
-        PIRO_CHECK_HIP(hipStreamSynchronize(s));
+        if (solve_status(ctx, pl)) throw Error(PIRO_EINTERNAL, "triangular solve: a block wait exceeded its bound");
         worst = 0.0;
         for (int c = 0; c < nrhs; ++c) {
             double rr = std::sqrt(h[2 * c]), bb = std::sqrt(h[2 * c + 1]);
@@ -903,7 +1003,7 @@ int piro_lu_solve_refine(piro_ctx *ctx, int n, const float *d_a, int lda, const 
             worst = std::max(worst, rel);
         }
         if (worst <= tol || sw >= max_sweeps) break;
-        solve_f32(ctx, n, d_lu, ldlu, d_ipiv, nrhs, r32, nrhs);
+        solve_f32(ctx, pl, d_lu, ldlu, nrhs, r32, nrhs);
         k_axpy_correct<<<grid_for(nn, 256), 256, 0, s>>>(n, nrhs, r32, d_x, ldx);
         check_launch();
         ++sw;
