@@ -9,20 +9,20 @@
 //
 // Layout: row-major, leading dimension lda (row swaps are contiguous runs).
 // Factorisation, right-looking, 64-column blocks of four 16-column panels:
-//   panel  -- one 1024-lane workgroup holds the m x 16 panel in registers
-//             (m <= 4096; taller panels work in global memory): pivot search
-//             with DPP wave reductions, row exchange through LDS, scaling and
-//             the rank-1 updates;
+//   panel  -- one workgroup (256-1024 lanes) holds the m x 16 panel in
+//             registers (m <= 4096; taller panels work in global memory):
+//             pivot search with DPP wave reductions, row exchanges as label
+//             swaps, scaling and the rank-1 updates;
 //   in-block -- each panel's exchanges and U12 = L11^-1 A12 for the block's
 //             later columns, then a K = 16 MFMA update of those columns only;
-//   block  -- the block's 64 exchanges applied to the trailing columns
-//             (LASWP), the deferred K = 16 updates of the block's rows, the
-//             block TRSM, then one K = 64 MFMA update (v_mfma_f32_16x16x4f32,
-//             FP32 in, FP32 accumulate) of the trailing matrix, one 128 x 128
-//             tile per workgroup.
-// The exchanges of later blocks reach the already-factored L columns once, at
-// the end.  Solves: P b, then blocked forward and backward substitution in
-// 64-row steps over up to 32 right-hand sides per launch.
+//   block  -- the block's 64 exchanges applied to the trailing columns and
+//             to the finished L columns on the left (LASWP), the deferred
+//             K = 16 updates of the block's rows and the block's U rows,
+//             then one K = 64 MFMA update (v_mfma_f32_16x16x4f32, FP32 in,
+//             FP32 accumulate) of the trailing matrix, one 128 x 128 tile
+//             per workgroup.
+// Solves: P b as a gather, then blocked forward and backward substitution
+// over 64-row blocks, one persistent launch per triangle.
 // Refinement: r = b - A x with FP64 accumulation over the FP32 entries of A,
 // correction d = U^-1 L^-1 P r in FP32, x += d in FP64.
 #include "device_utils.hpp"
@@ -37,161 +37,223 @@ namespace {
 constexpr int PB = 16;  // panel width
 constexpr int NB = 64;  // factorisation block: 4 panels, then one K = 64 update
 
+typedef float f4 __attribute__((ext_vector_type(4)));
+
 // ------------------------------------------------------------- panel
 
-// ------------------------------------------- 1024-lane panel (m <= 4096)
+// ------------------------------------------------ register panel (m <= 4096)
 //
-// Thread t owns panel rows k + t + 1024 r (R <= 4 rows, 16 columns in
-// registers), so each SIMD runs four waves and the column steps' latencies
-// overlap.  Per column: the pivot is the first row of largest magnitude
-// (LAPACK isamax; FP32 magnitudes compare exactly), found by a wave
-// reduction and a redundant 16-way reduction in every thread after one
-// barrier; the pivot row and the row it replaces go through LDS, which also
-// broadcasts the pivot row to the update -- two barriers per column.
+// NT threads (256, 512 or 1024); thread t holds R register rows (16
+// columns each), loaded from panel rows k + t + NT r.  Row exchanges move no
+// data: every register row carries a label, the matrix row it currently is,
+// and exchanging rows gj and piv swaps two labels; the rows go back to the
+// rows their labels name at the end.  Per column J, with one barrier: the
+// pivot is the first row of largest magnitude among labels >= gj (LAPACK
+// isamax; FP32 magnitudes compare exactly as bit patterns).  Candidates are
+// 64-bit keys (magnitude bits + 1) << 32 | (2^31 - 1 - label), so "larger
+// magnitude, then smaller row" is one unsigned compare.  Each wave finds its
+// best key (DPP reduction); the owner lane posts (key, which of its rows,
+// all its rows) in the wave's LDS slot.  After the barrier every wave
+// reduces the NT/64 slots over lanes (DPP), reads the pivot row from the
+// winning slot by address, swaps the two labels and updates the rows with
+// labels > gj.  Slots alternate between two sets by column parity, so a
+// wave that is already writing column J + 1 cannot disturb a wave still
+// reading column J.
 
-struct Panel1kShared {
-    int mag[16];   // candidate magnitudes as float bits (>= 0), -1 = none
-    int row[16];
-    float swp[2][PB];
+template <int R>
+struct PanelCand {
+    uint64_t key;  // 0 = no candidate
+    int sel;       // which of the owner's register rows
+    int pad;
+    float v[R][PB];
 };
 
-// (magnitude bits, row) pair a beats b: larger magnitude, then smaller row
-__device__ __forceinline__ bool pivot_better(int am, int ar, int bm, int br)
+template <int NT, int R>
+struct PanelShared {
+    PanelCand<R> cand[2][NT / 64];
+};
+
+__device__ __forceinline__ uint64_t pivot_key(int mag_bits, int label)
 {
-    return am > bm || (am == bm && ar < br);
+    return ((uint64_t)(uint32_t)(mag_bits + 1) << 32) | (uint32_t)(0x7fffffff - label);
 }
 
-// wave-wide best (magnitude, row): DPP within each row of 16 lanes (xor 1,
-// xor 2, rotate 4, rotate 8), then the four rows' results through readlane;
-// the result is uniform across the wave
-__device__ __forceinline__ void wave_pivot(int &m, int &r)
+__device__ __forceinline__ int key_label(uint64_t key) { return 0x7fffffff - (int)(uint32_t)key; }
+
+// one DPP step of a max-reduction of 64-bit keys
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_max_key(uint64_t key)
 {
-#define PIRO_DPP_STEP(ctrl)                                                   \
-    {                                                                         \
-        const int om = __builtin_amdgcn_update_dpp(m, m, ctrl, 0xF, 0xF, false); \
-        const int orr = __builtin_amdgcn_update_dpp(r, r, ctrl, 0xF, 0xF, false); \
-        if (pivot_better(om, orr, m, r)) { m = om; r = orr; }                 \
-    }
-    PIRO_DPP_STEP(0xB1)   // quad_perm [1,0,3,2]
-    PIRO_DPP_STEP(0x4E)   // quad_perm [2,3,0,1]
-    PIRO_DPP_STEP(0x124)  // row_ror:4
-    PIRO_DPP_STEP(0x128)  // row_ror:8
-#undef PIRO_DPP_STEP
-    int bm = __builtin_amdgcn_readlane(m, 0), br = __builtin_amdgcn_readlane(r, 0);
+    const uint32_t lo = (uint32_t)key, hi = (uint32_t)(key >> 32);
+    const uint32_t olo = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo, CTRL, 0xF, 0xF, false);
+    const uint32_t ohi = (uint32_t)__builtin_amdgcn_mov_dpp((int)hi, CTRL, 0xF, 0xF, false);
+    const uint64_t o = ((uint64_t)ohi << 32) | olo;
+    return o > key ? o : key;
+}
+
+// best key within each row of 16 lanes: xor 1, xor 2, rotate 4, rotate 8
+// (the first log2(SPAN) of them for groups of SPAN lanes); every lane of a
+// group ends with the group's result
+template <int SPAN>
+__device__ __forceinline__ uint64_t group_max_key(uint64_t key)
+{
+    if constexpr (SPAN >= 2) key = dpp_max_key<0xB1>(key);   // quad_perm [1,0,3,2]
+    if constexpr (SPAN >= 4) key = dpp_max_key<0x4E>(key);   // quad_perm [2,3,0,1]
+    if constexpr (SPAN >= 8) key = dpp_max_key<0x124>(key);  // row_ror:4
+    if constexpr (SPAN >= 16) key = dpp_max_key<0x128>(key); // row_ror:8
+    return key;
+}
+
+// wave-wide best key, uniform across the wave
+__device__ __forceinline__ uint64_t wave_max_key(uint64_t key)
+{
+    key = group_max_key<16>(key);
+    uint64_t best = ((uint64_t)__builtin_amdgcn_readlane((int)(key >> 32), 0) << 32) |
+                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, 0);
 #pragma unroll
     for (int q = 16; q < 64; q += 16) {
-        const int qm = __builtin_amdgcn_readlane(m, q), qr = __builtin_amdgcn_readlane(r, q);
-        if (pivot_better(qm, qr, bm, br)) { bm = qm; br = qr; }
+        const uint64_t o = ((uint64_t)__builtin_amdgcn_readlane((int)(key >> 32), q) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, q);
+        best = o > best ? o : best;
     }
-    m = bm;
-    r = br;
+    return best;
 }
 
-template <int R, int J>
-__device__ __forceinline__ void panel1k_step(float (&p)[R][PB], Panel1kShared &sh, int t, int n, int k, int jb,
-                                             int32_t *__restrict__ ipiv, int *__restrict__ info)
+__device__ __forceinline__ void put_row(float *dst, const float (&v)[PB])
 {
+#pragma unroll
+    for (int q = 0; q < PB; q += 4) *(f4 *)(dst + q) = f4{v[q], v[q + 1], v[q + 2], v[q + 3]};
+}
+
+__device__ __forceinline__ void get_row(float (&v)[PB], const float *src)
+{
+#pragma unroll
+    for (int q = 0; q < PB; q += 4) {
+        const f4 x = *(const f4 *)(src + q);
+        v[q] = x[0];
+        v[q + 1] = x[1];
+        v[q + 2] = x[2];
+        v[q + 3] = x[3];
+    }
+}
+
+template <int NT, int R, int J>
+__device__ __forceinline__ void panel_step(float (&p)[R][PB], int (&lab)[R], PanelShared<NT, R> &sh, int t, int n,
+                                           int k, int jb, int32_t *__restrict__ ipiv, int *__restrict__ info)
+{
+    constexpr int W = NT / 64;
     if (J >= jb) return;  // uniform: jb < 16 only for the last panel
+    constexpr int par = J & 1;
     const int lane = t & 63, w = t >> 6, gj = k + J;
-    // magnitudes compare as the bit patterns of non-negative floats
-    int best = -1, brow = 0x7fffffff;
+    uint64_t best = 0;
+    int bsel = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int gi = k + t + 1024 * r;
-        const int m = (gi >= gj && gi < n) ? __float_as_int(fabsf(p[r][J])) : -1;
-        if (m > best) { best = m; brow = gi; }
-    }
-    wave_pivot(best, brow);
-    if (lane == 0) { sh.mag[w] = best; sh.row[w] = brow; }
-    __syncthreads();
-    int b = sh.mag[0], piv = sh.row[0];
-#pragma unroll
-    for (int q = 1; q < 16; ++q)
-        if (pivot_better(sh.mag[q], sh.row[q], b, piv)) { b = sh.mag[q]; piv = sh.row[q]; }
-    if (t == 0) {
-        ipiv[gj] = piv;
-        if (b == 0 && *info == 0) *info = gj + 1;
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int gi = k + t + 1024 * r;
-        if (gi == piv)
-#pragma unroll
-            for (int c = 0; c < PB; ++c) sh.swp[0][c] = p[r][c];
-        if (gi == gj)
-#pragma unroll
-            for (int c = 0; c < PB; ++c) sh.swp[1][c] = p[r][c];
-    }
-    __syncthreads();
-    if (piv != gj) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int gi = k + t + 1024 * r;
-            if (gi == gj)
-#pragma unroll
-                for (int c = 0; c < PB; ++c) p[r][c] = sh.swp[0][c];
-            if (gi == piv)
-#pragma unroll
-                for (int c = 0; c < PB; ++c) p[r][c] = sh.swp[1][c];
+        const uint64_t key =
+            (lab[r] >= gj && lab[r] < n) ? pivot_key(__float_as_int(fabsf(p[r][J])), lab[r]) : (uint64_t)0;
+        if (key > best) {
+            best = key;
+            bsel = r;
         }
     }
-    const float d = sh.swp[0][J];
+    const uint64_t wbest = wave_max_key(best);
+    PanelCand<R> &mine = sh.cand[par][w];
+    if (wbest == 0) {
+        if (lane == 0) mine.key = 0;
+    } else if (best == wbest) {  // the owner lane (labels are distinct)
+        mine.key = wbest;
+        mine.sel = bsel;
+#pragma unroll
+        for (int r = 0; r < R; ++r) put_row(mine.v[r], p[r]);
+    }
+    __syncthreads();
+    // slot index in the low 4 bits of the row part (labels < 2^27)
+    const uint64_t sk = sh.cand[par][lane % W].key;
+    uint64_t key = (sk & 0xffffffff00000000ull) | ((sk & 0x7ffffffull) << 4) | (uint32_t)(lane % W);
+    key = group_max_key<W>(key);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(key >> 32), 0);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, 0);
+    const int slot = lo & 15;
+    const int piv = 0x7ffffff - (int)(lo >> 4);
+    if (t == 0) {
+        ipiv[gj] = piv;
+        if (hi == 1 && *info == 0) *info = gj + 1;  // largest magnitude is zero
+    }
+    const PanelCand<R> &win = sh.cand[par][slot];
+    float u[PB];
+    get_row(u, win.v[win.sel]);
+    if (piv != gj)
+#pragma unroll
+        for (int r = 0; r < R; ++r) lab[r] = lab[r] == piv ? gj : (lab[r] == gj ? piv : lab[r]);
+    const float d = u[J];
     if (d != 0.0f) {
-        float u[PB];
 #pragma unroll
-        for (int c = J + 1; c < PB; ++c) u[c] = sh.swp[0][c];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int gi = k + t + 1024 * r;
-            if (gi > gj && gi < n) {
+        for (int r = 0; r < R; ++r)
+            if (lab[r] > gj) {
                 const float l = p[r][J] / d;
                 p[r][J] = l;
 #pragma unroll
-                for (int c = J + 1; c < PB; ++c) p[r][c] = p[r][c] - l * u[c];
+                for (int c = J + 1; c < PB; ++c) p[r][c] = __builtin_fmaf(-l, u[c], p[r][c]);
             }
-        }
     }
 }
 
-template <int R, int J>
-struct Panel1kLoop {
-    static __device__ __forceinline__ void run(float (&p)[R][PB], Panel1kShared &sh, int t, int n, int k, int jb,
-                                               int32_t *ipiv, int *info)
+template <int NT, int R, int J>
+struct PanelLoop {
+    static __device__ __forceinline__ void run(float (&p)[R][PB], int (&lab)[R], PanelShared<NT, R> &sh, int t, int n,
+                                               int k, int jb, int32_t *ipiv, int *info)
     {
-        panel1k_step<R, J>(p, sh, t, n, k, jb, ipiv, info);
-        Panel1kLoop<R, J + 1>::run(p, sh, t, n, k, jb, ipiv, info);
+        panel_step<NT, R, J>(p, lab, sh, t, n, k, jb, ipiv, info);
+        PanelLoop<NT, R, J + 1>::run(p, lab, sh, t, n, k, jb, ipiv, info);
     }
 };
 
-template <int R>
-struct Panel1kLoop<R, PB> {
-    static __device__ __forceinline__ void run(float (&)[R][PB], Panel1kShared &, int, int, int, int, int32_t *,
-                                               int *)
+template <int NT, int R>
+struct PanelLoop<NT, R, PB> {
+    static __device__ __forceinline__ void run(float (&)[R][PB], int (&)[R], PanelShared<NT, R> &, int, int, int, int,
+                                               int32_t *, int *)
     {
     }
 };
 
-template <int R>
-__global__ __launch_bounds__(1024) void k_panel1k(float *__restrict__ a, int lda, int n, int k, int jb,
-                                                  int32_t *__restrict__ ipiv, int *__restrict__ info)
+// VEC: lda and the matrix base allow 16-byte row segments (k is a multiple
+// of 16)
+template <int NT, int R, bool VEC>
+__global__ __launch_bounds__(NT) void k_panel(float *__restrict__ a, int lda, int n, int k, int jb,
+                                              int32_t *__restrict__ ipiv, int *__restrict__ info)
 {
-    __shared__ Panel1kShared sh;
+    __shared__ PanelShared<NT, R> sh;
     const int t = threadIdx.x;
     float p[R][PB];
+    int lab[R];
+    const bool full = VEC && jb == PB;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int gi = k + t + 1024 * r;
+        const int gi = k + t + NT * r;
+        lab[r] = gi;
+        const float *src = a + (int64_t)gi * lda + k;
+        if (full) {
+            if (gi < n) get_row(p[r], src);
+            else
 #pragma unroll
-        for (int c = 0; c < PB; ++c) p[r][c] = (gi < n && c < jb) ? a[(int64_t)gi * lda + k + c] : 0.0f;
+                for (int c = 0; c < PB; ++c) p[r][c] = 0.0f;
+        } else {
+#pragma unroll
+            for (int c = 0; c < PB; ++c) p[r][c] = (gi < n && c < jb) ? src[c] : 0.0f;
+        }
     }
-    Panel1kLoop<R, 0>::run(p, sh, t, n, k, jb, ipiv, info);
+    PanelLoop<NT, R, 0>::run(p, lab, sh, t, n, k, jb, ipiv, info);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int gi = k + t + 1024 * r;
-        if (gi < n)
+        const int gi = lab[r];
+        float *dst = a + (int64_t)gi * lda + k;
+        if (gi < n) {
+            if (full) put_row(dst, p[r]);
+            else
 #pragma unroll
-            for (int c = 0; c < PB; ++c)
-                if (c < jb) a[(int64_t)gi * lda + k + c] = p[r][c];
+                for (int c = 0; c < PB; ++c)
+                    if (c < jb) dst[c] = p[r][c];
+        }
     }
 }
 
@@ -253,10 +315,13 @@ __global__ __launch_bounds__(256) void k_panel_global(float *__restrict__ a, int
 // ----------------------------------------------------- swap + U12 TRSM
 
 // columns [c0, c1): the exchanges of steps k0 .. k1-1, in order
-__global__ __launch_bounds__(256) void k_laswp(float *__restrict__ a, int lda, int k0, int k1, int c0, int c1,
+// the exchanges of steps k0 .. k1-1 on columns [0, nl) (finished L columns)
+// and [c0, c1) (trailing columns): one thread per column, exchanges in order
+__global__ __launch_bounds__(256) void k_laswp(float *__restrict__ a, int lda, int k0, int k1, int nl, int c0, int c1,
                                                const int32_t *__restrict__ ipiv)
 {
-    const int c = c0 + blockIdx.x * 256 + threadIdx.x;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int c = i < nl ? i : c0 + (i - nl);
     if (c >= c1) return;
     for (int j = k0; j < k1; ++j) {
         const int pj = ipiv[j];
@@ -392,24 +457,6 @@ __global__ __launch_bounds__(256) void k_swap_block_left(float *__restrict__ a, 
     }
 }
 
-// exchanges of the blocks after a column's own block, applied to the L part
-// after the factorisation
-__global__ __launch_bounds__(256) void k_swap_left(float *__restrict__ a, int lda, int n,
-                                                   const int32_t *__restrict__ ipiv)
-{
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= n) return;
-    const int start = (c / NB + 1) * NB;
-    for (int j = start; j < n; ++j) {
-        const int pj = ipiv[j];
-        if (pj != j) {
-            float x = a[(int64_t)j * lda + c];
-            a[(int64_t)j * lda + c] = a[(int64_t)pj * lda + c];
-            a[(int64_t)pj * lda + c] = x;
-        }
-    }
-}
-
 // ------------------------------------------------------------ refinement
 
 // r = b - A x in FP64 (A FP32), one wave per row, all right-hand sides
@@ -441,8 +488,9 @@ __global__ __launch_bounds__(256) void k_residual(const float *__restrict__ A, i
 }
 
 // the same for 5..32 right-hand sides: workgroup = 16 rows x 32 right-hand
-// sides, A and x staged through LDS in 64-column chunks; thread (row t >> 4,
-// right-hand sides 2 (t & 15), +1); sums in column order
+// sides, A and x staged through LDS in 64-column chunks (the next chunk is
+// loaded into registers during the current one's products); thread (row
+// t >> 4, right-hand sides 2 (t & 15), +1); sums in column order
 __global__ __launch_bounds__(256) void k_residual_tile(const float *__restrict__ A, int lda, int n, int nrhs,
                                                        const double *__restrict__ b, int ldb,
                                                        const double *__restrict__ x, int ldx,
@@ -452,22 +500,34 @@ __global__ __launch_bounds__(256) void k_residual_tile(const float *__restrict__
     __shared__ double2 s_x[64][16];
     const int t = threadIdx.x, rr = t >> 4, cp = t & 15;
     const int r0 = blockIdx.x * 16;
+    // staging: thread t moves A[r0 + (t >> 4)][k0 + 4 (t & 15) .. +3] and
+    // x rows k0 + (t >> 4) + 16 q, right-hand sides 2 (t & 15), +1
+    const int ai = t >> 4, aj = 4 * (t & 15);
+    float ra[4];
+    double2 rx[4];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            ra[q] = (r0 + ai < n && k0 + aj + q < n) ? A[(int64_t)(r0 + ai) * lda + k0 + aj + q] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int kk = k0 + (t >> 4) + 16 * q, c = 2 * (t & 15);
+            const bool kin = kk < n;
+            const double *src = x + (int64_t)kk * ldx;
+            rx[q] = make_double2((kin && c < nrhs) ? src[c] : 0.0, (kin && c + 1 < nrhs) ? src[c + 1] : 0.0);
+        }
+    };
     double a0 = 0.0, a1 = 0.0;
+    fetch(0);
     for (int k0 = 0; k0 < n; k0 += 64) {
         __syncthreads();
-        {
-            const int i = t >> 4, j = 4 * (t & 15);
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                s_a[i][j + q] = (r0 + i < n && k0 + j + q < n) ? A[(int64_t)(r0 + i) * lda + k0 + j + q] : 0.0f;
-        }
-        for (int e = t; e < 64 * 16; e += 256) {
-            const int k = e >> 4, c = 2 * (e & 15);
-            const bool kin = k0 + k < n;
-            const double *src = x + (int64_t)(k0 + k) * ldx;
-            s_x[k][e & 15] = make_double2((kin && c < nrhs) ? src[c] : 0.0, (kin && c + 1 < nrhs) ? src[c + 1] : 0.0);
+        for (int q = 0; q < 4; ++q) {
+            s_a[ai][aj + q] = ra[q];
+            s_x[(t >> 4) + 16 * q][t & 15] = rx[q];
         }
         __syncthreads();
+        if (k0 + 64 < n) fetch(k0 + 64);  // in flight during the chunk's products
 #pragma unroll 8
         for (int k = 0; k < 64; ++k) {
             const double av = (double)s_a[rr][k];
@@ -626,12 +686,10 @@ struct TrsvArgs {
     int ldin;
     float *out;             // solution rows (read back by later blocks)
     int ldout, nrhs;
-    int *done;              // per wave slot: blocks finished, in solve order; zero at launch
+    int *done;              // per wave (<= 8): blocks finished, in solve order; zero at launch
     int *ticket;            // zero at launch
     int *err;               // set when a wait exceeds its bound
 };
-
-typedef float f4 __attribute__((ext_vector_type(4)));
 
 // wait until *done >= need (every lane reads the same word); returns the
 // value seen, or need after raising *err when the bound is exceeded
@@ -667,17 +725,18 @@ __device__ __forceinline__ void load_row(f4 (&r)[16], const float *__restrict__ 
     }
 }
 
-// One workgroup per 64-row block, blocks in ticket order.  Right-hand sides
-// are independent, so each wave solves its own NRW of them with no barrier
+// One workgroup (WAVES waves) per 64-row block, blocks in ticket order.
+// Right-hand sides are independent, so each wave solves its own NRW of
+// them with no barrier
 // after the start: it counts finished blocks in its own `done` word, keeps
 // its row of the next off-diagonal block in registers (lane = row; loaded
 // while the wave waits for that block) and takes x_j by readlane (NRW = 1)
-// or from a wave-private LDS slice (NRW = 8).
-template <bool LOWER, int NRW, bool VEC>
-__global__ __launch_bounds__(256) void k_trsv(TrsvArgs p)
+// or from a wave-private LDS slice (NRW > 1).
+template <bool LOWER, int NRW, int WAVES, bool VEC>
+__global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
 {
     __shared__ float s_dinv[TB][TB + 1];
-    __shared__ float s_x[4][TB][8];
+    __shared__ float s_x[WAVES][TB][NRW];
     __shared__ int s_tk;
     const int t = threadIdx.x, lane = t & 63, g = t >> 6;
     if (t == 0) s_tk = atomicAdd(p.ticket, 1);
@@ -685,7 +744,7 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs p)
     const int tk = s_tk;
     const int b = LOWER ? tk : p.nb - 1 - tk;
     const int i0 = b * TB, bs = min(TB, p.n - i0);
-    for (int e = t; e < TB * TB; e += 256) s_dinv[e >> 6][e & 63] = p.dinv[(int64_t)b * TB * TB + e];
+    for (int e = t; e < TB * TB; e += 64 * WAVES) s_dinv[e >> 6][e & 63] = p.dinv[(int64_t)b * TB * TB + e];
     __syncthreads();
     const int c0 = g * NRW;
     if (c0 >= p.nrhs) return;
@@ -730,8 +789,8 @@ __global__ __launch_bounds__(256) void k_trsv(TrsvArgs p)
                 // spills): the accumulators pass through an empty asm that
                 // later memory accesses may not cross
                 if ((jj & 3) == 3)
-                    __asm__ volatile("" : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]),
-                                     "+v"(acc[5]), "+v"(acc[6]), "+v"(acc[7])::"memory");
+#pragma unroll
+                    for (int q = 0; q < NRW; ++q) __asm__ volatile("" : "+v"(acc[q])::"memory");
             }
         }
         // the next block's row is in flight while this wave waits for it
@@ -777,7 +836,7 @@ struct SolvePlan {
     int n = 0, nb = 0;
     int32_t *perm = nullptr;
     float *linv = nullptr, *uinv = nullptr, *y = nullptr;
-    int *sync = nullptr;  // [fwd done 4][fwd ticket][bwd done 4][bwd ticket][err]
+    int *sync = nullptr;  // [fwd done 8][fwd ticket][bwd done 8][bwd ticket][err]
 };
 
 SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const int32_t *d_ipiv)
@@ -787,7 +846,7 @@ SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const in
     pl.n = n;
     pl.nb = (n + TB - 1) / TB;
     const size_t blk = (size_t)pl.nb * TB * TB;
-    const size_t bytes = (size_t)n * 32 * 4 + 2 * blk * 4 + ((size_t)n + 16) * 4 + 256;
+    const size_t bytes = (size_t)n * 32 * 4 + 2 * blk * 4 + ((size_t)n + 24) * 4 + 256;
     char *ws = (char *)ctx->scratch[3].get(bytes);
     pl.y = (float *)ws;
     pl.linv = pl.y + (size_t)n * 32;
@@ -808,20 +867,21 @@ SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const in
     PIRO_CHECK_HIP(hipStreamSynchronize(s));
     k_diag_inv<<<2 * pl.nb, 256, 0, s>>>(lu, lda, n, pl.linv, pl.uinv);
     check_launch();
-    k_zero_i32<<<1, 256, 0, s>>>(pl.sync + 10, 1);
+    k_zero_i32<<<1, 256, 0, s>>>(pl.sync + 18, 1);
     check_launch();
     return pl;
 }
 
+// up to 4 right-hand sides: one per wave; up to 32: four per wave, 8 waves
 template <bool LOWER>
 void launch_trsv(hipStream_t s, const TrsvArgs &a, bool vec)
 {
     if (a.nrhs <= 4) {
-        if (vec) k_trsv<LOWER, 1, true><<<a.nb, 256, 0, s>>>(a);
-        else k_trsv<LOWER, 1, false><<<a.nb, 256, 0, s>>>(a);
+        if (vec) k_trsv<LOWER, 1, 4, true><<<a.nb, 256, 0, s>>>(a);
+        else k_trsv<LOWER, 1, 4, false><<<a.nb, 256, 0, s>>>(a);
     } else {
-        if (vec) k_trsv<LOWER, 8, true><<<a.nb, 256, 0, s>>>(a);
-        else k_trsv<LOWER, 8, false><<<a.nb, 256, 0, s>>>(a);
+        if (vec) k_trsv<LOWER, 4, 8, true><<<a.nb, 512, 0, s>>>(a);
+        else k_trsv<LOWER, 4, 8, false><<<a.nb, 512, 0, s>>>(a);
     }
     check_launch();
 }
@@ -832,14 +892,14 @@ void solve_f32(piro_ctx *ctx, const SolvePlan &pl, const float *lu, int lda, int
     hipStream_t s = ctx->stream;
     const int nb = pl.nb;
     const bool vec = (lda % 4 == 0) && ((uintptr_t)lu % 16 == 0);
-    int *fwd = pl.sync, *bwd = pl.sync + 5, *err = pl.sync + 10;
+    int *fwd = pl.sync, *bwd = pl.sync + 9, *err = pl.sync + 18;
     for (int c0 = 0; c0 < nrhs; c0 += 32) {
         const int nc = min(32, nrhs - c0);
-        k_zero_i32<<<1, 256, 0, s>>>(pl.sync, 10);
+        k_zero_i32<<<1, 256, 0, s>>>(pl.sync, 18);
         check_launch();
-        launch_trsv<true>(s, TrsvArgs{lu, lda, pl.n, nb, pl.linv, pl.perm, b + c0, ldb, pl.y, 32, nc, fwd, fwd + 4, err},
+        launch_trsv<true>(s, TrsvArgs{lu, lda, pl.n, nb, pl.linv, pl.perm, b + c0, ldb, pl.y, 32, nc, fwd, fwd + 8, err},
                           vec);
-        launch_trsv<false>(s, TrsvArgs{lu, lda, pl.n, nb, pl.uinv, nullptr, pl.y, 32, b + c0, ldb, nc, bwd, bwd + 4, err},
+        launch_trsv<false>(s, TrsvArgs{lu, lda, pl.n, nb, pl.uinv, nullptr, pl.y, 32, b + c0, ldb, nc, bwd, bwd + 8, err},
                            vec);
     }
 }
@@ -848,7 +908,7 @@ void solve_f32(piro_ctx *ctx, const SolvePlan &pl, const float *lu, int lda, int
 int solve_status(piro_ctx *ctx, const SolvePlan &pl)
 {
     int h = 0;
-    PIRO_CHECK_HIP(hipMemcpyAsync(&h, pl.sync + 10, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PIRO_CHECK_HIP(hipMemcpyAsync(&h, pl.sync + 18, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     return h;
 }
@@ -858,10 +918,19 @@ void panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int32_t *ipiv
     hipStream_t s = ctx->stream;
     const int m = n - k;
     // register panels hold 1024 x R rows; taller panels work in global memory
-    if (m <= 1024) k_panel1k<1><<<1, 1024, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
-    else if (m <= 2048) k_panel1k<2><<<1, 1024, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
-    else if (m <= 4096) k_panel1k<4><<<1, 1024, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
+    const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0);
+#define PIRO_PANEL(NT_, R_)                                                                   \
+    {                                                                                         \
+        if (vec) k_panel<NT_, R_, true><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);    \
+        else k_panel<NT_, R_, false><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);       \
+    }
+    if (m <= 256) PIRO_PANEL(256, 1)
+    else if (m <= 512) PIRO_PANEL(256, 2)
+    else if (m <= 1024) PIRO_PANEL(256, 4)
+    else if (m <= 2048) PIRO_PANEL(512, 4)
+    else if (m <= 4096) PIRO_PANEL(1024, 4)
     else k_panel_global<<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
+#undef PIRO_PANEL
     check_launch();
 }
 
@@ -893,9 +962,13 @@ void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
         // updates confined to the block's own rows, then one K = nbk update
         // of every row below the block
         const int c0 = kb + nbk;
-        if (c0 < n) {
-            k_laswp<<<grid_for(n - c0, 256), 256, 0, s>>>(a, lda, kb, kb + nbk, c0, n, ipiv);
+        // the block's exchanges reach the finished L columns [0, kb) and the
+        // trailing columns in one launch
+        if (kb + n - c0 > 0) {
+            k_laswp<<<grid_for(kb + n - c0, 256), 256, 0, s>>>(a, lda, kb, kb + nbk, kb, c0, n, ipiv);
             check_launch();
+        }
+        if (c0 < n) {
             for (int k = kb; k < kb + nbk; k += PB) {
                 const int jb = min(PB, kb + nbk - k);
                 k_swap_trsm<PB, false><<<grid_for(n - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, n, ipiv);
@@ -911,8 +984,6 @@ void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
             check_launch();
         }
     }
-    k_swap_left<<<grid_for(n, 256), 256, 0, s>>>(a, lda, n, ipiv);
-    check_launch();
 }
 
 }  // namespace
@@ -936,6 +1007,7 @@ int piro_lu_factor_f32(piro_ctx *ctx, int n, float *d_a, int lda, int32_t *d_ipi
     PIRO_API_BEGIN
     ctx_of(ctx);
     PIRO_REQUIRE(n > 0 && lda >= n && d_a && d_ipiv, "bad arguments");
+    PIRO_REQUIRE(n < (1 << 27), "n too large (row labels are packed in 27 bits)");
     int *d_info = (int *)ctx->scratch[0].get(sizeof(int));
     PIRO_CHECK_HIP(hipMemsetAsync(d_info, 0, sizeof(int), ctx->stream));
     factor(ctx, n, d_a, lda, d_ipiv, d_info);
