@@ -367,14 +367,17 @@ int piro_batch_run(piro_batch *b, float *d_x, float *d_work, int nsteps,
 /* ------------------------------------------------------ learned surrogate */
 
 /* No reference counterpart (PIRO has no learned part).  A 6-layer 3x3x3
- * residual CNN advancing a heat field k forward-Euler steps at a time:
- * inputs u/scale (FP32) and (alpha-0.0125)/0.0075; L1 2->32 and L6 32->1 on
- * VALU, L2-L5 32->32 on BF16 MFMA with FP32 accumulation, ReLU between
- * layers; output u + scale*delta on interior cells, 0 on boundary cells.
- * Fields are [n][D][H][W] FP32 (n instances of one shape, zero padding at
- * instance boundaries).  Parameters: flat FP32 vector of
- * piro_surrogate_num_params() values: W1[27][32][2], b1[32], then for
- * L2..L5 W[27][32 co][32 ci], b[32], then W6[27][32], b6[1]; tap index
+ * residual CNN advancing a heat field k forward-Euler steps at a time.
+ * Linear layers without biases (the k-step heat map is linear in u): L1
+ * 1->32 on u/scale (FP32, VALU; its centre tap is minus the sum of the other
+ * 26, so a constant field maps to 0), L2-L5 32->32 on BF16 MFMA with FP32
+ * accumulation, L6 32->3 (VALU) combined per cell with
+ * a = (alpha-0.0125)/0.0075 as delta = o0 + a (o1 + a o2); output
+ * u + scale*delta on interior cells, 0 on boundary cells.  Fields are
+ * [n][D][H][W] FP32 (n instances of one shape, zero padding at instance
+ * boundaries).  Parameters: flat FP32 vector of piro_surrogate_num_params()
+ * values: W1[27][32] (the stored centre tap 13 is unused), then for L2..L5
+ * W[27][32 co][32 ci], then W6[27][3][32]; tap index
  * (dz+1)*9 + (dy+1)*3 + (dx+1).  The output layer is zero-initialised, so an
  * untrained surrogate is the identity. */
 typedef struct piro_surrogate piro_surrogate;

@@ -517,7 +517,9 @@ __global__ __launch_bounds__(256) void k_residual_tile(const float *__restrict__
             rx[q] = make_double2((kin && c < nrhs) ? src[c] : 0.0, (kin && c + 1 < nrhs) ? src[c + 1] : 0.0);
         }
     };
-    double a0 = 0.0, a1 = 0.0;
+    // four partial sums per output (k mod 4) keep four FP64 add chains in
+    // flight; they are combined in a fixed order at the end
+    double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
     fetch(0);
     for (int k0 = 0; k0 < n; k0 += 64) {
         __syncthreads();
@@ -532,20 +534,21 @@ __global__ __launch_bounds__(256) void k_residual_tile(const float *__restrict__
         for (int k = 0; k < 64; ++k) {
             const double av = (double)s_a[rr][k];
             const double2 xv = s_x[k][cp];
-            a0 += av * xv.x;
-            a1 += av * xv.y;
+            a0[k & 3] += av * xv.x;
+            a1[k & 3] += av * xv.y;
         }
     }
+    const double s0 = (a0[0] + a0[1]) + (a0[2] + a0[3]), s1 = (a1[0] + a1[1]) + (a1[2] + a1[3]);
     const int row = r0 + rr;
     if (row >= n) return;
     const int c = 2 * cp;
     if (c < nrhs) {
-        const double rv = b[(int64_t)row * ldb + c] - a0;
+        const double rv = b[(int64_t)row * ldb + c] - s0;
         r[(int64_t)row * nrhs + c] = rv;
         r32[(int64_t)row * nrhs + c] = (float)rv;
     }
     if (c + 1 < nrhs) {
-        const double rv = b[(int64_t)row * ldb + c + 1] - a1;
+        const double rv = b[(int64_t)row * ldb + c + 1] - s1;
         r[(int64_t)row * nrhs + c + 1] = rv;
         r32[(int64_t)row * nrhs + c + 1] = (float)rv;
     }

@@ -2,12 +2,23 @@
 // CNN trained and evaluated by the project's own kernels (no reference
 // counterpart; PIRO has no learned part).
 //
-//   in  : u/s (FP32) and the normalised diffusivity (alpha-0.0125)/0.0075
-//   L1  : conv 2->32 + ReLU     (VALU, FP32 inputs and weights, BF16 output)
-//   L2-5: conv 32->32 + ReLU    (MFMA v_mfma_f32_16x16x32_bf16: BF16 weights
+//   in  : u/s (FP32)
+//   L1  : conv 1->32            (VALU, FP32 input and weights, BF16 output);
+//                                the centre tap is minus the sum of the other
+//                                26, so a constant field maps to zero
+//   L2-5: conv 32->32           (MFMA v_mfma_f32_16x16x32_bf16: BF16 weights
 //                                and activations, FP32 accumulation)
-//   L6  : conv 32->1            (VALU, FP32 accumulation)
+//   L6  : conv 32->3            (VALU, FP32 accumulation), combined per cell
+//                                with the normalised diffusivity
+//                                a = (alpha-0.0125)/0.0075 of that cell:
+//                                delta = o0 + a (o1 + a o2)
 //   out : u + s*delta on interior cells, 0 on the boundary (zero Dirichlet)
+//
+// The layers are linear and carry no biases: the k-step heat map is linear
+// in u with coefficients set by alpha, so the network is too -- zero input
+// gives exactly zero output (also after BF16 rounding), which keeps the
+// surrogate exact on the empty parts of a field -- and alpha enters where the
+// heat operator applies it, as a per-cell factor on the output.
 //
 // Fields are channels-last [n][D][H][W][C]; zero padding at every instance
 // boundary.  The input layer reads u in FP32 because the network's job is a
@@ -18,11 +29,10 @@
 // receives D[i = 4*(l/16) + r][j = l%16], r = 0..3.
 //
 // Backward: dgrad of L2-5 is the forward MFMA kernel run on the output
-// gradient with the flipped, transposed weights and the ReLU mask of the
-// layer input fused in its epilogue; wgrad of L2-5 is an MFMA GEMM over
-// voxels with the operands transposed through LDS; L1/L6 gradients on VALU.
-// Per-workgroup weight-gradient partials are summed in a fixed order, so a
-// training step is deterministic.
+// gradient with the flipped, transposed weights; wgrad of L2-5 is an MFMA
+// GEMM over voxels with the operands transposed through LDS; L1/L6
+// gradients on VALU.  Per-workgroup weight-gradient partials are summed in a
+// fixed order, so a training step is deterministic.
 #include "device_utils.hpp"
 
 #include <cmath>
@@ -39,14 +49,14 @@ namespace {
 
 constexpr int CH = 32;
 constexpr int NT = 27;
-// parameter offsets in the flat FP32 vector
-constexpr int W1_OFF = 0;                    // [27][32][2]
-constexpr int B1_OFF = W1_OFF + NT * CH * 2; // [32]
-constexpr int WM_OFF = B1_OFF + CH;          // 4 x ([27][32][32] + [32])
-constexpr int WM_SZ = NT * CH * CH + CH;
-constexpr int W6_OFF = WM_OFF + 4 * WM_SZ;   // [27][32]
-constexpr int B6_OFF = W6_OFF + NT * CH;     // [1]
-constexpr int N_PARAMS = B6_OFF + 1;
+constexpr int TC = 13;  // centre tap
+constexpr int NO = 3;   // L6 outputs combined by the diffusivity factor
+// parameter offsets in the flat FP32 vector (no biases)
+constexpr int W1_OFF = 0;                     // [27][32]; the centre tap is implied
+constexpr int WM_OFF = W1_OFF + NT * CH;      // 4 x [27][32 out][32 in]
+constexpr int WM_SZ = NT * CH * CH;
+constexpr int W6_OFF = WM_OFF + 4 * WM_SZ;    // [27][3][32]
+constexpr int N_PARAMS = W6_OFF + NT * NO * CH;
 
 constexpr float ALPHA_MU = 0.0125f, ALPHA_INV = 1.0f / 0.0075f;
 
@@ -73,94 +83,113 @@ __device__ __forceinline__ int64_t vidx(int b, int z, int y, int x, const Shape 
 
 // ------------------------------------------------------------------ L1
 
-__global__ __launch_bounds__(256) void k_l1_fwd(Shape s, const float *__restrict__ u, const float *__restrict__ al,
-                                                float inv_scale, const float *__restrict__ w1,
-                                                const float *__restrict__ b1, bf16 *__restrict__ x1)
+// L1 weights with the implied centre tap, staged in LDS: sw[t*CH + c]
+__device__ __forceinline__ void stage_w1(float *sw, const float *__restrict__ w1)
 {
-    __shared__ float sw[NT * CH * 2 + CH];
-    for (int i = threadIdx.x; i < NT * CH * 2 + CH; i += 256) sw[i] = i < NT * CH * 2 ? w1[i] : b1[i - NT * CH * 2];
+    for (int i = threadIdx.x; i < NT * CH; i += blockDim.x) sw[i] = i / CH == TC ? 0.0f : w1[i];
     __syncthreads();
+    if (threadIdx.x < CH) {
+        float sum = 0.0f;
+        for (int tp = 0; tp < NT; ++tp) sum += sw[tp * CH + threadIdx.x];
+        sw[TC * CH + threadIdx.x] = -sum;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_l1_fwd(Shape s, const float *__restrict__ u, float inv_scale,
+                                                const float *__restrict__ w1, bf16 *__restrict__ x1)
+{
+    __shared__ float sw[NT * CH];
+    stage_w1(sw, w1);
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
     int b, z, y, x;
     if (v >= s.nv() || !voxel_coords(v, s, b, z, y, x)) return;
     float acc[CH];
 #pragma unroll
-    for (int c = 0; c < CH; ++c) acc[c] = sw[NT * CH * 2 + c];
+    for (int c = 0; c < CH; ++c) acc[c] = 0.0f;
     for (int t = 0; t < NT; ++t) {
         const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
         if (zz < 0 || zz >= s.D || yy < 0 || yy >= s.H || xx < 0 || xx >= s.W) continue;
-        const int64_t q = vidx(b, zz, yy, xx, s);
-        const float in0 = u[q] * inv_scale, in1 = (al[q] - ALPHA_MU) * ALPHA_INV;
+        const float in0 = u[vidx(b, zz, yy, xx, s)] * inv_scale;
 #pragma unroll
-        for (int c = 0; c < CH; ++c) acc[c] += sw[(t * CH + c) * 2] * in0 + sw[(t * CH + c) * 2 + 1] * in1;
+        for (int c = 0; c < CH; ++c) acc[c] += sw[t * CH + c] * in0;
     }
     bf16x8 o[4];
 #pragma unroll
-    for (int c = 0; c < CH; ++c) o[c / 8][c % 8] = (bf16)fmaxf(acc[c], 0.0f);
+    for (int c = 0; c < CH; ++c) o[c / 8][c % 8] = (bf16)acc[c];
     bf16x8 *dst = (bf16x8 *)(x1 + v * CH);
 #pragma unroll
     for (int g = 0; g < 4; ++g) dst[g] = o[g];
 }
 
-// dW1[t][c][k] = sum_v dz[v][c] * in_k[v+t]; db1[c] = sum_v dz[v][c]
-// one block per chunk of voxels; partials[block][NT*CH*2 + CH]
-__global__ __launch_bounds__(256) void k_l1_wgrad(Shape s, const float *__restrict__ u,
-                                                  const float *__restrict__ al, float inv_scale,
+// gradient of the effective L1 weights: dW1[t][c] = sum_v dz[v][c] * in[v+t]
+// one block per chunk of voxels; partials[block][NT*CH]
+__global__ __launch_bounds__(256) void k_l1_wgrad(Shape s, const float *__restrict__ u, float inv_scale,
                                                   const bf16 *__restrict__ dz, int64_t chunk,
                                                   float *__restrict__ partial)
 {
     // thread layout: 32 channels x 8 voxel lanes
     const int c = threadIdx.x & 31, lane8 = threadIdx.x >> 5;
-    float g[NT * 2];
+    float g[NT];
 #pragma unroll
-    for (int i = 0; i < NT * 2; ++i) g[i] = 0.0f;
-    float gb = 0.0f;
+    for (int i = 0; i < NT; ++i) g[i] = 0.0f;
     const int64_t v0 = (int64_t)blockIdx.x * chunk, v1 = min(v0 + chunk, s.nv());
     for (int64_t v = v0 + lane8; v < v1; v += 8) {
         int b, z, y, x;
         voxel_coords(v, s, b, z, y, x);
         const float d = (float)dz[v * CH + c];
         if (d == 0.0f) continue;
-        gb += d;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
             if (zz < 0 || zz >= s.D || yy < 0 || yy >= s.H || xx < 0 || xx >= s.W) continue;
-            const int64_t q = vidx(b, zz, yy, xx, s);
-            g[2 * t] += d * (u[q] * inv_scale);
-            g[2 * t + 1] += d * ((al[q] - ALPHA_MU) * ALPHA_INV);
+            g[t] += d * (u[vidx(b, zz, yy, xx, s)] * inv_scale);
         }
     }
     __shared__ float red[8][CH];
-    float *out = partial + (int64_t)blockIdx.x * (NT * CH * 2 + CH);
-    for (int i = 0; i <= NT * 2; ++i) {
-        red[lane8][c] = i < NT * 2 ? g[i] : gb;
+    float *out = partial + (int64_t)blockIdx.x * (NT * CH);
+    for (int i = 0; i < NT; ++i) {
+        red[lane8][c] = g[i];
         __syncthreads();
         if (lane8 == 0) {
             float ssum = 0.0f;
             for (int q = 0; q < 8; ++q) ssum += red[q][c];
-            if (i < NT * 2) out[((i / 2) * CH + c) * 2 + (i % 2)] = ssum;
-            else out[NT * CH * 2 + c] = ssum;
+            out[i * CH + c] = ssum;
         }
         __syncthreads();
     }
 }
 
+// effective -> stored L1 gradient: the centre weight is -(sum of the others),
+// so dL/dw[t] = dL/dw_eff[t] - dL/dw_eff[centre]; the stored centre is unused
+__global__ __launch_bounds__(CH) void k_l1_project(float *__restrict__ g)
+{
+    const int c = threadIdx.x;
+    const float gc = g[TC * CH + c];
+    for (int t = 0; t < NT; ++t) g[t * CH + c] = t == TC ? 0.0f : g[t * CH + c] - gc;
+}
+
 // ------------------------------------------------------------------ L6
 
-// training: delta[v] = b6 + sum; inference: out[v] = interior ? u + s*delta : 0
+__device__ __forceinline__ float alpha_factor(const float *__restrict__ al, int64_t v)
+{
+    return (al[v] - ALPHA_MU) * ALPHA_INV;
+}
+
+// delta[v] = o0 + a (o1 + a o2), o = conv(x5, w6); training: delta;
+// inference: out[v] = interior ? u + s*delta : 0
 template <bool INFER>
 __global__ __launch_bounds__(256) void k_l6_fwd(Shape s, const bf16 *__restrict__ x5, const float *__restrict__ w6,
-                                                const float *__restrict__ b6, const float *__restrict__ u, float scale,
-                                                float *__restrict__ out)
+                                                const float *__restrict__ al, const float *__restrict__ u,
+                                                float scale, float *__restrict__ out)
 {
-    __shared__ float sw[NT * CH];
-    for (int i = threadIdx.x; i < NT * CH; i += 256) sw[i] = w6[i];
+    __shared__ float sw[NT * NO * CH];
+    for (int i = threadIdx.x; i < NT * NO * CH; i += 256) sw[i] = w6[i];
     __syncthreads();
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
     int b, z, y, x;
     if (v >= s.nv() || !voxel_coords(v, s, b, z, y, x)) return;
-    float acc = b6[0];
+    float acc[NO] = {0.0f, 0.0f, 0.0f};
     for (int t = 0; t < NT; ++t) {
         const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
         if (zz < 0 || zz >= s.D || yy < 0 || yy >= s.H || xx < 0 || xx >= s.W) continue;
@@ -169,14 +198,20 @@ __global__ __launch_bounds__(256) void k_l6_fwd(Shape s, const bf16 *__restrict_
         for (int g = 0; g < 4; ++g) {
             bf16x8 q = src[g];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) acc += sw[t * CH + g * 8 + e] * (float)q[e];
+            for (int e = 0; e < 8; ++e) {
+                const float xv = (float)q[e];
+#pragma unroll
+                for (int o = 0; o < NO; ++o) acc[o] += sw[(t * NO + o) * CH + g * 8 + e] * xv;
+            }
         }
     }
+    const float a = alpha_factor(al, v);
+    const float delta = acc[0] + a * (acc[1] + a * acc[2]);
     if (INFER) {
         const bool interior = z > 0 && z < s.D - 1 && y > 0 && y < s.H - 1 && x > 0 && x < s.W - 1;
-        out[v] = interior ? u[v] + scale * acc : 0.0f;
+        out[v] = interior ? u[v] + scale * delta : 0.0f;
     } else {
-        out[v] = acc;
+        out[v] = delta;
     }
 }
 
@@ -198,12 +233,12 @@ __global__ __launch_bounds__(256) void k_mse(Shape s, const float *__restrict__ 
     if (threadIdx.x == 0) sse_part[blockIdx.x] = t;
 }
 
-// dX5[v][c] = (sum_t w6[t][c] * dd[v - off(t)]) * (x5[v][c] > 0)
+// dX5[v][c] = sum_t sum_o w6[t][o][c] * dd[q] a[q]^o, q = v - off(t)
 __global__ __launch_bounds__(256) void k_l6_dgrad(Shape s, const float *__restrict__ dd, const float *__restrict__ w6,
-                                                  const bf16 *__restrict__ x5, bf16 *__restrict__ dx5)
+                                                  const float *__restrict__ al, bf16 *__restrict__ dx5)
 {
-    __shared__ float sw[NT * CH];
-    for (int i = threadIdx.x; i < NT * CH; i += 256) sw[i] = w6[i];
+    __shared__ float sw[NT * NO * CH];
+    for (int i = threadIdx.x; i < NT * NO * CH; i += 256) sw[i] = w6[i];
     __syncthreads();
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
     int b, z, y, x;
@@ -214,63 +249,67 @@ __global__ __launch_bounds__(256) void k_l6_dgrad(Shape s, const float *__restri
     for (int t = 0; t < NT; ++t) {
         const int zz = z - (t / 9 - 1), yy = y - ((t / 3) % 3 - 1), xx = x - (t % 3 - 1);
         if (zz < 0 || zz >= s.D || yy < 0 || yy >= s.H || xx < 0 || xx >= s.W) continue;
-        const float d = dd[vidx(b, zz, yy, xx, s)];
+        const int64_t q = vidx(b, zz, yy, xx, s);
+        const float d0 = dd[q], a = alpha_factor(al, q);
+        const float d1 = d0 * a, d2 = d1 * a;
 #pragma unroll
-        for (int c = 0; c < CH; ++c) acc[c] += sw[t * CH + c] * d;
+        for (int c = 0; c < CH; ++c)
+            acc[c] += sw[(t * NO + 0) * CH + c] * d0 + sw[(t * NO + 1) * CH + c] * d1 + sw[(t * NO + 2) * CH + c] * d2;
     }
-    const bf16x8 *m = (const bf16x8 *)(x5 + v * CH);
     bf16x8 *dst = (bf16x8 *)(dx5 + v * CH);
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-        bf16x8 mm = m[g], o;
+        bf16x8 o;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) o[e] = (bf16)((float)mm[e] > 0.0f ? acc[g * 8 + e] : 0.0f);
+        for (int e = 0; e < 8; ++e) o[e] = (bf16)acc[g * 8 + e];
         dst[g] = o;
     }
 }
 
-// dW6[t][c] = sum_v dd[v] x5[v+off(t)][c]; db6 = sum dd; partial[block][NT*CH+1]
+// dW6[t][o][c] = sum_v dd[v] a[v]^o x5[v+off(t)][c]; partial[block][NT*NO*CH]
 __global__ __launch_bounds__(256) void k_l6_wgrad(Shape s, const float *__restrict__ dd,
-                                                  const bf16 *__restrict__ x5, int64_t chunk,
-                                                  float *__restrict__ partial)
+                                                  const float *__restrict__ al, const bf16 *__restrict__ x5,
+                                                  int64_t chunk, float *__restrict__ partial)
 {
     const int c = threadIdx.x & 31, lane8 = threadIdx.x >> 5;
-    float g[NT];
+    float g[NT][NO];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) g[t] = 0.0f;
-    float gb = 0.0f;
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int o = 0; o < NO; ++o) g[t][o] = 0.0f;
     const int64_t v0 = (int64_t)blockIdx.x * chunk, v1 = min(v0 + chunk, s.nv());
     for (int64_t v = v0 + lane8; v < v1; v += 8) {
         int b, z, y, x;
         voxel_coords(v, s, b, z, y, x);
-        const float d = dd[v];
-        if (d == 0.0f) continue;
-        gb += d;
+        const float d0 = dd[v];
+        if (d0 == 0.0f) continue;
+        const float a = alpha_factor(al, v);
+        const float d1 = d0 * a, d2 = d1 * a;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
             if (zz < 0 || zz >= s.D || yy < 0 || yy >= s.H || xx < 0 || xx >= s.W) continue;
-            g[t] += d * (float)x5[vidx(b, zz, yy, xx, s) * CH + c];
+            const float xv = (float)x5[vidx(b, zz, yy, xx, s) * CH + c];
+            g[t][0] += d0 * xv;
+            g[t][1] += d1 * xv;
+            g[t][2] += d2 * xv;
         }
     }
     __shared__ float red[8][CH];
-    float *out = partial + (int64_t)blockIdx.x * (NT * CH + 1);
-    for (int t = 0; t <= NT; ++t) {
-        red[lane8][c] = t < NT ? g[t] : gb;
-        __syncthreads();
-        if (lane8 == 0) {
-            float ssum = 0.0f;
-            for (int q = 0; q < 8; ++q) ssum += red[q][c];
-            if (t < NT) out[t * CH + c] = ssum;
+    float *out = partial + (int64_t)blockIdx.x * (NT * NO * CH);
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+            red[lane8][c] = g[t][o];
+            __syncthreads();
+            if (lane8 == 0) {
+                float ssum = 0.0f;
+                for (int q = 0; q < 8; ++q) ssum += red[q][c];
+                out[(t * NO + o) * CH + c] = ssum;
+            }
+            __syncthreads();
         }
-        if (t == NT && threadIdx.x == 0) {
-            // every channel lane accumulated the same sum of d: take channel 0's
-            float ssum = 0.0f;
-            for (int q = 0; q < 8; ++q) ssum += red[q][0];
-            out[NT * CH] = ssum;
-        }
-        __syncthreads();
-    }
 }
 
 // ------------------------------------------------- 32 -> 32 MFMA conv
@@ -279,10 +318,9 @@ constexpr int BZ = 4, BY = 8, BX = 8;          // output brick (one z-slice per 
 constexpr int HZ = BZ + 2, HY = BY + 2, HX = BX + 2;
 constexpr int HALO = HZ * HY * HX;             // 600 voxels
 
-// MODE 0: y = relu(conv(x, w) + bias); MODE 1 (dgrad): y = conv(x, w) * (mask > 0)
-template <int MODE>
+// y = conv(x, w): the forward layer, or (flipped, transposed weights) the
+// data gradient of one
 __global__ __launch_bounds__(256) void k_conv32(Shape s, const bf16 *__restrict__ x, const bf16 *__restrict__ w,
-                                                const float *__restrict__ bias, const bf16 *__restrict__ mask,
                                                 bf16 *__restrict__ y)
 {
     __shared__ __attribute__((aligned(16))) bf16 sx[HALO * CH];
@@ -295,11 +333,6 @@ __global__ __launch_bounds__(256) void k_conv32(Shape s, const bf16 *__restrict_
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt)
             wb[tp][nt] = *(const bf16x8 *)(w + ((int64_t)tp * CH + nt * 16 + li) * CH + kg * 8);
-    float bsum[2] = {0.0f, 0.0f};
-    if (MODE == 0) {
-        bsum[0] = bias[li];
-        bsum[1] = bias[16 + li];
-    }
     const int nbz = (s.D + BZ - 1) / BZ, nby = (s.H + BY - 1) / BY, nbx = (s.W + BX - 1) / BX;
     const int64_t nbricks = (int64_t)s.n * nbz * nby * nbx;
     for (int64_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
@@ -349,20 +382,14 @@ __global__ __launch_bounds__(256) void k_conv32(Shape s, const bf16 *__restrict_
                 if (zz >= s.D || yy >= s.H || xx >= s.W) continue;
                 const int64_t vo = vidx(bn, zz, yy, xx, s) * CH;
 #pragma unroll
-                for (int nt = 0; nt < 2; ++nt) {
-                    const int c = nt * 16 + li;
-                    float o = acc[m][nt][r];
-                    if (MODE == 0) o = fmaxf(o + bsum[nt], 0.0f);
-                    else o = (float)mask[vo + c] > 0.0f ? o : 0.0f;
-                    y[vo + c] = (bf16)o;
-                }
+                for (int nt = 0; nt < 2; ++nt) y[vo + nt * 16 + li] = (bf16)acc[m][nt][r];
             }
     }
 }
 
 // -------------------------------------------------- 32 -> 32 MFMA wgrad
 
-// dW[t][co][ci] = sum_v dz[v][co] x[v + off(t)][ci], db[co] = sum_v dz[v][co]
+// dW[t][co][ci] = sum_v dz[v][co] x[v + off(t)][ci]
 // brick of 4x8x8 voxels; dz transposed in LDS [co][256]; x halo transposed
 // in LDS as three x-shifted copies [s][ci][6][10][8] so that every operand
 // fragment is one aligned 16-byte read.  Wave w owns taps w, w+4, ...
@@ -382,7 +409,6 @@ __global__ __launch_bounds__(256) void k_wgrad32(Shape s, const bf16 *__restrict
         for (int m = 0; m < 2; ++m)
 #pragma unroll
             for (int n2 = 0; n2 < 2; ++n2) acc[a][m][n2] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
-    float dbias = 0.0f;  // thread t < 32: channel t
     const int nbz = (s.D + BZ - 1) / BZ, nby = (s.H + BY - 1) / BY, nbx = (s.W + BX - 1) / BX;
     const int64_t nbricks = (int64_t)s.n * nbz * nby * nbx;
     for (int64_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
@@ -424,8 +450,6 @@ __global__ __launch_bounds__(256) void k_wgrad32(Shape s, const bf16 *__restrict
             }
         }
         __syncthreads();
-        if (t < CH)
-            for (int v = 0; v < 256; ++v) dbias += (float)sdz[t * 256 + v];
         // K loop: 8 steps of 32 voxels (4 x-rows of 8)
         for (int ks = 0; ks < 8; ++ks) {
             const int row = ks * 4 + kg;          // x-row index 0..31 = z*8 + y
@@ -449,8 +473,8 @@ __global__ __launch_bounds__(256) void k_wgrad32(Shape s, const bf16 *__restrict
             }
         }
     }
-    // partial[block][NT*CH*CH + CH]: D[i = co (4*kg + r) ][j = ci (li)]
-    float *out = partial + (int64_t)blockIdx.x * (NT * CH * CH + CH);
+    // partial[block][NT*CH*CH]: D[i = co (4*kg + r) ][j = ci (li)]
+    float *out = partial + (int64_t)blockIdx.x * (NT * CH * CH);
 #pragma unroll
     for (int a = 0; a < WG_TAPS; ++a) {
         const int tp = wv + 4 * a;
@@ -465,7 +489,6 @@ __global__ __launch_bounds__(256) void k_wgrad32(Shape s, const bf16 *__restrict
                     out[((int64_t)tp * CH + co) * CH + ci] = acc[a][m][n2][r];
                 }
     }
-    if (t < CH) out[NT * CH * CH + t] = dbias;
 }
 
 // out[j] = sum_b partial[b][j] (fixed order)
@@ -572,17 +595,16 @@ void repack(piro_surrogate *s)
 }
 
 // forward through L5; L6 by the caller
-void forward_body(piro_surrogate *s, const Shape &sh, const float *u, const float *al, float inv_scale)
+void forward_body(piro_surrogate *s, const Shape &sh, const float *u, float inv_scale)
 {
     hipStream_t st = s->ctx->stream;
     const int64_t nv = sh.nv();
     bf16 *x[5];
     for (int l = 0; l < 5; ++l) x[l] = s->act + (int64_t)l * nv * CH;
-    k_l1_fwd<<<grid_for(nv, 256), 256, 0, st>>>(sh, u, al, inv_scale, s->params + W1_OFF, s->params + B1_OFF, x[0]);
+    k_l1_fwd<<<grid_for(nv, 256), 256, 0, st>>>(sh, u, inv_scale, s->params + W1_OFF, x[0]);
     check_launch();
     for (int l = 0; l < 4; ++l) {
-        k_conv32<0><<<conv_grid(s), 256, 0, st>>>(sh, x[l], s->wf + (int64_t)l * NT * CH * CH,
-                                                   s->params + WM_OFF + l * WM_SZ + NT * CH * CH, nullptr, x[l + 1]);
+        k_conv32<<<conv_grid(s), 256, 0, st>>>(sh, x[l], s->wf + (int64_t)l * NT * CH * CH, x[l + 1]);
         check_launch();
     }
 }
@@ -603,11 +625,10 @@ double forward_backward(piro_surrogate *s, const Shape &sh, const float *u, cons
     const int64_t nv = sh.nv();
     ensure_acts(s, nv);
     const float inv_scale = 1.0f / scale;
-    forward_body(s, sh, u, al, inv_scale);
+    forward_body(s, sh, u, inv_scale);
     bf16 *x[5];
     for (int l = 0; l < 5; ++l) x[l] = s->act + (int64_t)l * nv * CH;
-    k_l6_fwd<false><<<grid_for(nv, 256), 256, 0, st>>>(sh, x[4], s->params + W6_OFF, s->params + B6_OFF, nullptr,
-                                                       1.0f, s->delta);
+    k_l6_fwd<false><<<grid_for(nv, 256), 256, 0, st>>>(sh, x[4], s->params + W6_OFF, al, nullptr, 1.0f, s->delta);
     check_launch();
     const int64_t n_int = (int64_t)sh.n * std::max(sh.D - 2, 0) * std::max(sh.H - 2, 0) * std::max(sh.W - 2, 0);
     PIRO_REQUIRE(n_int > 0, "instances need interior cells");
@@ -620,32 +641,33 @@ double forward_backward(piro_surrogate *s, const Shape &sh, const float *u, cons
     const int64_t chunk = std::max<int64_t>(4096, (nv + 1023) / 1024);
     const int nchunks = (int)((nv + chunk - 1) / chunk);
     grow(s->partial, s->cap_partial,
-         std::max<int64_t>((int64_t)gparts * (NT * CH * CH + CH), (int64_t)nchunks * (NT * CH * 2 + CH)));
+         std::max<int64_t>((int64_t)gparts * (NT * CH * CH), (int64_t)nchunks * (NT * NO * CH)));
     // L6
-    k_l6_wgrad<<<nchunks, 256, 0, st>>>(sh, s->dd, x[4], chunk, s->partial);
+    k_l6_wgrad<<<nchunks, 256, 0, st>>>(sh, s->dd, al, x[4], chunk, s->partial);
     check_launch();
-    k_sum_partials<<<grid_for(NT * CH + 1, 256), 256, 0, st>>>(s->partial, nchunks, NT * CH + 1, s->grads + W6_OFF);
+    k_sum_partials<<<grid_for(NT * NO * CH, 256), 256, 0, st>>>(s->partial, nchunks, NT * NO * CH, s->grads + W6_OFF);
     check_launch();
     bf16 *g0 = s->gbuf, *g1 = s->gbuf + nv * CH;
-    k_l6_dgrad<<<grid_for(nv, 256), 256, 0, st>>>(sh, s->dd, s->params + W6_OFF, x[4], g0);
+    k_l6_dgrad<<<grid_for(nv, 256), 256, 0, st>>>(sh, s->dd, s->params + W6_OFF, al, g0);
     check_launch();
     // L5..L2 (layer l maps x[l-1] -> x[l], l = 4..1 in x indices)
     for (int l = 3; l >= 0; --l) {
-        // g0 = dL/d(pre-activation of x[l+1]) (masked)
+        // g0 = dL/d(output of layer l + 2)
         k_wgrad32<<<gparts, 256, 0, st>>>(sh, x[l], g0, s->partial);
         check_launch();
-        k_sum_partials<<<grid_for(NT * CH * CH + CH, 256), 256, 0, st>>>(s->partial, gparts, NT * CH * CH + CH,
-                                                                          s->grads + WM_OFF + l * WM_SZ);
+        k_sum_partials<<<grid_for(NT * CH * CH, 256), 256, 0, st>>>(s->partial, gparts, NT * CH * CH,
+                                                                     s->grads + WM_OFF + l * WM_SZ);
         check_launch();
-        k_conv32<1><<<conv_grid(s), 256, 0, st>>>(sh, g0, s->wd + (int64_t)l * NT * CH * CH, nullptr, x[l], g1);
+        k_conv32<<<conv_grid(s), 256, 0, st>>>(sh, g0, s->wd + (int64_t)l * NT * CH * CH, g1);
         check_launch();
         std::swap(g0, g1);
     }
     // L1
-    k_l1_wgrad<<<nchunks, 256, 0, st>>>(sh, u, al, inv_scale, g0, chunk, s->partial);
+    k_l1_wgrad<<<nchunks, 256, 0, st>>>(sh, u, inv_scale, g0, chunk, s->partial);
     check_launch();
-    k_sum_partials<<<grid_for(NT * CH * 2 + CH, 256), 256, 0, st>>>(s->partial, nchunks, NT * CH * 2 + CH,
-                                                                     s->grads + W1_OFF);
+    k_sum_partials<<<grid_for(NT * CH, 256), 256, 0, st>>>(s->partial, nchunks, NT * CH, s->grads + W1_OFF);
+    check_launch();
+    k_l1_project<<<1, CH, 0, st>>>(s->grads + W1_OFF);
     check_launch();
     std::vector<double> h(nb);
     PIRO_CHECK_HIP(hipMemcpyAsync(h.data(), s->sse, nb * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -675,16 +697,17 @@ int piro_surrogate_create(piro_ctx *ctx, uint64_t seed, piro_surrogate **out)
         PIRO_CHECK_HIP(hipMalloc(&s->adam_v, N_PARAMS * sizeof(float)));
         PIRO_CHECK_HIP(hipMalloc(&s->wf, 4 * NT * CH * CH * sizeof(bf16)));
         PIRO_CHECK_HIP(hipMalloc(&s->wd, 4 * NT * CH * CH * sizeof(bf16)));
-        // He-style uniform init; the output layer starts at zero so the
-        // untrained surrogate is the identity map
+        // variance-preserving uniform init for the linear layers; the output
+        // layer starts at zero so the untrained surrogate is the identity map
         std::vector<float> p(N_PARAMS, 0.0f);
         std::mt19937_64 rng(seed);
         auto fill = [&](int off, int count, int fan_in) {
-            float lim = std::sqrt(6.0f / (float)fan_in);
+            float lim = std::sqrt(3.0f / (float)fan_in);
             std::uniform_real_distribution<float> U(-lim, lim);
             for (int i = 0; i < count; ++i) p[off + i] = U(rng);
         };
-        fill(W1_OFF, NT * CH * 2, NT * 2);
+        fill(W1_OFF, NT * CH, NT - 1);
+        for (int c = 0; c < CH; ++c) p[W1_OFF + TC * CH + c] = 0.0f;  // implied centre tap
         for (int l = 0; l < 4; ++l) fill(WM_OFF + l * WM_SZ, NT * CH * CH, NT * CH);
         PIRO_CHECK_HIP(hipMemcpy(s->params, p.data(), N_PARAMS * sizeof(float), hipMemcpyHostToDevice));
         PIRO_CHECK_HIP(hipMemset(s->adam_m, 0, N_PARAMS * sizeof(float)));
@@ -746,9 +769,9 @@ int piro_surrogate_forward(piro_surrogate *s, int n, int D, int H, int W, const 
     PIRO_REQUIRE(scale > 0.0f && std::isfinite(scale), "scale must be positive and finite");
     Shape sh = mkshape(n, D, H, W);
     ensure_acts(s, sh.nv());
-    forward_body(s, sh, d_u, d_alpha, 1.0f / scale);
+    forward_body(s, sh, d_u, 1.0f / scale);
     k_l6_fwd<true><<<grid_for(sh.nv(), 256), 256, 0, s->ctx->stream>>>(
-        sh, s->act + 4 * sh.nv() * CH, s->params + W6_OFF, s->params + B6_OFF, d_u, scale, d_u_out);
+        sh, s->act + 4 * sh.nv() * CH, s->params + W6_OFF, d_alpha, d_u, scale, d_u_out);
     check_launch();
     PIRO_API_END
 }

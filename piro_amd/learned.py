@@ -79,9 +79,10 @@ class TrainLog:
 
 
 def train(ctx: Context, sur: Surrogate, n_traj=2000, per_batch=50, side=24, segments=4, k=16, passes=2,
-          lr=2e-3, lr_final=2e-4, seed=1234, log_every=0) -> TrainLog:
+          lr=2e-3, lr_final=2e-4, seed=1234, log_every=0, blob_widths=(3.0, 14.0)) -> TrainLog:
     """Generate n_traj trajectories with the batched stepper and train on
-    every (u_t, u_{t+k}) pair as it is produced."""
+    every (u_t, u_{t+k}) pair as it is produced; blob_widths (in cells) are
+    the initial blobs' Gaussian widths."""
     rng = np.random.default_rng(seed)
     dims = (side, side, side)
     log = TrainLog()
@@ -93,7 +94,7 @@ def train(ctx: Context, sur: Surrogate, n_traj=2000, per_batch=50, side=24, segm
     shape = (per_batch, side, side, side)
     for _ in range(n_batches):
         alpha = np.concatenate([smooth_alpha(dims, rng) for _ in range(per_batch)])
-        u0 = np.concatenate([blob_ic(dims, rng) for _ in range(per_batch)])
+        u0 = np.concatenate([blob_ic(dims, rng, width_cells=blob_widths) for _ in range(per_batch)])
         da = ctx.from_host(alpha)
         b = Batch(ctx, [dims] * per_batch, da, [dt] * per_batch)
         x = ctx.from_host(u0)

@@ -9,7 +9,7 @@ import oracle as po
 
 pytestmark = pytest.mark.gpu
 
-NT, CH = 27, 32
+NT, CH, TC, NO = 27, 32, 13, 3
 
 
 def bf16(a):
@@ -22,35 +22,47 @@ def bf16(a):
 
 def unpack(p):
     o = 0
-    w1 = p[o:o + NT * CH * 2].reshape(NT, CH, 2); o += NT * CH * 2
-    b1 = p[o:o + CH]; o += CH
+    w1 = p[o:o + NT * CH].reshape(NT, CH); o += NT * CH
     mids = []
     for _ in range(4):
-        w = p[o:o + NT * CH * CH].reshape(NT, CH, CH); o += NT * CH * CH
-        b = p[o:o + CH]; o += CH
-        mids.append((w, b))
-    w6 = p[o:o + NT * CH].reshape(NT, 1, CH); o += NT * CH
-    b6 = p[o:o + 1]; o += 1
+        mids.append(p[o:o + NT * CH * CH].reshape(NT, CH, CH)); o += NT * CH * CH
+    w6 = p[o:o + NT * NO * CH].reshape(NT, NO, CH); o += NT * NO * CH
     assert o == p.size
-    return w1, b1, mids, w6, b6
+    return w1, mids, w6
+
+
+def effective_w1(w1):
+    """L1 weights as the device uses them: the centre tap is minus the FP32
+    sum, in tap order, of the other 26"""
+    w = np.array(w1, np.float32)
+    w[TC] = 0.0
+    c = np.zeros(CH, np.float32)
+    for t in range(NT):
+        c = (c + w[t]).astype(np.float32)
+    w[TC] = -c
+    return w.astype(np.float64)
+
+
+def alpha_factor(al):
+    return ((al.astype(np.float32) - np.float32(0.0125)) * (np.float32(1.0) / np.float32(0.0075))).astype(np.float64)
 
 
 def ref_forward(p, u, al, scale):
     """FP64 network with the device's BF16 rounding points; returns inputs,
     activations and delta (shape [n,D,H,W])"""
-    w1, b1, mids, w6, b6 = unpack(p.astype(np.float64))
+    w1, mids, w6 = unpack(p)
     in0 = (u.astype(np.float32) * np.float32(1.0 / scale)).astype(np.float64)
-    in1 = ((al.astype(np.float32) - np.float32(0.0125)) * np.float32(1.0 / 0.0075)).astype(np.float64)
-    x_in = np.stack([in0, in1], -1)
+    x_in = in0[..., None]
     acts = []
-    z = po.conv3d_fwd_f64(x_in, w1, b1)
-    x = bf16(np.maximum(z, 0).astype(np.float32)).astype(np.float64)
+    x = bf16(po.conv3d_fwd_f64(x_in, effective_w1(w1)[:, :, None]).astype(np.float32)).astype(np.float64)
     acts.append(x)
-    for w, b in mids:
-        z = po.conv3d_fwd_f64(x, bf16(w.astype(np.float32)).astype(np.float64), b)
-        x = bf16(np.maximum(z, 0).astype(np.float32)).astype(np.float64)
+    for w in mids:
+        z = po.conv3d_fwd_f64(x, bf16(w.astype(np.float32)).astype(np.float64))
+        x = bf16(z.astype(np.float32)).astype(np.float64)
         acts.append(x)
-    delta = po.conv3d_fwd_f64(x, w6, b6)[..., 0]
+    o = po.conv3d_fwd_f64(x, w6.astype(np.float64))
+    a = alpha_factor(al)
+    delta = o[..., 0] + a * (o[..., 1] + a * o[..., 2])
     return x_in, acts, delta
 
 
@@ -62,54 +74,55 @@ def interior_mask(shape):
 
 
 def ref_grads(p, u, al, target, scale):
-    w1, b1, mids, w6, b6 = unpack(p.astype(np.float64))
+    w1, mids, w6 = unpack(p)
     x_in, acts, delta = ref_forward(p, u, al, scale)
     m = interior_mask(u.shape)
     n_int = m.sum()
     dd = np.where(m, 2.0 * (delta - target) / n_int, 0.0)
     loss = float(np.sum(np.where(m, (delta - target) ** 2, 0.0)) / n_int)
+    a = alpha_factor(al)
     g = {}
-    # the device stores each ReLU-masked upstream gradient in BF16 before the
-    # next layer's data and weight gradients use it
-    dx, g["w6"], g["b6"] = po.conv3d_bwd_f64(acts[4], w6, dd[..., None])
-    dz = bf16((dx * (acts[4] > 0)).astype(np.float32)).astype(np.float64)
+    # the device stores each upstream data gradient in BF16 before the next
+    # layer's data and weight gradients use it
+    dx, g["w6"], _ = po.conv3d_bwd_f64(acts[4], w6.astype(np.float64), np.stack([dd, dd * a, dd * a * a], -1))
+    dz = bf16(dx.astype(np.float32)).astype(np.float64)
     for l in range(3, -1, -1):
-        w, b = mids[l]
-        wb = bf16(w.astype(np.float32)).astype(np.float64)
-        dx, g[f"w{l + 2}"], g[f"b{l + 2}"] = po.conv3d_bwd_f64(acts[l], wb, dz)
-        dz = bf16((dx * (acts[l] > 0)).astype(np.float32)).astype(np.float64)
-    _, g["w1"], g["b1"] = po.conv3d_bwd_f64(x_in, w1, dz, need_dx=False)
-    flat = np.concatenate([g["w1"].ravel(), g["b1"]] +
-                          sum([[g[f"w{l}"].ravel(), g[f"b{l}"]] for l in range(2, 6)], []) +
-                          [g["w6"].ravel(), g["b6"]])
+        wb = bf16(mids[l].astype(np.float32)).astype(np.float64)
+        dx, g[f"w{l + 2}"], _ = po.conv3d_bwd_f64(acts[l], wb, dz)
+        dz = bf16(dx.astype(np.float32)).astype(np.float64)
+    _, gw1, _ = po.conv3d_bwd_f64(x_in, effective_w1(w1)[:, :, None], dz, need_dx=False)
+    gw1 = gw1[:, :, 0]
+    g["w1"] = gw1 - gw1[TC]
+    g["w1"][TC] = 0.0
+    flat = np.concatenate([g["w1"].ravel()] + [g[f"w{l}"].ravel() for l in range(2, 6)] + [g["w6"].ravel()])
     return flat, loss, delta
 
 
 def param_blocks():
     """name -> [start, stop) in the flat parameter vector"""
-    offs = [0, NT * CH * 2, NT * CH * 2 + CH]
+    offs = [0, NT * CH]
     for _ in range(4):
-        offs += [offs[-1] + NT * CH * CH, offs[-1] + NT * CH * CH + CH]
-    offs += [offs[-1] + NT * CH, offs[-1] + NT * CH + 1]
-    names = ["w1", "b1"] + sum([[f"w{l}", f"b{l}"] for l in range(2, 6)], []) + ["w6", "b6"]
+        offs.append(offs[-1] + NT * CH * CH)
+    offs.append(offs[-1] + NT * NO * CH)
+    names = ["w1"] + [f"w{l}" for l in range(2, 6)] + ["w6"]
     return {nm: (offs[i], offs[i + 1]) for i, nm in enumerate(names)}
 
 
 def dyadic_network(rng):
     """Sparse parameters for which every value of the forward and backward
     passes is exactly representable: two +-1 weights per output channel
-    (mid layers: from two distinct input channels, so each input channel
-    feeds at most two outputs; mostly +1 so that activations stay alive),
-    biases 0 or 1, eight output weights +-1/8."""
+    (L1: at two non-centre taps, so the implied centre is 0 or +-2; mid
+    layers: from two distinct input channels, so each input channel feeds at
+    most two outputs), eight output weights +-1/8 on the first output."""
     blk = param_blocks()
     p = np.zeros(max(b for _, b in blk.values()), np.float32)
-    sign = lambda: rng.choice([1.0, 1.0, 1.0, -1.0])
-    w1 = np.zeros((NT, CH, 2), np.float32)
+    sign = lambda: rng.choice([1.0, -1.0])
+    w1 = np.zeros((NT, CH), np.float32)
+    taps = [t for t in range(NT) if t != TC]
     for co in range(CH):
-        for tp in rng.choice(NT, 2, replace=False):
-            w1[tp, co, 0] = sign()
+        for tp in rng.choice(taps, 2, replace=False):
+            w1[tp, co] = sign()
     p[slice(*blk["w1"])] = w1.ravel()
-    p[slice(*blk["b1"])] = rng.choice([0.0, 1.0], CH)
     for l in range(2, 6):
         w = np.zeros((NT, CH, CH), np.float32)
         p1, p2 = rng.permutation(CH), rng.permutation(CH)
@@ -117,10 +130,9 @@ def dyadic_network(rng):
             for ci in {p1[co], p2[co]}:
                 w[rng.integers(NT), co, ci] = sign()
         p[slice(*blk[f"w{l}"])] = w.ravel()
-        p[slice(*blk[f"b{l}"])] = rng.choice([0.0, 1.0], CH)
-    w6 = np.zeros((NT, CH), np.float32)
+    w6 = np.zeros((NT, NO, CH), np.float32)
     for ci in rng.choice(CH, 8, replace=False):
-        w6[rng.integers(NT), ci] = rng.choice([-0.125, 0.125])
+        w6[rng.integers(NT), 0, ci] = rng.choice([-0.125, 0.125])
     p[slice(*blk["w6"])] = w6.ravel()
     return p
 
@@ -137,9 +149,9 @@ def random_params(sur, seed):
     rng = np.random.default_rng(seed)
     p = sur.get_params()
     # non-zero output layer so every gradient path is exercised
-    o = p.size - (NT * CH + 1)
-    p[o:] = rng.uniform(-0.05, 0.05, NT * CH + 1).astype(np.float32)
-    p[: NT * CH * 2 + CH] += rng.uniform(-0.05, 0.05, NT * CH * 2 + CH).astype(np.float32)
+    o = p.size - NT * NO * CH
+    p[o:] = rng.uniform(-0.05, 0.05, NT * NO * CH).astype(np.float32)
+    p[: NT * CH] += rng.uniform(-0.05, 0.05, NT * CH).astype(np.float32)
     return p
 
 
@@ -221,7 +233,7 @@ def test_gradients_match_oracle(ctx, shape):
 
 
 def test_gradients_exact_on_dyadic_network(ctx):
-    # integer inputs, alpha = 0.0125 (normalised input 0), scale 1, 4^3
+    # integer inputs, alpha = 0.0125 (diffusivity factor 0), scale 1, 4^3
     # interior cells (the loss scale 2/64 is a power of two) and targets
     # within +-1/8 of the output: every activation is a small integer, every
     # upstream gradient a small multiple of 2^-11 -- all exact in BF16 -- and
@@ -235,7 +247,7 @@ def test_gradients_exact_on_dyadic_network(ctx):
         u = rng.integers(0, 3, shape).astype(np.float32)
         al = np.full(shape, 0.0125, np.float32)
         _, acts, delta = ref_forward(p, u, al, 1.0)
-        assert max(float(a.max()) for a in acts) < 256          # integers exact in BF16
+        assert max(float(np.abs(a).max()) for a in acts) < 256  # integers exact in BF16
         target = (delta + rng.choice([-0.125, 0.0, 0.125], shape)).astype(np.float32)
         assert np.array_equal(target.astype(np.float64) - delta, np.round((target - delta) * 8) / 8)
         sur = Surrogate(ctx, seed=1)
@@ -252,8 +264,7 @@ def test_gradients_exact_on_dyadic_network(ctx):
             a, b = g[o0:o1].astype(np.float64), gr[o0:o1]
             assert np.array_equal(a, b), (seed, nm, np.abs(a - b).max(), np.abs(b).max())
         for nm, (o0, o1) in param_blocks().items():
-            if nm != "b6":
-                assert np.abs(gr[o0:o1]).max() > 0, (seed, nm)   # every block is exercised
+            assert np.abs(gr[o0:o1]).max() > 0, (seed, nm)   # every block is exercised
 
 
 def test_training_reduces_loss(ctx):
