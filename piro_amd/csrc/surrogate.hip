@@ -8,7 +8,8 @@
 //                                26, so a constant field maps to zero
 //   L2-5: conv 32->32           (MFMA v_mfma_f32_16x16x32_bf16: BF16 weights
 //                                and activations, FP32 accumulation)
-//   L6  : conv 32->3            (VALU, FP32 accumulation), combined per cell
+//   L6  : conv 32->3            (MFMA, weights split into two BF16 parts,
+//                                FP32 accumulation), combined per cell
 //                                with the normalised diffusivity
 //                                a = (alpha-0.0125)/0.0075 of that cell:
 //                                delta = o0 + a (o1 + a o2)
@@ -176,45 +177,6 @@ __device__ __forceinline__ float alpha_factor(const float *__restrict__ al, int6
     return (al[v] - ALPHA_MU) * ALPHA_INV;
 }
 
-// delta[v] = o0 + a (o1 + a o2), o = conv(x5, w6); training: delta;
-// inference: out[v] = interior ? u + s*delta : 0
-template <bool INFER>
-__global__ __launch_bounds__(256) void k_l6_fwd(Shape s, const bf16 *__restrict__ x5, const float *__restrict__ w6,
-                                                const float *__restrict__ al, const float *__restrict__ u,
-                                                float scale, float *__restrict__ out)
-{
-    __shared__ float sw[NT * NO * CH];
-    for (int i = threadIdx.x; i < NT * NO * CH; i += 256) sw[i] = w6[i];
-    __syncthreads();
-    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    int b, z, y, x;
-    if (v >= s.nv() || !voxel_coords(v, s, b, z, y, x)) return;
-    float acc[NO] = {0.0f, 0.0f, 0.0f};
-    for (int t = 0; t < NT; ++t) {
-        const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
-        if (zz < 0 || zz >= s.D || yy < 0 || yy >= s.H || xx < 0 || xx >= s.W) continue;
-        const bf16x8 *src = (const bf16x8 *)(x5 + vidx(b, zz, yy, xx, s) * CH);
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            bf16x8 q = src[g];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float xv = (float)q[e];
-#pragma unroll
-                for (int o = 0; o < NO; ++o) acc[o] += sw[(t * NO + o) * CH + g * 8 + e] * xv;
-            }
-        }
-    }
-    const float a = alpha_factor(al, v);
-    const float delta = acc[0] + a * (acc[1] + a * acc[2]);
-    if (INFER) {
-        const bool interior = z > 0 && z < s.D - 1 && y > 0 && y < s.H - 1 && x > 0 && x < s.W - 1;
-        out[v] = interior ? u[v] + scale * delta : 0.0f;
-    } else {
-        out[v] = delta;
-    }
-}
-
 // loss on interior cells: d[v] = 2 (delta - target)/n_int; per-block SSE
 __global__ __launch_bounds__(256) void k_mse(Shape s, const float *__restrict__ delta,
                                              const float *__restrict__ target, float inv_n, float *__restrict__ dd,
@@ -314,16 +276,63 @@ __global__ __launch_bounds__(256) void k_l6_wgrad(Shape s, const float *__restri
 
 // ------------------------------------------------- 32 -> 32 MFMA conv
 
-constexpr int BZ = 4, BY = 8, BX = 8;          // output brick (one z-slice per wave)
+constexpr int BZ = 4, BY = 8, BX = 8;          // output brick
 constexpr int HZ = BZ + 2, HY = BY + 2, HX = BX + 2;
 constexpr int HALO = HZ * HY * HX;             // 600 voxels
+constexpr int HCHUNK = (HALO * 4 + 255) / 256; // 16-byte halo pieces per thread
+
+struct Brick {
+    int n, z0, y0, x0;
+};
+
+__device__ __forceinline__ Brick brick_of(int64_t br, const Shape &s)
+{
+    const int nbz = (s.D + BZ - 1) / BZ, nby = (s.H + BY - 1) / BY, nbx = (s.W + BX - 1) / BX;
+    int64_t q = br;
+    const int bx = (int)(q % nbx); q /= nbx;
+    const int by = (int)(q % nby); q /= nby;
+    const int bz = (int)(q % nbz);
+    return Brick{(int)(q / nbz), bz * BZ, by * BY, bx * BX};
+}
+
+// this thread's pieces of a brick's halo (zero outside the instance)
+__device__ __forceinline__ void fetch_halo(bf16x8 (&h)[HCHUNK], const bf16 *__restrict__ x, const Shape &s,
+                                           const Brick &b, int t)
+{
+#pragma unroll
+    for (int i = 0; i < HCHUNK; ++i) {
+        const int e = t + 256 * i;
+        const int hv = e >> 2, g = e & 3;
+        const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+        const int zz = b.z0 + hz - 1, yy = b.y0 + hy - 1, xx = b.x0 + hx - 1;
+        bf16x8 val = {};
+        if (e < HALO * 4 && zz >= 0 && zz < s.D && yy >= 0 && yy < s.H && xx >= 0 && xx < s.W)
+            val = *(const bf16x8 *)(x + vidx(b.n, zz, yy, xx, s) * CH + g * 8);
+        h[i] = val;
+    }
+}
+
+__device__ __forceinline__ void store_halo(bf16 *__restrict__ sx, const bf16x8 (&h)[HCHUNK], int t)
+{
+#pragma unroll
+    for (int i = 0; i < HCHUNK; ++i) {
+        const int e = t + 256 * i;
+        if (e < HALO * 4) *(bf16x8 *)(sx + (e >> 2) * CH + (e & 3) * 8) = h[i];
+    }
+}
 
 // y = conv(x, w): the forward layer, or (flipped, transposed weights) the
-// data gradient of one
+// data gradient of one.  Persistent workgroups walk 4x8x8 output bricks.
+// Wave w owns the 16 voxels (y = 2w + i/8, x = i%8) of the brick through
+// all four z-slices: for each in-plane tap (ty, tx) it reads the six halo
+// slices' A fragments once and uses each for every (z, tz) pair it feeds
+// (24 MFMAs per six 16-byte LDS reads per lane).  The next brick's halo is
+// loaded into registers during the current brick's products and stored to
+// the other LDS buffer afterwards.
 __global__ __launch_bounds__(256) void k_conv32(Shape s, const bf16 *__restrict__ x, const bf16 *__restrict__ w,
                                                 bf16 *__restrict__ y)
 {
-    __shared__ __attribute__((aligned(16))) bf16 sx[HALO * CH];
+    __shared__ __attribute__((aligned(16))) bf16 sx[2][HALO * CH];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int li = lane & 15, kg = lane >> 4;
     // B fragments of all taps, both output-channel tiles, in registers
@@ -335,55 +344,171 @@ __global__ __launch_bounds__(256) void k_conv32(Shape s, const bf16 *__restrict_
             wb[tp][nt] = *(const bf16x8 *)(w + ((int64_t)tp * CH + nt * 16 + li) * CH + kg * 8);
     const int nbz = (s.D + BZ - 1) / BZ, nby = (s.H + BY - 1) / BY, nbx = (s.W + BX - 1) / BX;
     const int64_t nbricks = (int64_t)s.n * nbz * nby * nbx;
-    for (int64_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
-        int64_t q = br;
-        const int bx = (int)(q % nbx); q /= nbx;
-        const int by = (int)(q % nby); q /= nby;
-        const int bz = (int)(q % nbz);
-        const int bn = (int)(q / nbz);
-        const int z0 = bz * BZ, y0 = by * BY, x0 = bx * BX;
-        __syncthreads();
-        // halo brick -> LDS, 16 bytes per (voxel, channel group)
-        for (int e = t; e < HALO * 4; e += 256) {
-            const int hv = e >> 2, g = e & 3;
-            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
-            const int zz = z0 + hz - 1, yy = y0 + hy - 1, xx = x0 + hx - 1;
-            bf16x8 val = {};
-            if (zz >= 0 && zz < s.D && yy >= 0 && yy < s.H && xx >= 0 && xx < s.W)
-                val = *(const bf16x8 *)(x + vidx(bn, zz, yy, xx, s) * CH + g * 8);
-            *(bf16x8 *)(sx + hv * CH + g * 8) = val;
-        }
-        __syncthreads();
-        f32x4 acc[4][2];
+    int64_t br = blockIdx.x;
+    if (br >= nbricks) return;
+    bf16x8 h[HCHUNK];
+    fetch_halo(h, x, s, brick_of(br, s), t);
+    store_halo(sx[0], h, t);
+    __syncthreads();
+    // this lane's A row: voxel (y = 2 wv + li/8, x = li%8) of each slice
+    const int vy = 2 * wv + (li >> 3), vx = li & 7;
+    for (int buf = 0; br < nbricks; br += gridDim.x, buf ^= 1) {
+        const Brick b = brick_of(br, s);
+        const int64_t nxt = br + gridDim.x;
+        if (nxt < nbricks) fetch_halo(h, x, s, brick_of(nxt, s), t);
+        // this lane's A row in the halo; every fragment is a constant offset from it
+        const bf16 *hb = sx[buf] + (vy * HX + vx) * CH + kg * 8;
+        f32x4 acc[BZ][2];
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+        for (int z = 0; z < BZ; ++z)
 #pragma unroll
-            for (int nt = 0; nt < 2; ++nt) acc[m][nt] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+            for (int nt = 0; nt < 2; ++nt) acc[z][nt] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+        // A fragments of in-plane tap group g = 3 ty + tx for the six halo
+        // slices; the next group's are read while this group's 24 MFMAs run
+        bf16x8 a[HZ], an[HZ];
 #pragma unroll
-        for (int tp = 0; tp < NT; ++tp) {
-            const int tz = tp / 9, ty = (tp / 3) % 3, tx = tp % 3;
+        for (int hz = 0; hz < HZ; ++hz) a[hz] = *(const bf16x8 *)(hb + hz * HY * HX * CH);
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                // A row i = li: voxel (y = 2m + li/8, x = li%8) of slice wv
-                const int hv = ((wv + tz) * HY + (2 * m + (li >> 3) + ty)) * HX + ((li & 7) + tx);
-                const bf16x8 a = *(const bf16x8 *)(sx + hv * CH + kg * 8);
-                acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb[tp][0], acc[m][0], 0, 0, 0);
-                acc[m][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb[tp][1], acc[m][1], 0, 0, 0);
+        for (int g = 0; g < 9; ++g) {
+            const int ty = g / 3, tx = g % 3;
+            if (g + 1 < 9) {
+                const int ny = (g + 1) / 3, nx = (g + 1) % 3;
+#pragma unroll
+                for (int hz = 0; hz < HZ; ++hz)
+                    an[hz] = *(const bf16x8 *)(hb + ((hz * HY + ny) * HX + nx) * CH);
+            }
+#pragma unroll
+            for (int tz = 0; tz < 3; ++tz)
+#pragma unroll
+                for (int z = 0; z < BZ; ++z)
+#pragma unroll
+                    for (int nt = 0; nt < 2; ++nt)
+                        acc[z][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[z + tz], wb[tz * 9 + ty * 3 + tx][nt],
+                                                                            acc[z][nt], 0, 0, 0);
+            if (g + 1 < 9) {
+                // interleave: 4 MFMAs, 1 LDS read, six times
+#pragma unroll
+                for (int q = 0; q < HZ; ++q) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                }
+#pragma unroll
+                for (int hz = 0; hz < HZ; ++hz) a[hz] = an[hz];
             }
         }
-        // epilogue: D[i = 4*kg + r][j = li] of tile (m, nt)
-        const int zz = z0 + wv;
+        // epilogue: D[i = 4*kg + r][j = li] of tile (z, nt), voxel i = (y = 2 wv + i/8, x = i%8)
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+        for (int z = 0; z < BZ; ++z)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = 4 * kg + r;
-                const int yy = y0 + 2 * m + (i >> 3), xx = x0 + (i & 7);
+                const int zz = b.z0 + z, yy = b.y0 + 2 * wv + (i >> 3), xx = b.x0 + (i & 7);
                 if (zz >= s.D || yy >= s.H || xx >= s.W) continue;
-                const int64_t vo = vidx(bn, zz, yy, xx, s) * CH;
+                const int64_t vo = vidx(b.n, zz, yy, xx, s) * CH;
 #pragma unroll
-                for (int nt = 0; nt < 2; ++nt) y[vo + nt * 16 + li] = (bf16)acc[m][nt][r];
+                for (int nt = 0; nt < 2; ++nt) y[vo + nt * 16 + li] = (bf16)acc[z][nt][r];
             }
+        if (nxt < nbricks) {
+            // the other buffer was read during the previous brick, which
+            // every wave finished before the barrier that ended it
+            store_halo(sx[buf ^ 1], h, t);
+            __syncthreads();
+        }
+    }
+}
+
+// L6 on MFMA: o = conv(x5, W6) with the brick machinery of k_conv32 and one
+// 16-column output tile.  Columns 0-2 carry BF16(w6) and columns 3-5 the
+// BF16 rounding of the remainder w6 - BF16(w6), so o_j = D[j] + D[j + 3]
+// sees the weights to about 16 significant bits.  The epilogue combines
+// the three outputs of a voxel (lanes li = 0..5 of one row group) through
+// lane shuffles: delta = o0 + a (o1 + a o2); training writes delta,
+// inference out = interior ? u + s*delta : 0.
+__device__ __forceinline__ bf16x8 l6_fragment(const float *__restrict__ w6, int tp, int li, int kg)
+{
+    bf16x8 f = {};
+    if (li < 2 * NO) {
+        const int o = li % NO;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float wv = w6[(tp * NO + o) * CH + kg * 8 + e];
+            const bf16 hi = (bf16)wv;
+            f[e] = li < NO ? hi : (bf16)(wv - (float)hi);
+        }
+    }
+    return f;
+}
+
+template <bool INFER>
+__global__ __launch_bounds__(256) void k_l6(Shape s, const bf16 *__restrict__ x5, const float *__restrict__ w6,
+                                            const float *__restrict__ al, const float *__restrict__ u, float scale,
+                                            float *__restrict__ out)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sx[2][HALO * CH];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int li = lane & 15, kg = lane >> 4;
+    bf16x8 wb[NT];
+#pragma unroll
+    for (int tp = 0; tp < NT; ++tp) wb[tp] = l6_fragment(w6, tp, li, kg);
+    const int nbz = (s.D + BZ - 1) / BZ, nby = (s.H + BY - 1) / BY, nbx = (s.W + BX - 1) / BX;
+    const int64_t nbricks = (int64_t)s.n * nbz * nby * nbx;
+    int64_t br = blockIdx.x;
+    if (br >= nbricks) return;
+    bf16x8 h[HCHUNK];
+    fetch_halo(h, x5, s, brick_of(br, s), t);
+    store_halo(sx[0], h, t);
+    __syncthreads();
+    const int vy = 2 * wv + (li >> 3), vx = li & 7;
+    for (int buf = 0; br < nbricks; br += gridDim.x, buf ^= 1) {
+        const Brick b = brick_of(br, s);
+        const int64_t nxt = br + gridDim.x;
+        if (nxt < nbricks) fetch_halo(h, x5, s, brick_of(nxt, s), t);
+        const bf16 *hb = sx[buf] + (vy * HX + vx) * CH + kg * 8;
+        f32x4 acc[BZ];
+#pragma unroll
+        for (int z = 0; z < BZ; ++z) acc[z] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ty = 0; ty < 3; ++ty)
+#pragma unroll
+            for (int tx = 0; tx < 3; ++tx) {
+                bf16x8 a[HZ];
+#pragma unroll
+                for (int hz = 0; hz < HZ; ++hz)
+                    a[hz] = *(const bf16x8 *)(hb + ((hz * HY + ty) * HX + tx) * CH);
+#pragma unroll
+                for (int tz = 0; tz < 3; ++tz)
+#pragma unroll
+                    for (int z = 0; z < BZ; ++z)
+                        acc[z] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[z + tz], wb[tz * 9 + ty * 3 + tx], acc[z],
+                                                                        0, 0, 0);
+            }
+        // D[i = 4*kg + r][j = li]: lane li = 0 gathers columns 0..5 of its row
+#pragma unroll
+        for (int z = 0; z < BZ; ++z)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float d = acc[z][r];
+                float o[2 * NO];
+#pragma unroll
+                for (int j = 0; j < 2 * NO; ++j) o[j] = __shfl(d, (lane & ~15) + j, 64);
+                const int i = 4 * kg + r;
+                const int zz = b.z0 + z, yy = b.y0 + 2 * wv + (i >> 3), xx = b.x0 + (i & 7);
+                if (li != 0 || zz >= s.D || yy >= s.H || xx >= s.W) continue;
+                const int64_t v = vidx(b.n, zz, yy, xx, s);
+                const float o0 = o[0] + o[3], o1 = o[1] + o[4], o2 = o[2] + o[5];
+                const float a = alpha_factor(al, v);
+                const float delta = o0 + a * (o1 + a * o2);
+                if (INFER) {
+                    const bool interior = zz > 0 && zz < s.D - 1 && yy > 0 && yy < s.H - 1 && xx > 0 && xx < s.W - 1;
+                    out[v] = interior ? u[v] + scale * delta : 0.0f;
+                } else {
+                    out[v] = delta;
+                }
+            }
+        if (nxt < nbricks) {
+            store_halo(sx[buf ^ 1], h, t);
+            __syncthreads();
+        }
     }
 }
 
@@ -628,7 +753,7 @@ double forward_backward(piro_surrogate *s, const Shape &sh, const float *u, cons
     forward_body(s, sh, u, inv_scale);
     bf16 *x[5];
     for (int l = 0; l < 5; ++l) x[l] = s->act + (int64_t)l * nv * CH;
-    k_l6_fwd<false><<<grid_for(nv, 256), 256, 0, st>>>(sh, x[4], s->params + W6_OFF, al, nullptr, 1.0f, s->delta);
+    k_l6<false><<<conv_grid(s), 256, 0, st>>>(sh, x[4], s->params + W6_OFF, al, nullptr, 1.0f, s->delta);
     check_launch();
     const int64_t n_int = (int64_t)sh.n * std::max(sh.D - 2, 0) * std::max(sh.H - 2, 0) * std::max(sh.W - 2, 0);
     PIRO_REQUIRE(n_int > 0, "instances need interior cells");
@@ -770,8 +895,8 @@ int piro_surrogate_forward(piro_surrogate *s, int n, int D, int H, int W, const 
     Shape sh = mkshape(n, D, H, W);
     ensure_acts(s, sh.nv());
     forward_body(s, sh, d_u, 1.0f / scale);
-    k_l6_fwd<true><<<grid_for(sh.nv(), 256), 256, 0, s->ctx->stream>>>(
-        sh, s->act + 4 * sh.nv() * CH, s->params + W6_OFF, d_alpha, d_u, scale, d_u_out);
+    k_l6<true><<<conv_grid(s), 256, 0, s->ctx->stream>>>(sh, s->act + 4 * sh.nv() * CH, s->params + W6_OFF, d_alpha,
+                                                         d_u, scale, d_u_out);
     check_launch();
     PIRO_API_END
 }

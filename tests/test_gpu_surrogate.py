@@ -60,7 +60,11 @@ def ref_forward(p, u, al, scale):
         z = po.conv3d_fwd_f64(x, bf16(w.astype(np.float32)).astype(np.float64))
         x = bf16(z.astype(np.float32)).astype(np.float64)
         acts.append(x)
-    o = po.conv3d_fwd_f64(x, w6.astype(np.float64))
+    # the device's L6 weights: BF16(w) plus the BF16 rounding of the remainder
+    w6f = np.asarray(w6, np.float32)
+    hi = bf16(w6f)
+    lo = bf16((w6f - hi).astype(np.float32))
+    o = po.conv3d_fwd_f64(x, hi.astype(np.float64) + lo.astype(np.float64))
     a = alpha_factor(al)
     delta = o[..., 0] + a * (o[..., 1] + a * o[..., 2])
     return x_in, acts, delta
