@@ -251,6 +251,24 @@ def bench_surrogate(args):
     ctx.close()
 
 
+def bench_surrogate_fwd(args):
+    """Forward calls only, on the 128^3 run's field (timing does not depend on
+    the weights, so the untrained network is used)."""
+    from piro_amd import learned
+    ctx = Context(0)
+    sur = Surrogate(ctx, seed=5)
+    run = learned.HeatRun(ctx, side=128)
+    shape = (1, 128, 128, 128)
+    y = ctx.empty_like(run.u0)
+    sur.forward(shape, run.u0, run.alpha, 1.0, out=y)  # warm-up (allocates the activations)
+    t_call = timed(ctx, lambda: sur.forward(shape, run.u0, run.alpha, 1.0, out=y), reps=10)
+    flop = 228096 * 128 ** 3
+    print(json.dumps({"metric": "surrogate forward, 128^3 (one k=16 call)", "ms_per_call": t_call * 1e3,
+                      "tflops": flop / t_call / 1e12, "frac_bf16_peak": flop / t_call / MFMA_BF16}), flush=True)
+    sur.close()
+    ctx.close()
+
+
 # ---------------------------------------------------------------------- JLI
 
 def bench_jli(args):
@@ -291,7 +309,7 @@ def bench_jli(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["lu", "batch", "advection", "surrogate", "jli"])
+    ap.add_argument("what", choices=["lu", "batch", "advection", "surrogate", "surrogate_fwd", "jli"])
     ap.add_argument("--side", type=int, default=256)
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--traj", type=int, default=8000)
@@ -299,7 +317,7 @@ def main():
     ap.add_argument("--passes", type=int, default=3)
     args = ap.parse_args()
     {"lu": bench_lu, "batch": bench_batch, "advection": bench_advection, "surrogate": bench_surrogate,
-     "jli": bench_jli}[args.what](args)
+     "surrogate_fwd": bench_surrogate_fwd, "jli": bench_jli}[args.what](args)
 
 
 if __name__ == "__main__":
