@@ -292,6 +292,26 @@ int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz,
                    float *d_x, float *d_work, int *iters, float *res);
 
 
+/* ------------------------------------------------------ DENSE format */
+
+/* Replaces Piro::DENSE::laplacian (src/operatorlaplacian.cpp:65-102,
+ * DENSE::stencils::laplacian_{uw,dw,cd}, src/stencils.cpp:431-571): the
+ * N x N FP32 operator, row-major with leading dimension ld (N = nx*ny*nz,
+ * padded dims), diagonal -6/norm and neighbours 1/norm computed as
+ * int/float in FP32; uw assigns its "+" neighbours onto the diagonal and dw
+ * its "-" neighbours (the last assignment wins); no boundary rule and no
+ * diffusivity scaling, as in the reference. */
+int piro_assemble_laplacian_dense(piro_ctx *ctx, int nx, int ny, int nz,
+                                  int scheme, float norm, float *d_a,
+                                  int64_t ld);
+/* Replaces DENSE TIMESCHEME_11 (SpMVDENSE + addVectors,
+ * src/kernelprocess.cpp:42-62): over all n rows,
+ * y = x + fl(dt * sum_{k=0}^{n-1} fl(a[r][k]*x[k])), the sum in column
+ * order, bit-identical to that loop (exact zero terms cannot change it and
+ * are not added).  d_x and d_y must differ. */
+int piro_fe_step_dense(piro_ctx *ctx, int n, const float *d_a, int64_t ld,
+                       float dt, const float *d_x, float *d_y);
+
 /* -------------------------------------------------- mixed-precision LU */
 
 /* Densify CSR into a row-major FP32 matrix (d_dense[r*ld + c]); duplicate

@@ -94,13 +94,14 @@ struct Field {
 // their own layout and unpacked to the CSR the step kernels read).
 struct Operator {
     Case *c = nullptr;
-    int format = 0;              // 0 CSR, 2 HT, 3 COO
+    int format = 0;              // 0 CSR, 1 DENSE, 2 HT, 3 COO
     int32_t *rp = nullptr, *col = nullptr;
     float *val = nullptr;
     int64_t nnz = 0;
     int32_t *ht_keys = nullptr;  // HT only
     float *ht_vals = nullptr;
     int ht_size = 0;
+    float *dense = nullptr;      // DENSE only: N x N row-major
 };
 
 struct VectorRef {
@@ -117,6 +118,9 @@ class process {
     Operator laplacian(const std::string &var1, const std::string &var2);
     Operator gradient(const std::string &var);
     VectorRef vector(const std::string &var);
+    // process::div (src/process.cpp:168-212), CSR only: the divergence
+    // matrices with each component's rows scaled by var2's component
+    Operator div(const std::string &var1, const std::string &var2);
 
   private:
     Case *c_;
@@ -175,8 +179,8 @@ class Case {
     int ts = 0;
     CaseOptions opt;
     int match(const std::string &var) const;  // matchscalartovar: 0 if absent
-    Operator lap, grad_x, grad_y, grad_z, rhs;
-    bool lap_init = false, grad_init = false, dot_init = false, lu_init = false;
+    Operator lap, grad_x, grad_y, grad_z, rhs, div_rhs;
+    bool lap_init = false, grad_init = false, dot_init = false, lu_init = false, div_init = false;
     float *lu = nullptr, *lu_a = nullptr;
     int32_t *lu_piv = nullptr;
     void export_paraview(int step);

@@ -192,6 +192,56 @@ def geam3(Ga: CSR, Gb: CSR, Gc: CSR, v3):
     return CSR(rp, col, val)
 
 
+def div_csr(nx, ny, nz, scheme, norm, u3):
+    """process::div (src/process.cpp:168-212) on the CSR path: the divergence
+    matrices are the gradient stencils (src/stencils.cpp:995-1003) with
+    norm = 2*(l0/float(n0-2)) (src/operatordivergenceCSR.cpp:18); SpMSCSR
+    (SpMSCSR.cl, launched by src/kernelcsrgeam.cpp:88-133) scales every entry
+    of component d's matrix by u_d at its row; csrgeam_2 (csrgeam_2.cl) is
+    csrgeam_3's concatenation-merge without factors.  u3 = [ux|uy|uz] (3N)."""
+    N = nx * ny * nz
+    u3 = np.ascontiguousarray(u3, np.float32)
+    G = [row_scale(grad_csr(nx, ny, nz, scheme, d, norm), u3[d * N:(d + 1) * N]) for d in range(3)]
+    return geam3(G[0], G[1], G[2], np.ones(3 * N, np.float32))
+
+
+def lap_dense(nx, ny, nz, scheme, norm):
+    """PIRO's DENSE Laplacian (src/operatorlaplacian.cpp:65-102, stencils
+    src/stencils.cpp:431-571): N x N FP32, every row written in the order
+    diagonal, x-, x+, y-, y+, z-, z+ with plain assignment (uw sends the "+"
+    neighbours to the diagonal, dw the "-" neighbours; the last write wins).
+    Values -6/norm and 1/norm in FP32 (int / float)."""
+    N = nx * ny * nz
+    a = np.zeros((N, N), np.float32)
+    norm = np.float32(norm)
+    diag, off = np.float32(-6.0) / norm, np.float32(1.0) / norm
+    l = np.arange(N)
+    i, j, k = l % nx, (l // nx) % ny, l // (nx * ny)
+    a[l, l] = diag
+    steps = [(i > 0, True, 1), (i < nx - 1, False, 1), (j > 0, True, nx), (j < ny - 1, False, nx),
+             (k > 0, True, nx * ny), (k < nz - 1, False, nx * ny)]
+    for cond, minus, st in steps:
+        rows = l[cond]
+        if scheme == UW and not minus or scheme == DW and minus:
+            tgt = rows
+        else:
+            tgt = rows - st if minus else rows + st
+        a[rows, tgt] = off
+    return a
+
+
+def fe_dense_step(a, dt, x):
+    """SpMVDENSE + addVectors (src/kernelprocess.cpp:42-62): over all rows,
+    y = x + fl(dt * s), s the FP32 sum of fl(a[r][k]*x[k]) for k = 0..N-1 in
+    order (every term, zeros included)."""
+    a = np.asarray(a, np.float32)
+    x = np.asarray(x, np.float32)
+    run = np.zeros(a.shape[0], np.float32)
+    for k in range(a.shape[1]):
+        run = (run + a[:, k] * x[k]).astype(np.float32)
+    return (x + np.float32(dt) * run).astype(np.float32)
+
+
 def geam2(alpha, A: CSR, beta, B: CSR):
     n = A.n_rows
     nnz = lib.po_geam2(n, np.float32(alpha), A.row_ptr, A.col, A.val, np.float32(beta), B.row_ptr, B.col,

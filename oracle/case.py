@@ -31,9 +31,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (CD, CSR, face_indices, fe_csr_step, fe_ht_step, geam3, grad_csr, grad_norm, ht_build,
-               ht_table_size, ic_gaussian, jli, lap_csr, lap_norm, lu_factor_f64, lu_solve_f64, row_scale,
-               spmv_allrows_step)
+from . import (CD, CSR, div_csr, face_indices, fe_csr_step, fe_dense_step, fe_ht_step, geam3, grad_csr,
+               grad_norm, ht_build, ht_table_size, ic_gaussian, jli, lap_csr, lap_dense, lap_norm,
+               lu_factor_f64, lu_solve_f64, row_scale, spmv_allrows_step)
 
 _NUM = re.compile(r"\s*([+-]?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?|[+-]?(?:inf|nan))", re.IGNORECASE)
 _INT = re.compile(r"\s*([+-]?\d+)")
@@ -272,6 +272,9 @@ def setup(assets_dir: str, case_name: str) -> CaseSpec:
 def _operator(spec: CaseSpec, alpha_row_scaled: bool):
     n = spec.n
     if spec.simindex == 0:
+        if spec.data_type == 1:
+            # Piro::DENSE::laplacian: no diffusivity scaling (csrscale is CSR-only)
+            return ("dense", lap_dense(*n, spec.space_scheme, lap_norm(spec.l[0], n[0] - 2)))
         A = lap_csr(*n, spec.space_scheme, lap_norm(spec.l[0], n[0] - 2))
         if spec.data_type == 0 and alpha_row_scaled:
             A = row_scale(A, spec.slots[spec.match("Alpha")].device[: spec.N])
@@ -285,7 +288,13 @@ def _operator(spec: CaseSpec, alpha_row_scaled: bool):
         G = [grad_csr(*n, spec.space_scheme, d, g) for d in range(3)]
         v3 = spec.slots[spec.match("v")].device
         return ("csr", geam3(G[0], G[1], G[2], v3))
-    raise ValueError("simindex must be 0 or 1")
+    if spec.simindex == 2:
+        # process::div("rho", "u") (src/process.cpp:168-212), CSR only
+        if spec.data_type != 0:
+            raise ValueError("div is CSR only")
+        u3 = spec.slots[spec.match("u")].device
+        return ("csr", div_csr(*n, spec.space_scheme, grad_norm(spec.l[0], n[0] - 2), u3))
+    raise ValueError("simindex must be 0, 1 or 2")
 
 
 def _ht_dense(spec, keys, vals):
@@ -311,6 +320,8 @@ def run(spec: CaseSpec, max_steps: int = -1, alpha_row_scaled: bool = True):
         if spec.time_scheme == 11:
             if op[0] == "ht":
                 y = fe_ht_step(N, op[1], op[2], spec.dt, x)
+            elif op[0] == "dense":
+                y = fe_dense_step(op[1], spec.dt, x)
             elif spec.data_type == 3:
                 y = spmv_allrows_step(op[1], spec.dt, x)
             else:

@@ -8,6 +8,7 @@
 // Reference entry points exercised:
 //   Piro::CSR::stencils::laplacian_{cd,uw,dw}   src/stencils.cpp:233-429
 //   Piro::CSR::stencils::gradient_{cd,uw,dw}_*  src/stencils.cpp:7-231
+//   Piro::DENSE::stencils::laplacian_{cd,uw,dw} src/stencils.cpp:431-571
 //   Piro::methods::sethash                      src/methods.cpp:74-118
 //   Piro::calculategaussian                     src/ic.cpp:16-27
 //   Piro::bc::prepbc + Piro::paraview::writevth src/bc.cpp:80-105,
@@ -17,6 +18,7 @@
 // Usage:
 //   piro_ref_harness lap  <scheme 1|2|3> nx ny nz norm_hex out.bin
 //   piro_ref_harness grad <scheme> <dir 0|1|2> nx ny nz norm_hex out.bin
+//   piro_ref_harness dense <scheme> nx ny nz norm_hex out.bin  (N x N floats)
 //   piro_ref_harness ht   <scheme> nx ny nz norm_hex table_size out.bin
 //   piro_ref_harness gauss nx ny nz lx ly lz mx my mz sx sy sz out.bin
 //   piro_ref_harness vtk  nx ny nz n_scal n_vec step in.bin out_dir
@@ -104,6 +106,22 @@ int main(int argc, char **argv)
         tab[scheme - 1][dir](cd, n, norm);
         FILE *f = fopen(argv[8], "wb");
         put(f, cd.rowpointers); put(f, cd.columns); put(f, cd.values);
+        fclose(f);
+        return 0;
+    }
+    if (cmd == "dense" && argc == 8) {
+        int scheme = atoi(argv[2]);
+        std::vector<uint> n = {(uint)atoi(argv[3]), (uint)atoi(argv[4]), (uint)atoi(argv[5])};
+        set_mesh(n[0], n[1], n[2]);
+        const size_t N = (size_t)n[0] * n[1] * n[2];
+        Piro::CellData cd;
+        cd.matrix.assign(N * N, 0.0f);
+        const float norm = from_hex(argv[6]);
+        if (scheme == 3) Piro::DENSE::stencils::laplacian_cd(cd, n, norm);
+        else if (scheme == 2) Piro::DENSE::stencils::laplacian_dw(cd, n, norm);
+        else Piro::DENSE::stencils::laplacian_uw(cd, n, norm);
+        FILE *f = fopen(argv[7], "wb");
+        put(f, cd.matrix);
         fclose(f);
         return 0;
     }

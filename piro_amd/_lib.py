@@ -73,6 +73,8 @@ _SIGS = {
     "piro_fe_step_csr": [vp, i32, i32, i32, vp, vp, vp, i64, f32, vp, vp, vp],
     "piro_fe_run_csr": [vp, i32, i32, i32, vp, vp, vp, i64, f32, i32, vp, vp, vp],
     "piro_fe_step_allrows": [vp, i32, vp, vp, vp, i64, f32, vp, vp],
+    "piro_assemble_laplacian_dense": [vp, i32, i32, i32, i32, f32, vp, i64],
+    "piro_fe_step_dense": [vp, i32, vp, i64, f32, vp, vp],
     "piro_spmv_csr": [vp, i32, vp, vp, vp, i64, vp, vp],
     "piro_fe_set_variant": [vp, i32],
     "piro_fe_get_variant": [vp, P_i32],

@@ -221,6 +221,31 @@ class Context:
                                              _p(G.val)))
         return G
 
+    def divergence_csr(self, nx, ny, nz, scheme, norm, u3: DevArray) -> DevCSR:
+        """process::div (src/process.cpp:168-212): the gradient stencils with
+        the divergence norm, component d's rows scaled by u_d (SpMSCSR) and
+        merged by the concatenation-merge (csrgeam_2); u3 = [ux|uy|uz]."""
+        N = nx * ny * nz
+        G = []
+        for d in range(3):
+            g = self.gradient_csr(nx, ny, nz, scheme, d, norm)
+            self.row_scale(g, u3.view(d * N, (d + 1) * N))
+            G.append(g)
+        ones = self.full(3 * N, 1.0)
+        return self.geam3(G[0], G[1], G[2], ones)
+
+    def laplacian_dense(self, nx, ny, nz, scheme, norm) -> DevArray:
+        """PIRO's DENSE Laplacian, N x N row-major (N = nx*ny*nz)."""
+        N = nx * ny * nz
+        a = self.empty(N * N)
+        check(lib.piro_assemble_laplacian_dense(self.h, nx, ny, nz, scheme, float(norm), _p(a), N))
+        return a
+
+    def fe_step_dense(self, a: DevArray, n, dt, x, y=None):
+        y = self.empty(n) if y is None else y
+        check(lib.piro_fe_step_dense(self.h, int(n), _p(a), int(n), float(dt), _p(x), _p(y)))
+        return y
+
     def csr_from_host(self, row_ptr, col, val) -> DevCSR:
         return DevCSR(self.from_host(row_ptr, np.int32), self.from_host(col, np.int32), self.from_host(val, np.float32))
 

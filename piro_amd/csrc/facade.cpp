@@ -188,7 +188,10 @@ Case::Case(const std::string &assets_dir, const std::string &case_name, const Ca
     totaltime = stof_req(reader, "Solve", "TotalTime");
     save = stoi_req(reader, "Solve", "Save");
     data_type = stoi_req(reader, "Solve", "Data");
-    if (data_type == 1) throw SolverError("Data = 1 (DENSE) is not supported: N^2 storage (use CSR, COO or HT)");
+    // DENSE keeps N x N floats on the device (the reference's format):
+    // refuse cases whose matrix would not fit in 64 GiB
+    if (data_type == 1 && (double)N * (double)N * 4.0 > 64.0 * (1ull << 30))
+        throw SolverError("Data = 1 (DENSE) needs N^2 floats; this grid is too large for it");
     if (data_type < 0 || data_type > 3) throw SolverError("Invalid DataType");  // src/process.cpp:93-96
     if (data_type == 2) {
         IniReader rh(assets_dir + "/Data/hashtable.ini");
@@ -362,8 +365,13 @@ int Case::solve()
             UEqn.Solve(time);
             break;
         }
+        case 2: {
+            scalarMatrix UEqn(solver.ddt_r("rho") = solver.div("rho", "u"));
+            UEqn.Solve(time);
+            break;
+        }
         default:
-            throw SolverError("Invalid sim index (2 = continuity is not part of this path)");
+            throw SolverError("Invalid sim index");
         }
         time += timestep;
         ++steps;
@@ -391,6 +399,15 @@ Operator process::laplacian(const std::string &var1, const std::string &var2)
     Operator op;
     op.c = c_;
     op.format = c.data_type;
+    if (c.data_type == 1) {
+        // Piro::DENSE::laplacian (src/operatorlaplacian.cpp:65-102): N x N,
+        // no diffusivity scaling
+        op.dense = (float *)c.alloc((size_t)c.N * (size_t)c.N * sizeof(float));
+        ck(piro_assemble_laplacian_dense(c.ctx(), c.n[0], c.n[1], c.n[2], c.space_scheme, norm, op.dense, c.N));
+        c.lap = op;
+        c.lap_init = true;
+        return op;
+    }
     op.rp = (int32_t *)c.alloc((size_t)(c.N + 1) * sizeof(int32_t));
     op.col = (int32_t *)c.alloc((size_t)nnz * sizeof(int32_t));
     op.val = (float *)c.alloc((size_t)nnz * sizeof(float));
@@ -485,6 +502,50 @@ Operator operator&(const VectorRef &v, const Operator &grad)
     return c.rhs;
 }
 
+Operator process::div(const std::string &var1, const std::string &var2)
+{
+    (void)var1;  // PIRO does not use it (src/process.cpp:173-178)
+    Case &c = *c_;
+    if (c.data_type != 0) throw SolverError("Invalid DataType (div is CSR only, src/process.cpp:207-210)");
+    if (c.div_init) return c.div_rhs;
+    // Piro::CSR::div (src/operatordivergenceCSR.cpp): the gradient stencils
+    // with norm 2*(l0/float(n0-2)); SpMSCSR scales component d's entries by
+    // var2's component d at their row; csrgeam_2 merges the three as the
+    // concatenation-merge without factors
+    const int slot = c.match(var2);
+    if (c.types[slot] != 2) throw SolverError("div needs a vector field as its second argument");
+    const float norm = piro_grad_norm(c.l[0], c.n[0] - 2);
+    Operator g[3];
+    for (int d = 0; d < 3; ++d) {
+        int64_t nnz = 0;
+        ck(piro_grad_nnz(c.n[0], c.n[1], c.n[2], d, &nnz));
+        g[d].c = c_;
+        g[d].rp = (int32_t *)c.alloc((size_t)(c.N + 1) * sizeof(int32_t));
+        g[d].col = (int32_t *)c.alloc((size_t)nnz * sizeof(int32_t));
+        g[d].val = (float *)c.alloc((size_t)nnz * sizeof(float));
+        g[d].nnz = nnz;
+        ck(piro_assemble_gradient_csr(c.ctx(), c.n[0], c.n[1], c.n[2], c.space_scheme, d, norm, g[d].rp, g[d].col,
+                                      g[d].val));
+        ck(piro_row_scale(c.ctx(), (int)c.N, g[d].rp, g[d].val, c.fields[slot] + (size_t)d * c.N));
+    }
+    std::vector<float> ones(3 * (size_t)c.N, 1.0f);
+    float *d_ones = (float *)c.alloc(ones.size() * sizeof(float));
+    ck(piro_memcpy_h2d(c.ctx(), d_ones, ones.data(), ones.size() * sizeof(float)));
+    Operator &D = c.div_rhs;
+    D.c = c_;
+    D.format = 0;
+    D.rp = (int32_t *)c.alloc((size_t)(c.N + 1) * sizeof(int32_t));
+    int64_t nnz = 0;
+    ck(piro_geam3_rowptr(c.ctx(), (int)c.N, g[0].rp, g[0].col, g[1].rp, g[1].col, g[2].rp, g[2].col, D.rp, &nnz));
+    D.col = (int32_t *)c.alloc((size_t)nnz * sizeof(int32_t));
+    D.val = (float *)c.alloc((size_t)nnz * sizeof(float));
+    D.nnz = nnz;
+    ck(piro_geam3_fill(c.ctx(), (int)c.N, g[0].rp, g[0].col, g[0].val, g[1].rp, g[1].col, g[1].val, g[2].rp, g[2].col,
+                       g[2].val, d_ones, D.rp, D.col, D.val));
+    c.div_init = true;
+    return D;
+}
+
 // ------------------------------------------------------- time integration
 
 Field Field::operator=(const Operator &op)
@@ -497,6 +558,8 @@ Field Field::operator=(const Operator &op)
         if (op.format == 0)
             ck(piro_fe_step_csr(x, cs.n[0], cs.n[1], cs.n[2], op.rp, op.col, op.val, op.nnz, cs.timestep, d,
                                 cs.has_bc ? cs.snapshot : nullptr, out));
+        else if (op.format == 1)
+            ck(piro_fe_step_dense(x, (int)cs.N, op.dense, cs.N, cs.timestep, d, out));
         else
             ck(piro_fe_step_allrows(x, (int)cs.N, op.rp, op.col, op.val, op.nnz, cs.timestep, d, out));
     } else if (cs.time_scheme == 12 && cs.solver_scheme == 40) {
@@ -508,6 +571,7 @@ Field Field::operator=(const Operator &op)
     } else if (cs.time_scheme == 12 && cs.solver_scheme == 27) {
         // backward Euler (I - dt L) x = b by the mixed-precision dense LU; the
         // reference instead eliminates L once and returns a zero field
+        if (op.format == 1) throw SolverError("Wrong data type (the LU path takes CSR, HT or COO)");
         const int n = (int)cs.N;
         if (n > 16384) throw SolverError("the dense LU path is limited to N <= 16384");
         if (!cs.lu_init) {

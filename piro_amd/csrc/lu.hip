@@ -16,11 +16,10 @@
 //   in-block -- each panel's exchanges and U12 = L11^-1 A12 for the block's
 //             later columns, then a K = 16 MFMA update of those columns only;
 //   block  -- the block's 64 exchanges applied to the trailing columns and
-//             to the finished L columns on the left (LASWP), the deferred
-//             K = 16 updates of the block's rows and the block's U rows,
-//             then one K = 64 MFMA update (v_mfma_f32_16x16x4f32, FP32 in,
-//             FP32 accumulate) of the trailing matrix, one 128 x 128 tile
-//             per workgroup.
+//             to the finished L columns on the left (LASWP), the block's U
+//             rows (one TRSM with the 64 x 64 L11), then one K = 64 MFMA
+//             update (v_mfma_f32_16x16x4f32, FP32 in, FP32 accumulate) of
+//             the trailing matrix, one 128 x 128 tile per workgroup.
 // Solves: P b as a gather, then blocked forward and backward substitution
 // over 64-row blocks, one persistent launch per triangle.
 // Refinement: r = b - A x with FP64 accumulation over the FP32 entries of A,
@@ -315,14 +314,18 @@ __global__ __launch_bounds__(256) void k_panel_global(float *__restrict__ a, int
 // ----------------------------------------------------- swap + U12 TRSM
 
 // columns [c0, c1): the exchanges of steps k0 .. k1-1, in order
-// the exchanges of steps k0 .. k1-1 on columns [0, nl) (finished L columns)
-// and [c0, c1) (trailing columns): one thread per column, exchanges in order
-__global__ __launch_bounds__(256) void k_laswp(float *__restrict__ a, int lda, int k0, int k1, int nl, int c0, int c1,
+// the exchanges of a block's steps kb .. kb+nbk-1, one thread per column of
+// the whole matrix, exchanges in order: finished L columns on the left and
+// trailing columns take all of them; the block's own columns take those of
+// the panels after their own (the earlier ones were applied as the panels
+// were factored)
+__global__ __launch_bounds__(256) void k_laswp(float *__restrict__ a, int lda, int n, int kb, int nbk,
                                                const int32_t *__restrict__ ipiv)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int c = i < nl ? i : c0 + (i - nl);
-    if (c >= c1) return;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const int k1 = kb + nbk;
+    const int k0 = (c >= kb && c < k1) ? kb + ((c - kb) / PB + 1) * PB : kb;
     for (int j = k0; j < k1; ++j) {
         const int pj = ipiv[j];
         if (pj != j) {
@@ -333,9 +336,9 @@ __global__ __launch_bounds__(256) void k_laswp(float *__restrict__ a, int lda, i
     }
 }
 
-// columns [c0, c1): apply the exchanges of steps k .. k+jb-1 (jb <= W) if
-// SWAP, then U12 = L11^-1 A12 with L11 the unit lower jb x jb block at (k, k)
-template <int W, bool SWAP>
+// columns [c0, c1): apply the exchanges of steps k .. k+jb-1 (jb <= W),
+// then U12 = L11^-1 A12 with L11 the unit lower jb x jb block at (k, k)
+template <int W>
 __global__ __launch_bounds__(256) void k_swap_trsm(float *__restrict__ a, int lda, int k, int jb, int c0, int c1,
                                                    const int32_t *__restrict__ ipiv)
 {
@@ -350,7 +353,7 @@ __global__ __launch_bounds__(256) void k_swap_trsm(float *__restrict__ a, int ld
     __syncthreads();
     const int c = c0 + blockIdx.x * 256 + t;
     if (c >= c1) return;
-    for (int j = 0; SWAP && j < jb; ++j) {
+    for (int j = 0; j < jb; ++j) {
         const int pj = s_p[j];
         if (pj != k + j) {
             const float x = a[(int64_t)(k + j) * lda + c];
@@ -368,6 +371,41 @@ __global__ __launch_bounds__(256) void k_swap_trsm(float *__restrict__ a, int ld
 #pragma unroll
     for (int i = 0; i < W; ++i)
         if (i < jb) a[(int64_t)(k + i) * lda + c] = u[i];
+}
+
+// U12 = L11^-1 A12 for the trailing columns [c0, c1) of a whole block: L11
+// the unit lower nbk x nbk block at (kb, kb) (nbk <= W), staged in LDS; one
+// thread per column, rows in registers, each row's sum as four interleaved
+// partial sums.
+template <int W>
+__global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int lda, int kb, int nbk, int c0, int c1)
+{
+    __shared__ __attribute__((aligned(16))) float s_l[W][W];
+    const int t = threadIdx.x;
+    for (int e = t; e < W * W; e += 256) {
+        const int i = e / W, j = e % W;
+        s_l[i][j] = (i < nbk && j < i) ? a[(int64_t)(kb + i) * lda + kb + j] : 0.0f;
+    }
+    __syncthreads();
+    const int c = c0 + blockIdx.x * 256 + t;
+    if (c >= c1) return;
+    float u[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) u[i] = i < nbk ? a[(int64_t)(kb + i) * lda + c] : 0.0f;
+#pragma unroll
+    for (int i = 1; i < W; ++i) {
+        // four partial sums (j mod 4) keep four dependent chains in flight
+        float p4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < i; ++j) p4[j & 3] += s_l[i][j] * u[j];
+        u[i] = u[i] - ((p4[0] + p4[1]) + (p4[2] + p4[3]));
+        // row i is final: later rows' LDS reads may not move above it
+        // (hoisting all of them spills)
+        __asm__ volatile("" : "+v"(u[i])::"memory");
+    }
+#pragma unroll
+    for (int i = 0; i < W; ++i)
+        if (i < nbk) a[(int64_t)(kb + i) * lda + c] = u[i];
 }
 
 // ------------------------------------------------- MFMA trailing update
@@ -439,23 +477,6 @@ __global__ __launch_bounds__(256) void k_update_mfma(float *__restrict__ a, int 
         }
 }
 
-// within one NB-column block: the exchanges of the block's later panels
-// applied to its earlier panels' columns, so that the block's L11 is in
-// final row order before the block-level TRSM
-__global__ __launch_bounds__(256) void k_swap_block_left(float *__restrict__ a, int lda, int kb, int nbk,
-                                                         const int32_t *__restrict__ ipiv)
-{
-    const int c = kb + blockIdx.x * 256 + threadIdx.x;
-    if (c >= kb + nbk) return;
-    for (int j = kb + ((c - kb) / PB + 1) * PB; j < kb + nbk; ++j) {
-        const int pj = ipiv[j];
-        if (pj != j) {
-            const float x = a[(int64_t)j * lda + c];
-            a[(int64_t)j * lda + c] = a[(int64_t)pj * lda + c];
-            a[(int64_t)pj * lda + c] = x;
-        }
-    }
-}
 
 // ------------------------------------------------------------ refinement
 
@@ -948,40 +969,23 @@ void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
             panel(ctx, n, a, lda, k, jb, ipiv, d_info);
             const int c0 = k + jb, c1 = kb + nbk;
             if (c0 < c1) {
-                k_swap_trsm<PB, true><<<grid_for(c1 - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, c1, ipiv);
+                k_swap_trsm<PB><<<grid_for(c1 - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, c1, ipiv);
                 check_launch();
                 dim3 g((c1 - c0 + 127) / 128, (n - c0 + 127) / 128);
                 k_update_mfma<PB><<<g, 256, 0, s>>>(a, lda, n, k, jb, c0, c0, c1);
                 check_launch();
             }
         }
-        if (nbk > PB) {
-            k_swap_block_left<<<grid_for(nbk, 256), 256, 0, s>>>(a, lda, kb, nbk, ipiv);
-            check_launch();
-        }
-        // trailing columns: all of the block's exchanges first (rows below
-        // the block carry deferred updates, so no exchange may come after
-        // an update), then each panel's U rows in turn with the K = 16
-        // updates confined to the block's own rows, then one K = nbk update
-        // of every row below the block
+        // all of the block's exchanges first (rows below the block carry
+        // deferred updates, so no exchange may come after an update), then
+        // U12 = L11^-1 A12 for the whole block, then one K = nbk update of
+        // every row below the block
+        k_laswp<<<grid_for(n, 256), 256, 0, s>>>(a, lda, n, kb, nbk, ipiv);
+        check_launch();
         const int c0 = kb + nbk;
-        // the block's exchanges reach the finished L columns [0, kb) and the
-        // trailing columns in one launch
-        if (kb + n - c0 > 0) {
-            k_laswp<<<grid_for(kb + n - c0, 256), 256, 0, s>>>(a, lda, kb, kb + nbk, kb, c0, n, ipiv);
-            check_launch();
-        }
         if (c0 < n) {
-            for (int k = kb; k < kb + nbk; k += PB) {
-                const int jb = min(PB, kb + nbk - k);
-                k_swap_trsm<PB, false><<<grid_for(n - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, n, ipiv);
-                check_launch();
-                if (k + jb < c0) {
-                    dim3 g((n - c0 + 127) / 128, (c0 - (k + jb) + 127) / 128);
-                    k_update_mfma<PB><<<g, 256, 0, s>>>(a, lda, c0, k, jb, k + jb, c0, n);
-                    check_launch();
-                }
-            }
+            k_trsm_block<NB><<<grid_for(n - c0, 256), 256, 0, s>>>(a, lda, kb, nbk, c0, n);
+            check_launch();
             dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
             k_update_mfma<NB><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n);
             check_launch();

@@ -11,6 +11,7 @@ Runs oracle/_ref/piro_ref_harness (built from the PIRO sources by
 Reference functions exercised (paths relative to the PIRO tree):
   CSR::stencils::laplacian_{cd,uw,dw}   src/stencils.cpp:233-429
   CSR::stencils::gradient_*             src/stencils.cpp:7-231
+  DENSE::stencils::laplacian_*          src/stencils.cpp:431-571
   methods::sethash                      src/methods.cpp:74-118
   calculategaussian                     src/ic.cpp:16-27
   bc::prepbc + paraview::writevth/vti   src/bc.cpp:80-105, src/paraview.cpp:12-137
@@ -74,6 +75,10 @@ def vtk_fields(dims, n_scal, n_vec, seed):
             v[idx + (N if k >= n_scal else 0)] = special[(j + k) % len(special)]
         out.append(v)
     return out
+
+
+# DENSE Laplacian grids (N x N floats; frozen as non-zero triplets + digest)
+DENSE_GRIDS = [(4, 5, 6), (6, 6, 6)]
 
 
 def f32_hex(x: float) -> str:
@@ -146,6 +151,17 @@ def main() -> int:
                 arrays[key + "_rp"], arrays[key + "_col"], arrays[key + "_val"] = rp, col, val
                 meta["grad"].append({"key": key, "scheme": scheme, "dir": d, "dims": [nx, ny, nz],
                                      "norm_hex": f32_hex(norm)})
+    for scheme in (1, 2, 3):
+        for (nx, ny, nz) in DENSE_GRIDS:
+            norm = lap_norm(1.0, nx - 2)
+            (m,) = run(["dense", scheme, nx, ny, nz, f32_hex(norm)], ["<f4"])
+            N = nx * ny * nz
+            m = m.reshape(N, N)
+            r, c = np.nonzero(m)
+            key = f"dense_s{scheme}_{nx}x{ny}x{nz}"
+            arrays[key + "_r"], arrays[key + "_c"], arrays[key + "_v"] = r.astype(np.int32), c.astype(np.int32), m[r, c]
+            meta.setdefault("dense", []).append({"key": key, "scheme": scheme, "dims": [nx, ny, nz],
+                                                 "norm_hex": f32_hex(norm), "digest": digest(m)})
     for scheme, (nx, ny, nz), l0 in HT_CASES:
         norm = lap_norm(l0, nx - 2)
         N = nx * ny * nz

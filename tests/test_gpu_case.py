@@ -81,6 +81,29 @@ def test_heat_forward_euler_bit_exact(tmp_path, data):
     assert field.max() == 1.0 and np.count_nonzero(field) > np.count_nonzero(spec.snapshot)
 
 
+def test_heat_forward_euler_dense_bit_exact(tmp_path):
+    # HeatEq.ini on the DENSE format (N = 12*12*33 = 4752, a 90 MB matrix):
+    # all rows advance, then the faces are rewritten from the snapshot
+    spec, field, ref, steps = run_both(tmp_path, "HeatEq.ini",
+                                       {"Time": 11, "Data": 1, "TotalTime": 0.01, "Save": 5})
+    assert steps >= 5
+    assert field.tobytes() == ref.tobytes()
+    assert field.max() == 1.0
+
+
+def test_continuity_case(tmp_path):
+    # AdvEq.ini turned into the continuity equation (simindex 2): rho from the
+    # Gaussian IC, u = (0, 0, -100), div("rho", "u") stepped by forward Euler
+    spec, field, ref, steps = run_both(
+        tmp_path, "AdvEq.ini", {"simindex": 2, "Scalars": "rho", "Vectors": "u", "TotalTime": 0.01, "Save": 2},
+        {"property": "rho rho rho rho rho rho"})
+    assert steps >= 5
+    err = po.rel_l2(field, ref)
+    print(f"continuity: rel L2 {err:.2e}")
+    assert err <= 1e-5
+    assert np.abs(field - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
 def test_heat_forward_euler_shipped_operator(tmp_path):
     # alpha_row_scaled = 0: the CSR operator without the diffusivity, as the
     # reference's csrscale leaves it

@@ -323,3 +323,73 @@ def test_lu_f64_matches_lapack():
     b = rng.standard_normal((n, 3))
     x = po.lu_solve_f64(lu, piv, b)
     np.testing.assert_allclose(a @ x, b, rtol=1e-10, atol=1e-10)
+
+
+# ------------------------------------------------ DENSE format and divergence
+
+@pytest.mark.parametrize("case", META["dense"], ids=lambda c: c["key"])
+def test_lap_dense_matches_reference(case):
+    # the oracle's DENSE Laplacian against PIRO's own DENSE::stencils,
+    # frozen as non-zero triplets and a digest of the whole N x N matrix
+    nx, ny, nz = case["dims"]
+    a = po.lap_dense(nx, ny, nz, case["scheme"], hexf(case["norm_hex"]))
+    k = case["key"]
+    r, c = np.nonzero(a)
+    np.testing.assert_array_equal(r, REF[k + "_r"])
+    np.testing.assert_array_equal(c, REF[k + "_c"])
+    assert a[r, c].tobytes() == REF[k + "_v"].tobytes()
+    assert sha(a) == case["digest"]
+
+
+@pytest.mark.parametrize("scheme", [po.UW, po.DW, po.CD])
+def test_fe_dense_step_exact_dyadic(scheme):
+    # norm = (l0/n0)^2 = 1/4 makes the entries -24 and 4; integer x and
+    # dt = 1/64 make every product and partial sum exact in FP32, so the
+    # step must equal the FP64 product exactly (an index, sign or order slip
+    # changes the value)
+    nx, ny, nz = 5, 4, 6
+    norm = po.lap_norm(2.0, 4)
+    assert norm == np.float32(0.25)
+    a = po.lap_dense(nx, ny, nz, scheme, norm)
+    rng = np.random.default_rng(scheme)
+    x = rng.integers(-3, 4, nx * ny * nz).astype(np.float32)
+    dt = np.float32(1.0 / 64.0)
+    y = po.fe_dense_step(a, dt, x)
+    ref = x.astype(np.float64) + float(dt) * (a.astype(np.float64) @ x.astype(np.float64))
+    np.testing.assert_array_equal(y.astype(np.float64), ref)
+
+
+def test_fe_dense_step_rounding_bound():
+    # non-dyadic values: the FP32 sequential sum stays within N*eps of FP64
+    nx, ny, nz = 6, 5, 4
+    a = po.lap_dense(nx, ny, nz, po.CD, po.lap_norm(1.0, nx - 2))
+    x = np.random.default_rng(3).standard_normal(nx * ny * nz).astype(np.float32)
+    dt = np.float32(1e-3)
+    y = po.fe_dense_step(a, dt, x)
+    ref = x.astype(np.float64) + float(dt) * (a.astype(np.float64) @ x.astype(np.float64))
+    assert np.max(np.abs(y - ref) / (np.abs(ref) + 1.0)) < 1e-6
+
+
+@pytest.mark.parametrize("scheme", [po.UW, po.DW, po.CD])
+def test_div_csr_linear_field_closed_form(scheme):
+    # f = i + 2 j + 3 k (index units) and u = (ux, uy, uz) constant: on
+    # interior rows every one-sided or central difference of f along d is
+    # a_d/h_norm, so div(f) = (ux*1 + uy*2 + uz*3) * c/norm with c = 2 for
+    # cd and 1 for uw/dw (entries +-1/norm, PIRO's gradient stencils)
+    nx, ny, nz = 6, 7, 5
+    N = nx * ny * nz
+    norm = po.grad_norm(1.0, nx - 2)
+    ux, uy, uz = 2.0, -1.0, 0.5
+    u3 = np.concatenate([np.full(N, ux), np.full(N, uy), np.full(N, uz)]).astype(np.float32)
+    D = po.div_csr(nx, ny, nz, scheme, norm, u3)
+    l = np.arange(N)
+    i, j, k = l % nx, (l // nx) % ny, l // (nx * ny)
+    f = (i + 2 * j + 3 * k).astype(np.float32)
+    g = D.dense().astype(np.float64) @ f.astype(np.float64)
+    inner = (i > 0) & (i < nx - 1) & (j > 0) & (j < ny - 1) & (k > 0) & (k < nz - 1)
+    c = 2.0 if scheme == po.CD else 1.0
+    expect = (ux * 1 + uy * 2 + uz * 3) * c / float(norm)
+    np.testing.assert_allclose(g[inner], expect, rtol=1e-6)
+    # per row: the merged row holds exactly the three gradient rows' entries
+    G = [po.grad_csr(nx, ny, nz, scheme, d, norm) for d in range(3)]
+    assert D.nnz == sum(x.nnz for x in G)

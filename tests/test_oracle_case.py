@@ -153,3 +153,49 @@ def test_adveq_fields():
     # setup.ini's BC property T is not a slot of this case: the face values go
     # to slot 0 and the snapshot is U's initial condition
     assert spec.snapshot.tobytes() == u.device.tobytes()
+
+
+def _continuity_assets(tmp_path):
+    """AdvEq.ini turned into the continuity equation (simindex 2): scalar rho
+    (Gaussian IC), vector u (uniform (0,0,-100)), BCs on rho"""
+    import re
+    import shutil
+    d = tmp_path / "assets"
+    shutil.copytree(CASES, d, copy_function=shutil.copyfile)
+    p = d / "AdvEq.ini"
+    t = p.read_text()
+    for k, v in (("simindex", "2"), ("Scalars", "rho"), ("Vectors", "u"), ("TotalTime", "0.003")):
+        t, n = re.subn(rf"(?m)^{k}\s*=.*$", f"{k} = {v}", t)
+        assert n == 1, k
+    p.write_text(t)
+    s = d / "setup.ini"
+    t, n = re.subn(r"(?m)^property\s*=.*$", "property = rho rho rho rho rho rho", s.read_text())
+    assert n == 1
+    s.write_text(t)
+    return str(d)
+
+
+def test_continuity_operator_equals_advection_with_v_u(tmp_path):
+    # process::div("rho", "u") scales each divergence matrix by u's component
+    # (SpMSCSR) and merges without factors (csrgeam_2); the advection path
+    # merges the same gradient stencils with factors v (csrgeam_3).  With the
+    # same norm and v = u the two routes must give the same matrix bit for bit
+    spec = pc.setup(_continuity_assets(tmp_path), "AdvEq.ini")
+    assert spec.simindex == 2
+    kind, D = pc._operator(spec, True)
+    n = spec.n
+    g = po.grad_norm(spec.l[0], n[0] - 2)
+    G = [po.grad_csr(*n, spec.space_scheme, d, g) for d in range(3)]
+    A = po.geam3(G[0], G[1], G[2], spec.slots[spec.match("u")].device)
+    np.testing.assert_array_equal(D.row_ptr, A.row_ptr)
+    np.testing.assert_array_equal(D.col, A.col)
+    assert D.val.tobytes() == A.val.tobytes()
+    # u = (0, 0, -100): only the z stencil survives with non-zero values
+    assert np.count_nonzero(D.val) == np.count_nonzero(G[2].val)
+
+
+def test_continuity_case_runs(tmp_path):
+    spec = pc.setup(_continuity_assets(tmp_path), "AdvEq.ini")
+    steps, x, exports = pc.run(spec)
+    assert steps == 3 and np.all(np.isfinite(x))
+    assert len(exports) == 3
