@@ -312,7 +312,7 @@ def main():
     ap.add_argument("what", choices=["lu", "batch", "advection", "surrogate", "surrogate_fwd", "jli"])
     ap.add_argument("--side", type=int, default=256)
     ap.add_argument("--steps", type=int, default=0)
-    ap.add_argument("--traj", type=int, default=8000)
+    ap.add_argument("--traj", type=int, default=32000)
     ap.add_argument("--train-side", type=int, default=32)
     ap.add_argument("--passes", type=int, default=3)
     args = ap.parse_args()

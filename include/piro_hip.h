@@ -320,18 +320,22 @@ int piro_csr_to_dense(piro_ctx *ctx, int n_rows, int n_cols,
                       const int32_t *d_row_ptr, const int32_t *d_col,
                       const float *d_val, float *d_dense, int ld);
 /* P A = L U in FP32 with partial pivoting (pivot = first row of maximal
- * magnitude, magnitudes compared in FP64), in place on the row-major n x n
+ * magnitude, magnitudes compared exactly), in place on the row-major n x n
  * matrix d_a; d_ipiv[k] (0-based) is the row exchanged with row k at step k
  * (LAPACK getrf convention).  The trailing updates run on MFMA
  * (v_mfma_f32_16x16x4f32).  Replaces the reference's hash-table elimination
  * matrix_operations::HT::lu_decomposition (src/matrixoperations.cpp:10-104,
  * lu_decompHT.cl:22-62), which factors without pivoting and never solves.
- * *info = 0, or k+1 for an exactly zero pivot at step k (PIRO_ESINGULAR). */
+ * *info = 0, or k+1 for an exactly zero pivot at step k (PIRO_ESINGULAR).
+ * Requires n < 2^27 (row labels are packed into 27 bits of the pivot keys). */
 int piro_lu_factor_f32(piro_ctx *ctx, int n, float *d_a, int lda,
                        int32_t *d_ipiv, int *info);
 /* Solve with the FP32 factors, in place on d_b (row-major n x nrhs).
  * Takes the role of the never-enqueued forward/backward_substitution_csr
- * kernels (forward_substitution_csr.cl, backward_substitution_csr.cl). */
+ * kernels (forward_substitution_csr.cl, backward_substitution_csr.cl).
+ * Synchronises: the pivots are read to the host once to form the row
+ * permutation; PIRO_EINTERNAL if a block wait of the solve exceeded its
+ * bound. */
 int piro_lu_solve_f32(piro_ctx *ctx, int n, const float *d_lu, int lda,
                       const int32_t *d_ipiv, int nrhs, float *d_b, int ldb);
 /* Mixed-precision solve: x0 from the FP32 factors, then repeat

@@ -421,8 +421,8 @@ __global__ __launch_bounds__(256) void k_conv32(Shape s, const bf16 *__restrict_
 // 16-column output tile.  Columns 0-2 carry BF16(w6) and columns 3-5 the
 // BF16 rounding of the remainder w6 - BF16(w6), so o_j = D[j] + D[j + 3]
 // sees the weights to about 16 significant bits.  The epilogue combines
-// the three outputs of a voxel (lanes li = 0..5 of one row group) through
-// lane shuffles: delta = o0 + a (o1 + a o2); training writes delta,
+// the three outputs of a voxel (lanes li = 0..5 of one row group) by DPP row
+// shifts: delta = o0 + a (o1 + a o2); training writes delta,
 // inference out = interior ? u + s*delta : 0.
 __device__ __forceinline__ bf16x8 l6_fragment(const float *__restrict__ w6, int tp, int li, int kg)
 {
@@ -464,6 +464,20 @@ __global__ __launch_bounds__(256) void k_l6(Shape s, const bf16 *__restrict__ x5
         const int64_t nxt = br + gridDim.x;
         if (nxt < nbricks) fetch_halo(h, x5, s, brick_of(nxt, s), t);
         const bf16 *hb = sx[buf] + (vy * HX + vx) * CH + kg * 8;
+        // the epilogue's per-voxel inputs, loaded now so that they arrive
+        // during the products: voxel (z, i = 4 kg + r) of this wave
+        float a_in[BZ][4], u_in[BZ][4];
+#pragma unroll
+        for (int z = 0; z < BZ; ++z)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 4 * kg + r;
+                const int zz = b.z0 + z, yy = b.y0 + 2 * wv + (i >> 3), xx = b.x0 + (i & 7);
+                const bool in = zz < s.D && yy < s.H && xx < s.W;
+                const int64_t v = in ? vidx(b.n, zz, yy, xx, s) : 0;
+                a_in[z][r] = in ? alpha_factor(al, v) : 0.0f;
+                u_in[z][r] = (INFER && in) ? u[v] : 0.0f;
+            }
         f32x4 acc[BZ];
 #pragma unroll
         for (int z = 0; z < BZ; ++z) acc[z] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
@@ -487,20 +501,25 @@ __global__ __launch_bounds__(256) void k_l6(Shape s, const bf16 *__restrict__ x5
         for (int z = 0; z < BZ; ++z)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
+                // column j of the row arrives at lane li = 0 by DPP row_shl:j
                 const float d = acc[z][r];
                 float o[2 * NO];
-#pragma unroll
-                for (int j = 0; j < 2 * NO; ++j) o[j] = __shfl(d, (lane & ~15) + j, 64);
+                o[0] = d;
+                o[1] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(d), 0x101, 0xF, 0xF, true));
+                o[2] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(d), 0x102, 0xF, 0xF, true));
+                o[3] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(d), 0x103, 0xF, 0xF, true));
+                o[4] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(d), 0x104, 0xF, 0xF, true));
+                o[5] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(d), 0x105, 0xF, 0xF, true));
                 const int i = 4 * kg + r;
                 const int zz = b.z0 + z, yy = b.y0 + 2 * wv + (i >> 3), xx = b.x0 + (i & 7);
                 if (li != 0 || zz >= s.D || yy >= s.H || xx >= s.W) continue;
                 const int64_t v = vidx(b.n, zz, yy, xx, s);
                 const float o0 = o[0] + o[3], o1 = o[1] + o[4], o2 = o[2] + o[5];
-                const float a = alpha_factor(al, v);
+                const float a = a_in[z][r];
                 const float delta = o0 + a * (o1 + a * o2);
                 if (INFER) {
                     const bool interior = zz > 0 && zz < s.D - 1 && yy > 0 && yy < s.H - 1 && xx > 0 && xx < s.W - 1;
-                    out[v] = interior ? u[v] + scale * delta : 0.0f;
+                    out[v] = interior ? u_in[z][r] + scale * delta : 0.0f;
                 } else {
                     out[v] = delta;
                 }
