@@ -393,17 +393,17 @@ int piro_batch_run(piro_batch *b, float *d_x, float *d_work, int nsteps,
 /* No reference counterpart (PIRO has no learned part).  A 6-layer 3x3x3
  * residual CNN advancing a heat field k forward-Euler steps at a time.
  * Linear layers without biases (the k-step heat map is linear in u): L1
- * 1->32 on u/scale (FP32, VALU; its centre tap is minus the sum of the other
- * 26, so a constant field maps to 0), L2-L5 32->32 on BF16 MFMA with FP32
- * accumulation, L6 32->3 (VALU) combined per cell with
- * a = (alpha-0.0125)/0.0075 as delta = o0 + a (o1 + a o2); output
- * u + scale*delta on interior cells, 0 on boundary cells.  Fields are
- * [n][D][H][W] FP32 (n instances of one shape, zero padding at instance
- * boundaries).  Parameters: flat FP32 vector of piro_surrogate_num_params()
- * values: W1[27][32] (the stored centre tap 13 is unused), then for L2..L5
- * W[27][32 co][32 ci], then W6[27][3][32]; tap index
- * (dz+1)*9 + (dy+1)*3 + (dx+1).  The output layer is zero-initialised, so an
- * untrained surrogate is the identity. */
+ * 1->32 on u/scale (FP32; its centre tap is minus the sum of the other 26,
+ * so a constant field maps to 0), L2-L5 32->32 (BF16 weights), L6 32->3,
+ * combined per cell with a = (alpha-0.0125)/0.0075 as
+ * delta = o0 + a (o1 + a o2); output u + scale*delta on interior cells, 0 on
+ * boundary cells.  The network is defined on u extended by zeros beyond the
+ * instance (no layer's output is cut off at the instance boundary).  Fields
+ * are [n][D][H][W] FP32 (n instances of one shape).  Parameters: flat FP32
+ * vector of piro_surrogate_num_params() values: W1[27][32] (the stored
+ * centre tap 13 is unused), then for L2..L5 W[27][32 co][32 ci], then
+ * W6[27][3][32]; tap index (dz+1)*9 + (dy+1)*3 + (dx+1).  The output layer
+ * is zero-initialised, so an untrained surrogate is the identity. */
 typedef struct piro_surrogate piro_surrogate;
 int piro_surrogate_num_params(void);
 int piro_surrogate_create(piro_ctx *ctx, uint64_t seed, piro_surrogate **out);
@@ -411,12 +411,22 @@ int piro_surrogate_create(piro_ctx *ctx, uint64_t seed, piro_surrogate **out);
 int piro_surrogate_destroy(piro_surrogate *s);
 int piro_surrogate_get_params(piro_surrogate *s, float *h_params);
 int piro_surrogate_set_params(piro_surrogate *s, const float *h_params);
+/* Inference: the layers composed into three 13^3 stencils (rebuilt after
+ * the weights change), applied with BF16 MFMA to u split into two BF16
+ * parts, FP32 accumulation. */
 int piro_surrogate_forward(piro_surrogate *s, int n, int D, int H, int W,
                            const float *d_u, const float *d_alpha, float scale,
                            float *d_u_out);
-/* Forward + backward of the loss mean_interior((delta - target)^2) where
+/* The composed stencils in FP64: h_k[((dz*13 + dy)*13 + dx)*3 + c] is the
+ * weight of u(z + dz - 6, y + dy - 6, x + dx - 6) in output o_c. */
+int piro_surrogate_stencil(piro_surrogate *s, double *h_k);
+/* Forward + backward of the loss mean((delta - target)^2) over the cells at
+ * least 5 cells from every face (each side must be >= 11), where
  * target = (u_after_k_steps - u)/scale is supplied in d_target; FP32
- * gradients to h_grads (may be NULL). */
+ * gradients to h_grads (may be NULL).  There the layered network with zero
+ * padding at the instance boundary equals the zero-extended network that
+ * inference evaluates, and a Dirichlet face is too far to shape 16 heat
+ * steps. */
 int piro_surrogate_gradients(piro_surrogate *s, int n, int D, int H, int W,
                              const float *d_u, const float *d_alpha,
                              const float *d_target, float scale,

@@ -108,6 +108,7 @@ _SIGS = {
     "piro_surrogate_get_params": [vp, P_f32],
     "piro_surrogate_set_params": [vp, P_f32],
     "piro_surrogate_forward": [vp, i32, i32, i32, i32, vp, vp, f32, vp],
+    "piro_surrogate_stencil": [vp, P_f64],
     "piro_surrogate_gradients": [vp, i32, i32, i32, i32, vp, vp, vp, f32, P_f32, P_f64],
     "piro_surrogate_train_step": [vp, i32, i32, i32, i32, vp, vp, vp, f32, f32, P_f64],
 }

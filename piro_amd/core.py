@@ -541,6 +541,12 @@ class Surrogate:
         check(lib.piro_surrogate_forward(self.h, n, D, H, W, _p(u), _p(alpha), float(scale), _p(out)))
         return out
 
+    def stencil(self):
+        """the composed 13^3 stencils, [dz][dy][dx][c] (offsets -6..6), FP64"""
+        k = np.empty((13, 13, 13, 3), np.float64)
+        check(lib.piro_surrogate_stencil(self.h, k.ctypes.data_as(C.POINTER(C.c_double))))
+        return k
+
     def gradients(self, shape, u, alpha, target, scale):
         n, D, H, W = shape
         g = np.empty(self.n_params, np.float32)

@@ -47,9 +47,9 @@ def smooth_alpha(dims, rng, n_gauss=8, width_cells=(13.0, 38.0)):
     return (ALPHA_LO + (ALPHA_HI - ALPHA_LO) * f).astype(np.float32).ravel()
 
 
-def blob_ic(dims, rng, n_blobs=None, width_cells=(3.0, 14.0)):
-    """1-4 Gaussian blobs (widths in cells), peak normalised to 1, zero on the
-    boundary cells"""
+def blob_ic(dims, rng, n_blobs=None, width_cells=(3.0, 14.0), noise=0.0):
+    """1-4 Gaussian blobs (widths in cells), peak normalised to 1, plus white
+    noise of standard deviation `noise`, zero on the boundary cells"""
     nx, ny, nz = dims
     n_blobs = int(rng.integers(1, 5)) if n_blobs is None else n_blobs
     z, y, x = _grid(dims)
@@ -59,6 +59,8 @@ def blob_ic(dims, rng, n_blobs=None, width_cells=(3.0, 14.0)):
         w = rng.uniform(*width_cells)
         f += rng.uniform(0.3, 1.0) * np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / (2 * w * w))
     f /= max(np.abs(f).max(), 1e-12)
+    if noise > 0:
+        f += noise * rng.standard_normal(f.shape)
     f[0, :, :] = f[-1, :, :] = 0
     f[:, 0, :] = f[:, -1, :] = 0
     f[:, :, 0] = f[:, :, -1] = 0
@@ -79,10 +81,14 @@ class TrainLog:
 
 
 def train(ctx: Context, sur: Surrogate, n_traj=2000, per_batch=50, side=24, segments=4, k=16, passes=2,
-          lr=2e-3, lr_final=2e-4, seed=1234, log_every=0, blob_widths=(3.0, 14.0)) -> TrainLog:
+          lr=2e-3, lr_final=2e-4, seed=1234, log_every=0, blob_widths=(3.0, 14.0), noise=0.01) -> TrainLog:
     """Generate n_traj trajectories with the batched stepper and train on
     every (u_t, u_{t+k}) pair as it is produced; blob_widths (in cells) are
-    the initial blobs' Gaussian widths."""
+    the initial blobs' Gaussian widths.  The initial fields carry white
+    noise (standard deviation `noise`): smooth fields alone leave the
+    network's response to fine-scale content unconstrained, and a response
+    above 1 there grows rounding noise from call to call, where the stepper
+    damps it."""
     rng = np.random.default_rng(seed)
     dims = (side, side, side)
     log = TrainLog()
@@ -94,7 +100,7 @@ def train(ctx: Context, sur: Surrogate, n_traj=2000, per_batch=50, side=24, segm
     shape = (per_batch, side, side, side)
     for _ in range(n_batches):
         alpha = np.concatenate([smooth_alpha(dims, rng) for _ in range(per_batch)])
-        u0 = np.concatenate([blob_ic(dims, rng, width_cells=blob_widths) for _ in range(per_batch)])
+        u0 = np.concatenate([blob_ic(dims, rng, width_cells=blob_widths, noise=noise) for _ in range(per_batch)])
         da = ctx.from_host(alpha)
         b = Batch(ctx, [dims] * per_batch, da, [dt] * per_batch)
         x = ctx.from_host(u0)
@@ -118,6 +124,19 @@ def train(ctx: Context, sur: Surrogate, n_traj=2000, per_batch=50, side=24, segm
     ctx.sync()
     log.seconds = time.perf_counter() - t0
     return log
+
+
+def max_amplification(stencil, n=64, a_values=(-1.0, -0.5, 0.0, 0.5, 1.0)):
+    """Largest |1 + K0^(w) + a (K1^(w) + a K2^(w))| over the Fourier modes w
+    of an n^3 periodic grid, for constant normalised diffusivity a: above 1,
+    the surrogate grows that mode from call to call.  stencil[dz][dy][dx][c]
+    with offsets -6..6 (Surrogate.stencil())."""
+    big = np.zeros((n, n, n, 3))
+    r = np.arange(-6, 7) % n
+    big[np.ix_(r, r, r)] = stencil
+    # o(x) = sum_r K[r] u(x + r): symbol sum_r K[r] e^{+i w r}
+    sym = np.conj(np.fft.fftn(big, axes=(0, 1, 2)))
+    return max(float(np.abs(1.0 + sym[..., 0] + a * (sym[..., 1] + a * sym[..., 2])).max()) for a in a_values)
 
 
 @dataclass
@@ -166,9 +185,9 @@ class HeatRun:
         ctx.sync()
         t0 = time.perf_counter()
         for c in range(calls):
-            scale = ctx.reduce("absmax", x)
-            scale = scale if scale > 0 else 1.0
-            sur.forward(shape, x, self.alpha, scale, out=y)
+            # the network is linear and bias-free and BF16 has FP32's exponent
+            # range, so no input scaling is needed (and no host round trip)
+            sur.forward(shape, x, self.alpha, 1.0, out=y)
             res.calls += 1
             if (c + 1) % check_every == 0:
                 ref.copy_(x)

@@ -1,7 +1,15 @@
 """GPU tests of the learned surrogate: forward and gradients of the project's
 own BF16-MFMA kernels against an FP64 CPU oracle of the same network (the
 oracle's conv3d is pinned by brute force and finite differences in
-tests/test_oracle_surrogate.py), plus a training run that must cut the loss."""
+tests/test_oracle_surrogate.py), plus a training run that must cut the loss.
+
+The network is defined on u extended by zeros beyond the instance.  The
+oracle evaluates it layer by layer on the field embedded in a zero border of
+PAD cells (enough for every interior cell: layer l is needed at most 5 - l
+cells beyond the instance and the border keeps layers 1 and 2 zero there);
+inference on the device applies the composed 13^3 stencils instead.
+Training runs the layers with zero padding at the instance boundary (PAD 0)
+and takes the loss on cells at least WALL cells from every face."""
 import numpy as np
 import pytest
 
@@ -47,27 +55,54 @@ def alpha_factor(al):
     return ((al.astype(np.float32) - np.float32(0.0125)) * (np.float32(1.0) / np.float32(0.0075))).astype(np.float64)
 
 
-def ref_forward(p, u, al, scale):
-    """FP64 network with the device's BF16 rounding points; returns inputs,
-    activations and delta (shape [n,D,H,W])"""
+PAD = 2
+WALL = 5
+
+
+def embed(f, pad, fill):
+    n, D, H, W = f.shape
+    g = np.full((n, D + 2 * pad, H + 2 * pad, W + 2 * pad), fill, f.dtype)
+    g[:, pad:pad + D, pad:pad + H, pad:pad + W] = f
+    return g
+
+
+def crop(f, pad):
+    return f[:, pad:f.shape[1] - pad, pad:f.shape[2] - pad, pad:f.shape[3] - pad]
+
+
+def device_weights(p):
+    """the weights the device applies: L1 with its implied centre, BF16 L2-L5,
+    L6 as BF16(w) plus the BF16 rounding of the remainder"""
     w1, mids, w6 = unpack(p)
-    in0 = (u.astype(np.float32) * np.float32(1.0 / scale)).astype(np.float64)
-    x_in = in0[..., None]
-    acts = []
-    x = bf16(po.conv3d_fwd_f64(x_in, effective_w1(w1)[:, :, None]).astype(np.float32)).astype(np.float64)
-    acts.append(x)
-    for w in mids:
-        z = po.conv3d_fwd_f64(x, bf16(w.astype(np.float32)).astype(np.float64))
-        x = bf16(z.astype(np.float32)).astype(np.float64)
-        acts.append(x)
-    # the device's L6 weights: BF16(w) plus the BF16 rounding of the remainder
     w6f = np.asarray(w6, np.float32)
     hi = bf16(w6f)
     lo = bf16((w6f - hi).astype(np.float32))
-    o = po.conv3d_fwd_f64(x, hi.astype(np.float64) + lo.astype(np.float64))
-    a = alpha_factor(al)
-    delta = o[..., 0] + a * (o[..., 1] + a * o[..., 2])
-    return x_in, acts, delta
+    return (effective_w1(w1), [bf16(w.astype(np.float32)).astype(np.float64) for w in mids],
+            hi.astype(np.float64) + lo.astype(np.float64))
+
+
+def ref_forward(p, u, al, scale, pad=PAD, round_acts=True):
+    """FP64 network on the field embedded in a zero border of `pad` cells,
+    with the device's BF16 rounding points for the layered training pass
+    (round_acts) or without them (the composed stencils round no
+    activation).  Returns inputs, activations (embedded shape), delta on the
+    embedded shape and delta on the instance."""
+    w1e, mids, w6e = device_weights(p)
+    rnd = (lambda z: bf16(z.astype(np.float32)).astype(np.float64)) if round_acts else (lambda z: z)
+    up = embed(np.asarray(u, np.float32), pad, 0.0)
+    alp = embed(np.asarray(al, np.float32), pad, 0.0125)
+    in0 = (up * np.float32(1.0 / scale)).astype(np.float64)
+    x_in = in0[..., None]
+    acts = []
+    x = rnd(po.conv3d_fwd_f64(x_in, w1e[:, :, None]))
+    acts.append(x)
+    for w in mids:
+        x = rnd(po.conv3d_fwd_f64(x, w))
+        acts.append(x)
+    o = po.conv3d_fwd_f64(x, w6e)
+    a = alpha_factor(alp)
+    delta_e = o[..., 0] + a * (o[..., 1] + a * o[..., 2])
+    return x_in, acts, delta_e, crop(delta_e, pad)
 
 
 def interior_mask(shape):
@@ -77,14 +112,24 @@ def interior_mask(shape):
     return m
 
 
-def ref_grads(p, u, al, target, scale):
+def loss_mask(shape):
+    n, D, H, W = shape
+    m = np.zeros(shape, bool)
+    m[:, WALL:D - WALL, WALL:H - WALL, WALL:W - WALL] = True
+    return m
+
+
+def ref_grads(p, u, al, target, scale, pad=0):
+    """loss and gradients of the layered training pass (zero padding at the
+    instance boundary), loss on the cells WALL or more from every face"""
     w1, mids, w6 = unpack(p)
-    x_in, acts, delta = ref_forward(p, u, al, scale)
-    m = interior_mask(u.shape)
+    x_in, acts, delta_e, delta = ref_forward(p, u, al, scale, pad)
+    m = embed(loss_mask(u.shape), pad, False)
+    te = embed(np.asarray(target, np.float64), pad, 0.0)
     n_int = m.sum()
-    dd = np.where(m, 2.0 * (delta - target) / n_int, 0.0)
-    loss = float(np.sum(np.where(m, (delta - target) ** 2, 0.0)) / n_int)
-    a = alpha_factor(al)
+    dd = np.where(m, 2.0 * (delta_e - te) / n_int, 0.0)
+    loss = float(np.sum(np.where(m, (delta_e - te) ** 2, 0.0)) / n_int)
+    a = alpha_factor(embed(np.asarray(al, np.float32), pad, 0.0125))
     g = {}
     # the device stores each upstream data gradient in BF16 before the next
     # layer's data and weight gradients use it
@@ -166,7 +211,53 @@ def fields(shape, seed):
     return u, al
 
 
-@pytest.mark.parametrize("shape", [(1, 8, 8, 8), (2, 5, 9, 10), (1, 12, 16, 20)])
+def test_training_region_sees_the_zero_extended_network():
+    # on the loss cells (WALL or more from every face) the layered pass with
+    # zero padding at the instance boundary equals the zero-extended network,
+    # for any field; one cell nearer it does not
+    rng = np.random.default_rng(41)
+    p = rng.uniform(-0.2, 0.2, sum(b - a for a, b in param_blocks().values())).astype(np.float32)
+    shape = (1, 13, 12, 14)
+    u, al = fields(shape, 42)
+    _, _, _, masked = ref_forward(p, u, al, 1.0, pad=0, round_acts=False)
+    _, _, _, free = ref_forward(p, u, al, 1.0, pad=PAD, round_acts=False)
+    m = loss_mask(shape)
+    scale = np.abs(free).max()
+    assert np.abs(masked - free)[m].max() <= 1e-12 * scale
+    ring = np.zeros(shape, bool)
+    ring[:, WALL - 1:-WALL + 1, WALL - 1:-WALL + 1, WALL - 1:-WALL + 1] = True
+    assert np.abs(masked - free)[ring & ~m].max() > 1e-6 * scale
+
+
+def test_stencil_is_the_networks_impulse_response(ctx):
+    # the device composes the layers' taps; the oracle pushes a unit impulse
+    # through the layered FP64 network (25^3 box: no activation reaches the
+    # box edge, so zero padding never acts).  o_c(x) = sum_r K_c[r] u(x + r)
+    # gives K_c[r] = o_c(p0 - r) for an impulse at p0.
+    from piro_amd.core import Surrogate
+    sur = Surrogate(ctx, seed=31)
+    p = random_params(sur, 32)
+    sur.set_params(p)
+    k = sur.stencil()
+    sur.close()
+    w1e, mids, w6e = device_weights(p)
+    u = np.zeros((1, 25, 25, 25, 1))
+    u[0, 12, 12, 12, 0] = 1.0
+    x = po.conv3d_fwd_f64(u, w1e[:, :, None])
+    for w in mids:
+        x = po.conv3d_fwd_f64(x, w)
+    o = po.conv3d_fwd_f64(x, w6e)[0]
+    ref = o[18:5:-1, 18:5:-1, 18:5:-1, :]          # index 12 - r for r = -6..6
+    assert np.abs(ref).max() > 0
+    box = np.zeros(o.shape[:3], bool)
+    box[6:19, 6:19, 6:19] = True
+    assert np.all(o[~box] == 0.0)                    # radius 6 exactly
+    err = np.abs(k - ref).max() / np.abs(ref).max()
+    print(f"composed stencil vs impulse response: max rel {err:.2e}")
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(1, 8, 8, 8), (2, 5, 9, 10), (1, 12, 16, 20), (1, 20, 33, 17)])
 def test_forward_matches_oracle(ctx, shape):
     print(f"case surrogate fwd {shape}", flush=True)
     from piro_amd.core import Surrogate
@@ -179,13 +270,15 @@ def test_forward_matches_oracle(ctx, shape):
     da = ctx.from_host(al.ravel())
     out = sur.forward(shape, du, da, scale)
     got = out.host().reshape(shape)
-    _, _, delta = ref_forward(p, u, al, scale)
+    # the stencils carry the weights to about 17 bits and see u as two BF16
+    # parts: about 2^-16 relative per product, FP32 accumulation
+    _, _, _, delta = ref_forward(p, u, al, scale, round_acts=False)
     m = interior_mask(shape)
     assert np.all(got[~m] == 0)
     dg = (got[m].astype(np.float64) - u[m]) / scale
     err = np.linalg.norm(dg - delta[m]) / np.linalg.norm(delta[m])
     print(f"surrogate forward {shape}: rel L2 of delta vs FP64 oracle {err:.2e}")
-    assert err < 5e-3
+    assert err < 1e-4
     sur.close()
 
 
@@ -202,7 +295,7 @@ def test_untrained_surrogate_is_identity_on_interior(ctx):
     sur.close()
 
 
-@pytest.mark.parametrize("shape", [(1, 8, 8, 8), (2, 6, 9, 11)])
+@pytest.mark.parametrize("shape", [(1, 12, 12, 12), (2, 11, 13, 14)])
 def test_gradients_match_oracle(ctx, shape):
     print(f"case surrogate grad {shape}", flush=True)
     from piro_amd.core import Surrogate
@@ -238,22 +331,24 @@ def test_gradients_match_oracle(ctx, shape):
 
 def test_gradients_exact_on_dyadic_network(ctx):
     # integer inputs, alpha = 0.0125 (diffusivity factor 0), scale 1, 4^3
-    # interior cells (the loss scale 2/64 is a power of two) and targets
-    # within +-1/8 of the output: every activation is a small integer, every
+    # loss cells (the loss scale 2/64 is a power of two) and targets within
+    # +-1/8 of the output: every activation is a small integer, every
     # upstream gradient a small multiple of 2^-11 -- all exact in BF16 -- and
-    # every sum exact in FP32.  Device and FP64 reference must agree exactly.
+    # every sum exact in FP32.  Device and FP64 reference must agree exactly,
+    # the layered training pass and the composed-stencil forward alike.
     from piro_amd.core import Surrogate
-    shape = (1, 6, 6, 6)
+    shape = (1, 14, 14, 14)
     for seed in range(3):
         print(f"case surrogate exact seed={seed}", flush=True)
         rng = np.random.default_rng(100 + seed)
         p = dyadic_network(rng)
         u = rng.integers(0, 3, shape).astype(np.float32)
         al = np.full(shape, 0.0125, np.float32)
-        _, acts, delta = ref_forward(p, u, al, 1.0)
+        _, acts, _, delta = ref_forward(p, u, al, 1.0)
         assert max(float(np.abs(a).max()) for a in acts) < 256  # integers exact in BF16
-        target = (delta + rng.choice([-0.125, 0.0, 0.125], shape)).astype(np.float32)
-        assert np.array_equal(target.astype(np.float64) - delta, np.round((target - delta) * 8) / 8)
+        _, _, _, delta0 = ref_forward(p, u, al, 1.0, pad=0)
+        target = (delta0 + rng.choice([-0.125, 0.0, 0.125], shape)).astype(np.float32)
+        assert np.array_equal(target.astype(np.float64) - delta0, np.round((target - delta0) * 8) / 8)
         sur = Surrogate(ctx, seed=1)
         sur.set_params(p)
         out = sur.forward(shape, ctx.from_host(u.ravel()), ctx.from_host(al.ravel()), 1.0).host().reshape(shape)
@@ -274,7 +369,7 @@ def test_gradients_exact_on_dyadic_network(ctx):
 def test_training_reduces_loss(ctx):
     from piro_amd.core import Surrogate
     sur = Surrogate(ctx, seed=21)
-    shape = (4, 12, 12, 12)
+    shape = (4, 16, 16, 16)
     n, D, H, W = shape
     rng = np.random.default_rng(22)
     # targets: 4 FE steps of the 7-point operator per instance (r = 0.1)
