@@ -232,7 +232,8 @@ def bench_surrogate(args):
     ctx = Context(0)
     sur = Surrogate(ctx, seed=5)
     log = learned.train(ctx, sur, n_traj=args.traj, per_batch=50, side=args.train_side, segments=4, k=16,
-                        passes=args.passes, lr=1e-3, lr_final=1e-5, blob_widths=(4.0, 20.0), log_every=200)
+                        passes=args.passes, lr=1e-3, lr_final=1e-5, blob_widths=(4.0, 20.0), noise=0.03,
+                        log_every=200)
     run = learned.HeatRun(ctx, side=128)
     steps = 1600
     run.plain(160)  # warm-up
@@ -240,6 +241,7 @@ def bench_surrogate(args):
     run.accelerated(sur, 160)  # warm-up
     acc, res = run.accelerated(sur, steps)
     err = ctx.rel_l2(acc, ref)
+    amp = learned.max_amplification(sur.stencil())
     shape = (1, 128, 128, 128)
     y = ctx.empty_like(run.u0)
     t_call = timed(ctx, lambda: sur.forward(shape, run.u0, run.alpha, 1.0, out=y), reps=10)
@@ -249,7 +251,7 @@ def bench_surrogate(args):
                       "train_loss_first": log.losses[0], "train_loss_last": log.losses[-1],
                       "plain_seconds": t_plain, "accelerated_seconds": res.seconds,
                       "speedup": t_plain / res.seconds, "rel_l2_vs_plain": err,
-                      "checks": res.checks, "rejected": res.rejected,
+                      "checks": res.checks, "rejected": res.rejected, "max_amplification": amp,
                       "check_errors": [float(f"{e:.3e}") for e in res.check_errors],
                       "surrogate_call_ms": t_call * 1e3,
                       "surrogate_useful_tflops": SUR_USEFUL_FLOP * cells / t_call / 1e12,

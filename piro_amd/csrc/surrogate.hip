@@ -834,17 +834,32 @@ __global__ __launch_bounds__(256) void k_stencil13(Shape s, const float *__restr
 #pragma unroll
     for (int f = 0; f < NFRAG; ++f) bw[f] = fb[(p0 * NFRAG + f) * 64];
 
-    // stage: item = (plane, row, 8-wide x chunk)
-    for (int e = t; e < SIN * SIN * 4; e += 256) {
+    // stage: item = (plane, row, 8-wide x chunk); every load of the
+    // thread's items is issued before the first conversion
+    constexpr int NITEM = SIN * SIN * 4, IPT = (NITEM + 255) / 256;
+    float sv[IPT][8];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int e = t + 256 * k;
         const int c4 = e & 3, py = (e >> 2) % SIN, pz = (e >> 2) / SIN;
         const int zz = z0 - KR + pz, yy = y0 - KR + py, xb = x0 - KR + 8 * c4;
-        const bool rowin = zz >= 0 && zz < s.D && yy >= 0 && yy < s.H;
+        const bool rowin = e < NITEM && zz >= 0 && zz < s.D && yy >= 0 && yy < s.H;
         const float *row = u + vidx(b, rowin ? zz : 0, rowin ? yy : 0, 0, s);
-        bf16x8 hi, lo;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int xx = xb + i;
-            const float v = (rowin && xx >= 0 && xx < s.W) ? row[xx] * inv_scale : 0.0f;
+            sv[k][i] = (rowin && xx >= 0 && xx < s.W) ? row[xx] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int e = t + 256 * k;
+        if (e >= NITEM) continue;
+        const int c4 = e & 3, py = (e >> 2) % SIN, pz = (e >> 2) / SIN;
+        bf16x8 hi, lo;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float v = sv[k][i] * inv_scale;
             const bf16 h = (bf16)v;
             hi[i] = h;
             lo[i] = (bf16)(v - (float)h);
@@ -852,6 +867,21 @@ __global__ __launch_bounds__(256) void k_stencil13(Shape s, const float *__restr
         *(bf16x8 *)(su + (pz * SIN + py) * SXP + 8 * c4) = hi;
         *(bf16x8 *)(su + STAGE + (pz * SIN + py) * SXP + 8 * c4) = lo;
     }
+    // the epilogue's u and diffusivity factor of this lane's eight output
+    // voxels: slices 8 h + 2 wv + k2, rows y0 + 4 kg + r, column x0 + li
+    float eu[2][2][4], ea[2][2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int k2 = 0; k2 < 2; ++k2)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int zz = z0 + 8 * h + 2 * wv + k2, yy = y0 + 4 * kg + r, xx = x0 + li;
+                const bool in = zz < s.D && yy < s.H && xx < s.W;
+                const int64_t v = in ? vidx(b, zz, yy, xx, s) : 0;
+                eu[h][k2][r] = in ? u[v] : 0.0f;
+                ea[h][k2][r] = in ? alpha_factor(al, v) : 0.0f;
+            }
     __syncthreads();
 
     f32x4 acc[SB][NO];
@@ -931,9 +961,9 @@ __global__ __launch_bounds__(256) void k_stencil13(Shape s, const float *__restr
                 if (zz >= s.D || yy >= s.H || xx >= s.W) continue;
                 const int64_t v = vidx(b, zz, yy, xx, s);
                 const bool interior = zz > 0 && zz < s.D - 1 && yy > 0 && yy < s.H - 1 && xx > 0 && xx < s.W - 1;
-                const float a = alpha_factor(al, v);
+                const float a = ea[h][k2][r];
                 const float delta = o[0][r] + a * (o[1][r] + a * o[2][r]);
-                out[v] = interior ? u[v] + scale * delta : 0.0f;
+                out[v] = interior ? eu[h][k2][r] + scale * delta : 0.0f;
             }
         }
         __syncthreads();

@@ -81,7 +81,7 @@ class TrainLog:
 
 
 def train(ctx: Context, sur: Surrogate, n_traj=2000, per_batch=50, side=24, segments=4, k=16, passes=2,
-          lr=2e-3, lr_final=2e-4, seed=1234, log_every=0, blob_widths=(3.0, 14.0), noise=0.01) -> TrainLog:
+          lr=2e-3, lr_final=2e-4, seed=1234, log_every=0, blob_widths=(3.0, 14.0), noise=0.03) -> TrainLog:
     """Generate n_traj trajectories with the batched stepper and train on
     every (u_t, u_{t+k}) pair as it is produced; blob_widths (in cells) are
     the initial blobs' Gaussian widths.  The initial fields carry white
