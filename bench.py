@@ -83,12 +83,51 @@ def cpu_baseline(budget_s: float = 15.0):
     }
 
 
+def surrogate_leg(ctx):
+    """The metric's second half: the learned k = 16 propagator, trained on
+    trajectories of the batched stepper by the library's own kernels, then a
+    1600-step 128^3 run with every 10th call checked against 16 plain CSR
+    steps (rejected above 1e-3 relative L2), timed against the plain
+    stepper on the same field."""
+    from piro_amd import learned
+    from piro_amd.core import Surrogate
+
+    sur = Surrogate(ctx, seed=5)
+    log = learned.train(ctx, sur, n_traj=32000, per_batch=50, side=32, segments=4, k=16, passes=3, lr=1e-3,
+                        lr_final=1e-5, blob_widths=(4.0, 20.0), noise=0.03, log_every=400)
+    run = learned.HeatRun(ctx, side=128)
+    steps = 1600
+    run.plain(160)
+    ref, t_plain = run.plain(steps)
+    run.accelerated(sur, 160)
+    acc, res = run.accelerated(sur, steps)
+    out = {
+        "workload": "heat 128^3, smooth alpha field in [0.005,0.02], Gaussian initial condition, 1600 FE steps; "
+                    "surrogate calls of k=16 steps, every 10th checked with 16 plain steps",
+        "speedup_vs_plain": t_plain / res.seconds,
+        "plain_seconds": t_plain,
+        "accelerated_seconds": res.seconds,
+        "checks": res.checks,
+        "rejected": res.rejected,
+        "check_rel_l2_max": max(res.check_errors) if res.check_errors else None,
+        "check_tolerance": 1e-3,
+        "final_rel_l2_vs_plain": ctx.rel_l2(acc, ref),
+        "max_amplification": learned.max_amplification(sur.stencil()),
+        "train_trajectories": log.trajectories,
+        "train_seconds": log.seconds,
+    }
+    sur.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate leg (about two minutes of training on the batched stepper)")
     ap.add_argument("--variant", type=int, default=-1, help="force an FE kernel variant (-1 = autotune)")
     ap.add_argument("--copy-reps", type=int, default=10,
                     help="after the timed region, time the library's streaming copy on a 256 MiB vector this many "
@@ -161,6 +200,9 @@ def main() -> int:
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline()
+    for arr in (A.row_ptr, A.col, A.val, x, y, snap):
+        arr.free()
+    sur = None if args.no_surrogate else surrogate_leg(ctx)
 
     out = {
         "metric": METRIC,
@@ -184,6 +226,7 @@ def main() -> int:
                      "frac": gbs / HBM_PEAK_GBS, "traffic": load_traffic(vname),
                      "algorithmic_bytes_per_launch": byts, "measured_copy_gbs": copy_gbs},
         "cpu_baseline": base,
+        "surrogate": sur,
         "field_finite": finite,
         "host_wall_s": t_host,
     }

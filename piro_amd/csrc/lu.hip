@@ -184,12 +184,19 @@ __device__ __forceinline__ void panel_step(float (&p)[R][PB], int (&lab)[R], Pan
     if (piv != gj)
 #pragma unroll
         for (int r = 0; r < R; ++r) lab[r] = lab[r] == piv ? gj : (lab[r] == gj ? piv : lab[r]);
+    // multipliers as LAPACK's sgetf2 forms them: times the pivot's
+    // reciprocal, or divided by the pivot when it is below the smallest
+    // normal number (the test is uniform across the workgroup)
     const float d = u[J];
     if (d != 0.0f) {
+        const bool rcp = fabsf(d) >= __FLT_MIN__;
+        const float rd = 1.0f / d;
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (lab[r] > gj) {
-                const float l = p[r][J] / d;
+                float l;
+                if (__builtin_expect(rcp, 1)) l = p[r][J] * rd;
+                else l = p[r][J] / d;
                 p[r][J] = l;
 #pragma unroll
                 for (int c = J + 1; c < PB; ++c) p[r][c] = __builtin_fmaf(-l, u[c], p[r][c]);
@@ -301,9 +308,12 @@ __global__ __launch_bounds__(256) void k_panel_global(float *__restrict__ a, int
         if (t < PB) s_prow[t] = t < jb ? a[(int64_t)gj * lda + k + t] : 0.0f;
         __syncthreads();
         const float d = s_prow[j];
+        const bool rcp = fabsf(d) >= __FLT_MIN__;  // as in the register panel
+        const float rd = 1.0f / d;
         if (d != 0.0f)
             for (int gi = gj + 1 + t; gi < n; gi += 256) {
-                float l = a[(int64_t)gi * lda + gj] / d;
+                const float av = a[(int64_t)gi * lda + gj];
+                float l = rcp ? av * rd : av / d;
                 a[(int64_t)gi * lda + gj] = l;
                 for (int c = j + 1; c < jb; ++c) a[(int64_t)gi * lda + k + c] -= l * s_prow[c];
             }

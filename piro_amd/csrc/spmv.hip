@@ -294,80 +294,68 @@ __device__ __forceinline__ void stage_tail(const Params &p, int q0, int ka, int 
     }
 }
 
-template <int RB, int MODE, int QPL>
-__global__ __launch_bounds__(256) void k_pf(Params p)
-{
-    constexpr int RPL = RB / 256;      // rows per lane
-    constexpr int CAP = QPL * 256 * 4; // products staged per block
-    __shared__ __attribute__((aligned(16))) float s_p[CAP];
-
-    const int b = logical_block(p);
-    if (b >= p.n_blocks) return;
-    if (MODE == JLI && *p.done) return;
-    const int t = threadIdx.x;
-    const int r0 = b * RB;
-    const int nr = min(RB, p.n_rows - r0);
-    const int k0 = p.rp[r0], k1 = p.rp[r0 + nr];
-    const int ka = k0 & ~3;
-    const bool staged = k1 - ka <= CAP;
-    // fast: at least one entry from ka on, and the quads up to the one
-    // holding entry k1 - 1 lie inside the arrays
-    const bool fast = staged && k1 > ka && (int64_t)((k1 + 3) & ~3) <= p.nnz;
-
-    Quads<QPL> qd;
-    if (fast) load_quads<QPL>(p, ka >> 2, k1, qd);
-
-    // per-row operands
+// a block's per-row operands: row_ptr pair, class, x[r] and (face rows)
+// the snapshot value
+template <int RPL>
+struct RowOps {
     int rb[RPL], re[RPL], cls[RPL];
     float xr[RPL], sv[RPL];
+};
+
+template <int RB, int MODE>
+__device__ __forceinline__ void load_rows(const Params &p, int r0, int nr, RowOps<RB / 256> &ro)
+{
 #pragma unroll
-    for (int j = 0; j < RPL; ++j) {
-        const int rr = t + 256 * j;
+    for (int j = 0; j < RB / 256; ++j) {
+        const int rr = (int)threadIdx.x + 256 * j;
         const int r = r0 + min(rr, nr - 1);
-        rb[j] = p.rp[r];
-        re[j] = p.rp[r + 1];
-        cls[j] = (MODE == FE_INTERIOR || MODE == JLI) ? row_class((uint32_t)r, p.g) : 0;
-        xr[j] = (MODE == SPMV) ? 0.0f : p.x[r];
-        sv[j] = 0.0f;
-        if (MODE == JLI) sv[j] = p.snap[r];
+        ro.rb[j] = p.rp[r];
+        ro.re[j] = p.rp[r + 1];
+        ro.cls[j] = (MODE == FE_INTERIOR || MODE == JLI) ? row_class((uint32_t)r, p.g) : 0;
+        ro.xr[j] = (MODE == SPMV) ? 0.0f : p.x[r];
+        ro.sv[j] = 0.0f;
+        if (MODE == JLI) ro.sv[j] = p.snap[r];
         if (MODE == FE_INTERIOR) {
             // face rows read the snapshot; other rows re-read x[r] (a cache
             // hit) instead of branching around the load
-            const bool face = cls[j] == 1 && p.snap;
+            const bool face = ro.cls[j] == 1 && p.snap;
             const float f = *(face ? p.snap + r : p.x + r);
-            sv[j] = face ? f : 0.0f;
+            ro.sv[j] = face ? f : 0.0f;
         }
     }
+}
 
-    if (fast) store_products<MODE, QPL>(p, ka, qd, s_p);
-    else if (staged) stage_tail<MODE, QPL>(p, ka >> 2, ka, k1, s_p);
-    __syncthreads();
-
+// row sums (staged products from LDS, or the unstaged walk) and the
+// epilogue of block b; JLI adds the block's squared updates to acc[b]
+template <int RB, int MODE>
+__device__ __forceinline__ void finish_block(const Params &p, int b, int r0, int nr, int ka, bool staged,
+                                             const RowOps<RB / 256> &ro, const float *s_p)
+{
     double jli_sq = 0.0;
 #pragma unroll
-    for (int j = 0; j < RPL; ++j) {
-        const int rr = t + 256 * j;
+    for (int j = 0; j < RB / 256; ++j) {
+        const int rr = (int)threadIdx.x + 256 * j;
         if (rr >= nr) continue;
         const int r = r0 + rr;
         float acc = 0.0f;
         if (staged) {
-            for (int k = rb[j] - ka; k < re[j] - ka; ++k) acc = acc + s_p[k];
+            for (int k = ro.rb[j] - ka; k < ro.re[j] - ka; ++k) acc = acc + s_p[k];
         } else {
-            for (int k = rb[j]; k < re[j]; ++k) acc = acc + product<MODE>(p.val[k], p.x[p.col[k]], p.dt);
+            for (int k = ro.rb[j]; k < ro.re[j]; ++k) acc = acc + product<MODE>(p.val[k], p.x[p.col[k]], p.dt);
         }
         if (MODE == FE_INTERIOR) {
-            p.y[r] = cls[j] == 0 ? acc + xr[j] : (cls[j] == 1 ? sv[j] : 0.0f);
+            p.y[r] = ro.cls[j] == 0 ? acc + ro.xr[j] : (ro.cls[j] == 1 ? ro.sv[j] : 0.0f);
         } else if (MODE == ALLROWS) {
             const float cc = p.dt * acc;
-            p.y[r] = xr[j] + cc;
+            p.y[r] = ro.xr[j] + cc;
         } else if (MODE == SPMV) {
             p.y[r] = acc;
         } else {
-            const float ci = cls[j] == 0 ? xr[j] - acc : 0.0f;
-            const float cc = sv[j] - ci;
+            const float ci = ro.cls[j] == 0 ? ro.xr[j] - acc : 0.0f;
+            const float cc = ro.sv[j] - ci;
             const float inc = cc * p.urf;
-            const float xn = xr[j] + inc;
-            const float d = xn - xr[j];
+            const float xn = ro.xr[j] + inc;
+            const float d = xn - ro.xr[j];
             p.y[r] = xn;
             jli_sq += (double)d * (double)d;
         }
@@ -376,6 +364,51 @@ __global__ __launch_bounds__(256) void k_pf(Params p)
         double s = block_sum_f64<256>(jli_sq);
         if (threadIdx.x == 0) p.acc[b] = s;
     }
+}
+
+// a block's non-zero range: staged when it fits CAP products; fast when at
+// least one entry lies from ka on and the quads up to the one holding entry
+// k1 - 1 lie inside the arrays
+struct BlockRange {
+    int r0, nr, ka, k1;
+    bool staged, fast;
+};
+
+template <int RB, int CAP>
+__device__ __forceinline__ BlockRange block_range(const Params &p, int b, int k0, int k1)
+{
+    BlockRange br;
+    br.r0 = b * RB;
+    br.nr = min(RB, p.n_rows - br.r0);
+    br.k1 = k1;
+    br.ka = k0 & ~3;
+    br.staged = k1 - br.ka <= CAP;
+    br.fast = br.staged && k1 > br.ka && (int64_t)((k1 + 3) & ~3) <= p.nnz;
+    return br;
+}
+
+template <int RB, int MODE, int QPL>
+__global__ __launch_bounds__(256) void k_pf(Params p)
+{
+    constexpr int CAP = QPL * 256 * 4; // products staged per block
+    __shared__ __attribute__((aligned(16))) float s_p[CAP];
+
+    const int b = logical_block(p);
+    if (b >= p.n_blocks) return;
+    if (MODE == JLI && *p.done) return;
+    const int r0 = b * RB;
+    const int nr = min(RB, p.n_rows - r0);
+    const BlockRange br = block_range<RB, CAP>(p, b, p.rp[r0], p.rp[r0 + nr]);
+
+    Quads<QPL> qd;
+    if (br.fast) load_quads<QPL>(p, br.ka >> 2, br.k1, qd);
+    RowOps<RB / 256> ro;
+    load_rows<RB, MODE>(p, br.r0, br.nr, ro);
+
+    if (br.fast) store_products<MODE, QPL>(p, br.ka, qd, s_p);
+    else if (br.staged) stage_tail<MODE, QPL>(p, br.ka >> 2, br.ka, br.k1, s_p);
+    __syncthreads();
+    finish_block<RB, MODE>(p, b, br.r0, br.nr, br.ka, br.staged, ro, s_p);
 }
 
 // ------------------------------------------------------- row per lane
@@ -432,6 +465,9 @@ constexpr Variant kVariants[] = {
     {"pf_rb256_c8", 256, true, false, true, 8},
     {"pf_rb256_c32", 256, true, false, true, 32},
     {"pf_rb512_c16", 512, true, false, true, 16},
+    {"pf_rb256_c16", 256, true, false, true, 16},
+    {"pf_rb256_c64", 256, true, false, true, 64},
+    {"pf_rb256_c128", 256, true, false, true, 128},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 constexpr int kDefaultVariant = 9;   // pf_rb256

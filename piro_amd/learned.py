@@ -14,6 +14,7 @@ with dt = r_max*h^2/alpha_max reproduce the per-cell map of the large run.
 """
 from __future__ import annotations
 
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -118,7 +119,7 @@ def train(ctx: Context, sur: Surrogate, n_traj=2000, per_batch=50, side=24, segm
                 log.losses.append(loss)
                 step += 1
                 if log_every and step % log_every == 0:
-                    print(f"train step {step}/{total_steps} loss {loss:.3e}", flush=True)
+                    print(f"train step {step}/{total_steps} loss {loss:.3e}", file=sys.stderr, flush=True)
         log.trajectories += per_batch
         b.close()
     ctx.sync()

@@ -59,13 +59,14 @@ def test_lu_factor_reconstructs(ctx, n):
 @pytest.mark.parametrize("n", [48, 160, 300])
 def test_mfma_update_layout_exact(ctx, n):
     # A = L0 U0 with L0 unit lower (entries in {0, +-1/4, +-1/2}) and U0
-    # upper with small integers: no row exchange is chosen (|l| < 1) and every
+    # upper with small integers and power-of-two pivots (so the pivots'
+    # reciprocals are exact): no row exchange is chosen (|l| < 1) and every
     # intermediate of the elimination is exactly representable, so the
     # factors must come back bit-exact -- this pins the MFMA operand/result
     # lane layout of the trailing update
     rng = np.random.default_rng(n)
     L0 = np.tril(rng.choice([0.0, 0.25, -0.25, 0.5, -0.5], (n, n)), -1) + np.eye(n)
-    U0 = np.triu(rng.integers(-3, 4, (n, n)).astype(np.float64), 1) + np.diag(rng.integers(4, 8, n).astype(np.float64))
+    U0 = np.triu(rng.integers(-3, 4, (n, n)).astype(np.float64), 1) + np.diag(rng.choice([4.0, 8.0], n))
     a = (L0 @ U0).astype(np.float32)
     assert np.array_equal(a.astype(np.float64), L0 @ U0)
     d = dev(a, ctx)
