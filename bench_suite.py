@@ -238,6 +238,9 @@ def bench_surrogate(args):
     steps = 1600
     run.plain(160)  # warm-up
     ref, t_plain = run.plain(steps)
+    run.accelerated(sur, 160, correct=False)  # warm-up
+    acc0, res0 = run.accelerated(sur, steps, correct=False)
+    err0 = ctx.rel_l2(acc0, ref)
     run.accelerated(sur, 160)  # warm-up
     acc, res = run.accelerated(sur, steps)
     err = ctx.rel_l2(acc, ref)
@@ -252,6 +255,9 @@ def bench_surrogate(args):
                       "plain_seconds": t_plain, "accelerated_seconds": res.seconds,
                       "speedup": t_plain / res.seconds, "rel_l2_vs_plain": err,
                       "checks": res.checks, "rejected": res.rejected, "max_amplification": amp,
+                      "uncorrected": {"speedup": t_plain / res0.seconds, "rel_l2_vs_plain": err0,
+                                      "rejected": res0.rejected,
+                                      "check_errors": [float(f"{e:.3e}") for e in res0.check_errors]},
                       "check_errors": [float(f"{e:.3e}") for e in res.check_errors],
                       "surrogate_call_ms": t_call * 1e3,
                       "surrogate_useful_tflops": SUR_USEFUL_FLOP * cells / t_call / 1e12,

@@ -393,3 +393,32 @@ def test_training_reduces_loss(ctx):
     print("surrogate training loss: " + ", ".join(f"{i}: {losses[i]:.3e}" for i in (0, 50, 100, 150, 200, 300, 399)))
     assert np.all(np.isfinite(losses))
     assert np.mean(losses[-10:]) < 0.5 * losses[0]
+
+
+def test_accelerated_run_defect_correction(ctx):
+    # the accelerated run's bookkeeping, replayed with the same device
+    # operations: an untrained surrogate (the identity on interior cells),
+    # three calls of 16 steps with every 2nd call checked from the first on.
+    # Call 0 is checked: it takes the plain result P(x0) and leaves the
+    # defect d = P(x0) - S(x0); call 1 is S(x1) + d; call 2 is checked again
+    # and takes P(x2).
+    from piro_amd import learned
+    from piro_amd.core import Surrogate
+    run = learned.HeatRun(ctx, side=24, seed=5)
+    sur = Surrogate(ctx, seed=2)
+    shape = (1, 24, 24, 24)
+    got, res = run.accelerated(sur, 48, k=16, check_every=2, tol=1e-3, correct=True)
+    assert (res.calls, res.checks) == (3, 2)
+
+    def plain(v):
+        out = v.clone()
+        ctx.fe_run_csr(run.dims, run.A, run.dt, 16, out, ctx.empty_like(out), run.snap)
+        return out
+
+    x0 = run.u0.clone()
+    x1 = plain(x0)
+    d = ctx.axpby(1.0, x1, -1.0, sur.forward(shape, x0, run.alpha, 1.0))
+    y1 = ctx.axpby(1.0, sur.forward(shape, x1, run.alpha, 1.0), 1.0, d)
+    want = plain(y1)
+    assert got.host().tobytes() == want.host().tobytes()
+    sur.close()
