@@ -288,6 +288,110 @@ def bench_surrogate_fwd(args):
     ctx.close()
 
 
+# ------------------------------------------------------------------ formats
+
+def bench_formats(args):
+    """Assembly, the COO/HT/DENSE all-rows steps, the divergence operator and
+    the reductions: device time per call (HIP events on the library's
+    stream) against each operation's algorithmic bytes."""
+    from piro_amd.core import _p, check, lib
+    ctx = Context(0)
+    rows = []
+
+    def row(name, seconds, nbytes, note=""):
+        r = {"op": name, "us": seconds * 1e6}
+        if nbytes:
+            r.update({"algorithmic_bytes": nbytes, "gbs": nbytes / seconds / 1e9,
+                      "roofline_frac": nbytes / seconds / 1e9 / HBM})
+        if note:
+            r["note"] = note
+        rows.append(r)
+        print(json.dumps(r), flush=True)
+
+    side = 256
+    dims = (side,) * 3
+    N = side ** 3
+    norm = ctx.lap_norm(1.0, side - 2)
+    nnz = ctx.lap_nnz(*dims)
+    A = ctx.laplacian_csr(*dims, CD, norm)
+    t = timed(ctx, lambda: check(lib.piro_assemble_laplacian_csr(ctx.h, *dims, CD, float(norm), _p(A.row_ptr),
+                                                                 _p(A.col), _p(A.val))), reps=5)
+    row("laplacian CSR assembly 256^3 (cd)", t, 4 * (N + 1) + 8 * nnz, "writes only")
+    alpha = ctx.alpha_uniform(N, 7, 0.005, 0.02)
+    t = timed(ctx, lambda: ctx.row_scale(A, alpha), reps=5)
+    row("alpha row scaling 256^3", t, 4 * (N + 1) + 4 * N + 8 * nnz, "row_ptr, alpha and val read, val written")
+    g = ctx.grad_norm(1.0, side - 2)
+    G0 = ctx.gradient_csr(*dims, UW, 0, g)
+    t = timed(ctx, lambda: check(lib.piro_assemble_gradient_csr(ctx.h, *dims, UW, 0, float(g), _p(G0.row_ptr),
+                                                                _p(G0.col), _p(G0.val))), reps=5)
+    row("gradient CSR assembly 256^3 (uw, x)", t, 4 * (N + 1) + 8 * G0.nnz, "writes only")
+    u3 = ctx.from_host(np.full(3 * N, 0.5, np.float32))
+    ctx.sync()
+    t0 = time.perf_counter()
+    Dv = ctx.divergence_csr(*dims, CD, g, u3)
+    ctx.sync()
+    row("divergence operator 256^3 (3 gradients, row scaling, merge), host wall", time.perf_counter() - t0, 0,
+        f"nnz {Dv.nnz}; includes allocation of the intermediates")
+    for arr in (Dv.row_ptr, Dv.col, Dv.val, G0.row_ptr, G0.col, G0.val, u3):
+        arr.free()
+
+    # COO all-rows FE step (row_ptr recovered from the sorted row indices)
+    rws, col, val = ctx.laplacian_coo(*dims, CD, norm)
+    t = timed(ctx, lambda: check(lib.piro_coo_row_ptr(ctx.h, N, nnz, _p(rws), _p(A.row_ptr))), reps=5)
+    row("COO row_ptr recovery 256^3", t, 4 * nnz + 4 * (N + 1))
+    Ac = ctx.coo_to_csr(N, rws, col, val)
+    x = ctx.ic_gaussian(dims, [1.0] * 3, [0.5] * 3, [0.1] * 3, 1.0)
+    y = ctx.empty_like(x)
+    dt = np.float32(0.1 * float(norm) / 0.02)
+    ctx.fe_step_allrows(Ac, dt, x, y)
+    t = timed(ctx, lambda: ctx.fe_step_allrows(Ac, dt, x, y), reps=20)
+    row("COO all-rows FE step 256^3", t, 8 * nnz + 12 * N + 4, "SpMVCOO + addVectors, fused")
+    for arr in (rws, col, val, Ac.row_ptr, A.row_ptr, A.col, A.val, x, y, alpha):
+        arr.free()
+
+    # HT near the largest grid the shipped table-size fit covers (N ~ 18652)
+    hs = 26
+    hd = (hs,) * 3
+    Nh = hs ** 3
+    hn = ctx.lap_norm(1.0, hs - 2)
+    Ah = ctx.laplacian_csr(*hd, CD, hn)
+    ts = ctx.ht_table_size(Ah.nnz, Nh)
+    keys, vals = ctx.ht_build(Ah, ts)
+    t = timed(ctx, lambda: check(lib.piro_ht_build(ctx.h, Nh, _p(Ah.row_ptr), _p(Ah.col), _p(Ah.val), int(ts),
+                                                   _p(keys), _p(vals))), reps=5)
+    row(f"HT build 26^3 (table {ts} slots)", t, 0, "launch-bound at this size")
+    Ahc = ctx.ht_to_csr(Nh, keys, vals)
+    xh = ctx.ic_gaussian(hd, [1.0] * 3, [0.5] * 3, [0.1] * 3, 1.0)
+    yh = ctx.empty_like(xh)
+    t = timed(ctx, lambda: ctx.fe_step_allrows(Ahc, dt, xh, yh), reps=50)
+    row("HT all-rows FE step 26^3 (unpacked to CSR)", t, 8 * Ahc.nnz + 12 * Nh + 4, "launch-bound at this size")
+
+    # DENSE: the N x N matrix streams once per step
+    ds = 16
+    dd = (ds,) * 3
+    Nd = ds ** 3
+    dn = ctx.lap_norm(1.0, ds - 2)
+    Ad = ctx.laplacian_dense(*dd, 3, dn)
+    t = timed(ctx, lambda: check(lib.piro_assemble_laplacian_dense(ctx.h, *dd, 3, float(dn), _p(Ad), Nd)), reps=5)
+    row("DENSE Laplacian assembly 16^3 (4096 x 4096)", t, 4 * Nd * Nd, "clear + entries, writes only")
+    xd = ctx.ic_gaussian(dd, [1.0] * 3, [0.5] * 3, [0.1] * 3, 1.0)
+    yd = ctx.empty(Nd)
+    t = timed(ctx, lambda: ctx.fe_step_dense(Ad, Nd, dt, xd, yd), reps=20)
+    row("DENSE all-rows FE step 16^3", t, 4 * Nd * Nd + 12 * Nd, "SpMVDENSE + addVectors")
+
+    # reductions on 64 Mi floats (FP64 accumulation); each call ends in an
+    # 8-byte read-back, included in its device time
+    n = 1 << 26
+    a = ctx.alpha_uniform(n, 1, -1, 1)
+    b = ctx.alpha_uniform(n, 2, -1, 1)
+    for op, nb in (("sum", 4 * n), ("sumsq", 4 * n), ("absmax", 4 * n), ("dot", 8 * n), ("diffsq", 8 * n)):
+        ctx.reduce(op, a, b)
+        t = sum(timed(ctx, lambda: ctx.reduce(op, a, b)) for _ in range(5)) / 5
+        row(f"reduce {op} 64Mi floats", t, nb)
+    print(json.dumps({"metric": "formats, assembly and reductions", "rows": len(rows)}), flush=True)
+    ctx.close()
+
+
 # ---------------------------------------------------------------------- JLI
 
 def bench_jli(args):
@@ -328,7 +432,7 @@ def bench_jli(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["lu", "batch", "advection", "surrogate", "surrogate_fwd", "jli"])
+    ap.add_argument("what", choices=["lu", "batch", "advection", "surrogate", "surrogate_fwd", "jli", "formats"])
     ap.add_argument("--side", type=int, default=256)
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--traj", type=int, default=32000)
@@ -336,7 +440,7 @@ def main():
     ap.add_argument("--passes", type=int, default=3)
     args = ap.parse_args()
     {"lu": bench_lu, "batch": bench_batch, "advection": bench_advection, "surrogate": bench_surrogate,
-     "surrogate_fwd": bench_surrogate_fwd, "jli": bench_jli}[args.what](args)
+     "surrogate_fwd": bench_surrogate_fwd, "jli": bench_jli, "formats": bench_formats}[args.what](args)
 
 
 if __name__ == "__main__":

@@ -369,24 +369,23 @@ __global__ __launch_bounds__(256) void k_sort_rows(int n_rows, const int32_t *__
     }
 }
 
-__global__ __launch_bounds__(256) void k_coo_count(int64_t nnz, const int32_t *__restrict__ rows,
-                                                   int32_t *__restrict__ count)
+// CSR row_ptr of sorted COO rows in one pass: position k (0 <= k <= nnz)
+// closes the rows (rows[k-1], rows[k]] -- every row_ptr entry in that range
+// is k (rows[-1] = -1, rows[nnz] = n_rows) -- so each entry is written by
+// exactly one thread, empty rows included.  A decrease or a row outside
+// [0, n_rows) raises *bad and writes nothing.
+__global__ __launch_bounds__(256) void k_coo_row_ptr(int64_t nnz, int n_rows, const int32_t *__restrict__ rows,
+                                                     int32_t *__restrict__ rp, int *__restrict__ bad)
 {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nnz) return;
-    // rows are sorted: the last entry of each row's run records the count
-    int r = rows[k];
-    if (k + 1 == nnz || rows[k + 1] != r) {
-        int64_t b = k;
-        // find the first entry of the run by binary search over [0, k]
-        int64_t lo = 0, hi = k;
-        while (lo < hi) {
-            int64_t mid = (lo + hi) >> 1;
-            if (rows[mid] < r) lo = mid + 1; else hi = mid;
-        }
-        b = lo;
-        count[r] = (int32_t)(k - b + 1);
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nnz) return;
+    const int prev = k == 0 ? -1 : rows[k - 1];
+    const int cur = k == nnz ? n_rows : rows[k];
+    if ((k < nnz && (cur < 0 || cur >= n_rows)) || (k > 0 && (prev < 0 || prev >= n_rows)) || cur < prev) {
+        *bad = 1;
+        return;
     }
+    for (int r = prev + 1; r <= cur; ++r) rp[r] = (int32_t)k;
 }
 
 Dims mkdims(int nx, int ny, int nz)
@@ -653,14 +652,15 @@ int piro_coo_row_ptr(piro_ctx *ctx, int n_rows, int64_t nnz, const int32_t *d_ro
     PIRO_API_BEGIN
     ctx_of(ctx);
     PIRO_REQUIRE(n_rows > 0 && nnz >= 0 && d_rows && d_row_ptr, "bad arguments");
-    int32_t *count = (int32_t *)ctx->scratch[1].get((size_t)n_rows * sizeof(int32_t));
-    PIRO_CHECK_HIP(hipMemsetAsync(count, 0, (size_t)n_rows * sizeof(int32_t), ctx->stream));
-    if (nnz > 0) {
-        k_coo_count<<<grid_for(nnz, 256), 256, 0, ctx->stream>>>(nnz, d_rows, count);
-        check_launch();
-    }
-    int64_t tot = exclusive_scan_i32(ctx, 2, count, n_rows, d_row_ptr);
-    PIRO_REQUIRE(tot == nnz, "COO rows must be sorted and within [0, n_rows)");
+    PIRO_REQUIRE(nnz <= 0x7fffffffLL, "nnz must fit int32 row pointers");
+    int *bad = (int *)ctx->scratch[1].get(sizeof(int));
+    PIRO_CHECK_HIP(hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
+    k_coo_row_ptr<<<grid_for(nnz + 1, 256), 256, 0, ctx->stream>>>(nnz, n_rows, d_rows, d_row_ptr, bad);
+    check_launch();
+    int h = 0;
+    PIRO_CHECK_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    PIRO_REQUIRE(h == 0, "COO rows must be sorted and within [0, n_rows)");
     PIRO_API_END
 }
 

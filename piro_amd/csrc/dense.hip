@@ -14,7 +14,8 @@
 // running FP32 sum that started at +0 (it never becomes -0), so only the
 // terms with A[r][k] != 0 or a non-finite x[k] are added -- in column order,
 // broadcast to the whole wave -- and the result is the reference's sum bit
-// for bit while the wave reads N floats per row at streaming rate.
+// for bit while the wave reads N floats per row at streaming rate (16 bytes
+// per lane when N and the buffers allow).
 #include "device_utils.hpp"
 
 #include <cmath>
@@ -60,7 +61,10 @@ __global__ __launch_bounds__(256) void k_lap_dense(int nx, int ny, int nz, float
 }
 
 // y[r] = x[r] + fl(dt * sum_k fl(a[r][k] * x[k])), all rows; 4 rows per
-// block, one wave per row
+// block, one wave per row.  VEC: the row is read 16 bytes per lane (columns
+// c0 + 4 lane + e, e = 0..3); the live terms are added lane by lane and, in
+// a lane, e by e -- ascending columns either way.
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_fe_dense(int n, const float *__restrict__ a, int64_t ld, float dt,
                                                   const float *__restrict__ x, float *__restrict__ y)
 {
@@ -69,17 +73,45 @@ __global__ __launch_bounds__(256) void k_fe_dense(int n, const float *__restrict
     if (r >= n) return;
     const float *row = a + r * ld;
     float run = 0.0f;  // identical in every lane
-    for (int c0 = 0; c0 < n; c0 += 64) {
-        const int c = c0 + lane;
-        const float av = c < n ? row[c] : 0.0f;
-        const float xv = c < n ? x[c] : 0.0f;
-        const float p = av * xv;
-        const bool live = c < n && (av != 0.0f || !isfinite(xv));
-        uint64_t m = __ballot(live);
-        while (m) {
-            const int src = __builtin_ctzll(m);
-            m &= m - 1;
-            run = run + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p), src));
+    if (VEC) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        for (int c0 = 0; c0 < n; c0 += 256) {
+            const int c = c0 + 4 * lane;
+            f4 av = {0.0f, 0.0f, 0.0f, 0.0f}, xv = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (c < n) {  // n % 4 == 0: a lane's four columns are all in or all out
+                av = *(const f4 *)(row + c);
+                xv = *(const f4 *)(x + c);
+            }
+            float p[4];
+            int live = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                p[e] = av[e] * xv[e];
+                if (c < n && (av[e] != 0.0f || !isfinite(xv[e]))) live |= 1 << e;
+            }
+            uint64_t m = __ballot(live != 0);
+            while (m) {
+                const int src = __builtin_ctzll(m);
+                m &= m - 1;
+                const int lv = __builtin_amdgcn_readlane(live, src);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (lv & (1 << e)) run = run + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p[e]), src));
+            }
+        }
+    } else {
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int c = c0 + lane;
+            const float av = c < n ? row[c] : 0.0f;
+            const float xv = c < n ? x[c] : 0.0f;
+            const float p = av * xv;
+            const bool live = c < n && (av != 0.0f || !isfinite(xv));
+            uint64_t m = __ballot(live);
+            while (m) {
+                const int src = __builtin_ctzll(m);
+                m &= m - 1;
+                run = run + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p), src));
+            }
         }
     }
     if (lane == 0) y[r] = x[r] + dt * run;
@@ -117,7 +149,11 @@ int piro_fe_step_dense(piro_ctx *ctx, int n, const float *d_a, int64_t ld, float
     ctx_of(ctx);
     PIRO_REQUIRE(n > 0 && ld >= n, "bad shape");
     PIRO_REQUIRE(d_a && d_x && d_y && d_x != d_y, "bad buffers (x and y must differ)");
-    k_fe_dense<<<grid_for(n, 4), 256, 0, ctx->stream>>>(n, d_a, ld, dt, d_x, d_y);
+    // 16-byte rows when the row length, the leading dimension and both
+    // bases allow it
+    const bool vec = n % 4 == 0 && ld % 4 == 0 && ((uintptr_t)d_a % 16) == 0 && ((uintptr_t)d_x % 16) == 0;
+    if (vec) k_fe_dense<true><<<grid_for(n, 4), 256, 0, ctx->stream>>>(n, d_a, ld, dt, d_x, d_y);
+    else k_fe_dense<false><<<grid_for(n, 4), 256, 0, ctx->stream>>>(n, d_a, ld, dt, d_x, d_y);
     check_launch();
     PIRO_API_END
 }

@@ -398,3 +398,23 @@ def test_vec_copy(ctx, n, offset):
     got = host(dst)
     assert got[offset:offset + n].tobytes() == host(src)[offset:offset + n].tobytes()
     assert np.all(got[:offset] == -7.0) and np.all(got[offset + n:] == -7.0)
+
+
+def test_coo_row_ptr_edges(ctx):
+    # row_ptr[i] = number of entries in rows below i, for sorted rows with
+    # empty leading, inner and trailing rows and for no entries at all;
+    # unsorted or out-of-range rows are rejected
+    from piro_amd._lib import PiroError
+    from piro_amd.core import _p, check, lib
+    for n, rows in ((6, [0, 0, 2, 2, 2, 5]), (5, [3, 3, 4]), (7, [0, 1]), (4, []), (1, [0] * 1000)):
+        r = np.array(rows, np.int32)
+        d_rows = dev(r if r.size else np.zeros(1, np.int32), ctx)
+        rp = ctx.empty(n + 1, np.int32)
+        check(lib.piro_coo_row_ptr(ctx.h, n, r.size, _p(d_rows), _p(rp)))
+        want = np.searchsorted(r, np.arange(n + 1), side="left").astype(np.int32)
+        np.testing.assert_array_equal(host(rp), want)
+    for n, rows in ((5, [0, 2, 1]), (5, [0, 5]), (5, [-1, 0])):
+        d_rows = dev(np.array(rows, np.int32), ctx)
+        rp = ctx.empty(n + 1, np.int32)
+        with pytest.raises(PiroError):
+            check(lib.piro_coo_row_ptr(ctx.h, n, len(rows), _p(d_rows), _p(rp)))

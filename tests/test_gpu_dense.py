@@ -27,11 +27,13 @@ def test_laplacian_dense_bit_exact(ctx, scheme):
     assert a.tobytes() == ref.ravel().tobytes()
 
 
+# N = 504 takes the 16-byte row path, N = 125 the 4-byte one
+@pytest.mark.parametrize("dims", [(9, 8, 7), (5, 5, 5)])
 @pytest.mark.parametrize("scheme", [po.UW, po.CD])
-def test_fe_step_dense_bit_exact(ctx, scheme):
-    nx, ny, nz = 9, 8, 7
+def test_fe_step_dense_bit_exact(ctx, scheme, dims):
+    nx, ny, nz = dims
     N = nx * ny * nz
-    print(f"case dense step scheme={scheme}", flush=True)
+    print(f"case dense step scheme={scheme} dims={dims}", flush=True)
     norm = po.lap_norm(1.0, nx - 2)
     a = ctx.laplacian_dense(nx, ny, nz, scheme, norm)
     ref_a = po.lap_dense(nx, ny, nz, scheme, norm)
