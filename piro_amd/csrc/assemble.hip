@@ -45,40 +45,67 @@ __device__ __forceinline__ int64_t lap_before(int x, int y, int z, int64_t i, co
 // column to 10.0f; in closed form (cd/dw): the z+ entry whose target is a
 // boundary cell, the y+ entry of a z=0 row, the x+ entry of a z=0,y=0 row,
 // and row 0's self entry; (uw): the self entry of every boundary row.
+//
+// A block's rows are consecutive, so their entries are one contiguous run:
+// each thread builds its row in LDS and the block writes the run out with
+// consecutive lanes on consecutive entries.
 template <bool COO>
 __global__ __launch_bounds__(256) void k_lap_csr(Dims d, int scheme, float vself, float vnb,
                                                  int32_t *__restrict__ row_ptr_or_rows,
                                                  int32_t *__restrict__ col, float *__restrict__ val)
 {
-    int64_t N = d.sxy * d.nz;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    int x = (int)(i % d.nx);
-    int64_t t = i / d.nx;
-    int y = (int)(t % d.ny);
-    int z = (int)(t / d.ny);
-    int64_t k = lap_before(x, y, z, i, d);
-    if (!COO) {
-        row_ptr_or_rows[i] = (int32_t)k;
-        if (i == N - 1) row_ptr_or_rows[N] = (int32_t)(7 * N - 2 * ((int64_t)d.ny * d.nz + (int64_t)d.nx * d.nz + d.sxy));
+    __shared__ int32_t s_col[7 * 256];
+    __shared__ float s_val[7 * 256];
+    __shared__ int32_t s_row[COO ? 7 * 256 : 1];
+    __shared__ int64_t s_k0;
+    __shared__ int s_nk;
+    const int64_t N = d.sxy * d.nz;
+    const int64_t i0 = (int64_t)blockIdx.x * 256;
+    const int64_t i = i0 + threadIdx.x;
+    const bool live = i < N;
+    // N < 2^31: 32-bit coordinates
+    const uint32_t ui = (uint32_t)(live ? i : i0);
+    const int x = (int)(ui % (uint32_t)d.nx);
+    const uint32_t t = ui / (uint32_t)d.nx;
+    const int y = (int)(t % (uint32_t)d.ny);
+    const int z = (int)(t / (uint32_t)d.ny);
+    const int64_t k = lap_before(x, y, z, (int64_t)ui, d);
+    if (threadIdx.x == 0) s_k0 = k;  // row i0 < N always exists
+    __syncthreads();
+    const int64_t kb = s_k0;
+    if (live) {
+        if (!COO) {
+            row_ptr_or_rows[i] = (int32_t)k;
+            if (i == N - 1)
+                row_ptr_or_rows[N] = (int32_t)(7 * N - 2 * ((int64_t)d.ny * d.nz + (int64_t)d.nx * d.nz + d.sxy));
+        }
+        const bool uw = scheme == PIRO_UW, dw = scheme == PIRO_DW;
+        const bool bnd = is_bnd(x, y, z, d);
+        const int32_t ii = (int32_t)i;
+        int kl = (int)(k - kb);
+        auto emit = [&](int32_t c, float v) {
+            s_col[kl] = c;
+            s_val[kl] = v;
+            if (COO) s_row[kl] = ii;
+            ++kl;
+        };
+        // self
+        emit(ii, (uw ? bnd : (i == 0)) ? 10.0f : vself);
+        if (x > 0) emit(dw ? ii : ii - 1, vnb);
+        if (x < d.nx - 1) emit(uw ? ii : ii + 1, (!uw && z == 0 && y == 0) ? 10.0f : vnb);
+        if (y > 0) emit(dw ? ii : ii - d.nx, vnb);
+        if (y < d.ny - 1) emit(uw ? ii : ii + d.nx, (!uw && z == 0) ? 10.0f : vnb);
+        if (z > 0) emit(dw ? ii : (int32_t)(i - d.sxy), vnb);
+        if (z < d.nz - 1) emit(uw ? ii : (int32_t)(i + d.sxy), (!uw && is_bnd(x, y, z + 1, d)) ? 10.0f : vnb);
+        if (i == min(i0 + 255, N - 1)) s_nk = kl;  // the block's last row ends the run
     }
-    const bool uw = scheme == PIRO_UW, dw = scheme == PIRO_DW;
-    const bool bnd = is_bnd(x, y, z, d);
-    int32_t ii = (int32_t)i;
-    auto emit = [&](int32_t c, float v) {
-        col[k] = c;
-        val[k] = v;
-        if (COO) row_ptr_or_rows[k] = ii;
-        ++k;
-    };
-    // self
-    emit(ii, (uw ? bnd : (i == 0)) ? 10.0f : vself);
-    if (x > 0) emit(dw ? ii : ii - 1, vnb);
-    if (x < d.nx - 1) emit(uw ? ii : ii + 1, (!uw && z == 0 && y == 0) ? 10.0f : vnb);
-    if (y > 0) emit(dw ? ii : ii - d.nx, vnb);
-    if (y < d.ny - 1) emit(uw ? ii : ii + d.nx, (!uw && z == 0) ? 10.0f : vnb);
-    if (z > 0) emit(dw ? ii : (int32_t)(i - d.sxy), vnb);
-    if (z < d.nz - 1) emit(uw ? ii : (int32_t)(i + d.sxy), (!uw && is_bnd(x, y, z + 1, d)) ? 10.0f : vnb);
+    __syncthreads();
+    const int nk = s_nk;
+    for (int j = threadIdx.x; j < nk; j += 256) {
+        col[kb + j] = s_col[j];
+        val[kb + j] = s_val[j];
+        if (COO) row_ptr_or_rows[kb + j] = s_row[j];
+    }
 }
 
 // src/stencils.cpp:7-231: per row (lower, -1/norm) if d>0, (upper, +1/norm)
@@ -154,13 +181,30 @@ __global__ __launch_bounds__(256) void k_lap2d(int n, float inv_h2, int32_t *__r
 }
 
 // SpMDSM.cl:20-31 intended semantics, one thread per row.
+// one block per 256 rows: the block's entries are one contiguous run, walked
+// with consecutive lanes on consecutive entries; each entry finds its row by
+// binary search in the block's row_ptr, staged in LDS
 __global__ __launch_bounds__(256) void k_row_scale(int n_rows, const int32_t *__restrict__ rp,
                                                    float *__restrict__ val, const float *__restrict__ diag)
 {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_rows) return;
-    float s = diag[r];
-    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) val[k] = val[k] * s;
+    __shared__ int32_t s_rp[257];
+    __shared__ float s_d[256];
+    const int r0 = blockIdx.x * 256;
+    const int nr = min(256, n_rows - r0);
+    for (int j = threadIdx.x; j <= nr; j += 256) s_rp[j] = rp[r0 + j];
+    if ((int)threadIdx.x < nr) s_d[threadIdx.x] = diag[r0 + threadIdx.x];
+    __syncthreads();
+    const int32_t k1 = s_rp[nr];
+    for (int32_t k = s_rp[0] + (int32_t)threadIdx.x; k < k1; k += 256) {
+        // the last row j with s_rp[j] <= k (s_rp[0] <= k < s_rp[nr])
+        int lo = 0, hi = nr;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_rp[mid] <= k) lo = mid;
+            else hi = mid;
+        }
+        val[k] = val[k] * s_d[lo];
+    }
 }
 
 // src/ic.cpp:16-27,50-58 on padded coordinates idx*l/n_pad.
