@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_batch_step(const Unit *__restrict__ uni
                 }
                 out = acc + x[g];
             }
-            y[g] = out;
+            __builtin_nontemporal_store(out, y + g);  // read by the next step's gathers
         }
     }
 }
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(256) void k_batch_pf(const Unit *__restrict__ units
                 }
                 out = acc + xr[j];
             }
-            y[r0 + rr] = out;
+            __builtin_nontemporal_store(out, y + r0 + rr);  // read by the next step's gathers
         }
     }
 }
