@@ -99,6 +99,21 @@ def test_row_scale_bit_exact(ctx):
     assert host(A.val).tobytes() == R.val.tobytes()
 
 
+@pytest.mark.parametrize("long_len", [0, 9000])
+def test_row_scale_ragged_csr(ctx, long_len):
+    # arbitrary CSR: empty rows and whole empty row blocks, and (long_len)
+    # rows longer than a block's 256-entry stride
+    rng = np.random.default_rng(31 + long_len)
+    n = 5000
+    R = _random_csr(rng, n, n, long_rows=(0, 777, 4999) if long_len else (), long_len=long_len)
+    A = ctx.csr_from_host(R.row_ptr, R.col, R.val)
+    d = rng.uniform(-2, 2, n).astype(np.float32)
+    ctx.row_scale(A, dev(d, ctx))
+    rows = np.repeat(np.arange(n), np.diff(R.row_ptr))
+    want = (R.val * d[rows]).astype(np.float32)
+    assert host(A.val).tobytes() == want.tobytes()
+
+
 def test_alpha_generator_bit_exact(ctx):
     a = ctx.alpha_uniform(100003, 20261203, 0.005, 0.02)
     assert host(a).tobytes() == po.alpha_uniform(100003, 20261203, 0.005, 0.02).tobytes()
