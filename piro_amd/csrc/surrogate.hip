@@ -834,38 +834,47 @@ __global__ __launch_bounds__(256) void k_stencil13(Shape s, const float *__restr
 #pragma unroll
     for (int f = 0; f < NFRAG; ++f) bw[f] = fb[(p0 * NFRAG + f) * 64];
 
-    // stage: item = (plane, row, 8-wide x chunk); every load of the
-    // thread's items is issued before the first conversion
-    constexpr int NITEM = SIN * SIN * 4, IPT = (NITEM + 255) / 256;
-    float sv[IPT][8];
+    // stage: item = (plane, row, pair q of x positions x0 - 6 + 2q, +1);
+    // consecutive lanes take consecutive pairs, so a wave reads four 128-byte
+    // row segments per load (8-byte loads when W is even and u 8-byte aligned,
+    // which keeps every pair aligned), and every load of the thread is issued
+    // before the first
+    // conversion.  The BF16 parts go to LDS as packed pairs.
+    constexpr int NITEM = SIN * SIN * 16, IPT = (NITEM + 255) / 256;
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bool even_w = (s.W & 1) == 0 && ((uintptr_t)u & 7) == 0;
+    f2 sv[IPT];
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
         const int e = t + 256 * k;
-        const int c4 = e & 3, py = (e >> 2) % SIN, pz = (e >> 2) / SIN;
-        const int zz = z0 - KR + pz, yy = y0 - KR + py, xb = x0 - KR + 8 * c4;
+        const int q = e & 15, py = (e >> 4) % SIN, pz = (e >> 4) / SIN;
+        const int zz = z0 - KR + pz, yy = y0 - KR + py, xx = x0 - KR + 2 * q;
         const bool rowin = e < NITEM && zz >= 0 && zz < s.D && yy >= 0 && yy < s.H;
-        const float *row = u + vidx(b, rowin ? zz : 0, rowin ? yy : 0, 0, s);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int xx = xb + i;
-            sv[k][i] = (rowin && xx >= 0 && xx < s.W) ? row[xx] : 0.0f;
+        const float *src = u + vidx(b, rowin ? zz : 0, rowin ? yy : 0, 0, s) + xx;
+        const bool in0 = rowin && xx >= 0 && xx < s.W, in1 = rowin && xx + 1 >= 0 && xx + 1 < s.W;
+        if (even_w && in0 && in1) {
+            sv[k] = *(const f2 *)src;
+        } else {
+            sv[k][0] = in0 ? src[0] : 0.0f;
+            sv[k][1] = in1 ? src[1] : 0.0f;
         }
     }
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
         const int e = t + 256 * k;
         if (e >= NITEM) continue;
-        const int c4 = e & 3, py = (e >> 2) % SIN, pz = (e >> 2) / SIN;
-        bf16x8 hi, lo;
+        const int q = e & 15, py = (e >> 4) % SIN, pz = (e >> 4) / SIN;
+        bf16x2 hi, lo;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < 2; ++i) {
             const float v = sv[k][i] * inv_scale;
             const bf16 h = (bf16)v;
             hi[i] = h;
             lo[i] = (bf16)(v - (float)h);
         }
-        *(bf16x8 *)(su + (pz * SIN + py) * SXP + 8 * c4) = hi;
-        *(bf16x8 *)(su + STAGE + (pz * SIN + py) * SXP + 8 * c4) = lo;
+        *(bf16x2 *)(su + (pz * SIN + py) * SXP + 2 * q) = hi;
+        *(bf16x2 *)(su + STAGE + (pz * SIN + py) * SXP + 2 * q) = lo;
     }
     // the epilogue's u and diffusivity factor of this lane's eight output
     // voxels: slices 8 h + 2 wv + k2, rows y0 + 4 kg + r, column x0 + li
