@@ -422,67 +422,107 @@ __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int l
 
 typedef float float4v __attribute__((ext_vector_type(4)));
 
-// A[r0..r1, c0..c1) -= L21 (rows r0..r1, cols k..k+jb) x U12 (rows
-// k..k+jb, cols c0..c1), jb <= KW; the row bound r1 is passed as n.  Block tile 128 x 128, 4 waves in a 2 x 2
-// arrangement, 64 x 64 per wave (4 x 4 MFMA 16x16 tiles), K in steps of 4.
+// A[r0..n, c0..c1) -= L21 (rows r0..n, cols k..k+jb) x U12 (rows k..k+jb,
+// cols c0..c1), jb <= KW.  Workgroup tile (64 WGR) x (64 WGC), one wave per
+// 64 x 64 (4 x 4 tiles of 16 x 16), K in steps of 4.  The C entry is the
+// MFMA's initial accumulator, so every entry becomes c + sum_k (-l u) in
+// ascending k, whatever the tiling.
 // v_mfma_f32_16x16x4f32 lane layout: A[i][k]: i = lane%16, k = lane/16;
 // B[k][j]: j = lane%16, k = lane/16; D[i][j]: j = lane%16, i = 4*(lane/16)+r.
-template <int KW>
-__global__ __launch_bounds__(256) void k_update_mfma(float *__restrict__ a, int lda, int n, int k, int jb, int r0g,
-                                                     int c0g, int c1)
+// The product is formed transposed (A = U12^T, B = -L21^T, D = C^T), so a
+// lane's four accumulator entries are four consecutive columns of one C row
+// and move as one 16-byte access.  Range checks are per 16 x 16 tile and
+// uniform across the wave; only tiles cut by the matrix edge take the
+// per-entry path.  VEC: lda, the base, k and c0 allow 16-byte accesses.
+template <int KW, int WGR, int WGC>
+__global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restrict__ a, int lda, int n, int k, int jb,
+                                                               int r0g, int c0g, int c1, bool vec)
 {
-    __shared__ float s_a[128][KW + 1];
-    __shared__ float s_b[KW][128 + 4];
-    const int r0 = r0g + blockIdx.y * 128, c0 = c0g + blockIdx.x * 128;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    // stage the L21 tile (negated) and the U12 tile
-    for (int e = t; e < 128 * KW; e += 256) {
-        const int i = e / KW, kk = e % KW;
-        const int gi = r0 + i;
-        s_a[i][kk] = (gi < n && kk < jb) ? -a[(int64_t)gi * lda + k + kk] : 0.0f;
-    }
-    for (int e = t; e < KW * 128; e += 256) {
-        const int kk = e / 128, j = e % 128;
-        const int gj = c0 + j;
-        s_b[kk][j] = (gj < c1 && kk < jb) ? a[(int64_t)(k + kk) * lda + gj] : 0.0f;
-    }
-    __syncthreads();
-    const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
-    float4v acc[4][4];
+    constexpr int NT = 64 * WGR * WGC, TR = 64 * WGR, TC = 64 * WGC;
+    // pitches chosen so the operand reads of a wave hit 64 distinct banks
+    constexpr int PA = KW + 4, PBP = TC + 16;
+    __shared__ __attribute__((aligned(16))) float s_a[TR * PA];   // -L21
+    __shared__ __attribute__((aligned(16))) float s_b[KW * PBP];  // U12
+    const int r0 = r0g + blockIdx.y * TR, c0 = c0g + blockIdx.x * TC;
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int rw = r0 + (w / WGC) * 64, cw = c0 + (w % WGC) * 64;
     const int li = lane & 15, lk = lane >> 4;
+
+    // this lane's C entries: rows rw + 16 ti + li, columns cw + 16 tj + 4 lk + r
+    int64_t roff[4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) roff[ti] = (int64_t)min(rw + ti * 16 + li, n - 1) * lda;
+    auto tile_full = [&](int ti, int tj) { return vec && rw + ti * 16 + 16 <= n && cw + tj * 16 + 16 <= c1; };
+    auto tile_live = [&](int ti, int tj) { return rw + ti * 16 < n && cw + tj * 16 < c1; };
+    float4v acc[4][4];
 #pragma unroll
     for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
         for (int tj = 0; tj < 4; ++tj) {
-            const int gcol = c0 + wc + tj * 16 + li;
+            const int row = rw + ti * 16 + li, col = cw + tj * 16 + 4 * lk;
+            if (tile_full(ti, tj)) {
+                acc[ti][tj] = *(const float4v *)(a + roff[ti] + col);
+            } else {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int grow = r0 + wr + ti * 16 + 4 * lk + r;
-                acc[ti][tj][r] = (grow < n && gcol < c1) ? a[(int64_t)grow * lda + gcol] : 0.0f;
+                for (int r = 0; r < 4; ++r) acc[ti][tj][r] = (row < n && col + r < c1) ? a[roff[ti] + col + r] : 0.0f;
             }
         }
+
+    // stage the L21 tile (negated) and the U12 tile, four entries a thread
+    for (int e = t; e < TR * KW / 4; e += NT) {
+        const int i = e / (KW / 4), kk = 4 * (e % (KW / 4)), gi = r0 + i;
+        const float *src = a + (int64_t)min(gi, n - 1) * lda + k + kk;
+        float4v v;
+        if (vec && gi < n && kk + 4 <= jb) {
+            v = *(const float4v *)src;
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = (gi < n && kk + r < jb) ? src[r] : 0.0f;
+        }
+        *(float4v *)(s_a + i * PA + kk) = -v;
+    }
+    for (int e = t; e < KW * TC / 4; e += NT) {
+        const int kk = e / (TC / 4), j = 4 * (e % (TC / 4)), gj = c0 + j;
+        const float *src = a + (int64_t)(k + min(kk, jb - 1)) * lda + gj;
+        float4v v;
+        if (vec && kk < jb && gj + 4 <= c1) {
+            v = *(const float4v *)src;
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = (kk < jb && gj + r < c1) ? src[r] : 0.0f;
+        }
+        *(float4v *)(s_b + kk * PBP + j) = v;
+    }
+    __syncthreads();
+
+    const int wr = rw - r0, wc = cw - c0;
     for (int ks = 0; ks < KW; ks += 4) {
         if (ks >= jb) break;  // uniform; staged entries past jb are 0 anyway
         float fa[4], fb[4];
 #pragma unroll
-        for (int ti = 0; ti < 4; ++ti) fa[ti] = s_a[wr + ti * 16 + li][ks + lk];
+        for (int ti = 0; ti < 4; ++ti) fa[ti] = s_a[(wr + ti * 16 + li) * PA + ks + lk];
 #pragma unroll
-        for (int tj = 0; tj < 4; ++tj) fb[tj] = s_b[ks + lk][wc + tj * 16 + li];
+        for (int tj = 0; tj < 4; ++tj) fb[tj] = s_b[(ks + lk) * PBP + wc + tj * 16 + li];
 #pragma unroll
         for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
             for (int tj = 0; tj < 4; ++tj)
-                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[ti], fb[tj], acc[ti][tj], 0, 0, 0);
+                if (tile_live(ti, tj))
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[tj], fa[ti], acc[ti][tj], 0, 0, 0);
     }
+
 #pragma unroll
     for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
         for (int tj = 0; tj < 4; ++tj) {
-            const int gcol = c0 + wc + tj * 16 + li;
+            const int row = rw + ti * 16 + li, col = cw + tj * 16 + 4 * lk;
+            if (tile_full(ti, tj)) {
+                *(float4v *)(a + roff[ti] + col) = acc[ti][tj];
+            } else {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int grow = r0 + wr + ti * 16 + 4 * lk + r;
-                if (grow < n && gcol < c1) a[(int64_t)grow * lda + gcol] = acc[ti][tj][r];
+                for (int r = 0; r < 4; ++r)
+                    if (row < n && col + r < c1) a[roff[ti] + col + r] = acc[ti][tj][r];
             }
         }
 }
@@ -951,7 +991,12 @@ void panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int32_t *ipiv
 {
     hipStream_t s = ctx->stream;
     const int m = n - k;
-    // register panels hold 1024 x R rows; taller panels work in global memory
+    // register panels hold 1024 x R rows; taller panels work in global memory.
+    // Many waves with few rows each: the column chain is latency-bound, and
+    // four waves per SIMD hide more of it than fewer waves with fewer
+    // per-wave reductions (256 x 8 / 512 x 8 measured 8% slower overall,
+    // 512 x 1 / 1024 x 1 / 1024 x 2 2% faster than 256 x 2 / 256 x 4 /
+    // 512 x 4).
     const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0);
 #define PIRO_PANEL(NT_, R_)                                                                   \
     {                                                                                         \
@@ -959,9 +1004,9 @@ void panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int32_t *ipiv
         else k_panel<NT_, R_, false><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);       \
     }
     if (m <= 256) PIRO_PANEL(256, 1)
-    else if (m <= 512) PIRO_PANEL(256, 2)
-    else if (m <= 1024) PIRO_PANEL(256, 4)
-    else if (m <= 2048) PIRO_PANEL(512, 4)
+    else if (m <= 512) PIRO_PANEL(512, 1)
+    else if (m <= 1024) PIRO_PANEL(1024, 1)
+    else if (m <= 2048) PIRO_PANEL(1024, 2)
     else if (m <= 4096) PIRO_PANEL(1024, 4)
     else k_panel_global<<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
 #undef PIRO_PANEL
@@ -971,6 +1016,7 @@ void panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int32_t *ipiv
 void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
 {
     hipStream_t s = ctx->stream;
+    const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0);
     for (int kb = 0; kb < n; kb += NB) {
         const int nbk = min(NB, n - kb);
         // panels of the block; each updates only the block's columns
@@ -981,8 +1027,9 @@ void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
             if (c0 < c1) {
                 k_swap_trsm<PB><<<grid_for(c1 - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, c1, ipiv);
                 check_launch();
-                dim3 g((c1 - c0 + 127) / 128, (n - c0 + 127) / 128);
-                k_update_mfma<PB><<<g, 256, 0, s>>>(a, lda, n, k, jb, c0, c0, c1);
+                // at most 48 columns: one-wave 64 x 64 tiles
+                dim3 g((c1 - c0 + 63) / 64, (n - c0 + 63) / 64);
+                k_update_mfma<PB, 1, 1><<<g, 64, 0, s>>>(a, lda, n, k, jb, c0, c0, c1, vec && k % 4 == 0 && c0 % 4 == 0);
                 check_launch();
             }
         }
@@ -997,7 +1044,7 @@ void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
             k_trsm_block<NB><<<grid_for(n - c0, 256), 256, 0, s>>>(a, lda, kb, nbk, c0, n);
             check_launch();
             dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
-            k_update_mfma<NB><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n);
+            k_update_mfma<NB, 2, 2><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vec && kb % 4 == 0 && c0 % 4 == 0);
             check_launch();
         }
     }
