@@ -332,12 +332,16 @@ __global__ __launch_bounds__(256) void k_panel_global(float *__restrict__ a, int
 __global__ __launch_bounds__(256) void k_laswp(float *__restrict__ a, int lda, int n, int kb, int nbk,
                                                const int32_t *__restrict__ ipiv)
 {
+    // the block's pivots, read once (nbk <= NB)
+    __shared__ int s_p[NB];
+    if (threadIdx.x < nbk) s_p[threadIdx.x] = ipiv[kb + threadIdx.x];
+    __syncthreads();
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
     const int k1 = kb + nbk;
     const int k0 = (c >= kb && c < k1) ? kb + ((c - kb) / PB + 1) * PB : kb;
     for (int j = k0; j < k1; ++j) {
-        const int pj = ipiv[j];
+        const int pj = s_p[j - kb];
         if (pj != j) {
             const float x = a[(int64_t)j * lda + c];
             a[(int64_t)j * lda + c] = a[(int64_t)pj * lda + c];
@@ -371,9 +375,13 @@ __global__ __launch_bounds__(256) void k_swap_trsm(float *__restrict__ a, int ld
             a[(int64_t)pj * lda + c] = x;
         }
     }
+    // all W loads in flight together (rows clamped into the block); rows
+    // past jb are cleared after
     float u[W];
 #pragma unroll
-    for (int i = 0; i < W; ++i) u[i] = i < jb ? a[(int64_t)(k + i) * lda + c] : 0.0f;
+    for (int i = 0; i < W; ++i) u[i] = a[(int64_t)(k + min(i, jb - 1)) * lda + c];
+#pragma unroll
+    for (int i = 0; i < W; ++i) u[i] = i < jb ? u[i] : 0.0f;
 #pragma unroll
     for (int i = 1; i < W; ++i)
 #pragma unroll
@@ -384,43 +392,91 @@ __global__ __launch_bounds__(256) void k_swap_trsm(float *__restrict__ a, int ld
 }
 
 // U12 = L11^-1 A12 for the trailing columns [c0, c1) of a whole block: L11
-// the unit lower nbk x nbk block at (kb, kb) (nbk <= W), staged in LDS; one
-// thread per column, rows in registers, each row's sum as four interleaved
-// partial sums.
-template <int W>
+// the unit lower nbk x nbk block at (kb, kb) (nbk <= 64), staged negated in
+// LDS.  One wave per 16 columns, its 64 x 16 block of A12 in LDS, solved by
+// 16-row blocks p = 0..3: the diagonal 16 x 16 triangle by substitution
+// (lane = column, each row's sum in ascending column order), then every
+// later block q takes X_q += (-L_qp) X_p as four v_mfma_f32_16x16x4f32
+// (lane layout as in the trailing update below).
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+constexpr int TW = 64;   // block rows
+constexpr int TXP = 17;  // pitch of a wave's 64 x 16 block in LDS
 __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int lda, int kb, int nbk, int c0, int c1)
 {
-    __shared__ __attribute__((aligned(16))) float s_l[W][W];
-    const int t = threadIdx.x;
-    for (int e = t; e < W * W; e += 256) {
-        const int i = e / W, j = e % W;
-        s_l[i][j] = (i < nbk && j < i) ? a[(int64_t)(kb + i) * lda + kb + j] : 0.0f;
+    __shared__ __attribute__((aligned(16))) float s_nl[TW][TW + 4];  // -L11, strictly lower
+    __shared__ float s_x[4][TW * TXP];
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    // every load is issued from a clamped, valid address before the first
+    // LDS write (a predicated load would wait out its own round trip)
+    float ld[TW * TW / 256];
+#pragma unroll
+    for (int it = 0; it < TW * TW / 256; ++it) {
+        const int e = t + 256 * it, i = e / TW, j = e % TW;
+        ld[it] = a[(int64_t)(kb + min(i, nbk - 1)) * lda + kb + min(j, nbk - 1)];
+    }
+    // this wave's columns cw .. cw + 15; lane moves column cw + (lane & 15)
+    // of rows (lane >> 4) + 4 q
+    const int cw = c0 + blockIdx.x * 64 + w * 16, lc = lane & 15, lr = lane >> 4;
+    const bool colin = cw + lc < c1;
+    float xv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        xv[q] = a[(int64_t)(kb + min(lr + 4 * q, nbk - 1)) * lda + min(cw + lc, c1 - 1)];
+#pragma unroll
+    for (int it = 0; it < TW * TW / 256; ++it) {
+        const int e = t + 256 * it, i = e / TW, j = e % TW;
+        s_nl[i][j] = (i < nbk && j < i) ? -ld[it] : 0.0f;
+    }
+    float *x = s_x[w];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int i = lr + 4 * q;
+        x[i * TXP + lc] = (colin && i < nbk) ? xv[q] : 0.0f;
     }
     __syncthreads();
-    const int c = c0 + blockIdx.x * 256 + t;
-    if (c >= c1) return;
-    float u[W];
+    const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int i = 0; i < W; ++i) u[i] = i < nbk ? a[(int64_t)(kb + i) * lda + c] : 0.0f;
+    for (int p = 0; p < 4; ++p) {
+        // diagonal triangle: every lane solves column li (four copies),
+        // lanes 0..15 write it back
+        float v[16];
 #pragma unroll
-    for (int i = 1; i < W; ++i) {
-        // four partial sums (j mod 4) keep four dependent chains in flight
-        float p4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < 16; ++i) v[i] = x[(16 * p + i) * TXP + li];
 #pragma unroll
-        for (int j = 0; j < i; ++j) p4[j & 3] += s_l[i][j] * u[j];
-        u[i] = u[i] - ((p4[0] + p4[1]) + (p4[2] + p4[3]));
-        // row i is final: later rows' LDS reads may not move above it
-        // (hoisting all of them spills)
-        __asm__ volatile("" : "+v"(u[i])::"memory");
+        for (int i = 1; i < 16; ++i)
+#pragma unroll
+            for (int j = 0; j < i; ++j) v[i] = __builtin_fmaf(s_nl[16 * p + i][16 * p + j], v[j], v[i]);
+        if (lane < 16)
+#pragma unroll
+            for (int i = 1; i < 16; ++i) x[(16 * p + i) * TXP + li] = v[i];
+        __syncthreads();
+        // X_q += (-L_qp) X_p for the later blocks: A[i][k] = -L[16q + i][16p + k],
+        // B[k][j] = X[16p + k][j], D[i][j] = X[16q + i][j]
+#pragma unroll
+        for (int q = p + 1; q < 4; ++q) {
+            float4v acc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[r] = x[(16 * q + 4 * lk + r) * TXP + li];
+#pragma unroll
+            for (int ks = 0; ks < 16; ks += 4)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(s_nl[16 * q + li][16 * p + ks + lk],
+                                                           x[(16 * p + ks + lk) * TXP + li], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) x[(16 * q + 4 * lk + r) * TXP + li] = acc[r];
+        }
+        __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < W; ++i)
-        if (i < nbk) a[(int64_t)(kb + i) * lda + c] = u[i];
+    for (int q = 0; q < 16; ++q) {
+        const int i = lr + 4 * q;
+        if (colin && i < nbk) a[(int64_t)(kb + i) * lda + cw + lc] = x[i * TXP + lc];
+    }
 }
 
 // ------------------------------------------------- MFMA trailing update
 
-typedef float float4v __attribute__((ext_vector_type(4)));
 
 // A[r0..n, c0..c1) -= L21 (rows r0..n, cols k..k+jb) x U12 (rows k..k+jb,
 // cols c0..c1), jb <= KW.  Workgroup tile (64 WGR) x (64 WGC), one wave per
@@ -431,9 +487,9 @@ typedef float float4v __attribute__((ext_vector_type(4)));
 // B[k][j]: j = lane%16, k = lane/16; D[i][j]: j = lane%16, i = 4*(lane/16)+r.
 // The product is formed transposed (A = U12^T, B = -L21^T, D = C^T), so a
 // lane's four accumulator entries are four consecutive columns of one C row
-// and move as one 16-byte access.  Range checks are per 16 x 16 tile and
-// uniform across the wave; only tiles cut by the matrix edge take the
-// per-entry path.  VEC: lda, the base, k and c0 allow 16-byte accesses.
+// and move as one 16-byte access.  Tiles wholly outside the matrix skip
+// their MFMAs (a uniform test).  vec: lda, the base, k and c0 allow 16-byte
+// accesses.
 template <int KW, int WGR, int WGC>
 __global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restrict__ a, int lda, int n, int k, int jb,
                                                                int r0g, int c0g, int c1, bool vec)
@@ -453,46 +509,77 @@ __global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restric
     int64_t roff[4];
 #pragma unroll
     for (int ti = 0; ti < 4; ++ti) roff[ti] = (int64_t)min(rw + ti * 16 + li, n - 1) * lda;
-    auto tile_full = [&](int ti, int tj) { return vec && rw + ti * 16 + 16 <= n && cw + tj * 16 + 16 <= c1; };
     auto tile_live = [&](int ti, int tj) { return rw + ti * 16 < n && cw + tj * 16 < c1; };
+    // with 16-byte rows and c0, c1 multiples of 4, a column quad lies wholly
+    // inside or outside the matrix: every quad is loaded from a clamped
+    // address, all loads in flight at once.  An outside quad's entries are
+    // never stored, and an accumulator entry depends on its own C entry
+    // only, so they need no clearing.
+    const bool quads = vec && c1 % 4 == 0;  // uniform
     float4v acc[4][4];
+    if (quads) {
 #pragma unroll
-    for (int ti = 0; ti < 4; ++ti)
+        for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < 4; ++tj) {
-            const int row = rw + ti * 16 + li, col = cw + tj * 16 + 4 * lk;
-            if (tile_full(ti, tj)) {
-                acc[ti][tj] = *(const float4v *)(a + roff[ti] + col);
-            } else {
+            for (int tj = 0; tj < 4; ++tj)
+                acc[ti][tj] = *(const float4v *)(a + roff[ti] + min(cw + tj * 16 + 4 * lk, c1 - 4));
+    } else {
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj) {
+                const int row = rw + ti * 16 + li, col = cw + tj * 16 + 4 * lk;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) acc[ti][tj][r] = (row < n && col + r < c1) ? a[roff[ti] + col + r] : 0.0f;
             }
-        }
+    }
 
-    // stage the L21 tile (negated) and the U12 tile, four entries a thread
-    for (int e = t; e < TR * KW / 4; e += NT) {
-        const int i = e / (KW / 4), kk = 4 * (e % (KW / 4)), gi = r0 + i;
-        const float *src = a + (int64_t)min(gi, n - 1) * lda + k + kk;
-        float4v v;
-        if (vec && gi < n && kk + 4 <= jb) {
-            v = *(const float4v *)src;
-        } else {
+    // stage the L21 tile (negated) and the U12 tile, four entries a thread.
+    // With 16-byte rows and a full K, every load is issued (rows and column
+    // quads clamped into the matrix) before the first LDS write, and the
+    // entries out of range are zeroed after; otherwise entry by entry.
+    constexpr int NQA = TR * KW / 4 / NT, NQB = KW * TC / 4 / NT;
+    static_assert(NQA * NT * 4 == TR * KW && NQB * NT * 4 == KW * TC, "staging split");
+    if (vec && jb == KW && c1 % 4 == 0) {  // uniform
+        float4v va[NQA], vb[NQB];
+#pragma unroll
+        for (int it = 0; it < NQA; ++it) {
+            const int e = t + NT * it, i = e / (KW / 4), kk = 4 * (e % (KW / 4));
+            va[it] = *(const float4v *)(a + (int64_t)min(r0 + i, n - 1) * lda + k + kk);
+        }
+#pragma unroll
+        for (int it = 0; it < NQB; ++it) {
+            const int e = t + NT * it, kk = e / (TC / 4), j = 4 * (e % (TC / 4));
+            // c0 and c1 are multiples of 4: a quad is wholly in or out
+            vb[it] = *(const float4v *)(a + (int64_t)(k + kk) * lda + min(c0 + j, c1 - 4));
+        }
+#pragma unroll
+        for (int it = 0; it < NQA; ++it) {
+            const int e = t + NT * it, i = e / (KW / 4), kk = 4 * (e % (KW / 4));
+            *(float4v *)(s_a + i * PA + kk) = r0 + i < n ? -va[it] : (float4v){0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int it = 0; it < NQB; ++it) {
+            const int e = t + NT * it, kk = e / (TC / 4), j = 4 * (e % (TC / 4));
+            *(float4v *)(s_b + kk * PBP + j) = c0 + j < c1 ? vb[it] : (float4v){0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    } else {
+        for (int e = t; e < TR * KW / 4; e += NT) {
+            const int i = e / (KW / 4), kk = 4 * (e % (KW / 4)), gi = r0 + i;
+            const float *src = a + (int64_t)min(gi, n - 1) * lda + k + kk;
+            float4v v;
 #pragma unroll
             for (int r = 0; r < 4; ++r) v[r] = (gi < n && kk + r < jb) ? src[r] : 0.0f;
+            *(float4v *)(s_a + i * PA + kk) = -v;
         }
-        *(float4v *)(s_a + i * PA + kk) = -v;
-    }
-    for (int e = t; e < KW * TC / 4; e += NT) {
-        const int kk = e / (TC / 4), j = 4 * (e % (TC / 4)), gj = c0 + j;
-        const float *src = a + (int64_t)(k + min(kk, jb - 1)) * lda + gj;
-        float4v v;
-        if (vec && kk < jb && gj + 4 <= c1) {
-            v = *(const float4v *)src;
-        } else {
+        for (int e = t; e < KW * TC / 4; e += NT) {
+            const int kk = e / (TC / 4), j = 4 * (e % (TC / 4)), gj = c0 + j;
+            const float *src = a + (int64_t)(k + min(kk, jb - 1)) * lda + gj;
+            float4v v;
 #pragma unroll
             for (int r = 0; r < 4; ++r) v[r] = (kk < jb && gj + r < c1) ? src[r] : 0.0f;
+            *(float4v *)(s_b + kk * PBP + j) = v;
         }
-        *(float4v *)(s_b + kk * PBP + j) = v;
     }
     __syncthreads();
 
@@ -512,19 +599,25 @@ __global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restric
                     acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[tj], fa[ti], acc[ti][tj], 0, 0, 0);
     }
 
+    if (quads) {
 #pragma unroll
-    for (int ti = 0; ti < 4; ++ti)
+        for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < 4; ++tj) {
-            const int row = rw + ti * 16 + li, col = cw + tj * 16 + 4 * lk;
-            if (tile_full(ti, tj)) {
-                *(float4v *)(a + roff[ti] + col) = acc[ti][tj];
-            } else {
+            for (int tj = 0; tj < 4; ++tj) {
+                const int row = rw + ti * 16 + li, col = cw + tj * 16 + 4 * lk;
+                if (row < n && col < c1) *(float4v *)(a + roff[ti] + col) = acc[ti][tj];
+            }
+    } else {
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj) {
+                const int row = rw + ti * 16 + li, col = cw + tj * 16 + 4 * lk;
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
                     if (row < n && col + r < c1) a[roff[ti] + col + r] = acc[ti][tj][r];
             }
-        }
+    }
 }
 
 
@@ -1041,7 +1134,8 @@ void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
         check_launch();
         const int c0 = kb + nbk;
         if (c0 < n) {
-            k_trsm_block<NB><<<grid_for(n - c0, 256), 256, 0, s>>>(a, lda, kb, nbk, c0, n);
+            static_assert(NB == TW, "block TRSM rows");
+            k_trsm_block<<<(n - c0 + 63) / 64, 256, 0, s>>>(a, lda, kb, nbk, c0, n);
             check_launch();
             dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
             k_update_mfma<NB, 2, 2><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vec && kb % 4 == 0 && c0 % 4 == 0);
