@@ -48,11 +48,12 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 // and exchanging rows gj and piv swaps two labels; the rows go back to the
 // rows their labels name at the end.  Per column J, with one barrier: the
 // pivot is the first row of largest magnitude among labels >= gj (LAPACK
-// isamax; FP32 magnitudes compare exactly as bit patterns).  Candidates are
-// 64-bit keys (magnitude bits + 1) << 32 | (2^31 - 1 - label), so "larger
-// magnitude, then smaller row" is one unsigned compare.  Each wave finds its
-// best key (DPP reduction); the owner lane posts (key, which of its rows,
-// all its rows) in the wave's LDS slot.  After the barrier every wave
+// isamax; FP32 magnitudes compare exactly as bit patterns).  Each wave finds
+// its candidate with two 32-bit DPP reductions (largest magnitude, then the
+// smallest label holding it); the owner lane posts (key, which of its rows,
+// all its rows) in the wave's LDS slot, the key being the 64-bit
+// (magnitude bits + 1) << 32 | (2^31 - 1 - label), so that "larger
+// magnitude, then smaller row" is one unsigned compare.  After the barrier every wave
 // reduces the NT/64 slots over lanes (DPP), reads the pivot row from the
 // winning slot by address, swaps the two labels and updates the rows with
 // labels > gj.  Slots alternate between two sets by column parity, so a
@@ -71,13 +72,6 @@ template <int NT, int R>
 struct PanelShared {
     PanelCand<R> cand[2][NT / 64];
 };
-
-__device__ __forceinline__ uint64_t pivot_key(int mag_bits, int label)
-{
-    return ((uint64_t)(uint32_t)(mag_bits + 1) << 32) | (uint32_t)(0x7fffffff - label);
-}
-
-__device__ __forceinline__ int key_label(uint64_t key) { return 0x7fffffff - (int)(uint32_t)key; }
 
 // one DPP step of a max-reduction of 64-bit keys
 template <int CTRL>
@@ -103,19 +97,20 @@ __device__ __forceinline__ uint64_t group_max_key(uint64_t key)
     return key;
 }
 
-// wave-wide best key, uniform across the wave
-__device__ __forceinline__ uint64_t wave_max_key(uint64_t key)
+// wave-wide max / min of 32-bit values by the same four DPP steps (rows of
+// 16 lanes) and the four rows' results read out; uniform across the wave
+template <bool MAX>
+__device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t v)
 {
-    key = group_max_key<16>(key);
-    uint64_t best = ((uint64_t)__builtin_amdgcn_readlane((int)(key >> 32), 0) << 32) |
-                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, 0);
+    auto op = [](uint32_t x, uint32_t y) { return MAX ? (x > y ? x : y) : (x < y ? x : y); };
+    v = op(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = op(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = op(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x124, 0xF, 0xF, false));  // row_ror:4
+    v = op(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false));  // row_ror:8
+    uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
 #pragma unroll
-    for (int q = 16; q < 64; q += 16) {
-        const uint64_t o = ((uint64_t)__builtin_amdgcn_readlane((int)(key >> 32), q) << 32) |
-                           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, q);
-        best = o > best ? o : best;
-    }
-    return best;
+    for (int q = 16; q < 64; q += 16) r = op(r, (uint32_t)__builtin_amdgcn_readlane((int)v, q));
+    return r;
 }
 
 __device__ __forceinline__ void put_row(float *dst, const float (&v)[PB])
@@ -144,23 +139,31 @@ __device__ __forceinline__ void panel_step(float (&p)[R][PB], int (&lab)[R], Pan
     if (J >= jb) return;  // uniform: jb < 16 only for the last panel
     constexpr int par = J & 1;
     const int lane = t & 63, w = t >> 6, gj = k + J;
-    uint64_t best = 0;
-    int bsel = 0;
+    // the wave's candidate in two 32-bit reductions: the largest magnitude,
+    // as the bits of |x| + 1 (0 marks rows out of the search, NaN ranks
+    // above every number), then the smallest label holding it (LAPACK's
+    // isamax takes the first index of the maximum).  The candidate's key
+    // for the workgroup step is (magnitude + 1) << 32 | (2^31 - 1 - label).
+    uint32_t mag[R], mloc = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const uint64_t key =
-            (lab[r] >= gj && lab[r] < n) ? pivot_key(__float_as_int(fabsf(p[r][J])), lab[r]) : (uint64_t)0;
-        if (key > best) {
-            best = key;
-            bsel = r;
-        }
+        const bool live = (uint32_t)(lab[r] - gj) < (uint32_t)(n - gj);  // gj <= lab < n
+        mag[r] = live ? ((uint32_t)__float_as_int(p[r][J]) & 0x7fffffffu) + 1u : 0u;
+        mloc = mag[r] > mloc ? mag[r] : mloc;
     }
-    const uint64_t wbest = wave_max_key(best);
+    const uint32_t wmag = wave_reduce_u32<true>(mloc);
+    uint32_t lloc = 0xffffffffu;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lloc = (mag[r] == wmag && (uint32_t)lab[r] < lloc) ? (uint32_t)lab[r] : lloc;
+    const uint32_t wlab = wave_reduce_u32<false>(lloc);
     PanelCand<R> &mine = sh.cand[par][w];
-    if (wbest == 0) {
+    if (wmag == 0) {
         if (lane == 0) mine.key = 0;
-    } else if (best == wbest) {  // the owner lane (labels are distinct)
-        mine.key = wbest;
+    } else if (lloc == wlab) {  // the owner lane (labels are distinct)
+        int bsel = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) bsel = (uint32_t)lab[r] == wlab ? r : bsel;
+        mine.key = ((uint64_t)wmag << 32) | (uint32_t)(0x7fffffff - (int)wlab);
         mine.sel = bsel;
 #pragma unroll
         for (int r = 0; r < R; ++r) put_row(mine.v[r], p[r]);

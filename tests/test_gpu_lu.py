@@ -77,6 +77,29 @@ def test_mfma_update_layout_exact(ctx, n):
     np.testing.assert_array_equal(np.tril(lu, -1), np.tril(L0, -1))
 
 
+@pytest.mark.parametrize("n", [48, 300, 1100])
+def test_pivot_ties_take_the_first_row(ctx, n):
+    # A = L0 U0 with many unit multipliers: after j steps, column j holds
+    # l_ij u_jj, and |l_ij| = 1 ties the diagonal row with every such row.
+    # LAPACK's isamax takes the first row of largest magnitude, so no
+    # exchange may happen; the arithmetic is exact (integers and halves), so
+    # the factors come back bit-exact.  n = 1100 puts two rows on a lane.
+    rng = np.random.default_rng(7 + n)
+    L0 = np.tril(rng.choice([0.0, 1.0, -1.0, 0.5, -0.5], (n, n)), -1) + np.eye(n)
+    U0 = np.triu(rng.integers(-3, 4, (n, n)).astype(np.float64), 1) + np.diag(rng.choice([4.0, -8.0], n))
+    a = (L0 @ U0).astype(np.float32)
+    assert np.array_equal(a.astype(np.float64), L0 @ U0)
+    # nearly every column has rows tied with the pivot
+    ties = [(np.abs(np.tril(L0, -1)[:, j]) == 1.0).any() for j in range(n - 1)]
+    assert np.mean(ties) > 0.9
+    d = dev(a, ctx)
+    ipiv = ctx.lu_factor(d, n)
+    lu = d.host().reshape(n, n).astype(np.float64)
+    np.testing.assert_array_equal(ipiv.host(), np.arange(n))
+    np.testing.assert_array_equal(np.triu(lu), U0)
+    np.testing.assert_array_equal(np.tril(lu, -1), np.tril(L0, -1))
+
+
 def test_lu_solve_f32_matches_numpy(ctx):
     n = 300
     rng = np.random.default_rng(3)
