@@ -878,20 +878,28 @@ __device__ __forceinline__ int wait_done(int *done, int need, int *err)
     return d;
 }
 
-// row `row` of lu, columns j0 .. j0+63 (zero from js on, and for dead rows)
+// row `row` of lu, columns j0 .. j0+63 (zero from js on, and for dead rows).
+// The loads are unconditional (a dead lane reads row `any`, a live row) so
+// that all of them are in flight together; the zeros are selected after.
 template <bool VEC>
 __device__ __forceinline__ void load_row(f4 (&r)[16], const float *__restrict__ lu, int lda, int row, bool live,
-                                         int j0, int js)
+                                         int any, int j0, int js)
 {
-    const float *src = lu + (int64_t)row * lda + j0;
+    const float *src = lu + (int64_t)(live ? row : any) * lda + j0;
     if (VEC && js == TB) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) r[q] = live ? *(const f4 *)(src + 4 * q) : f4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int q = 0; q < 16; ++q) r[q] = *(const f4 *)(src + 4 * q);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) r[q] = live ? r[q] : f4{0.0f, 0.0f, 0.0f, 0.0f};
     } else {
 #pragma unroll
         for (int q = 0; q < 16; ++q)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) r[q][e] = (live && 4 * q + e < js) ? src[4 * q + e] : 0.0f;
+            for (int e = 0; e < 4; ++e) r[q][e] = src[min(4 * q + e, js - 1)];
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) r[q][e] = (live && 4 * q + e < js) ? r[q][e] : 0.0f;
     }
 }
 
@@ -930,12 +938,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
     f4 cur[16];
     if (tk > 0) {
         const int j = LOWER ? 0 : p.nb - 1;
-        load_row<VEC>(cur, p.lu, p.lda, row, live, j * TB, min(TB, p.n - j * TB));
+        load_row<VEC>(cur, p.lu, p.lda, row, live, i0, j * TB, min(TB, p.n - j * TB));
     }
     int avail = 0;
     for (int s = 0; s < tk; ++s) {
         const int j = LOWER ? s : p.nb - 1 - s;
         const int j0 = j * TB, js = min(TB, p.n - j0);
+        // the following block's row is in flight during this block's wait
+        // and products
+        f4 nxt[16];
+        if (s + 1 < tk) {
+            const int jn = LOWER ? s + 1 : p.nb - 2 - s;
+            load_row<VEC>(nxt, p.lu, p.lda, row, live, i0, jn * TB, min(TB, p.n - jn * TB));
+        }
         if (s >= avail) avail = wait_done(done, s + 1, p.err);
         float xv[NRW];
 #pragma unroll
@@ -963,11 +978,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
                     for (int q = 0; q < NRW; ++q) __asm__ volatile("" : "+v"(acc[q])::"memory");
             }
         }
-        // the next block's row is in flight while this wave waits for it
-        if (s + 1 < tk) {
-            const int jn = LOWER ? s + 1 : p.nb - 2 - s;
-            load_row<VEC>(cur, p.lu, p.lda, row, live, jn * TB, min(TB, p.n - jn * TB));
-        }
+        if (s + 1 < tk)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) cur[q] = nxt[q];
     }
     // x_b = D_b^-1 acc
     float x[NRW];
