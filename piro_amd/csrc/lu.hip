@@ -194,9 +194,11 @@ __device__ __forceinline__ void panel_step(float (&p)[R][PB], int (&lab)[R], Pan
     if (d != 0.0f) {
         const bool rcp = fabsf(d) >= __FLT_MIN__;
         const float rd = 1.0f / d;
+        // rows past n (labels >= n, never stored) are skipped: a wave whose
+        // register row r lies wholly past n branches over it
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            if (lab[r] > gj) {
+            if ((uint32_t)(lab[r] - gj - 1) < (uint32_t)(n - gj - 1)) {  // gj < lab < n
                 float l;
                 if (__builtin_expect(rcp, 1)) l = p[r][J] * rd;
                 else l = p[r][J] / d;
