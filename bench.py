@@ -142,8 +142,8 @@ def main() -> int:
                          "readings are calibrated against); 0 skips it")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are out of scope "
-                                   "(z-slab sharding is described in DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "one GPU only: the operator would shard into z-slabs with one halo plane "
+                                   "exchanged per step, which is not built", "n_gpus": args.gpus}))
         return 2
 
     import numpy as np

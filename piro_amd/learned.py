@@ -9,7 +9,7 @@
   checked call takes the plain result, counts as rejected when the
   surrogate's was more than `tol` (relative L2) away, and its defect
   (plain minus surrogate) corrects the calls that follow -- the plain
-  stepper stays the fallback and the judge.
+  stepper stays the fallback and the reference for every check.
 
 Index-space dynamics depend only on r = alpha*dt/h^2, so instances of any size
 with dt = r_max*h^2/alpha_max reproduce the per-cell map of the large run.

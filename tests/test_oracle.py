@@ -106,7 +106,8 @@ def test_gaussian_ic_matches_reference(case):
 def test_lap_nnz_closed_form(dims):
     nx, ny, nz = dims
     N = nx * ny * nz
-    # SURVEY-independent count: N + 2 * (number of neighbour pairs)
+    # counted independently of the assembly: every cell's diagonal plus two
+    # entries per neighbour pair
     pairs = (nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1)
     assert po.lib.po_lap_nnz(nx, ny, nz) == N + 2 * pairs
 
