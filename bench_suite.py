@@ -388,8 +388,59 @@ def bench_formats(args):
         ctx.reduce(op, a, b)
         t = sum(timed(ctx, lambda: ctx.reduce(op, a, b)) for _ in range(5)) / 5
         row(f"reduce {op} 64Mi floats", t, nb)
+    a.free()
+    b.free()
+
+    # 2-D operators on a 4096^2 grid: the Laplacian and the true merge
+    # I - dt*alpha*L (count, scan, fill), host wall with the outputs'
+    # allocation
+    s2 = 4096
+    n2 = s2 * s2
+    h2 = 1.0 / (s2 + 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    L2 = ctx.laplacian2d_csr(s2, np.float32(1.0 / (h2 * h2)))
+    ctx.sync()
+    row("2-D Laplacian CSR 4096^2, host wall", time.perf_counter() - t0, 0,
+        f"nnz {L2.nnz}; includes allocation")
+    I2 = DevCSR(ctx.from_host(np.arange(n2 + 1, dtype=np.int32)), ctx.from_host(np.arange(n2, dtype=np.int32)),
+                ctx.full(n2, 1.0))
+    ctx.sync()
+    t0 = time.perf_counter()
+    M2 = ctx.geam2(1.0, I2, np.float32(-0.1), L2)
+    ctx.sync()
+    row("geam2 merge I - dt*alpha*L 4096^2, host wall", time.perf_counter() - t0, 0,
+        f"nnz {M2.nnz}; count, scan and fill, includes allocation")
+    for m in (L2, I2, M2):
+        for arr in (m.row_ptr, m.col, m.val):
+            arr.free()
     print(json.dumps({"metric": "formats, assembly and reductions", "rows": len(rows)}), flush=True)
     ctx.close()
+
+    # host paths: the case driver on PIRO's own HeatEq case and the Paraview
+    # writer on a 128^3 scalar field
+    import ctypes as C
+    from piro_amd.core import case_run
+    cases = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "cases")
+    with tempfile.TemporaryDirectory() as td:
+        case_run(cases, "HeatEq.ini", td, -1)
+        t0 = time.perf_counter()
+        steps, _ = case_run(cases, "HeatEq.ini", td, -1)
+        row(f"case driver, PIRO's HeatEq case ({steps} steps, 10x10x31 cells, with its Paraview exports), "
+            "host wall", time.perf_counter() - t0, 0, "context, assembly, stepping and export")
+        pd = 130
+        field = np.random.default_rng(3).random(pd ** 3, dtype=np.float32)
+        names = (C.c_char_p * 1)(b"T")
+        types = (C.c_int * 1)(1)
+        ptrs = (C.c_void_p * 1)(field.ctypes.data)
+        dims = (C.c_int * 3)(pd, pd, pd)
+        t0 = time.perf_counter()
+        check(lib.piro_paraview_write(td.encode(), b"bench", 1, dims, None, None, 1, names, types, ptrs))
+        tw = time.perf_counter() - t0
+        nbytes = sum(os.path.getsize(os.path.join(r, f)) for r, _, fs in os.walk(os.path.join(td, "bench"))
+                     for f in fs)
+        row("Paraview VTH/VTI writer, 128^3 interior scalar field, host wall", tw, 0,
+            f"{nbytes / 1e6:.1f} MB of text, {nbytes / tw / 1e6:.0f} MB/s")
 
 
 # ---------------------------------------------------------------------- JLI
