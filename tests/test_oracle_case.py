@@ -77,6 +77,31 @@ def test_library_writer_rejects_bad_arguments(tmp_path):
     assert lib.piro_paraview_write(str(tmp_path).encode(), b"c", -1, dims, None, None, 0, None, None, None) == 1
 
 
+@pytest.mark.parametrize("dims", [(5, 6, 3), (21, 18, 67)])
+def test_library_writer_values_on_many_slabs(dims, tmp_path):
+    # a random field with special values, over enough z-slabs that the
+    # writer splits it between threads: the array text is the interior
+    # cells, x fastest, each spelled as the reference's stream spells a
+    # float, joined by single spaces in cell order
+    from piro_amd._lib import check, lib
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nz)
+    f = (rng.standard_normal(nx * ny * nz) * 10.0 ** rng.integers(-30, 30, nx * ny * nz)).astype(np.float32)
+    inner = f.reshape(nz, ny, nx)[1:-1, 1:-1, 1:-1]  # a view: the specials go to interior cells
+    for k, v in enumerate([0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, 1e-45, 3.4e38, 1e5]):
+        inner[np.unravel_index(k, inner.shape)] = v
+    names = (C.c_char_p * 1)(b"T")
+    types = (C.c_int * 1)(1)
+    ptrs = (C.c_void_p * 1)(f.ctypes.data)
+    check(lib.piro_paraview_write(str(tmp_path).encode(), b"c", 7, (C.c_int * 3)(*dims), None, None, 1, names,
+                                  types, ptrs))
+    text = (tmp_path / "c" / "mesh" / "level" / "00007_level_0.vti").read_text()
+    body = text.split('format="ascii">\n', 1)[1].split("</DataArray>", 1)[0]
+    g = f.reshape(nz, ny, nx)[1:-1, 1:-1, 1:-1].ravel()
+    assert body == " ".join(pc.fmt_float(v) for v in g)
+    assert "-inf" in body and "-nan" in body and "1.4013e-45" in body
+
+
 def test_fmt_float_spellings():
     cases = [(0.0, "0"), (-0.0, "-0"), (1.0 / 3, "0.333333"), (123456789.0, "1.23457e+08"), (1e-5, "1e-05"),
              (100000.0, "100000"), (1e6, "1e+06"), (float("inf"), "inf"), (float("-inf"), "-inf")]
