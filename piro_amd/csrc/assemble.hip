@@ -162,22 +162,40 @@ __global__ __launch_bounds__(256) void k_grad_csr(Dims d, int scheme, float vlo,
 __global__ __launch_bounds__(256) void k_lap2d(int n, float inv_h2, int32_t *__restrict__ rp,
                                                int32_t *__restrict__ col, float *__restrict__ val)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // a block's 256 rows own one contiguous run of entries: each row is
+    // built in LDS, then the run is written with consecutive lanes on
+    // consecutive entries
+    __shared__ int32_t s_col[5 * 256];
+    __shared__ float s_val[5 * 256];
     const int64_t N = (int64_t)n * n;
-    if (i >= N) return;
-    const int x = (int)(i % n), y = (int)(i / n);
-    // rows before i: 5 each minus the missing neighbours
-    const int64_t miss = 2 * (int64_t)y + (x > 0) + (int64_t)n * (y > 0) + (int64_t)x * ((y == 0) + (y == n - 1));
-    int64_t k = 5 * i - miss;
-    rp[i] = (int32_t)k;
-    if (i == N - 1) rp[N] = (int32_t)(5 * N - 4 * (int64_t)n);
-    const float vd = -4.0f * inv_h2;
-    const int32_t ii = (int32_t)i;
-    if (y > 0) { col[k] = ii - n; val[k++] = inv_h2; }
-    if (x > 0) { col[k] = ii - 1; val[k++] = inv_h2; }
-    col[k] = ii; val[k++] = vd;
-    if (x < n - 1) { col[k] = ii + 1; val[k++] = inv_h2; }
-    if (y < n - 1) { col[k] = ii + n; val[k] = inv_h2; }
+    // first entry of row r (r < N): 5 per earlier row minus the missing
+    // neighbours; the total is 5 N - 4 n
+    auto start = [&](int64_t r) {
+        const int x = (int)(r % n), y = (int)(r / n);
+        const int64_t miss = 2 * (int64_t)y + (x > 0) + (int64_t)n * (y > 0) + (int64_t)x * ((y == 0) + (y == n - 1));
+        return 5 * r - miss;
+    };
+    const int64_t i0 = (int64_t)blockIdx.x * 256, i = i0 + threadIdx.x;
+    const int64_t kb = start(i0), ke = i0 + 256 < N ? start(i0 + 256) : 5 * N - 4 * (int64_t)n;
+    if (i < N) {
+        const int x = (int)(i % n), y = (int)(i / n);
+        const int64_t k = start(i);
+        rp[i] = (int32_t)k;
+        if (i == N - 1) rp[N] = (int32_t)(5 * N - 4 * (int64_t)n);
+        const float vd = -4.0f * inv_h2;
+        const int32_t ii = (int32_t)i;
+        int o = (int)(k - kb);
+        if (y > 0) { s_col[o] = ii - n; s_val[o++] = inv_h2; }
+        if (x > 0) { s_col[o] = ii - 1; s_val[o++] = inv_h2; }
+        s_col[o] = ii; s_val[o++] = vd;
+        if (x < n - 1) { s_col[o] = ii + 1; s_val[o++] = inv_h2; }
+        if (y < n - 1) { s_col[o] = ii + n; s_val[o] = inv_h2; }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < (int)(ke - kb); e += 256) {
+        col[kb + e] = s_col[e];
+        val[kb + e] = s_val[e];
+    }
 }
 
 // SpMDSM.cl:20-31 intended semantics, one thread per row.
