@@ -82,19 +82,44 @@ __global__ __launch_bounds__(256) void k_geam2_fill(int n, float alpha, const in
                                                     const int32_t *__restrict__ rpd, int32_t *__restrict__ cd,
                                                     float *__restrict__ vd)
 {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    int i = rpa[r], ie = rpa[r + 1], j = rpb[r], je = rpb[r + 1], d = rpd[r];
-    while (i < ie || j < je) {
-        if (j >= je || (i < ie && ca[i] < cb[j])) { cd[d] = ca[i]; vd[d] = alpha * va[i]; ++i; }
-        else if (i >= ie || cb[j] < ca[i]) { cd[d] = cb[j]; vd[d] = beta * vb[j]; ++j; }
-        else {
-            float ta = alpha * va[i], tb = beta * vb[j];
-            cd[d] = ca[i];
-            vd[d] = ta + tb;
-            ++i; ++j;
+    // a block's 256 rows own one contiguous output run: when it fits
+    // G2CAP entries (uniform per block) the rows are merged into LDS and
+    // the run is written with consecutive lanes on consecutive entries;
+    // longer runs are written row by row
+    constexpr int G2CAP = 2048;
+    __shared__ int32_t s_c[G2CAP];
+    __shared__ float s_v[G2CAP];
+    const int r0 = blockIdx.x * 256, r = r0 + threadIdx.x;
+    const int d0 = rpd[r0], d1 = rpd[min(r0 + 256, n)];
+    const bool staged = d1 - d0 <= G2CAP;
+    if (r < n) {
+        int i = rpa[r], ie = rpa[r + 1], j = rpb[r], je = rpb[r + 1], d = rpd[r];
+        auto emit = [&](int c, float v) {
+            if (staged) {
+                s_c[d - d0] = c;
+                s_v[d - d0] = v;
+            } else {
+                cd[d] = c;
+                vd[d] = v;
+            }
+            ++d;
+        };
+        while (i < ie || j < je) {
+            if (j >= je || (i < ie && ca[i] < cb[j])) { emit(ca[i], alpha * va[i]); ++i; }
+            else if (i >= ie || cb[j] < ca[i]) { emit(cb[j], beta * vb[j]); ++j; }
+            else {
+                float ta = alpha * va[i], tb = beta * vb[j];
+                emit(ca[i], ta + tb);
+                ++i; ++j;
+            }
         }
-        ++d;
+    }
+    if (staged) {  // uniform
+        __syncthreads();
+        for (int e = threadIdx.x; e < d1 - d0; e += 256) {
+            cd[d0 + e] = s_c[e];
+            vd[d0 + e] = s_v[e];
+        }
     }
 }
 

@@ -372,6 +372,36 @@ def test_geam2_true_merge_bit_exact(ctx):
     assert val.tobytes() == R.val.tobytes()
 
 
+def random_sorted_csr(rng, n, lens, n_cols):
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(n_cols, k, replace=False)) for k in lens]).astype(np.int32)
+    val = rng.standard_normal(int(rp[-1])).astype(np.float32)
+    return po.CSR(rp, col, val)
+
+
+def test_geam2_merge_long_and_short_rows(ctx):
+    # rows of 0..40 entries in both operands: blocks whose merged run exceeds
+    # the LDS staging buffer write row by row, the others are staged
+    rng = np.random.default_rng(11)
+    n = 1500
+    lens_a = np.where(np.arange(n) < 700, rng.integers(0, 4, n), rng.integers(10, 41, n))
+    lens_b = np.where(np.arange(n) % 3 == 0, rng.integers(0, 41, n), rng.integers(0, 3, n))
+    A = random_sorted_csr(rng, n, lens_a, 300)
+    B = random_sorted_csr(rng, n, lens_b, 300)
+    from piro_amd.core import DevCSR
+    dA = DevCSR(dev(A.row_ptr, ctx), dev(A.col, ctx), dev(A.val, ctx))
+    dB = DevCSR(dev(B.row_ptr, ctx), dev(B.col, ctx), dev(B.val, ctx))
+    D = ctx.geam2(np.float32(0.75), dA, np.float32(-1.5), dB)
+    R = po.geam2(np.float32(0.75), A, np.float32(-1.5), B)
+    rp, col, val = D.host()
+    runs = np.diff(R.row_ptr[::256].tolist() + [int(R.row_ptr[-1])])
+    assert runs.max() > 2048 and runs.min() <= 2048
+    np.testing.assert_array_equal(rp, R.row_ptr)
+    np.testing.assert_array_equal(col, R.col)
+    assert val.tobytes() == R.val.tobytes()
+
+
 def test_jli_matches_oracle(ctx):
     dims = (10, 10, 12)
     N = 1200
