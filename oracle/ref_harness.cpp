@@ -9,8 +9,13 @@
 //   Piro::CSR::stencils::laplacian_{cd,uw,dw}   src/stencils.cpp:233-429
 //   Piro::CSR::stencils::gradient_{cd,uw,dw}_*  src/stencils.cpp:7-231
 //   Piro::DENSE::stencils::laplacian_{cd,uw,dw} src/stencils.cpp:431-571
+//   Piro::COO::stencils::laplacian_{cd,uw,dw}   src/stencils.cpp:572-795
+//   Piro::HT::stencils::laplacian_{cd,uw,dw}    src/stencils.cpp:797-993
+//     (+ the sethash loop of HT::laplacian,      src/operatorlaplacian.cpp:194-204)
 //   Piro::methods::sethash                      src/methods.cpp:74-118
 //   Piro::calculategaussian                     src/ic.cpp:16-27
+//   Piro::calculatecoulomb                      src/ic.cpp:29-32 (coordinates
+//     and std::stof parameter parsing as in     src/ic.cpp:62-79)
 //   Piro::bc::prepbc + Piro::paraview::writevth src/bc.cpp:80-105,
 //     (+ writevti, writefile)                    src/paraview.cpp:12-137,
 //                                                src/fileutilities.cpp:24-40
@@ -20,7 +25,12 @@
 //   piro_ref_harness grad <scheme> <dir 0|1|2> nx ny nz norm_hex out.bin
 //   piro_ref_harness dense <scheme> nx ny nz norm_hex out.bin  (N x N floats)
 //   piro_ref_harness ht   <scheme> nx ny nz norm_hex table_size out.bin
+//   piro_ref_harness coo  <scheme> nx ny nz norm_hex out.bin     (rows, cols, vals)
+//   piro_ref_harness htst <scheme> nx ny nz norm_hex table_size out.bin
+//     (HT stencil row_ptr, columns, values, then the table keys, values)
 //   piro_ref_harness gauss nx ny nz lx ly lz mx my mz sx sy sz out.bin
+//   piro_ref_harness coulomb nx ny nz lx ly lz cx cy cz Z e eps0 out.bin
+//     (Z, e, eps0 as text, parsed with std::stof like the reference)
 //   piro_ref_harness vtk  nx ny nz n_scal n_vec step in.bin out_dir
 //     in.bin: n_scal fields of nx*ny*nz floats, then n_vec fields of
 //     3*nx*ny*nz floats (raw little-endian); fields are named S0.., V0..
@@ -140,6 +150,66 @@ int main(int argc, char **argv)
                 Piro::methods::sethash(i * N + cd.columns[j], cd.values[j], ts, keys, vals);
         FILE *f = fopen(argv[8], "wb");
         put(f, keys); put(f, vals);
+        fclose(f);
+        return 0;
+    }
+    if (cmd == "coo" && argc == 8) {
+        int scheme = atoi(argv[2]);
+        std::vector<uint> n = {(uint)atoi(argv[3]), (uint)atoi(argv[4]), (uint)atoi(argv[5])};
+        set_mesh(n[0], n[1], n[2]);
+        Piro::CellData cd;
+        const float norm = from_hex(argv[6]);
+        if (scheme == 3) Piro::COO::stencils::laplacian_cd(cd, n, norm);
+        else if (scheme == 2) Piro::COO::stencils::laplacian_dw(cd, n, norm);
+        else Piro::COO::stencils::laplacian_uw(cd, n, norm);
+        FILE *f = fopen(argv[7], "wb");
+        put(f, cd.rows); put(f, cd.columns); put(f, cd.values);
+        fclose(f);
+        return 0;
+    }
+    if (cmd == "htst" && argc == 9) {
+        int scheme = atoi(argv[2]);
+        std::vector<uint> n = {(uint)atoi(argv[3]), (uint)atoi(argv[4]), (uint)atoi(argv[5])};
+        set_mesh(n[0], n[1], n[2]);
+        const int N = (int)(n[0] * n[1] * n[2]);
+        Piro::CellData cd;
+        cd.rowpointers.assign(N + 1, 0);  // as HT::laplacian does before the stencil
+        const float norm = from_hex(argv[6]);
+        if (scheme == 3) Piro::HT::stencils::laplacian_cd(cd, n, norm);
+        else if (scheme == 2) Piro::HT::stencils::laplacian_dw(cd, n, norm);
+        else Piro::HT::stencils::laplacian_uw(cd, n, norm);
+        const int ts = atoi(argv[7]);
+        std::vector<int> keys(ts, -1);
+        std::vector<float> vals(ts, 0.0f);
+        for (int i = 0; i < N; i++)
+            for (int j = cd.rowpointers[i]; j < cd.rowpointers[i + 1]; j++)
+                Piro::methods::sethash(i * N + cd.columns[j], cd.values[j], ts, keys, vals);
+        FILE *f = fopen(argv[8], "wb");
+        put(f, cd.rowpointers); put(f, cd.columns); put(f, cd.values);
+        put(f, keys); put(f, vals);
+        fclose(f);
+        return 0;
+    }
+    if (cmd == "coulomb" && argc == 15) {
+        std::vector<uint> n = {(uint)atoi(argv[2]), (uint)atoi(argv[3]), (uint)atoi(argv[4])};
+        std::vector<float> l = {(float)atof(argv[5]), (float)atof(argv[6]), (float)atof(argv[7])};
+        std::vector<float> center = {(float)atof(argv[8]), (float)atof(argv[9]), (float)atof(argv[10])};
+        float Z = std::stof(argv[11]);
+        double e = std::stof(argv[12]);
+        double eps0 = std::stof(argv[13]);
+        std::vector<float> out((size_t)n[0] * n[1] * n[2]);
+        std::vector<float> c(3);
+        for (int x = 0; x < (int)n[0]; ++x)
+            for (int y = 0; y < (int)n[1]; ++y)
+                for (int z = 0; z < (int)n[2]; ++z) {
+                    c[0] = x * l[0] / n[0];
+                    c[1] = y * l[1] / n[1];
+                    c[2] = z * l[2] / n[2];
+                    out[x + (size_t)y * n[0] + (size_t)z * n[0] * n[1]] =
+                        Piro::calculatecoulomb(c, center, Z, e, eps0);
+                }
+        FILE *f = fopen(argv[14], "wb");
+        put(f, out);
         fclose(f);
         return 0;
     }

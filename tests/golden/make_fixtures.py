@@ -12,8 +12,12 @@ Reference functions exercised (paths relative to the PIRO tree):
   CSR::stencils::laplacian_{cd,uw,dw}   src/stencils.cpp:233-429
   CSR::stencils::gradient_*             src/stencils.cpp:7-231
   DENSE::stencils::laplacian_*          src/stencils.cpp:431-571
+  COO::stencils::laplacian_*            src/stencils.cpp:572-795
+  HT::stencils::laplacian_* + sethash   src/stencils.cpp:797-993,
+                                        src/operatorlaplacian.cpp:194-204
   methods::sethash                      src/methods.cpp:74-118
   calculategaussian                     src/ic.cpp:16-27
+  calculatecoulomb                      src/ic.cpp:29-32, 62-79
   bc::prepbc + paraview::writevth/vti   src/bc.cpp:80-105, src/paraview.cpp:12-137
 
 * ``ref_vtk/<case>/``    -- the reference writer's .vth/.vti files for small
@@ -44,6 +48,17 @@ HT_CASES = [  # (scheme, padded dims, l0): Test/LUDecomp.ini is n=3 3 3, l=5
     (3, (5, 5, 5), 5.0),
     (1, (5, 5, 5), 5.0),
     (2, (6, 5, 4), 4.0),
+]
+# COO and HT stencil functions of the reference (their own copies of the
+# 7-point rows and the boundary 10.0 rule); HT tables sized as HT::laplacian
+COO_GRIDS = [(3, 3, 3), (5, 6, 7), (7, 5, 4), (6, 3, 5)]
+HTST_GRIDS = [(3, 3, 3), (5, 5, 5), (6, 5, 4), (4, 7, 3)]
+# (padded dims, l, centre, Z, e, epsilon_0): the first is the reference's
+# distribution.ini [Coulomb] section on a 30-long box; the second puts the
+# centre on a grid coordinate (r = 0, where the 1e-10 offset matters)
+COULOMB_CASES = [
+    ((8, 9, 10), (30.0, 30.0, 30.0), (15.0, 15.0, 15.0), "1", "8.854187817E-12", "1.602176634E-19"),
+    ((12, 7, 9), (1.0, 1.4, 0.9), (0.5, 0.6, 0.3), "2", "1.602176634E-19", "8.854187817E-12"),
 ]
 GAUSS_CASES = [
     ((8, 9, 10), (1.0, 1.2, 0.8), (0.5, 0.55, 0.4), (0.1, 0.2, 0.15)),
@@ -172,6 +187,32 @@ def main() -> int:
         arrays[key + "_keys"], arrays[key + "_vals"] = keys, vals
         meta["ht"].append({"key": key, "scheme": scheme, "dims": [nx, ny, nz], "norm_hex": f32_hex(norm),
                            "table_size": ts, "sparsecount": nnz, "l0": l0})
+    for scheme in (1, 2, 3):
+        for (nx, ny, nz) in COO_GRIDS:
+            norm = lap_norm(1.0, nx - 2)
+            r, c, v = run(["coo", scheme, nx, ny, nz, f32_hex(norm)], ["<i4", "<i4", "<f4"])
+            key = f"coo_s{scheme}_{nx}x{ny}x{nz}"
+            arrays[key + "_rows"], arrays[key + "_cols"], arrays[key + "_vals"] = r, c, v
+            meta.setdefault("coo", []).append({"key": key, "scheme": scheme, "dims": [nx, ny, nz],
+                                               "norm_hex": f32_hex(norm)})
+        for (nx, ny, nz) in HTST_GRIDS:
+            norm = lap_norm(1.0, nx - 2)
+            N = nx * ny * nz
+            nnz = 7 * N - 2 * (ny * nz + nx * nz + nx * ny)
+            ts = ht_table_size(nnz, N)
+            rp, c, v, keys, vals = run(["htst", scheme, nx, ny, nz, f32_hex(norm), ts],
+                                       ["<i4", "<i4", "<f4", "<i4", "<f4"])
+            key = f"htst_s{scheme}_{nx}x{ny}x{nz}"
+            for name, a in (("_rp", rp), ("_col", c), ("_val", v), ("_keys", keys), ("_vals", vals)):
+                arrays[key + name] = a
+            meta.setdefault("htst", []).append({"key": key, "scheme": scheme, "dims": [nx, ny, nz],
+                                                "norm_hex": f32_hex(norm), "table_size": ts})
+    for i, (dims, l, centre, Z, e, eps0) in enumerate(COULOMB_CASES):
+        (g,) = run(["coulomb", *dims, *l, *centre, Z, e, eps0], ["<f4"])
+        key = f"coulomb_{i}"
+        arrays[key] = g
+        meta.setdefault("coulomb", []).append({"key": key, "dims": list(dims), "l": list(l),
+                                               "center": list(centre), "Z": Z, "e": e, "epsilon_0": eps0})
     for i, (dims, l, mean, sigma) in enumerate(GAUSS_CASES):
         (g,) = run(["gauss", *dims, *l, *mean, *sigma], ["<f4"])
         key = f"gauss_{i}"

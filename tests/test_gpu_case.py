@@ -151,3 +151,15 @@ def test_bc_on_other_slot_is_printed_from_host(tmp_path):
     # vector stays as initialised
     run_both(tmp_path, "AdvEq.ini", {"TotalTime": 0.002},
              setup_edits={"property": "U U v v U U", "values": "0.5 0.0 7.0 7.0 0.0 0.0"})
+
+
+def test_heat_forward_euler_coulomb_ic(tmp_path):
+    # the Coulomb IC (the facade's host code, from distribution.ini's
+    # [Coulomb] section) stepped by forward Euler: bit-exact, so the device
+    # run starts from the same field as the restatement, which the reference's
+    # own calculatecoulomb pins (tests/test_oracle_case.py)
+    spec, field, ref, steps = run_both(tmp_path, "HeatEq.ini",
+                                       {"Time": 11, "filename": "Coulomb", "TotalTime": 0.005, "Save": 5})
+    assert steps >= 5
+    assert field.tobytes() == ref.tobytes()
+    assert np.all(np.isfinite(field)) and field.min() < -1e9

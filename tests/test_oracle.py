@@ -84,6 +84,34 @@ def test_hash_table_matches_reference(case):
     assert vals.tobytes() == REF[k + "_vals"].tobytes()
 
 
+@pytest.mark.parametrize("case", META["coo"], ids=lambda c: c["key"])
+def test_coo_stencils_match_reference(case):
+    # PIRO's COO::stencils are their own copy of the 7-point rows: the COO
+    # form (row of every entry, CSR order) must equal them
+    nx, ny, nz = case["dims"]
+    A = po.lap_csr(nx, ny, nz, case["scheme"], hexf(case["norm_hex"]))
+    k = case["key"]
+    np.testing.assert_array_equal(po.coo_rows(A), REF[k + "_rows"])
+    np.testing.assert_array_equal(A.col, REF[k + "_cols"])
+    assert A.val.tobytes() == REF[k + "_vals"].tobytes()
+
+
+@pytest.mark.parametrize("case", META["htst"], ids=lambda c: c["key"])
+def test_ht_stencils_and_table_match_reference(case):
+    # PIRO's HT::stencils (another copy of the rows) and the sethash loop of
+    # HT::laplacian over them
+    nx, ny, nz = case["dims"]
+    A = po.lap_csr(nx, ny, nz, case["scheme"], hexf(case["norm_hex"]))
+    k = case["key"]
+    np.testing.assert_array_equal(A.row_ptr, REF[k + "_rp"])
+    np.testing.assert_array_equal(A.col, REF[k + "_col"])
+    assert A.val.tobytes() == REF[k + "_val"].tobytes()
+    assert po.ht_table_size(A.nnz, nx * ny * nz) == case["table_size"]
+    keys, vals = po.ht_build(A, case["table_size"])
+    np.testing.assert_array_equal(keys, REF[k + "_keys"])
+    assert vals.tobytes() == REF[k + "_vals"].tobytes()
+
+
 def test_ht_table_size_ludecomp_closed_form():
     # Test/LUDecomp.ini: n=3 3 3 -> N=125, nnz=725; load = 3.3608*125^-0.5552-0.0143
     load = np.float32(3.3608 * 125 ** -0.5552 - 0.0143)
@@ -243,6 +271,74 @@ def test_geam3_concat_matches_dense(scheme):
         assert sorted(cols) == sorted(sum(srcs, []))
 
 
+# Hand-derived rows of the concatenation merge (csrgeam_3.cl:37-88): while
+# all three rows are non-empty take the smallest column, A (x) before B (y)
+# before C (z) on ties; then the rest of A, then of B, then of C.  Each entry
+# is (column, source direction); its value is G_d[row, col] * v_d[row].
+#   cd 4x5x6, row 104 = (0,1,5): Gx=[105] Gy=[100,108] Gz=[84] -> z84 ends the
+#     loop (Gz empty), tails x105, y100, y108
+#   cd row 116 = (0,4,5): Gx=[117] Gy=[112] Gz=[96] -> z96, x117, y112
+#   cd row 119 = (3,4,5): Gx=[118] Gy=[115] Gz=[99] -> z99, x118, y115
+#   cd row 25 = (1,1,1), interior: z5 y21 x24 x26 (Gx empty), then y29 z45
+#   The one-sided rows keep the central rows' presence tests with the self
+#   column substituted (stencils.cpp:81-231): uw = [d- if d > 0, self if
+#   d < n-1], dw = [self if d > 0, d+ if d < n-1].
+#   dw row 116 = (0,4,5): Gx=[117] Gy=[116] Gz=[116] -> y116 (Gy empty), tails
+#     x117, z116
+#   dw row 25: Gx=[25,26] Gy=[25,29] Gz=[25,45] -> ties at 25 go x, y, z; x26
+#     (Gx empty), tails y29 z45
+#   uw row 25: Gx=[24,25] Gy=[21,25] Gz=[5,25] -> z5 y21 x24 x25, tails y25 z25
+#   uw row 119 = (3,4,5): Gx=[118] Gy=[115] Gz=[99] -> z99, x118, y115
+#   cd 1x4x5 (Gx rows empty, the loop never runs): row 5 = (0,1,1): y4 y6 z1 z9;
+#     row 19 = (0,3,4): y18 z15
+#   cd 4x5x1 (Gz rows empty): row 5 = (1,1,0): x4 x6 y1 y9
+GEAM3_ROWS = [
+    (po.CD, (4, 5, 6), 104, [(84, 2), (105, 0), (100, 1), (108, 1)]),
+    (po.CD, (4, 5, 6), 116, [(96, 2), (117, 0), (112, 1)]),
+    (po.CD, (4, 5, 6), 119, [(99, 2), (118, 0), (115, 1)]),
+    (po.CD, (4, 5, 6), 25, [(5, 2), (21, 1), (24, 0), (26, 0), (29, 1), (45, 2)]),
+    (po.DW, (4, 5, 6), 116, [(116, 1), (117, 0), (116, 2)]),
+    (po.DW, (4, 5, 6), 25, [(25, 0), (25, 1), (25, 2), (26, 0), (29, 1), (45, 2)]),
+    (po.UW, (4, 5, 6), 25, [(5, 2), (21, 1), (24, 0), (25, 0), (25, 1), (25, 2)]),
+    (po.UW, (4, 5, 6), 119, [(99, 2), (118, 0), (115, 1)]),
+    (po.CD, (1, 4, 5), 5, [(4, 1), (6, 1), (1, 2), (9, 2)]),
+    (po.CD, (1, 4, 5), 19, [(18, 1), (15, 2)]),
+    (po.CD, (4, 5, 1), 5, [(4, 0), (6, 0), (1, 1), (9, 1)]),
+]
+
+
+@pytest.mark.parametrize("scheme,dims,row,expect", GEAM3_ROWS,
+                         ids=lambda v: str(v) if not isinstance(v, list) else "")
+def test_geam3_entry_order_hand_derived(scheme, dims, row, expect):
+    nx, ny, nz = dims
+    N = nx * ny * nz
+    g = np.float32(0.5)
+    G = [po.grad_csr(*dims, scheme, d, g) for d in range(3)]
+    rng = np.random.default_rng(7)
+    v3 = rng.uniform(0.5, 2.0, 3 * N).astype(np.float32)
+    D = po.geam3(*G, v3)
+    lo, hi = D.row_ptr[row], D.row_ptr[row + 1]
+    assert D.col[lo:hi].tolist() == [c for c, _ in expect]
+    want = []
+    for c, d in expect:
+        a, b = G[d].row_ptr[row], G[d].row_ptr[row + 1]
+        k = [j for j in range(a, b) if G[d].col[j] == c]
+        assert len(k) == 1
+        want.append(np.float32(G[d].val[k[0]] * v3[d * N + row]))
+    assert D.val[lo:hi].tobytes() == np.array(want, np.float32).tobytes()
+
+
+def test_geam3_merge_is_not_a_sorted_merge():
+    # 16 of the 120 rows of the cd 4x5x6 merge come out with columns out of
+    # ascending order (every one a boundary row whose z row empties first)
+    dims = (4, 5, 6)
+    G = [po.grad_csr(*dims, po.CD, d, np.float32(0.5)) for d in range(3)]
+    D = po.geam3(*G, np.ones(360, np.float32))
+    unsorted = [r for r in range(120)
+                if D.col[D.row_ptr[r]:D.row_ptr[r + 1]].tolist() != sorted(D.col[D.row_ptr[r]:D.row_ptr[r + 1]])]
+    assert len(unsorted) == 16 and {104, 116, 119} <= set(unsorted)
+
+
 def test_geam2_true_merge_dense():
     n = 6
     L = po.lap2d_csr(n, np.float32(49.0))
@@ -275,6 +371,55 @@ def test_jli_converges_to_direct_solve():
     xs = np.linalg.solve(M, b[inter].astype(np.float64))
     assert po.rel_l2(x[inter], xs) < 1e-5
     assert np.all(x[np.setdiff1d(np.arange(N), inter)] == 0)
+
+
+def test_jli_stopping_rule_closed_form():
+    # b is the highest interior sine mode s (zero on the boundary rows), an
+    # eigenvector of the FP32 operator: L s = lam s with lam = d + 2 o sum cos,
+    # d = -6/norm, o = 1/norm as stored.  With M = I - dt L, mu = 1 - dt lam,
+    # x0 = b and x <- x + w (b - M x), the update is
+    #   delta_k = w (1 - mu) (1 - w mu)^k b,
+    # so res_k = |delta_k| / sqrt(N) = c q^k, c = w (mu - 1) |b| / sqrt(N),
+    # q = |1 - w mu| (kernelprocess.cpp:195-203, extras.cpp:24-30).  The run
+    # stops at the first k with (res_k - res_{k-1})^2 < tol, res_{-1} = 0.
+    # tol sits at the geometric mean of that measure at K-1 and K, a factor
+    # 1/q = 5 from both, so FP32 rounding cannot move the stop.
+    n = 6
+    dims = (n + 2,) * 3
+    nx = n + 2
+    N = nx ** 3
+    norm = po.lap_norm(1.0, n)
+    A = po.lap_csr(*dims, po.CD, norm)
+    d, o = float(np.float32(-6.0 / float(norm))), float(np.float32(1.0 / float(norm)))
+    th = n * math.pi / (n + 1)
+    lam = d + 2.0 * o * 3.0 * math.cos(th)
+    w = 0.9
+    mu_target = 1.2 / w  # 1 - w mu = -0.2
+    dt = np.float32((mu_target - 1.0) / -lam)
+    mu = 1.0 - float(dt) * lam
+    q = abs(1.0 - w * mu)
+    assert 0.19 < q < 0.21
+    i = np.arange(nx)
+    s1 = np.sin(n * math.pi * i / (n + 1))
+    s1[0] = s1[-1] = 0.0
+    b = np.einsum("k,j,i->kji", s1, s1, s1).ravel().astype(np.float32)
+    c = w * (mu - 1.0) * float(np.linalg.norm(b.astype(np.float64))) / math.sqrt(N)
+    meas = [c * c] + [(c * q ** (k - 1) * (1.0 - q)) ** 2 for k in range(1, 40)]
+    # iterations executed: the stop comes at k = K - 1, where res is still
+    # 1e-3 of res_0, far above the FP32 rounding of x (6e-8 of |x|)
+    K = 5
+    tol = math.sqrt(meas[K - 2] * meas[K - 1])
+    x, it, res = po.jli(dims, A, dt, b, w, tol, 100)
+    assert it == K
+    assert res == pytest.approx(c * q ** (K - 1), rel=1e-3)
+    # the iterate: x_K = b/mu + (1 - w mu)^K (b - b/mu)
+    xs = b.astype(np.float64) / mu + (1.0 - w * mu) ** K * (b.astype(np.float64) - b / mu)
+    assert po.rel_l2(x, xs) < 1e-5
+    # plausible slips stop elsewhere: |res_k - res_{k-1}| < tol, or a
+    # comparison with res_0 instead of the previous residual
+    k_abs = next(k for k in range(40) if math.sqrt(meas[k]) < tol) + 1
+    k_first = next(k for k in range(1, 40) if (c * q ** k - c) ** 2 < tol or k == 39) + 1
+    assert k_abs != K and k_first != K
 
 
 # ----------------------------------------------------------- reductions/IC

@@ -224,3 +224,20 @@ def test_continuity_case_runs(tmp_path):
     steps, x, exports = pc.run(spec)
     assert steps == 3 and np.all(np.isfinite(x))
     assert len(exports) == 3
+
+
+def coulomb_cases():
+    with open(os.path.join(GOLD, "ref_digests.json")) as f:
+        return json.load(f)["coulomb"]
+
+
+@pytest.mark.parametrize("meta", coulomb_cases(), ids=lambda m: m["key"])
+def test_coulomb_ic_matches_reference(meta):
+    # the Coulomb initial condition (src/ic.cpp:29-32, 62-79) against PIRO's
+    # own calculatecoulomb, compiled from its sources (tests/golden fixture),
+    # with Z, e and epsilon_0 read as distribution.ini text
+    ref = np.load(os.path.join(GOLD, "ref_assembly.npz"))[meta["key"]]
+    ic = {"Coulomb": {"center": " ".join(repr(float(c)) for c in meta["center"]), "Z": meta["Z"],
+                      "e": meta["e"], "epsilon_0": meta["epsilon_0"]}}
+    v = pc.initial_condition("Coulomb", tuple(meta["dims"]), tuple(meta["l"]), ic)
+    assert v.tobytes() == ref.tobytes()
