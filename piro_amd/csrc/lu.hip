@@ -269,38 +269,44 @@ __global__ __launch_bounds__(NT) void k_panel(float *__restrict__ a, int lda, in
 }
 
 // generic (slow) panel for n beyond the register path: same arithmetic,
-// panel in global memory
+// panel in global memory.  The pivot key is the register panel's:
+// (bits(|x|) + 1) << 32 | (2^31 - 1 - row), so "larger magnitude, then
+// smaller row" is one unsigned compare, NaN ranks above every number and
+// every row gj <= row < n has a key >= 2^32: the chosen row is always in
+// [gj, n), whatever the column holds.
 __global__ __launch_bounds__(256) void k_panel_global(float *__restrict__ a, int lda, int n, int k, int jb,
                                                       int32_t *__restrict__ ipiv, int *__restrict__ info)
 {
-    __shared__ double s_mag[4];
-    __shared__ int s_row[4];
+    __shared__ uint64_t s_key[4];
     __shared__ int s_piv;
     __shared__ float s_prow[PB];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     for (int j = 0; j < jb; ++j) {
         const int gj = k + j;
-        double best = -1.0;
-        int brow = 0x7fffffff;
+        uint64_t best = 0;
         for (int gi = gj + t; gi < n; gi += 256) {
-            double m = fabs((double)a[(int64_t)gi * lda + gj]);
-            if (m > best) { best = m; brow = gi; }
+            const uint32_t m = ((uint32_t)__float_as_int(a[(int64_t)gi * lda + gj]) & 0x7fffffffu) + 1u;
+            const uint64_t key = ((uint64_t)m << 32) | (uint32_t)(0x7fffffff - gi);
+            best = key > best ? key : best;
         }
         for (int off = 32; off > 0; off >>= 1) {
-            double ob = __shfl_xor(best, off, 64);
-            int orow = __shfl_xor(brow, off, 64);
-            if (ob > best || (ob == best && orow < brow)) { best = ob; brow = orow; }
+            const uint32_t olo = (uint32_t)__shfl_xor((int)(uint32_t)best, off, 64);
+            const uint32_t ohi = (uint32_t)__shfl_xor((int)(uint32_t)(best >> 32), off, 64);
+            const uint64_t o = ((uint64_t)ohi << 32) | olo;
+            best = o > best ? o : best;
         }
-        if (lane == 0) { s_mag[w] = best; s_row[w] = brow; }
+        if (lane == 0) s_key[w] = best;
         __syncthreads();
         if (t == 0) {
-            double b = s_mag[0];
-            int br = s_row[0];
-            for (int q = 1; q < 4; ++q)
-                if (s_mag[q] > b || (s_mag[q] == b && s_row[q] < br)) { b = s_mag[q]; br = s_row[q]; }
+            uint64_t b = s_key[0];
+            for (int q = 1; q < 4; ++q) b = s_key[q] > b ? s_key[q] : b;
+            // b >= 2^32 (row gj is always a candidate); the clamp keeps the
+            // exchange in range even so
+            int br = b >> 32 ? 0x7fffffff - (int)(uint32_t)b : gj;
+            br = br < gj ? gj : (br >= n ? n - 1 : br);
             s_piv = br;
             ipiv[gj] = br;
-            if (b == 0.0 && *info == 0) *info = gj + 1;
+            if ((b >> 32) == 1 && *info == 0) *info = gj + 1;  // largest magnitude is zero
         }
         __syncthreads();
         const int piv = s_piv;
@@ -320,7 +326,8 @@ __global__ __launch_bounds__(256) void k_panel_global(float *__restrict__ a, int
                 const float av = a[(int64_t)gi * lda + gj];
                 float l = rcp ? av * rd : av / d;
                 a[(int64_t)gi * lda + gj] = l;
-                for (int c = j + 1; c < jb; ++c) a[(int64_t)gi * lda + k + c] -= l * s_prow[c];
+                for (int c = j + 1; c < jb; ++c)
+                    a[(int64_t)gi * lda + k + c] = __builtin_fmaf(-l, s_prow[c], a[(int64_t)gi * lda + k + c]);
             }
         __syncthreads();
     }

@@ -56,6 +56,25 @@ def test_lu_factor_reconstructs(ctx, n):
     assert agree > 0.95
 
 
+@pytest.mark.parametrize("n", [300, 4160])
+def test_lu_factor_nan_pivots_stay_in_range(ctx, n):
+    # one NaN in the first column: it is chosen as the first pivot (NaN ranks
+    # above every magnitude) and spreads through the trailing matrix, so every
+    # later pivot column is all NaN.  Every pivot must still name a row in
+    # [j, n) -- at n = 4160 the first panels (more than 4096 rows) take the
+    # global-memory panel kernel
+    print(f"case lu nan n={n}", flush=True)
+    rng = np.random.default_rng(n + 1)
+    a = rng.standard_normal((n, n)).astype(np.float32)
+    a[5, 0] = np.nan
+    d = dev(a, ctx)
+    ipiv = ctx.lu_factor(d, n).host()
+    j = np.arange(n)
+    assert np.all(ipiv >= j) and np.all(ipiv < n)
+    assert ipiv[0] == 5
+    assert np.isnan(d.host().reshape(n, n)[n - 1, n - 1])
+
+
 @pytest.mark.parametrize("n", [48, 160, 300])
 def test_mfma_update_layout_exact(ctx, n):
     # A = L0 U0 with L0 unit lower (entries in {0, +-1/4, +-1/2}) and U0
