@@ -290,6 +290,21 @@ int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz,
                    const float *d_val, int64_t nnz, float dt,
                    const float *d_b, float urf, float tol, int max_iter,
                    float *d_x, float *d_work, int *iters, float *res);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): piro_jli_solve with the same arguments and results
+ * (bit-identical), whose iteration state and partial sums live in a static
+ * device array of the library (about 1 MiB, held by each GPU the library
+ * runs on until the process ends) instead of context scratch.  It calls no
+ * allocating routine.  At most 2^25 = 33,554,432 rows (nx*ny*nz); larger
+ * grids return PIRO_EINVAL.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (from several contexts or threads) race. */
+int piro_jli_solve_noalloc(piro_ctx *ctx, int nx, int ny, int nz,
+                           const int32_t *d_row_ptr, const int32_t *d_col,
+                           const float *d_val, int64_t nnz, float dt,
+                           const float *d_b, float urf, float tol,
+                           int max_iter, float *d_x, float *d_work,
+                           int *iters, float *res);
 
 
 /* ------------------------------------------------------ DENSE format */

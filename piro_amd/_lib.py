@@ -91,6 +91,7 @@ _SIGS = {
     "piro_lu_solve_f32": [vp, i32, vp, i32, vp, i32, vp, i32],
     "piro_lu_solve_refine": [vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, f64, i32, P_i32, P_f64],
     "piro_jli_solve": [vp, i32, i32, i32, vp, vp, vp, i64, f32, vp, f32, f32, i32, vp, vp, P_i32, P_f32],
+    "piro_jli_solve_noalloc": [vp, i32, i32, i32, vp, vp, vp, i64, f32, vp, f32, f32, i32, vp, vp, P_i32, P_f32],
     "piro_batch_create": [vp, i32, P_i32, vp, P_f32, C.POINTER(vp)],
     "piro_batch_destroy": [vp],
     "piro_batch_info": [vp, P_i64, P_i64, P_i64],

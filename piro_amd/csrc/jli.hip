@@ -16,6 +16,7 @@
 #include "device_utils.hpp"
 
 #include <cmath>
+#include <mutex>
 
 using namespace piro;
 
@@ -92,21 +93,14 @@ __global__ __launch_bounds__(256) void k_copy(int64_t n, const float *__restrict
     if (i < n) b[i] = a[i];
 }
 
-}  // namespace
 
-extern "C" int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row_ptr, const int32_t *d_col,
-                              const float *d_val, int64_t nnz, float dt, const float *d_b, float urf, float tol,
-                              int max_iter, float *d_x, float *d_work, int *iters, float *res)
+// the solve over caller-chosen device state: st (one JliState), part (at
+// least N/256 + 2 doubles) and stage (JLI_STAGE doubles)
+void jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row_ptr, const int32_t *d_col,
+               const float *d_val, int64_t nnz, float dt, const float *d_b, float urf, float tol, int max_iter,
+               float *d_x, float *d_work, int *iters, float *res, JliState *st, double *part, double *stage)
 {
-    PIRO_API_BEGIN
-    ctx_of(ctx);
-    PIRO_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && (int64_t)nx * ny * nz < 0x7fffffffLL, "bad grid");
-    PIRO_REQUIRE(d_b && d_x && d_work && d_x != d_work && d_x != d_b && d_work != d_b, "bad buffers");
-    PIRO_REQUIRE(max_iter >= 0, "max_iter < 0");
     const int64_t N = (int64_t)nx * ny * nz;
-    JliState *st = (JliState *)ctx->scratch[0].get(sizeof(JliState));
-    double *part = (double *)ctx->scratch[2].get((size_t)(N / 256 + 2 + JLI_STAGE) * sizeof(double));
-    double *stage = part + (N / 256 + 2);
     k_jli_init<<<1, 1, 0, ctx->stream>>>(st);
     check_launch();
     // x0 = b (src/kernelprocess.cpp:146)
@@ -142,5 +136,79 @@ extern "C" int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32
     PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     if (iters) *iters = h.iters;
     if (res) *res = h.last_res;
+}
+
+void jli_check_args(int nx, int ny, int nz, const float *d_b, float *d_x, float *d_work, int max_iter)
+{
+    PIRO_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && (int64_t)nx * ny * nz < 0x7fffffffLL, "bad grid");
+    PIRO_REQUIRE(d_b && d_x && d_work && d_x != d_work && d_x != d_b && d_work != d_b, "bad buffers");
+    PIRO_REQUIRE(max_iter >= 0, "max_iter < 0");
+}
+
+// Device state of piro_jli_solve_noalloc: the JliState in the first 256
+// bytes, then the partial sums (JLI_NOALLOC_MAX_ROWS / 256 + 2 doubles), then
+// the JLI_STAGE first-stage sums.  Each device holds its own copy for the
+// life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device.
+constexpr int64_t JLI_NOALLOC_MAX_ROWS = int64_t(1) << 25;
+constexpr size_t JLI_NOALLOC_PARTS = JLI_NOALLOC_MAX_ROWS / 256 + 2;
+constexpr size_t JLI_NOALLOC_BYTES = 256 + (JLI_NOALLOC_PARTS + JLI_STAGE) * sizeof(double);
+static_assert(sizeof(JliState) <= 256, "state slot");
+static __device__ __align__(256) unsigned char scratch[JLI_NOALLOC_BYTES];
+
+constexpr int JLI_NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[JLI_NOALLOC_MAX_DEVICES];
+
+// the device address of this device's copy of `scratch`, looked up on the
+// first call on the device (the caller made the device current)
+unsigned char *noalloc_scratch(int device)
+{
+    PIRO_REQUIRE(device >= 0 && device < JLI_NOALLOC_MAX_DEVICES, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void *ptr = nullptr;
+        PIRO_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        PIRO_REQUIRE(ptr != nullptr, "hipGetSymbolAddress returned no address");
+        g_noalloc_ptr[device] = ptr;
+    }
+    return (unsigned char *)g_noalloc_ptr[device];
+}
+
+}  // namespace
+
+extern "C" int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row_ptr, const int32_t *d_col,
+                              const float *d_val, int64_t nnz, float dt, const float *d_b, float urf, float tol,
+                              int max_iter, float *d_x, float *d_work, int *iters, float *res)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    jli_check_args(nx, ny, nz, d_b, d_x, d_work, max_iter);
+    const int64_t N = (int64_t)nx * ny * nz;
+    JliState *st = (JliState *)ctx->scratch[0].get(sizeof(JliState));
+    double *part = (double *)ctx->scratch[2].get((size_t)(N / 256 + 2 + JLI_STAGE) * sizeof(double));
+    double *stage = part + (N / 256 + 2);
+    jli_solve(ctx, nx, ny, nz, d_row_ptr, d_col, d_val, nnz, dt, d_b, urf, tol, max_iter, d_x, d_work, iters, res,
+              st, part, stage);
+    PIRO_API_END
+}
+
+extern "C" int piro_jli_solve_noalloc(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row_ptr,
+                                      const int32_t *d_col, const float *d_val, int64_t nnz, float dt,
+                                      const float *d_b, float urf, float tol, int max_iter, float *d_x,
+                                      float *d_work, int *iters, float *res)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    jli_check_args(nx, ny, nz, d_b, d_x, d_work, max_iter);
+    PIRO_REQUIRE((int64_t)nx * ny * nz <= JLI_NOALLOC_MAX_ROWS,
+                 "piro_jli_solve_noalloc: more than 2^25 rows (its static device state holds the partial sums of "
+                 "at most 33554432 rows)");
+    unsigned char *base = noalloc_scratch(ctx->device);
+    JliState *st = (JliState *)base;
+    double *part = (double *)(base + 256);
+    double *stage = part + JLI_NOALLOC_PARTS;
+    jli_solve(ctx, nx, ny, nz, d_row_ptr, d_col, d_val, nnz, dt, d_b, urf, tol, max_iter, d_x, d_work, iters, res,
+              st, part, stage);
     PIRO_API_END
 }
