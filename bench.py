@@ -49,9 +49,23 @@ def load_traffic(variant_name):
     return None if rec is None else rec.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(budget_s: float = 15.0):
-    """The CPU oracle (C restatement of the reference's step) timed on this
-    host's cores on a bounded sample of the same 256^3 workload."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(x0_host, x_dev_host, n_dev_steps, budget_s: float = 12.0, one_thread_s: float = 4.0):
+    """The CPU oracle (C restatement of the reference's step) on the same
+    256^3 workload: (1) the parity and FP64-error check of the device's final
+    field -- the FP32 oracle and an FP64 twin run the same number of steps
+    from the same initial field; (2) the CPU baseline, timed on this host's
+    cores on a bounded sample of steps, with every thread and with one."""
     import numpy as np
     import oracle as po
 
@@ -61,26 +75,54 @@ def cpu_baseline(budget_s: float = 15.0):
     alpha = po.alpha_uniform(A.n_rows, SEED, 0.005, 0.02)
     A = po.row_scale(A, alpha)
     dt = np.float32(np.float32(0.15) * norm / np.float32(0.02))
-    x = po.ic_gaussian(DIMS, np.float32([1, 1, 1]), np.float32([0.5] * 3), np.float32([0.1] * 3), 1.0)
+    snap = np.zeros(A.n_rows, np.float32)
+    # (1) parity of the timed run's field
+    t0 = time.perf_counter()
+    ref32 = po.fe_csr_run(DIMS, A, dt, n_dev_steps, x0_host, snap)
+    ref64 = po.fe_csr_run_f64(DIMS, A, float(dt), n_dev_steps, x0_host, snap)
+    parity = {
+        "steps": n_dev_steps,
+        "bit_exact_vs_fp32_oracle": bool(x_dev_host.tobytes() == ref32.tobytes()),
+        "rel_l2_vs_fp32_oracle": po.rel_l2(x_dev_host, ref32),
+        "fp64_rel_l2": po.rel_l2(x_dev_host, ref64),
+        "fp32_oracle_fp64_rel_l2": po.rel_l2(ref32, ref64),
+        "seconds": time.perf_counter() - t0,
+    }
+    # (2) the baseline
+    x = x0_host.copy()
     y = np.empty_like(x)
     threads = int(po.lib.po_omp_threads())
-    t0 = time.perf_counter()
-    po.lib.po_fe_csr_step(nx, ny, nz, A.row_ptr, A.col, A.val, dt, x, None, y)
-    t1 = time.perf_counter() - t0
-    steps = max(1, min(3000, int(budget_s / max(t1, 1e-3))))
-    t0 = time.perf_counter()
-    for _ in range(steps):
+
+    def timed(budget):
+        t0 = time.perf_counter()
         po.lib.po_fe_csr_step(nx, ny, nz, A.row_ptr, A.col, A.val, dt, x, None, y)
-        x, y = y, x
-    el = time.perf_counter() - t0
-    return {
+        t1 = time.perf_counter() - t0
+        steps = max(1, min(3000, int(budget / max(t1, 1e-3))))
+        a, b = x, y
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            po.lib.po_fe_csr_step(nx, ny, nz, A.row_ptr, A.col, A.val, dt, a, None, b)
+            a, b = b, a
+        return steps, time.perf_counter() - t0
+
+    steps, el = timed(budget_s)
+    po.lib.po_set_omp_threads(1)
+    steps1, el1 = timed(one_thread_s)
+    po.lib.po_set_omp_threads(threads)
+    base = {
         "value": A.n_rows * steps / el,
         "unit": "cell-steps/s",
         "cores": threads,
         "kind": "port",
+        "label": "CPU restatement of PIRO's path",
         "sample": f"{steps} FE steps of the 256^3 CSR heat step (oracle/piro_oracle.c, OpenMP over rows, "
-                  f"{threads} threads), {el:.1f} s",
+                  f"{threads} threads), {el:.1f} s; one thread: {steps1} steps, {el1:.1f} s",
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+        "one_thread_value": A.n_rows * steps1 / el1,
     }
+    return base, parity
 
 
 def surrogate_leg(ctx):
@@ -158,6 +200,7 @@ def main() -> int:
     ctx.row_scale(A, alpha)
     dt = np.float32(np.float32(0.15) * np.float32(norm) / np.float32(0.02))
     x = ctx.ic_gaussian(DIMS, [1.0] * 3, [0.5] * 3, [0.1] * 3, 1.0)
+    x0_host = x.host()
     snap = ctx.apply_face_values(DIMS, x, [0.0] * 6)
     y = ctx.empty_like(x)
     if args.variant >= 0:
@@ -204,9 +247,9 @@ def main() -> int:
         src.free()
         dst.free()
 
-    base = None
+    base = parity = None
     if not args.no_cpu_baseline:
-        base = cpu_baseline()
+        base, parity = cpu_leg(x0_host, x.host(), args.warmup + args.steps)
     for arr in (A.row_ptr, A.col, A.val, x, y, snap):
         arr.free()
     sur = None if args.no_surrogate else surrogate_leg(ctx)
@@ -233,6 +276,8 @@ def main() -> int:
                      "frac": gbs / HBM_PEAK_GBS, "traffic": load_traffic(vname),
                      "algorithmic_bytes_per_launch": byts, "measured_copy_gbs": copy_gbs},
         "cpu_baseline": base,
+        "parity": parity,
+        "fp64_rel_l2": None if parity is None else parity["fp64_rel_l2"],
         "surrogate": sur,
         "field_finite": finite,
         "host_wall_s": t_host,

@@ -51,6 +51,8 @@ typedef struct piro_ctx piro_ctx;
 /* Replaces Piro::opencl_init (src/opencl.cpp:67-138): one device, one
  * in-order queue.  A fresh context owns a non-blocking HIP stream. */
 int piro_ctx_create(int device, piro_ctx **out);
+/* Returns PIRO_EINVAL (and destroys nothing) while batches or surrogates made
+ * on the context are alive: destroy those first. */
 int piro_ctx_destroy(piro_ctx *ctx);
 /* Use an external hipStream_t (e.g. torch's current stream); NULL restores
  * the context's own stream. */
@@ -282,7 +284,9 @@ int piro_reduce_diff_sumsq(piro_ctx *ctx, int64_t n, const float *d_x,
  * x += urf*c; res = sqrt(sum dx^2)/sqrt(N) } until (res-res_old)^2 < tol or
  * max_iter.  One fused kernel per iteration; the residual is reduced on the
  * device in FP64 (no per-iteration N-float read-back).  The stopping test is
- * evaluated every `check_every` iterations from one 8-byte read-back.
+ * evaluated on the device every iteration; the host reads its 8-byte state
+ * after iterations 1, 2, 4, 8, 16 and then every 16 (iterations queued past
+ * convergence do nothing).
  * d_x (out) and d_work are N floats each.  *iters receives the iteration
  * count, *res the last residual. */
 int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz,

@@ -77,6 +77,7 @@ def _load() -> C.CDLL:
         "po_lu_factor_f64": (C.c_int, [C.c_int, _f64p, _i32p]),
         "po_lu_solve_f64": (None, [C.c_int, _f64p, _i32p, C.c_int, _f64p]),
         "po_omp_threads": (C.c_int, []),
+        "po_set_omp_threads": (None, [C.c_int]),
         "po_lap2d_padded_nnz": (C.c_int64, [C.c_int, C.c_int]),
         "po_lap2d_padded_csr": (C.c_int, [C.c_int, C.c_int, C.c_float, _i32p, _i32p, _f32p]),
         "po_conv3d_fwd_f64": (None, [C.c_int] * 6 + [_f64p, _f64p, C.c_void_p, _f64p]),

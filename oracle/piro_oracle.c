@@ -379,14 +379,16 @@ int64_t po_geam2(int n_rows, float alpha,
  * excluded), each in increasing index order, flattened in list order 0..5
  * (extras.cpp:4-12).  List 0: x=nx-1, 1: x=0, 2: y=ny-1, 3: y=0, 4: z=0,
  * 5: z=nz-1. */
+int po_face_indices(int nx, int ny, int nz, int32_t *idx);
+
+/* the length of po_face_indices' list, counted by the same rule (a grid
+ * with a dimension below 3 still has face cells along its other axes) */
 int po_face_count(int nx, int ny, int nz)
 {
-    int64_t a = (int64_t)(nx - 2) * (ny - 2), b = (int64_t)(nx - 2) * (nz - 2),
-            c = (int64_t)(ny - 2) * (nz - 2);
-    if (nx < 3 || ny < 3 || nz < 3) return 0;
-    return (int)(2 * (a + b + c));
+    return po_face_indices(nx, ny, nz, NULL);
 }
 
+/* idx may be NULL to count only */
 int po_face_indices(int nx, int ny, int nz, int32_t *idx)
 {
     int64_t N = (int64_t)nx * ny * nz;
@@ -405,7 +407,10 @@ int po_face_indices(int nx, int ny, int nz, int32_t *idx)
             else if (kd == nz - 1 && xi && yi) which = 5;
             if (id == 0 && yi && zi) which = 1;
             else if (id == nx - 1 && yi && zi) which = 0;
-            if (which == list) idx[q++] = (int32_t)ind;
+            if (which == list) {
+                if (idx) idx[q] = (int32_t)ind;
+                ++q;
+            }
         }
     return q;
 }
@@ -823,6 +828,7 @@ void po_lu_solve_f64(int n, const double *lu, const int32_t *piv, int nrhs,
 /* number of OpenMP threads the parallel loops above use */
 #include <omp.h>
 int po_omp_threads(void) { return omp_get_max_threads(); }
+void po_set_omp_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 
 /* ---------------------------------------- 2-D padded operator (batched) */
 

@@ -112,6 +112,7 @@ void    po_csr_to_dense_f64(int n_rows, int n_cols, const int32_t *row_ptr,
                             const int32_t *col, const float *val, double *d);
 int     po_lu_factor_f64(int n, double *a, int32_t *piv);
 int     po_omp_threads(void);
+void    po_set_omp_threads(int n);
 int64_t po_lap2d_padded_nnz(int nx, int ny);
 int     po_lap2d_padded_csr(int nx, int ny, float norm, int32_t *row_ptr,
                             int32_t *col, float *val);

@@ -405,6 +405,7 @@ int piro_batch_create(piro_ctx *ctx, int n_inst, const int *h_dims, const float 
         throw;
     }
     *out = b;
+    ++ctx->live_children;
     PIRO_API_END
 }
 
@@ -420,6 +421,7 @@ int piro_batch_destroy(piro_batch *b)
     (void)hipFree(b->col);
     (void)hipFree(b->val);
     (void)hipFree(b->d_units);
+    --b->ctx->live_children;
     delete b;
     PIRO_API_END
 }

@@ -40,6 +40,9 @@ int piro_ctx_destroy(piro_ctx *ctx)
 {
     PIRO_API_BEGIN
     if (!ctx) return PIRO_OK;
+    PIRO_REQUIRE(ctx->live_children == 0,
+                 "context still has " + std::to_string(ctx->live_children) +
+                     " live batch/surrogate object(s): destroy them first");
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     for (auto &s : ctx->scratch) s.release();

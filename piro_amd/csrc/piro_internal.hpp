@@ -89,6 +89,9 @@ struct piro_ctx {
     // autotuned FE-step choice keyed by the last matrix shape
     int fe_variant = -1;
     int64_t fe_tuned_rows = -1, fe_tuned_nnz = -1;
+    // live piro_batch / piro_surrogate objects made on this context: it
+    // refuses to be destroyed under them
+    int live_children = 0;
 };
 
 namespace piro {

@@ -1209,6 +1209,7 @@ int piro_surrogate_create(piro_ctx *ctx, uint64_t seed, piro_surrogate **out)
         throw;
     }
     *out = s;
+    ++ctx->live_children;
     PIRO_API_END
 }
 
@@ -1222,6 +1223,7 @@ int piro_surrogate_destroy(piro_surrogate *s)
                     (void *)s->wd, (void *)s->act, (void *)s->gbuf, (void *)s->delta, (void *)s->dd,
                     (void *)s->partial, (void *)s->sse, (void *)s->kbox, (void *)s->frag})
         if (p) (void)hipFree(p);
+    --s->ctx->live_children;
     delete s;
     PIRO_API_END
 }

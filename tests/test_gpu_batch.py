@@ -122,3 +122,32 @@ def test_batch_autotune_keeps_results(ctx):
         s, e = b.offsets[i], b.offsets[i + 1]
         assert got[s:e].tobytes() == oracle_instance(d, a, dts[i], x0[s:e], 5).tobytes()
     b.close()
+
+
+def test_ctx_destroy_refuses_live_batch_and_surrogate():
+    # through the C ABI: a context with a live batch or surrogate is not
+    # destroyed (PIRO_EINVAL, the objects stay usable); once they are gone it is
+    import ctypes as C
+
+    from piro_amd._lib import lib
+    h = C.c_void_p()
+    assert lib.piro_ctx_create(0, C.byref(h)) == 0
+    dims = (C.c_int * 3)(10, 12, 1)
+    dt = (C.c_float * 1)(1e-3)
+    d_alpha = C.c_void_p()
+    assert lib.piro_malloc(h, 120 * 4, C.byref(d_alpha)) == 0
+    alpha = np.full(120, 0.01, np.float32)
+    assert lib.piro_memcpy_h2d(h, d_alpha, alpha.ctypes.data, 480) == 0
+    b = C.c_void_p()
+    assert lib.piro_batch_create(h, 1, dims, d_alpha, dt, C.byref(b)) == 0
+    s = C.c_void_p()
+    assert lib.piro_surrogate_create(h, 7, C.byref(s)) == 0
+    assert lib.piro_ctx_destroy(h) == 1
+    assert b"live" in lib.piro_last_error()
+    rows = C.c_int64()
+    assert lib.piro_batch_info(b, C.byref(rows), None, None) == 0 and rows.value == 120
+    assert lib.piro_batch_destroy(b) == 0
+    assert lib.piro_ctx_destroy(h) == 1
+    assert lib.piro_surrogate_destroy(s) == 0
+    assert lib.piro_free(h, d_alpha) == 0
+    assert lib.piro_ctx_destroy(h) == 0

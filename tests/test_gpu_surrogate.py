@@ -422,3 +422,38 @@ def test_accelerated_run_defect_correction(ctx):
     want = plain(y1)
     assert got.host().tobytes() == want.host().tobytes()
     sur.close()
+
+
+def test_fallback_rejects_a_corrupted_surrogate(ctx):
+    # fault injection: random weights in the output layer make every call far
+    # from 16 plain steps.  Three calls with calls 0 and 2 checked (no defect
+    # correction): both checks exceed 1e-3 and are rejected, and each checked
+    # call takes the plain stepper's result, so the run's field is exactly
+    # P(S(P(x0))) -- P = 16 plain CSR steps (the judge), S = one call
+    from piro_amd import learned
+    from piro_amd.core import Surrogate
+    run = learned.HeatRun(ctx, side=24, seed=9)
+    sur = Surrogate(ctx, seed=3)
+    p = sur.get_params()
+    o0, o1 = param_blocks()["w6"]
+    p[o0:o1] = np.random.default_rng(4).normal(0.0, 0.1, o1 - o0).astype(np.float32)
+    sur.set_params(p)
+    shape = (1, 24, 24, 24)
+    got, res = run.accelerated(sur, 48, k=16, check_every=2, tol=1e-3, correct=False)
+    print(f"corrupted surrogate: checks {res.check_errors}")
+    assert (res.calls, res.checks, res.rejected) == (3, 2, 2)
+    assert min(res.check_errors) > 1e-3
+
+    def plain(v):
+        out = v.clone()
+        ctx.fe_run_csr(run.dims, run.A, run.dt, 16, out, ctx.empty_like(out), run.snap)
+        return out
+
+    x1 = plain(run.u0)
+    x2 = sur.forward(shape, x1, run.alpha, 1.0)
+    want = plain(x2)
+    assert got.host().tobytes() == want.host().tobytes()
+    # a single checked call equals 16 plain steps bit for bit
+    one, r1 = run.accelerated(sur, 16, k=16, check_every=1, tol=1e-3, correct=False)
+    assert r1.rejected == 1 and one.host().tobytes() == x1.host().tobytes()
+    sur.close()
