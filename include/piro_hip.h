@@ -372,7 +372,7 @@ int piro_lu_solve_refine(piro_ctx *ctx, int n, const float *d_a, int lda,
 /* ------------------------------------------------------- batched stepping */
 
 /* No reference counterpart (PIRO advances one grid per process): many
- * independent heat instances stepped by ONE persistent launch per step.
+ * independent heat and advection instances stepped by ONE launch per step.
  * Instance i has padded dims h_dims[3i..3i+2] (nz == 1: the build's 2-D
  * 5-point operator [self,x-,x+,y-,y+], -4/h^2, 1/h^2; nz >= 3: the
  * reference's 3-D cd operator with its boundary rule), h = 1/(nx-2), its
@@ -385,6 +385,20 @@ typedef struct piro_batch piro_batch;
 int piro_batch_create(piro_ctx *ctx, int n_inst, const int *h_dims,
                       const float *d_alpha, const float *h_dt,
                       piro_batch **out);
+/* The same with a kind per instance: h_kind[i] = 0 heat (as above), or
+ * 1/2/3 advection by the reference's operator D = sum_d diag(v_d) G_d
+ * (replaces process::gradient + vector & operator=, src/operatoroverload.cpp:
+ * 102-123, kernelmethods::csrgeam src/kernelcsrgeam.cpp:9-86, with the
+ * gradient rows of CSR::stencils::gradient_{uw,dw,cd}_*, src/stencils.cpp:
+ * 7-231, for kinds 1 uw, 2 dw, 3 cd), norm 2/(nx-2) and the uniform velocity
+ * h_vel3[3i..3i+2] in the reference's sign convention (dU/dt = +D U); nz == 1
+ * gives the 2-D operator with an empty z gradient.  Advection rows ignore
+ * d_alpha.  Each advection instance steps bit-identically to the single-grid
+ * path (gradients, piro_geam3_fill, piro_fe_run_csr with zero faces). */
+int piro_batch_create_mixed(piro_ctx *ctx, int n_inst, const int *h_dims,
+                            const int *h_kind, const float *h_vel3,
+                            const float *d_alpha, const float *h_dt,
+                            piro_batch **out);
 /* destroy every batch before the context it was created on */
 int piro_batch_destroy(piro_batch *b);
 /* total rows, nnz and the n_inst+1 row offsets of the instances */
@@ -397,7 +411,15 @@ int piro_batch_operator(piro_batch *b, const int32_t **d_row_ptr,
  * (0 = 8 workgroups per CU) */
 int piro_batch_set_tiling(piro_batch *b, int rows_per_unit, int grid);
 int piro_batch_get_tiling(piro_batch *b, int *rows_per_unit, int *grid);
-/* time every tiling on these buffers and keep the fastest */
+/* the full schedule: chunk = -1 persistent with `grid` workgroups (as
+ * set_tiling); chunk >= 0 one workgroup per unit, runs of `chunk`
+ * consecutive units dealt to each XCD (0: hardware order; grid ignored).
+ * get_schedule reports the launch's workgroup count in *grid. */
+int piro_batch_set_schedule(piro_batch *b, int rows_per_unit, int grid,
+                            int chunk);
+int piro_batch_get_schedule(piro_batch *b, int *rows_per_unit, int *grid,
+                            int *chunk);
+/* time every schedule on these buffers and keep the fastest */
 int piro_batch_autotune(piro_batch *b, float *d_x, float *d_work,
                         int *best_rows_per_unit, int *best_grid,
                         float *best_ms);

@@ -436,8 +436,8 @@ class Context:
         return x, sweeps.value, list(rel)
 
     # ------------------------------------------------------------- batch
-    def batch(self, dims_list, alpha, dts):
-        return Batch(self, dims_list, alpha, dts)
+    def batch(self, dims_list, alpha, dts, kinds=None, vel=None):
+        return Batch(self, dims_list, alpha, dts, kinds, vel)
 
     # ---------------------------------------------------------------- JLI
     def jli_solve(self, dims, A: DevCSR, dt, b, urf=1.0, tol=1e-5, max_iter=1000):
@@ -450,15 +450,24 @@ class Context:
 
 
 class Batch:
-    """Many independent heat instances stepped by one launch per step."""
+    """Many independent heat and advection instances stepped by one launch
+    per step.  kinds[i]: 0 heat, 1/2/3 advection with uw/dw/cd gradients and
+    the velocity vel[i] (three values, the reference's sign convention)."""
 
-    def __init__(self, ctx: Context, dims_list, alpha: DevArray, dts):
+    def __init__(self, ctx: Context, dims_list, alpha: DevArray, dts, kinds=None, vel=None):
         self.ctx = ctx
         n = len(dims_list)
         flat = (C.c_int * (3 * n))(*[int(v) for d in dims_list for v in d])
         dt = (C.c_float * n)(*[float(v) for v in dts])
         h = C.c_void_p()
-        check(lib.piro_batch_create(ctx.h, n, flat, _p(alpha), dt, C.byref(h)))
+        if kinds is None:
+            check(lib.piro_batch_create(ctx.h, n, flat, _p(alpha), dt, C.byref(h)))
+        else:
+            kk = (C.c_int * n)(*[int(k) for k in kinds])
+            vv = None
+            if vel is not None:
+                vv = (C.c_float * (3 * n))(*[float(c) for v in vel for c in v])
+            check(lib.piro_batch_create_mixed(ctx.h, n, flat, kk, vv, _p(alpha), dt, C.byref(h)))
         self.h = h
         ctx._adopt(self)
         tot, nnz = C.c_int64(), C.c_int64()
@@ -487,6 +496,14 @@ class Batch:
         a, b = C.c_int(), C.c_int()
         check(lib.piro_batch_get_tiling(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def set_schedule(self, rows_per_unit, grid=0, chunk=-1):
+        check(lib.piro_batch_set_schedule(self.h, int(rows_per_unit), int(grid), int(chunk)))
+
+    def schedule(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(lib.piro_batch_get_schedule(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def autotune(self, x, work):
         rb, g, ms = C.c_int(), C.c_int(), C.c_float()

@@ -1,17 +1,26 @@
-// batch.hip -- many independent heat instances advanced by one launch per step.
+// batch.hip -- many independent heat and advection instances advanced by one
+// launch per step.
 //
 // No reference counterpart (PIRO steps one grid per process).  A batch is the
-// block-diagonal CSR of all instances (2-D 5-point or the reference's 3-D
-// 7-point operator, each row-scaled by its diffusivity and pre-multiplied by
-// its own dt, which leaves every product fl(fl(dt*v)*x) unchanged), cut into
-// work units of RB consecutive rows that never straddle two instances.  One
-// persistent launch per step walks the unit list with a static stride, so a
-// captured hipGraph of N steps replays without any counter to reset.  Each
-// unit is processed like the single-grid kernel (non-zeros streamed into LDS,
-// per-row sequential sums), so every instance's field is bit-identical to
-// stepping it alone, whatever RB and the grid size the autotuner picks.
-// Boundary rows are zero (zero Dirichlet data); interior rows get
-// y = sum + x.
+// block-diagonal CSR of all instances, each instance's rows pre-multiplied by
+// its own dt (which leaves every product fl(fl(dt*v)*x) unchanged):
+//   heat: the 2-D 5-point or the reference's 3-D 7-point Laplacian
+//     (src/stencils.cpp:233-297, boundary 10.0 rule included), row-scaled by
+//     the diffusivity;
+//   advection: the reference's advection operator D = sum_d diag(v_d) G_d
+//     (src/operatoroverload.cpp:102-123, src/kernelcsrgeam.cpp:9-86) with
+//     uniform velocity: the gradient rows of src/stencils.cpp:7-231 (uw, dw or
+//     cd, norm 2 dx) in the concatenation-merge order of csrgeam_3.cl:37-88.
+// The rows are cut into work units of RB consecutive rows that never
+// straddle two instances.  A launch per step either walks the unit list with
+// a static stride (persistent) or gives each unit one workgroup, optionally
+// dealing runs of C consecutive units to each XCD so that units sharing x
+// lines sit behind the same L2.  Either way a captured hipGraph of N steps
+// replays without any counter to reset.  Each unit is processed like the
+// single-grid kernel (non-zeros streamed into LDS, per-row sequential sums),
+// so every instance's field is bit-identical to stepping it alone, whatever
+// schedule the autotuner picks.  Boundary rows are zero (zero Dirichlet
+// data); interior rows get y = sum + x.
 #include "device_utils.hpp"
 
 #include <algorithm>
@@ -46,13 +55,14 @@ struct piro_batch {
     int rb = 0;
     Unit *d_units = nullptr;
     int n_units = 0;
-    int grid = 0;
+    int grid = 0;   // persistent workgroups (0: the default, eight per CU)
+    int chunk = -1; // one workgroup per unit, runs of `chunk` units per XCD (0: plain order); -1: persistent
     // captured graph
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     const float *g_x = nullptr;
     float *g_w = nullptr;
-    int g_steps = -1, g_rb = 0, g_grid = 0;
+    int g_steps = -1, g_rb = 0, g_grid = 0, g_chunk = -2;
 };
 
 namespace {
@@ -68,11 +78,34 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint64_t m, int s) { return
 
 // ---------------------------------------------------------------- assembly
 
-// per-instance operator rows, dt folded in: val = fl(dt * fl(v * alpha_row))
+// one gradient row of direction d (src/stencils.cpp:7-231): (lower or self,
+// -1/norm) if c > 0, (upper or self, +1/norm) if c < nd - 1; uw puts self
+// in place of the upper neighbour, dw in place of the lower one
+__device__ __forceinline__ int grad_row(int c, int nd, int32_t stride, int scheme, int32_t gi, float vlo, float vhi,
+                                        int32_t (&col)[2], float (&val)[2])
+{
+    int n = 0;
+    if (c > 0) {
+        col[n] = scheme == PIRO_DW ? gi : gi - stride;
+        val[n] = vlo;
+        ++n;
+    }
+    if (c < nd - 1) {
+        col[n] = scheme == PIRO_UW ? gi : gi + stride;
+        val[n] = vhi;
+        ++n;
+    }
+    return n;
+}
+
+// per-instance operator rows, dt folded in.  Heat (kind 0):
+// val = fl(dt * fl(v * alpha_row)); advection (kind = gradient scheme 1..3):
+// val = fl(dt * fl(g * v_d)), g = +-1/norm the gradient entry
 __global__ __launch_bounds__(256) void k_batch_assemble(int n_inst, const InstDesc *__restrict__ inst,
                                                         const int64_t *__restrict__ nnz_off,
                                                         const float *__restrict__ alpha, const float *__restrict__ dt,
                                                         const float *__restrict__ vself, const float *__restrict__ vnb,
+                                                        const int *__restrict__ kind, const float *__restrict__ vel,
                                                         int64_t total_rows, int32_t *__restrict__ rp,
                                                         int32_t *__restrict__ col, float *__restrict__ val)
 {
@@ -85,6 +118,7 @@ __global__ __launch_bounds__(256) void k_batch_assemble(int n_inst, const InstDe
         if (inst[mid].row0 <= g) lo = mid; else hi = mid;
     }
     const InstDesc d = inst[lo];
+    const int kd = kind[lo];
     const int64_t i = g - d.row0;
     const int x = (int)(i % d.nx);
     const int64_t tq = i / d.nx;
@@ -92,6 +126,8 @@ __global__ __launch_bounds__(256) void k_batch_assemble(int n_inst, const InstDe
     const int z = (int)(tq / d.ny);
     const float a = alpha[g], h = dt[lo], vs = vself[lo], vn = vnb[lo];
     const int64_t sxy = (int64_t)d.nx * d.ny;
+    // entries in the instance's rows before this one: the Laplacian's (self
+    // plus one per in-grid neighbour); the advection rows lack the self entry
     int64_t k;
     if (d.nz == 1) {
         // rows before (x, y) of a 5-point operator: 5 per row minus missing
@@ -107,52 +143,89 @@ __global__ __launch_bounds__(256) void k_batch_assemble(int n_inst, const InstDe
         int64_t sz = sxy * (z > 0) + ((int64_t)y * d.nx + x) * mz;
         k = 7 * i - sx - sy - sz;
     }
+    if (kd != 0) k -= i;
     k += nnz_off[lo];
     rp[g] = (int32_t)k;
     if (g == total_rows - 1) rp[total_rows] = (int32_t)nnz_off[n_inst];
     const int32_t gi = (int32_t)g;
-    auto emit = [&](int32_t c, float v) {
+    auto emit = [&](int32_t c, float v, float f) {
         col[k] = c;
-        float sv = v * a;
+        float sv = v * f;
         val[k] = h * sv;
         ++k;
     };
-    if (d.nz == 1) {
+    if (kd != 0) {
+        // the concatenation merge of the x, y and z gradient rows: while all
+        // three are non-empty the smallest column, x before y before z on
+        // ties; then the rest of x, of y, of z
+        const float vlo = vs, vhi = vn;  // -1/norm, +1/norm of this instance
+        const float vx = vel[3 * lo], vy = vel[3 * lo + 1], vz = vel[3 * lo + 2];
+        int32_t ca[2], cb[2], cc[2];
+        float va[2], vb[2], vc[2];
+        const int na = grad_row(x, d.nx, 1, kd, gi, vlo, vhi, ca, va);
+        const int nb = grad_row(y, d.ny, d.nx, kd, gi, vlo, vhi, cb, vb);
+        const int nc = grad_row(z, d.nz, (int32_t)sxy, kd, gi, vlo, vhi, cc, vc);
+        int ia = 0, ib = 0, ic = 0;
+        while (ia < na && ib < nb && ic < nc) {
+            const int32_t mn = min(ca[ia], min(cb[ib], cc[ic]));
+            if (ca[ia] == mn) { emit(ca[ia], va[ia], vx); ++ia; }
+            else if (cb[ib] == mn) { emit(cb[ib], vb[ib], vy); ++ib; }
+            else { emit(cc[ic], vc[ic], vz); ++ic; }
+        }
+        for (; ia < na; ++ia) emit(ca[ia], va[ia], vx);
+        for (; ib < nb; ++ib) emit(cb[ib], vb[ib], vy);
+        for (; ic < nc; ++ic) emit(cc[ic], vc[ic], vz);
+    } else if (d.nz == 1) {
         // build-only 2-D operator: [self, x-, x+, y-, y+], -4/h^2 and 1/h^2
-        emit(gi, vs);
-        if (x > 0) emit(gi - 1, vn);
-        if (x < d.nx - 1) emit(gi + 1, vn);
-        if (y > 0) emit(gi - d.nx, vn);
-        if (y < d.ny - 1) emit(gi + d.nx, vn);
+        emit(gi, vs, a);
+        if (x > 0) emit(gi - 1, vn, a);
+        if (x < d.nx - 1) emit(gi + 1, vn, a);
+        if (y > 0) emit(gi - d.nx, vn, a);
+        if (y < d.ny - 1) emit(gi + d.nx, vn, a);
     } else {
         // the reference's cd operator including its boundary 10.0 rule
         // (src/stencils.cpp:233-297)
         auto bnd = [&](int xx, int yy, int zz) {
             return xx == 0 || xx == d.nx - 1 || yy == 0 || yy == d.ny - 1 || zz == 0 || zz == d.nz - 1;
         };
-        emit(gi, i == 0 ? 10.0f : vs);
-        if (x > 0) emit(gi - 1, vn);
-        if (x < d.nx - 1) emit(gi + 1, (z == 0 && y == 0) ? 10.0f : vn);
-        if (y > 0) emit(gi - d.nx, vn);
-        if (y < d.ny - 1) emit(gi + d.nx, z == 0 ? 10.0f : vn);
-        if (z > 0) emit((int32_t)(g - sxy), vn);
-        if (z < d.nz - 1) emit((int32_t)(g + sxy), bnd(x, y, z + 1) ? 10.0f : vn);
+        emit(gi, i == 0 ? 10.0f : vs, a);
+        if (x > 0) emit(gi - 1, vn, a);
+        if (x < d.nx - 1) emit(gi + 1, (z == 0 && y == 0) ? 10.0f : vn, a);
+        if (y > 0) emit(gi - d.nx, vn, a);
+        if (y < d.ny - 1) emit(gi + d.nx, z == 0 ? 10.0f : vn, a);
+        if (z > 0) emit((int32_t)(g - sxy), vn, a);
+        if (z < d.nz - 1) emit((int32_t)(g + sxy), bnd(x, y, z + 1) ? 10.0f : vn, a);
     }
 }
 
 // ------------------------------------------------------------------ step
+
+// Workgroups are dealt to the 8 XCDs round robin (workgroup b runs on XCD
+// b % 8).  chunk < 0: persistent, the workgroup takes units b, b + grid, ...;
+// chunk = 0: one unit, b; chunk C > 0: one unit, XCD j taking the runs
+// [(8q + j) C, (8q + j + 1) C) of consecutive units, so neighbouring units
+// (whose rows read each other's x) sit behind the same L2 while the device
+// still streams the operator front to back.
+__device__ __forceinline__ int first_unit(int chunk)
+{
+    const int b = blockIdx.x;
+    if (chunk <= 0) return b;
+    const int k = b >> 3;
+    return ((k / chunk) * 8 + (b & 7)) * chunk + k % chunk;
+}
 
 template <int RB>
 __global__ __launch_bounds__(256) void k_batch_step(const Unit *__restrict__ units, int n_units,
                                                     const InstDesc *__restrict__ inst,
                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
                                                     const float *__restrict__ val, const float *__restrict__ x,
-                                                    float *__restrict__ y)
+                                                    float *__restrict__ y, int chunk)
 {
     constexpr int CAP = RB * 8 + 8;
     __shared__ int32_t s_rp[RB + 1];
     __shared__ float s_p[CAP];
-    for (int u = blockIdx.x; u < n_units; u += gridDim.x) {
+    const int ustep = chunk < 0 ? (int)gridDim.x : n_units;  // one unit unless persistent
+    for (int u = first_unit(chunk); u < n_units; u += ustep) {
         const Unit un = units[u];
         const InstDesc d = inst[un.inst];
         __syncthreads();  // LDS reuse across units
@@ -208,14 +281,15 @@ __global__ __launch_bounds__(256) void k_batch_pf(const Unit *__restrict__ units
                                                   const InstDesc *__restrict__ inst,
                                                   const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
                                                   const float *__restrict__ val, int64_t nnz,
-                                                  const float *__restrict__ x, float *__restrict__ y)
+                                                  const float *__restrict__ x, float *__restrict__ y, int chunk)
 {
     constexpr int RPL = RB / 256;
     constexpr int CAP = QPL * 1024;
     __shared__ __attribute__((aligned(16))) float s_p[2][CAP];
     const int t = threadIdx.x;
     int buf = 0;
-    for (int u = blockIdx.x; u < n_units; u += gridDim.x, buf ^= 1) {
+    const int ustep = chunk < 0 ? (int)gridDim.x : n_units;  // one unit unless persistent
+    for (int u = first_unit(chunk); u < n_units; u += ustep, buf ^= 1) {
         const Unit un = units[u];
         const InstDesc d = inst[un.inst];
         const int64_t r0 = un.row0;
@@ -301,19 +375,34 @@ void build_units(piro_batch *b, int rb)
     b->rb = rb;
 }
 
+// the launch's workgroup count for the current schedule
+int launch_grid(const piro_batch *b)
+{
+    if (b->chunk < 0) return b->grid > 0 ? b->grid : std::min(b->n_units, b->ctx->num_cus * 8);
+    if (b->chunk == 0) return b->n_units;
+    const int span = 8 * b->chunk;
+    return (b->n_units + span - 1) / span * span;
+}
+
 void launch_step(piro_batch *b, const float *x, float *y, hipStream_t s)
 {
-    const int g = b->grid > 0 ? b->grid : std::min(b->n_units, b->ctx->num_cus * 8);
+    const int g = launch_grid(b), c = b->chunk;
     switch (b->rb) {
-    case 128: k_batch_step<128><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y); break;
+    case 128:
+        k_batch_step<128><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y, c);
+        break;
     // 7 entries per row at most, plus the quad alignment slack
     case 256:
-        k_batch_pf<256, 2><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x, y);
+        k_batch_pf<256, 2><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x, y,
+                                             c);
         break;
     case 512:
-        k_batch_pf<512, 4><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x, y);
+        k_batch_pf<512, 4><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x, y,
+                                             c);
         break;
-    default: k_batch_step<1024><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y); break;
+    default:
+        k_batch_step<1024><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y, c);
+        break;
     }
     check_launch();
 }
@@ -334,37 +423,50 @@ piro_batch *bat(piro_batch *b)
     return b;
 }
 
-}  // namespace
 
-extern "C" {
 
-int piro_batch_create(piro_ctx *ctx, int n_inst, const int *h_dims, const float *d_alpha, const float *h_dt,
-                      piro_batch **out)
+piro_batch *batch_create(piro_ctx *ctx, int n_inst, const int *h_dims, const int *h_kind, const float *h_vel3,
+                         const float *d_alpha, const float *h_dt)
 {
-    PIRO_API_BEGIN
-    ctx_of(ctx);
-    PIRO_REQUIRE(n_inst > 0 && h_dims && d_alpha && h_dt && out, "bad arguments");
+    PIRO_REQUIRE(n_inst > 0 && h_dims && d_alpha && h_dt, "bad arguments");
     piro_batch *b = new piro_batch();
+    int64_t *d_nnz_off = nullptr;
+    float *d_small = nullptr;
+    int *d_kind = nullptr;
     try {
         b->ctx = ctx;
         b->n_inst = n_inst;
         b->row_off.assign(n_inst + 1, 0);
         std::vector<int64_t> nnz_off(n_inst + 1, 0);
-        std::vector<float> vself(n_inst), vnb(n_inst), dt(h_dt, h_dt + n_inst);
+        std::vector<float> vself(n_inst), vnb(n_inst), dt(h_dt, h_dt + n_inst), vel(3 * (size_t)n_inst, 0.0f);
+        std::vector<int> kind(n_inst, 0);
         b->h_inst.resize(n_inst);
         for (int i = 0; i < n_inst; ++i) {
             int nx = h_dims[3 * i], ny = h_dims[3 * i + 1], nz = h_dims[3 * i + 2];
             PIRO_REQUIRE(nx >= 3 && ny >= 3 && nz >= 1 && nz != 2, "instance dims must be >= 3 (nz == 1 for 2-D)");
+            kind[i] = h_kind ? h_kind[i] : 0;
+            PIRO_REQUIRE(kind[i] >= 0 && kind[i] <= 3, "instance kind must be 0 (heat) or 1..3 (advection, uw/dw/cd)");
+            PIRO_REQUIRE(kind[i] == 0 || h_vel3, "advection instances need velocities");
             int64_t N = (int64_t)nx * ny * nz;
             int64_t nnz = nz == 1 ? 5 * N - 2 * ((int64_t)nx + ny)
                                   : 7 * N - 2 * ((int64_t)ny * nz + (int64_t)nx * nz + (int64_t)nx * ny);
+            if (kind[i] != 0) nnz -= N;  // the gradient rows have no self entry of their own
             b->row_off[i + 1] = b->row_off[i] + N;
             nnz_off[i + 1] = nnz_off[i] + nnz;
-            // spacing h = 1/(nx-2) (unit length, the reference's dx = l/n)
-            float hh = 1.0f / (float)(nx - 2);
-            float norm = (float)std::pow((double)hh, 2.0);
-            vself[i] = (float)((nz == 1 ? -4.0 : -6.0) / (double)norm);
-            vnb[i] = (float)(1.0 / (double)norm);
+            if (kind[i] == 0) {
+                // spacing h = 1/(nx-2) (unit length, the reference's dx = l/n)
+                float hh = 1.0f / (float)(nx - 2);
+                float norm = (float)std::pow((double)hh, 2.0);
+                vself[i] = (float)((nz == 1 ? -4.0 : -6.0) / (double)norm);
+                vnb[i] = (float)(1.0 / (double)norm);
+            } else {
+                // src/operatorgradient.cpp:25 with unit length: 2 (l0 / float(n0)),
+                // entries -1/norm and 1/norm as float divisions
+                const float norm = 2.0f * (1.0f / (float)(nx - 2));
+                vself[i] = -1.0f / norm;
+                vnb[i] = 1.0f / norm;
+                for (int q = 0; q < 3; ++q) vel[3 * (size_t)i + q] = h_vel3[3 * (size_t)i + q];
+            }
             InstDesc &d = b->h_inst[i];
             d.row0 = b->row_off[i];
             d.nx = nx; d.ny = ny; d.nz = nz;
@@ -379,23 +481,31 @@ int piro_batch_create(piro_ctx *ctx, int n_inst, const int *h_dims, const float 
         PIRO_CHECK_HIP(hipMalloc(&b->rp, (b->total_rows + 1) * sizeof(int32_t)));
         PIRO_CHECK_HIP(hipMalloc(&b->col, std::max<int64_t>(b->nnz, 1) * sizeof(int32_t)));
         PIRO_CHECK_HIP(hipMalloc(&b->val, std::max<int64_t>(b->nnz, 1) * sizeof(float)));
-        int64_t *d_nnz_off;
-        float *d_small;
         PIRO_CHECK_HIP(hipMalloc(&d_nnz_off, (n_inst + 1) * sizeof(int64_t)));
-        PIRO_CHECK_HIP(hipMalloc(&d_small, 3 * n_inst * sizeof(float)));
+        PIRO_CHECK_HIP(hipMalloc(&d_small, 6 * (size_t)n_inst * sizeof(float)));
+        PIRO_CHECK_HIP(hipMalloc(&d_kind, n_inst * sizeof(int)));
         PIRO_CHECK_HIP(hipMemcpy(d_nnz_off, nnz_off.data(), (n_inst + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
         PIRO_CHECK_HIP(hipMemcpy(d_small, dt.data(), n_inst * sizeof(float), hipMemcpyHostToDevice));
         PIRO_CHECK_HIP(hipMemcpy(d_small + n_inst, vself.data(), n_inst * sizeof(float), hipMemcpyHostToDevice));
         PIRO_CHECK_HIP(hipMemcpy(d_small + 2 * n_inst, vnb.data(), n_inst * sizeof(float), hipMemcpyHostToDevice));
+        PIRO_CHECK_HIP(hipMemcpy(d_small + 3 * n_inst, vel.data(), 3 * n_inst * sizeof(float), hipMemcpyHostToDevice));
+        PIRO_CHECK_HIP(hipMemcpy(d_kind, kind.data(), n_inst * sizeof(int), hipMemcpyHostToDevice));
         k_batch_assemble<<<grid_for(b->total_rows, 256), 256, 0, ctx->stream>>>(
-            n_inst, b->d_inst, d_nnz_off, d_alpha, d_small, d_small + n_inst, d_small + 2 * n_inst, b->total_rows,
-            b->rp, b->col, b->val);
+            n_inst, b->d_inst, d_nnz_off, d_alpha, d_small, d_small + n_inst, d_small + 2 * n_inst, d_kind,
+            d_small + 3 * n_inst, b->total_rows, b->rp, b->col, b->val);
         check_launch();
         PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
         PIRO_CHECK_HIP(hipFree(d_nnz_off));
+        d_nnz_off = nullptr;
         PIRO_CHECK_HIP(hipFree(d_small));
+        d_small = nullptr;
+        PIRO_CHECK_HIP(hipFree(d_kind));
+        d_kind = nullptr;
         build_units(b, 256);
     } catch (...) {
+        if (d_nnz_off) (void)hipFree(d_nnz_off);
+        if (d_small) (void)hipFree(d_small);
+        if (d_kind) (void)hipFree(d_kind);
         if (b->d_inst) (void)hipFree(b->d_inst);
         if (b->rp) (void)hipFree(b->rp);
         if (b->col) (void)hipFree(b->col);
@@ -404,8 +514,31 @@ int piro_batch_create(piro_ctx *ctx, int n_inst, const int *h_dims, const float 
         delete b;
         throw;
     }
-    *out = b;
     ++ctx->live_children;
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int piro_batch_create(piro_ctx *ctx, int n_inst, const int *h_dims, const float *d_alpha, const float *h_dt,
+                      piro_batch **out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(out, "out is null");
+    *out = batch_create(ctx, n_inst, h_dims, nullptr, nullptr, d_alpha, h_dt);
+    PIRO_API_END
+}
+
+int piro_batch_create_mixed(piro_ctx *ctx, int n_inst, const int *h_dims, const int *h_kind, const float *h_vel3,
+                            const float *d_alpha, const float *h_dt, piro_batch **out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(out && h_kind, "null argument");
+    *out = batch_create(ctx, n_inst, h_dims, h_kind, h_vel3, d_alpha, h_dt);
     PIRO_API_END
 }
 
@@ -447,26 +580,39 @@ int piro_batch_operator(piro_batch *b, const int32_t **d_row_ptr, const int32_t 
     PIRO_API_END
 }
 
-int piro_batch_set_tiling(piro_batch *b, int rows_per_unit, int grid)
+int piro_batch_set_schedule(piro_batch *b, int rows_per_unit, int grid, int chunk)
 {
     PIRO_API_BEGIN
     bat(b);
     PIRO_REQUIRE(rows_per_unit == 128 || rows_per_unit == 256 || rows_per_unit == 512 || rows_per_unit == 1024,
                  "rows_per_unit must be 128, 256, 512 or 1024");
     PIRO_REQUIRE(grid >= 0, "grid < 0");
+    PIRO_REQUIRE(chunk >= -1 && chunk <= 4096, "chunk must be -1 (persistent) or 0..4096");
     if (rows_per_unit != b->rb) build_units(b, rows_per_unit);
     b->grid = grid;
+    b->chunk = chunk;
     drop_graph(b);
     PIRO_API_END
 }
 
-int piro_batch_get_tiling(piro_batch *b, int *rows_per_unit, int *grid)
+int piro_batch_get_schedule(piro_batch *b, int *rows_per_unit, int *grid, int *chunk)
 {
     PIRO_API_BEGIN
     bat(b);
     if (rows_per_unit) *rows_per_unit = b->rb;
-    if (grid) *grid = b->grid > 0 ? b->grid : std::min(b->n_units, b->ctx->num_cus * 8);
+    if (grid) *grid = launch_grid(b);
+    if (chunk) *chunk = b->chunk;
     PIRO_API_END
+}
+
+int piro_batch_set_tiling(piro_batch *b, int rows_per_unit, int grid)
+{
+    return piro_batch_set_schedule(b, rows_per_unit, grid, -1);
+}
+
+int piro_batch_get_tiling(piro_batch *b, int *rows_per_unit, int *grid)
+{
+    return piro_batch_get_schedule(b, rows_per_unit, grid, nullptr);
 }
 
 int piro_batch_step(piro_batch *b, const float *d_x, float *d_y)
@@ -494,9 +640,9 @@ int piro_batch_run(piro_batch *b, float *d_x, float *d_work, int nsteps, int use
             PIRO_CHECK_HIP(hipMemcpyAsync(d_x, a, b->total_rows * sizeof(float), hipMemcpyDeviceToDevice, s));
         return PIRO_OK;
     }
-    const int g = b->grid > 0 ? b->grid : std::min(b->n_units, b->ctx->num_cus * 8);
+    const int g = launch_grid(b);
     if (!(b->gexec && b->g_x == d_x && b->g_w == d_work && b->g_steps == nsteps && b->g_rb == b->rb &&
-          b->g_grid == g)) {
+          b->g_grid == g && b->g_chunk == b->chunk)) {
         drop_graph(b);
         hipStream_t cs;
         PIRO_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
@@ -519,6 +665,7 @@ int piro_batch_run(piro_batch *b, float *d_x, float *d_work, int nsteps, int use
         b->g_steps = nsteps;
         b->g_rb = b->rb;
         b->g_grid = g;
+        b->g_chunk = b->chunk;
     }
     PIRO_CHECK_HIP(hipGraphLaunch(b->gexec, s));
     if (nsteps % 2 == 1)
@@ -536,14 +683,23 @@ int piro_batch_autotune(piro_batch *b, float *d_x, float *d_work, int *best_rows
     hipEvent_t e0, e1;
     PIRO_CHECK_HIP(hipEventCreate(&e0));
     PIRO_CHECK_HIP(hipEventCreate(&e1));
-    int best_rb = 256, best_g = 0;
+    int best_rb = 256, best_g = 0, best_c = 0;
     float best_t = 1e30f;
     const int rbs[4] = {128, 256, 512, 1024};
-    const int per_cu[6] = {2, 4, 8, 16, 64, 0};  // 0: one workgroup per unit
+    // persistent grids (workgroups per CU), then one workgroup per unit in
+    // plain order and with XCD runs of 4, 16 and 64 units
+    const int per_cu[5] = {2, 4, 8, 16, 64};
+    const int chunks[4] = {0, 4, 16, 64};
     for (int rb : rbs) {
         build_units(b, rb);
-        for (int pc : per_cu) {
-            b->grid = pc ? std::min(b->n_units, b->ctx->num_cus * pc) : b->n_units;
+        for (int v = 0; v < 9; ++v) {
+            if (v < 5) {
+                b->chunk = -1;
+                b->grid = std::min(b->n_units, b->ctx->num_cus * per_cu[v]);
+            } else {
+                b->chunk = chunks[v - 5];
+                b->grid = 0;
+            }
             launch_step(b, d_x, d_work, s);
             PIRO_CHECK_HIP(hipEventRecord(e0, s));
             for (int i = 0; i < 4; ++i) launch_step(b, d_x, d_work, s);
@@ -552,16 +708,17 @@ int piro_batch_autotune(piro_batch *b, float *d_x, float *d_work, int *best_rows
             float ms = 0.0f;
             PIRO_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
             ms /= 4;
-            if (ms < best_t) { best_t = ms; best_rb = rb; best_g = b->grid; }
+            if (ms < best_t) { best_t = ms; best_rb = rb; best_g = b->grid; best_c = b->chunk; }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     build_units(b, best_rb);
     b->grid = best_g;
+    b->chunk = best_c;
     drop_graph(b);
     if (best_rows_per_unit) *best_rows_per_unit = best_rb;
-    if (best_grid) *best_grid = best_g;
+    if (best_grid) *best_grid = launch_grid(b);
     if (best_ms) *best_ms = best_t;
     PIRO_API_END
 }

@@ -151,3 +151,150 @@ def test_ctx_destroy_refuses_live_batch_and_surrogate():
     assert lib.piro_surrogate_destroy(s) == 0
     assert lib.piro_free(h, d_alpha) == 0
     assert lib.piro_ctx_destroy(h) == 0
+
+
+# ------------------------------------------------ heat and advection mixed
+
+def oracle_advection(dims, kind, vel, dt, x0, steps):
+    """the reference's advection path for one instance: gradients of scheme
+    `kind`, the concatenation merge with the uniform velocity, forward Euler
+    with zero faces (2-D: the build's 2-D interior rule)"""
+    nx, ny, nz = dims
+    N = nx * ny * nz
+    g = po.grad_norm(1.0, nx - 2)
+    G = [po.grad_csr(nx, ny, nz, kind, d, g) for d in range(3)]
+    v3 = np.concatenate([np.full(N, v, np.float32) for v in vel])
+    D = po.geam3(*G, v3)
+    if nz == 1:
+        return po.fe_csr_run2d(nx, ny, D, dt, steps, x0)
+    return po.fe_csr_run(dims, D, dt, steps, x0, None)
+
+
+def oracle_heat_rows(dims, alpha_rows, dt, x0, steps):
+    """heat with a per-row diffusivity (the instance's own slice)"""
+    nx, ny, nz = dims
+    norm = po.lap_norm(1.0, nx - 2)
+    A = po.lap2d_padded_csr(nx, ny, norm) if nz == 1 else po.lap_csr(nx, ny, nz, po.CD, norm)
+    A = po.row_scale(A, alpha_rows)
+    if nz == 1:
+        return po.fe_csr_run2d(nx, ny, A, dt, steps, x0)
+    return po.fe_csr_run(dims, A, dt, steps, x0, None)
+
+
+MIXED = [  # (dims, kind, velocity)
+    ((17, 13, 1), 0, None), ((30, 31, 1), 0, None), ((9, 8, 7), 0, None),
+    ((20, 18, 1), 1, (-1.0, -0.5, 0.0)), ((16, 21, 1), 2, (0.75, 0.25, 0.0)), ((19, 19, 1), 3, (0.5, -1.0, 0.0)),
+    ((11, 10, 9), 1, (-1.0, -0.5, -0.25)), ((10, 12, 8), 3, (0.3, -0.6, 0.9)), ((8, 9, 10), 2, (1.0, 0.5, 0.25)),
+    ((40, 3, 1), 0, None), ((3, 3, 3), 1, (-1.0, -1.0, -1.0)),
+]
+
+
+def mixed_inputs(seed):
+    rng = np.random.default_rng(seed)
+    dims = [d for d, _, _ in MIXED]
+    kinds = [k for _, k, _ in MIXED]
+    vel = [v if v is not None else (0.0, 0.0, 0.0) for _, _, v in MIXED]
+    alpha, dts, xs = [], [], []
+    for (nx, ny, nz), k, v in MIXED:
+        N = nx * ny * nz
+        # a per-row diffusivity: a wrong row index into alpha shows
+        alpha.append(rng.uniform(0.005, 0.02, N).astype(np.float32))
+        if k == 0:
+            norm = np.float32(float(np.float32(1.0) / np.float32(nx - 2)) ** 2)
+            dts.append(np.float32(np.float32(0.2) * norm / np.float32(0.02)))
+        else:
+            gn = float(po.grad_norm(1.0, nx - 2))
+            dts.append(np.float32(0.5 * gn / sum(abs(c) for c in v)))
+        xs.append(rng.standard_normal(N).astype(np.float32))
+    return dims, kinds, vel, np.concatenate(alpha), dts, np.concatenate(xs)
+
+
+@pytest.mark.parametrize("schedule", [(256, 0, -1), (512, 5, -1), (128, 0, 0), (256, 0, 4), (1024, 0, 16), (512, 0, 1)])
+def test_batch_mixed_heat_and_advection_match_oracle(ctx, schedule):
+    print(f"case mixed batch schedule={schedule}", flush=True)
+    dims, kinds, vel, alpha, dts, x0 = mixed_inputs(31)
+    b = ctx.batch(dims, ctx.from_host(alpha), dts, kinds=kinds, vel=vel)
+    b.set_schedule(*schedule)
+    assert b.schedule()[2] == schedule[2]
+    x = ctx.from_host(x0)
+    w = ctx.empty_like(x)
+    steps = 7
+    b.run(x, w, steps, use_graph=schedule[2] >= 0)
+    got = x.host()
+    for i, ((d, k, v), dt) in enumerate(zip(MIXED, dts)):
+        s, e = b.offsets[i], b.offsets[i + 1]
+        if k == 0:
+            ref = oracle_heat_rows(d, alpha[s:e], dt, x0[s:e], steps)
+        else:
+            ref = oracle_advection(d, k, v, dt, x0[s:e], steps)
+        assert got[s:e].tobytes() == ref.tobytes(), f"instance {i} dims {d} kind {k}"
+    b.close()
+
+
+def test_batch_3d_advection_equals_single_grid_path(ctx):
+    # the 3-D advection instance against the library's own single-grid path:
+    # gradient assembly, the concatenation merge, FE with zero faces
+    dims = (14, 12, 10)
+    N = 14 * 12 * 10
+    vel = (-1.0, -0.5, -0.25)
+    gn = ctx.grad_norm(1.0, 12)
+    dt = np.float32(0.5 * float(gn) / 1.75)
+    x0 = np.random.default_rng(8).standard_normal(N).astype(np.float32)
+    b = ctx.batch([dims], ctx.full(N, 0.01), [dt], kinds=[1], vel=[vel])
+    x = ctx.from_host(x0)
+    w = ctx.empty_like(x)
+    b.run(x, w, 9, use_graph=True)
+    G = [ctx.gradient_csr(*dims, 1, d, gn) for d in range(3)]
+    v3 = ctx.from_host(np.concatenate([np.full(N, c, np.float32) for c in vel]))
+    D = ctx.geam3(G[0], G[1], G[2], v3)
+    assert b.nnz == D.nnz
+    xs = ctx.from_host(x0)
+    ctx.fe_run_csr(dims, D, dt, 9, xs, snap=None)
+    assert x.host().tobytes() == xs.host().tobytes()
+    b.close()
+
+
+def test_batch_c4_shape_every_instance_matches_oracle(ctx):
+    # the batched configuration at full size: 512 2-D instances, side
+    # log-uniform in [32, 256], alpha log-uniform in [1e-3, 1e-1], r = 0.2,
+    # initial fields sums of 4-16 Gaussians; autotuned schedule, 200 steps
+    # under the captured graph; every instance against the oracle
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(4)
+    n_inst = 512
+    sides = np.exp(rng.uniform(np.log(32), np.log(256), n_inst)).astype(int)
+    alphas = np.exp(rng.uniform(np.log(1e-3), np.log(1e-1), n_inst)).astype(np.float32)
+    dims = [(int(s), int(s), 1) for s in sides]
+    a_parts, dts, x_parts = [], [], []
+    for (nx, ny, _), a in zip(dims, alphas):
+        a_parts.append(np.full(nx * ny, a, np.float32))
+        hh = np.float32(1.0) / np.float32(nx - 2)
+        dts.append(np.float32(np.float32(0.2) * np.float32(float(hh) ** 2) / a))
+        X, Y = np.meshgrid(np.linspace(0, 1, nx), np.linspace(0, 1, ny), indexing="xy")
+        f = np.zeros_like(X)
+        for _ in range(int(rng.integers(4, 17))):
+            c = rng.uniform(0.1, 0.9, 2)
+            wd = rng.uniform(0.03, 0.2)
+            f += np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2) / (2 * wd * wd))
+        x_parts.append(f.astype(np.float32).ravel())
+    alpha = np.concatenate(a_parts)
+    x0 = np.concatenate(x_parts)
+    b = ctx.batch(dims, ctx.from_host(alpha), dts)
+    x = ctx.from_host(x0)
+    w = ctx.empty_like(x)
+    rb, g, ms = b.autotune(x.clone(), w)
+    print(f"C4 batch: {b.total_rows} rows, {b.nnz} nnz; schedule {b.schedule()}, {ms * 1e3:.1f} us/step", flush=True)
+    steps = 200
+    b.run(x, w, steps, use_graph=True)
+    got = x.host()
+
+    def check(i):
+        s, e = b.offsets[i], b.offsets[i + 1]
+        nx, ny, _ = dims[i]
+        ref = oracle_heat_rows(dims[i], alpha[s:e], dts[i], x0[s:e], steps)
+        return got[s:e].tobytes() == ref.tobytes()
+
+    with ThreadPoolExecutor(8) as pool:
+        ok = list(pool.map(check, range(n_inst)))
+    assert all(ok), [i for i, o in enumerate(ok) if not o][:10]
+    b.close()
