@@ -156,7 +156,7 @@ def bench_batch(args):
     x = ctx.from_host(x0h)
     w = ctx.empty_like(x)
     rb, g, tune_ms = b.autotune(x.clone(), w)
-    steps = 200
+    steps = args.steps or 200
     x.upload(x0h)
     b.run(x, w, steps, use_graph=True)  # capture + warm
     x.upload(x0h)
@@ -176,7 +176,8 @@ def bench_batch(args):
                       "rows": b.total_rows, "nnz": b.nnz, "seconds": t, "us_per_step": t / steps * 1e6,
                       "cell_steps_per_s": b.total_rows * steps / t, "hbm_gbs": byts * steps / t / 1e9,
                       "roofline_frac": byts * steps / t / 1e9 / HBM, "tiling_rows_per_unit": rb,
-                      "tiling_grid": g, "bit_exact_vs_single_instance": ok}), flush=True)
+                      "tiling_grid": g, "schedule_chunk": b.schedule()[2], "bit_exact_vs_single_instance": ok}),
+          flush=True)
     b.close()
     ctx.close()
 
@@ -481,6 +482,18 @@ def bench_jli(args):
     ctx.close()
 
 
+def calibrate_copy(n_floats=64 << 20, reps=10):
+    ctx = Context(0)
+    src = ctx.full(n_floats, 1.0)
+    dst = ctx.empty_like(src)
+    t = timed(ctx, lambda: ctx.copy(src, dst), reps)
+    print(json.dumps({"calibration": "k_copy4 (piro_vec_copy)", "bytes_read": 4 * n_floats,
+                      "bytes_written": 4 * n_floats, "dispatches": reps, "gbs": 8 * n_floats / t / 1e9}), flush=True)
+    src.free()
+    dst.free()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("what", choices=["lu", "batch", "advection", "surrogate", "surrogate_fwd", "jli", "formats"])
@@ -489,7 +502,12 @@ def main():
     ap.add_argument("--traj", type=int, default=32000)
     ap.add_argument("--train-side", type=int, default=32)
     ap.add_argument("--passes", type=int, default=3)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first time the library's streaming copy of a 256 MiB vector 10 times (the known-size "
+                         "kernel that HBM counter readings are calibrated against)")
     args = ap.parse_args()
+    if args.calibrate:
+        calibrate_copy()
     {"lu": bench_lu, "batch": bench_batch, "advection": bench_advection, "surrogate": bench_surrogate,
      "surrogate_fwd": bench_surrogate_fwd, "jli": bench_jli, "formats": bench_formats}[args.what](args)
 

@@ -46,6 +46,7 @@ struct piro_batch {
     piro_ctx *ctx = nullptr;
     int n_inst = 0;
     int64_t total_rows = 0, nnz = 0;
+    int max_row_nnz = 7;           // the widest operator row of any instance
     std::vector<int64_t> row_off;  // n_inst + 1
     std::vector<InstDesc> h_inst;
     InstDesc *d_inst = nullptr;
@@ -276,7 +277,10 @@ __global__ __launch_bounds__(256) void k_batch_step(const Unit *__restrict__ uni
 typedef int int4v __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
-template <int RB, int QPL>
+// NB LDS buffers: two alternate between a persistent workgroup's
+// consecutive units; one unit per workgroup needs one (and leaves room for
+// more resident workgroups)
+template <int RB, int QPL, int NB>
 __global__ __launch_bounds__(256) void k_batch_pf(const Unit *__restrict__ units, int n_units,
                                                   const InstDesc *__restrict__ inst,
                                                   const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
@@ -285,7 +289,7 @@ __global__ __launch_bounds__(256) void k_batch_pf(const Unit *__restrict__ units
 {
     constexpr int RPL = RB / 256;
     constexpr int CAP = QPL * 1024;
-    __shared__ __attribute__((aligned(16))) float s_p[2][CAP];
+    __shared__ __attribute__((aligned(16))) float s_p[NB][CAP];
     const int t = threadIdx.x;
     int buf = 0;
     const int ustep = chunk < 0 ? (int)gridDim.x : n_units;  // one unit unless persistent
@@ -298,7 +302,7 @@ __global__ __launch_bounds__(256) void k_batch_pf(const Unit *__restrict__ units
         const int ka = k0 & ~3;
         const bool staged = k1 - ka <= CAP;
         const bool fast = staged && k1 > ka && (int64_t)((k1 + 3) & ~3) <= nnz;
-        float *sp = s_p[buf];
+        float *sp = s_p[NB == 2 ? buf : 0];
         int4v c[QPL];
         float4v v[QPL];
         int kb[QPL];
@@ -316,12 +320,25 @@ __global__ __launch_bounds__(256) void k_batch_pf(const Unit *__restrict__ units
         int rb[RPL], re[RPL];
         float xr[RPL];
         bool interior[RPL];
+        // a row's end is the next row's start: lanes 0..62 take it from the
+        // next lane, lane 63 from a scalar load at the wave's edge (row
+        // pointers are read at min(row, nr): rp[r0 + nr] ends the unit's last
+        // row, and lanes past it read that pointer again)
+        const int lane = t & 63;
+        const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+        int edge[RPL];
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+            // every load first; the exchange waits for them
+            rb[j] = rp[r0 + min(t + 256 * j, nr)];
+            edge[j] = rp[r0 + min(64 * wv + 64 + 256 * j, nr)];  // uniform across the wave
+            xr[j] = x[r0 + min(t + 256 * j, nr - 1)];
+        }
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
             const int64_t g = r0 + min(t + 256 * j, nr - 1);
-            rb[j] = rp[g];
-            re[j] = rp[g + 1];
-            xr[j] = x[g];
+            const int nxt = __shfl_down(rb[j], 1, 64);
+            re[j] = lane == 63 ? edge[j] : nxt;
             const uint32_t li = (uint32_t)(g - d.row0);
             const uint32_t zz = fdiv(li, d.mxy, d.sxy);
             const uint32_t rem = li - zz * (uint32_t)(d.nx * d.ny);
@@ -384,24 +401,34 @@ int launch_grid(const piro_batch *b)
     return (b->n_units + span - 1) / span * span;
 }
 
+template <int RB, int QPL>
+void launch_pf(piro_batch *b, const float *x, float *y, hipStream_t s, int g, int c)
+{
+    if (c < 0)
+        k_batch_pf<RB, QPL, 2><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x,
+                                                 y, c);
+    else
+        k_batch_pf<RB, QPL, 1><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x,
+                                                 y, c);
+}
+
 void launch_step(piro_batch *b, const float *x, float *y, hipStream_t s)
 {
     const int g = launch_grid(b), c = b->chunk;
+    // quads per lane: the unit's entries (rows times the widest row) plus
+    // the alignment slack
     switch (b->rb) {
     case 128:
         k_batch_step<128><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y, c);
         break;
-    // 7 entries per row at most, plus the quad alignment slack
-    case 256:
-        k_batch_pf<256, 2><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x, y,
-                                             c);
-        break;
+    case 256: launch_pf<256, 2>(b, x, y, s, g, c); break;
     case 512:
-        k_batch_pf<512, 4><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x, y,
-                                             c);
+        if (b->max_row_nnz <= 5) launch_pf<512, 3>(b, x, y, s, g, c);
+        else launch_pf<512, 4>(b, x, y, s, g, c);
         break;
     default:
-        k_batch_step<1024><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y, c);
+        if (b->max_row_nnz <= 5) launch_pf<1024, 6>(b, x, y, s, g, c);
+        else k_batch_step<1024><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, x, y, c);
         break;
     }
     check_launch();
@@ -453,6 +480,8 @@ piro_batch *batch_create(piro_ctx *ctx, int n_inst, const int *h_dims, const int
             if (kind[i] != 0) nnz -= N;  // the gradient rows have no self entry of their own
             b->row_off[i + 1] = b->row_off[i] + N;
             nnz_off[i + 1] = nnz_off[i] + nnz;
+            const int wide = (nz == 1 ? 5 : 7) - (kind[i] != 0 ? 1 : 0);
+            b->max_row_nnz = i == 0 ? wide : std::max(b->max_row_nnz, wide);
             if (kind[i] == 0) {
                 // spacing h = 1/(nx-2) (unit length, the reference's dx = l/n)
                 float hh = 1.0f / (float)(nx - 2);
@@ -687,12 +716,12 @@ int piro_batch_autotune(piro_batch *b, float *d_x, float *d_work, int *best_rows
     float best_t = 1e30f;
     const int rbs[4] = {128, 256, 512, 1024};
     // persistent grids (workgroups per CU), then one workgroup per unit in
-    // plain order and with XCD runs of 4, 16 and 64 units
+    // plain order and with XCD runs of 4 to 256 units
     const int per_cu[5] = {2, 4, 8, 16, 64};
-    const int chunks[4] = {0, 4, 16, 64};
+    const int chunks[6] = {0, 4, 16, 64, 128, 256};
     for (int rb : rbs) {
         build_units(b, rb);
-        for (int v = 0; v < 9; ++v) {
+        for (int v = 0; v < 11; ++v) {
             if (v < 5) {
                 b->chunk = -1;
                 b->grid = std::min(b->n_units, b->ctx->num_cus * per_cu[v]);
