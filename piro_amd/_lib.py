@@ -88,6 +88,7 @@ _SIGS = {
     "piro_vec_copy": [vp, i64, vp, vp],
     "piro_csr_to_dense": [vp, i32, i32, vp, vp, vp, vp, i32],
     "piro_lu_factor_f32": [vp, i32, vp, i32, vp, P_i32],
+    "piro_lu_set_speculation": [vp, i32],
     "piro_lu_solve_f32": [vp, i32, vp, i32, vp, i32, vp, i32],
     "piro_lu_solve_refine": [vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, f64, i32, P_i32, P_f64],
     "piro_jli_solve": [vp, i32, i32, i32, vp, vp, vp, i64, f32, vp, f32, f32, i32, vp, vp, P_i32, P_f32],

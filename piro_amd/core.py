@@ -421,6 +421,9 @@ class Context:
         check(lib.piro_lu_factor_f32(self.h, n, _p(a), n, _p(ipiv), C.byref(info)))
         return ipiv
 
+    def lu_set_speculation(self, enable: bool):
+        check(lib.piro_lu_set_speculation(self.h, 1 if enable else 0))
+
     def lu_solve_f32(self, lu: DevArray, n: int, ipiv, b: DevArray):
         nrhs = b.numel() // n
         check(lib.piro_lu_solve_f32(self.h, n, _p(lu), n, _p(ipiv), nrhs, _p(b), nrhs))

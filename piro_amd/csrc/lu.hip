@@ -227,45 +227,258 @@ struct PanelLoop<NT, R, PB> {
     }
 };
 
-// VEC: lda and the matrix base allow 16-byte row segments (k is a multiple
-// of 16)
-template <int NT, int R, bool VEC>
-__global__ __launch_bounds__(NT) void k_panel(float *__restrict__ a, int lda, int n, int k, int jb,
-                                              int32_t *__restrict__ ipiv, int *__restrict__ info)
+// ------------------------------------- exchange-free columns, speculatively
+//
+// Partial pivoting takes the first row of largest magnitude among the rows
+// at or below gj, and the row at gj comes first: it is the pivot exactly when
+// no row below it has a larger magnitude, magnitudes compared as the bits of
+// |x| as in the pivoting step's key (so NaN ranks above every number and NaN
+// payloads decide between NaNs).  A fast step assumes so: the pivot row gj is
+// register row 0 of lane J (labels never move here), posted to LDS at the end
+// of the previous step; no reduction and one barrier per column; the scaling
+// and update are the pivoting step's own arithmetic.  Each lane flags any row
+// of its own that would have won.  Nothing is written to memory until every
+// column of the panel is known to be exchange-free; otherwise the panel is
+// reloaded and factored by the pivoting steps, so the factors and pivots are
+// those of the pivoting steps in either case.
+
+// one row's multiplier and update, as in the pivoting step
+template <int J>
+__device__ __forceinline__ void fast_row(float (&q)[PB], const float (&u)[PB], float d, float rd, bool rcp)
 {
-    __shared__ PanelShared<NT, R> sh;
-    const int t = threadIdx.x;
-    float p[R][PB];
-    int lab[R];
-    const bool full = VEC && jb == PB;
+    float l;
+    if (__builtin_expect(rcp, 1)) l = q[J] * rd;
+    else l = q[J] / d;
+    q[J] = l;
+#pragma unroll
+    for (int c = J + 1; c < PB; ++c) q[c] = __builtin_fmaf(-l, u[c], q[c]);
+}
+
+template <int NT, int R, int J>
+__device__ __forceinline__ void fast_step(float (&p)[R][PB], float (*s_row)[PB], int t, int n, int k, int jb,
+                                          uint32_t &viol, int &zero_col)
+{
+    if (J >= jb) return;  // uniform
+    constexpr int par = J & 1;
+    __syncthreads();
+    float u[PB];
+    get_row(u, s_row[par]);
+    const float d = u[J];
+    const uint32_t db = (uint32_t)__float_as_int(d) & 0x7fffffffu;
+    // rows below gj: register row r of lane t is row k + t + NT r and NT > J,
+    // so only register row 0 holds rows at or above gj.  Rows past n are
+    // zeros that are never stored: they are updated like the others, and
+    // can only raise the flag (never hide a winning row) should an infinity
+    // in the pivot row turn them into NaN.
+    const bool below0 = t > J;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int gi = k + t + NT * r;
-        lab[r] = gi;
-        const float *src = a + (int64_t)gi * lda + k;
-        if (full) {
-            if (gi < n) get_row(p[r], src);
-            else
+        const uint32_t xb = (uint32_t)__float_as_int(p[r][J]) & 0x7fffffffu;
+        viol |= ((r > 0 || below0) && xb > db) ? 1u : 0u;
+    }
+    if (d != 0.0f) {
+        const bool rcp = fabsf(d) >= __FLT_MIN__;
+        const float rd = 1.0f / d;
+        // register row 0 first: lane J + 1 then posts the next pivot row
+        // while the other rows are updated
+        if (below0) fast_row<J>(p[0], u, d, rd, rcp);
+        if (J + 1 < jb && t == J + 1) put_row(s_row[par ^ 1], p[0]);
 #pragma unroll
-                for (int c = 0; c < PB; ++c) p[r][c] = 0.0f;
-        } else {
+        for (int r = 1; r < R; ++r) fast_row<J>(p[r], u, d, rd, rcp);
+    } else {
+        if (zero_col < 0) zero_col = J;  // d is the same in every lane
+        if (J + 1 < jb && t == J + 1) put_row(s_row[par ^ 1], p[0]);
+    }
+}
+
+template <int NT, int R, int J>
+struct FastLoop {
+    static __device__ __forceinline__ void run(float (&p)[R][PB], float (*s_row)[PB], int t, int n, int k, int jb,
+                                               uint32_t &viol, int &zero_col)
+    {
+        fast_step<NT, R, J>(p, s_row, t, n, k, jb, viol, zero_col);
+        FastLoop<NT, R, J + 1>::run(p, s_row, t, n, k, jb, viol, zero_col);
+    }
+};
+
+template <int NT, int R>
+struct FastLoop<NT, R, PB> {
+    static __device__ __forceinline__ void run(float (&)[R][PB], float (*)[PB], int, int, int, int, uint32_t &, int &)
+    {
+    }
+};
+
+// the panel's rows into registers: thread t holds rows k + t + NT r (zeros
+// past n); VEC: a full panel (jb = 16) whose lda and base allow 16-byte row
+// segments (k is a multiple of 16)
+template <int NT, int R, bool VEC>
+__device__ __forceinline__ void load_panel(float (&p)[R][PB], const float *__restrict__ a, int lda, int n, int k,
+                                           int jb, int t)
+{
+    if constexpr (VEC) {
+        // every row from a clamped address, all loads in flight together;
+        // rows past n are cleared after
+#pragma unroll
+        for (int r = 0; r < R; ++r) get_row(p[r], a + (int64_t)min(k + t + NT * r, n - 1) * lda + k);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int c = 0; c < PB; ++c) p[r][c] = k + t + NT * r < n ? p[r][c] : 0.0f;
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int gi = k + t + NT * r;
+            const float *src = a + (int64_t)gi * lda + k;
 #pragma unroll
             for (int c = 0; c < PB; ++c) p[r][c] = (gi < n && c < jb) ? src[c] : 0.0f;
         }
     }
-    PanelLoop<NT, R, 0>::run(p, lab, sh, t, n, k, jb, ipiv, info);
+}
+
+// register row r goes back to matrix row lab[r]
+template <int NT, int R, bool VEC>
+__device__ __forceinline__ void store_panel(const float (&p)[R][PB], const int (&lab)[R], float *__restrict__ a,
+                                            int lda, int n, int k, int jb)
+{
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int gi = lab[r];
         float *dst = a + (int64_t)gi * lda + k;
         if (gi < n) {
-            if (full) put_row(dst, p[r]);
+            if constexpr (VEC) put_row(dst, p[r]);
             else
 #pragma unroll
                 for (int c = 0; c < PB; ++c)
                     if (c < jb) dst[c] = p[r][c];
         }
     }
+}
+
+// The same through LDS: global memory is read and written as coalesced
+// 16-byte chunks (four consecutive lanes cover one 64-byte row segment, a
+// wave 16 rows), one register row of every lane per round, and each lane
+// moves its own row between LDS and registers.  stage: NT x SP floats.
+constexpr int SP = PB + 4;  // row pitch in LDS: 80 bytes, no bank conflicts on row-wise reads
+
+template <int NT, int R>
+__device__ __forceinline__ void load_panel_staged(float (&p)[R][PB], const float *__restrict__ a, int lda, int n,
+                                                  int k, int t, float *stage)
+{
+    const int sub = t >> 2, ch = 4 * (t & 3);
+    f4 v[4];
+    auto fetch = [&](int r) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = NT * r + sub + (NT / 4) * q;
+            v[q] = *(const f4 *)(a + (int64_t)min(k + row, n - 1) * lda + k + ch);
+        }
+    };
+    fetch(0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (r > 0) __syncthreads();  // the previous round's reads are done
+#pragma unroll
+        for (int q = 0; q < 4; ++q) *(f4 *)(stage + (sub + (NT / 4) * q) * SP + ch) = v[q];
+        if (r + 1 < R) fetch(r + 1);  // next round's loads in flight during this one
+        __syncthreads();
+        // element by element: 16-byte moves here tie the panel's registers
+        // into aligned quads, and the four-row kernel spills
+#pragma unroll
+        for (int c = 0; c < PB; ++c) p[r][c] = stage[t * SP + c];
+        // rows past n are zeros
+#pragma unroll
+        for (int c = 0; c < PB; ++c) p[r][c] = k + t + NT * r < n ? p[r][c] : 0.0f;
+    }
+}
+
+// register row r of lane t back to matrix row k + t + NT r (rows in place)
+template <int NT, int R>
+__device__ __forceinline__ void store_panel_staged(const float (&p)[R][PB], float *__restrict__ a, int lda, int n,
+                                                   int k, int t, float *stage)
+{
+    const int sub = t >> 2, ch = 4 * (t & 3);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        __syncthreads();  // the previous round's (or the caller's) LDS reads are done
+#pragma unroll
+        for (int c = 0; c < PB; ++c) stage[t * SP + c] = p[r][c];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = NT * r + sub + (NT / 4) * q;
+            if (k + row < n) *(f4 *)(a + (int64_t)(k + row) * lda + k + ch) = *(const f4 *)(stage + (sub + (NT / 4) * q) * SP + ch);
+        }
+    }
+}
+
+// One panel: the speculative exchange-free pass first (when `speculate`),
+// then, if some column needs an exchange, the panel reloaded and factored by
+// the pivoting steps.  Full 16-column panels with 16-byte rows (VEC) move
+// through LDS as coalesced chunks; the pivoting pass writes its rows back
+// one 64-byte row per lane, since exchanged rows go to their labels.
+// PASS: 0 both passes; 1 the speculative pass alone, recording in *done
+// whether it factored the panel; 2 the pivoting pass alone, returning at
+// once when *done says the launch before it factored the panel (done may be
+// NULL).  The 1024 x 4 shape takes passes 1 and 2 as two launches: both
+// passes in one kernel would spill its registers.
+template <int NT, int R, bool VEC, int PASS>
+__global__ __launch_bounds__(NT) void k_panel(float *__restrict__ a, int lda, int n, int k, int jb,
+                                              int32_t *__restrict__ ipiv, int *__restrict__ info, int speculate,
+                                              int *__restrict__ done)
+{
+    constexpr bool SPLIT = PASS != 0;
+    if constexpr (PASS == 2) {
+        if (done && *done) return;  // uniform
+        speculate = 0;
+    }
+    __shared__ __attribute__((aligned(16))) float s_row[2][PB];
+    __shared__ __attribute__((aligned(16))) float s_stage[VEC ? NT * SP : 4];
+    __shared__ PanelShared<NT, R> sh;
+    const int t = threadIdx.x;
+    float p[R][PB];
+    auto load = [&]() {
+        if constexpr (VEC) load_panel_staged<NT, R>(p, a, lda, n, k, t, s_stage);
+        else load_panel<NT, R, VEC>(p, a, lda, n, k, jb, t);
+    };
+    load();
+    if (speculate) {
+        uint32_t viol = 0;
+        int zero_col = -1;
+        if (t == 0) put_row(s_row[0], p[0]);
+        // a column that needs an exchange is almost always found in the
+        // first column, checked alone
+        fast_step<NT, R, 0>(p, s_row, t, n, k, jb, viol, zero_col);
+        bool ok = !__syncthreads_or((int)viol);
+        if (ok) {
+            FastLoop<NT, R, 1>::run(p, s_row, t, n, k, jb, viol, zero_col);
+            ok = !__syncthreads_or((int)viol);
+        }
+        if constexpr (SPLIT) {
+            if (t == 0) *done = ok ? 1 : 0;
+        }
+        if (ok) {
+            if (t < jb) ipiv[k + t] = k + t;
+            if (t == 0 && zero_col >= 0 && *info == 0) *info = k + zero_col + 1;
+            if constexpr (VEC) {
+                store_panel_staged<NT, R>(p, a, lda, n, k, t, s_stage);
+            } else {
+                int lab[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) lab[r] = k + t + NT * r;
+                store_panel<NT, R, VEC>(p, lab, a, lda, n, k, jb);
+            }
+            return;
+        }
+        if constexpr (SPLIT) return;
+        __syncthreads();  // the stage buffer is reused by the reload
+        load();
+    }
+    if constexpr (PASS == 1) return;
+    int lab[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) lab[r] = k + t + NT * r;
+    PanelLoop<NT, R, 0>::run(p, lab, sh, t, n, k, jb, ipiv, info);
+    store_panel<NT, R, VEC>(p, lab, a, lda, n, k, jb);
 }
 
 // generic (slow) panel for n beyond the register path: same arithmetic,
@@ -1109,25 +1322,38 @@ void panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int32_t *ipiv
 {
     hipStream_t s = ctx->stream;
     const int m = n - k;
-    // register panels hold 1024 x R rows; taller panels work in global memory.
-    // Many waves with few rows each: the column chain is latency-bound, and
-    // four waves per SIMD hide more of it than fewer waves with fewer
-    // per-wave reductions (256 x 8 / 512 x 8 measured 8% slower overall,
-    // 512 x 1 / 1024 x 1 / 1024 x 2 2% faster than 256 x 2 / 256 x 4 /
-    // 512 x 4).
-    const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0);
-#define PIRO_PANEL(NT_, R_)                                                                   \
-    {                                                                                         \
-        if (vec) k_panel<NT_, R_, true><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);    \
-        else k_panel<NT_, R_, false><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);       \
+    // register panels hold NT x R rows; taller panels work in global memory.
+    // Shapes by panel height, from kernel times of the exchange-free pass
+    // (the pass the backward-Euler systems take): at 4096 rows 1024 x 4 takes
+    // 19 us, 512 x 8 22 us; at 3072 rows 512 x 6 19 us; at 2048 512 x 4 14
+    // us, 1024 x 2 19 us; at 1024 rows 256 x 4 12 us; at 512 256 x 2 10 us.
+    const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0) && jb == PB;
+    const int spec = ctx->lu_speculate;
+    int *done = d_info + 1;
+#define PIRO_PANEL(NT_, R_)                                                                                   \
+    {                                                                                                         \
+        if (vec) k_panel<NT_, R_, true, 0><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, spec, done);     \
+        else k_panel<NT_, R_, false, 0><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, spec, done);        \
+    }
+#define PIRO_PANEL_SPLIT(NT_, R_)                                                                                \
+    {                                                                                                            \
+        if (vec) {                                                                                               \
+            if (spec) k_panel<NT_, R_, true, 1><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 1, done);      \
+            k_panel<NT_, R_, true, 2><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 0, spec ? done : 0);    \
+        } else {                                                                                                 \
+            if (spec) k_panel<NT_, R_, false, 1><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 1, done);     \
+            k_panel<NT_, R_, false, 2><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 0, spec ? done : 0);   \
+        }                                                                                                        \
     }
     if (m <= 256) PIRO_PANEL(256, 1)
-    else if (m <= 512) PIRO_PANEL(512, 1)
-    else if (m <= 1024) PIRO_PANEL(1024, 1)
-    else if (m <= 2048) PIRO_PANEL(1024, 2)
-    else if (m <= 4096) PIRO_PANEL(1024, 4)
+    else if (m <= 512) PIRO_PANEL(256, 2)
+    else if (m <= 1024) PIRO_PANEL(256, 4)
+    else if (m <= 2048) PIRO_PANEL(512, 4)
+    else if (m <= 3072) PIRO_PANEL(512, 6)
+    else if (m <= 4096) PIRO_PANEL_SPLIT(1024, 4)
     else k_panel_global<<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
 #undef PIRO_PANEL
+#undef PIRO_PANEL_SPLIT
     check_launch();
 }
 
@@ -1191,14 +1417,23 @@ int piro_lu_factor_f32(piro_ctx *ctx, int n, float *d_a, int lda, int32_t *d_ipi
     ctx_of(ctx);
     PIRO_REQUIRE(n > 0 && lda >= n && d_a && d_ipiv, "bad arguments");
     PIRO_REQUIRE(n < (1 << 27), "n too large (row labels are packed in 27 bits)");
-    int *d_info = (int *)ctx->scratch[0].get(sizeof(int));
-    PIRO_CHECK_HIP(hipMemsetAsync(d_info, 0, sizeof(int), ctx->stream));
+    // info, then the speculative panels' done flag
+    int *d_info = (int *)ctx->scratch[0].get(2 * sizeof(int));
+    PIRO_CHECK_HIP(hipMemsetAsync(d_info, 0, 2 * sizeof(int), ctx->stream));
     factor(ctx, n, d_a, lda, d_ipiv, d_info);
     int h = 0;
     PIRO_CHECK_HIP(hipMemcpyAsync(&h, d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     if (info) *info = h;
     if (h) throw Error(PIRO_ESINGULAR, "exactly zero pivot at step " + std::to_string(h));
+    PIRO_API_END
+}
+
+int piro_lu_set_speculation(piro_ctx *ctx, int enable)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    ctx->lu_speculate = enable ? 1 : 0;
     PIRO_API_END
 }
 

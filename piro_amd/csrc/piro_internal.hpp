@@ -92,6 +92,8 @@ struct piro_ctx {
     // live piro_batch / piro_surrogate objects made on this context: it
     // refuses to be destroyed under them
     int live_children = 0;
+    // LU panels try the exchange-free columns first (same factors either way)
+    int lu_speculate = 1;
 };
 
 namespace piro {

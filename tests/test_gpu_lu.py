@@ -75,8 +75,15 @@ def test_lu_factor_nan_pivots_stay_in_range(ctx, n):
     assert np.isnan(d.host().reshape(n, n)[n - 1, n - 1])
 
 
+@pytest.fixture(params=[True, False], ids=["speculative", "pivoting"])
+def strategy(ctx, request):
+    ctx.lu_set_speculation(request.param)
+    yield request.param
+    ctx.lu_set_speculation(True)
+
+
 @pytest.mark.parametrize("n", [48, 160, 300])
-def test_mfma_update_layout_exact(ctx, n):
+def test_mfma_update_layout_exact(ctx, strategy, n):
     # A = L0 U0 with L0 unit lower (entries in {0, +-1/4, +-1/2}) and U0
     # upper with small integers and power-of-two pivots (so the pivots'
     # reciprocals are exact): no row exchange is chosen (|l| < 1) and every
@@ -97,7 +104,7 @@ def test_mfma_update_layout_exact(ctx, n):
 
 
 @pytest.mark.parametrize("n", [48, 300, 1100])
-def test_pivot_ties_take_the_first_row(ctx, n):
+def test_pivot_ties_take_the_first_row(ctx, strategy, n):
     # A = L0 U0 with many unit multipliers: after j steps, column j holds
     # l_ij u_jj, and |l_ij| = 1 ties the diagonal row with every such row.
     # LAPACK's isamax takes the first row of largest magnitude, so no
@@ -117,6 +124,47 @@ def test_pivot_ties_take_the_first_row(ctx, n):
     np.testing.assert_array_equal(ipiv.host(), np.arange(n))
     np.testing.assert_array_equal(np.triu(lu), U0)
     np.testing.assert_array_equal(np.tril(lu, -1), np.tril(L0, -1))
+
+
+def exchange_mix(n, seed):
+    """a diagonally dominant matrix (no exchange anywhere) with some row pairs
+    swapped, so that some panels need exchanges and the others do not"""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-1.0, 1.0, (n, n)).astype(np.float32)
+    a[np.arange(n), np.arange(n)] = (n + rng.uniform(1.0, 2.0, n)).astype(np.float32)
+    for i, j in ((n // 5, n - 3), (n // 2, n // 2 + 40), (n - 70, n - 9)):
+        a[[i, j]] = a[[j, i]]
+    return a
+
+
+@pytest.mark.parametrize("n", [300, 1000, 2100, 4096])
+@pytest.mark.parametrize("kind", ["random", "mixed", "dominant"])
+def test_lu_strategies_bit_identical(ctx, n, kind):
+    # the speculative exchange-free panels and the pivoting panels give the
+    # same factors and pivots, bit for bit: on random matrices (exchanges in
+    # every panel), on matrices where only a few panels need one, and on a
+    # diagonally dominant one (none); the pivots match the FP64 oracle's
+    print(f"case lu strategies n={n} {kind}", flush=True)
+    if kind == "random":
+        a = np.random.default_rng(n + 5).standard_normal((n, n)).astype(np.float32)
+    else:
+        a = exchange_mix(n, n)
+        if kind == "dominant":
+            a = a[np.argsort(np.argmax(np.abs(a), axis=1))]
+    out = []
+    for spec in (True, False):
+        ctx.lu_set_speculation(spec)
+        d = dev(a, ctx)
+        ipiv = ctx.lu_factor(d, n)
+        out.append((d.host().tobytes(), ipiv.host().tobytes()))
+    ctx.lu_set_speculation(True)
+    assert out[0] == out[1]
+    piv = np.frombuffer(out[0][1], np.int32)
+    if kind != "random":
+        _, piv64, _ = po.lu_factor_f64(a.astype(np.float64))
+        np.testing.assert_array_equal(piv, piv64)
+        if kind == "dominant":
+            np.testing.assert_array_equal(piv, np.arange(n))
 
 
 def test_lu_solve_f32_matches_numpy(ctx):
