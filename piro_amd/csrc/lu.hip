@@ -133,7 +133,8 @@ __device__ __forceinline__ void get_row(float (&v)[PB], const float *src)
 
 template <int NT, int R, int J>
 __device__ __forceinline__ void panel_step(float (&p)[R][PB], int (&lab)[R], PanelShared<NT, R> &sh, int t, int n,
-                                           int k, int jb, int32_t *__restrict__ ipiv, int *__restrict__ info)
+                                           int k, int jb, int32_t *__restrict__ ipiv, int *__restrict__ info,
+                                           int (&pv)[PB])
 {
     constexpr int W = NT / 64;
     if (J >= jb) return;  // uniform: jb < 16 only for the last panel
@@ -177,6 +178,7 @@ __device__ __forceinline__ void panel_step(float (&p)[R][PB], int (&lab)[R], Pan
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, 0);
     const int slot = lo & 15;
     const int piv = 0x7ffffff - (int)(lo >> 4);
+    pv[J] = piv;  // the same in every lane
     if (t == 0) {
         ipiv[gj] = piv;
         if (hi == 1 && *info == 0) *info = gj + 1;  // largest magnitude is zero
@@ -212,17 +214,17 @@ __device__ __forceinline__ void panel_step(float (&p)[R][PB], int (&lab)[R], Pan
 template <int NT, int R, int J>
 struct PanelLoop {
     static __device__ __forceinline__ void run(float (&p)[R][PB], int (&lab)[R], PanelShared<NT, R> &sh, int t, int n,
-                                               int k, int jb, int32_t *ipiv, int *info)
+                                               int k, int jb, int32_t *ipiv, int *info, int (&pv)[PB])
     {
-        panel_step<NT, R, J>(p, lab, sh, t, n, k, jb, ipiv, info);
-        PanelLoop<NT, R, J + 1>::run(p, lab, sh, t, n, k, jb, ipiv, info);
+        panel_step<NT, R, J>(p, lab, sh, t, n, k, jb, ipiv, info, pv);
+        PanelLoop<NT, R, J + 1>::run(p, lab, sh, t, n, k, jb, ipiv, info, pv);
     }
 };
 
 template <int NT, int R>
 struct PanelLoop<NT, R, PB> {
     static __device__ __forceinline__ void run(float (&)[R][PB], int (&)[R], PanelShared<NT, R> &, int, int, int, int,
-                                               int32_t *, int *)
+                                               int32_t *, int *, int (&)[PB])
     {
     }
 };
@@ -411,6 +413,79 @@ __device__ __forceinline__ void store_panel_staged(const float (&p)[R][PB], floa
     }
 }
 
+// The panel's follow-up on the block's later columns [c0, c1) (at most
+// NB - PB = 48), which the panel's workgroup does itself (as k_swap_trsm
+// would as a launch of its own): its exchanges (steps k .. k + jb - 1, in
+// order) applied to those columns, then U12 = L11^-1 A12 with the unit lower
+// jb x jb block L11 staged in s_l; one lane per column, each row's sum in
+// ascending column order, unfused.  The block's rows k .. k + PB - 1 of those
+// columns were read into s_a12 when the kernel started; an exchange with a
+// row below the block reads and writes that row in memory.  PIV: apply the
+// pivots pv (false: there are none).
+constexpr int FW = NB - PB;  // follow-up columns at most
+
+template <int NT>
+__device__ __forceinline__ void prefetch_a12(const float *__restrict__ a, int lda, int k, int jb, int c0, int c1,
+                                             int t, float (&pre)[(PB * FW + NT - 1) / NT])
+{
+#pragma unroll
+    for (int q = 0; q < (PB * FW + NT - 1) / NT; ++q) {
+        const int e = t + NT * q, i = e / FW, cc = e % FW;
+        pre[q] = (e < PB * FW && i < jb && c0 + cc < c1) ? a[(int64_t)(k + i) * lda + c0 + cc] : 0.0f;
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void stage_a12(float (*s_a12)[FW + 1], int t, const float (&pre)[(PB * FW + NT - 1) / NT])
+{
+#pragma unroll
+    for (int q = 0; q < (PB * FW + NT - 1) / NT; ++q) {
+        const int e = t + NT * q;
+        if (e < PB * FW) s_a12[e / FW][e % FW] = pre[q];
+    }
+}
+
+template <bool PIV>
+__device__ __forceinline__ void panel_follow_up(float *__restrict__ a, int lda, int k, int jb, int c0, int c1,
+                                                const int (&pv)[PB], const float (*s_l)[PB + 1],
+                                                float (*s_a12)[FW + 1], int t)
+{
+    if (t >= c1 - c0) return;
+    const int c = c0 + t;
+    if constexpr (PIV) {
+#pragma unroll
+        for (int j = 0; j < PB; ++j) {
+            const int pj = pv[j];
+            if (j < jb && pj != k + j) {
+                const float x = s_a12[j][t];
+                if (pj < k + PB) {
+                    s_a12[j][t] = s_a12[pj - k][t];
+                    s_a12[pj - k][t] = x;
+                } else {
+                    s_a12[j][t] = a[(int64_t)pj * lda + c];
+                    a[(int64_t)pj * lda + c] = x;
+                }
+            }
+        }
+    }
+    float u[PB];
+#pragma unroll
+    for (int i = 0; i < PB; ++i) u[i] = s_a12[i][t];
+    // column by column: every u[i] still takes its terms in ascending j;
+    // the scheduling barriers keep each column's L reads next to their use
+    // (hoisted, the 120 reads would each hold a register)
+#pragma unroll
+    for (int j = 0; j < PB - 1; ++j) {
+#pragma unroll
+        for (int i = j + 1; i < PB; ++i)
+            if (i < jb) u[i] = u[i] - s_l[i][j] * u[j];
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int i = 0; i < PB; ++i)
+        if (i < jb) a[(int64_t)(k + i) * lda + c] = u[i];
+}
+
 // One panel: the speculative exchange-free pass first (when `speculate`),
 // then, if some column needs an exchange, the panel reloaded and factored by
 // the pivoting steps.  Full 16-column panels with 16-byte rows (VEC) move
@@ -424,7 +499,7 @@ __device__ __forceinline__ void store_panel_staged(const float (&p)[R][PB], floa
 template <int NT, int R, bool VEC, int PASS>
 __global__ __launch_bounds__(NT) void k_panel(float *__restrict__ a, int lda, int n, int k, int jb,
                                               int32_t *__restrict__ ipiv, int *__restrict__ info, int speculate,
-                                              int *__restrict__ done)
+                                              int *__restrict__ done, int c1)
 {
     constexpr bool SPLIT = PASS != 0;
     if constexpr (PASS == 2) {
@@ -434,13 +509,23 @@ __global__ __launch_bounds__(NT) void k_panel(float *__restrict__ a, int lda, in
     __shared__ __attribute__((aligned(16))) float s_row[2][PB];
     __shared__ __attribute__((aligned(16))) float s_stage[VEC ? NT * SP : 4];
     __shared__ PanelShared<NT, R> sh;
+    __shared__ float s_l[PB][PB + 1];
+    __shared__ float s_a12[PB][FW + 1];
     const int t = threadIdx.x;
+    const int c0 = k + jb;
     float p[R][PB];
     auto load = [&]() {
         if constexpr (VEC) load_panel_staged<NT, R>(p, a, lda, n, k, t, s_stage);
         else load_panel<NT, R, VEC>(p, a, lda, n, k, jb, t);
     };
-    load();
+    {
+        // the follow-up's rows of the block's later columns, in flight with
+        // the panel's own loads
+        float pre[(PB * FW + NT - 1) / NT];
+        prefetch_a12<NT>(a, lda, k, jb, c0, c1, t, pre);
+        load();
+        stage_a12<NT>(s_a12, t, pre);
+    }
     if (speculate) {
         uint32_t viol = 0;
         int zero_col = -1;
@@ -467,6 +552,15 @@ __global__ __launch_bounds__(NT) void k_panel(float *__restrict__ a, int lda, in
                 for (int r = 0; r < R; ++r) lab[r] = k + t + NT * r;
                 store_panel<NT, R, VEC>(p, lab, a, lda, n, k, jb);
             }
+            if (c0 < c1) {  // uniform
+                // L11: rows k .. k + jb - 1, register row 0 of lanes 0 .. jb - 1
+                if (t < jb)
+#pragma unroll
+                    for (int c = 0; c < PB; ++c) s_l[t][c] = p[0][c];
+                __syncthreads();
+                const int none[PB] = {};
+                panel_follow_up<false>(a, lda, k, jb, c0, c1, none, s_l, s_a12, t);
+            }
             return;
         }
         if constexpr (SPLIT) return;
@@ -477,8 +571,19 @@ __global__ __launch_bounds__(NT) void k_panel(float *__restrict__ a, int lda, in
     int lab[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) lab[r] = k + t + NT * r;
-    PanelLoop<NT, R, 0>::run(p, lab, sh, t, n, k, jb, ipiv, info);
+    int pv[PB];
+    PanelLoop<NT, R, 0>::run(p, lab, sh, t, n, k, jb, ipiv, info, pv);
     store_panel<NT, R, VEC>(p, lab, a, lda, n, k, jb);
+    if (c0 < c1) {  // uniform
+        // L11: the rows now labelled k .. k + jb - 1
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if ((uint32_t)(lab[r] - k) < (uint32_t)jb)
+#pragma unroll
+                for (int c = 0; c < PB; ++c) s_l[lab[r] - k][c] = p[r][c];
+        __syncthreads();
+        panel_follow_up<true>(a, lda, k, jb, c0, c1, pv, s_l, s_a12, t);
+    }
 }
 
 // generic (slow) panel for n beyond the register path: same arithmetic,
@@ -1318,7 +1423,9 @@ int solve_status(piro_ctx *ctx, const SolvePlan &pl)
     return h;
 }
 
-void panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int32_t *ipiv, int *d_info)
+// factors the panel and, unless it is the global-memory panel (returns
+// false), also applies its exchanges and U12 to the block's columns [k + jb, c1)
+bool panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int c1, int32_t *ipiv, int *d_info)
 {
     hipStream_t s = ctx->stream;
     const int m = n - k;
@@ -1332,17 +1439,17 @@ void panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int32_t *ipiv
     int *done = d_info + 1;
 #define PIRO_PANEL(NT_, R_)                                                                                   \
     {                                                                                                         \
-        if (vec) k_panel<NT_, R_, true, 0><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, spec, done);     \
-        else k_panel<NT_, R_, false, 0><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, spec, done);        \
+        if (vec) k_panel<NT_, R_, true, 0><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, spec, done, c1);     \
+        else k_panel<NT_, R_, false, 0><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, spec, done, c1);        \
     }
 #define PIRO_PANEL_SPLIT(NT_, R_)                                                                                \
     {                                                                                                            \
         if (vec) {                                                                                               \
-            if (spec) k_panel<NT_, R_, true, 1><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 1, done);      \
-            k_panel<NT_, R_, true, 2><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 0, spec ? done : 0);    \
+            if (spec) k_panel<NT_, R_, true, 1><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 1, done, c1);      \
+            k_panel<NT_, R_, true, 2><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 0, spec ? done : 0, c1);    \
         } else {                                                                                                 \
-            if (spec) k_panel<NT_, R_, false, 1><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 1, done);     \
-            k_panel<NT_, R_, false, 2><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 0, spec ? done : 0);   \
+            if (spec) k_panel<NT_, R_, false, 1><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 1, done, c1);     \
+            k_panel<NT_, R_, false, 2><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 0, spec ? done : 0, c1);   \
         }                                                                                                        \
     }
     if (m <= 256) PIRO_PANEL(256, 1)
@@ -1351,10 +1458,15 @@ void panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int32_t *ipiv
     else if (m <= 2048) PIRO_PANEL(512, 4)
     else if (m <= 3072) PIRO_PANEL(512, 6)
     else if (m <= 4096) PIRO_PANEL_SPLIT(1024, 4)
-    else k_panel_global<<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
+    else {
+        k_panel_global<<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
+        check_launch();
+        return false;
+    }
 #undef PIRO_PANEL
 #undef PIRO_PANEL_SPLIT
     check_launch();
+    return true;
 }
 
 void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
@@ -1366,11 +1478,13 @@ void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
         // panels of the block; each updates only the block's columns
         for (int k = kb; k < kb + nbk; k += PB) {
             const int jb = min(PB, kb + nbk - k);
-            panel(ctx, n, a, lda, k, jb, ipiv, d_info);
             const int c0 = k + jb, c1 = kb + nbk;
+            const bool followed = panel(ctx, n, a, lda, k, jb, c1, ipiv, d_info);
             if (c0 < c1) {
-                k_swap_trsm<PB><<<grid_for(c1 - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, c1, ipiv);
-                check_launch();
+                if (!followed) {
+                    k_swap_trsm<PB><<<grid_for(c1 - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, c1, ipiv);
+                    check_launch();
+                }
                 // at most 48 columns: one-wave 64 x 64 tiles
                 dim3 g((c1 - c0 + 63) / 64, (n - c0 + 63) / 64);
                 k_update_mfma<PB, 1, 1><<<g, 64, 0, s>>>(a, lda, n, k, jb, c0, c0, c1, vec && k % 4 == 0 && c0 % 4 == 0);
