@@ -9,17 +9,22 @@
 //
 // Layout: row-major, leading dimension lda (row swaps are contiguous runs).
 // Factorisation, right-looking, 64-column blocks of four 16-column panels:
-//   panel  -- one workgroup (256-1024 lanes) holds the m x 16 panel in
-//             registers (m <= 4096; taller panels work in global memory):
-//             pivot search with DPP wave reductions, row exchanges as label
-//             swaps, scaling and the rank-1 updates;
-//   in-block -- each panel's exchanges and U12 = L11^-1 A12 for the block's
-//             later columns, then a K = 16 MFMA update of those columns only;
-//   block  -- the block's 64 exchanges applied to the trailing columns and
-//             to the finished L columns on the left (LASWP), the block's U
-//             rows (one TRSM with the 64 x 64 L11), then one K = 64 MFMA
-//             update (v_mfma_f32_16x16x4f32, FP32 in, FP32 accumulate) of
-//             the trailing matrix, one 128 x 128 tile per workgroup.
+//   panel  -- first a speculative exchange-free pass over many workgroups
+//             (one lane per panel row, no barrier), exact whenever partial
+//             pivoting would keep every diagonal; then one workgroup
+//             (256-1024 lanes) holding the m x 16 panel in registers (m <=
+//             4096; taller panels work in global memory) either takes the
+//             speculative result or restores the panel and factors it by
+//             the pivoting steps: pivot search with DPP wave reductions, row
+//             exchanges as label swaps, scaling and the rank-1 updates;
+//   in-block -- each panel's exchanges on the block's other columns and U12
+//             = L11^-1 A12 for its later ones, then a K = 16 MFMA update of
+//             those columns only;
+//   block  -- the block's 64 exchanges applied to the columns outside the
+//             block and the block's U rows (one TRSM with the 64 x 64 L11)
+//             in one launch, then one K = 64 MFMA update
+//             (v_mfma_f32_16x16x4f32, FP32 in, FP32 accumulate) of the
+//             trailing matrix, one 128 x 128 tile per workgroup.
 // Solves: P b as a gather, then blocked forward and backward substitution
 // over 64-row blocks, one persistent launch per triangle.
 // Refinement: r = b - A x with FP64 accumulation over the FP32 entries of A,
@@ -229,22 +234,7 @@ struct PanelLoop<NT, R, PB> {
     }
 };
 
-// ------------------------------------- exchange-free columns, speculatively
-//
-// Partial pivoting takes the first row of largest magnitude among the rows
-// at or below gj, and the row at gj comes first: it is the pivot exactly when
-// no row below it has a larger magnitude, magnitudes compared as the bits of
-// |x| as in the pivoting step's key (so NaN ranks above every number and NaN
-// payloads decide between NaNs).  A fast step assumes so: the pivot row gj is
-// register row 0 of lane J (labels never move here), posted to LDS at the end
-// of the previous step; no reduction and one barrier per column; the scaling
-// and update are the pivoting step's own arithmetic.  Each lane flags any row
-// of its own that would have won.  Nothing is written to memory until every
-// column of the panel is known to be exchange-free; otherwise the panel is
-// reloaded and factored by the pivoting steps, so the factors and pivots are
-// those of the pivoting steps in either case.
-
-// one row's multiplier and update, as in the pivoting step
+// one row's multiplier and update, as the pivoting step forms them
 template <int J>
 __device__ __forceinline__ void fast_row(float (&q)[PB], const float (&u)[PB], float d, float rd, bool rcp)
 {
@@ -256,83 +246,29 @@ __device__ __forceinline__ void fast_row(float (&q)[PB], const float (&u)[PB], f
     for (int c = J + 1; c < PB; ++c) q[c] = __builtin_fmaf(-l, u[c], q[c]);
 }
 
-template <int NT, int R, int J>
-__device__ __forceinline__ void fast_step(float (&p)[R][PB], float (*s_row)[PB], int t, int n, int k, int jb,
-                                          uint32_t &viol, int &zero_col)
-{
-    if (J >= jb) return;  // uniform
-    constexpr int par = J & 1;
-    __syncthreads();
-    float u[PB];
-    get_row(u, s_row[par]);
-    const float d = u[J];
-    const uint32_t db = (uint32_t)__float_as_int(d) & 0x7fffffffu;
-    // rows below gj: register row r of lane t is row k + t + NT r and NT > J,
-    // so only register row 0 holds rows at or above gj.  Rows past n are
-    // zeros that are never stored: they are updated like the others, and
-    // can only raise the flag (never hide a winning row) should an infinity
-    // in the pivot row turn them into NaN.
-    const bool below0 = t > J;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const uint32_t xb = (uint32_t)__float_as_int(p[r][J]) & 0x7fffffffu;
-        viol |= ((r > 0 || below0) && xb > db) ? 1u : 0u;
-    }
-    if (d != 0.0f) {
-        const bool rcp = fabsf(d) >= __FLT_MIN__;
-        const float rd = 1.0f / d;
-        // register row 0 first: lane J + 1 then posts the next pivot row
-        // while the other rows are updated
-        if (below0) fast_row<J>(p[0], u, d, rd, rcp);
-        if (J + 1 < jb && t == J + 1) put_row(s_row[par ^ 1], p[0]);
-#pragma unroll
-        for (int r = 1; r < R; ++r) fast_row<J>(p[r], u, d, rd, rcp);
-    } else {
-        if (zero_col < 0) zero_col = J;  // d is the same in every lane
-        if (J + 1 < jb && t == J + 1) put_row(s_row[par ^ 1], p[0]);
-    }
-}
-
-template <int NT, int R, int J>
-struct FastLoop {
-    static __device__ __forceinline__ void run(float (&p)[R][PB], float (*s_row)[PB], int t, int n, int k, int jb,
-                                               uint32_t &viol, int &zero_col)
-    {
-        fast_step<NT, R, J>(p, s_row, t, n, k, jb, viol, zero_col);
-        FastLoop<NT, R, J + 1>::run(p, s_row, t, n, k, jb, viol, zero_col);
-    }
-};
-
-template <int NT, int R>
-struct FastLoop<NT, R, PB> {
-    static __device__ __forceinline__ void run(float (&)[R][PB], float (*)[PB], int, int, int, int, uint32_t &, int &)
-    {
-    }
-};
-
-// the panel's rows into registers: thread t holds rows k + t + NT r (zeros
-// past n); VEC: a full panel (jb = 16) whose lda and base allow 16-byte row
-// segments (k is a multiple of 16)
+// the panel's rows into registers: thread t holds panel rows t + NT r, row
+// i read from src + i ld (zeros from row m on, and in columns from jb on);
+// VEC: a full panel (jb = 16) whose rows are 16-byte aligned
 template <int NT, int R, bool VEC>
-__device__ __forceinline__ void load_panel(float (&p)[R][PB], const float *__restrict__ a, int lda, int n, int k,
-                                           int jb, int t)
+__device__ __forceinline__ void load_panel(float (&p)[R][PB], const float *__restrict__ src, int ld, int m, int jb,
+                                           int t)
 {
     if constexpr (VEC) {
         // every row from a clamped address, all loads in flight together;
-        // rows past n are cleared after
+        // rows past m are cleared after
 #pragma unroll
-        for (int r = 0; r < R; ++r) get_row(p[r], a + (int64_t)min(k + t + NT * r, n - 1) * lda + k);
+        for (int r = 0; r < R; ++r) get_row(p[r], src + (int64_t)min(t + NT * r, m - 1) * ld);
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
-            for (int c = 0; c < PB; ++c) p[r][c] = k + t + NT * r < n ? p[r][c] : 0.0f;
+            for (int c = 0; c < PB; ++c) p[r][c] = t + NT * r < m ? p[r][c] : 0.0f;
     } else {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int gi = k + t + NT * r;
-            const float *src = a + (int64_t)gi * lda + k;
+            const int i = t + NT * r;
+            const float *row = src + (int64_t)i * ld;
 #pragma unroll
-            for (int c = 0; c < PB; ++c) p[r][c] = (gi < n && c < jb) ? src[c] : 0.0f;
+            for (int c = 0; c < PB; ++c) p[r][c] = (i < m && c < jb) ? row[c] : 0.0f;
         }
     }
 }
@@ -363,8 +299,8 @@ __device__ __forceinline__ void store_panel(const float (&p)[R][PB], const int (
 constexpr int SP = PB + 4;  // row pitch in LDS: 80 bytes, no bank conflicts on row-wise reads
 
 template <int NT, int R>
-__device__ __forceinline__ void load_panel_staged(float (&p)[R][PB], const float *__restrict__ a, int lda, int n,
-                                                  int k, int t, float *stage)
+__device__ __forceinline__ void load_panel_staged(float (&p)[R][PB], const float *__restrict__ src, int ld, int m,
+                                                  int t, float *stage)
 {
     const int sub = t >> 2, ch = 4 * (t & 3);
     f4 v[4];
@@ -372,7 +308,7 @@ __device__ __forceinline__ void load_panel_staged(float (&p)[R][PB], const float
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int row = NT * r + sub + (NT / 4) * q;
-            v[q] = *(const f4 *)(a + (int64_t)min(k + row, n - 1) * lda + k + ch);
+            v[q] = *(const f4 *)(src + (int64_t)min(row, m - 1) * ld + ch);
         }
     };
     fetch(0);
@@ -387,29 +323,9 @@ __device__ __forceinline__ void load_panel_staged(float (&p)[R][PB], const float
         // into aligned quads, and the four-row kernel spills
 #pragma unroll
         for (int c = 0; c < PB; ++c) p[r][c] = stage[t * SP + c];
-        // rows past n are zeros
+        // rows past m are zeros
 #pragma unroll
-        for (int c = 0; c < PB; ++c) p[r][c] = k + t + NT * r < n ? p[r][c] : 0.0f;
-    }
-}
-
-// register row r of lane t back to matrix row k + t + NT r (rows in place)
-template <int NT, int R>
-__device__ __forceinline__ void store_panel_staged(const float (&p)[R][PB], float *__restrict__ a, int lda, int n,
-                                                   int k, int t, float *stage)
-{
-    const int sub = t >> 2, ch = 4 * (t & 3);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        __syncthreads();  // the previous round's (or the caller's) LDS reads are done
-#pragma unroll
-        for (int c = 0; c < PB; ++c) stage[t * SP + c] = p[r][c];
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int row = NT * r + sub + (NT / 4) * q;
-            if (k + row < n) *(f4 *)(a + (int64_t)(k + row) * lda + k + ch) = *(const f4 *)(stage + (sub + (NT / 4) * q) * SP + ch);
-        }
+        for (int c = 0; c < PB; ++c) p[r][c] = t + NT * r < m ? p[r][c] : 0.0f;
     }
 }
 
@@ -425,13 +341,13 @@ __device__ __forceinline__ void store_panel_staged(const float (&p)[R][PB], floa
 constexpr int FW = NB - PB;  // follow-up columns at most
 
 template <int NT>
-__device__ __forceinline__ void prefetch_a12(const float *__restrict__ a, int lda, int k, int jb, int c0, int c1,
-                                             int t, float (&pre)[(PB * FW + NT - 1) / NT])
+__device__ __forceinline__ void prefetch_a12(const float *__restrict__ src, int ld, int jb, int nc, int t,
+                                             float (&pre)[(PB * FW + NT - 1) / NT])
 {
 #pragma unroll
     for (int q = 0; q < (PB * FW + NT - 1) / NT; ++q) {
         const int e = t + NT * q, i = e / FW, cc = e % FW;
-        pre[q] = (e < PB * FW && i < jb && c0 + cc < c1) ? a[(int64_t)(k + i) * lda + c0 + cc] : 0.0f;
+        pre[q] = (e < PB * FW && i < jb && cc < nc) ? src[(int64_t)i * ld + cc] : 0.0f;
     }
 }
 
@@ -486,94 +402,226 @@ __device__ __forceinline__ void panel_follow_up(float *__restrict__ a, int lda, 
         if (i < jb) a[(int64_t)(k + i) * lda + c] = u[i];
 }
 
-// One panel: the speculative exchange-free pass first (when `speculate`),
-// then, if some column needs an exchange, the panel reloaded and factored by
-// the pivoting steps.  Full 16-column panels with 16-byte rows (VEC) move
-// through LDS as coalesced chunks; the pivoting pass writes its rows back
-// one 64-byte row per lane, since exchanged rows go to their labels.
-// PASS: 0 both passes; 1 the speculative pass alone, recording in *done
-// whether it factored the panel; 2 the pivoting pass alone, returning at
-// once when *done says the launch before it factored the panel (done may be
-// NULL).  The 1024 x 4 shape takes passes 1 and 2 as two launches: both
-// passes in one kernel would spill its registers.
-template <int NT, int R, bool VEC, int PASS>
-__global__ __launch_bounds__(NT) void k_panel(float *__restrict__ a, int lda, int n, int k, int jb,
-                                              int32_t *__restrict__ ipiv, int *__restrict__ info, int speculate,
-                                              int *__restrict__ done, int c1)
+// ------------------------------------- exchange-free panels, speculatively
+//
+// Partial pivoting takes the first row of largest magnitude among the rows
+// at or below gj, and the row at gj comes first: it is the pivot exactly when
+// no row below it has a larger magnitude, magnitudes compared as the bits of
+// |x| as in the pivoting step's key (so NaN ranks above every number and NaN
+// payloads decide between NaNs).  Assuming that for every column of the
+// panel, the pivot row of step J is the panel's row J after its own J steps,
+// and every other row is independent of the rest: row i takes steps
+// J < min(i, jb), each with the pivoting step's own multiplier and update.
+// So the speculative pass runs over many workgroups with no barrier:
+//   * every wave factors the panel's top jb rows itself, lane J holding row
+//     k + J, and reads the pivot row of step J out of lane J;
+//   * each lane takes one panel row, and flags the step's pivot search
+//     should that row have won it (a larger magnitude than the pivot);
+//   * the rows' original values go to a backup, their results in place,
+//     except the top rows, which every workgroup reads at its start: their
+//     results go to a side buffer that the finishing launch copies in;
+//   * block 0's wave also records identity pivots and the first exactly zero
+//     pivot column, and (from the top block's L11) applies U12 = L11^-1 A12
+//     to the block's later columns, after backing those rows up.
+// The finishing launch (k_panel below) reads the flag: clear, the panel is
+// factored and the pivots are those of the pivoting steps; raised, it
+// restores the panel and the U12 rows from the backup and factors the panel
+// by the pivoting steps.  The factors are the pivoting steps' in either case.
+//
+// Workspace: top[PB * PB] (the top rows' results), bk12[PB * FW] (the U12
+// rows' originals), bk[m * PB] (the panel's originals).  flag: 0 on entry,
+// set by any flagged row, cleared again by the finishing launch.
+
+template <int J>
+__device__ __forceinline__ void spec_step(float (&top)[PB], float (&x)[PB], int lane, int i, bool own, int jb,
+                                          uint32_t &viol, int &zero_col)
 {
-    constexpr bool SPLIT = PASS != 0;
-    if constexpr (PASS == 2) {
-        if (done && *done) return;  // uniform
-        speculate = 0;
+    if (J >= jb) return;  // uniform
+    float u[PB];
+#pragma unroll
+    for (int c = J; c < PB; ++c) u[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(top[c]), J));
+    const float d = u[J];
+    const uint32_t db = (uint32_t)__float_as_int(d) & 0x7fffffffu;
+    const bool below = own && i > J;
+    viol |= (below && ((uint32_t)__float_as_int(x[J]) & 0x7fffffffu) > db) ? 1u : 0u;
+    if (d != 0.0f) {
+        const bool rcp = fabsf(d) >= __FLT_MIN__;  // uniform
+        const float rd = 1.0f / d;
+        if (lane > J) fast_row<J>(top, u, d, rd, rcp);
+        if (below) fast_row<J>(x, u, d, rd, rcp);
+    } else if (zero_col < 0) {
+        zero_col = J;  // d is the same in every lane
     }
-    __shared__ __attribute__((aligned(16))) float s_row[2][PB];
+}
+
+template <int J>
+struct SpecLoop {
+    static __device__ __forceinline__ void run(float (&top)[PB], float (&x)[PB], int lane, int i, bool own, int jb,
+                                               uint32_t &viol, int &zero_col)
+    {
+        spec_step<J>(top, x, lane, i, own, jb, viol, zero_col);
+        SpecLoop<J + 1>::run(top, x, lane, i, own, jb, viol, zero_col);
+    }
+};
+
+template <>
+struct SpecLoop<PB> {
+    static __device__ __forceinline__ void run(float (&)[PB], float (&)[PB], int, int, bool, int, uint32_t &, int &)
+    {
+    }
+};
+
+// one panel row (jb columns, zeros after) from src; VEC: 16-byte aligned
+template <bool VEC>
+__device__ __forceinline__ void load_row(float (&v)[PB], const float *__restrict__ src, int jb)
+{
+    if constexpr (VEC) {
+        get_row(v, src);
+    } else {
+#pragma unroll
+        for (int c = 0; c < PB; ++c) v[c] = c < jb ? src[c] : 0.0f;
+    }
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(64) void k_panel_spec(float *__restrict__ a, int lda, int n, int k, int jb, int c1,
+                                                   float *__restrict__ top_out, float *__restrict__ bk12,
+                                                   float *__restrict__ bk, int *__restrict__ flag,
+                                                   int *__restrict__ zc, int32_t *__restrict__ ipiv)
+{
+    const int lane = threadIdx.x, m = n - k;
+    const int i = blockIdx.x * 64 + lane;  // panel row k + i
+    const bool own = i < m;
+    const float *src = a + (int64_t)k * lda + k;
+    float top[PB], x[PB];
+    // lanes from jb on hold copies of the last top row, never read out
+    load_row<VEC>(top, src + (int64_t)min(lane, jb - 1) * lda, jb);
+    load_row<VEC>(x, src + (int64_t)min(i, m - 1) * lda, jb);
+    // block 0: column c0 + lane of the block's later columns, rows k .. k + jb - 1
+    const int c0 = k + jb, nc = c1 - c0;
+    const bool follow = blockIdx.x == 0 && nc > 0;  // uniform
+    float col[PB];
+    if (follow) {
+#pragma unroll
+        for (int q = 0; q < PB; ++q) col[q] = a[(int64_t)(k + min(q, jb - 1)) * lda + c0 + min(lane, nc - 1)];
+    }
+    if (own) put_row(bk + (int64_t)i * PB, x);
+    uint32_t viol = 0;
+    int zero_col = -1;
+    SpecLoop<0>::run(top, x, lane, i, own, jb, viol, zero_col);
+    if (own) {
+        if (i < PB) {
+            put_row(top_out + i * PB, x);
+        } else if constexpr (VEC) {
+            put_row(a + (int64_t)(k + i) * lda + k, x);
+        } else {
+#pragma unroll
+            for (int c = 0; c < PB; ++c)
+                if (c < jb) a[(int64_t)(k + i) * lda + k + c] = x[c];
+        }
+    }
+    if (__ballot(viol != 0) != 0 && lane == 0) __hip_atomic_fetch_or(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (blockIdx.x == 0) {
+        if (lane < jb) ipiv[k + lane] = k + lane;
+        if (lane == 0) *zc = zero_col + 1;
+        if (follow) {
+            if (lane < nc)
+#pragma unroll
+                for (int q = 0; q < PB; ++q)
+                    if (q < jb) bk12[q * FW + lane] = col[q];
+            // U12 = L11^-1 A12 as the pivoting launch's follow-up forms it:
+            // each row's terms in ascending column order, unfused; L11[q][j]
+            // is lane q's multiplier j
+#pragma unroll
+            for (int j = 0; j < PB - 1; ++j) {
+#pragma unroll
+                for (int q = j + 1; q < PB; ++q)
+                    if (q < jb) col[q] = col[q] - __int_as_float(__builtin_amdgcn_readlane(__float_as_int(top[j]), q)) * col[j];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (lane < nc)
+#pragma unroll
+                for (int q = 0; q < PB; ++q)
+                    if (q < jb) a[(int64_t)(k + q) * lda + c0 + lane] = col[q];
+        }
+    }
+}
+
+// One panel by the pivoting steps, after the speculative launch when flag is
+// not NULL: a clear flag means that launch factored the panel, and only its
+// top rows (and the first zero pivot, into info) are taken over; a raised
+// flag means the panel and the U12 rows are restored from the backup first.
+// Full 16-column panels with 16-byte rows (VEC) are loaded through LDS as
+// coalesced chunks; the rows are written back one 64-byte row per lane,
+// since exchanged rows go to their labels.  Then the panel's exchanges and
+// U12 on the block's later columns [k + jb, c1), and the exchanges on the
+// block's earlier columns [kb, k).
+template <int NT, int R, bool VEC>
+__global__ __launch_bounds__(NT) void k_panel(float *__restrict__ a, int lda, int n, int kb, int k, int jb,
+                                              int32_t *__restrict__ ipiv, int *__restrict__ info, int c1,
+                                              int *__restrict__ flag, const float *__restrict__ top,
+                                              const float *__restrict__ bk12, const float *__restrict__ bk,
+                                              const int *__restrict__ zc)
+{
     __shared__ __attribute__((aligned(16))) float s_stage[VEC ? NT * SP : 4];
     __shared__ PanelShared<NT, R> sh;
     __shared__ float s_l[PB][PB + 1];
     __shared__ float s_a12[PB][FW + 1];
     const int t = threadIdx.x;
-    const int c0 = k + jb;
+    const int c0 = k + jb, m = n - k;
+    const float *src = a + (int64_t)k * lda + k, *src12 = a + (int64_t)k * lda + c0;
+    int ld = lda, ld12 = lda;
+    if (flag) {
+        const int raised = *flag;
+        const float tv = top[t & (PB * PB - 1)];
+        if (!raised) {  // uniform
+            if (t < PB * PB) {
+                const int r = t >> 4, c = t & (PB - 1);
+                if (r < jb && c < jb) a[(int64_t)(k + r) * lda + k + c] = tv;
+            }
+            if (t == 0) {
+                const int z = *zc;
+                if (z && *info == 0) *info = k + z;
+            }
+            return;
+        }
+        __syncthreads();  // every lane has read the flag
+        if (t == 0) *flag = 0;
+        src = bk;
+        ld = PB;
+        src12 = bk12;
+        ld12 = FW;
+    }
     float p[R][PB];
-    auto load = [&]() {
-        if constexpr (VEC) load_panel_staged<NT, R>(p, a, lda, n, k, t, s_stage);
-        else load_panel<NT, R, VEC>(p, a, lda, n, k, jb, t);
-    };
     {
         // the follow-up's rows of the block's later columns, in flight with
         // the panel's own loads
         float pre[(PB * FW + NT - 1) / NT];
-        prefetch_a12<NT>(a, lda, k, jb, c0, c1, t, pre);
-        load();
+        prefetch_a12<NT>(src12, ld12, jb, c1 - c0, t, pre);
+        if constexpr (VEC) load_panel_staged<NT, R>(p, src, ld, m, t, s_stage);
+        else load_panel<NT, R, VEC>(p, src, ld, m, jb, t);
         stage_a12<NT>(s_a12, t, pre);
     }
-    if (speculate) {
-        uint32_t viol = 0;
-        int zero_col = -1;
-        if (t == 0) put_row(s_row[0], p[0]);
-        // a column that needs an exchange is almost always found in the
-        // first column, checked alone
-        fast_step<NT, R, 0>(p, s_row, t, n, k, jb, viol, zero_col);
-        bool ok = !__syncthreads_or((int)viol);
-        if (ok) {
-            FastLoop<NT, R, 1>::run(p, s_row, t, n, k, jb, viol, zero_col);
-            ok = !__syncthreads_or((int)viol);
-        }
-        if constexpr (SPLIT) {
-            if (t == 0) *done = ok ? 1 : 0;
-        }
-        if (ok) {
-            if (t < jb) ipiv[k + t] = k + t;
-            if (t == 0 && zero_col >= 0 && *info == 0) *info = k + zero_col + 1;
-            if constexpr (VEC) {
-                store_panel_staged<NT, R>(p, a, lda, n, k, t, s_stage);
-            } else {
-                int lab[R];
-#pragma unroll
-                for (int r = 0; r < R; ++r) lab[r] = k + t + NT * r;
-                store_panel<NT, R, VEC>(p, lab, a, lda, n, k, jb);
-            }
-            if (c0 < c1) {  // uniform
-                // L11: rows k .. k + jb - 1, register row 0 of lanes 0 .. jb - 1
-                if (t < jb)
-#pragma unroll
-                    for (int c = 0; c < PB; ++c) s_l[t][c] = p[0][c];
-                __syncthreads();
-                const int none[PB] = {};
-                panel_follow_up<false>(a, lda, k, jb, c0, c1, none, s_l, s_a12, t);
-            }
-            return;
-        }
-        if constexpr (SPLIT) return;
-        __syncthreads();  // the stage buffer is reused by the reload
-        load();
-    }
-    if constexpr (PASS == 1) return;
     int lab[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) lab[r] = k + t + NT * r;
     int pv[PB];
     PanelLoop<NT, R, 0>::run(p, lab, sh, t, n, k, jb, ipiv, info, pv);
     store_panel<NT, R, VEC>(p, lab, a, lda, n, k, jb);
+    // the block's earlier columns [kb, k) take this panel's exchanges (the
+    // pivots are the same in every lane; at most 48 columns)
+    if (t < k - kb) {
+        const int c = kb + t;
+#pragma unroll
+        for (int j = 0; j < PB; ++j) {
+            const int pj = pv[j];
+            if (j < jb && pj != k + j) {
+                const float x = a[(int64_t)(k + j) * lda + c];
+                a[(int64_t)(k + j) * lda + c] = a[(int64_t)pj * lda + c];
+                a[(int64_t)pj * lda + c] = x;
+            }
+        }
+    }
     if (c0 < c1) {  // uniform
         // L11: the rows now labelled k .. k + jb - 1
 #pragma unroll
@@ -653,31 +701,28 @@ __global__ __launch_bounds__(256) void k_panel_global(float *__restrict__ a, int
 
 // ----------------------------------------------------- swap + U12 TRSM
 
-// columns [c0, c1): the exchanges of steps k0 .. k1-1, in order
-// the exchanges of a block's steps kb .. kb+nbk-1, one thread per column of
-// the whole matrix, exchanges in order: finished L columns on the left and
-// trailing columns take all of them; the block's own columns take those of
-// the panels after their own (the earlier ones were applied as the panels
-// were factored)
-__global__ __launch_bounds__(256) void k_laswp(float *__restrict__ a, int lda, int n, int kb, int nbk,
-                                               const int32_t *__restrict__ ipiv)
+// the exchanges of steps k0 .. k0 + cnt - 1 (cnt <= NB), in order, applied
+// to the columns [cb, ce): one thread per column
+__device__ __forceinline__ void swap_columns(float *__restrict__ a, int lda, int k0, int cnt, const int *s_p, int c)
 {
-    // the block's pivots, read once (nbk <= NB)
-    __shared__ int s_p[NB];
-    if (threadIdx.x < nbk) s_p[threadIdx.x] = ipiv[kb + threadIdx.x];
-    __syncthreads();
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= n) return;
-    const int k1 = kb + nbk;
-    const int k0 = (c >= kb && c < k1) ? kb + ((c - kb) / PB + 1) * PB : kb;
-    for (int j = k0; j < k1; ++j) {
-        const int pj = s_p[j - kb];
-        if (pj != j) {
-            const float x = a[(int64_t)j * lda + c];
-            a[(int64_t)j * lda + c] = a[(int64_t)pj * lda + c];
+    for (int j = 0; j < cnt; ++j) {
+        const int pj = s_p[j];
+        if (pj != k0 + j) {
+            const float x = a[(int64_t)(k0 + j) * lda + c];
+            a[(int64_t)(k0 + j) * lda + c] = a[(int64_t)pj * lda + c];
             a[(int64_t)pj * lda + c] = x;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_laswp(float *__restrict__ a, int lda, int k0, int cnt,
+                                               const int32_t *__restrict__ ipiv, int cb, int ce)
+{
+    __shared__ int s_p[NB];
+    if (threadIdx.x < cnt) s_p[threadIdx.x] = ipiv[k0 + threadIdx.x];
+    __syncthreads();
+    const int c = cb + blockIdx.x * 256 + threadIdx.x;
+    if (c < ce) swap_columns(a, lda, k0, cnt, s_p, c);
 }
 
 // columns [c0, c1): apply the exchanges of steps k .. k+jb-1 (jb <= W),
@@ -732,20 +777,38 @@ typedef float float4v __attribute__((ext_vector_type(4)));
 
 constexpr int TW = 64;   // block rows
 constexpr int TXP = 17;  // pitch of a wave's 64 x 16 block in LDS
-__global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int lda, int kb, int nbk, int c0, int c1)
+//
+// The block's exchanges come first, in the same launch: each workgroup
+// applies them to its own 64 columns before reading them, and the
+// workgroups past the trailing columns' apply them to the finished columns
+// on the left, [0, kb) (the block's own columns took them from the panels).
+__global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int lda, int kb, int nbk, int c0, int c1,
+                                                    const int32_t *__restrict__ ipiv)
 {
     __shared__ __attribute__((aligned(16))) float s_nl[TW][TW + 4];  // -L11, strictly lower
     __shared__ float s_x[4][TW * TXP];
+    __shared__ int s_p[NB];
     const int t = threadIdx.x, lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    if (t < nbk) s_p[t] = ipiv[kb + t];
+    __syncthreads();
+    const int ntrsm = (c1 - c0 + 63) / 64;
+    if ((int)blockIdx.x >= ntrsm) {  // uniform: a left-column workgroup
+        const int c = ((int)blockIdx.x - ntrsm) * 256 + t;
+        if (c < kb) swap_columns(a, lda, kb, nbk, s_p, c);
+        return;
+    }
     // every load is issued from a clamped, valid address before the first
-    // LDS write (a predicated load would wait out its own round trip)
+    // LDS write (a predicated load would wait out its own round trip); L11
+    // lies in the block's columns, which these exchanges leave alone
     float ld[TW * TW / 256];
 #pragma unroll
     for (int it = 0; it < TW * TW / 256; ++it) {
         const int e = t + 256 * it, i = e / TW, j = e % TW;
         ld[it] = a[(int64_t)(kb + min(i, nbk - 1)) * lda + kb + min(j, nbk - 1)];
     }
+    if (t < 64 && c0 + (int)blockIdx.x * 64 + t < c1) swap_columns(a, lda, kb, nbk, s_p, c0 + blockIdx.x * 64 + t);
+    __syncthreads();  // the exchanged rows are read below by other lanes
     // this wave's columns cw .. cw + 15; lane moves column cw + (lane & 15)
     // of rows (lane >> 4) + 4 q
     const int cw = c0 + blockIdx.x * 64 + w * 16, lc = lane & 15, lr = lane >> 4;
@@ -1423,55 +1486,78 @@ int solve_status(piro_ctx *ctx, const SolvePlan &pl)
     return h;
 }
 
-// factors the panel and, unless it is the global-memory panel (returns
-// false), also applies its exchanges and U12 to the block's columns [k + jb, c1)
-bool panel(piro_ctx *ctx, int n, float *a, int lda, int k, int jb, int c1, int32_t *ipiv, int *d_info)
-{
-    hipStream_t s = ctx->stream;
-    const int m = n - k;
-    // register panels hold NT x R rows; taller panels work in global memory.
-    // Shapes by panel height, from kernel times of the exchange-free pass
-    // (the pass the backward-Euler systems take): at 4096 rows 1024 x 4 takes
-    // 19 us, 512 x 8 22 us; at 3072 rows 512 x 6 19 us; at 2048 512 x 4 14
-    // us, 1024 x 2 19 us; at 1024 rows 256 x 4 12 us; at 512 256 x 2 10 us.
-    const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0) && jb == PB;
-    const int spec = ctx->lu_speculate;
-    int *done = d_info + 1;
-#define PIRO_PANEL(NT_, R_)                                                                                   \
-    {                                                                                                         \
-        if (vec) k_panel<NT_, R_, true, 0><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, spec, done, c1);     \
-        else k_panel<NT_, R_, false, 0><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, spec, done, c1);        \
+// The factorisation's workspace: info (int 0), the speculative panels' flag
+// (int 32) and first zero pivot column (int 64), each in its own 128-byte
+// line; then the speculative panel's side buffer and backups.
+struct FactorWs {
+    void *base;
+    int *info, *flag, *zc;
+    float *top, *bk12, *bk;
+    static constexpr size_t HEAD = 512, TOP = PB * PB * 4, BK12 = PB * FW * 4;
+    static size_t bytes(int n) { return HEAD + TOP + BK12 + (size_t)n * PB * 4; }
+    explicit FactorWs(void *b_) : base(b_)
+    {
+        char *b = (char *)b_;
+        info = (int *)b;
+        flag = info + 32;
+        zc = info + 64;
+        top = (float *)(b + HEAD);
+        bk12 = (float *)(b + HEAD + TOP);
+        bk = (float *)(b + HEAD + TOP + BK12);
     }
-#define PIRO_PANEL_SPLIT(NT_, R_)                                                                                \
-    {                                                                                                            \
-        if (vec) {                                                                                               \
-            if (spec) k_panel<NT_, R_, true, 1><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 1, done, c1);      \
-            k_panel<NT_, R_, true, 2><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 0, spec ? done : 0, c1);    \
-        } else {                                                                                                 \
-            if (spec) k_panel<NT_, R_, false, 1><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 1, done, c1);     \
-            k_panel<NT_, R_, false, 2><<<1, NT_, 0, s>>>(a, lda, n, k, jb, ipiv, d_info, 0, spec ? done : 0, c1);   \
-        }                                                                                                        \
+};
+
+// factors the panel, applies its exchanges to the block's earlier columns
+// [kb, k) and, unless it is the global-memory panel (returns false), its
+// exchanges and U12 to the block's later columns [k + jb, c1)
+bool panel(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int kb, int k, int jb, int c1, int32_t *ipiv,
+           const FactorWs &ws)
+{
+    const int m = n - k;
+    if (m > 4096) {
+        k_panel_global<<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, ws.info);
+        check_launch();
+        if (k > kb) {  // the block's earlier columns take the panel's exchanges
+            k_laswp<<<1, 256, 0, s>>>(a, lda, k, jb, ipiv, kb, k);
+            check_launch();
+        }
+        return false;
+    }
+    const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0) && jb == PB;
+    const bool spec = ctx->lu_speculate != 0;
+    if (spec) {
+        const int g = (m + 63) / 64;
+        if (vec) k_panel_spec<true><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, ws.top, ws.bk12, ws.bk, ws.flag, ws.zc, ipiv);
+        else k_panel_spec<false><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, ws.top, ws.bk12, ws.bk, ws.flag, ws.zc, ipiv);
+        check_launch();
+    }
+    int *flag = spec ? ws.flag : nullptr;
+    // register panels hold NT x R rows (the pivoting steps' shapes by panel
+    // height, R as small as the 1024-lane limit allows)
+#define PIRO_PANEL(NT_, R_)                                                                                          \
+    {                                                                                                                \
+        if (vec)                                                                                                     \
+            k_panel<NT_, R_, true><<<1, NT_, 0, s>>>(a, lda, n, kb, k, jb, ipiv, ws.info, c1, flag, ws.top, ws.bk12, ws.bk, \
+                                                     ws.zc);                                                         \
+        else                                                                                                         \
+            k_panel<NT_, R_, false><<<1, NT_, 0, s>>>(a, lda, n, kb, k, jb, ipiv, ws.info, c1, flag, ws.top, ws.bk12,      \
+                                                      ws.bk, ws.zc);                                                 \
     }
     if (m <= 256) PIRO_PANEL(256, 1)
     else if (m <= 512) PIRO_PANEL(256, 2)
     else if (m <= 1024) PIRO_PANEL(256, 4)
     else if (m <= 2048) PIRO_PANEL(512, 4)
     else if (m <= 3072) PIRO_PANEL(512, 6)
-    else if (m <= 4096) PIRO_PANEL_SPLIT(1024, 4)
-    else {
-        k_panel_global<<<1, 256, 0, s>>>(a, lda, n, k, jb, ipiv, d_info);
-        check_launch();
-        return false;
-    }
+    else PIRO_PANEL(1024, 4)
 #undef PIRO_PANEL
-#undef PIRO_PANEL_SPLIT
     check_launch();
     return true;
 }
 
-void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
+void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const FactorWs &ws)
 {
-    hipStream_t s = ctx->stream;
+    k_zero_i32<<<1, 256, 0, s>>>(ws.info, (int)(FactorWs::HEAD / 4));
+    check_launch();
     const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0);
     for (int kb = 0; kb < n; kb += NB) {
         const int nbk = min(NB, n - kb);
@@ -1479,7 +1565,7 @@ void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
         for (int k = kb; k < kb + nbk; k += PB) {
             const int jb = min(PB, kb + nbk - k);
             const int c0 = k + jb, c1 = kb + nbk;
-            const bool followed = panel(ctx, n, a, lda, k, jb, c1, ipiv, d_info);
+            const bool followed = panel(ctx, s, n, a, lda, kb, k, jb, c1, ipiv, ws);
             if (c0 < c1) {
                 if (!followed) {
                     k_swap_trsm<PB><<<grid_for(c1 - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, c1, ipiv);
@@ -1491,19 +1577,22 @@ void factor(piro_ctx *ctx, int n, float *a, int lda, int32_t *ipiv, int *d_info)
                 check_launch();
             }
         }
-        // all of the block's exchanges first (rows below the block carry
-        // deferred updates, so no exchange may come after an update), then
-        // U12 = L11^-1 A12 for the whole block, then one K = nbk update of
-        // every row below the block
-        k_laswp<<<grid_for(n, 256), 256, 0, s>>>(a, lda, n, kb, nbk, ipiv);
-        check_launch();
+        // all of the block's exchanges first on the columns outside the
+        // block (rows below the block carry deferred updates, so no exchange
+        // may come after an update), then U12 = L11^-1 A12 for the whole
+        // block (both in one launch), then one K = nbk update of every row
+        // below the block
         const int c0 = kb + nbk;
         if (c0 < n) {
             static_assert(NB == TW, "block TRSM rows");
-            k_trsm_block<<<(n - c0 + 63) / 64, 256, 0, s>>>(a, lda, kb, nbk, c0, n);
+            const int gt = (n - c0 + 63) / 64 + (kb + 255) / 256;
+            k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv);
             check_launch();
             dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
             k_update_mfma<NB, 2, 2><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vec && kb % 4 == 0 && c0 % 4 == 0);
+            check_launch();
+        } else if (kb > 0) {
+            k_laswp<<<grid_for(kb, 256), 256, 0, s>>>(a, lda, kb, nbk, ipiv, 0, kb);
             check_launch();
         }
     }
@@ -1531,13 +1620,12 @@ int piro_lu_factor_f32(piro_ctx *ctx, int n, float *d_a, int lda, int32_t *d_ipi
     ctx_of(ctx);
     PIRO_REQUIRE(n > 0 && lda >= n && d_a && d_ipiv, "bad arguments");
     PIRO_REQUIRE(n < (1 << 27), "n too large (row labels are packed in 27 bits)");
-    // info, then the speculative panels' done flag
-    int *d_info = (int *)ctx->scratch[0].get(2 * sizeof(int));
-    PIRO_CHECK_HIP(hipMemsetAsync(d_info, 0, 2 * sizeof(int), ctx->stream));
-    factor(ctx, n, d_a, lda, d_ipiv, d_info);
+    const FactorWs ws(ctx->scratch[0].get(FactorWs::bytes(n)));
+    hipStream_t s = ctx->stream;
+    factor(ctx, s, n, d_a, lda, d_ipiv, ws);
     int h = 0;
-    PIRO_CHECK_HIP(hipMemcpyAsync(&h, d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    PIRO_CHECK_HIP(hipMemcpyAsync(&h, ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
+    PIRO_CHECK_HIP(hipStreamSynchronize(s));
     if (info) *info = h;
     if (h) throw Error(PIRO_ESINGULAR, "exactly zero pivot at step " + std::to_string(h));
     PIRO_API_END

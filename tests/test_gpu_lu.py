@@ -167,6 +167,42 @@ def test_lu_strategies_bit_identical(ctx, n, kind):
             np.testing.assert_array_equal(piv, np.arange(n))
 
 
+@pytest.mark.parametrize("n", [300, 4096])
+def test_lu_refactoring_one_buffer(ctx, n):
+    # one buffer refactored with matrices that need exchanges in every panel
+    # (random), in a few (mixed) or in none (dominant), in an order that goes
+    # from each kind to every other: the workspace (the speculative panels'
+    # flag, backups and side buffer) carries over from call to call, and
+    # every call must give the factors and pivots that a factorisation of
+    # the same matrix on a second context gives, under both strategies
+    from piro_amd.core import Context
+    print(f"case lu refactoring n={n}", flush=True)
+    rng = np.random.default_rng(n + 77)
+    dom = exchange_mix(n, n + 1)
+    dom = dom[np.argsort(np.argmax(np.abs(dom), axis=1))]
+    mats = [rng.standard_normal((n, n)).astype(np.float32), exchange_mix(n, n + 2), dom,
+            rng.standard_normal((n, n)).astype(np.float32)]
+    ref_ctx = Context(0)
+    try:
+        for spec in (True, False):
+            ref_ctx.lu_set_speculation(spec)
+            ctx.lu_set_speculation(spec)
+            refs = []
+            for a in mats:
+                d = dev(a, ref_ctx)
+                ipiv = ref_ctx.lu_factor(d, n)
+                refs.append((d.host().tobytes(), ipiv.host().tobytes()))
+            buf = dev(mats[0], ctx)
+            for it, j in enumerate([0, 1, 2, 3, 0, 2, 1]):
+                buf.upload(mats[j])
+                ipiv = ctx.lu_factor(buf, n)
+                assert buf.host().tobytes() == refs[j][0], (spec, it, j)
+                assert ipiv.host().tobytes() == refs[j][1], (spec, it, j)
+    finally:
+        ctx.lu_set_speculation(True)
+        ref_ctx.close()
+
+
 def test_lu_solve_f32_matches_numpy(ctx):
     n = 300
     rng = np.random.default_rng(3)
