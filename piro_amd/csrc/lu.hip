@@ -884,32 +884,24 @@ __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int l
 // their MFMAs (a uniform test).  vec: lda, the base, k and c0 allow 16-byte
 // accesses.
 template <int KW, int WGR, int WGC>
-__global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restrict__ a, int lda, int n, int k, int jb,
-                                                               int r0g, int c0g, int c1, bool vec)
-{
-    constexpr int NT = 64 * WGR * WGC, TR = 64 * WGR, TC = 64 * WGC;
+struct UpdShape {
+    static constexpr int NT = 64 * WGR * WGC, TR = 64 * WGR, TC = 64 * WGC;
     // pitches chosen so the operand reads of a wave hit 64 distinct banks
-    constexpr int PA = KW + 4, PBP = TC + 16;
-    __shared__ __attribute__((aligned(16))) float s_a[TR * PA];   // -L21
-    __shared__ __attribute__((aligned(16))) float s_b[KW * PBP];  // U12
-    const int r0 = r0g + blockIdx.y * TR, c0 = c0g + blockIdx.x * TC;
-    const int t = threadIdx.x, lane = t & 63;
-    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
-    const int rw = r0 + (w / WGC) * 64, cw = c0 + (w % WGC) * 64;
-    const int li = lane & 15, lk = lane >> 4;
+    static constexpr int PA = KW + 4, PBP = TC + 16;
+    static constexpr int NQA = TR * KW / 4 / NT, NQB = KW * TC / 4 / NT;
+    static_assert(NQA * NT * 4 == TR * KW && NQB * NT * 4 == KW * TC, "staging split");
+};
 
-    // this lane's C entries: rows rw + 16 ti + li, columns cw + 16 tj + 4 lk + r
-    int64_t roff[4];
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti) roff[ti] = (int64_t)min(rw + ti * 16 + li, n - 1) * lda;
-    auto tile_live = [&](int ti, int tj) { return rw + ti * 16 < n && cw + tj * 16 < c1; };
-    // with 16-byte rows and c0, c1 multiples of 4, a column quad lies wholly
-    // inside or outside the matrix: every quad is loaded from a clamped
-    // address, all loads in flight at once.  An outside quad's entries are
-    // never stored, and an accumulator entry depends on its own C entry
-    // only, so they need no clearing.
-    const bool quads = vec && c1 % 4 == 0;  // uniform
-    float4v acc[4][4];
+// this lane's C entries: rows rw + 16 ti + li, columns cw + 16 tj + 4 lk + r.
+// With 16-byte rows and c0, c1 multiples of 4 (quads), a column quad lies
+// wholly inside or outside the matrix: every quad is loaded from a clamped
+// address, all loads in flight at once.  An outside quad's entries are never
+// stored, and an accumulator entry depends on its own C entry only, so they
+// need no clearing.
+__device__ __forceinline__ void update_load_c(float4v (&acc)[4][4], const float *__restrict__ a,
+                                              const int64_t (&roff)[4], int n, int rw, int cw, int c1, bool quads,
+                                              int li, int lk)
+{
     if (quads) {
 #pragma unroll
         for (int ti = 0; ti < 4; ++ti)
@@ -926,33 +918,38 @@ __global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restric
                 for (int r = 0; r < 4; ++r) acc[ti][tj][r] = (row < n && col + r < c1) ? a[roff[ti] + col + r] : 0.0f;
             }
     }
+}
 
-    // stage the L21 tile (negated) and the U12 tile, four entries a thread.
-    // With 16-byte rows and a full K, every load is issued (rows and column
-    // quads clamped into the matrix) before the first LDS write, and the
-    // entries out of range are zeroed after; otherwise entry by entry.
-    constexpr int NQA = TR * KW / 4 / NT, NQB = KW * TC / 4 / NT;
-    static_assert(NQA * NT * 4 == TR * KW && NQB * NT * 4 == KW * TC, "staging split");
+// stage the L21 tile (negated) and the U12 tile, four entries a thread.
+// With 16-byte rows and a full K, every load is issued (rows and column
+// quads clamped into the matrix) before the first LDS write, and the
+// entries out of range are zeroed after; otherwise entry by entry.
+template <int KW, int WGR, int WGC>
+__device__ __forceinline__ void update_stage(float *s_a, float *s_b, const float *__restrict__ a, int lda, int n,
+                                             int k, int jb, int r0, int c0, int c1, bool vec, int t)
+{
+    using S = UpdShape<KW, WGR, WGC>;
+    constexpr int NT = S::NT, TR = S::TR, TC = S::TC, PA = S::PA, PBP = S::PBP;
     if (vec && jb == KW && c1 % 4 == 0) {  // uniform
-        float4v va[NQA], vb[NQB];
+        float4v va[S::NQA], vb[S::NQB];
 #pragma unroll
-        for (int it = 0; it < NQA; ++it) {
+        for (int it = 0; it < S::NQA; ++it) {
             const int e = t + NT * it, i = e / (KW / 4), kk = 4 * (e % (KW / 4));
             va[it] = *(const float4v *)(a + (int64_t)min(r0 + i, n - 1) * lda + k + kk);
         }
 #pragma unroll
-        for (int it = 0; it < NQB; ++it) {
+        for (int it = 0; it < S::NQB; ++it) {
             const int e = t + NT * it, kk = e / (TC / 4), j = 4 * (e % (TC / 4));
             // c0 and c1 are multiples of 4: a quad is wholly in or out
             vb[it] = *(const float4v *)(a + (int64_t)(k + kk) * lda + min(c0 + j, c1 - 4));
         }
 #pragma unroll
-        for (int it = 0; it < NQA; ++it) {
+        for (int it = 0; it < S::NQA; ++it) {
             const int e = t + NT * it, i = e / (KW / 4), kk = 4 * (e % (KW / 4));
             *(float4v *)(s_a + i * PA + kk) = r0 + i < n ? -va[it] : (float4v){0.0f, 0.0f, 0.0f, 0.0f};
         }
 #pragma unroll
-        for (int it = 0; it < NQB; ++it) {
+        for (int it = 0; it < S::NQB; ++it) {
             const int e = t + NT * it, kk = e / (TC / 4), j = 4 * (e % (TC / 4));
             *(float4v *)(s_b + kk * PBP + j) = c0 + j < c1 ? vb[it] : (float4v){0.0f, 0.0f, 0.0f, 0.0f};
         }
@@ -974,24 +971,52 @@ __global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restric
             *(float4v *)(s_b + kk * PBP + j) = v;
         }
     }
-    __syncthreads();
+}
 
-    const int wr = rw - r0, wc = cw - c0;
+// acc[ti][tj] += (-L21) U12 over K in steps of 4, tiles wholly outside the
+// matrix skipped; (wr, wc) the wave's tile within the workgroup's
+template <int KW, int WGR, int WGC>
+__device__ __forceinline__ void update_mfma(float4v (&acc)[4][4], const float *s_a, const float *s_b, int jb, int wr,
+                                            int wc, int rw, int cw, int n, int c1, int li, int lk)
+{
+    using S = UpdShape<KW, WGR, WGC>;
     for (int ks = 0; ks < KW; ks += 4) {
         if (ks >= jb) break;  // uniform; staged entries past jb are 0 anyway
         float fa[4], fb[4];
 #pragma unroll
-        for (int ti = 0; ti < 4; ++ti) fa[ti] = s_a[(wr + ti * 16 + li) * PA + ks + lk];
+        for (int ti = 0; ti < 4; ++ti) fa[ti] = s_a[(wr + ti * 16 + li) * S::PA + ks + lk];
 #pragma unroll
-        for (int tj = 0; tj < 4; ++tj) fb[tj] = s_b[(ks + lk) * PBP + wc + tj * 16 + li];
+        for (int tj = 0; tj < 4; ++tj) fb[tj] = s_b[(ks + lk) * S::PBP + wc + tj * 16 + li];
 #pragma unroll
         for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
             for (int tj = 0; tj < 4; ++tj)
-                if (tile_live(ti, tj))
+                if (rw + ti * 16 < n && cw + tj * 16 < c1)
                     acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[tj], fa[ti], acc[ti][tj], 0, 0, 0);
     }
+}
 
+template <int KW, int WGR, int WGC>
+__global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restrict__ a, int lda, int n, int k, int jb,
+                                                               int r0g, int c0g, int c1, bool vec)
+{
+    using S = UpdShape<KW, WGR, WGC>;
+    __shared__ __attribute__((aligned(16))) float s_a[S::TR * S::PA];   // -L21
+    __shared__ __attribute__((aligned(16))) float s_b[KW * S::PBP];  // U12
+    const int r0 = r0g + blockIdx.y * S::TR, c0 = c0g + blockIdx.x * S::TC;
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int rw = r0 + (w / WGC) * 64, cw = c0 + (w % WGC) * 64;
+    const int li = lane & 15, lk = lane >> 4;
+    int64_t roff[4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) roff[ti] = (int64_t)min(rw + ti * 16 + li, n - 1) * lda;
+    const bool quads = vec && c1 % 4 == 0;  // uniform
+    float4v acc[4][4];
+    update_load_c(acc, a, roff, n, rw, cw, c1, quads, li, lk);
+    update_stage<KW, WGR, WGC>(s_a, s_b, a, lda, n, k, jb, r0, c0, c1, vec, t);
+    __syncthreads();
+    update_mfma<KW, WGR, WGC>(acc, s_a, s_b, jb, rw - r0, cw - c0, rw, cw, n, c1, li, lk);
     if (quads) {
 #pragma unroll
         for (int ti = 0; ti < 4; ++ti)
@@ -1013,6 +1038,113 @@ __global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restric
     }
 }
 
+
+// The in-block update of a full panel's later columns [c0, c1) (c0 = k + 16,
+// c1 - c0 <= 48: one wave per 64-row tile, the update k_update_mfma<16, 1, 1>
+// makes) fused with the speculative pass of the next panel (columns c0 ..
+// c0 + 15, rows c0 .. n - 1; it is full and at most 4096 rows high).  Each
+// wave holds its rows' next-panel columns after the update; the next
+// panel's top block (rows c0 .. c0 + 15) after the update is the owner's
+// (block row 0) tile (0, 0), and every wave forms it itself, from the same
+// operands by the same MFMAs, so it is the same bits.  The pass then runs
+// as in k_panel_spec.  The top block's rows keep their values before the
+// update in memory (every wave reads them there): their results go to the
+// side buffer, which the finishing launch copies in.  The owner also holds
+// the next panel's A12 (the top rows' later columns) and applies that
+// panel's U12 = L11^-1 A12 to it.
+__global__ __launch_bounds__(64) void k_update16_spec(float *__restrict__ a, int lda, int n, int k, int c1,
+                                                      float *__restrict__ top_out, float *__restrict__ bk12,
+                                                      float *__restrict__ bk, int *__restrict__ flag,
+                                                      int *__restrict__ zc, int32_t *__restrict__ ipiv)
+{
+    using S = UpdShape<PB, 1, 1>;
+    __shared__ __attribute__((aligned(16))) float s_a[S::TR * S::PA];  // -L21
+    __shared__ __attribute__((aligned(16))) float s_b[PB * S::PBP];    // U12
+    __shared__ __attribute__((aligned(16))) float s_at[PB * S::PA];    // -L21 of the next panel's top rows
+    __shared__ float s_rows[64][PB + 1];  // the tile's next-panel columns, one row per lane
+    __shared__ float s_top[PB][PB + 1];
+    __shared__ float s_fu[PB][2 * PB + 1];  // the owner's top rows, columns c0 + 16 .. c0 + 47
+    const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+    const int c0 = k + PB, r0 = c0 + blockIdx.y * 64;
+    const bool owner = blockIdx.y == 0;  // uniform
+    int64_t roff[4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) roff[ti] = (int64_t)min(r0 + ti * 16 + li, n - 1) * lda;
+    float4v acc[4][4];
+    update_load_c(acc, a, roff, n, r0, c0, c1, true, li, lk);
+    // the next panel's top block before the update, and its rows' L21
+    float4v ctop = *(const float4v *)(a + (int64_t)(c0 + li) * lda + c0 + 4 * lk);
+    const float4v vat = *(const float4v *)(a + (int64_t)(c0 + li) * lda + k + 4 * lk);
+    update_stage<PB, 1, 1>(s_a, s_b, a, lda, n, k, PB, r0, c0, c1, true, lane);
+    *(float4v *)(s_at + li * S::PA + 4 * lk) = -vat;
+    __syncthreads();
+    update_mfma<PB, 1, 1>(acc, s_a, s_b, PB, 0, 0, r0, c0, n, c1, li, lk);
+#pragma unroll
+    for (int ks = 0; ks < PB; ks += 4)
+        ctop = __builtin_amdgcn_mfma_f32_16x16x4f32(s_b[(ks + lk) * S::PBP + li], s_at[li * S::PA + ks + lk], ctop, 0,
+                                                    0, 0);
+    // rows to lanes through LDS
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s_rows[16 * ti + li][4 * lk + r] = acc[ti][0][r];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s_top[li][4 * lk + r] = ctop[r];
+    if (owner)
+#pragma unroll
+        for (int tj = 1; tj < 3; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) s_fu[li][16 * (tj - 1) + 4 * lk + r] = acc[0][tj][r];
+    __syncthreads();
+    float top[PB], x[PB];
+#pragma unroll
+    for (int c = 0; c < PB; ++c) {
+        top[c] = s_top[min(lane, PB - 1)][c];
+        x[c] = s_rows[lane][c];
+    }
+    const int i = r0 - c0 + lane;  // the next panel's row c0 + i
+    const bool own = r0 + lane < n;
+    if (own) put_row(bk + (int64_t)i * PB, x);
+    uint32_t viol = 0;
+    int zero_col = -1;
+    SpecLoop<0>::run(top, x, lane, i, own, PB, viol, zero_col);
+    if (own) put_row(i < PB ? top_out + i * PB : a + (int64_t)(r0 + lane) * lda + c0, x);
+    if (__ballot(viol != 0) != 0 && lane == 0) __hip_atomic_fetch_or(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the rest of the tile (the owner's top rows take U12 below)
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 1; tj < 4; ++tj) {
+            const int row = r0 + ti * 16 + li, col = c0 + tj * 16 + 4 * lk;
+            if (row < n && col < c1 && !(owner && ti == 0)) *(float4v *)(a + roff[ti] + col) = acc[ti][tj];
+        }
+    if (owner) {
+        if (lane < PB) ipiv[c0 + lane] = c0 + lane;
+        if (lane == 0) *zc = zero_col + 1;
+        const int nc = c1 - c0 - PB;  // 0, 16 or 32
+        if (nc > 0) {
+            float col[PB];
+#pragma unroll
+            for (int q = 0; q < PB; ++q) col[q] = s_fu[q][min(lane, nc - 1)];
+            if (lane < nc)
+#pragma unroll
+                for (int q = 0; q < PB; ++q) bk12[q * FW + lane] = col[q];
+            // as the finishing launch's follow-up forms it: each row's terms
+            // in ascending column order, unfused; L11[q][j] is lane q's
+            // multiplier j
+#pragma unroll
+            for (int j = 0; j < PB - 1; ++j) {
+#pragma unroll
+                for (int q = j + 1; q < PB; ++q)
+                    col[q] = col[q] - __int_as_float(__builtin_amdgcn_readlane(__float_as_int(top[j]), q)) * col[j];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (lane < nc)
+#pragma unroll
+                for (int q = 0; q < PB; ++q) a[(int64_t)(c0 + q) * lda + c0 + PB + lane] = col[q];
+        }
+    }
+}
 
 // ------------------------------------------------------------ refinement
 
@@ -1511,7 +1643,7 @@ struct FactorWs {
 // [kb, k) and, unless it is the global-memory panel (returns false), its
 // exchanges and U12 to the block's later columns [k + jb, c1)
 bool panel(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int kb, int k, int jb, int c1, int32_t *ipiv,
-           const FactorWs &ws)
+           const FactorWs &ws, bool pre)
 {
     const int m = n - k;
     if (m > 4096) {
@@ -1525,7 +1657,7 @@ bool panel(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int kb, int k
     }
     const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0) && jb == PB;
     const bool spec = ctx->lu_speculate != 0;
-    if (spec) {
+    if (spec && !pre) {  // pre: the speculative pass ran in the previous in-block update
         const int g = (m + 63) / 64;
         if (vec) k_panel_spec<true><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, ws.top, ws.bk12, ws.bk, ws.flag, ws.zc, ipiv);
         else k_panel_spec<false><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, ws.top, ws.bk12, ws.bk, ws.flag, ws.zc, ipiv);
@@ -1562,18 +1694,27 @@ void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipi
     for (int kb = 0; kb < n; kb += NB) {
         const int nbk = min(NB, n - kb);
         // panels of the block; each updates only the block's columns
+        bool pre = false;
         for (int k = kb; k < kb + nbk; k += PB) {
             const int jb = min(PB, kb + nbk - k);
             const int c0 = k + jb, c1 = kb + nbk;
-            const bool followed = panel(ctx, s, n, a, lda, kb, k, jb, c1, ipiv, ws);
+            const bool followed = panel(ctx, s, n, a, lda, kb, k, jb, c1, ipiv, ws, pre);
+            pre = false;
             if (c0 < c1) {
                 if (!followed) {
                     k_swap_trsm<PB><<<grid_for(c1 - c0, 256), 256, 0, s>>>(a, lda, k, jb, c0, c1, ipiv);
                     check_launch();
                 }
-                // at most 48 columns: one-wave 64 x 64 tiles
+                // at most 48 columns: one-wave 64 x 64 tiles; with the next
+                // panel's speculative pass when it is a full register panel
                 dim3 g((c1 - c0 + 63) / 64, (n - c0 + 63) / 64);
-                k_update_mfma<PB, 1, 1><<<g, 64, 0, s>>>(a, lda, n, k, jb, c0, c0, c1, vec && k % 4 == 0 && c0 % 4 == 0);
+                const bool vu = vec && k % 4 == 0 && c0 % 4 == 0;
+                if (ctx->lu_speculate && vu && jb == PB && c0 + PB <= c1 && n - c0 <= 4096) {
+                    k_update16_spec<<<g, 64, 0, s>>>(a, lda, n, k, c1, ws.top, ws.bk12, ws.bk, ws.flag, ws.zc, ipiv);
+                    pre = true;
+                } else {
+                    k_update_mfma<PB, 1, 1><<<g, 64, 0, s>>>(a, lda, n, k, jb, c0, c0, c1, vu);
+                }
                 check_launch();
             }
         }
