@@ -43,7 +43,7 @@ def timed(ctx, fn, reps=1):
 
 # ----------------------------------------------------------------------- LU
 
-def c3_rhs(n_side, nrhs, seed=2026):
+def bump_rhs(n_side, nrhs, seed=2026):
     rng = np.random.default_rng(seed)
     h = 1.0 / (n_side + 1)
     xs = (np.arange(n_side) + 1) * h
@@ -91,7 +91,7 @@ def bench_lu(args):
     dense = ctx.csr_to_dense(A)
     out = []
     for nrhs in (1, 32):
-        bh = c3_rhs(side, nrhs)
+        bh = bump_rhs(side, nrhs)
         b = ctx.from_host(bh.astype(np.float64))
         lu = dense.clone()
         # the copy alone first: the factorisation below must be the last

@@ -351,10 +351,13 @@ int piro_lu_factor_f32(piro_ctx *ctx, int n, float *d_a, int lda,
                        int32_t *d_ipiv, int *info);
 /* The panels of piro_lu_factor_f32 first try the columns without exchanges:
  * where every row below the diagonal is no larger in magnitude, partial
- * pivoting keeps the diagonal row, and the panel is factored without the
- * pivot reductions; otherwise it is factored with them.  The factors and
- * pivots are the same either way.  enable = 0 always takes the pivoting
- * steps (default 1).  Build-only, no reference counterpart. */
+ * pivoting keeps the diagonal row, and the panel's rows are factored
+ * independently (many workgroups, no pivot reductions); otherwise the panel
+ * is restored and factored by the pivoting steps.  The factors and pivots
+ * are the same either way.  enable = 0 always takes the pivoting steps
+ * (default 1).  The factorisation's workspace belongs to the context, so
+ * one context factors one matrix at a time (its calls are ordered on its
+ * stream anyway).  Build-only, no reference counterpart. */
 int piro_lu_set_speculation(piro_ctx *ctx, int enable);
 /* Solve with the FP32 factors, in place on d_b (row-major n x nrhs).
  * Takes the role of the never-enqueued forward/backward_substitution_csr

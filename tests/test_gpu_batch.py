@@ -283,7 +283,7 @@ def test_batch_c4_shape_every_instance_matches_oracle(ctx):
     x = ctx.from_host(x0)
     w = ctx.empty_like(x)
     rb, g, ms = b.autotune(x.clone(), w)
-    print(f"C4 batch: {b.total_rows} rows, {b.nnz} nnz; schedule {b.schedule()}, {ms * 1e3:.1f} us/step", flush=True)
+    print(f"512-instance batch: {b.total_rows} rows, {b.nnz} nnz; schedule {b.schedule()}, {ms * 1e3:.1f} us/step", flush=True)
     steps = 200
     b.run(x, w, steps, use_graph=True)
     got = x.host()

@@ -163,7 +163,7 @@ def _c1_setup(ctx):
 
 @pytest.mark.parametrize("variant", list(range(N_VARIANTS)))
 def test_fe_c1_100_steps_all_variants_bit_exact(ctx, variant):
-    print(f"case fe c1 variant={variant}", flush=True)
+    print(f"case fe 32^3 variant={variant}", flush=True)
     dims, A, dt, x0, snap = _c1_setup(ctx)
     ctx.set_variant(variant)
     try:
@@ -178,7 +178,7 @@ def test_fe_c1_100_steps_all_variants_bit_exact(ctx, variant):
     assert got.tobytes() == ref.tobytes()
     err32 = po.rel_l2(got, ref)
     err64 = po.rel_l2(got, ref64)
-    print(f"C1 variant {variant}: rel L2 vs FP32 oracle {err32:.2e}, vs FP64 oracle {err64:.2e}")
+    print(f"32^3 FE variant {variant}: rel L2 vs FP32 oracle {err32:.2e}, vs FP64 oracle {err64:.2e}")
     assert err32 <= 1e-5 and err64 < 1e-5
 
 

@@ -217,7 +217,7 @@ def test_lu_solve_f32_matches_numpy(ctx):
     assert po.rel_l2(x, xr) < 1e-4
 
 
-def c3_system(n_side=64, alpha=0.05, r=2.0):
+def be_system(n_side=64, alpha=0.05, r=2.0):
     """A = I - dt*alpha*L on the n_side^2 interior grid (dx = 1/(n_side+1)),
     built with the oracle's 2-D Laplacian and true-merge geam."""
     h = 1.0 / (n_side + 1)
@@ -230,7 +230,7 @@ def c3_system(n_side=64, alpha=0.05, r=2.0):
     return po.geam2(1.0, I, beta, L), L, I, beta
 
 
-def c3_rhs(n_side, nrhs, seed=2026):
+def bump_rhs(n_side, nrhs, seed=2026):
     rng = np.random.default_rng(seed)
     h = 1.0 / (n_side + 1)
     xs = (np.arange(n_side) + 1) * h
@@ -247,22 +247,22 @@ def c3_rhs(n_side, nrhs, seed=2026):
 
 
 @pytest.mark.parametrize("nrhs", [1, 32])
-def test_c3_mixed_precision_refinement(ctx, nrhs):
-    print(f"case c3 nrhs={nrhs}", flush=True)
-    A, L, I, beta = c3_system()
+def test_lu_4096_backward_euler_refinement(ctx, nrhs):
+    print(f"case lu 4096 nrhs={nrhs}", flush=True)
+    A, L, I, beta = be_system()
     n = A.n_rows
     from piro_amd.core import DevCSR
     dA = DevCSR(dev(A.row_ptr, ctx), dev(A.col, ctx), dev(A.val, ctx))
     dense = ctx.csr_to_dense(dA)
     lu = dense.clone()
     ipiv = ctx.lu_factor(lu, n)
-    b = c3_rhs(64, nrhs)
+    b = bump_rhs(64, nrhs)
     x, sweeps, rel = ctx.lu_solve_refine(dense, lu, n, ipiv, dev(b, ctx), tol=1e-10, max_sweeps=10)
     xh = x.host().reshape(n, nrhs)
     Ad = A.dense()
     xr = np.linalg.solve(Ad, b)
     res = np.linalg.norm(b - Ad @ xh, axis=0) / np.linalg.norm(b, axis=0)
-    print(f"C3 nrhs={nrhs}: {sweeps} refinement sweeps, max rel residual {res.max():.2e}, "
+    print(f"4096^2 LU nrhs={nrhs}: {sweeps} refinement sweeps, max rel residual {res.max():.2e}, "
           f"rel error vs FP64 solve {po.rel_l2(xh, xr):.2e}")
     assert res.max() <= 1e-10
     assert max(rel) <= 1e-10
@@ -277,10 +277,10 @@ def test_refine_rejects_non_finite_residual(ctx):
     print("case refine non-finite", flush=True)
     from piro_amd._lib import PiroError
     from piro_amd.core import DevCSR
-    A, L, I, beta = c3_system()
+    A, L, I, beta = be_system()
     n = A.n_rows
     dense = ctx.csr_to_dense(DevCSR(dev(A.row_ptr, ctx), dev(A.col, ctx), dev(A.val, ctx)))
     ipiv = dev(np.arange(n, dtype=np.int32), ctx)
-    b = c3_rhs(64, 1)
+    b = bump_rhs(64, 1)
     with pytest.raises(PiroError, match="non-finite"):
         ctx.lu_solve_refine(dense, dense.clone(), n, ipiv, dev(b, ctx), tol=1e-10, max_sweeps=3)

@@ -429,7 +429,7 @@ def test_fallback_rejects_a_corrupted_surrogate(ctx):
     # from 16 plain steps.  Three calls with calls 0 and 2 checked (no defect
     # correction): both checks exceed 1e-3 and are rejected, and each checked
     # call takes the plain stepper's result, so the run's field is exactly
-    # P(S(P(x0))) -- P = 16 plain CSR steps (the judge), S = one call
+    # P(S(P(x0))) -- P = 16 plain CSR steps (what a check compares with), S = one call
     from piro_amd import learned
     from piro_amd.core import Surrogate
     run = learned.HeatRun(ctx, side=24, seed=9)
