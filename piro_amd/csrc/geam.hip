@@ -12,6 +12,8 @@
 // equal columns summed), used to form I - dt*alpha*L.
 #include "device_utils.hpp"
 
+#include <utility>
+
 using namespace piro;
 
 namespace {
@@ -25,7 +27,15 @@ __global__ __launch_bounds__(256) void k_geam3_count(int n, const int32_t *__res
     cnt[r] = (rpa[r + 1] - rpa[r]) + (rpb[r + 1] - rpb[r]) + (rpc[r + 1] - rpc[r]);
 }
 
-// csrgeam_3.cl:88-156
+// csrgeam_3.cl:88-156.  A block's 256 rows read three contiguous input runs
+// and write one contiguous output run.  When the runs fit the staging
+// buffers (uniform per block) they move between memory and LDS with
+// consecutive lanes on consecutive entries, and each lane merges its row in
+// LDS; otherwise each lane merges its row in memory.  Same arithmetic and
+// entry order either way.
+constexpr int G3IN = 512;        // entries per staged input run
+constexpr int G3OUT = 3 * G3IN;  // entries of the staged output run
+
 __global__ __launch_bounds__(256) void k_geam3_fill(int n, const int32_t *__restrict__ rpa,
                                                     const int32_t *__restrict__ ca, const float *__restrict__ va,
                                                     const int32_t *__restrict__ rpb,
@@ -35,22 +45,51 @@ __global__ __launch_bounds__(256) void k_geam3_fill(int n, const int32_t *__rest
                                                     const float *__restrict__ v3, const int32_t *__restrict__ rpd,
                                                     int32_t *__restrict__ cd, float *__restrict__ vd)
 {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    int i = rpa[r], ie = rpa[r + 1], j = rpb[r], je = rpb[r + 1], k = rpc[r], ke = rpc[r + 1];
-    int d = rpd[r];
-    const float sa = v3[r], sb = v3[(int64_t)n + r], sc = v3[2 * (int64_t)n + r];
-    while (i < ie && j < je && k < ke) {
-        int a = ca[i], b = cb[j], c = cc[k];
-        int mn = min(a, min(c, b));
-        if (a == mn) { cd[d] = a; vd[d] = va[i] * sa; ++i; }
-        else if (b == mn) { cd[d] = b; vd[d] = vb[j] * sb; ++j; }
-        else { cd[d] = c; vd[d] = vc[k] * sc; ++k; }
-        ++d;
+    __shared__ int32_t s_c[G3OUT + G3OUT];  // three input runs, then the output run
+    __shared__ float s_v[G3OUT + G3OUT];
+    const int t = threadIdx.x, r0 = blockIdx.x * 256, r1 = min(r0 + 256, n), r = r0 + t;
+    const int a0 = rpa[r0], b0 = rpb[r0], c0 = rpc[r0], d0 = rpd[r0];
+    const int na = rpa[r1] - a0, nb = rpb[r1] - b0, nc = rpc[r1] - c0, nd = rpd[r1] - d0;
+    const bool staged = na <= G3IN && nb <= G3IN && nc <= G3IN && nd <= G3OUT;  // uniform
+    // the merge of row r from (col, val) sources x = A, B, C into emit()
+    auto merge = [&](auto ga, auto gb, auto gc, int i, int ie, int j, int je, int k, int ke, auto emit) {
+        const float sa = v3[r], sb = v3[(int64_t)n + r], sc = v3[2 * (int64_t)n + r];
+        while (i < ie && j < je && k < ke) {
+            const int a = ga(i).first, b = gb(j).first, c = gc(k).first;
+            const int mn = min(a, min(c, b));
+            if (a == mn) { emit(a, ga(i).second * sa); ++i; }
+            else if (b == mn) { emit(b, gb(j).second * sb); ++j; }
+            else { emit(c, gc(k).second * sc); ++k; }
+        }
+        for (; i < ie; ++i) emit(ga(i).first, ga(i).second * sa);
+        for (; j < je; ++j) emit(gb(j).first, gb(j).second * sb);
+        for (; k < ke; ++k) emit(gc(k).first, gc(k).second * sc);
+    };
+    using cv = std::pair<int, float>;
+    if (staged) {
+        for (int e = t; e < na; e += 256) { s_c[e] = ca[a0 + e]; s_v[e] = va[a0 + e]; }
+        for (int e = t; e < nb; e += 256) { s_c[G3IN + e] = cb[b0 + e]; s_v[G3IN + e] = vb[b0 + e]; }
+        for (int e = t; e < nc; e += 256) { s_c[2 * G3IN + e] = cc[c0 + e]; s_v[2 * G3IN + e] = vc[c0 + e]; }
+        __syncthreads();
+        if (r < r1) {
+            int d = rpd[r] - d0 + G3OUT;
+            merge([&](int i) { return cv(s_c[i], s_v[i]); },
+                  [&](int j) { return cv(s_c[G3IN + j], s_v[G3IN + j]); },
+                  [&](int k) { return cv(s_c[2 * G3IN + k], s_v[2 * G3IN + k]); },
+                  rpa[r] - a0, rpa[r + 1] - a0, rpb[r] - b0, rpb[r + 1] - b0, rpc[r] - c0, rpc[r + 1] - c0,
+                  [&](int c, float v) { s_c[d] = c; s_v[d] = v; ++d; });
+        }
+        __syncthreads();
+        for (int e = t; e < nd; e += 256) {
+            cd[d0 + e] = s_c[G3OUT + e];
+            vd[d0 + e] = s_v[G3OUT + e];
+        }
+    } else if (r < r1) {
+        int d = rpd[r];
+        merge([&](int i) { return cv(ca[i], va[i]); }, [&](int j) { return cv(cb[j], vb[j]); },
+              [&](int k) { return cv(cc[k], vc[k]); }, rpa[r], rpa[r + 1], rpb[r], rpb[r + 1], rpc[r], rpc[r + 1],
+              [&](int c, float v) { cd[d] = c; vd[d] = v; ++d; });
     }
-    for (; i < ie; ++i, ++d) { cd[d] = ca[i]; vd[d] = va[i] * sa; }
-    for (; j < je; ++j, ++d) { cd[d] = cb[j]; vd[d] = vb[j] * sb; }
-    for (; k < ke; ++k, ++d) { cd[d] = cc[k]; vd[d] = vc[k] * sc; }
 }
 
 __global__ __launch_bounds__(256) void k_geam2_count(int n, const int32_t *__restrict__ rpa,

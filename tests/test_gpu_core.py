@@ -396,6 +396,35 @@ def test_geam3_concat_bit_exact(ctx):
         assert val.tobytes() == R.val.tobytes()
 
 
+def test_geam3_concat_general_rows_bit_exact(ctx):
+    # three general CSR inputs, short rows with unsorted and repeated
+    # columns, plus blocks whose long rows overflow the staging buffers (so
+    # some 256-row blocks merge in LDS and others in memory), against the
+    # oracle bit for bit
+    from piro_amd.core import DevCSR
+    rng = np.random.default_rng(33)
+    N = 3000
+
+    def rand_csr(long_rows):
+        lens = rng.integers(0, 5, N)
+        lens[long_rows] = 400
+        rp = np.zeros(N + 1, np.int32)
+        rp[1:] = np.cumsum(lens)
+        col = rng.integers(0, N, rp[-1]).astype(np.int32)
+        val = rng.standard_normal(rp[-1]).astype(np.float32)
+        return po.CSR(rp, col, val)
+
+    H = [rand_csr([700, 703]), rand_csr([1800]), rand_csr([2990, 2995])]
+    v3 = rng.standard_normal(3 * N).astype(np.float32)
+    G = [DevCSR(ctx.from_host(h.row_ptr), ctx.from_host(h.col), ctx.from_host(h.val)) for h in H]
+    D = ctx.geam3(*G, ctx.from_host(v3))
+    R = po.geam3(*H, v3)
+    rp, col, val = D.host()
+    np.testing.assert_array_equal(rp, R.row_ptr)
+    np.testing.assert_array_equal(col, R.col)
+    assert val.tobytes() == R.val.tobytes()
+
+
 def test_geam2_true_merge_bit_exact(ctx):
     n = 64
     inv_h2 = np.float32(65.0 * 65.0)
