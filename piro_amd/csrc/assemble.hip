@@ -198,22 +198,34 @@ __global__ __launch_bounds__(256) void k_lap2d(int n, float inv_h2, int32_t *__r
     }
 }
 
-// SpMDSM.cl:20-31 intended semantics, one thread per row.
-// one block per 256 rows: the block's entries are one contiguous run, walked
-// with consecutive lanes on consecutive entries; each entry finds its row by
-// binary search in the block's row_ptr, staged in LDS
+// SpMDSM.cl:20-31 intended semantics: val[k] *= diag[row of k].  One block
+// per 256 rows: the block's entries are one contiguous run, walked with
+// consecutive lanes on consecutive entries.  When the run fits RS_MAP
+// entries (uniform per block), each lane first writes its row's index over
+// its own entries in an LDS map, so an entry finds its row with one LDS
+// read; longer runs binary-search the block's row_ptr, staged in LDS.
+constexpr int RS_MAP = 4096;
+
 __global__ __launch_bounds__(256) void k_row_scale(int n_rows, const int32_t *__restrict__ rp,
                                                    float *__restrict__ val, const float *__restrict__ diag)
 {
     __shared__ int32_t s_rp[257];
     __shared__ float s_d[256];
-    const int r0 = blockIdx.x * 256;
+    __shared__ uint8_t s_row[RS_MAP];
+    const int t = threadIdx.x, r0 = blockIdx.x * 256;
     const int nr = min(256, n_rows - r0);
-    for (int j = threadIdx.x; j <= nr; j += 256) s_rp[j] = rp[r0 + j];
-    if ((int)threadIdx.x < nr) s_d[threadIdx.x] = diag[r0 + threadIdx.x];
+    for (int j = t; j <= nr; j += 256) s_rp[j] = rp[r0 + j];
+    if (t < nr) s_d[t] = diag[r0 + t];
     __syncthreads();
-    const int32_t k1 = s_rp[nr];
-    for (int32_t k = s_rp[0] + (int32_t)threadIdx.x; k < k1; k += 256) {
+    const int32_t k0 = s_rp[0], k1 = s_rp[nr];
+    if (k1 - k0 <= RS_MAP) {  // uniform
+        if (t < nr)
+            for (int32_t k = s_rp[t]; k < s_rp[t + 1]; ++k) s_row[k - k0] = (uint8_t)t;
+        __syncthreads();
+        for (int32_t k = k0 + t; k < k1; k += 256) val[k] = val[k] * s_d[s_row[k - k0]];
+        return;
+    }
+    for (int32_t k = k0 + t; k < k1; k += 256) {
         // the last row j with s_rp[j] <= k (s_rp[0] <= k < s_rp[nr])
         int lo = 0, hi = nr;
         while (hi - lo > 1) {
@@ -435,19 +447,36 @@ __global__ __launch_bounds__(256) void k_sort_rows(int n_rows, const int32_t *__
 // closes the rows (rows[k-1], rows[k]] -- every row_ptr entry in that range
 // is k (rows[-1] = -1, rows[nnz] = n_rows) -- so each entry is written by
 // exactly one thread, empty rows included.  A decrease or a row outside
-// [0, n_rows) raises *bad and writes nothing.
+// [0, n_rows) raises *bad and writes nothing.  Each thread takes the eight
+// positions 8t .. 8t + 7; VEC: rows is 16-byte aligned, read as two quads.
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_coo_row_ptr(int64_t nnz, int n_rows, const int32_t *__restrict__ rows,
                                                      int32_t *__restrict__ rp, int *__restrict__ bad)
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > nnz) return;
-    const int prev = k == 0 ? -1 : rows[k - 1];
-    const int cur = k == nnz ? n_rows : rows[k];
-    if ((k < nnz && (cur < 0 || cur >= n_rows)) || (k > 0 && (prev < 0 || prev >= n_rows)) || cur < prev) {
-        *bad = 1;
-        return;
+    const int64_t k0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+    if (k0 > nnz) return;
+    int cur[8];
+    if (VEC && k0 + 8 <= nnz) {
+        const int4 q0 = *(const int4 *)(rows + k0), q1 = *(const int4 *)(rows + k0 + 4);
+        cur[0] = q0.x; cur[1] = q0.y; cur[2] = q0.z; cur[3] = q0.w;
+        cur[4] = q1.x; cur[5] = q1.y; cur[6] = q1.z; cur[7] = q1.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) cur[j] = k0 + j < nnz ? rows[k0 + j] : n_rows;
     }
-    for (int r = prev + 1; r <= cur; ++r) rp[r] = (int32_t)k;
+    int prev = k0 == 0 ? -1 : rows[k0 - 1];
+    bool ok = k0 == 0 || (prev >= 0 && prev < n_rows);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int64_t k = k0 + j;
+        if (k > nnz) break;
+        const int c = cur[j];  // n_rows at k == nnz
+        ok = ok && (k == nnz || (c >= 0 && c < n_rows)) && c >= prev;
+        if (!ok) break;
+        for (int r = prev + 1; r <= c; ++r) rp[r] = (int32_t)k;
+        prev = c;
+    }
+    if (!ok) *bad = 1;
 }
 
 Dims mkdims(int nx, int ny, int nz)
@@ -717,7 +746,9 @@ int piro_coo_row_ptr(piro_ctx *ctx, int n_rows, int64_t nnz, const int32_t *d_ro
     PIRO_REQUIRE(nnz <= 0x7fffffffLL, "nnz must fit int32 row pointers");
     int *bad = (int *)ctx->scratch[1].get(sizeof(int));
     PIRO_CHECK_HIP(hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
-    k_coo_row_ptr<<<grid_for(nnz + 1, 256), 256, 0, ctx->stream>>>(nnz, n_rows, d_rows, d_row_ptr, bad);
+    const unsigned g = grid_for((nnz + 8) / 8, 256);
+    if ((uintptr_t)d_rows % 16 == 0) k_coo_row_ptr<true><<<g, 256, 0, ctx->stream>>>(nnz, n_rows, d_rows, d_row_ptr, bad);
+    else k_coo_row_ptr<false><<<g, 256, 0, ctx->stream>>>(nnz, n_rows, d_rows, d_row_ptr, bad);
     check_launch();
     int h = 0;
     PIRO_CHECK_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

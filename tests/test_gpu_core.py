@@ -515,6 +515,33 @@ def test_vec_copy(ctx, n, offset):
     assert np.all(got[:offset] == -7.0) and np.all(got[offset + n:] == -7.0)
 
 
+def test_coo_row_ptr_random_and_unaligned(ctx):
+    # a large sorted row list with empty rows, read through a 16-byte
+    # aligned pointer (quads) and through one 4 bytes off (entry by entry);
+    # decreases at the first position of a thread's eight and inside them
+    import ctypes as C
+    from piro_amd._lib import PiroError
+    from piro_amd.core import _p, check, lib
+    rng = np.random.default_rng(57)
+    n, nnz = 20000, 100003
+    rows = np.sort(rng.integers(0, n, nnz)).astype(np.int32)
+    want = np.searchsorted(rows, np.arange(n + 1), side="left").astype(np.int32)
+    buf = dev(np.concatenate([rows, [0]]).astype(np.int32), ctx)
+    off = dev(np.concatenate([[0], rows]).astype(np.int32), ctx)
+    for ptr in (_p(buf), C.c_void_p(off.ptr + 4)):
+        rp = ctx.empty(n + 1, np.int32)
+        check(lib.piro_coo_row_ptr(ctx.h, n, nnz, ptr, _p(rp)))
+        np.testing.assert_array_equal(host(rp), want)
+    for pos in (16, 21):
+        bad = rows[:64].copy()
+        bad[pos] = bad[pos - 1] - 1 if bad[pos - 1] > 0 else 0
+        bad[pos - 1] = max(bad[pos - 1], 1)
+        assert np.any(np.diff(bad) < 0)
+        d = dev(bad, ctx)
+        with pytest.raises(PiroError):
+            check(lib.piro_coo_row_ptr(ctx.h, n, bad.size, _p(d), _p(ctx.empty(n + 1, np.int32))))
+
+
 def test_coo_row_ptr_edges(ctx):
     # row_ptr[i] = number of entries in rows below i, for sorted rows with
     # empty leading, inner and trailing rows and for no entries at all;
