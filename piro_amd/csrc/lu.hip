@@ -1376,20 +1376,49 @@ struct TrsvArgs {
     const int32_t *perm;    // forward pass: b row of solution row i; else null
     const float *in;        // right-hand side rows
     int ldin;
-    float *out;             // solution rows (read back by later blocks)
+    float *out;             // solution rows (read back by later blocks when NRW > 1)
     int ldout, nrhs;
-    int *done;              // per wave (<= 8): blocks finished, in solve order; zero at launch
+    uint64_t *mb;           // NRW = 1: mailbox, n x 32 words, (tag << 32) | bits of x
+    uint32_t tag;           // this launch's tag (never 0; the mailbox starts zeroed)
+    int *done;              // NRW > 1: per wave (<= 8): blocks finished, in solve order; zero at launch
     int *ticket;            // zero at launch
     int *err;               // set when a wait exceeds its bound
 };
 
-// wait until *done >= need (every lane reads the same word); returns the
-// value seen, or need after raising *err when the bound is exceeded
+// Two ways for a block's x to reach the blocks below it.
+//
+// One right-hand side per wave: each value travels as one 64-bit word, the
+// launch's tag in the high half.  The word validates itself, so no
+// completion flag, fence or second read is needed; each lane waits for its
+// own word.  mb_wait returns the value once the word carries the tag; past
+// the bound (or once another wait has passed it) it raises *err and
+// returns what is there.
+__device__ __forceinline__ float mb_wait(uint64_t *w, uint32_t tag, int *err)
+{
+    uint64_t v;
+    int spins = 0;
+    while ((uint32_t)((v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) != tag) {
+        if (++spins > TRSV_SPIN_LIMIT ||
+            ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return __uint_as_float((uint32_t)v);
+}
+
+// Several right-hand sides per wave: a word per value would be polled by
+// every lane for each of them (measured slower); instead each wave counts
+// finished blocks in its own word, released after the values are stored.
+// wait_done waits until *done >= need (every lane reads the same word) and
+// returns the value seen, or need after raising *err past the bound.
 __device__ __forceinline__ int wait_done(int *done, int need, int *err)
 {
     int spins = 0, d;
     while ((d = __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
-        if (++spins > TRSV_SPIN_LIMIT) {
+        if (++spins > TRSV_SPIN_LIMIT ||
+            ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             d = need;
             break;
@@ -1425,13 +1454,13 @@ __device__ __forceinline__ void load_row(f4 (&r)[16], const float *__restrict__ 
     }
 }
 
-// One workgroup (WAVES waves) per 64-row block, blocks in ticket order.
+// One workgroup (WAVES waves) per 64-row block, blocks in ticket order (a
+// block waits only on blocks with smaller tickets, already running).
 // Right-hand sides are independent, so each wave solves its own NRW of
-// them with no barrier
-// after the start: it counts finished blocks in its own `done` word, keeps
-// its row of the next off-diagonal block in registers (lane = row; loaded
-// while the wave waits for that block) and takes x_j by readlane (NRW = 1)
-// or from a wave-private LDS slice (NRW > 1).
+// them with no barrier after the start: it waits for block j (mailbox words
+// for NRW = 1, its done word otherwise), keeps its row of the next
+// off-diagonal block in registers (lane = row; loaded while it waits) and
+// takes x_j by readlane (NRW = 1) or from a wave-private LDS slice (NRW > 1).
 template <bool LOWER, int NRW, int WAVES, bool VEC>
 __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
 {
@@ -1473,11 +1502,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
             const int jn = LOWER ? s + 1 : p.nb - 2 - s;
             load_row<VEC>(nxt, p.lu, p.lda, row, live, i0, jn * TB, min(TB, p.n - jn * TB));
         }
-        if (s >= avail) avail = wait_done(done, s + 1, p.err);
         float xv[NRW];
+        if constexpr (NRW == 1) {
+            xv[0] = lane < js ? mb_wait(p.mb + (int64_t)(j0 + lane) * 32 + c0, p.tag, p.err) : 0.0f;
+        } else {
+            if (s >= avail) avail = wait_done(done, s + 1, p.err);
 #pragma unroll
-        for (int q = 0; q < NRW; ++q)
-            xv[q] = (lane < js && c0 + q < p.nrhs) ? p.out[(int64_t)(j0 + lane) * p.ldout + c0 + q] : 0.0f;
+            for (int q = 0; q < NRW; ++q)
+                xv[q] = (lane < js && c0 + q < p.nrhs) ? p.out[(int64_t)(j0 + lane) * p.ldout + c0 + q] : 0.0f;
+        }
         if constexpr (NRW == 1) {
 #pragma unroll
             for (int jj = 0; jj < TB; ++jj) {
@@ -1526,8 +1559,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
 #pragma unroll
         for (int q = 0; q < NRW; ++q)
             if (c0 + q < p.nrhs) p.out[(int64_t)row * p.ldout + c0 + q] = x[q];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_store(done, tk + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if constexpr (NRW == 1) {
+        if (live)
+            __hip_atomic_store(p.mb + (int64_t)row * 32 + c0, ((uint64_t)p.tag << 32) | __float_as_uint(x[0]),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(done, tk + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_zero_i32(int *__restrict__ p, int count)
@@ -1539,9 +1578,11 @@ __global__ __launch_bounds__(256) void k_zero_i32(int *__restrict__ p, int count
 // forward solution and the progress counters of both passes
 struct SolvePlan {
     int n = 0, nb = 0;
+    uint64_t *mb = nullptr;  // the solves' mailbox, n x 32 words, zeroed by prepare_solve
     int32_t *perm = nullptr;
     float *linv = nullptr, *uinv = nullptr, *y = nullptr;
     int *sync = nullptr;  // [fwd done 8][fwd ticket][bwd done 8][bwd ticket][err]
+    mutable uint32_t tag = 0;  // the last launch's mailbox tag
 };
 
 SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const int32_t *d_ipiv)
@@ -1551,9 +1592,10 @@ SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const in
     pl.n = n;
     pl.nb = (n + TB - 1) / TB;
     const size_t blk = (size_t)pl.nb * TB * TB;
-    const size_t bytes = (size_t)n * 32 * 4 + 2 * blk * 4 + ((size_t)n + 24) * 4 + 256;
+    const size_t bytes = (size_t)n * 32 * 8 + (size_t)n * 32 * 4 + 2 * blk * 4 + ((size_t)n + 24) * 4 + 256;
     char *ws = (char *)ctx->scratch[3].get(bytes);
-    pl.y = (float *)ws;
+    pl.mb = (uint64_t *)ws;
+    pl.y = (float *)(pl.mb + (size_t)n * 32);
     pl.linv = pl.y + (size_t)n * 32;
     pl.uinv = pl.linv + blk;
     pl.perm = (int32_t *)(pl.uinv + blk);
@@ -1573,6 +1615,8 @@ SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const in
     k_diag_inv<<<2 * pl.nb, 256, 0, s>>>(lu, lda, n, pl.linv, pl.uinv);
     check_launch();
     k_zero_i32<<<1, 256, 0, s>>>(pl.sync + 18, 1);
+    check_launch();
+    k_zero_i32<<<(int)std::min<int64_t>(1024, ((int64_t)n * 64 + 255) / 256), 256, 0, s>>>((int *)pl.mb, n * 64);
     check_launch();
     return pl;
 }
@@ -1602,9 +1646,11 @@ void solve_f32(piro_ctx *ctx, const SolvePlan &pl, const float *lu, int lda, int
         const int nc = min(32, nrhs - c0);
         k_zero_i32<<<1, 256, 0, s>>>(pl.sync, 18);
         check_launch();
-        launch_trsv<true>(s, TrsvArgs{lu, lda, pl.n, nb, pl.linv, pl.perm, b + c0, ldb, pl.y, 32, nc, fwd, fwd + 8, err},
+        launch_trsv<true>(s, TrsvArgs{lu, lda, pl.n, nb, pl.linv, pl.perm, b + c0, ldb, pl.y, 32, nc, pl.mb, ++pl.tag,
+                                      fwd, fwd + 8, err},
                           vec);
-        launch_trsv<false>(s, TrsvArgs{lu, lda, pl.n, nb, pl.uinv, nullptr, pl.y, 32, b + c0, ldb, nc, bwd, bwd + 8, err},
+        launch_trsv<false>(s, TrsvArgs{lu, lda, pl.n, nb, pl.uinv, nullptr, pl.y, 32, b + c0, ldb, nc, pl.mb, ++pl.tag,
+                                       bwd, bwd + 8, err},
                            vec);
     }
 }
