@@ -73,7 +73,16 @@ for _ in range(3):
     x = run()
 e1.record(); torch.cuda.synchronize()
 rel = (torch.linalg.vector_norm(b - a @ x, dim=0) / torch.linalg.vector_norm(b, dim=0)).max().item()
-print(json.dumps({"seconds": e0.elapsed_time(e1) / 3 / 1e3, "rel": rel}))
+t64 = e0.elapsed_time(e1) / 3 / 1e3
+# the vendor FP32 factorisation (rocSOLVER sgetrf) of the same matrix, for
+# the factor-only comparison
+a32 = a.float()
+torch.linalg.lu_factor(a32); torch.cuda.synchronize()
+e0.record()
+for _ in range(5):
+    torch.linalg.lu_factor(a32)
+e1.record(); torch.cuda.synchronize()
+print(json.dumps({"seconds": t64, "rel": rel, "f32_factor_seconds": e0.elapsed_time(e1) / 5 / 1e3}))
 """
 
 
@@ -108,7 +117,7 @@ def bench_lu(args):
         solve()
         t_sol = timed(ctx, solve)
         x, sweeps, rel = res["x"]
-        f64 = {"seconds": None, "rel": None}
+        f64 = {"seconds": None, "rel": None, "f32_factor_seconds": None}
         if not args.no_fp64:
             with tempfile.TemporaryDirectory() as td:
                 fa, fb = os.path.join(td, "a.npy"), os.path.join(td, "b.npy")
@@ -124,6 +133,9 @@ def bench_lu(args):
                     "refinement_sweeps": sweeps, "mixed_max_rel_residual": max(rel),
                     "fp64_torch_total_s": f64["seconds"], "fp64_max_rel_residual": f64["rel"],
                     "speedup_vs_fp64": (f64["seconds"] / tot) if f64["seconds"] else None,
+                    "torch_f32_factor_s": f64.get("f32_factor_seconds"),
+                    "factor_speedup_vs_torch_f32": (f64["f32_factor_seconds"] / t_fac)
+                    if f64.get("f32_factor_seconds") else None,
                     "factor_tflops": flop / t_fac / 1e12, "factor_frac_of_fp32_mfma_peak": flop / t_fac / MFMA_F32})
     for o in out:
         print(json.dumps(o), flush=True)

@@ -335,61 +335,112 @@ __global__ __launch_bounds__(256) void k_set_faces(Dims d, const float *__restri
 // ------------------------------------------------------------ hash table
 
 // methods::sethash (src/methods.cpp:74-118) applied to every CSR entry in
-// CSR order.  Linear-probing layout depends on insertion order, so one lane
-// performs the insertions in exactly the reference's order; the table is
-// sparse (load ~0.2 at N=125 and falling with N), so a probe usually ends
-// at the home slot.  The other lanes of the block stage the next chunk of
-// (key, value) pairs in LDS.
-constexpr int HT_CHUNK = 1024;
+// CSR order, computed in parallel with the same slot layout.
+//
+// Sequential linear probing leaves each key at the first slot, from its home
+// key % ts onwards, that no earlier-inserted key holds; that condition has
+// exactly one solution (by induction over the insertion order), so any
+// procedure reaching it reproduces the reference's table.  Here every
+// distinct key carries its insertion rank t (the CSR position of its first
+// occurrence) and probes with atomicMin(slot, t) on slots that start at
+// 0xffffffff (-1, empty): a slot holding a smaller rank is passed, an empty
+// slot is taken, and a slot holding a larger rank is taken over while the
+// displaced rank continues probing from the next slot.  Slot values only
+// fall, and a rank leaves a slot only to a smaller one, so at the end the
+// slots between every key's home and its own slot hold smaller ranks: the
+// sequential layout.
+//
+// A repeated key (same row and column) stops at its own slot in the
+// sequential insert and overwrites the value there, so it never moves any
+// key: only first occurrences are inserted, each with the value of its
+// row's last occurrence.
 
-__global__ __launch_bounds__(256) void k_ht_build(int n_rows, const int32_t *__restrict__ rp,
-                                                  const int32_t *__restrict__ col,
-                                                  const float *__restrict__ val, int ts,
-                                                  int32_t *__restrict__ keys, float *__restrict__ hv,
-                                                  int *__restrict__ status)
+// one thread per entry: the entry's key and whether it is its key's first
+// occurrence (kv = key, vv = the last occurrence's value), else kv = -1;
+// counts the distinct keys into cnt[0]; a column outside [0, n_rows) sets
+// status 2
+__global__ __launch_bounds__(256) void k_ht_prep(int n_rows, const int32_t *__restrict__ rp,
+                                                 const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                 int32_t *__restrict__ kv, float *__restrict__ vv,
+                                                 int *__restrict__ cnt, int *__restrict__ status)
 {
-    __shared__ int32_t s_key[HT_CHUNK];
-    __shared__ float s_val[HT_CHUNK];
-    __shared__ int s_fail;
-    if (threadIdx.x == 0) s_fail = 0;
     const int64_t nnz = rp[n_rows];
-    for (int64_t base = 0; base < nnz; base += HT_CHUNK) {
-        __syncthreads();
-        int64_t cnt = nnz - base < HT_CHUNK ? nnz - base : HT_CHUNK;
-        for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
-            int64_t k = base + j;
-            // row of entry k: binary search of row_ptr
-            int lo = 0, hi = n_rows;
-            while (hi - lo > 1) {
-                int mid = (lo + hi) >> 1;
-                if (rp[mid] <= k) lo = mid; else hi = mid;
-            }
-            s_key[j] = lo * n_rows + col[k];
-            s_val[j] = val[k];
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int first = 0;
+    if (k < nnz) {
+        int lo = 0, hi = n_rows;  // row of entry k: rp[lo] <= k < rp[lo + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (rp[mid] <= k) lo = mid; else hi = mid;
         }
-        __syncthreads();
-        if (threadIdx.x == 0 && !s_fail) {
-            for (int j = 0; j < cnt; ++j) {
-                int key = s_key[j];
-                int h = key % ts, attempts = 0;
-                bool done = false;
-                while (attempts < ts) {
-                    int kh = keys[h];
-                    if (kh == key || kh == -1) {
-                        keys[h] = key;
-                        hv[h] = s_val[j];
-                        done = true;
-                        break;
-                    }
-                    h = h + 1 == ts ? 0 : h + 1;
-                    ++attempts;
-                }
-                if (!done) { s_fail = 1; break; }
+        const int c = col[k];
+        if (c < 0 || c >= n_rows) {
+            atomicOr(status, 2);
+            kv[k] = -1;
+        } else {
+            const int b = rp[lo], e = rp[lo + 1];
+            first = 1;
+            for (int j = b; j < (int)k; ++j)
+                if (col[j] == c) { first = 0; break; }
+            if (first) {
+                float v = val[k];
+                for (int j = (int)k + 1; j < e; ++j)
+                    if (col[j] == c) v = val[j];
+                kv[k] = lo * n_rows + c;
+                vv[k] = v;
+            } else {
+                kv[k] = -1;
             }
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) *status = s_fail;
+    const int n = wave_sum_i32(first);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(cnt, n);
+}
+
+// one thread per first occurrence: the rank-ordered insert; every slot taken
+// from empty is listed in occ (cnt[1] of them).  More distinct keys than
+// slots is the reference's full table (status 1, nothing inserted).
+__global__ __launch_bounds__(256) void k_ht_insert(int64_t nnz, const int32_t *__restrict__ kv, int ts,
+                                                   uint32_t *__restrict__ slots, int *__restrict__ cnt,
+                                                   int32_t *__restrict__ occ, int *__restrict__ status)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    if (cnt[0] > ts) {
+        if (threadIdx.x == 0) atomicOr(status, 1);
+        return;
+    }
+    const int key = kv[k];
+    if (key < 0) return;
+    uint32_t t = (uint32_t)k;
+    int p = key % ts;
+    // each step places the carried rank, passes a smaller one, or takes
+    // over a larger one (at most one take-over per distinct key); with no
+    // more keys than slots this ends, and the bound only guards the loop
+    const int64_t limit = 2 * (int64_t)ts + 2 * nnz + 64;
+    for (int64_t it = 0; it < limit; ++it) {
+        const uint32_t old = atomicMin(slots + p, t);
+        if (old == 0xffffffffu) {
+            occ[atomicAdd(cnt + 1, 1)] = p;
+            return;
+        }
+        if (old > t) t = old;  // the displaced rank probes on
+        p = p + 1 == ts ? 0 : p + 1;
+    }
+    atomicOr(status, 4);
+}
+
+// one thread per occupied slot: rank -> key and value
+__global__ __launch_bounds__(256) void k_ht_finish(const int *__restrict__ cnt, const int32_t *__restrict__ occ,
+                                                   const int32_t *__restrict__ kv, const float *__restrict__ vv,
+                                                   int32_t *__restrict__ keys, float *__restrict__ hv)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt[1]) return;
+    const int p = occ[i];
+    const int t = keys[p];
+    keys[p] = kv[t];
+    hv[p] = vv[t];
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int32_t *p, int64_t n, int32_t v)
@@ -694,12 +745,36 @@ int piro_ht_build(piro_ctx *ctx, int n_rows, const int32_t *d_row_ptr, const int
     k_fill_i32<<<grid_for(ts, 256), 256, 0, ctx->stream>>>(d_keys, ts, -1);
     check_launch();
     PIRO_CHECK_HIP(hipMemsetAsync(d_vals, 0, (size_t)ts * sizeof(float), ctx->stream));
-    int *status = (int *)ctx->scratch[0].get(sizeof(int));
-    k_ht_build<<<1, 256, 0, ctx->stream>>>(n_rows, d_row_ptr, d_col, d_val, ts, d_keys, d_vals, status);
+    int64_t nnz = 0;
+    {
+        int32_t last = 0;
+        PIRO_CHECK_HIP(hipMemcpyAsync(&last, d_row_ptr + n_rows, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+        nnz = last;
+    }
+    PIRO_REQUIRE(nnz >= 0, "row_ptr[n_rows] is negative");
+    // counters: cnt[0] distinct keys, cnt[1] occupied slots, cnt[2] status
+    int *cnt = (int *)ctx->scratch[0].get(4 * sizeof(int));
+    k_fill_i32<<<1, 256, 0, ctx->stream>>>(cnt, 4, 0);
     check_launch();
+    if (nnz > 0) {
+        char *w = (char *)ctx->scratch[1].get((size_t)nnz * 12 + 64);
+        int32_t *kv = (int32_t *)w;
+        float *vv = (float *)(w + (size_t)nnz * 4);
+        int32_t *occ = (int32_t *)(w + (size_t)nnz * 8);
+        const int g = (int)((nnz + 255) / 256);
+        k_ht_prep<<<g, 256, 0, ctx->stream>>>(n_rows, d_row_ptr, d_col, d_val, kv, vv, cnt, cnt + 2);
+        check_launch();
+        k_ht_insert<<<g, 256, 0, ctx->stream>>>(nnz, kv, ts, (uint32_t *)d_keys, cnt, occ, cnt + 2);
+        check_launch();
+        k_ht_finish<<<g, 256, 0, ctx->stream>>>(cnt, occ, kv, vv, d_keys, d_vals);
+        check_launch();
+    }
     int h = 0;
-    PIRO_CHECK_HIP(hipMemcpyAsync(&h, status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PIRO_CHECK_HIP(hipMemcpyAsync(&h, cnt + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    PIRO_REQUIRE(!(h & 2), "a column index lies outside [0, n_rows)");
+    PIRO_REQUIRE(!(h & 4), "hash-table insert did not settle");
     if (h) throw Error(PIRO_EFULL, "hash table full; reduce LoadFactor (src/methods.cpp:109-116)");
     PIRO_API_END
 }

@@ -364,6 +364,80 @@ def test_ht_larger_table(ctx):
     assert host(vals).tobytes() == rv.tobytes()
 
 
+def _distinct_keys(R):
+    return sum(len(set(R.col[R.row_ptr[r]:R.row_ptr[r + 1]].tolist())) for r in range(R.n_rows))
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3])
+@pytest.mark.parametrize("load", [0.5, 0.95, 1.0])
+def test_ht_build_high_load(ctx, scheme, load):
+    # long clusters: the parallel rank-ordered insert must leave the
+    # sequential layout (keys and values), repeated self entries included
+    dims = (9, 7, 6)
+    N = 9 * 7 * 6
+    A = ctx.laplacian_csr(*dims, scheme, po.lap_norm(1.0, 7))
+    R = po.CSR(*A.host())
+    ts = int(_distinct_keys(R) / load)
+    keys, vals = ctx.ht_build(A, ts)
+    rk, rv = po.ht_build(R, ts)
+    np.testing.assert_array_equal(host(keys), rk)
+    assert host(vals).tobytes() == rv.tobytes()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_ht_build_general_csr_with_repeats(ctx, seed):
+    rng = np.random.default_rng(seed)
+    n = 3000
+    lens = rng.integers(0, 12, n)
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    # a narrow column window per row repeats keys inside rows
+    base = rng.integers(0, n - 8, n)
+    col = np.concatenate([base[r] + rng.integers(0, 6, lens[r]) for r in range(n)]).astype(np.int32)
+    val = rng.standard_normal(int(rp[-1])).astype(np.float32)
+    R = po.CSR(rp, col, val)
+    from piro_amd.core import DevCSR
+    A = DevCSR(dev(rp, ctx), dev(col, ctx), dev(val, ctx))
+    for ts in (_distinct_keys(R), _distinct_keys(R) + 13, 5 * _distinct_keys(R)):
+        keys, vals = ctx.ht_build(A, ts)
+        rk, rv = po.ht_build(R, ts)
+        np.testing.assert_array_equal(host(keys), rk)
+        assert host(vals).tobytes() == rv.tobytes()
+
+
+def test_ht_build_26cubed_shipped_size(ctx):
+    # the largest grid class the shipped size fit serves: 354M slots
+    dims = (26, 26, 26)
+    N = 26 ** 3
+    A = ctx.laplacian_csr(*dims, 3, po.lap_norm(1.0, 24))
+    R = po.CSR(*A.host())
+    ts = ctx.ht_table_size(A.nnz, N)
+    assert ts == po.ht_table_size(A.nnz, N) and ts > 300_000_000
+    keys, vals = ctx.ht_build(A, ts)
+    rk, rv = po.ht_build(R, ts)
+    hk = host(keys)
+    occ = np.nonzero(rk != -1)[0]
+    assert len(occ) == A.nnz
+    np.testing.assert_array_equal(np.nonzero(hk != -1)[0], occ)
+    np.testing.assert_array_equal(hk[occ], rk[occ])
+    hv = host(vals)
+    assert hv[occ].tobytes() == rv[occ].tobytes()
+    assert not hv[hk == -1].any()
+
+
+def test_ht_build_errors(ctx):
+    from piro_amd._lib import PiroError
+    A = ctx.laplacian_csr(3, 3, 3, 3, np.float32(1.0))
+    with pytest.raises(PiroError) as e:
+        ctx.ht_build(A, 20)  # 27 distinct keys, 20 slots
+    assert e.value.code == 4
+    rp = np.array([0, 2, 3], np.int32)
+    from piro_amd.core import DevCSR
+    bad = DevCSR(dev(rp, ctx), dev(np.array([0, 2, 1], np.int32), ctx), dev(np.ones(3, np.float32), ctx))
+    with pytest.raises(PiroError):
+        ctx.ht_build(bad, 16)
+
+
 def test_reductions(ctx):
     rng = np.random.default_rng(0)
     for n in (1, 7, 1000, 1 << 20, 3_000_001):
