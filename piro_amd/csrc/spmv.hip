@@ -18,7 +18,9 @@
 //   * the interior mask, the face-BC rewrite and the edge zeroing are applied
 //     in the same pass, and y is written once, coalesced.
 //
-// Variants differ only in rows per workgroup, the load path and the block ->
+// Variants differ only in rows per workgroup, the load path, where the x
+// values come from (gathered through the caches, or for grid operators the
+// three x runs a row block touches staged in LDS at launch) and the block ->
 // row mapping (XCD-contiguous or the hardware's round robin); all compute
 // the same bits, and an autotuner picks the fastest for the operand shape.
 #include "device_utils.hpp"
@@ -413,6 +415,103 @@ __global__ __launch_bounds__(256) void k_pf(Params p)
     finish_block<RB, MODE>(p, b, br.r0, br.nr, br.ka, br.staged, ro, s_p);
 }
 
+// ---------------------------------------------- x runs staged in LDS
+//
+// On a grid operator the x values a row block gathers lie in three runs:
+// the block's rows widened by one grid line on either side, and the block's
+// rows one plane down and one plane up.  k_pfx loads the runs into LDS at
+// launch, beside the col/val stream (their addresses depend on nothing), so
+// the gathers read LDS instead of making a third dependent trip to L2.  A
+// column outside the runs (any operator that is not a 7-point grid stencil)
+// is read from x as before, so the products, and the bits, are k_pf's.
+// The host launches it only when nx <= AXMAX.
+template <int RB, int AXMAX>
+struct XRuns {
+    static constexpr int ALEN = RB + 2 * AXMAX;   // run A capacity
+    static constexpr int N = ALEN + 2 * RB;       // + runs B and C
+    static constexpr int PER_LANE = (N + 255) / 256;
+};
+
+template <int RB, int AXMAX>
+__device__ __forceinline__ int64_t xrun_row(int t, int r0, int nx, int nxy)
+{
+    using X = XRuns<RB, AXMAX>;
+    if (t < RB + 2 * nx) return (int64_t)r0 - nx + t;
+    if (t < X::ALEN) return -1;
+    if (t < X::ALEN + RB) return (int64_t)r0 - nxy + (t - X::ALEN);
+    return (int64_t)r0 + nxy + (t - X::ALEN - RB);
+}
+
+// LDS position of column c (0 <= c < 2^31), or -1 when c lies in no run;
+// offsets modulo 2^32, so a run starting below row 0 or ending past 2^31
+// still matches exactly its in-range rows
+template <int RB, int AXMAX>
+__device__ __forceinline__ int xrun_pos(int c, int r0, int nx, int nxy)
+{
+    using X = XRuns<RB, AXMAX>;
+    const uint32_t uc = (uint32_t)c, ur0 = (uint32_t)r0;
+    const uint32_t ta = uc - (ur0 - (uint32_t)nx);
+    const uint32_t tb = uc - (ur0 - (uint32_t)nxy);
+    const uint32_t tc = uc - (ur0 + (uint32_t)nxy);
+    return ta < (uint32_t)(RB + 2 * nx) ? (int)ta
+         : tb < (uint32_t)RB ? X::ALEN + (int)tb
+         : tc < (uint32_t)RB ? X::ALEN + RB + (int)tc : -1;
+}
+
+template <int RB, int MODE, int QPL, int AXMAX>
+__global__ __launch_bounds__(256) void k_pfx(Params p)
+{
+    using X = XRuns<RB, AXMAX>;
+    constexpr int CAP = QPL * 256 * 4;
+    __shared__ __attribute__((aligned(16))) float s_p[CAP];
+    __shared__ float s_x[X::N];
+
+    const int b = logical_block(p);
+    if (b >= p.n_blocks) return;
+    if (MODE == JLI && *p.done) return;
+    const int r0 = b * RB;
+    const int nr = min(RB, p.n_rows - r0);
+    const int nx = p.g.nx, nxy = (int)p.g.fxy.d;
+
+    // the runs first: waiting for them waits for nothing issued later
+    float xv[X::PER_LANE];
+#pragma unroll
+    for (int j = 0; j < X::PER_LANE; ++j) {
+        const int t = (int)threadIdx.x + 256 * j;
+        const int64_t row = t < X::N ? xrun_row<RB, AXMAX>(t, r0, nx, nxy) : -1;
+        xv[j] = (row >= 0 && row < p.n_rows) ? p.x[row] : 0.0f;
+    }
+    const BlockRange br = block_range<RB, CAP>(p, b, p.rp[r0], p.rp[r0 + nr]);
+    Quads<QPL> qd;
+    if (br.fast) load_quads<QPL>(p, br.ka >> 2, br.k1, qd);
+    RowOps<RB / 256> ro;
+    load_rows<RB, MODE>(p, br.r0, br.nr, ro);
+#pragma unroll
+    for (int j = 0; j < X::PER_LANE; ++j) {
+        const int t = (int)threadIdx.x + 256 * j;
+        if (t < X::N) s_x[t] = xv[j];
+    }
+    __syncthreads();
+    if (br.fast) {
+#pragma unroll
+        for (int j = 0; j < QPL; ++j) {
+            float4v out;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int c = qd.c[j][e];
+                const int pos = xrun_pos<RB, AXMAX>(c, r0, nx, nxy);
+                const float xc = pos >= 0 ? s_x[pos] : p.x[c];
+                out[e] = product<MODE>(qd.v[j][e], xc, p.dt);
+            }
+            *(float4v *)(s_p + (qd.kb[j] - br.ka)) = out;
+        }
+    } else if (br.staged) {
+        stage_tail<MODE, QPL>(p, br.ka >> 2, br.ka, br.k1, s_p);
+    }
+    __syncthreads();
+    finish_block<RB, MODE>(p, b, br.r0, br.nr, br.ka, br.staged, ro, s_p);
+}
+
 // ------------------------------------------------------- row per lane
 
 template <int MODE>
@@ -446,6 +545,7 @@ struct Variant {
     bool xcd;
     bool pf;              // prefetching stream kernel
     int chunk;            // chunked XCD remap (0 = off)
+    bool xt = false;      // x runs staged in LDS (grid operators, nx <= kXtMaxNx)
 };
 
 constexpr Variant kVariants[] = {
@@ -470,12 +570,19 @@ constexpr Variant kVariants[] = {
     {"pf_rb256_c16", 256, true, false, true, 16},
     {"pf_rb256_c64", 256, true, false, true, 64},
     {"pf_rb256_c128", 256, true, false, true, 128},
+    {"pfx_rb256", 256, true, false, true, 0, true},
+    {"pfx_rb256_c32", 256, true, false, true, 32, true},
+    {"pfx_rb256_c64", 256, true, false, true, 64, true},
+    {"pfx_rb256_c16", 256, true, false, true, 16, true},
+    {"pfx_rb256_c128", 256, true, false, true, 128, true},
+    {"pfx_rb256_xcd", 256, true, true, true, 0, true},
 };
+constexpr int kXtMaxNx = 512;
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
-constexpr int kDefaultVariant = 9;   // pf_rb256
+constexpr int kDefaultVariant = 23;  // pfx_rb256_c64, the fastest measured on 7-point grid operators
 constexpr int kScalarVariant = 6;    // stream_rb512_scalar: no 16-byte alignment needed
-static_assert(kVariants[kDefaultVariant].pf && kVariants[kDefaultVariant].rows_per_block == 256 &&
-                  !kVariants[kDefaultVariant].xcd && kVariants[kDefaultVariant].chunk == 0,
+static_assert(kVariants[kDefaultVariant].pf && kVariants[kDefaultVariant].xt &&
+                  kVariants[kDefaultVariant].rows_per_block == 256 && kVariants[kDefaultVariant].chunk == 64,
               "default variant");
 static_assert(!kVariants[kScalarVariant].vec, "scalar fallback variant");
 
@@ -499,6 +606,11 @@ void launch_mode(const Variant &v, Params p, hipStream_t s)
         grid = (int)((nb + span - 1) / span * span);
     }
     if (v.pf) {
+        if (v.xt && (MODE == FE_INTERIOR || MODE == JLI) && p.g.nx <= kXtMaxNx) {
+            k_pfx<256, MODE, 2, kXtMaxNx><<<grid, 256, 0, s>>>(p);
+            check_launch();
+            return;
+        }
         // quads per lane: 7 entries per row of the 7-point operator plus the
         // alignment slack; wider rows take the unstaged path
         switch (v.rows_per_block) {
@@ -564,7 +676,7 @@ Params make_params(int nx, int ny, int nz, int n_rows, const int32_t *rp, const 
 
 int pick_variant(piro_ctx *ctx, const Params &p)
 {
-    // untuned default: pf_rb256, the fastest measured on 7-point operators
+    // untuned default: pfx_rb256_c64, the fastest measured on 7-point operators
     int v = ctx->fe_variant >= 0 ? ctx->fe_variant : kDefaultVariant;
     if (kVariants[v].vec && !vec_ok(p.col, p.val)) v = kScalarVariant;
     return v;

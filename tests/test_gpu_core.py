@@ -236,6 +236,44 @@ def test_generic_csr_modes_all_variants(ctx, long_len):
         assert mv.tobytes() == ref_mv.tobytes(), v
 
 
+@pytest.mark.parametrize("dims,kind", [((20, 25, 10), "random"), ((512, 3, 4), "lap"), ((600, 4, 3), "lap"),
+                                       ((7, 5, 3), "random"), ((300, 2, 9), "random")])
+def test_fe_grid_step_all_variants(ctx, dims, kind):
+    # the interior/face step on grid shapes: operators whose columns leave
+    # the staged x runs (random), run A at its widest (nx = 512) and past it
+    # (nx = 600, which takes the unstaged-x kernel)
+    rng = np.random.default_rng(sum(dims))
+    n = dims[0] * dims[1] * dims[2]
+    if kind == "random":
+        R = _random_csr(rng, n, n) if n > 1700 else po.CSR(*_small_random_csr(rng, n))
+        A = ctx.csr_from_host(R.row_ptr, R.col, R.val)
+    else:
+        A = ctx.laplacian_csr(*dims, 3, po.lap_norm(1.0, dims[0] - 2))
+        R = po.CSR(*A.host())
+    xh = rng.uniform(-1, 1, n).astype(np.float32)
+    sh = rng.uniform(-1, 1, n).astype(np.float32)
+    x, snap = dev(xh, ctx), dev(sh, ctx)
+    dt = np.float32(0.01)
+    ref = po.fe_csr_step(dims, R, dt, xh, sh)
+    for v in range(N_VARIANTS):
+        print(f"case grid step dims={dims} kind={kind} variant={v}", flush=True)
+        ctx.set_variant(v)
+        try:
+            got = host(ctx.fe_step_csr(dims, A, dt, x, snap))
+        finally:
+            ctx.set_variant(-1)
+        assert got.tobytes() == ref.tobytes(), v
+
+
+def _small_random_csr(rng, n):
+    lens = rng.integers(0, 9, n)
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    col = rng.integers(0, n, int(rp[-1])).astype(np.int32)
+    val = rng.uniform(-1, 1, int(rp[-1])).astype(np.float32)
+    return rp, col, val
+
+
 def ref_mv_from(R, xh):
     # sequential per-row sum of fl(v * x[c]), the order of every variant
     y = np.zeros(R.n_rows, np.float32)
