@@ -195,7 +195,8 @@ def bench_batch(args):
                       "rows": b.total_rows, "nnz": b.nnz, "seconds": t, "us_per_step": t / steps * 1e6,
                       "cell_steps_per_s": b.total_rows * steps / t, "hbm_gbs": byts * steps / t / 1e9,
                       "roofline_frac": byts * steps / t / 1e9 / HBM, "tiling_rows_per_unit": rb,
-                      "tiling_grid": g, "schedule_chunk": b.schedule()[2], "bit_exact_vs_single_instance": ok,
+                      "tiling_grid": g, "schedule_chunk": b.schedule()[2], "xruns_in_lds": b.xruns(),
+                      "bit_exact_vs_single_instance": ok,
                       "fp64_rel_l2_8_instances": (num / den) ** 0.5}),
           flush=True)
     b.close()

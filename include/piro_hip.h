@@ -429,7 +429,14 @@ int piro_batch_set_schedule(piro_batch *b, int rows_per_unit, int grid,
                             int chunk);
 int piro_batch_get_schedule(piro_batch *b, int *rows_per_unit, int *grid,
                             int *chunk);
-/* time every schedule on these buffers and keep the fastest */
+/* x runs: the staged step kernels (256 rows per unit and up) load each
+ * unit's x runs (its rows widened by one grid line, and for 3-D instances
+ * the rows one plane down and up) into LDS beside the operator stream;
+ * results are bit-identical either way.  1 = on (the default). */
+int piro_batch_set_xruns(piro_batch *b, int enable);
+int piro_batch_get_xruns(piro_batch *b, int *enabled);
+/* time every schedule (with and without the x runs) on these buffers and
+ * keep the fastest */
 int piro_batch_autotune(piro_batch *b, float *d_x, float *d_work,
                         int *best_rows_per_unit, int *best_grid,
                         float *best_ms);

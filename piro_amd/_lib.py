@@ -97,6 +97,8 @@ _SIGS = {
     "piro_batch_create_mixed": [vp, i32, P_i32, P_i32, P_f32, vp, P_f32, C.POINTER(vp)],
     "piro_batch_set_schedule": [vp, i32, i32, i32],
     "piro_batch_get_schedule": [vp, P_i32, P_i32, P_i32],
+    "piro_batch_set_xruns": [vp, i32],
+    "piro_batch_get_xruns": [vp, P_i32],
     "piro_batch_destroy": [vp],
     "piro_batch_info": [vp, P_i64, P_i64, P_i64],
     "piro_batch_operator": [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)],

@@ -508,6 +508,14 @@ class Batch:
         check(lib.piro_batch_get_schedule(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
+    def set_xruns(self, enable: bool):
+        check(lib.piro_batch_set_xruns(self.h, int(bool(enable))))
+
+    def xruns(self) -> bool:
+        v = C.c_int()
+        check(lib.piro_batch_get_xruns(self.h, C.byref(v)))
+        return bool(v.value)
+
     def autotune(self, x, work):
         rb, g, ms = C.c_int(), C.c_int(), C.c_float()
         check(lib.piro_batch_autotune(self.h, _p(x), _p(work), C.byref(rb), C.byref(g), C.byref(ms)))

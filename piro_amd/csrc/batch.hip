@@ -58,12 +58,13 @@ struct piro_batch {
     int n_units = 0;
     int grid = 0;   // persistent workgroups (0: the default, eight per CU)
     int chunk = -1; // one workgroup per unit, runs of `chunk` units per XCD (0: plain order); -1: persistent
+    int xruns = 1;  // x runs staged in LDS by the staged step kernels
     // captured graph
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     const float *g_x = nullptr;
     float *g_w = nullptr;
-    int g_steps = -1, g_rb = 0, g_grid = 0, g_chunk = -2;
+    int g_steps = -1, g_rb = 0, g_grid = 0, g_chunk = -2, g_xruns = -1;
 };
 
 namespace {
@@ -277,20 +278,57 @@ __global__ __launch_bounds__(256) void k_batch_step(const Unit *__restrict__ uni
 typedef int int4v __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
+// x runs (XT): a unit's gathers on its instance's grid lie in the unit's rows
+// widened by one grid line on either side (run A) and, for a 3-D instance,
+// the unit's rows one plane down and one plane up (runs B and C).  They are
+// loaded into LDS when the unit starts, beside the col/val stream, so the
+// gathers read LDS; a column outside the runs reads x as without them, and
+// an instance wider than kBatchAx takes the plain gathers.  The products,
+// and so the bits, are the same either way.
+constexpr int kBatchAx = 256;
+
+template <int RB>
+struct BatchRuns {
+    static constexpr int ALEN = RB + 2 * kBatchAx;
+    static constexpr int N = ALEN + 2 * RB;
+    static constexpr int PER_LANE = (N + 255) / 256;
+};
+
+// LDS position of global column c for a unit starting at row r0 with nr
+// rows (run A of nr + 2 nx rows, runs B and C of nb rows each; nb = 0 for a
+// 2-D instance), or -1; offsets modulo 2^32
+template <int RB>
+__device__ __forceinline__ int batch_run_pos(int c, int64_t r0, int nr, int nx, int nxy, int nb)
+{
+    using X = BatchRuns<RB>;
+    const uint32_t uc = (uint32_t)c, ur0 = (uint32_t)r0;
+    const uint32_t ta = uc - (ur0 - (uint32_t)nx);
+    const uint32_t tb = uc - (ur0 - (uint32_t)nxy);
+    const uint32_t tc = uc - (ur0 + (uint32_t)nxy);
+    return ta < (uint32_t)(nr + 2 * nx) ? (int)ta
+         : tb < (uint32_t)nb ? X::ALEN + (int)tb
+         : tc < (uint32_t)nb ? X::ALEN + RB + (int)tc : -1;
+}
+
 // NB LDS buffers: two alternate between a persistent workgroup's
 // consecutive units; one unit per workgroup needs one (and leaves room for
 // more resident workgroups)
-template <int RB, int QPL, int NB>
+template <int RB, int QPL, int NB, bool XT>
 __global__ __launch_bounds__(256) void k_batch_pf(const Unit *__restrict__ units, int n_units,
                                                   const InstDesc *__restrict__ inst,
                                                   const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
                                                   const float *__restrict__ val, int64_t nnz,
-                                                  const float *__restrict__ x, float *__restrict__ y, int chunk)
+                                                  const float *__restrict__ x, float *__restrict__ y, int chunk,
+                                                  int64_t total_rows)
 {
     constexpr int RPL = RB / 256;
     constexpr int CAP = QPL * 1024;
+    using X = BatchRuns<RB>;
     __shared__ __attribute__((aligned(16))) float s_p[NB][CAP];
+    __shared__ float s_x[XT ? X::N : 1];
     const int t = threadIdx.x;
+    const int lane0 = t & 63;
+    const int wv0 = __builtin_amdgcn_readfirstlane(t >> 6);
     int buf = 0;
     const int ustep = chunk < 0 ? (int)gridDim.x : n_units;  // one unit unless persistent
     for (int u = first_unit(chunk); u < n_units; u += ustep, buf ^= 1) {
@@ -298,6 +336,33 @@ __global__ __launch_bounds__(256) void k_batch_pf(const Unit *__restrict__ units
         const InstDesc d = inst[un.inst];
         const int64_t r0 = un.row0;
         const int nr = un.n_rows;
+        // the runs first (their addresses need only the descriptors)
+        const bool runs = XT && d.nx <= kBatchAx;
+        const int nxy = d.nx * d.ny;
+        const int nb = d.nz > 1 ? nr : 0;
+        if (runs) {
+            // straight into LDS (no registers): each wave fills 64
+            // consecutive slots per load, run by run; a chunk of slots its
+            // run does not use is skipped by the whole wave, and a slot past
+            // the grid's rows is filled from a clamped row no column names.
+            // Rows and columns fit int32 (the operator's index type).
+            const int last = (int)(total_rows - 1);
+            const int ir0 = (int)r0;
+            auto fill = [&](int slot0, int len, int row0) {
+                for (int cs = 64 * wv0; cs < len; cs += 256) {
+                    int row = row0 + cs + lane0;
+                    row = row < 0 ? 0 : (row > last ? last : row);
+                    __builtin_amdgcn_global_load_lds((const void *)(x + row),
+                                                     (__attribute__((address_space(3))) void *)(s_x + slot0 + cs), 4,
+                                                     0, 0);
+                }
+            };
+            fill(0, nr + 2 * d.nx, ir0 - d.nx);
+            if (nb) {
+                fill(X::ALEN, nb, ir0 - nxy);
+                fill(X::ALEN + RB, nb, ir0 + nxy);
+            }
+        }
         const int k0 = rp[r0], k1 = rp[r0 + nr];
         const int ka = k0 & ~3;
         const bool staged = k1 - ka <= CAP;
@@ -347,12 +412,22 @@ __global__ __launch_bounds__(256) void k_batch_pf(const Unit *__restrict__ units
             interior[j] = xx > 0 && xx < (uint32_t)d.nx - 1 && yy > 0 && yy < (uint32_t)d.ny - 1 &&
                           (d.nz == 1 || (zz > 0 && zz < (uint32_t)d.nz - 1));
         }
+        if (runs) __syncthreads();  // the runs are in LDS
         if (fast) {
 #pragma unroll
             for (int j = 0; j < QPL; ++j) {
                 float4v o;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) o[e] = v[j][e] * x[c[j][e]];
+                for (int e = 0; e < 4; ++e) {
+                    float xc;
+                    if (XT) {
+                        const int pos = runs ? batch_run_pos<RB>(c[j][e], r0, nr, d.nx, nxy, nb) : -1;
+                        xc = pos >= 0 ? s_x[pos] : x[c[j][e]];
+                    } else {
+                        xc = x[c[j][e]];
+                    }
+                    o[e] = v[j][e] * xc;
+                }
                 *(float4v *)(sp + (kb[j] - ka)) = o;
             }
         } else if (staged) {
@@ -404,12 +479,17 @@ int launch_grid(const piro_batch *b)
 template <int RB, int QPL>
 void launch_pf(piro_batch *b, const float *x, float *y, hipStream_t s, int g, int c)
 {
-    if (c < 0)
-        k_batch_pf<RB, QPL, 2><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x,
-                                                 y, c);
-    else
-        k_batch_pf<RB, QPL, 1><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, x,
-                                                 y, c);
+#define PIRO_BATCH_PF(NB_, XT_)                                                                                    \
+    k_batch_pf<RB, QPL, NB_, XT_><<<g, 256, 0, s>>>(b->d_units, b->n_units, b->d_inst, b->rp, b->col, b->val, b->nnz, \
+                                                    x, y, c, b->total_rows)
+    if (b->xruns) {
+        if (c < 0) PIRO_BATCH_PF(2, true);
+        else PIRO_BATCH_PF(1, true);
+    } else {
+        if (c < 0) PIRO_BATCH_PF(2, false);
+        else PIRO_BATCH_PF(1, false);
+    }
+#undef PIRO_BATCH_PF
 }
 
 void launch_step(piro_batch *b, const float *x, float *y, hipStream_t s)
@@ -644,6 +724,25 @@ int piro_batch_get_tiling(piro_batch *b, int *rows_per_unit, int *grid)
     return piro_batch_get_schedule(b, rows_per_unit, grid, nullptr);
 }
 
+int piro_batch_set_xruns(piro_batch *b, int enable)
+{
+    PIRO_API_BEGIN
+    bat(b);
+    PIRO_REQUIRE(enable == 0 || enable == 1, "enable must be 0 or 1");
+    b->xruns = enable;
+    drop_graph(b);
+    PIRO_API_END
+}
+
+int piro_batch_get_xruns(piro_batch *b, int *enabled)
+{
+    PIRO_API_BEGIN
+    bat(b);
+    PIRO_REQUIRE(enabled, "null pointer");
+    *enabled = b->xruns;
+    PIRO_API_END
+}
+
 int piro_batch_step(piro_batch *b, const float *d_x, float *d_y)
 {
     PIRO_API_BEGIN
@@ -671,7 +770,7 @@ int piro_batch_run(piro_batch *b, float *d_x, float *d_work, int nsteps, int use
     }
     const int g = launch_grid(b);
     if (!(b->gexec && b->g_x == d_x && b->g_w == d_work && b->g_steps == nsteps && b->g_rb == b->rb &&
-          b->g_grid == g && b->g_chunk == b->chunk)) {
+          b->g_grid == g && b->g_chunk == b->chunk && b->g_xruns == b->xruns)) {
         drop_graph(b);
         hipStream_t cs;
         PIRO_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
@@ -695,6 +794,7 @@ int piro_batch_run(piro_batch *b, float *d_x, float *d_work, int nsteps, int use
         b->g_rb = b->rb;
         b->g_grid = g;
         b->g_chunk = b->chunk;
+        b->g_xruns = b->xruns;
     }
     PIRO_CHECK_HIP(hipGraphLaunch(b->gexec, s));
     if (nsteps % 2 == 1)
@@ -712,32 +812,36 @@ int piro_batch_autotune(piro_batch *b, float *d_x, float *d_work, int *best_rows
     hipEvent_t e0, e1;
     PIRO_CHECK_HIP(hipEventCreate(&e0));
     PIRO_CHECK_HIP(hipEventCreate(&e1));
-    int best_rb = 256, best_g = 0, best_c = 0;
+    int best_rb = 256, best_g = 0, best_c = 0, best_x = 1;
     float best_t = 1e30f;
     const int rbs[4] = {128, 256, 512, 1024};
     // persistent grids (workgroups per CU), then one workgroup per unit in
-    // plain order and with XCD runs of 4 to 256 units
+    // plain order and with XCD runs of 4 to 256 units; each with and
+    // without the x runs in LDS
     const int per_cu[5] = {2, 4, 8, 16, 64};
     const int chunks[6] = {0, 4, 16, 64, 128, 256};
     for (int rb : rbs) {
         build_units(b, rb);
-        for (int v = 0; v < 11; ++v) {
-            if (v < 5) {
-                b->chunk = -1;
-                b->grid = std::min(b->n_units, b->ctx->num_cus * per_cu[v]);
-            } else {
-                b->chunk = chunks[v - 5];
-                b->grid = 0;
+        for (int xr = 0; xr < 2; ++xr) {
+            b->xruns = xr;
+            for (int v = 0; v < 11; ++v) {
+                if (v < 5) {
+                    b->chunk = -1;
+                    b->grid = std::min(b->n_units, b->ctx->num_cus * per_cu[v]);
+                } else {
+                    b->chunk = chunks[v - 5];
+                    b->grid = 0;
+                }
+                launch_step(b, d_x, d_work, s);
+                PIRO_CHECK_HIP(hipEventRecord(e0, s));
+                for (int i = 0; i < 4; ++i) launch_step(b, d_x, d_work, s);
+                PIRO_CHECK_HIP(hipEventRecord(e1, s));
+                PIRO_CHECK_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                PIRO_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+                ms /= 4;
+                if (ms < best_t) { best_t = ms; best_rb = rb; best_g = b->grid; best_c = b->chunk; best_x = xr; }
             }
-            launch_step(b, d_x, d_work, s);
-            PIRO_CHECK_HIP(hipEventRecord(e0, s));
-            for (int i = 0; i < 4; ++i) launch_step(b, d_x, d_work, s);
-            PIRO_CHECK_HIP(hipEventRecord(e1, s));
-            PIRO_CHECK_HIP(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            PIRO_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-            ms /= 4;
-            if (ms < best_t) { best_t = ms; best_rb = rb; best_g = b->grid; best_c = b->chunk; }
         }
     }
     (void)hipEventDestroy(e0);
@@ -745,6 +849,7 @@ int piro_batch_autotune(piro_batch *b, float *d_x, float *d_work, int *best_rows
     build_units(b, best_rb);
     b->grid = best_g;
     b->chunk = best_c;
+    b->xruns = best_x;
     drop_graph(b);
     if (best_rows_per_unit) *best_rows_per_unit = best_rb;
     if (best_grid) *best_grid = launch_grid(b);

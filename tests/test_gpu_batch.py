@@ -55,13 +55,16 @@ def oracle_instance(dims, a, dt, x0, steps):
     return po.fe_csr_run((nx, ny, nz), A, dt, steps, x0, None)
 
 
+@pytest.mark.parametrize("xruns", [True, False])
 @pytest.mark.parametrize("tiling", [(128, 0), (256, 0), (512, 7), (1024, 3)])
-def test_batch_matches_single_instances(ctx, tiling):
-    print(f"case batch tiling={tiling}", flush=True)
+def test_batch_matches_single_instances(ctx, tiling, xruns):
+    print(f"case batch tiling={tiling} xruns={xruns}", flush=True)
     dims, alphas = make_instances(12, 1)
     alpha, dts, x0 = instance_inputs(ctx, dims, alphas, 2)
     b = ctx.batch(dims, ctx.from_host(alpha), dts)
     b.set_tiling(*tiling)
+    b.set_xruns(xruns)
+    assert b.xruns() == xruns
     x = ctx.from_host(x0)
     w = ctx.empty_like(x)
     steps = 9
@@ -209,12 +212,14 @@ def mixed_inputs(seed):
     return dims, kinds, vel, np.concatenate(alpha), dts, np.concatenate(xs)
 
 
+@pytest.mark.parametrize("xruns", [True, False])
 @pytest.mark.parametrize("schedule", [(256, 0, -1), (512, 5, -1), (128, 0, 0), (256, 0, 4), (1024, 0, 16), (512, 0, 1)])
-def test_batch_mixed_heat_and_advection_match_oracle(ctx, schedule):
-    print(f"case mixed batch schedule={schedule}", flush=True)
+def test_batch_mixed_heat_and_advection_match_oracle(ctx, schedule, xruns):
+    print(f"case mixed batch schedule={schedule} xruns={xruns}", flush=True)
     dims, kinds, vel, alpha, dts, x0 = mixed_inputs(31)
     b = ctx.batch(dims, ctx.from_host(alpha), dts, kinds=kinds, vel=vel)
     b.set_schedule(*schedule)
+    b.set_xruns(xruns)
     assert b.schedule()[2] == schedule[2]
     x = ctx.from_host(x0)
     w = ctx.empty_like(x)
@@ -229,6 +234,33 @@ def test_batch_mixed_heat_and_advection_match_oracle(ctx, schedule):
             ref = oracle_advection(d, k, v, dt, x0[s:e], steps)
         assert got[s:e].tobytes() == ref.tobytes(), f"instance {i} dims {d} kind {k}"
     b.close()
+
+
+@pytest.mark.parametrize("schedule", [(256, 0, 0), (512, 0, 4), (512, 3, -1), (1024, 0, 0)])
+def test_batch_xruns_wide_and_3d_instances(ctx, schedule):
+    # instances wider than the run cap (nx > 256 takes the plain gathers),
+    # at the cap, thin ones, and 3-D instances (runs one plane down and up),
+    # in one batch; the runs on and off give the oracle's bits
+    print(f"case xruns wide batch schedule={schedule}", flush=True)
+    dims = [(300, 5, 1), (256, 4, 1), (3, 40, 1), (258, 3, 1), (20, 18, 9), (260, 4, 3), (7, 6, 5), (64, 64, 1)]
+    alphas = np.array([0.01, 0.02, 0.005, 0.01, 0.015, 0.01, 0.02, 0.01], np.float32)
+    alpha, dts, x0 = instance_inputs(ctx, dims, alphas, 5)
+    outs = []
+    for xr in (True, False):
+        b = ctx.batch(dims, ctx.from_host(alpha), dts)
+        b.set_schedule(*schedule)
+        b.set_xruns(xr)
+        x = ctx.from_host(x0)
+        w = ctx.empty_like(x)
+        b.run(x, w, 6, use_graph=schedule[2] >= 0)
+        outs.append(x.host())
+        offsets = b.offsets
+        b.close()
+    assert outs[0].tobytes() == outs[1].tobytes()
+    for i, (d, a) in enumerate(zip(dims, alphas)):
+        s, e = offsets[i], offsets[i + 1]
+        ref = oracle_instance(d, a, dts[i], x0[s:e], 6)
+        assert outs[0][s:e].tobytes() == ref.tobytes(), f"instance {i} dims {d}"
 
 
 def test_batch_3d_advection_equals_single_grid_path(ctx):
