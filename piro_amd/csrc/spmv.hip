@@ -420,8 +420,9 @@ __global__ __launch_bounds__(256) void k_pf(Params p)
 // On a grid operator the x values a row block gathers lie in three runs:
 // the block's rows widened by one grid line on either side, and the block's
 // rows one plane down and one plane up.  k_pfx loads the runs into LDS at
-// launch, beside the col/val stream (their addresses depend on nothing), so
-// the gathers read LDS instead of making a third dependent trip to L2.  A
+// launch, beside the col/val stream (their addresses depend on nothing;
+// global_load_lds puts them straight into LDS), so the gathers read LDS
+// instead of making a third dependent trip to L2.  A
 // column outside the runs (any operator that is not a 7-point grid stencil)
 // is read from x as before, so the products, and the bits, are k_pf's.
 // The host launches it only when nx <= AXMAX.
@@ -429,18 +430,7 @@ template <int RB, int AXMAX>
 struct XRuns {
     static constexpr int ALEN = RB + 2 * AXMAX;   // run A capacity
     static constexpr int N = ALEN + 2 * RB;       // + runs B and C
-    static constexpr int PER_LANE = (N + 255) / 256;
 };
-
-template <int RB, int AXMAX>
-__device__ __forceinline__ int64_t xrun_row(int t, int r0, int nx, int nxy)
-{
-    using X = XRuns<RB, AXMAX>;
-    if (t < RB + 2 * nx) return (int64_t)r0 - nx + t;
-    if (t < X::ALEN) return -1;
-    if (t < X::ALEN + RB) return (int64_t)r0 - nxy + (t - X::ALEN);
-    return (int64_t)r0 + nxy + (t - X::ALEN - RB);
-}
 
 // LDS position of column c (0 <= c < 2^31), or -1 when c lies in no run;
 // offsets modulo 2^32, so a run starting below row 0 or ending past 2^31
@@ -473,25 +463,31 @@ __global__ __launch_bounds__(256) void k_pfx(Params p)
     const int nr = min(RB, p.n_rows - r0);
     const int nx = p.g.nx, nxy = (int)p.g.fxy.d;
 
-    // the runs first: waiting for them waits for nothing issued later
-    float xv[X::PER_LANE];
-#pragma unroll
-    for (int j = 0; j < X::PER_LANE; ++j) {
-        const int t = (int)threadIdx.x + 256 * j;
-        const int64_t row = t < X::N ? xrun_row<RB, AXMAX>(t, r0, nx, nxy) : -1;
-        xv[j] = (row >= 0 && row < p.n_rows) ? p.x[row] : 0.0f;
+    // the runs first, straight into LDS (no registers): each wave fills 64
+    // consecutive slots per load; a slot past the grid's rows is filled
+    // from a clamped row that no column can name
+    {
+        const int lane = threadIdx.x & 63;
+        const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+        const int64_t last = p.n_rows - 1;
+        auto fill = [&](int slot0, int len, int64_t row0) {
+            for (int cs = 64 * wv; cs < len; cs += 256) {
+                int64_t row = row0 + cs + lane;
+                row = row < 0 ? 0 : (row > last ? last : row);
+                __builtin_amdgcn_global_load_lds((const void *)(p.x + row),
+                                                 (__attribute__((address_space(3))) void *)(s_x + slot0 + cs), 4, 0, 0);
+            }
+        };
+        fill(0, RB + 2 * nx, (int64_t)r0 - nx);
+        fill(X::ALEN, RB, (int64_t)r0 - nxy);
+        fill(X::ALEN + RB, RB, (int64_t)r0 + nxy);
     }
     const BlockRange br = block_range<RB, CAP>(p, b, p.rp[r0], p.rp[r0 + nr]);
     Quads<QPL> qd;
     if (br.fast) load_quads<QPL>(p, br.ka >> 2, br.k1, qd);
     RowOps<RB / 256> ro;
     load_rows<RB, MODE>(p, br.r0, br.nr, ro);
-#pragma unroll
-    for (int j = 0; j < X::PER_LANE; ++j) {
-        const int t = (int)threadIdx.x + 256 * j;
-        if (t < X::N) s_x[t] = xv[j];
-    }
-    __syncthreads();
+    __syncthreads();  // the runs are in LDS
     if (br.fast) {
 #pragma unroll
         for (int j = 0; j < QPL; ++j) {
