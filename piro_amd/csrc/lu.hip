@@ -996,6 +996,65 @@ __device__ __forceinline__ void update_mfma(float4v (&acc)[4][4], const float *s
     }
 }
 
+// a workgroup tile wholly inside the matrix with a full K and 16-byte rows:
+// the same MFMAs in the same order with no bounds tests, the K loop
+// unrolled and every C load and store issued back to back
+template <int KW, int WGR, int WGC>
+__device__ __forceinline__ void update_full_tile(float *__restrict__ a, int lda, int k, int r0, int c0, int rw, int cw,
+                                                 int t, int li, int lk, float *s_a, float *s_b)
+{
+    using S = UpdShape<KW, WGR, WGC>;
+    constexpr int NT = S::NT, TC = S::TC, PA = S::PA, PBP = S::PBP;
+    float *rowp[4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) rowp[ti] = a + (int64_t)(rw + ti * 16 + li) * lda + cw + 4 * lk;
+    float4v acc[4][4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = *(const float4v *)(rowp[ti] + tj * 16);
+    float4v va[S::NQA], vb[S::NQB];
+#pragma unroll
+    for (int it = 0; it < S::NQA; ++it) {
+        const int e = t + NT * it, i = e / (KW / 4), kk = 4 * (e % (KW / 4));
+        va[it] = *(const float4v *)(a + (int64_t)(r0 + i) * lda + k + kk);
+    }
+#pragma unroll
+    for (int it = 0; it < S::NQB; ++it) {
+        const int e = t + NT * it, kk = e / (TC / 4), j = 4 * (e % (TC / 4));
+        vb[it] = *(const float4v *)(a + (int64_t)(k + kk) * lda + c0 + j);
+    }
+#pragma unroll
+    for (int it = 0; it < S::NQA; ++it) {
+        const int e = t + NT * it, i = e / (KW / 4), kk = 4 * (e % (KW / 4));
+        *(float4v *)(s_a + i * PA + kk) = -va[it];
+    }
+#pragma unroll
+    for (int it = 0; it < S::NQB; ++it) {
+        const int e = t + NT * it, kk = e / (TC / 4), j = 4 * (e % (TC / 4));
+        *(float4v *)(s_b + kk * PBP + j) = vb[it];
+    }
+    __syncthreads();
+    const int wr = rw - r0, wc = cw - c0;
+#pragma unroll
+    for (int ks = 0; ks < KW; ks += 4) {
+        float fa[4], fb[4];
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti) fa[ti] = s_a[(wr + ti * 16 + li) * PA + ks + lk];
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) fb[tj] = s_b[(ks + lk) * PBP + wc + tj * 16 + li];
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj)
+                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[tj], fa[ti], acc[ti][tj], 0, 0, 0);
+    }
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) *(float4v *)(rowp[ti] + tj * 16) = acc[ti][tj];
+}
+
 template <int KW, int WGR, int WGC>
 __global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restrict__ a, int lda, int n, int k, int jb,
                                                                int r0g, int c0g, int c1, bool vec)
@@ -1008,10 +1067,14 @@ __global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restric
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int rw = r0 + (w / WGC) * 64, cw = c0 + (w % WGC) * 64;
     const int li = lane & 15, lk = lane >> 4;
+    const bool quads = vec && c1 % 4 == 0;  // uniform
+    if (quads && jb == KW && r0 + S::TR <= n && c0 + S::TC <= c1) {  // uniform across the workgroup
+        update_full_tile<KW, WGR, WGC>(a, lda, k, r0, c0, rw, cw, t, li, lk, s_a, s_b);
+        return;
+    }
     int64_t roff[4];
 #pragma unroll
     for (int ti = 0; ti < 4; ++ti) roff[ti] = (int64_t)min(rw + ti * 16 + li, n - 1) * lda;
-    const bool quads = vec && c1 % 4 == 0;  // uniform
     float4v acc[4][4];
     update_load_c(acc, a, roff, n, rw, cw, c1, quads, li, lk);
     update_stage<KW, WGR, WGC>(s_a, s_b, a, lda, n, k, jb, r0, c0, c1, vec, t);
