@@ -248,7 +248,11 @@ def bench_advection(args):
     dt = np.float32(0.5 * 2 * dx / sum(abs(v) for v in vel))
     x0 = ctx.ic_tophat(dims, [1.0] * 3, [0.5] * 3, 0.15)
     snap = ctx.apply_face_values(dims, x0, [0.0] * 6)
-    ctx.autotune(dims, D, dt, x0, snap, ctx.empty_like(x0))
+    if args.variant >= 0:  # pinned (profiling runs: counters perturb the tuner's timings)
+        ctx.set_variant(args.variant)
+        variant = args.variant
+    else:
+        variant, _ = ctx.autotune(dims, D, dt, x0, snap, ctx.empty_like(x0))
     x = x0.clone()
     w = ctx.empty_like(x)
     steps = args.steps or 1000
@@ -268,7 +272,8 @@ def bench_advection(args):
         parity = {"bit_exact_vs_fp32_oracle": got.tobytes() == ref32.tobytes(),
                   "fp64_rel_l2": po.rel_l2(got, ref64)}
     print(json.dumps({"metric": f"advection {side}^3 upwind via geam3 concatenation, FE", "nnz": D.nnz, **parity,
-                      "geam_seconds": t_geam, "steps": steps, "us_per_step": t / steps * 1e6,
+                      "geam_seconds": t_geam, "steps": steps, "kernel_variant": ctx.variants()[variant],
+                      "us_per_step": t / steps * 1e6,
                       "cell_steps_per_s": N * steps / t, "hbm_gbs": byts * steps / t / 1e9,
                       "roofline_frac": byts * steps / t / 1e9 / HBM, "mass_ratio": mass1 / mass0,
                       "max_after": ctx.reduce("absmax", x)}), flush=True)
@@ -534,7 +539,8 @@ def bench_jli(args):
                       "residual": res, "seconds_to_converge": t, "fixed_iterations": fixed,
                       "us_per_iteration": per_it * 1e6, "hbm_gbs": byts / per_it / 1e9,
                       "roofline_frac": byts / per_it / 1e9 / HBM,
-                      "algorithmic_bytes_per_iteration": byts}), flush=True)
+                      "algorithmic_bytes_per_iteration": byts,
+                      "kernel_variant": "untuned default (pfx_rb256_c64)"}), flush=True)
     ctx.close()
 
 
@@ -560,6 +566,8 @@ def main():
     ap.add_argument("--passes", type=int, default=3)
     ap.add_argument("--no-oracle", action="store_true", help="skip the CPU oracle checks of the advection run")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 LU comparison (profiling runs)")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="advection: force an FE kernel variant (-1 = autotune)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first time the library's streaming copy of a 256 MiB vector 10 times (the known-size "
                          "kernel that HBM counter readings are calibrated against)")
