@@ -40,6 +40,10 @@ namespace {
 
 constexpr int PB = 16;  // panel width
 constexpr int NB = 64;  // factorisation block: 4 panels, then one K = 64 update
+// trailing updates of at most this many rows use 32 x 32 wave tiles (at
+// 4096^2 they are faster at every size: 4.68 -> 4.41 ms; larger updates keep
+// the 128 x 128 workgroup tiles, whose operands are re-read less)
+constexpr int kUpd32Max = 4096;
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -1102,6 +1106,142 @@ __global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restric
 }
 
 
+// The trailing update with 32 x 32 wave tiles: four waves, a 64 x 64
+// workgroup tile.  For updates too small to give every SIMD a 64 x 64 tile
+// (at most (m / 64)^2 waves), quartering the wave tile gives four times the
+// waves, each a quarter of the MFMA chain.  Operand layouts, LDS pitches and
+// every C entry's MFMAs (K in steps of 4, ascending) are k_update_mfma's, so
+// the bits are the same.
+template <int KW>
+__global__ __launch_bounds__(256) void k_update_mfma32(float *__restrict__ a, int lda, int n, int k, int jb, int r0g,
+                                                        int c0g, int c1, bool vec)
+{
+    constexpr int NT = 256, TR = 64, TC = 64, PA = KW + 4, PBP = TC + 16;
+    constexpr int NQA = TR * KW / 4 / NT, NQB = KW * TC / 4 / NT;
+    static_assert(NQA * NT * 4 == TR * KW && NQB * NT * 4 == KW * TC, "staging split");
+    __shared__ __attribute__((aligned(16))) float s_a[TR * PA];   // -L21
+    __shared__ __attribute__((aligned(16))) float s_b[KW * PBP];  // U12
+    const int r0 = r0g + blockIdx.y * TR, c0 = c0g + blockIdx.x * TC;
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int rw = r0 + wr, cw = c0 + wc;
+    const int li = lane & 15, lk = lane >> 4;
+    const bool quads = vec && c1 % 4 == 0;                                 // uniform
+    const bool full = quads && jb == KW && r0 + TR <= n && c0 + TC <= c1;  // uniform across the workgroup
+    int64_t roff[2];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti) roff[ti] = (int64_t)min(rw + ti * 16 + li, n - 1) * lda;
+    // C: with quads every load from a clamped address (an outside quad is
+    // never stored, and an entry depends on its own C entry only)
+    float4v acc[2][2];
+    if (quads) {
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+                acc[ti][tj] = *(const float4v *)(a + roff[ti] + min(cw + tj * 16 + 4 * lk, c1 - 4));
+    } else {
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) {
+                const int row = rw + ti * 16 + li, col = cw + tj * 16 + 4 * lk;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[ti][tj][r] = (row < n && col + r < c1) ? a[roff[ti] + col + r] : 0.0f;
+            }
+    }
+    // stage -L21 and U12 (as update_stage does, for a 64 x 64 tile)
+    if (vec && jb == KW && c1 % 4 == 0) {
+        float4v va[NQA], vb[NQB];
+#pragma unroll
+        for (int it = 0; it < NQA; ++it) {
+            const int e = t + NT * it, i = e / (KW / 4), kk = 4 * (e % (KW / 4));
+            va[it] = *(const float4v *)(a + (int64_t)min(r0 + i, n - 1) * lda + k + kk);
+        }
+#pragma unroll
+        for (int it = 0; it < NQB; ++it) {
+            const int e = t + NT * it, kk = e / (TC / 4), j = 4 * (e % (TC / 4));
+            vb[it] = *(const float4v *)(a + (int64_t)(k + kk) * lda + min(c0 + j, c1 - 4));
+        }
+#pragma unroll
+        for (int it = 0; it < NQA; ++it) {
+            const int e = t + NT * it, i = e / (KW / 4), kk = 4 * (e % (KW / 4));
+            *(float4v *)(s_a + i * PA + kk) = r0 + i < n ? -va[it] : (float4v){0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int it = 0; it < NQB; ++it) {
+            const int e = t + NT * it, kk = e / (TC / 4), j = 4 * (e % (TC / 4));
+            *(float4v *)(s_b + kk * PBP + j) = c0 + j < c1 ? vb[it] : (float4v){0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    } else {
+        for (int e = t; e < TR * KW / 4; e += NT) {
+            const int i = e / (KW / 4), kk = 4 * (e % (KW / 4)), gi = r0 + i;
+            const float *src = a + (int64_t)min(gi, n - 1) * lda + k + kk;
+            float4v v;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = (gi < n && kk + r < jb) ? src[r] : 0.0f;
+            *(float4v *)(s_a + i * PA + kk) = -v;
+        }
+        for (int e = t; e < KW * TC / 4; e += NT) {
+            const int kk = e / (TC / 4), j = 4 * (e % (TC / 4)), gj = c0 + j;
+            const float *src = a + (int64_t)(k + min(kk, jb - 1)) * lda + gj;
+            float4v v;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = (kk < jb && gj + r < c1) ? src[r] : 0.0f;
+            *(float4v *)(s_b + kk * PBP + j) = v;
+        }
+    }
+    __syncthreads();
+    if (full) {
+#pragma unroll
+        for (int ks = 0; ks < KW; ks += 4) {
+            float fa[2], fb[2];
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti) fa[ti] = s_a[(wr + ti * 16 + li) * PA + ks + lk];
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) fb[tj] = s_b[(ks + lk) * PBP + wc + tj * 16 + li];
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj)
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[tj], fa[ti], acc[ti][tj], 0, 0, 0);
+        }
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) *(float4v *)(a + roff[ti] + cw + tj * 16 + 4 * lk) = acc[ti][tj];
+        return;
+    }
+    for (int ks = 0; ks < KW; ks += 4) {
+        if (ks >= jb) break;  // uniform; staged entries past jb are 0 anyway
+        float fa[2], fb[2];
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti) fa[ti] = s_a[(wr + ti * 16 + li) * PA + ks + lk];
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) fb[tj] = s_b[(ks + lk) * PBP + wc + tj * 16 + li];
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+                if (rw + ti * 16 < n && cw + tj * 16 < c1)
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[tj], fa[ti], acc[ti][tj], 0, 0, 0);
+    }
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+            const int row = rw + ti * 16 + li, col = cw + tj * 16 + 4 * lk;
+            if (quads) {
+                if (row < n && col < c1) *(float4v *)(a + roff[ti] + col) = acc[ti][tj];
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (row < n && col + r < c1) a[roff[ti] + col + r] = acc[ti][tj][r];
+            }
+        }
+}
+
 // The in-block update of a full panel's later columns [c0, c1) (c0 = k + 16,
 // c1 - c0 <= 48: one wave per 64-row tile, the update k_update_mfma<16, 1, 1>
 // makes) fused with the speculative pass of the next panel (columns c0 ..
@@ -1838,8 +1978,14 @@ void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipi
             const int gt = (n - c0 + 63) / 64 + (kb + 255) / 256;
             k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv);
             check_launch();
-            dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
-            k_update_mfma<NB, 2, 2><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vec && kb % 4 == 0 && c0 % 4 == 0);
+            const bool vu = vec && kb % 4 == 0 && c0 % 4 == 0;
+            if (n - c0 <= kUpd32Max) {  // quarter wave tiles: four times the waves
+                dim3 g((n - c0 + 63) / 64, (n - c0 + 63) / 64);
+                k_update_mfma32<NB><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu);
+            } else {
+                dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
+                k_update_mfma<NB, 2, 2><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu);
+            }
             check_launch();
         } else if (kb > 0) {
             k_laswp<<<grid_for(kb, 256), 256, 0, s>>>(a, lda, kb, nbk, ipiv, 0, kb);
