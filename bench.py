@@ -139,7 +139,9 @@ def surrogate_leg(ctx):
                         lr_final=1e-5, blob_widths=(4.0, 20.0), noise=0.03, log_every=400)
     run = learned.HeatRun(ctx, side=128)
     steps = 1600
-    run.plain(160)
+    # each path warmed by a run of the timed length (the first ~100 launches
+    # on a fresh box run slower than the steady state)
+    run.plain(steps)
     ref, t_plain = run.plain(steps)
     out = {
         "workload": "heat 128^3, smooth alpha field in [0.005,0.02], Gaussian initial condition, 1600 FE steps; "
@@ -153,7 +155,7 @@ def surrogate_leg(ctx):
         "train_seconds": log.seconds,
     }
     for key, correct in (("corrected", True), ("uncorrected", False)):
-        run.accelerated(sur, 160, correct=correct)
+        run.accelerated(sur, steps, correct=correct)
         acc, res = run.accelerated(sur, steps, correct=correct)
         out[key] = {
             "speedup_vs_plain": t_plain / res.seconds,

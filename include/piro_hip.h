@@ -276,6 +276,14 @@ int piro_vec_copy(piro_ctx *ctx, int64_t n, const float *d_src, float *d_dst);
 /* sum (x_i - y_i)^2 in FP64 (relative-L2 checks without a read-back) */
 int piro_reduce_diff_sumsq(piro_ctx *ctx, int64_t n, const float *d_x,
                            const float *d_y, double *h_out);
+/* d_out[0] = sum (x_i - y_i)^2 and d_out[1] = sum y_i^2 in FP64, written
+ * to device memory in stream order with no wait (the same sums as the
+ * calls above): relative-L2 checks inside a launch sequence, read back
+ * once at its end.  No reference counterpart (PIRO sums residuals on the
+ * host after a read-back, src/extras.cpp:24-30). */
+int piro_reduce_diff_sumsq_pair_async(piro_ctx *ctx, int64_t n,
+                                      const float *d_x, const float *d_y,
+                                      double *d_out);
 
 /* ------------------------------------------------------------- JLI solve */
 
@@ -475,6 +483,14 @@ int piro_surrogate_set_params(piro_surrogate *s, const float *h_params);
 int piro_surrogate_forward(piro_surrogate *s, int n, int D, int H, int W,
                            const float *d_u, const float *d_alpha, float scale,
                            float *d_u_out);
+/* the same, plus d_add (may be NULL) added to every output voxel after the
+ * surrogate's own result is rounded: bit-identical to forward followed by
+ * piro_vec_axpby(1, out, 1, add), one pass fewer (the accelerated run's
+ * defect correction) */
+int piro_surrogate_forward_add(piro_surrogate *s, int n, int D, int H, int W,
+                               const float *d_u, const float *d_alpha,
+                               float scale, const float *d_add,
+                               float *d_u_out);
 /* The composed stencils in FP64: h_k[((dz*13 + dy)*13 + dx)*3 + c] is the
  * weight of u(z + dz - 6, y + dy - 6, x + dx - 6) in output o_c. */
 int piro_surrogate_stencil(piro_surrogate *s, double *h_k);

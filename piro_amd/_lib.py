@@ -84,6 +84,7 @@ _SIGS = {
     "piro_reduce_dot": [vp, i64, vp, vp, P_f64],
     "piro_reduce_absmax": [vp, i64, vp, P_f64],
     "piro_reduce_diff_sumsq": [vp, i64, vp, vp, P_f64],
+    "piro_reduce_diff_sumsq_pair_async": [vp, i64, vp, vp, vp],
     "piro_vec_axpby": [vp, i64, f32, vp, f32, vp, vp],
     "piro_vec_copy": [vp, i64, vp, vp],
     "piro_csr_to_dense": [vp, i32, i32, vp, vp, vp, vp, i32],
@@ -115,6 +116,7 @@ _SIGS = {
     "piro_surrogate_get_params": [vp, P_f32],
     "piro_surrogate_set_params": [vp, P_f32],
     "piro_surrogate_forward": [vp, i32, i32, i32, i32, vp, vp, f32, vp],
+    "piro_surrogate_forward_add": [vp, i32, i32, i32, i32, vp, vp, f32, vp, vp],
     "piro_surrogate_stencil": [vp, P_f64],
     "piro_surrogate_gradients": [vp, i32, i32, i32, i32, vp, vp, vp, f32, P_f32, P_f64],
     "piro_surrogate_train_step": [vp, i32, i32, i32, i32, vp, vp, vp, f32, f32, P_f64],

@@ -401,6 +401,13 @@ class Context:
         check(lib.piro_vec_axpby(self.h, x.numel(), float(a), _p(x), float(b), _p(y), _p(out)))
         return out
 
+    def diff_sumsq_pair_async(self, a: DevArray, b: DevArray, out: DevArray, slot: int = 0):
+        """out[2 slot] = sum (a - b)^2, out[2 slot + 1] = sum b^2 (FP64, on the
+        device, no wait); out is a float64 DevArray"""
+        assert out.dtype == np.float64 and 2 * slot + 2 <= out.numel()
+        check(lib.piro_reduce_diff_sumsq_pair_async(self.h, a.numel(), _p(a), _p(b),
+                                                    C.c_void_p(out.ptr + 16 * slot)))
+
     def rel_l2(self, a: DevArray, b: DevArray) -> float:
         """||a - b|| / ||b|| on the device (FP64 sums)"""
         d = self.reduce("diffsq", a, b)
@@ -563,10 +570,15 @@ class Surrogate:
         assert p.size == self.n_params
         check(lib.piro_surrogate_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
 
-    def forward(self, shape, u, alpha, scale, out=None):
+    def forward(self, shape, u, alpha, scale, out=None, add=None):
+        """out = the surrogate's k-step result (+ add, when given)"""
         n, D, H, W = shape
         out = self.ctx.empty_like(u) if out is None else out
-        check(lib.piro_surrogate_forward(self.h, n, D, H, W, _p(u), _p(alpha), float(scale), _p(out)))
+        if add is None:
+            check(lib.piro_surrogate_forward(self.h, n, D, H, W, _p(u), _p(alpha), float(scale), _p(out)))
+        else:
+            check(lib.piro_surrogate_forward_add(self.h, n, D, H, W, _p(u), _p(alpha), float(scale), _p(add),
+                                                 _p(out)))
         return out
 
     def stencil(self):

@@ -87,13 +87,28 @@ __global__ __launch_bounds__(RBLOCK) void k_reduce_pass1(int64_t n, const float 
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// out: where the result goes (the partials' last slot, or a caller's word)
 template <int OP>
-__global__ __launch_bounds__(RBLOCK) void k_reduce_pass2(int nparts, double *__restrict__ partial)
+__global__ __launch_bounds__(RBLOCK) void k_reduce_pass2(int nparts, const double *__restrict__ partial,
+                                                         double *__restrict__ out)
 {
     double s = 0.0;
     for (int i = threadIdx.x; i < nparts; i += RBLOCK) s = combine<OP>(s, partial[i]);
     s = block_combine<OP>(s);
-    if (threadIdx.x == 0) partial[RGRID_MAX] = s;
+    if (threadIdx.x == 0) *out = s;
+}
+
+// both passes on the stream, the result in *out (device memory); no wait
+template <int OP>
+void launch_reduce(piro_ctx *ctx, int64_t n, const float *x, const float *y, double *out)
+{
+    double *part = (double *)ctx->scratch[3].get((RGRID_MAX + 1) * sizeof(double));
+    int64_t want = (n / 4 + RBLOCK - 1) / RBLOCK;
+    int grid = (int)(want < 1 ? 1 : (want > RGRID_MAX ? RGRID_MAX : want));
+    k_reduce_pass1<OP><<<grid, RBLOCK, 0, ctx->stream>>>(n, x, y, part);
+    check_launch();
+    k_reduce_pass2<OP><<<1, RBLOCK, 0, ctx->stream>>>(grid, part, out ? out : part + RGRID_MAX);
+    check_launch();
 }
 
 template <int OP>
@@ -101,13 +116,8 @@ double run(piro_ctx *ctx, int64_t n, const float *x, const float *y)
 {
     PIRO_REQUIRE(n >= 0 && x && ((OP != DOT && OP != DIFFSQ) || y), "bad reduction arguments");
     if (n == 0) return 0.0;
+    launch_reduce<OP>(ctx, n, x, y, nullptr);
     double *part = (double *)ctx->scratch[3].get((RGRID_MAX + 1) * sizeof(double));
-    int64_t want = (n / 4 + RBLOCK - 1) / RBLOCK;
-    int grid = (int)(want < 1 ? 1 : (want > RGRID_MAX ? RGRID_MAX : want));
-    k_reduce_pass1<OP><<<grid, RBLOCK, 0, ctx->stream>>>(n, x, y, part);
-    check_launch();
-    k_reduce_pass2<OP><<<1, RBLOCK, 0, ctx->stream>>>(grid, part);
-    check_launch();
     double h = 0.0;
     PIRO_CHECK_HIP(hipMemcpyAsync(&h, part + RGRID_MAX, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -214,6 +224,16 @@ int piro_reduce_diff_sumsq(piro_ctx *ctx, int64_t n, const float *d_x, const flo
     ctx_of(ctx);
     PIRO_REQUIRE(h_out, "h_out is null");
     *h_out = run<DIFFSQ>(ctx, n, d_x, d_y);
+    PIRO_API_END
+}
+
+int piro_reduce_diff_sumsq_pair_async(piro_ctx *ctx, int64_t n, const float *d_x, const float *d_y, double *d_out)
+{
+    PIRO_API_BEGIN
+    ctx_of(ctx);
+    PIRO_REQUIRE(n > 0 && d_x && d_y && d_out, "bad arguments");
+    launch_reduce<DIFFSQ>(ctx, n, d_x, d_y, d_out);
+    launch_reduce<SUMSQ>(ctx, n, d_y, nullptr, d_out + 1);
     PIRO_API_END
 }
 

@@ -813,7 +813,8 @@ constexpr int STAGE = SIN * SPLANE;  // one part
 
 __global__ __launch_bounds__(256) void k_stencil13(Shape s, const float *__restrict__ u,
                                                    const float *__restrict__ al, const bf16 *__restrict__ frag,
-                                                   float inv_scale, float scale, float *__restrict__ out)
+                                                   float inv_scale, float scale, const float *__restrict__ add,
+                                                   float *__restrict__ out)
 {
     __shared__ __attribute__((aligned(16))) bf16 su[2 * STAGE];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -972,7 +973,8 @@ __global__ __launch_bounds__(256) void k_stencil13(Shape s, const float *__restr
                 const bool interior = zz > 0 && zz < s.D - 1 && yy > 0 && yy < s.H - 1 && xx > 0 && xx < s.W - 1;
                 const float a = ea[h][k2][r];
                 const float delta = o[0][r] + a * (o[1][r] + a * o[2][r]);
-                out[v] = interior ? eu[h][k2][r] + scale * delta : 0.0f;
+                const float y = interior ? eu[h][k2][r] + scale * delta : 0.0f;
+                out[v] = add ? y + add[v] : y;  // y rounded first: as a separate addition would
             }
         }
         __syncthreads();
@@ -1253,15 +1255,23 @@ int piro_surrogate_set_params(piro_surrogate *s, const float *h_params)
 int piro_surrogate_forward(piro_surrogate *s, int n, int D, int H, int W, const float *d_u, const float *d_alpha,
                            float scale, float *d_u_out)
 {
+    return piro_surrogate_forward_add(s, n, D, H, W, d_u, d_alpha, scale, nullptr, d_u_out);
+}
+
+int piro_surrogate_forward_add(piro_surrogate *s, int n, int D, int H, int W, const float *d_u, const float *d_alpha,
+                               float scale, const float *d_add, float *d_u_out)
+{
     PIRO_API_BEGIN
     sur(s);
     PIRO_REQUIRE(d_u && d_alpha && d_u_out && d_u != d_u_out, "bad buffers");
+    PIRO_REQUIRE(d_add != d_u_out, "the addend must not alias the output");
     PIRO_REQUIRE(scale > 0.0f && std::isfinite(scale), "scale must be positive and finite");
     Shape sh = mkshape(n, D, H, W);
     ensure_stencil(s);
     const int64_t nblk = (int64_t)n * ((D + SB - 1) / SB) * ((H + SB - 1) / SB) * ((W + SB - 1) / SB);
     PIRO_REQUIRE(nblk < (1LL << 31), "shape too large");
-    k_stencil13<<<(unsigned)nblk, 256, 0, s->ctx->stream>>>(sh, d_u, d_alpha, s->frag, 1.0f / scale, scale, d_u_out);
+    k_stencil13<<<(unsigned)nblk, 256, 0, s->ctx->stream>>>(sh, d_u, d_alpha, s->frag, 1.0f / scale, scale, d_add,
+                                                            d_u_out);
     check_launch();
     PIRO_API_END
 }

@@ -193,41 +193,48 @@ class HeatRun:
         x = self.u0.clone()
         y = ctx.empty_like(x)
         w = ctx.empty_like(x)
-        ref = ctx.empty_like(x)
         corr = ctx.empty_like(x)
         have_corr = False
         res = RunResult(0.0)
         calls = steps // k
+        n_checks = (calls + check_every - 1) // check_every
+        # each check's sum (S - P)^2 and sum P^2, kept on the device until the
+        # run ends: no read-back stalls the launch stream
+        sums = ctx.empty(max(2 * n_checks, 2), np.float64)
         ctx.sync()
         t0 = time.perf_counter()
         for c in range(calls):
             # the network is linear and bias-free and BF16 has FP32's exponent
-            # range, so no input scaling is needed (and no host round trip)
-            sur.forward(shape, x, self.alpha, 1.0, out=y)
-            if have_corr:
-                ctx.axpby(1.0, y, 1.0, corr, y)
+            # range, so no input scaling is needed (and no host round trip);
+            # the defect correction is added as the kernel writes its output
+            sur.forward(shape, x, self.alpha, 1.0, out=y, add=corr if have_corr else None)
             res.calls += 1
             if c % check_every == 0:
-                ref.copy_(x)
-                ctx.fe_run_csr(self.dims, self.A, self.dt, k, ref, w, self.snap)
-                err = ctx.rel_l2(y, ref)
+                # k plain steps from the same field, in place: the checked
+                # call takes the plain result, which x now holds
+                ctx.fe_run_csr(self.dims, self.A, self.dt, k, x, w, self.snap)
+                ctx.diff_sumsq_pair_async(y, x, sums, slot=res.checks)
                 res.checks += 1
-                res.check_errors.append(err)
-                if err > tol:
-                    res.rejected += 1
                 if correct:
-                    # defect of the uncorrected call: ref - (y - corr)
-                    ctx.axpby(1.0, ref, -1.0, y, w)
+                    # defect of the uncorrected call: P - (y - corr)
+                    ctx.axpby(1.0, x, -1.0, y, w)
                     if have_corr:
                         ctx.axpby(1.0, w, 1.0, corr, corr)
                     else:
                         corr.copy_(w)
                     have_corr = True
-                y.copy_(ref)
-            x, y = y, x
+            else:
+                x, y = y, x
         rem = steps - calls * k
         if rem:
             ctx.fe_run_csr(self.dims, self.A, self.dt, rem, x, w, self.snap)
-        ctx.sync()
+        hs = sums.host()
         res.seconds = time.perf_counter() - t0
+        for j in range(res.checks):
+            d, nb = float(hs[2 * j]), float(hs[2 * j + 1])
+            err = (d / nb) ** 0.5 if nb > 0 else d ** 0.5
+            res.check_errors.append(err)
+            if err > tol:
+                res.rejected += 1
+        sums.free()
         return x, res

@@ -457,3 +457,36 @@ def test_fallback_rejects_a_corrupted_surrogate(ctx):
     one, r1 = run.accelerated(sur, 16, k=16, check_every=1, tol=1e-3, correct=False)
     assert r1.rejected == 1 and one.host().tobytes() == x1.host().tobytes()
     sur.close()
+
+
+def test_forward_add_equals_forward_then_axpby(ctx):
+    # the fused addend of the accelerated run's defect correction: the same
+    # bits as the forward result followed by a separate addition, on
+    # interior, face and edge voxels, odd shapes and a batch of two
+    from piro_amd.core import Surrogate
+    sur = Surrogate(ctx, seed=6)
+    p = sur.get_params()
+    p *= np.float32(1.7)  # away from the untrained network
+    sur.set_params(p)
+    rng = np.random.default_rng(11)
+    for shape in ((1, 24, 24, 24), (2, 19, 21, 23), (1, 33, 17, 40)):
+        n = int(np.prod(shape))
+        u = ctx.from_host(rng.standard_normal(n).astype(np.float32))
+        al = ctx.from_host(rng.uniform(0.005, 0.02, n).astype(np.float32))
+        add = ctx.from_host((rng.standard_normal(n) * 1e-3).astype(np.float32))
+        ref = ctx.axpby(1.0, sur.forward(shape, u, al, 1.0), 1.0, add)
+        got = sur.forward(shape, u, al, 1.0, add=add)
+        assert got.host().tobytes() == ref.host().tobytes(), shape
+    sur.close()
+
+
+def test_diff_sumsq_pair_async_matches_the_host_reductions(ctx):
+    rng = np.random.default_rng(12)
+    out = ctx.empty(6, np.float64)
+    for slot, n in enumerate((5, 4099, 1 << 21)):
+        a = ctx.from_host(rng.standard_normal(n).astype(np.float32))
+        b = ctx.from_host(rng.standard_normal(n).astype(np.float32))
+        ctx.diff_sumsq_pair_async(a, b, out, slot=slot)
+        h = out.host()
+        assert h[2 * slot] == ctx.reduce("diffsq", a, b)
+        assert h[2 * slot + 1] == ctx.reduce("sumsq", b)
