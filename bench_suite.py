@@ -421,7 +421,7 @@ def bench_formats(args):
     keys, vals = ctx.ht_build(Ah, ts)
     t = timed(ctx, lambda: check(lib.piro_ht_build(ctx.h, Nh, _p(Ah.row_ptr), _p(Ah.col), _p(Ah.val), int(ts),
                                                    _p(keys), _p(vals))), reps=5)
-    row(f"HT build 26^3 (table {ts} slots)", t, 0, "launch-bound at this size")
+    row(f"HT build 26^3 (table {ts} slots)", t, 8 * ts, "clearing the table (keys -1, values 0) is most of it; the parallel insert ~70 us")
     Ahc = ctx.ht_to_csr(Nh, keys, vals)
     xh = ctx.ic_gaussian(hd, [1.0] * 3, [0.5] * 3, [0.1] * 3, 1.0)
     yh = ctx.empty_like(xh)
