@@ -491,7 +491,8 @@ template <bool VEC>
 __global__ __launch_bounds__(64) void k_panel_spec(float *__restrict__ a, int lda, int n, int k, int jb, int c1,
                                                    float *__restrict__ top_out, float *__restrict__ bk12,
                                                    float *__restrict__ bk, int *__restrict__ flag,
-                                                   int *__restrict__ zc, int32_t *__restrict__ ipiv)
+                                                   int *__restrict__ zc, int32_t *__restrict__ ipiv,
+                                                   int *__restrict__ info_now)
 {
     const int lane = threadIdx.x, m = n - k;
     const int i = blockIdx.x * 64 + lane;  // panel row k + i
@@ -528,6 +529,9 @@ __global__ __launch_bounds__(64) void k_panel_spec(float *__restrict__ a, int ld
     if (blockIdx.x == 0) {
         if (lane < jb) ipiv[k + lane] = k + lane;
         if (lane == 0) *zc = zero_col + 1;
+        // no finishing launch follows: the first exactly zero pivot goes to
+        // info now (panels come in order, so the first one set stays)
+        if (info_now && lane == 0 && zero_col >= 0 && *info_now == 0) *info_now = k + zero_col + 1;
         if (follow) {
             if (lane < nc)
 #pragma unroll
@@ -786,8 +790,22 @@ constexpr int TXP = 17;  // pitch of a wave's 64 x 16 block in LDS
 // applies them to its own 64 columns before reading them, and the
 // workgroups past the trailing columns' apply them to the finished columns
 // on the left, [0, kb) (the block's own columns took them from the panels).
+__device__ __forceinline__ void take_tops(float *__restrict__ a, int lda, int kb, int nbk,
+                                          const float *__restrict__ tops, int t)
+{
+    for (int e = t; e < (NB / PB) * PB * PB; e += 256) {
+        const int p = e / (PB * PB), r = (e / PB) % PB, c = e % PB;
+        const int k = kb + PB * p, jb = min(PB, nbk - PB * p);
+        if (r < jb && c < jb) a[(int64_t)(k + r) * lda + k + c] = tops[e];
+    }
+}
+
+// tops (the end-to-end speculative pass; NULL otherwise): the block's
+// panels' top rows, slot p for panel p, not yet in the matrix.  L11's
+// diagonal 16 x 16 blocks are read from them and workgroup 0 copies them in
+// (no other workgroup reads those entries).
 __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int lda, int kb, int nbk, int c0, int c1,
-                                                    const int32_t *__restrict__ ipiv)
+                                                    const int32_t *__restrict__ ipiv, const float *__restrict__ tops)
 {
     __shared__ __attribute__((aligned(16))) float s_nl[TW][TW + 4];  // -L11, strictly lower
     __shared__ float s_x[4][TW * TXP];
@@ -809,8 +827,10 @@ __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int l
 #pragma unroll
     for (int it = 0; it < TW * TW / 256; ++it) {
         const int e = t + 256 * it, i = e / TW, j = e % TW;
-        ld[it] = a[(int64_t)(kb + min(i, nbk - 1)) * lda + kb + min(j, nbk - 1)];
+        ld[it] = (tops && i / PB == j / PB) ? tops[(i / PB) * PB * PB + (i % PB) * PB + j % PB]
+                                            : a[(int64_t)(kb + min(i, nbk - 1)) * lda + kb + min(j, nbk - 1)];
     }
+    if (tops && blockIdx.x == 0) take_tops(a, lda, kb, nbk, tops, t);
     if (t < 64 && c0 + (int)blockIdx.x * 64 + t < c1) swap_columns(a, lda, kb, nbk, s_p, c0 + blockIdx.x * 64 + t);
     __syncthreads();  // the exchanged rows are read below by other lanes
     // this wave's columns cw .. cw + 15; lane moves column cw + (lane & 15)
@@ -870,6 +890,14 @@ __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int l
         const int i = lr + 4 * q;
         if (colin && i < nbk) a[(int64_t)(kb + i) * lda + cw + lc] = x[i * TXP + lc];
     }
+}
+
+// the last block's panels' top rows into the matrix (the end-to-end
+// speculative pass; there is no block TRSM after the last block)
+__global__ __launch_bounds__(256) void k_take_tops(float *__restrict__ a, int lda, int kb, int nbk,
+                                                   const float *__restrict__ tops)
+{
+    take_tops(a, lda, kb, nbk, tops, threadIdx.x);
 }
 
 // ------------------------------------------------- MFMA trailing update
@@ -1258,7 +1286,8 @@ __global__ __launch_bounds__(256) void k_update_mfma32(float *__restrict__ a, in
 __global__ __launch_bounds__(64) void k_update16_spec(float *__restrict__ a, int lda, int n, int k, int c1,
                                                       float *__restrict__ top_out, float *__restrict__ bk12,
                                                       float *__restrict__ bk, int *__restrict__ flag,
-                                                      int *__restrict__ zc, int32_t *__restrict__ ipiv)
+                                                      int *__restrict__ zc, int32_t *__restrict__ ipiv,
+                                                      int *__restrict__ info_now)
 {
     using S = UpdShape<PB, 1, 1>;
     __shared__ __attribute__((aligned(16))) float s_a[S::TR * S::PA];  // -L21
@@ -1324,6 +1353,7 @@ __global__ __launch_bounds__(64) void k_update16_spec(float *__restrict__ a, int
     if (owner) {
         if (lane < PB) ipiv[c0 + lane] = c0 + lane;
         if (lane == 0) *zc = zero_col + 1;
+        if (info_now && lane == 0 && zero_col >= 0 && *info_now == 0) *info_now = c0 + zero_col + 1;
         const int nc = c1 - c0 - PB;  // 0, 16 or 32
         if (nc > 0) {
             float col[PB];
@@ -1874,7 +1904,9 @@ struct FactorWs {
     void *base;
     int *info, *flag, *zc;
     float *top, *bk12, *bk;
-    static constexpr size_t HEAD = 512, TOP = PB * PB * 4, BK12 = PB * FW * 4;
+    // top: one PB x PB slot per panel of a block (the end-to-end pass keeps
+    // a block's four; the per-panel pass uses slot 0)
+    static constexpr size_t HEAD = 512, TOP = (NB / PB) * PB * PB * 4, BK12 = PB * FW * 4;
     static size_t bytes(int n) { return HEAD + TOP + BK12 + (size_t)n * PB * 4; }
     explicit FactorWs(void *b_) : base(b_)
     {
@@ -1892,7 +1924,7 @@ struct FactorWs {
 // [kb, k) and, unless it is the global-memory panel (returns false), its
 // exchanges and U12 to the block's later columns [k + jb, c1)
 bool panel(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int kb, int k, int jb, int c1, int32_t *ipiv,
-           const FactorWs &ws, bool pre)
+           const FactorWs &ws, bool pre, bool spec)
 {
     const int m = n - k;
     if (m > 4096) {
@@ -1905,11 +1937,14 @@ bool panel(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int kb, int k
         return false;
     }
     const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0) && jb == PB;
-    const bool spec = ctx->lu_speculate != 0;
     if (spec && !pre) {  // pre: the speculative pass ran in the previous in-block update
         const int g = (m + 63) / 64;
-        if (vec) k_panel_spec<true><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, ws.top, ws.bk12, ws.bk, ws.flag, ws.zc, ipiv);
-        else k_panel_spec<false><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, ws.top, ws.bk12, ws.bk, ws.flag, ws.zc, ipiv);
+        if (vec)
+            k_panel_spec<true><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, ws.top, ws.bk12, ws.bk, ws.flag, ws.zc, ipiv,
+                                                nullptr);
+        else
+            k_panel_spec<false><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, ws.top, ws.bk12, ws.bk, ws.flag, ws.zc, ipiv,
+                                                 nullptr);
         check_launch();
     }
     int *flag = spec ? ws.flag : nullptr;
@@ -1935,7 +1970,9 @@ bool panel(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int kb, int k
     return true;
 }
 
-void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const FactorWs &ws)
+// panel by panel: with spec, each panel is first tried without exchanges
+// and a launch after it takes the result or runs the pivoting steps
+void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const FactorWs &ws, bool spec)
 {
     k_zero_i32<<<1, 256, 0, s>>>(ws.info, (int)(FactorWs::HEAD / 4));
     check_launch();
@@ -1947,7 +1984,7 @@ void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipi
         for (int k = kb; k < kb + nbk; k += PB) {
             const int jb = min(PB, kb + nbk - k);
             const int c0 = k + jb, c1 = kb + nbk;
-            const bool followed = panel(ctx, s, n, a, lda, kb, k, jb, c1, ipiv, ws, pre);
+            const bool followed = panel(ctx, s, n, a, lda, kb, k, jb, c1, ipiv, ws, pre, spec);
             pre = false;
             if (c0 < c1) {
                 if (!followed) {
@@ -1958,8 +1995,9 @@ void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipi
                 // panel's speculative pass when it is a full register panel
                 dim3 g((c1 - c0 + 63) / 64, (n - c0 + 63) / 64);
                 const bool vu = vec && k % 4 == 0 && c0 % 4 == 0;
-                if (ctx->lu_speculate && vu && jb == PB && c0 + PB <= c1 && n - c0 <= 4096) {
-                    k_update16_spec<<<g, 64, 0, s>>>(a, lda, n, k, c1, ws.top, ws.bk12, ws.bk, ws.flag, ws.zc, ipiv);
+                if (spec && vu && jb == PB && c0 + PB <= c1 && n - c0 <= 4096) {
+                    k_update16_spec<<<g, 64, 0, s>>>(a, lda, n, k, c1, ws.top, ws.bk12, ws.bk, ws.flag, ws.zc, ipiv,
+                                                     nullptr);
                     pre = true;
                 } else {
                     k_update_mfma<PB, 1, 1><<<g, 64, 0, s>>>(a, lda, n, k, jb, c0, c0, c1, vu);
@@ -1976,7 +2014,7 @@ void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipi
         if (c0 < n) {
             static_assert(NB == TW, "block TRSM rows");
             const int gt = (n - c0 + 63) / 64 + (kb + 255) / 256;
-            k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv);
+            k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, nullptr);
             check_launch();
             const bool vu = vec && kb % 4 == 0 && c0 % 4 == 0;
             if (n - c0 <= kUpd32Max) {  // quarter wave tiles: four times the waves
@@ -1989,6 +2027,74 @@ void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipi
             check_launch();
         } else if (kb > 0) {
             k_laswp<<<grid_for(kb, 256), 256, 0, s>>>(a, lda, kb, nbk, ipiv, 0, kb);
+            check_launch();
+        }
+    }
+}
+
+// The end-to-end speculative pass (n <= 4096, every panel a register
+// panel): every panel factored without exchanges, with no launch after it.
+// A panel's top rows stay in its own slot until the block TRSM (or, after
+// the last block, k_take_tops) copies them in; no launch before that reads
+// them from the matrix.  The first exactly zero pivot goes to info as the
+// panels come.  The flag, raised by any row that would have won a pivot
+// search, is looked at once, by the caller, after the pass: raised, the
+// caller restores the matrix and runs the pivoting steps; clear, the result
+// is what they would have produced, bit for bit (the same kernels on the
+// same operands as the per-panel pass, whose factors are the pivoting
+// steps').
+void factor_spec(hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const FactorWs &ws)
+{
+    k_zero_i32<<<1, 256, 0, s>>>(ws.info, (int)(FactorWs::HEAD / 4));
+    check_launch();
+    const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0);
+    for (int kb = 0; kb < n; kb += NB) {
+        const int nbk = min(NB, n - kb);
+        bool pre = false;  // the panel's pass ran in the previous in-block update
+        for (int p = 0, k = kb; k < kb + nbk; ++p, k += PB) {
+            const int jb = min(PB, kb + nbk - k);
+            const int c0 = k + jb, c1 = kb + nbk;
+            float *slot = ws.top + (size_t)p * PB * PB;
+            if (!pre) {
+                const int g = (n - k + 63) / 64;
+                if (vec && jb == PB)
+                    k_panel_spec<true><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, slot, ws.bk12, ws.bk, ws.flag, ws.zc,
+                                                        ipiv, ws.info);
+                else
+                    k_panel_spec<false><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, slot, ws.bk12, ws.bk, ws.flag, ws.zc,
+                                                         ipiv, ws.info);
+                check_launch();
+            }
+            pre = false;
+            if (c0 < c1) {
+                dim3 g((c1 - c0 + 63) / 64, (n - c0 + 63) / 64);
+                const bool vu = vec && k % 4 == 0 && c0 % 4 == 0;
+                if (vu && jb == PB && c0 + PB <= c1) {
+                    k_update16_spec<<<g, 64, 0, s>>>(a, lda, n, k, c1, slot + PB * PB, ws.bk12, ws.bk, ws.flag, ws.zc,
+                                                     ipiv, ws.info);
+                    pre = true;
+                } else {
+                    k_update_mfma<PB, 1, 1><<<g, 64, 0, s>>>(a, lda, n, k, jb, c0, c0, c1, vu);
+                }
+                check_launch();
+            }
+        }
+        const int c0 = kb + nbk;
+        if (c0 < n) {
+            const int gt = (n - c0 + 63) / 64 + (kb + 255) / 256;
+            k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, ws.top);
+            check_launch();
+            const bool vu = vec && kb % 4 == 0 && c0 % 4 == 0;
+            if (n - c0 <= kUpd32Max) {
+                dim3 g((n - c0 + 63) / 64, (n - c0 + 63) / 64);
+                k_update_mfma32<NB><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu);
+            } else {
+                dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
+                k_update_mfma<NB, 2, 2><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu);
+            }
+            check_launch();
+        } else {
+            k_take_tops<<<1, 256, 0, s>>>(a, lda, kb, nbk, ws.top);
             check_launch();
         }
     }
@@ -2018,10 +2124,30 @@ int piro_lu_factor_f32(piro_ctx *ctx, int n, float *d_a, int lda, int32_t *d_ipi
     PIRO_REQUIRE(n < (1 << 27), "n too large (row labels are packed in 27 bits)");
     const FactorWs ws(ctx->scratch[0].get(FactorWs::bytes(n)));
     hipStream_t s = ctx->stream;
-    factor(ctx, s, n, d_a, lda, d_ipiv, ws);
     int h = 0;
-    PIRO_CHECK_HIP(hipMemcpyAsync(&h, ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
-    PIRO_CHECK_HIP(hipStreamSynchronize(s));
+    if (ctx->lu_speculate && n <= 4096) {
+        // the whole factorisation speculatively, the matrix kept aside in
+        // case a row would have won a pivot search
+        float *keep = (float *)ctx->lu_keep.get((size_t)n * n * sizeof(float));
+        PIRO_CHECK_HIP(hipMemcpy2DAsync(keep, (size_t)n * sizeof(float), d_a, (size_t)lda * sizeof(float),
+                                        (size_t)n * sizeof(float), n, hipMemcpyDeviceToDevice, s));
+        factor_spec(s, n, d_a, lda, d_ipiv, ws);
+        int head[33];  // info, then the flag at int 32
+        PIRO_CHECK_HIP(hipMemcpyAsync(head, ws.info, sizeof(head), hipMemcpyDeviceToHost, s));
+        PIRO_CHECK_HIP(hipStreamSynchronize(s));
+        h = head[0];
+        if (head[32]) {  // raised: restore and factor by the pivoting steps
+            PIRO_CHECK_HIP(hipMemcpy2DAsync(d_a, (size_t)lda * sizeof(float), keep, (size_t)n * sizeof(float),
+                                            (size_t)n * sizeof(float), n, hipMemcpyDeviceToDevice, s));
+            factor(ctx, s, n, d_a, lda, d_ipiv, ws, false);
+            PIRO_CHECK_HIP(hipMemcpyAsync(&h, ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
+            PIRO_CHECK_HIP(hipStreamSynchronize(s));
+        }
+    } else {
+        factor(ctx, s, n, d_a, lda, d_ipiv, ws, ctx->lu_speculate != 0);
+        PIRO_CHECK_HIP(hipMemcpyAsync(&h, ws.info, sizeof(int), hipMemcpyDeviceToHost, s));
+        PIRO_CHECK_HIP(hipStreamSynchronize(s));
+    }
     if (info) *info = h;
     if (h) throw Error(PIRO_ESINGULAR, "exactly zero pivot at step " + std::to_string(h));
     PIRO_API_END

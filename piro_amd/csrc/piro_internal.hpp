@@ -85,6 +85,7 @@ struct piro_ctx {
     int num_cus = 256;
     int num_xcds = 8;
     piro::Scratch scratch[4];
+    piro::Scratch lu_keep;  // the LU's copy of the matrix while it speculates
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // piro_timer_start/stop
     // autotuned FE-step choice keyed by the last matrix shape
     int fe_variant = -1;

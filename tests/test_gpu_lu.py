@@ -284,3 +284,35 @@ def test_refine_rejects_non_finite_residual(ctx):
     b = bump_rhs(64, 1)
     with pytest.raises(PiroError, match="non-finite"):
         ctx.lu_solve_refine(dense, dense.clone(), n, ipiv, dev(b, ctx), tol=1e-10, max_sweeps=3)
+
+
+@pytest.mark.parametrize("spec", [True, False])
+@pytest.mark.parametrize("n,zcols", [(700, (300,)), (700, (699,)), (4096, (1500, 2047)), (130, (5, 64))])
+def test_lu_exactly_zero_column_reports_the_first_step(ctx, spec, n, zcols):
+    # a diagonally dominant matrix with whole columns zero: no row ever wins
+    # a pivot search, so the end-to-end speculative pass runs to the end and
+    # must report the first exactly zero pivot (1-based, LAPACK's info) as
+    # the pivoting steps do, with the same factors
+    from piro_amd._lib import PiroError
+    print(f"case zero column n={n} cols={zcols} spec={spec}", flush=True)
+    rng = np.random.default_rng(n + sum(zcols))
+    a = rng.uniform(-1, 1, (n, n)).astype(np.float32)
+    a[np.arange(n), np.arange(n)] = np.float32(2 * n)
+    for c in zcols:
+        a[:, c] = 0.0
+    _, _, info64 = po.lu_factor_f64(a.astype(np.float64))
+    assert info64 == min(zcols) + 1
+    ctx.lu_set_speculation(spec)
+    try:
+        d = dev(a, ctx)
+        with pytest.raises(PiroError) as e:
+            ctx.lu_factor(d, n)
+        assert f"step {min(zcols) + 1}" in str(e.value)
+        got = d.host()
+        ctx.lu_set_speculation(not spec)
+        d2 = dev(a, ctx)
+        with pytest.raises(PiroError):
+            ctx.lu_factor(d2, n)
+        assert got.tobytes() == d2.host().tobytes()
+    finally:
+        ctx.lu_set_speculation(True)
