@@ -510,7 +510,7 @@ __global__ __launch_bounds__(64) void k_panel_spec(float *__restrict__ a, int ld
 #pragma unroll
         for (int q = 0; q < PB; ++q) col[q] = a[(int64_t)(k + min(q, jb - 1)) * lda + c0 + min(lane, nc - 1)];
     }
-    if (own) put_row(bk + (int64_t)i * PB, x);
+    if (own && bk) put_row(bk + (int64_t)i * PB, x);  // bk, bk12: NULL when the whole matrix is kept
     uint32_t viol = 0;
     int zero_col = -1;
     SpecLoop<0>::run(top, x, lane, i, own, jb, viol, zero_col);
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(64) void k_panel_spec(float *__restrict__ a, int ld
         // info now (panels come in order, so the first one set stays)
         if (info_now && lane == 0 && zero_col >= 0 && *info_now == 0) *info_now = k + zero_col + 1;
         if (follow) {
-            if (lane < nc)
+            if (bk12 && lane < nc)
 #pragma unroll
                 for (int q = 0; q < PB; ++q)
                     if (q < jb) bk12[q * FW + lane] = col[q];
@@ -812,8 +812,13 @@ __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int l
     __shared__ int s_p[NB];
     const int t = threadIdx.x, lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
-    if (t < nbk) s_p[t] = ipiv[kb + t];
-    __syncthreads();
+    // the end-to-end pass exchanges no rows: no pivots to read or apply, and
+    // no workgroups for the left columns
+    const bool ident = tops != nullptr;
+    if (!ident) {
+        if (t < nbk) s_p[t] = ipiv[kb + t];
+        __syncthreads();
+    }
     const int ntrsm = (c1 - c0 + 63) / 64;
     if ((int)blockIdx.x >= ntrsm) {  // uniform: a left-column workgroup
         const int c = ((int)blockIdx.x - ntrsm) * 256 + t;
@@ -831,8 +836,10 @@ __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int l
                                             : a[(int64_t)(kb + min(i, nbk - 1)) * lda + kb + min(j, nbk - 1)];
     }
     if (tops && blockIdx.x == 0) take_tops(a, lda, kb, nbk, tops, t);
-    if (t < 64 && c0 + (int)blockIdx.x * 64 + t < c1) swap_columns(a, lda, kb, nbk, s_p, c0 + blockIdx.x * 64 + t);
-    __syncthreads();  // the exchanged rows are read below by other lanes
+    if (!ident) {
+        if (t < 64 && c0 + (int)blockIdx.x * 64 + t < c1) swap_columns(a, lda, kb, nbk, s_p, c0 + blockIdx.x * 64 + t);
+        __syncthreads();  // the exchanged rows are read below by other lanes
+    }
     // this wave's columns cw .. cw + 15; lane moves column cw + (lane & 15)
     // of rows (lane >> 4) + 4 q
     const int cw = c0 + blockIdx.x * 64 + w * 16, lc = lane & 15, lr = lane >> 4;
@@ -1336,7 +1343,7 @@ __global__ __launch_bounds__(64) void k_update16_spec(float *__restrict__ a, int
     }
     const int i = r0 - c0 + lane;  // the next panel's row c0 + i
     const bool own = r0 + lane < n;
-    if (own) put_row(bk + (int64_t)i * PB, x);
+    if (own && bk) put_row(bk + (int64_t)i * PB, x);  // bk, bk12: NULL when the whole matrix is kept
     uint32_t viol = 0;
     int zero_col = -1;
     SpecLoop<0>::run(top, x, lane, i, own, PB, viol, zero_col);
@@ -1359,7 +1366,7 @@ __global__ __launch_bounds__(64) void k_update16_spec(float *__restrict__ a, int
             float col[PB];
 #pragma unroll
             for (int q = 0; q < PB; ++q) col[q] = s_fu[q][min(lane, nc - 1)];
-            if (lane < nc)
+            if (bk12 && lane < nc)
 #pragma unroll
                 for (int q = 0; q < PB; ++q) bk12[q * FW + lane] = col[q];
             // as the finishing launch's follow-up forms it: each row's terms
@@ -2058,10 +2065,10 @@ void factor_spec(hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const F
             if (!pre) {
                 const int g = (n - k + 63) / 64;
                 if (vec && jb == PB)
-                    k_panel_spec<true><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, slot, ws.bk12, ws.bk, ws.flag, ws.zc,
+                    k_panel_spec<true><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, slot, nullptr, nullptr, ws.flag, ws.zc,
                                                         ipiv, ws.info);
                 else
-                    k_panel_spec<false><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, slot, ws.bk12, ws.bk, ws.flag, ws.zc,
+                    k_panel_spec<false><<<g, 64, 0, s>>>(a, lda, n, k, jb, c1, slot, nullptr, nullptr, ws.flag, ws.zc,
                                                          ipiv, ws.info);
                 check_launch();
             }
@@ -2070,7 +2077,7 @@ void factor_spec(hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const F
                 dim3 g((c1 - c0 + 63) / 64, (n - c0 + 63) / 64);
                 const bool vu = vec && k % 4 == 0 && c0 % 4 == 0;
                 if (vu && jb == PB && c0 + PB <= c1) {
-                    k_update16_spec<<<g, 64, 0, s>>>(a, lda, n, k, c1, slot + PB * PB, ws.bk12, ws.bk, ws.flag, ws.zc,
+                    k_update16_spec<<<g, 64, 0, s>>>(a, lda, n, k, c1, slot + PB * PB, nullptr, nullptr, ws.flag, ws.zc,
                                                      ipiv, ws.info);
                     pre = true;
                 } else {
@@ -2081,7 +2088,7 @@ void factor_spec(hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const F
         }
         const int c0 = kb + nbk;
         if (c0 < n) {
-            const int gt = (n - c0 + 63) / 64 + (kb + 255) / 256;
+            const int gt = (n - c0 + 63) / 64;  // no exchanges: no left-column workgroups
             k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, ws.top);
             check_launch();
             const bool vu = vec && kb % 4 == 0 && c0 % 4 == 0;
