@@ -804,8 +804,12 @@ __device__ __forceinline__ void take_tops(float *__restrict__ a, int lda, int kb
 // panels' top rows, slot p for panel p, not yet in the matrix.  L11's
 // diagonal 16 x 16 blocks are read from them and workgroup 0 copies them in
 // (no other workgroup reads those entries).
+// blk (the one-launch block kernel; NULL otherwise): the whole diagonal
+// block, row-major 64 x 64, not yet in the matrix: L11 is read from it and
+// workgroup 0 copies it in.
 __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int lda, int kb, int nbk, int c0, int c1,
-                                                    const int32_t *__restrict__ ipiv, const float *__restrict__ tops)
+                                                    const int32_t *__restrict__ ipiv, const float *__restrict__ tops,
+                                                    const float *__restrict__ blk)
 {
     __shared__ __attribute__((aligned(16))) float s_nl[TW][TW + 4];  // -L11, strictly lower
     __shared__ float s_x[4][TW * TXP];
@@ -814,7 +818,7 @@ __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int l
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     // the end-to-end pass exchanges no rows: no pivots to read or apply, and
     // no workgroups for the left columns
-    const bool ident = tops != nullptr;
+    const bool ident = tops != nullptr || blk != nullptr;
     if (!ident) {
         if (t < nbk) s_p[t] = ipiv[kb + t];
         __syncthreads();
@@ -832,10 +836,13 @@ __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int l
 #pragma unroll
     for (int it = 0; it < TW * TW / 256; ++it) {
         const int e = t + 256 * it, i = e / TW, j = e % TW;
-        ld[it] = (tops && i / PB == j / PB) ? tops[(i / PB) * PB * PB + (i % PB) * PB + j % PB]
+        ld[it] = blk ? blk[i * NB + j]
+               : (tops && i / PB == j / PB) ? tops[(i / PB) * PB * PB + (i % PB) * PB + j % PB]
                                             : a[(int64_t)(kb + min(i, nbk - 1)) * lda + kb + min(j, nbk - 1)];
     }
     if (tops && blockIdx.x == 0) take_tops(a, lda, kb, nbk, tops, t);
+    if (blk && blockIdx.x == 0)
+        for (int e = t; e < NB * NB; e += 256) a[(int64_t)(kb + e / NB) * lda + kb + e % NB] = blk[e];
     if (!ident) {
         if (t < 64 && c0 + (int)blockIdx.x * 64 + t < c1) swap_columns(a, lda, kb, nbk, s_p, c0 + blockIdx.x * 64 + t);
         __syncthreads();  // the exchanged rows are read below by other lanes
@@ -1386,6 +1393,263 @@ __global__ __launch_bounds__(64) void k_update16_spec(float *__restrict__ a, int
     }
 }
 
+// ------------------------------------------ one launch per full block
+//
+// The end-to-end speculative pass's in-block work for a full 64-column
+// block (all four panels) in one launch, instead of k_panel_spec and three
+// k_update16_spec launches.  One wave per 64 rows from the block's first
+// row; each holds its rows' 64 block columns in MFMA accumulators
+// (update_load_c's layout) from start to end.  Every wave also holds the
+// block's 64 x 64 diagonal block (the owner's rows, wave 0) and carries it
+// through the same steps the owner takes -- the top block factored in every
+// lane set (SpecLoop), U12 = L11^-1 A12 in the owner's order, the top rows'
+// multipliers, the K = 16 MFMA updates -- so each wave has each panel's
+// pivot rows and U12 without waiting for another.  The owner writes the
+// diagonal block to `blk` (64 x 64, row-major), not to the matrix, which
+// other waves read at their start; the block TRSM (or, after the last
+// block, k_take_blk) copies it in.  The other waves write their rows' block
+// columns in place.  Per element, every operation is the per-panel
+// kernels' (the same device functions on the same values, in the same
+// order), so the results are theirs bit for bit.
+
+// One step J of a panel in the one-launch block kernel, spec_step's
+// arithmetic as straight-line code: the multiplier is q[J] * (1 / d) with
+// the reciprocal formed once, and no lane masks where no later read needs
+// one.  A pivot that is zero, subnormal, infinite or NaN (where the
+// pivoting steps' multiplier takes another form, or a zero pivot skips the
+// step) raises the flag like a row that would have won a pivot search, so
+// the pivoting steps factor such a matrix.  `top` is updated in every lane:
+// lane J' is read at step J' before that step writes it, and afterwards only
+// for its multipliers (columns below J'), which no later step writes.
+// MASKX: this wave holds the panel's top rows (the owner), whose lanes at
+// or above the step keep their values; in the other waves every own row is
+// below the panel, and rows past n are never written.  XT: the wave's copy
+// of the diagonal block's rows, whose lanes above the panel's top rows are
+// discarded (staged as zeros).
+template <int J, bool MASKX, bool XT>
+__device__ __forceinline__ void block_step(float (&top)[PB], float (&x)[PB], float (&xt)[PB], bool below,
+                                           uint32_t &viol)
+{
+    float u[PB];
+#pragma unroll
+    for (int c = J; c < PB; ++c) u[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(top[c]), J));
+    const float d = u[J];
+    const uint32_t db = (uint32_t)__float_as_int(d) & 0x7fffffffu;
+    viol |= (below && ((uint32_t)__float_as_int(x[J]) & 0x7fffffffu) > db) ? 1u : 0u;
+    viol |= (db < 0x00800000u || db >= 0x7f800000u) ? 1u : 0u;  // uniform
+    const float rd = 1.0f / d;
+    {
+        const float l = top[J] * rd;
+        top[J] = l;
+#pragma unroll
+        for (int c = J + 1; c < PB; ++c) top[c] = __builtin_fmaf(-l, u[c], top[c]);
+    }
+    {
+        const float l = x[J] * rd;
+        if constexpr (MASKX) {
+            x[J] = below ? l : x[J];
+#pragma unroll
+            for (int c = J + 1; c < PB; ++c) x[c] = below ? __builtin_fmaf(-l, u[c], x[c]) : x[c];
+        } else {
+            x[J] = l;
+#pragma unroll
+            for (int c = J + 1; c < PB; ++c) x[c] = __builtin_fmaf(-l, u[c], x[c]);
+        }
+    }
+    if constexpr (XT) {
+        const float l = xt[J] * rd;
+        xt[J] = l;
+#pragma unroll
+        for (int c = J + 1; c < PB; ++c) xt[c] = __builtin_fmaf(-l, u[c], xt[c]);
+    }
+}
+
+template <int J, bool MASKX, bool XT>
+struct BlockSteps {
+    static __device__ __forceinline__ void run(float (&top)[PB], float (&x)[PB], float (&xt)[PB], int i, bool own,
+                                               uint32_t &viol)
+    {
+        block_step<J, MASKX, XT>(top, x, xt, own && i > J, viol);
+        BlockSteps<J + 1, MASKX, XT>::run(top, x, xt, i, own, viol);
+    }
+};
+
+template <bool MASKX, bool XT>
+struct BlockSteps<PB, MASKX, XT> {
+    static __device__ __forceinline__ void run(float (&)[PB], float (&)[PB], float (&)[PB], int, bool, uint32_t &) {}
+};
+
+constexpr int BRP = PB + 4;  // row pitch of the block kernel's row tiles (16-byte rows)
+
+struct __attribute__((aligned(16))) BlockLds {
+    float rows[64][BRP];      // a tile column as rows (own rows)
+    float rowt[64][BRP];      // the same for the diagonal block's rows
+    float top[PB][BRP];       // the panel's top block
+    float fu[PB][FW + 4];     // the top rows' later columns (A12)
+    float l[PB][PB];          // L11 by columns: l[j][q] = lane q's multiplier j
+    float a[64 * (PB + 4)];   // -L21, own rows
+    float at[64 * (PB + 4)];  // -L21, the diagonal block's rows
+    float b[PB * (64 + 16)];  // U12
+};
+
+__device__ __forceinline__ void row_from_lds(float (&v)[PB], const float *src)
+{
+#pragma unroll
+    for (int q = 0; q < PB / 4; ++q) {
+        const float4v w = *(const float4v *)(src + 4 * q);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[4 * q + r] = w[r];
+    }
+}
+
+// one panel P of the block; OWNER: this wave's rows are the diagonal
+// block's (acc), otherwise acct carries the diagonal block
+template <int P, bool OWNER>
+__device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int n, int kb, int r0, float *__restrict__ blk,
+                                            int *__restrict__ flag, int32_t *__restrict__ ipiv,
+                                            float4v (&acc)[4][4], float4v (&acct)[4][4],
+                                            BlockLds &sh, int lane, int li, int lk)
+{
+    constexpr int PA = PB + 4, PBP = 64 + 16, NC = NB - PB * (P + 1);  // later columns
+    const int k = kb + PB * P;
+    float4v (&dg)[4][4] = OWNER ? acc : acct;  // the diagonal block (compile-time choice)
+    // tile columns to rows through LDS
+    *(float4v *)&sh.top[li][4 * lk] = dg[P][P];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) *(float4v *)&sh.rows[16 * ti + li][4 * lk] = acc[ti][P];
+    if constexpr (!OWNER) {
+#pragma unroll
+        for (int ti = P + 1; ti < 4; ++ti) *(float4v *)&sh.rowt[16 * ti + li][4 * lk] = acct[ti][P];
+    }
+#pragma unroll
+    for (int tj = P + 1; tj < 4; ++tj) *(float4v *)&sh.fu[li][16 * (tj - P - 1) + 4 * lk] = dg[P][tj];
+    __syncthreads();
+    float top[PB], x[PB], xt[PB];
+    row_from_lds(top, &sh.top[min(lane, PB - 1)][0]);
+    row_from_lds(x, &sh.rows[lane][0]);
+    if constexpr (!OWNER) row_from_lds(xt, &sh.rowt[lane][0]);
+    // this wave's rows: panel row i = r0 - k + lane (the owner's rows above
+    // the panel have i < 0 and take no step)
+    const int i = r0 - k + lane;
+    const bool own = r0 + lane < n;
+    uint32_t viol = 0;
+    BlockSteps<0, OWNER, !OWNER>::run(top, x, xt, i, own, viol);
+    if (__ballot(viol != 0) != 0 && lane == 0) __hip_atomic_fetch_or(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if constexpr (OWNER) {
+        if (lane < PB) ipiv[k + lane] = k + lane;
+        if (lane >= PB * P) put_row(blk + lane * NB + PB * P, x);
+    } else {
+        if (own) put_row(a + (int64_t)(r0 + lane) * lda + k, x);
+    }
+    if constexpr (NC > 0) {
+        // the K = 16 update of the later columns, staged as update_stage
+        // stages it (-L21 by rows, U12 by rows; zeros past the matrix)
+        const bool arow = OWNER ? lane >= PB * (P + 1) : own;
+#pragma unroll
+        for (int c = 0; c < PB; ++c) sh.a[lane * PA + c] = arow ? -x[c] : 0.0f;
+        if constexpr (!OWNER) {
+            const bool trow = lane >= PB * (P + 1);
+#pragma unroll
+            for (int c = 0; c < PB; ++c) sh.at[lane * PA + c] = trow ? -xt[c] : 0.0f;
+        }
+        if (lane < PB)
+#pragma unroll
+            for (int j = 0; j < PB; ++j) sh.l[j][lane] = top[j];
+        __syncthreads();
+        // U12 = L11^-1 A12 as the per-panel kernels' owner forms it: each
+        // row's terms in ascending column order, unfused
+        float col[PB];
+#pragma unroll
+        for (int q = 0; q < PB; ++q) col[q] = sh.fu[q][min(lane, NC - 1)];
+#pragma unroll
+        for (int j = 0; j < PB - 1; ++j) {
+            float lj[PB];
+            row_from_lds(lj, &sh.l[j][0]);
+#pragma unroll
+            for (int q = j + 1; q < PB; ++q) col[q] = col[q] - lj[q] * col[j];
+        }
+        if constexpr (OWNER) {
+            if (lane < NC)
+#pragma unroll
+                for (int q = 0; q < PB; ++q) blk[(PB * P + q) * NB + PB * (P + 1) + lane] = col[q];
+        }
+#pragma unroll
+        for (int q = 0; q < PB; ++q) sh.b[q * PBP + lane] = lane < NC ? col[q] : 0.0f;
+        __syncthreads();
+        float fa[PB / 4][4], fat[PB / 4][4], fb[PB / 4][4];
+#pragma unroll
+        for (int ks = 0; ks < PB / 4; ++ks) {
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti) {
+                fa[ks][ti] = sh.a[(ti * 16 + li) * PA + 4 * ks + lk];
+                fat[ks][ti] = OWNER ? 0.0f : sh.at[(ti * 16 + li) * PA + 4 * ks + lk];
+            }
+#pragma unroll
+            for (int tj = P + 1; tj < 4; ++tj) fb[ks][tj] = sh.b[(4 * ks + lk) * PBP + (tj - P - 1) * 16 + li];
+        }
+        // the next panel's tile column first: its steps need only those
+#pragma unroll
+        for (int tj = P + 1; tj < 4; ++tj)
+#pragma unroll
+            for (int ks = 0; ks < PB / 4; ++ks) {
+#pragma unroll
+                for (int ti = OWNER ? P + 1 : 0; ti < 4; ++ti)
+                    if (r0 + ti * 16 < n)
+                        acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[ks][tj], fa[ks][ti], acc[ti][tj], 0, 0, 0);
+                if constexpr (!OWNER) {
+#pragma unroll
+                    for (int ti = P + 1; ti < 4; ++ti)
+                        acct[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[ks][tj], fat[ks][ti], acct[ti][tj], 0, 0, 0);
+                }
+            }
+    }
+    __syncthreads();  // LDS reuse by the next panel
+}
+
+template <bool OWNER>
+__device__ __forceinline__ void block_body(float *__restrict__ a, int lda, int n, int kb, float *__restrict__ blk,
+                                           int *__restrict__ flag, int32_t *__restrict__ ipiv, BlockLds &sh)
+{
+    const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+    const int r0 = kb + 64 * (int)blockIdx.x;
+    float4v acc[4][4], acct[4][4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) {
+        const int64_t ro = (int64_t)min(r0 + ti * 16 + li, n - 1) * lda + kb + 4 * lk;
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = *(const float4v *)(a + ro + tj * 16);
+    }
+    if constexpr (!OWNER) {
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti) {
+            const int64_t ro = (int64_t)(kb + ti * 16 + li) * lda + kb + 4 * lk;
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj) acct[ti][tj] = *(const float4v *)(a + ro + tj * 16);
+        }
+    }
+    block_panel<0, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh, lane, li, lk);
+    block_panel<1, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh, lane, li, lk);
+    block_panel<2, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh, lane, li, lk);
+    block_panel<3, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh, lane, li, lk);
+}
+
+// full block (kb + 64 <= n), 16-byte rows, n - kb <= 4096 (a wave per 64
+// rows, all resident at once is not needed: no wave waits for another)
+__global__ __launch_bounds__(64) void k_block_spec(float *__restrict__ a, int lda, int n, int kb,
+                                                   float *__restrict__ blk, int *__restrict__ flag,
+                                                   int32_t *__restrict__ ipiv)
+{
+    __shared__ BlockLds sh;
+    if (blockIdx.x == 0) block_body<true>(a, lda, n, kb, blk, flag, ipiv, sh);
+    else block_body<false>(a, lda, n, kb, blk, flag, ipiv, sh);
+}
+
+// the last block's diagonal block into the matrix (no block TRSM follows)
+__global__ __launch_bounds__(256) void k_take_blk(float *__restrict__ a, int lda, int kb, const float *__restrict__ blk)
+{
+    for (int e = threadIdx.x; e < NB * NB; e += 256) a[(int64_t)(kb + e / NB) * lda + kb + e % NB] = blk[e];
+}
+
 // ------------------------------------------------------------ refinement
 
 // r = b - A x in FP64 (A FP32), one wave per row, all right-hand sides
@@ -1910,11 +2174,11 @@ int solve_status(piro_ctx *ctx, const SolvePlan &pl)
 struct FactorWs {
     void *base;
     int *info, *flag, *zc;
-    float *top, *bk12, *bk;
+    float *top, *bk12, *bk, *blk;
     // top: one PB x PB slot per panel of a block (the end-to-end pass keeps
     // a block's four; the per-panel pass uses slot 0)
-    static constexpr size_t HEAD = 512, TOP = (NB / PB) * PB * PB * 4, BK12 = PB * FW * 4;
-    static size_t bytes(int n) { return HEAD + TOP + BK12 + (size_t)n * PB * 4; }
+    static constexpr size_t HEAD = 512, TOP = (NB / PB) * PB * PB * 4, BK12 = PB * FW * 4, BLK = NB * NB * 4;
+    static size_t bytes(int n) { return HEAD + TOP + BK12 + BLK + (size_t)n * PB * 4; }
     explicit FactorWs(void *b_) : base(b_)
     {
         char *b = (char *)b_;
@@ -1923,7 +2187,8 @@ struct FactorWs {
         zc = info + 64;
         top = (float *)(b + HEAD);
         bk12 = (float *)(b + HEAD + TOP);
-        bk = (float *)(b + HEAD + TOP + BK12);
+        blk = (float *)(b + HEAD + TOP + BK12);
+        bk = (float *)(b + HEAD + TOP + BK12 + BLK);
     }
 };
 
@@ -2021,7 +2286,7 @@ void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipi
         if (c0 < n) {
             static_assert(NB == TW, "block TRSM rows");
             const int gt = (n - c0 + 63) / 64 + (kb + 255) / 256;
-            k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, nullptr);
+            k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, nullptr, nullptr);
             check_launch();
             const bool vu = vec && kb % 4 == 0 && c0 % 4 == 0;
             if (n - c0 <= kUpd32Max) {  // quarter wave tiles: four times the waves
@@ -2057,8 +2322,15 @@ void factor_spec(hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const F
     const bool vec = (lda % 4 == 0) && ((uintptr_t)a % 16 == 0);
     for (int kb = 0; kb < n; kb += NB) {
         const int nbk = min(NB, n - kb);
+        // a full block with 16-byte rows: all four panels in one launch, the
+        // diagonal block in ws.blk
+        const bool whole = nbk == NB && vec;
+        if (whole) {
+            k_block_spec<<<(n - kb + 63) / 64, 64, 0, s>>>(a, lda, n, kb, ws.blk, ws.flag, ipiv);
+            check_launch();
+        }
         bool pre = false;  // the panel's pass ran in the previous in-block update
-        for (int p = 0, k = kb; k < kb + nbk; ++p, k += PB) {
+        for (int p = 0, k = kb; !whole && k < kb + nbk; ++p, k += PB) {
             const int jb = min(PB, kb + nbk - k);
             const int c0 = k + jb, c1 = kb + nbk;
             float *slot = ws.top + (size_t)p * PB * PB;
@@ -2089,7 +2361,8 @@ void factor_spec(hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const F
         const int c0 = kb + nbk;
         if (c0 < n) {
             const int gt = (n - c0 + 63) / 64;  // no exchanges: no left-column workgroups
-            k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, ws.top);
+            k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, whole ? nullptr : ws.top,
+                                            whole ? ws.blk : nullptr);
             check_launch();
             const bool vu = vec && kb % 4 == 0 && c0 % 4 == 0;
             if (n - c0 <= kUpd32Max) {
@@ -2099,6 +2372,9 @@ void factor_spec(hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const F
                 dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
                 k_update_mfma<NB, 2, 2><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu);
             }
+            check_launch();
+        } else if (whole) {
+            k_take_blk<<<1, 256, 0, s>>>(a, lda, kb, ws.blk);
             check_launch();
         } else {
             k_take_tops<<<1, 256, 0, s>>>(a, lda, kb, nbk, ws.top);

@@ -290,9 +290,10 @@ def test_refine_rejects_non_finite_residual(ctx):
 @pytest.mark.parametrize("n,zcols", [(700, (300,)), (700, (699,)), (4096, (1500, 2047)), (130, (5, 64))])
 def test_lu_exactly_zero_column_reports_the_first_step(ctx, spec, n, zcols):
     # a diagonally dominant matrix with whole columns zero: no row ever wins
-    # a pivot search, so the end-to-end speculative pass runs to the end and
-    # must report the first exactly zero pivot (1-based, LAPACK's info) as
-    # the pivoting steps do, with the same factors
+    # a pivot search; the speculative pass must report the first exactly zero
+    # pivot (1-based, LAPACK's info) as the pivoting steps do, with the same
+    # factors, whether it runs to the end (panels of a partial last block)
+    # or hands the matrix to the pivoting steps (full blocks)
     from piro_amd._lib import PiroError
     print(f"case zero column n={n} cols={zcols} spec={spec}", flush=True)
     rng = np.random.default_rng(n + sum(zcols))
@@ -316,3 +317,30 @@ def test_lu_exactly_zero_column_reports_the_first_step(ctx, spec, n, zcols):
         assert got.tobytes() == d2.host().tobytes()
     finally:
         ctx.lu_set_speculation(True)
+
+
+@pytest.mark.parametrize("n,cols", [(300, (70,)), (4096, (5, 1500, 4094))])
+def test_lu_subnormal_pivots_strategies_bit_identical(ctx, n, cols):
+    # columns of a diagonally dominant matrix scaled by 2^-140, so that their
+    # pivots are subnormal (the pivoting steps divide by such a pivot
+    # instead of multiplying by its reciprocal): both strategies give the
+    # same factors and pivots, bit for bit, and the factors are finite
+    print(f"case lu subnormal pivots n={n} cols={cols}", flush=True)
+    rng = np.random.default_rng(n + 11)
+    a = rng.uniform(-1.0, 1.0, (n, n)).astype(np.float32)
+    a[np.arange(n), np.arange(n)] = (n + rng.uniform(1.0, 2.0, n)).astype(np.float32)
+    for c in cols:
+        a[:, c] *= np.float32(2.0**-140)
+    assert np.all(np.abs(a[cols, cols]) < np.finfo(np.float32).tiny)
+    out = []
+    for spec in (True, False):
+        ctx.lu_set_speculation(spec)
+        d = dev(a, ctx)
+        ipiv = ctx.lu_factor(d, n)
+        out.append((d.host(), ipiv.host()))
+    ctx.lu_set_speculation(True)
+    assert out[0][0].tobytes() == out[1][0].tobytes()
+    assert out[0][1].tobytes() == out[1][1].tobytes()
+    lu = out[0][0].reshape(n, n)
+    assert np.all(np.isfinite(lu))
+    assert np.all(np.abs(lu[cols, cols]) < np.finfo(np.float32).tiny)
