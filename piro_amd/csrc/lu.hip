@@ -1155,8 +1155,11 @@ __global__ __launch_bounds__(64 * WGR * WGC) void k_update_mfma(float *__restric
 // every C entry's MFMAs (K in steps of 4, ascending) are k_update_mfma's, so
 // the bits are the same.
 template <int KW>
+// blk (the one-launch block kernel; NULL otherwise): the block's diagonal
+// block (64 x 64, row-major at rows and columns k ..), which workgroup (0, 0)
+// copies in; the update reads neither it nor anything it overlaps
 __global__ __launch_bounds__(256) void k_update_mfma32(float *__restrict__ a, int lda, int n, int k, int jb, int r0g,
-                                                        int c0g, int c1, bool vec)
+                                                        int c0g, int c1, bool vec, const float *__restrict__ blk)
 {
     constexpr int NT = 256, TR = 64, TC = 64, PA = KW + 4, PBP = TC + 16;
     constexpr int NQA = TR * KW / 4 / NT, NQB = KW * TC / 4 / NT;
@@ -1166,6 +1169,8 @@ __global__ __launch_bounds__(256) void k_update_mfma32(float *__restrict__ a, in
     const int r0 = r0g + blockIdx.y * TR, c0 = c0g + blockIdx.x * TC;
     const int t = threadIdx.x, lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    if (blk && blockIdx.x == 0 && blockIdx.y == 0)
+        for (int e = t; e < NB * NB; e += NT) a[(int64_t)(k + e / NB) * lda + k + e % NB] = blk[e];
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int rw = r0 + wr, cw = c0 + wc;
     const int li = lane & 15, lk = lane >> 4;
@@ -1492,6 +1497,29 @@ struct __attribute__((aligned(16))) BlockLds {
     float b[PB * (64 + 16)];  // U12
 };
 
+constexpr int BNP = NB + 4;   // pitch of -L11 (the A reads of a wave hit 64 banks)
+
+// The block kernel's LDS: the panel phase (wave 0), then the A12 tile's
+// solve in the same space (a wave per 16 columns, pitch TXP as in
+// k_trsm_block); -L11 kept for the solve, filled as the panels finish
+struct __attribute__((aligned(16))) BlockShared {
+    union {
+        BlockLds pan;
+        float tx[4][TW * TXP];
+    };
+    float nl[NB][BNP];
+};
+
+// the ordering __syncthreads() gives one wave's LDS accesses (the
+// workgroup-scope fences, which wait for the wave's LDS operations), without
+// the workgroup barrier: for LDS that only this wave reads and writes
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 __device__ __forceinline__ void row_from_lds(float (&v)[PB], const float *src)
 {
 #pragma unroll
@@ -1508,7 +1536,7 @@ template <int P, bool OWNER>
 __device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int n, int kb, int r0, float *__restrict__ blk,
                                             int *__restrict__ flag, int32_t *__restrict__ ipiv,
                                             float4v (&acc)[4][4], float4v (&acct)[4][4],
-                                            BlockLds &sh, int lane, int li, int lk)
+                                            BlockLds &sh, float (*__restrict__ nl)[BNP], int lane, int li, int lk)
 {
     constexpr int PA = PB + 4, PBP = 64 + 16, NC = NB - PB * (P + 1);  // later columns
     const int k = kb + PB * P;
@@ -1523,7 +1551,7 @@ __device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int 
     }
 #pragma unroll
     for (int tj = P + 1; tj < 4; ++tj) *(float4v *)&sh.fu[li][16 * (tj - P - 1) + 4 * lk] = dg[P][tj];
-    __syncthreads();
+    wave_sync();
     float top[PB], x[PB], xt[PB];
     row_from_lds(top, &sh.top[min(lane, PB - 1)][0]);
     row_from_lds(x, &sh.rows[lane][0]);
@@ -1540,6 +1568,28 @@ __device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int 
         if (lane >= PB * P) put_row(blk + lane * NB + PB * P, x);
     } else {
         if (own) put_row(a + (int64_t)(r0 + lane) * lda + k, x);
+        // -L11's panel columns for the A12 solve, as the block TRSM stages
+        // them (the negated multipliers below the diagonal, zeros on and
+        // above it): the diagonal block's rows below the panel from xt, the
+        // top rows' multipliers from `top` (lane q's columns below q)
+        float4v *row = (float4v *)&nl[lane][PB * P];
+        if (lane >= PB * (P + 1)) {
+#pragma unroll
+            for (int q = 0; q < PB / 4; ++q) row[q] = (float4v){-xt[4 * q], -xt[4 * q + 1], -xt[4 * q + 2], -xt[4 * q + 3]};
+        } else if (lane < PB * P) {
+#pragma unroll
+            for (int q = 0; q < PB / 4; ++q) row[q] = (float4v){0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        if (lane < PB) {
+            float4v *trow = (float4v *)&nl[PB * P + lane][PB * P];
+#pragma unroll
+            for (int q = 0; q < PB / 4; ++q) {
+                float4v v;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[r] = 4 * q + r < lane ? -top[4 * q + r] : 0.0f;
+                trow[q] = v;
+            }
+        }
     }
     if constexpr (NC > 0) {
         // the K = 16 update of the later columns, staged as update_stage
@@ -1555,7 +1605,7 @@ __device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int 
         if (lane < PB)
 #pragma unroll
             for (int j = 0; j < PB; ++j) sh.l[j][lane] = top[j];
-        __syncthreads();
+        wave_sync();
         // U12 = L11^-1 A12 as the per-panel kernels' owner forms it: each
         // row's terms in ascending column order, unfused
         float col[PB];
@@ -1575,7 +1625,7 @@ __device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int 
         }
 #pragma unroll
         for (int q = 0; q < PB; ++q) sh.b[q * PBP + lane] = lane < NC ? col[q] : 0.0f;
-        __syncthreads();
+        wave_sync();
         float fa[PB / 4][4], fat[PB / 4][4], fb[PB / 4][4];
 #pragma unroll
         for (int ks = 0; ks < PB / 4; ++ks) {
@@ -1592,10 +1642,11 @@ __device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int 
         for (int tj = P + 1; tj < 4; ++tj)
 #pragma unroll
             for (int ks = 0; ks < PB / 4; ++ks) {
+                // (a tile past n: its -L21 rows are staged as zeros, and it
+                // is never stored)
 #pragma unroll
                 for (int ti = OWNER ? P + 1 : 0; ti < 4; ++ti)
-                    if (r0 + ti * 16 < n)
-                        acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[ks][tj], fa[ks][ti], acc[ti][tj], 0, 0, 0);
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[ks][tj], fa[ks][ti], acc[ti][tj], 0, 0, 0);
                 if constexpr (!OWNER) {
 #pragma unroll
                     for (int ti = P + 1; ti < 4; ++ti)
@@ -1603,14 +1654,89 @@ __device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int 
                 }
             }
     }
-    __syncthreads();  // LDS reuse by the next panel
+    wave_sync();  // LDS reuse by the next panel
+}
+
+// U12 = L11^-1 A12 for 16 columns [cw, cw + 16) of the block's rows, as
+// k_trsm_block's wave forms it (the same operations on the same operands,
+// in the same layout): the 16 x 16 diagonal triangles by substitution, every
+// lane solving column li (four copies), lanes 0..15 writing it back; the
+// products with the later row blocks as v_mfma_f32_16x16x4f32.  nl: -L11
+// strictly lower (zeros on and above the diagonal).  xv: the wave's A12
+// entries, lane moving column cw + (lane & 15) of rows (lane >> 4) + 4 q
+// (zeros from n on; a column's results depend on its own entries only).
+// Every LDS access here is to this wave's x or the read-only nl.
+__device__ __forceinline__ void block_trsm16(float *__restrict__ a, int lda, int n, int kb, int cw,
+                                             const float (&xv)[16], float *__restrict__ x,
+                                             const float (*__restrict__ nl)[BNP], int lane)
+{
+    const int lc = lane & 15, lr = lane >> 4, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[(lr + 4 * q) * TXP + lc] = xv[q];
+    wave_sync();
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        // the triangle's multipliers all read first (one LDS round trip),
+        // then its terms column by column: each row still takes them in
+        // ascending column order
+        float v[16], l[16][16];
+#pragma unroll
+        for (int i = 1; i < 16; ++i)
+#pragma unroll
+            for (int j4 = 0; j4 < i; j4 += 4) {
+                const float4v w = *(const float4v *)&nl[16 * p + i][16 * p + j4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) l[i][j4 + r] = w[r];
+            }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = x[(16 * p + i) * TXP + li];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 15; ++j)
+#pragma unroll
+            for (int i = j + 1; i < 16; ++i) v[i] = __builtin_fmaf(l[i][j], v[j], v[i]);
+        if (lane < 16)
+#pragma unroll
+            for (int i = 1; i < 16; ++i) x[(16 * p + i) * TXP + li] = v[i];
+        wave_sync();
+        if (p < 3) {
+            // X_q += (-L_qp) X_p for the later blocks: A[i][k] = -L[16q + i][16p + k],
+            // B[k][j] = X[16p + k][j], D[i][j] = X[16q + i][j]
+            float fa[4][4], fb[4];
+            float4v acc[4];
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                fb[ks] = x[(16 * p + 4 * ks + lk) * TXP + li];
+#pragma unroll
+                for (int q = p + 1; q < 4; ++q) fa[ks][q] = nl[16 * q + li][16 * p + 4 * ks + lk];
+            }
+#pragma unroll
+            for (int q = p + 1; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[q][r] = x[(16 * q + 4 * lk + r) * TXP + li];
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                for (int q = p + 1; q < 4; ++q)
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[ks][q], fb[ks], acc[q], 0, 0, 0);
+#pragma unroll
+            for (int q = p + 1; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) x[(16 * q + 4 * lk + r) * TXP + li] = acc[q][r];
+            wave_sync();
+        }
+    }
+    if (cw + lc < n)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) a[(int64_t)(kb + lr + 4 * q) * lda + cw + lc] = x[(lr + 4 * q) * TXP + lc];
 }
 
 template <bool OWNER>
 __device__ __forceinline__ void block_body(float *__restrict__ a, int lda, int n, int kb, float *__restrict__ blk,
-                                           int *__restrict__ flag, int32_t *__restrict__ ipiv, BlockLds &sh)
+                                           int *__restrict__ flag, int32_t *__restrict__ ipiv, BlockShared &sh,
+                                           int lane)
 {
-    const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+    const int li = lane & 15, lk = lane >> 4;
     const int r0 = kb + 64 * (int)blockIdx.x;
     float4v acc[4][4], acct[4][4];
 #pragma unroll
@@ -1627,21 +1753,42 @@ __device__ __forceinline__ void block_body(float *__restrict__ a, int lda, int n
             for (int tj = 0; tj < 4; ++tj) acct[ti][tj] = *(const float4v *)(a + ro + tj * 16);
         }
     }
-    block_panel<0, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh, lane, li, lk);
-    block_panel<1, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh, lane, li, lk);
-    block_panel<2, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh, lane, li, lk);
-    block_panel<3, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh, lane, li, lk);
+    block_panel<0, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, lane, li, lk);
+    block_panel<1, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, lane, li, lk);
+    block_panel<2, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, lane, li, lk);
+    block_panel<3, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, lane, li, lk);
 }
 
-// full block (kb + 64 <= n), 16-byte rows, n - kb <= 4096 (a wave per 64
-// rows, all resident at once is not needed: no wave waits for another)
-__global__ __launch_bounds__(64) void k_block_spec(float *__restrict__ a, int lda, int n, int kb,
-                                                   float *__restrict__ blk, int *__restrict__ flag,
-                                                   int32_t *__restrict__ ipiv)
+// full block (kb + 64 <= n), 16-byte rows, n - kb <= 4096; a workgroup of
+// four waves per 64 rows, no workgroup waiting for another.  Wave 0 runs the
+// block's panels; in the workgroups below the diagonal block, the four waves
+// then form U12 for the block's rows, workgroup w taking the block TRSM's
+// column tile w - 1 (there are as many such workgroups as tiles), and write
+// it in place, so no block TRSM follows.  The diagonal block stays in blk
+// until the trailing update (or, after the last block, k_take_blk) copies
+// it in.
+__global__ __launch_bounds__(256) void k_block_spec(float *__restrict__ a, int lda, int n, int kb,
+                                                    float *__restrict__ blk, int *__restrict__ flag,
+                                                    int32_t *__restrict__ ipiv)
 {
-    __shared__ BlockLds sh;
-    if (blockIdx.x == 0) block_body<true>(a, lda, n, kb, blk, flag, ipiv, sh);
-    else block_body<false>(a, lda, n, kb, blk, flag, ipiv, sh);
+    __shared__ BlockShared sh;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (blockIdx.x == 0) {
+        if (w == 0) block_body<true>(a, lda, n, kb, blk, flag, ipiv, sh, lane);
+        return;  // the diagonal block's rows: no U12 columns
+    }
+    // this wave's A12 columns, read while wave 0 runs the panels
+    const int cw = kb + NB * (int)blockIdx.x + 16 * w, lc = lane & 15, lr = lane >> 4;
+    float xv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const float e = a[(int64_t)(kb + lr + 4 * q) * lda + min(cw + lc, n - 1)];
+        xv[q] = cw + lc < n ? e : 0.0f;
+    }
+    if (w == 0) block_body<false>(a, lda, n, kb, blk, flag, ipiv, sh, lane);
+    __syncthreads();  // -L11 complete in LDS, the panel phase's LDS free
+    if (cw < n) block_trsm16(a, lda, n, kb, cw, xv, sh.tx[w], sh.nl, lane);
 }
 
 // the last block's diagonal block into the matrix (no block TRSM follows)
@@ -2291,7 +2438,7 @@ void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipi
             const bool vu = vec && kb % 4 == 0 && c0 % 4 == 0;
             if (n - c0 <= kUpd32Max) {  // quarter wave tiles: four times the waves
                 dim3 g((n - c0 + 63) / 64, (n - c0 + 63) / 64);
-                k_update_mfma32<NB><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu);
+                k_update_mfma32<NB><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu, nullptr);
             } else {
                 dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
                 k_update_mfma<NB, 2, 2><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu);
@@ -2326,7 +2473,7 @@ void factor_spec(hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const F
         // diagonal block in ws.blk
         const bool whole = nbk == NB && vec;
         if (whole) {
-            k_block_spec<<<(n - kb + 63) / 64, 64, 0, s>>>(a, lda, n, kb, ws.blk, ws.flag, ipiv);
+            k_block_spec<<<(n - kb + 63) / 64, 256, 0, s>>>(a, lda, n, kb, ws.blk, ws.flag, ipiv);
             check_launch();
         }
         bool pre = false;  // the panel's pass ran in the previous in-block update
@@ -2360,19 +2507,25 @@ void factor_spec(hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const F
         }
         const int c0 = kb + nbk;
         if (c0 < n) {
-            const int gt = (n - c0 + 63) / 64;  // no exchanges: no left-column workgroups
-            k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, whole ? nullptr : ws.top,
-                                            whole ? ws.blk : nullptr);
-            check_launch();
+            if (!whole) {  // a full block's kernel formed U12 itself
+                const int gt = (n - c0 + 63) / 64;  // no exchanges: no left-column workgroups
+                k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, ws.top, nullptr);
+                check_launch();
+            }
             const bool vu = vec && kb % 4 == 0 && c0 % 4 == 0;
             if (n - c0 <= kUpd32Max) {
                 dim3 g((n - c0 + 63) / 64, (n - c0 + 63) / 64);
-                k_update_mfma32<NB><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu);
+                k_update_mfma32<NB><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu, whole ? ws.blk : nullptr);
+                check_launch();
             } else {
                 dim3 g((n - c0 + 127) / 128, (n - c0 + 127) / 128);
                 k_update_mfma<NB, 2, 2><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu);
+                check_launch();
+                if (whole) {
+                    k_take_blk<<<1, 256, 0, s>>>(a, lda, kb, ws.blk);
+                    check_launch();
+                }
             }
-            check_launch();
         } else if (whole) {
             k_take_blk<<<1, 256, 0, s>>>(a, lda, kb, ws.blk);
             check_launch();
