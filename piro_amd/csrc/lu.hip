@@ -804,12 +804,8 @@ __device__ __forceinline__ void take_tops(float *__restrict__ a, int lda, int kb
 // panels' top rows, slot p for panel p, not yet in the matrix.  L11's
 // diagonal 16 x 16 blocks are read from them and workgroup 0 copies them in
 // (no other workgroup reads those entries).
-// blk (the one-launch block kernel; NULL otherwise): the whole diagonal
-// block, row-major 64 x 64, not yet in the matrix: L11 is read from it and
-// workgroup 0 copies it in.
 __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int lda, int kb, int nbk, int c0, int c1,
-                                                    const int32_t *__restrict__ ipiv, const float *__restrict__ tops,
-                                                    const float *__restrict__ blk)
+                                                    const int32_t *__restrict__ ipiv, const float *__restrict__ tops)
 {
     __shared__ __attribute__((aligned(16))) float s_nl[TW][TW + 4];  // -L11, strictly lower
     __shared__ float s_x[4][TW * TXP];
@@ -818,7 +814,7 @@ __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int l
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     // the end-to-end pass exchanges no rows: no pivots to read or apply, and
     // no workgroups for the left columns
-    const bool ident = tops != nullptr || blk != nullptr;
+    const bool ident = tops != nullptr;
     if (!ident) {
         if (t < nbk) s_p[t] = ipiv[kb + t];
         __syncthreads();
@@ -836,13 +832,10 @@ __global__ __launch_bounds__(256) void k_trsm_block(float *__restrict__ a, int l
 #pragma unroll
     for (int it = 0; it < TW * TW / 256; ++it) {
         const int e = t + 256 * it, i = e / TW, j = e % TW;
-        ld[it] = blk ? blk[i * NB + j]
-               : (tops && i / PB == j / PB) ? tops[(i / PB) * PB * PB + (i % PB) * PB + j % PB]
+        ld[it] = (tops && i / PB == j / PB) ? tops[(i / PB) * PB * PB + (i % PB) * PB + j % PB]
                                             : a[(int64_t)(kb + min(i, nbk - 1)) * lda + kb + min(j, nbk - 1)];
     }
     if (tops && blockIdx.x == 0) take_tops(a, lda, kb, nbk, tops, t);
-    if (blk && blockIdx.x == 0)
-        for (int e = t; e < NB * NB; e += 256) a[(int64_t)(kb + e / NB) * lda + kb + e % NB] = blk[e];
     if (!ident) {
         if (t < 64 && c0 + (int)blockIdx.x * 64 + t < c1) swap_columns(a, lda, kb, nbk, s_p, c0 + blockIdx.x * 64 + t);
         __syncthreads();  // the exchanged rows are read below by other lanes
@@ -1426,21 +1419,24 @@ __global__ __launch_bounds__(64) void k_update16_spec(float *__restrict__ a, int
 // the pivoting steps factor such a matrix.  `top` is updated in every lane:
 // lane J' is read at step J' before that step writes it, and afterwards only
 // for its multipliers (columns below J'), which no later step writes.
-// MASKX: this wave holds the panel's top rows (the owner), whose lanes at
-// or above the step keep their values; in the other waves every own row is
-// below the panel, and rows past n are never written.  XT: the wave's copy
-// of the diagonal block's rows, whose lanes above the panel's top rows are
-// discarded (staged as zeros).
-template <int J, bool MASKX, bool XT>
-__device__ __forceinline__ void block_step(float (&top)[PB], float (&x)[PB], float (&xt)[PB], bool below,
-                                           uint32_t &viol)
+// MODE: STEP_TOP, the top block alone; STEP_ROWS, also the rows x, every
+// lane (rows below the panel, or rows whose values are discarded: rows past
+// n, which are never written, or the diagonal block's rows above the
+// panel's top rows, which are staged as zeros); STEP_ROWS_MASKED, the rows
+// x of a wave that holds the panel's top rows (the owner), whose lanes at or
+// above the step keep their values.  viol: only the row modes look at x.
+enum { STEP_TOP, STEP_ROWS, STEP_ROWS_MASKED };
+
+template <int J, int MODE>
+__device__ __forceinline__ void block_step(float (&top)[PB], float (&x)[PB], bool below, uint32_t &viol)
 {
     float u[PB];
 #pragma unroll
     for (int c = J; c < PB; ++c) u[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(top[c]), J));
     const float d = u[J];
     const uint32_t db = (uint32_t)__float_as_int(d) & 0x7fffffffu;
-    viol |= (below && ((uint32_t)__float_as_int(x[J]) & 0x7fffffffu) > db) ? 1u : 0u;
+    if constexpr (MODE != STEP_TOP)
+        viol |= (below && ((uint32_t)__float_as_int(x[J]) & 0x7fffffffu) > db) ? 1u : 0u;
     viol |= (db < 0x00800000u || db >= 0x7f800000u) ? 1u : 0u;  // uniform
     const float rd = 1.0f / d;
     {
@@ -1449,39 +1445,31 @@ __device__ __forceinline__ void block_step(float (&top)[PB], float (&x)[PB], flo
 #pragma unroll
         for (int c = J + 1; c < PB; ++c) top[c] = __builtin_fmaf(-l, u[c], top[c]);
     }
-    {
+    if constexpr (MODE == STEP_ROWS_MASKED) {
         const float l = x[J] * rd;
-        if constexpr (MASKX) {
-            x[J] = below ? l : x[J];
+        x[J] = below ? l : x[J];
 #pragma unroll
-            for (int c = J + 1; c < PB; ++c) x[c] = below ? __builtin_fmaf(-l, u[c], x[c]) : x[c];
-        } else {
-            x[J] = l;
+        for (int c = J + 1; c < PB; ++c) x[c] = below ? __builtin_fmaf(-l, u[c], x[c]) : x[c];
+    } else if constexpr (MODE == STEP_ROWS) {
+        const float l = x[J] * rd;
+        x[J] = l;
 #pragma unroll
-            for (int c = J + 1; c < PB; ++c) x[c] = __builtin_fmaf(-l, u[c], x[c]);
-        }
-    }
-    if constexpr (XT) {
-        const float l = xt[J] * rd;
-        xt[J] = l;
-#pragma unroll
-        for (int c = J + 1; c < PB; ++c) xt[c] = __builtin_fmaf(-l, u[c], xt[c]);
+        for (int c = J + 1; c < PB; ++c) x[c] = __builtin_fmaf(-l, u[c], x[c]);
     }
 }
 
-template <int J, bool MASKX, bool XT>
+template <int J, int MODE>
 struct BlockSteps {
-    static __device__ __forceinline__ void run(float (&top)[PB], float (&x)[PB], float (&xt)[PB], int i, bool own,
-                                               uint32_t &viol)
+    static __device__ __forceinline__ void run(float (&top)[PB], float (&x)[PB], int i, bool own, uint32_t &viol)
     {
-        block_step<J, MASKX, XT>(top, x, xt, own && i > J, viol);
-        BlockSteps<J + 1, MASKX, XT>::run(top, x, xt, i, own, viol);
+        block_step<J, MODE>(top, x, own && i > J, viol);
+        BlockSteps<J + 1, MODE>::run(top, x, i, own, viol);
     }
 };
 
-template <bool MASKX, bool XT>
-struct BlockSteps<PB, MASKX, XT> {
-    static __device__ __forceinline__ void run(float (&)[PB], float (&)[PB], float (&)[PB], int, bool, uint32_t &) {}
+template <int MODE>
+struct BlockSteps<PB, MODE> {
+    static __device__ __forceinline__ void run(float (&)[PB], float (&)[PB], int, bool, uint32_t &) {}
 };
 
 constexpr int BRP = PB + 4;  // row pitch of the block kernel's row tiles (16-byte rows)
@@ -1530,131 +1518,200 @@ __device__ __forceinline__ void row_from_lds(float (&v)[PB], const float *src)
     }
 }
 
-// one panel P of the block; OWNER: this wave's rows are the diagonal
-// block's (acc), otherwise acct carries the diagonal block
+// The panels of a full block in one four-wave workgroup per 64 rows, each
+// wave with its own part, every part by the same operations on the same
+// operands as the one-wave form (so the same bits):
+//   wave 0 (ROLE_ROWS): the workgroup's 64 rows, their 64 block columns in
+//     MFMA accumulators (acc) from start to end: each panel's steps, the
+//     flag, the rows written out, and the K = 16 updates of the later
+//     columns;
+//   wave 1 (ROLE_DIAG): the 64 x 64 diagonal block (acct), carried through
+//     the owner's steps and updates; it publishes each panel's top block and
+//     the top rows' later columns (A12), and fills -L11 (nl) as the panels
+//     finish;
+//   wave 2 (ROLE_U12): U12 = L11^-1 A12 of each panel, published for the
+//     updates (and, in the owner's workgroup, written to blk);
+//   wave 3: the solve only.
+// In the owner's workgroup (rows kb ..) wave 0's rows are the diagonal
+// block: it holds it, publishes the top block, and runs the masked steps;
+// wave 1 is idle.  Each wave runs the top block's steps itself, so no wave
+// waits for another's; per panel two workgroup barriers separate the top
+// block's publication (A) from U12's and the staged -L21's (B), and every
+// wave passes both.
+enum { ROLE_ROWS, ROLE_DIAG, ROLE_U12 };
+
+// the diagonal block's holder: panel P's top block and its A12
+template <int P>
+__device__ __forceinline__ void publish_top(const float4v (&dg)[4][4], BlockLds &sh, int li, int lk)
+{
+    *(float4v *)&sh.top[li][4 * lk] = dg[P][P];
+#pragma unroll
+    for (int tj = P + 1; tj < 4; ++tj) *(float4v *)&sh.fu[li][16 * (tj - P - 1) + 4 * lk] = dg[P][tj];
+}
+
+// between barriers A and B: each wave's steps for panel P
 template <int P, bool OWNER>
-__device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int n, int kb, int r0, float *__restrict__ blk,
-                                            int *__restrict__ flag, int32_t *__restrict__ ipiv,
-                                            float4v (&acc)[4][4], float4v (&acct)[4][4],
-                                            BlockLds &sh, float (*__restrict__ nl)[BNP], int lane, int li, int lk)
+__device__ __forceinline__ void block_phase2(float *__restrict__ a, int lda, int n, int kb, int r0,
+                                             float *__restrict__ blk, int *__restrict__ flag,
+                                             int32_t *__restrict__ ipiv, const float4v (&acc)[4][4],
+                                             const float4v (&acct)[4][4], BlockLds &sh,
+                                             float (*__restrict__ nl)[BNP], int role, int lane, int li, int lk)
 {
     constexpr int PA = PB + 4, PBP = 64 + 16, NC = NB - PB * (P + 1);  // later columns
     const int k = kb + PB * P;
-    float4v (&dg)[4][4] = OWNER ? acc : acct;  // the diagonal block (compile-time choice)
-    // tile columns to rows through LDS
-    *(float4v *)&sh.top[li][4 * lk] = dg[P][P];
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti) *(float4v *)&sh.rows[16 * ti + li][4 * lk] = acc[ti][P];
-    if constexpr (!OWNER) {
-#pragma unroll
-        for (int ti = P + 1; ti < 4; ++ti) *(float4v *)&sh.rowt[16 * ti + li][4 * lk] = acct[ti][P];
-    }
-#pragma unroll
-    for (int tj = P + 1; tj < 4; ++tj) *(float4v *)&sh.fu[li][16 * (tj - P - 1) + 4 * lk] = dg[P][tj];
-    wave_sync();
-    float top[PB], x[PB], xt[PB];
+    float top[PB];
     row_from_lds(top, &sh.top[min(lane, PB - 1)][0]);
-    row_from_lds(x, &sh.rows[lane][0]);
-    if constexpr (!OWNER) row_from_lds(xt, &sh.rowt[lane][0]);
-    // this wave's rows: panel row i = r0 - k + lane (the owner's rows above
-    // the panel have i < 0 and take no step)
-    const int i = r0 - k + lane;
-    const bool own = r0 + lane < n;
-    uint32_t viol = 0;
-    BlockSteps<0, OWNER, !OWNER>::run(top, x, xt, i, own, viol);
-    if (__ballot(viol != 0) != 0 && lane == 0) __hip_atomic_fetch_or(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if constexpr (OWNER) {
-        if (lane < PB) ipiv[k + lane] = k + lane;
-        if (lane >= PB * P) put_row(blk + lane * NB + PB * P, x);
-    } else {
-        if (own) put_row(a + (int64_t)(r0 + lane) * lda + k, x);
-        // -L11's panel columns for the A12 solve, as the block TRSM stages
-        // them (the negated multipliers below the diagonal, zeros on and
-        // above it): the diagonal block's rows below the panel from xt, the
-        // top rows' multipliers from `top` (lane q's columns below q)
-        float4v *row = (float4v *)&nl[lane][PB * P];
-        if (lane >= PB * (P + 1)) {
+    if (role == ROLE_ROWS) {
+        // this wave's rows: panel row i = r0 - k + lane (the owner's rows
+        // above the panel have i < 0 and take no step)
 #pragma unroll
-            for (int q = 0; q < PB / 4; ++q) row[q] = (float4v){-xt[4 * q], -xt[4 * q + 1], -xt[4 * q + 2], -xt[4 * q + 3]};
-        } else if (lane < PB * P) {
-#pragma unroll
-            for (int q = 0; q < PB / 4; ++q) row[q] = (float4v){0.0f, 0.0f, 0.0f, 0.0f};
-        }
-        if (lane < PB) {
-            float4v *trow = (float4v *)&nl[PB * P + lane][PB * P];
-#pragma unroll
-            for (int q = 0; q < PB / 4; ++q) {
-                float4v v;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) v[r] = 4 * q + r < lane ? -top[4 * q + r] : 0.0f;
-                trow[q] = v;
-            }
-        }
-    }
-    if constexpr (NC > 0) {
-        // the K = 16 update of the later columns, staged as update_stage
-        // stages it (-L21 by rows, U12 by rows; zeros past the matrix)
-        const bool arow = OWNER ? lane >= PB * (P + 1) : own;
-#pragma unroll
-        for (int c = 0; c < PB; ++c) sh.a[lane * PA + c] = arow ? -x[c] : 0.0f;
-        if constexpr (!OWNER) {
-            const bool trow = lane >= PB * (P + 1);
-#pragma unroll
-            for (int c = 0; c < PB; ++c) sh.at[lane * PA + c] = trow ? -xt[c] : 0.0f;
-        }
-        if (lane < PB)
-#pragma unroll
-            for (int j = 0; j < PB; ++j) sh.l[j][lane] = top[j];
+        for (int ti = 0; ti < 4; ++ti) *(float4v *)&sh.rows[16 * ti + li][4 * lk] = acc[ti][P];
         wave_sync();
-        // U12 = L11^-1 A12 as the per-panel kernels' owner forms it: each
-        // row's terms in ascending column order, unfused
-        float col[PB];
-#pragma unroll
-        for (int q = 0; q < PB; ++q) col[q] = sh.fu[q][min(lane, NC - 1)];
-#pragma unroll
-        for (int j = 0; j < PB - 1; ++j) {
-            float lj[PB];
-            row_from_lds(lj, &sh.l[j][0]);
-#pragma unroll
-            for (int q = j + 1; q < PB; ++q) col[q] = col[q] - lj[q] * col[j];
-        }
+        float x[PB];
+        row_from_lds(x, &sh.rows[lane][0]);
+        const int i = r0 - k + lane;
+        const bool own = r0 + lane < n;
+        uint32_t viol = 0;
+        BlockSteps<0, OWNER ? STEP_ROWS_MASKED : STEP_ROWS>::run(top, x, i, own, viol);
+        if (__ballot(viol != 0) != 0 && lane == 0)
+            __hip_atomic_fetch_or(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if constexpr (OWNER) {
-            if (lane < NC)
-#pragma unroll
-                for (int q = 0; q < PB; ++q) blk[(PB * P + q) * NB + PB * (P + 1) + lane] = col[q];
+            if (lane < PB) ipiv[k + lane] = k + lane;
+            if (lane >= PB * P) put_row(blk + lane * NB + PB * P, x);
+        } else {
+            if (own) put_row(a + (int64_t)(r0 + lane) * lda + k, x);
         }
+        if constexpr (NC > 0) {
+            // -L21 staged as update_stage stages it (zeros past the matrix)
+            const bool arow = OWNER ? lane >= PB * (P + 1) : own;
 #pragma unroll
-        for (int q = 0; q < PB; ++q) sh.b[q * PBP + lane] = lane < NC ? col[q] : 0.0f;
-        wave_sync();
-        float fa[PB / 4][4], fat[PB / 4][4], fb[PB / 4][4];
+            for (int c = 0; c < PB; ++c) sh.a[lane * PA + c] = arow ? -x[c] : 0.0f;
+        }
+    } else if (role == ROLE_DIAG) {
+        if constexpr (!OWNER) {
 #pragma unroll
-        for (int ks = 0; ks < PB / 4; ++ks) {
+            for (int ti = P + 1; ti < 4; ++ti) *(float4v *)&sh.rowt[16 * ti + li][4 * lk] = acct[ti][P];
+            wave_sync();
+            float xt[PB];
+            row_from_lds(xt, &sh.rowt[lane][0]);  // lanes above the panel's top rows: discarded
+            uint32_t viol = 0;
+            BlockSteps<0, STEP_ROWS>::run(top, xt, 0, false, viol);
+            // -L11's panel columns for the solve, as the block TRSM stages
+            // them (the negated multipliers below the diagonal, zeros on and
+            // above it): the rows below the panel from xt, the top rows'
+            // multipliers from `top` (lane q's columns below q)
+            float4v *row = (float4v *)&nl[lane][PB * P];
+            if (lane >= PB * (P + 1)) {
 #pragma unroll
-            for (int ti = 0; ti < 4; ++ti) {
-                fa[ks][ti] = sh.a[(ti * 16 + li) * PA + 4 * ks + lk];
-                fat[ks][ti] = OWNER ? 0.0f : sh.at[(ti * 16 + li) * PA + 4 * ks + lk];
+                for (int q = 0; q < PB / 4; ++q)
+                    row[q] = (float4v){-xt[4 * q], -xt[4 * q + 1], -xt[4 * q + 2], -xt[4 * q + 3]};
+            } else if (lane < PB * P) {
+#pragma unroll
+                for (int q = 0; q < PB / 4; ++q) row[q] = (float4v){0.0f, 0.0f, 0.0f, 0.0f};
             }
+            if (lane < PB) {
+                float4v *trow = (float4v *)&nl[PB * P + lane][PB * P];
 #pragma unroll
-            for (int tj = P + 1; tj < 4; ++tj) fb[ks][tj] = sh.b[(4 * ks + lk) * PBP + (tj - P - 1) * 16 + li];
-        }
-        // the next panel's tile column first: its steps need only those
+                for (int q = 0; q < PB / 4; ++q) {
+                    float4v v;
 #pragma unroll
-        for (int tj = P + 1; tj < 4; ++tj)
-#pragma unroll
-            for (int ks = 0; ks < PB / 4; ++ks) {
-                // (a tile past n: its -L21 rows are staged as zeros, and it
-                // is never stored)
-#pragma unroll
-                for (int ti = OWNER ? P + 1 : 0; ti < 4; ++ti)
-                    acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[ks][tj], fa[ks][ti], acc[ti][tj], 0, 0, 0);
-                if constexpr (!OWNER) {
-#pragma unroll
-                    for (int ti = P + 1; ti < 4; ++ti)
-                        acct[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[ks][tj], fat[ks][ti], acct[ti][tj], 0, 0, 0);
+                    for (int r = 0; r < 4; ++r) v[r] = 4 * q + r < lane ? -top[4 * q + r] : 0.0f;
+                    trow[q] = v;
                 }
             }
+            if constexpr (NC > 0) {
+                const bool trow_ = lane >= PB * (P + 1);
+#pragma unroll
+                for (int c = 0; c < PB; ++c) sh.at[lane * PA + c] = trow_ ? -xt[c] : 0.0f;
+            }
+        }
+    } else if (role == ROLE_U12) {
+        if constexpr (NC > 0) {
+            float dummy[PB];
+            uint32_t viol = 0;
+            BlockSteps<0, STEP_TOP>::run(top, dummy, 0, false, viol);
+            if (lane < PB)
+#pragma unroll
+                for (int j = 0; j < PB; ++j) sh.l[j][lane] = top[j];
+            wave_sync();
+            // U12 = L11^-1 A12 as the per-panel kernels' owner forms it: each
+            // row's terms in ascending column order, unfused
+            float col[PB];
+#pragma unroll
+            for (int q = 0; q < PB; ++q) col[q] = sh.fu[q][min(lane, NC - 1)];
+#pragma unroll
+            for (int j = 0; j < PB - 1; ++j) {
+                float lj[PB];
+                row_from_lds(lj, &sh.l[j][0]);
+#pragma unroll
+                for (int q = j + 1; q < PB; ++q) col[q] = col[q] - lj[q] * col[j];
+            }
+            if constexpr (OWNER) {
+                if (lane < NC)
+#pragma unroll
+                    for (int q = 0; q < PB; ++q) blk[(PB * P + q) * NB + PB * (P + 1) + lane] = col[q];
+            }
+            // U12 staged as update_stage stages it (zeros past the block)
+#pragma unroll
+            for (int q = 0; q < PB; ++q) sh.b[q * PBP + lane] = lane < NC ? col[q] : 0.0f;
+        }
     }
-    wave_sync();  // LDS reuse by the next panel
+}
+
+// after barrier B: the K = 16 updates of the later columns (the next
+// panel's tile column first), then the holder publishes the next top block
+template <int P, bool OWNER>
+__device__ __forceinline__ void block_phase3(float4v (&acc)[4][4], float4v (&acct)[4][4], BlockLds &sh, int role,
+                                             int li, int lk)
+{
+    constexpr int PA = PB + 4, PBP = 64 + 16;
+    if constexpr (P < 3) {
+        const bool rows = role == ROLE_ROWS, diag = !OWNER && role == ROLE_DIAG;
+        if (rows || diag) {
+            const float *sa = rows ? sh.a : sh.at;
+            float fa[PB / 4][4], fb[PB / 4][4];
+#pragma unroll
+            for (int ks = 0; ks < PB / 4; ++ks) {
+#pragma unroll
+                for (int ti = 0; ti < 4; ++ti) fa[ks][ti] = sa[(ti * 16 + li) * PA + 4 * ks + lk];
+#pragma unroll
+                for (int tj = P + 1; tj < 4; ++tj) fb[ks][tj] = sh.b[(4 * ks + lk) * PBP + (tj - P - 1) * 16 + li];
+            }
+            // (a row tile past n: its -L21 rows are staged as zeros, and it
+            // is never stored)
+            if (rows) {
+#pragma unroll
+                for (int tj = P + 1; tj < 4; ++tj)
+#pragma unroll
+                    for (int ks = 0; ks < PB / 4; ++ks)
+#pragma unroll
+                        for (int ti = OWNER ? P + 1 : 0; ti < 4; ++ti)
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[ks][tj], fa[ks][ti], acc[ti][tj], 0, 0, 0);
+                if constexpr (OWNER) publish_top<P + 1>(acc, sh, li, lk);
+            } else {
+#pragma unroll
+                for (int tj = P + 1; tj < 4; ++tj)
+#pragma unroll
+                    for (int ks = 0; ks < PB / 4; ++ks)
+#pragma unroll
+                        for (int ti = P + 1; ti < 4; ++ti)
+                            acct[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[ks][tj], fa[ks][ti], acct[ti][tj], 0, 0, 0);
+                publish_top<P + 1>(acct, sh, li, lk);
+            }
+        }
+    }
+}
+
+template <int P, bool OWNER>
+__device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int n, int kb, int r0, float *__restrict__ blk,
+                                            int *__restrict__ flag, int32_t *__restrict__ ipiv, float4v (&acc)[4][4],
+                                            float4v (&acct)[4][4], BlockLds &sh, float (*__restrict__ nl)[BNP], int role,
+                                            int lane, int li, int lk)
+{
+    __syncthreads();  // A: panel P's top block and A12 published
+    block_phase2<P, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh, nl, role, lane, li, lk);
+    __syncthreads();  // B: U12 and the staged -L21 complete; the top block read
+    block_phase3<P, OWNER>(acc, acct, sh, role, li, lk);
 }
 
 // U12 = L11^-1 A12 for 16 columns [cw, cw + 16) of the block's rows, as
@@ -1734,39 +1791,42 @@ __device__ __forceinline__ void block_trsm16(float *__restrict__ a, int lda, int
 template <bool OWNER>
 __device__ __forceinline__ void block_body(float *__restrict__ a, int lda, int n, int kb, float *__restrict__ blk,
                                            int *__restrict__ flag, int32_t *__restrict__ ipiv, BlockShared &sh,
-                                           int lane)
+                                           int role, int lane)
 {
     const int li = lane & 15, lk = lane >> 4;
     const int r0 = kb + 64 * (int)blockIdx.x;
     float4v acc[4][4], acct[4][4];
+    if (role == ROLE_ROWS) {
 #pragma unroll
-    for (int ti = 0; ti < 4; ++ti) {
-        const int64_t ro = (int64_t)min(r0 + ti * 16 + li, n - 1) * lda + kb + 4 * lk;
+        for (int ti = 0; ti < 4; ++ti) {
+            const int64_t ro = (int64_t)min(r0 + ti * 16 + li, n - 1) * lda + kb + 4 * lk;
 #pragma unroll
-        for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = *(const float4v *)(a + ro + tj * 16);
-    }
-    if constexpr (!OWNER) {
+            for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = *(const float4v *)(a + ro + tj * 16);
+        }
+        if constexpr (OWNER) publish_top<0>(acc, sh.pan, li, lk);
+    } else if (!OWNER && role == ROLE_DIAG) {
 #pragma unroll
         for (int ti = 0; ti < 4; ++ti) {
             const int64_t ro = (int64_t)(kb + ti * 16 + li) * lda + kb + 4 * lk;
 #pragma unroll
             for (int tj = 0; tj < 4; ++tj) acct[ti][tj] = *(const float4v *)(a + ro + tj * 16);
         }
+        publish_top<0>(acct, sh.pan, li, lk);
     }
-    block_panel<0, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, lane, li, lk);
-    block_panel<1, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, lane, li, lk);
-    block_panel<2, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, lane, li, lk);
-    block_panel<3, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, lane, li, lk);
+    block_panel<0, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, role, lane, li, lk);
+    block_panel<1, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, role, lane, li, lk);
+    block_panel<2, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, role, lane, li, lk);
+    block_panel<3, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, role, lane, li, lk);
 }
 
 // full block (kb + 64 <= n), 16-byte rows, n - kb <= 4096; a workgroup of
-// four waves per 64 rows, no workgroup waiting for another.  Wave 0 runs the
-// block's panels; in the workgroups below the diagonal block, the four waves
-// then form U12 for the block's rows, workgroup w taking the block TRSM's
-// column tile w - 1 (there are as many such workgroups as tiles), and write
-// it in place, so no block TRSM follows.  The diagonal block stays in blk
-// until the trailing update (or, after the last block, k_take_blk) copies
-// it in.
+// four waves per 64 rows, no workgroup waiting for another (the waves'
+// parts are above block_phase2).  In the workgroups below the diagonal block
+// the four waves then form U12 for the block's rows, workgroup w taking the
+// block TRSM's column tile w - 1 (there are as many such workgroups as
+// tiles), and write it in place, so no block TRSM follows.  The diagonal
+// block stays in blk until the trailing update (or, after the last block,
+// k_take_blk) copies it in.
 __global__ __launch_bounds__(256) void k_block_spec(float *__restrict__ a, int lda, int n, int kb,
                                                     float *__restrict__ blk, int *__restrict__ flag,
                                                     int32_t *__restrict__ ipiv)
@@ -1774,11 +1834,11 @@ __global__ __launch_bounds__(256) void k_block_spec(float *__restrict__ a, int l
     __shared__ BlockShared sh;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (blockIdx.x == 0) {
-        if (w == 0) block_body<true>(a, lda, n, kb, blk, flag, ipiv, sh, lane);
-        return;  // the diagonal block's rows: no U12 columns
+    if (blockIdx.x == 0) {  // the diagonal block's rows: no U12 columns
+        block_body<true>(a, lda, n, kb, blk, flag, ipiv, sh, w, lane);
+        return;
     }
-    // this wave's A12 columns, read while wave 0 runs the panels
+    // this wave's A12 columns, read while the panels run
     const int cw = kb + NB * (int)blockIdx.x + 16 * w, lc = lane & 15, lr = lane >> 4;
     float xv[16];
 #pragma unroll
@@ -1786,7 +1846,7 @@ __global__ __launch_bounds__(256) void k_block_spec(float *__restrict__ a, int l
         const float e = a[(int64_t)(kb + lr + 4 * q) * lda + min(cw + lc, n - 1)];
         xv[q] = cw + lc < n ? e : 0.0f;
     }
-    if (w == 0) block_body<false>(a, lda, n, kb, blk, flag, ipiv, sh, lane);
+    block_body<false>(a, lda, n, kb, blk, flag, ipiv, sh, w, lane);
     __syncthreads();  // -L11 complete in LDS, the panel phase's LDS free
     if (cw < n) block_trsm16(a, lda, n, kb, cw, xv, sh.tx[w], sh.nl, lane);
 }
@@ -2433,7 +2493,7 @@ void factor(piro_ctx *ctx, hipStream_t s, int n, float *a, int lda, int32_t *ipi
         if (c0 < n) {
             static_assert(NB == TW, "block TRSM rows");
             const int gt = (n - c0 + 63) / 64 + (kb + 255) / 256;
-            k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, nullptr, nullptr);
+            k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, nullptr);
             check_launch();
             const bool vu = vec && kb % 4 == 0 && c0 % 4 == 0;
             if (n - c0 <= kUpd32Max) {  // quarter wave tiles: four times the waves
@@ -2509,7 +2569,7 @@ void factor_spec(hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const F
         if (c0 < n) {
             if (!whole) {  // a full block's kernel formed U12 itself
                 const int gt = (n - c0 + 63) / 64;  // no exchanges: no left-column workgroups
-                k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, ws.top, nullptr);
+                k_trsm_block<<<gt, 256, 0, s>>>(a, lda, kb, nbk, c0, n, ipiv, ws.top);
                 check_launch();
             }
             const bool vu = vec && kb % 4 == 0 && c0 % 4 == 0;
