@@ -1887,70 +1887,78 @@ __global__ __launch_bounds__(256) void k_residual(const float *__restrict__ A, i
     }
 }
 
-// the same for 5..32 right-hand sides: workgroup = 16 rows x 32 right-hand
-// sides, A and x staged through LDS in 64-column chunks (the next chunk is
-// loaded into registers during the current one's products); thread (row
-// t >> 4, right-hand sides 2 (t & 15), +1); sums in column order
-__global__ __launch_bounds__(256) void k_residual_tile(const float *__restrict__ A, int lda, int n, int nrhs,
+// r = b - A x for 5..32 right-hand sides with FP64 MFMA
+// (v_mfma_f64_16x16x4f64): a workgroup per 16 rows, all 32 right-hand
+// sides as two 16 x 16 tiles, its four waves splitting K (wave w takes the
+// 16-column chunks w, w + 4, ...).  For MFMA step tt of a chunk at k0, lane
+// l supplies A[row l % 16][k0 + 4 (l / 16) + tt] (FP32, exactly widened)
+// and x[the same k][column l % 16 (+ 16)], so A and x agree on every
+// k slot; D[i][j] is lane j + 16 (i % 4), entry i / 4 (this instruction's
+// layout, unlike the FP32 16 x 16 x 4 one's).  The waves' partial
+// sums are added in LDS in a fixed order (deterministic results).
+typedef double double4v __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_residual_mfma(const float *__restrict__ A, int lda, int n, int nrhs,
                                                        const double *__restrict__ b, int ldb,
                                                        const double *__restrict__ x, int ldx,
                                                        double *__restrict__ r, float *__restrict__ r32)
 {
-    __shared__ float s_a[16][68];
-    __shared__ double2 s_x[64][16];
-    const int t = threadIdx.x, rr = t >> 4, cp = t & 15;
+    __shared__ double s_part[4][16][33];
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int li = lane & 15, lg = lane >> 4;
     const int r0 = blockIdx.x * 16;
-    // staging: thread t moves A[r0 + (t >> 4)][k0 + 4 (t & 15) .. +3] and
-    // x rows k0 + (t >> 4) + 16 q, right-hand sides 2 (t & 15), +1
-    const int ai = t >> 4, aj = 4 * (t & 15);
-    float ra[4];
-    double2 rx[4];
-    auto fetch = [&](int k0) {
+    const float *arow = A + (int64_t)min(r0 + li, n - 1) * lda;  // rows past n: never stored
+    const bool vec = (lda % 4 == 0) && ((uintptr_t)A % 16 == 0);
+    const bool c0in = li < nrhs, c1in = 16 + li < nrhs;
+    double4v acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    const int nch = (n + 15) / 16;
+    for (int ch = w; ch < nch; ch += 4) {
+        const int k0 = ch * 16 + 4 * lg;
+        float av[4];
+        if (vec && k0 + 4 <= n) {
+            const float4v v = *(const float4v *)(arow + k0);
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            ra[q] = (r0 + ai < n && k0 + aj + q < n) ? A[(int64_t)(r0 + ai) * lda + k0 + aj + q] : 0.0f;
+            for (int tt = 0; tt < 4; ++tt) av[tt] = v[tt];
+        } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int kk = k0 + (t >> 4) + 16 * q, c = 2 * (t & 15);
-            const bool kin = kk < n;
-            const double *src = x + (int64_t)kk * ldx;
-            rx[q] = make_double2((kin && c < nrhs) ? src[c] : 0.0, (kin && c + 1 < nrhs) ? src[c + 1] : 0.0);
+            for (int tt = 0; tt < 4; ++tt) {
+                const float e = arow[min(k0 + tt, n - 1)];
+                av[tt] = k0 + tt < n ? e : 0.0f;
+            }
         }
-    };
-    // four partial sums per output (k mod 4) keep four FP64 add chains in
-    // flight; they are combined in a fixed order at the end
-    double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
-    fetch(0);
-    for (int k0 = 0; k0 < n; k0 += 64) {
-        __syncthreads();
+        double xa[4], xb[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            s_a[ai][aj + q] = ra[q];
-            s_x[(t >> 4) + 16 * q][t & 15] = rx[q];
+        for (int tt = 0; tt < 4; ++tt) {
+            const int kk = min(k0 + tt, n - 1);
+            const double *xr = x + (int64_t)kk * ldx;
+            const double ea = xr[min(li, nrhs - 1)], eb = xr[min(16 + li, nrhs - 1)];
+            xa[tt] = c0in ? ea : 0.0;
+            xb[tt] = c1in ? eb : 0.0;
         }
-        __syncthreads();
-        if (k0 + 64 < n) fetch(k0 + 64);  // in flight during the chunk's products
-#pragma unroll 8
-        for (int k = 0; k < 64; ++k) {
-            const double av = (double)s_a[rr][k];
-            const double2 xv = s_x[k][cp];
-            a0[k & 3] += av * xv.x;
-            a1[k & 3] += av * xv.y;
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) {
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[tt], xa[tt], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[tt], xb[tt], acc1, 0, 0, 0);
         }
     }
-    const double s0 = (a0[0] + a0[1]) + (a0[2] + a0[3]), s1 = (a1[0] + a1[1]) + (a1[2] + a1[3]);
-    const int row = r0 + rr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        s_part[w][lg + 4 * v][li] = acc0[v];
+        s_part[w][lg + 4 * v][16 + li] = acc1[v];
+    }
+    __syncthreads();
+    const int i = t >> 4, c = 2 * (t & 15), row = r0 + i;
     if (row >= n) return;
-    const int c = 2 * cp;
-    if (c < nrhs) {
-        const double rv = b[(int64_t)row * ldb + c] - s0;
-        r[(int64_t)row * nrhs + c] = rv;
-        r32[(int64_t)row * nrhs + c] = (float)rv;
-    }
-    if (c + 1 < nrhs) {
-        const double rv = b[(int64_t)row * ldb + c + 1] - s1;
-        r[(int64_t)row * nrhs + c + 1] = rv;
-        r32[(int64_t)row * nrhs + c + 1] = (float)rv;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int cc = c + e;
+        if (cc < nrhs) {
+            const double sum = ((s_part[0][i][cc] + s_part[1][i][cc]) + s_part[2][i][cc]) + s_part[3][i][cc];
+            const double rv = b[(int64_t)row * ldb + cc] - sum;
+            r[(int64_t)row * nrhs + cc] = rv;
+            r32[(int64_t)row * nrhs + cc] = (float)rv;
+        }
     }
 }
 
@@ -2698,7 +2706,7 @@ int piro_lu_solve_refine(piro_ctx *ctx, int n, const float *d_a, int lda, const 
         if (nrhs <= 4)
             k_residual<4><<<(n + 3) / 4, 256, 0, s>>>(d_a, lda, n, nrhs, d_b, ldb, d_x, ldx, r, r32);
         else
-            k_residual_tile<<<(n + 15) / 16, 256, 0, s>>>(d_a, lda, n, nrhs, d_b, ldb, d_x, ldx, r, r32);
+            k_residual_mfma<<<(n + 15) / 16, 256, 0, s>>>(d_a, lda, n, nrhs, d_b, ldb, d_x, ldx, r, r32);
         check_launch();
         k_colnorms<<<nrhs, 256, 0, s>>>(n, nrhs, r, d_b, ldb, norms);
         check_launch();
