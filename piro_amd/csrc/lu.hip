@@ -1859,32 +1859,69 @@ __global__ __launch_bounds__(256) void k_take_blk(float *__restrict__ a, int lda
 
 // ------------------------------------------------------------ refinement
 
-// r = b - A x in FP64 (A FP32), one wave per row, all right-hand sides
+// r = b - A x in FP64 (A FP32) for up to NR right-hand sides: a wave per
+// RW rows (each x load serves all of them), lane l taking columns
+// 4 l + 256 i .. + 3 (16-byte loads of A where the rows allow), then a wave
+// sum per row and right-hand side
 template <int NR>
 __global__ __launch_bounds__(256) void k_residual(const float *__restrict__ A, int lda, int n, int nrhs,
                                                   const double *__restrict__ b, int ldb, const double *__restrict__ x,
                                                   int ldx, double *__restrict__ r, float *__restrict__ r32)
 {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (row >= n) return;
-    double acc[NR];
+    constexpr int RW = 4;
+    const int lane = threadIdx.x & 63;
+    const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RW;
+    if (row0 >= n) return;
+    const bool vec = (lda % 4 == 0) && ((uintptr_t)A % 16 == 0);
+    const float *arow[RW];
 #pragma unroll
-    for (int c = 0; c < NR; ++c) acc[c] = 0.0;
-    for (int j = lane; j < n; j += 64) {
-        const double av = (double)A[(int64_t)row * lda + j];
+    for (int q = 0; q < RW; ++q) arow[q] = A + (int64_t)min(row0 + q, n - 1) * lda;  // rows past n: not stored
+    double acc[RW][NR];
 #pragma unroll
-        for (int c = 0; c < NR; ++c)
-            if (c < nrhs) acc[c] += av * x[(int64_t)j * ldx + c];
-    }
+    for (int q = 0; q < RW; ++q)
 #pragma unroll
-    for (int c = 0; c < NR; ++c) {
-        double v = wave_sum_f64(acc[c]);
-        if (lane == 0 && c < nrhs) {
-            double rv = b[(int64_t)row * ldb + c] - v;
-            r[(int64_t)row * nrhs + c] = rv;
-            r32[(int64_t)row * nrhs + c] = (float)rv;
+        for (int c = 0; c < NR; ++c) acc[q][c] = 0.0;
+    for (int j = 4 * lane; j < n; j += 256) {
+        float av[RW][4];
+        if (vec && j + 4 <= n) {
+#pragma unroll
+            for (int q = 0; q < RW; ++q) {
+                const float4v v = *(const float4v *)(arow[q] + j);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) av[q][e] = v[e];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < RW; ++q)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float v = arow[q][min(j + e, n - 1)];
+                    av[q][e] = j + e < n ? v : 0.0f;
+                }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const double *xr = x + (int64_t)min(j + e, n - 1) * ldx;
+#pragma unroll
+            for (int c = 0; c < NR; ++c) {
+                const double xv = xr[min(c, nrhs - 1)];
+#pragma unroll
+                for (int q = 0; q < RW; ++q) acc[q][c] += (double)av[q][e] * xv;
+            }
         }
     }
+#pragma unroll
+    for (int q = 0; q < RW; ++q)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+            const double v = wave_sum_f64(acc[q][c]);
+            const int row = row0 + q;
+            if (lane == 0 && c < nrhs && row < n) {
+                const double rv = b[(int64_t)row * ldb + c] - v;
+                r[(int64_t)row * nrhs + c] = rv;
+                r32[(int64_t)row * nrhs + c] = (float)rv;
+            }
+        }
 }
 
 // r = b - A x for 5..32 right-hand sides with FP64 MFMA
@@ -2704,7 +2741,7 @@ int piro_lu_solve_refine(piro_ctx *ctx, int n, const float *d_a, int lda, const 
     int sw = 0;
     for (;;) {
         if (nrhs <= 4)
-            k_residual<4><<<(n + 3) / 4, 256, 0, s>>>(d_a, lda, n, nrhs, d_b, ldb, d_x, ldx, r, r32);
+            k_residual<4><<<(n + 15) / 16, 256, 0, s>>>(d_a, lda, n, nrhs, d_b, ldb, d_x, ldx, r, r32);
         else
             k_residual_mfma<<<(n + 15) / 16, 256, 0, s>>>(d_a, lda, n, nrhs, d_b, ldb, d_x, ldx, r, r32);
         check_launch();
