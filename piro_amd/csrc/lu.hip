@@ -1415,8 +1415,11 @@ __global__ __launch_bounds__(64) void k_update16_spec(float *__restrict__ a, int
 // the reciprocal formed once, and no lane masks where no later read needs
 // one.  A pivot that is zero, subnormal, infinite or NaN (where the
 // pivoting steps' multiplier takes another form, or a zero pivot skips the
-// step) raises the flag like a row that would have won a pivot search, so
-// the pivoting steps factor such a matrix.  `top` is updated in every lane:
+// step), or of magnitude 2^126 or more, raises the flag like a row that
+// would have won a pivot search, so the pivoting steps factor such a
+// matrix.  For every other pivot, v_rcp_f32 and one FMA correction give the
+// correctly rounded reciprocal, the same bits as the division 1.0f / d
+// (checked over every such float by piro_check_reciprocal).  `top` is updated in every lane:
 // lane J' is read at step J' before that step writes it, and afterwards only
 // for its multipliers (columns below J'), which no later step writes.
 // MODE: STEP_TOP, the top block alone; STEP_ROWS, also the rows x, every
@@ -1437,8 +1440,9 @@ __device__ __forceinline__ void block_step(float (&top)[PB], float (&x)[PB], boo
     const uint32_t db = (uint32_t)__float_as_int(d) & 0x7fffffffu;
     if constexpr (MODE != STEP_TOP)
         viol |= (below && ((uint32_t)__float_as_int(x[J]) & 0x7fffffffu) > db) ? 1u : 0u;
-    viol |= (db < 0x00800000u || db >= 0x7f800000u) ? 1u : 0u;  // uniform
-    const float rd = 1.0f / d;
+    viol |= (db < 0x00800000u || db >= 0x7e800000u) ? 1u : 0u;  // uniform
+    const float r0 = __builtin_amdgcn_rcpf(d);
+    const float rd = __builtin_fmaf(__builtin_fmaf(-d, r0, 1.0f), r0, r0);
     {
         const float l = top[J] * rd;
         top[J] = l;

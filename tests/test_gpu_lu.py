@@ -344,3 +344,21 @@ def test_lu_subnormal_pivots_strategies_bit_identical(ctx, n, cols):
     lu = out[0][0].reshape(n, n)
     assert np.all(np.isfinite(lu))
     assert np.all(np.abs(lu[cols, cols]) < np.finfo(np.float32).tiny)
+
+
+def test_block_kernel_reciprocal_matches_division():
+    # the block kernel takes a pivot's reciprocal as v_rcp_f32 plus one FMA
+    # correction instead of the division 1.0f / d of the pivoting steps; the
+    # two must agree bit for bit on every pivot it accepts (normal, below
+    # 2^126 in magnitude), checked over all 2^32 floats on the device; the
+    # pivots of 2^126 and above, where they differ, raise its flag
+    import ctypes
+    import os
+    print("case reciprocal check over every float", flush=True)
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "piro_amd", "lib",
+                        "libpiro_checks.so")
+    lib = ctypes.CDLL(path)
+    counts = (ctypes.c_ulonglong * 2)()
+    assert lib.piro_check_reciprocal(counts) == 0
+    assert counts[0] == 0
+    assert counts[1] > 0  # the flagged range does differ: the check can see a mismatch
