@@ -319,19 +319,25 @@ def test_lu_exactly_zero_column_reports_the_first_step(ctx, spec, n, zcols):
         ctx.lu_set_speculation(True)
 
 
-@pytest.mark.parametrize("n,cols", [(300, (70,)), (4096, (5, 1500, 4094))])
-def test_lu_subnormal_pivots_strategies_bit_identical(ctx, n, cols):
+@pytest.mark.parametrize("n,cols,scale_exp", [(300, (70,), -140), (4096, (5, 1500, 4094), -140), (300, (70,), 118)])
+def test_lu_subnormal_pivots_strategies_bit_identical(ctx, n, cols, scale_exp):
     # columns of a diagonally dominant matrix scaled by 2^-140, so that their
     # pivots are subnormal (the pivoting steps divide by such a pivot
-    # instead of multiplying by its reciprocal): both strategies give the
-    # same factors and pivots, bit for bit, and the factors are finite
-    print(f"case lu subnormal pivots n={n} cols={cols}", flush=True)
+    # instead of multiplying by its reciprocal), or by 2^118, so that a pivot
+    # is 2^126 or more (where the block kernel's reciprocal would differ):
+    # both strategies give the same factors and pivots, bit for bit, and the
+    # factors are finite
+    print(f"case lu extreme pivots n={n} cols={cols} scale 2^{scale_exp}", flush=True)
     rng = np.random.default_rng(n + 11)
     a = rng.uniform(-1.0, 1.0, (n, n)).astype(np.float32)
     a[np.arange(n), np.arange(n)] = (n + rng.uniform(1.0, 2.0, n)).astype(np.float32)
     for c in cols:
-        a[:, c] *= np.float32(2.0**-140)
-    assert np.all(np.abs(a[cols, cols]) < np.finfo(np.float32).tiny)
+        a[:, c] *= np.float32(2.0**scale_exp)
+    tiny, big = np.finfo(np.float32).tiny, np.float32(2.0**126)
+    if scale_exp < 0:
+        assert np.all(np.abs(a[cols, cols]) < tiny)
+    else:
+        assert np.all(np.abs(a[cols, cols]) >= big)
     out = []
     for spec in (True, False):
         ctx.lu_set_speculation(spec)
@@ -343,7 +349,10 @@ def test_lu_subnormal_pivots_strategies_bit_identical(ctx, n, cols):
     assert out[0][1].tobytes() == out[1][1].tobytes()
     lu = out[0][0].reshape(n, n)
     assert np.all(np.isfinite(lu))
-    assert np.all(np.abs(lu[cols, cols]) < np.finfo(np.float32).tiny)
+    if scale_exp < 0:
+        assert np.all(np.abs(lu[cols, cols]) < tiny)
+    else:
+        assert np.all(np.abs(lu[cols, cols]) >= big)
 
 
 def test_block_kernel_reciprocal_matches_division():
