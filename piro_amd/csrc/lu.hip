@@ -1491,15 +1491,14 @@ struct __attribute__((aligned(16))) BlockLds {
 
 constexpr int BNP = NB + 4;   // pitch of -L11 (the A reads of a wave hit 64 banks)
 
-// The block kernel's LDS: the panel phase (wave 0), then the A12 tile's
-// solve in the same space (a wave per 16 columns, pitch TXP as in
-// k_trsm_block); -L11 kept for the solve, filled as the panels finish
+constexpr int BXP = NB + 16;  // pitch of the A12 tile (the B reads of a wave hit 64 banks)
+
+// The block kernel's LDS: the panels' arrays; -L11, filled as the panels
+// finish; the workgroup's A12 tile, solved step by step by wave 3
 struct __attribute__((aligned(16))) BlockShared {
-    union {
-        BlockLds pan;
-        float tx[4][TW * TXP];
-    };
+    BlockLds pan;
     float nl[NB][BNP];
+    float tx[NB][BXP];
 };
 
 // the ordering __syncthreads() gives one wave's LDS accesses (the
@@ -1522,6 +1521,95 @@ __device__ __forceinline__ void row_from_lds(float (&v)[PB], const float *src)
     }
 }
 
+// Step p of U12 = L11^-1 A12 for the workgroup's 64-column tile of A12
+// (tx, in LDS; columns from n on are zeros, never stored, and a column's
+// results depend on its own entries only), as k_trsm_block forms each
+// column (the same operations on the same operands): the 16 x 16 diagonal
+// triangle p by substitution, one column per lane, each row's terms in
+// ascending column order; then the products with the later row blocks as
+// v_mfma_f32_16x16x4f32 with k_trsm_block's operand roles, over four
+// 16-column tiles.  nl: -L11 strictly lower (zeros on and above the
+// diagonal); step p reads only its panel-p columns.  One wave.
+template <int p>
+__device__ __forceinline__ void trsm_step(float (*__restrict__ tx)[BXP], const float (*__restrict__ nl)[BNP],
+                                          float *__restrict__ a, int lda, int n, int kb, int cc, int lane, int li, int lk)
+{
+    // the triangle's multipliers all read first (one LDS round trip), then
+    // its terms column by column
+    float x[16], l[16][16];
+#pragma unroll
+    for (int i = 1; i < 16; ++i)
+#pragma unroll
+        for (int j4 = 0; j4 < i; j4 += 4) {
+            const float4v w = *(const float4v *)&nl[16 * p + i][16 * p + j4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) l[i][j4 + r] = w[r];
+        }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = tx[16 * p + i][lane];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < 15; ++j)
+#pragma unroll
+        for (int i = j + 1; i < 16; ++i) x[i] = __builtin_fmaf(l[i][j], x[j], x[i]);
+#pragma unroll
+    for (int i = 1; i < 16; ++i) tx[16 * p + i][lane] = x[i];
+    wave_sync();
+    // rows 16p .. 16p + 15 are final: out to the matrix (lane: columns
+    // cc + 4 (lane % 16) .. + 3 of rows 16p + lane / 16 + 4 it)
+    {
+        const int q4 = 4 * (lane & 15), rq = lane >> 4;
+        const bool whole = cc + NB <= n;  // uniform
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int row = 16 * p + 4 * it + rq;
+            float *dst = a + (int64_t)(kb + row) * lda + cc + q4;
+            const float4v w = *(const float4v *)&tx[row][q4];
+            if (whole) {
+                *(float4v *)dst = w;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (cc + q4 + r < n) dst[r] = w[r];
+            }
+        }
+    }
+    if constexpr (p < 3) {
+        // X_q += (-L_qp) X_p for the later row blocks: A[i][k] = -L[16q + i][16p + k],
+        // B[k][j] = X[16p + k][j], D[i][j] = X[16q + i][j]; operands first, then
+        // the independent accumulators interleaved
+        float fa[4][4], fb[4][4];
+        float4v acc[4][4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+#pragma unroll
+            for (int q = p + 1; q < 4; ++q) fa[ks][q] = nl[16 * q + li][16 * p + 4 * ks + lk];
+#pragma unroll
+            for (int tc = 0; tc < 4; ++tc) fb[ks][tc] = tx[16 * p + 4 * ks + lk][16 * tc + li];
+        }
+#pragma unroll
+        for (int q = p + 1; q < 4; ++q)
+#pragma unroll
+            for (int tc = 0; tc < 4; ++tc)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[q][tc][r] = tx[16 * q + 4 * lk + r][16 * tc + li];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int q = p + 1; q < 4; ++q)
+#pragma unroll
+                for (int tc = 0; tc < 4; ++tc)
+                    acc[q][tc] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[ks][q], fb[ks][tc], acc[q][tc], 0, 0, 0);
+#pragma unroll
+        for (int q = p + 1; q < 4; ++q)
+#pragma unroll
+            for (int tc = 0; tc < 4; ++tc)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) tx[16 * q + 4 * lk + r][16 * tc + li] = acc[q][tc][r];
+        wave_sync();
+    }
+}
+
 // The panels of a full block in one four-wave workgroup per 64 rows, each
 // wave with its own part, every part by the same operations on the same
 // operands as the one-wave form (so the same bits):
@@ -1535,14 +1623,16 @@ __device__ __forceinline__ void row_from_lds(float (&v)[PB], const float *src)
 //     finish;
 //   wave 2 (ROLE_U12): U12 = L11^-1 A12 of each panel, published for the
 //     updates (and, in the owner's workgroup, written to blk);
-//   wave 3: the solve only.
+//   wave 3 (ROLE_TRSM): below the diagonal block, the block TRSM's share of
+//     the workgroup (trsm_step): step p of its A12 tile's solve during panel
+//     p + 1, the last after the panels.
 // In the owner's workgroup (rows kb ..) wave 0's rows are the diagonal
 // block: it holds it, publishes the top block, and runs the masked steps;
 // wave 1 is idle.  Each wave runs the top block's steps itself, so no wave
 // waits for another's; per panel two workgroup barriers separate the top
 // block's publication (A) from U12's and the staged -L21's (B), and every
 // wave passes both.
-enum { ROLE_ROWS, ROLE_DIAG, ROLE_U12 };
+enum { ROLE_ROWS, ROLE_DIAG, ROLE_U12, ROLE_TRSM };
 
 // the diagonal block's holder: panel P's top block and its A12
 template <int P>
@@ -1558,9 +1648,11 @@ template <int P, bool OWNER>
 __device__ __forceinline__ void block_phase2(float *__restrict__ a, int lda, int n, int kb, int r0,
                                              float *__restrict__ blk, int *__restrict__ flag,
                                              int32_t *__restrict__ ipiv, const float4v (&acc)[4][4],
-                                             const float4v (&acct)[4][4], BlockLds &sh,
-                                             float (*__restrict__ nl)[BNP], int role, int lane, int li, int lk)
+                                             const float4v (&acct)[4][4], const float4v (&tv)[NB / 4],
+                                             BlockShared &shs, int role, int lane, int li, int lk)
 {
+    BlockLds &sh = shs.pan;
+    float(*__restrict__ nl)[BNP] = shs.nl;
     constexpr int PA = PB + 4, PBP = 64 + 16, NC = NB - PB * (P + 1);  // later columns
     const int k = kb + PB * P;
     float top[PB];
@@ -1659,6 +1751,20 @@ __device__ __forceinline__ void block_phase2(float *__restrict__ a, int lda, int
 #pragma unroll
             for (int q = 0; q < PB; ++q) sh.b[q * PBP + lane] = lane < NC ? col[q] : 0.0f;
         }
+    } else if (role == ROLE_TRSM) {
+        // the previous panel's -L11 columns are complete (barrier B of that
+        // panel): its step of the A12 tile's solve; the tile, read into
+        // registers at the start, goes to LDS before the first step
+        if constexpr (!OWNER && P >= 1) {
+            const int cc = kb + NB * (int)blockIdx.x;
+            if constexpr (P == 1) {
+                const int q4 = 4 * (lane & 15), rq = lane >> 4;
+#pragma unroll
+                for (int it = 0; it < NB / 4; ++it) *(float4v *)&shs.tx[4 * it + rq][q4] = tv[it];
+                wave_sync();
+            }
+            trsm_step<P - 1>(shs.tx, nl, a, lda, n, kb, cc, lane, li, lk);
+        }
     }
 }
 
@@ -1709,87 +1815,13 @@ __device__ __forceinline__ void block_phase3(float4v (&acc)[4][4], float4v (&acc
 template <int P, bool OWNER>
 __device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int n, int kb, int r0, float *__restrict__ blk,
                                             int *__restrict__ flag, int32_t *__restrict__ ipiv, float4v (&acc)[4][4],
-                                            float4v (&acct)[4][4], BlockLds &sh, float (*__restrict__ nl)[BNP], int role,
-                                            int lane, int li, int lk)
+                                            float4v (&acct)[4][4], const float4v (&tv)[NB / 4], BlockShared &sh,
+                                            int role, int lane, int li, int lk)
 {
     __syncthreads();  // A: panel P's top block and A12 published
-    block_phase2<P, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh, nl, role, lane, li, lk);
-    __syncthreads();  // B: U12 and the staged -L21 complete; the top block read
-    block_phase3<P, OWNER>(acc, acct, sh, role, li, lk);
-}
-
-// U12 = L11^-1 A12 for 16 columns [cw, cw + 16) of the block's rows, as
-// k_trsm_block's wave forms it (the same operations on the same operands,
-// in the same layout): the 16 x 16 diagonal triangles by substitution, every
-// lane solving column li (four copies), lanes 0..15 writing it back; the
-// products with the later row blocks as v_mfma_f32_16x16x4f32.  nl: -L11
-// strictly lower (zeros on and above the diagonal).  xv: the wave's A12
-// entries, lane moving column cw + (lane & 15) of rows (lane >> 4) + 4 q
-// (zeros from n on; a column's results depend on its own entries only).
-// Every LDS access here is to this wave's x or the read-only nl.
-__device__ __forceinline__ void block_trsm16(float *__restrict__ a, int lda, int n, int kb, int cw,
-                                             const float (&xv)[16], float *__restrict__ x,
-                                             const float (*__restrict__ nl)[BNP], int lane)
-{
-    const int lc = lane & 15, lr = lane >> 4, li = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) x[(lr + 4 * q) * TXP + lc] = xv[q];
-    wave_sync();
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        // the triangle's multipliers all read first (one LDS round trip),
-        // then its terms column by column: each row still takes them in
-        // ascending column order
-        float v[16], l[16][16];
-#pragma unroll
-        for (int i = 1; i < 16; ++i)
-#pragma unroll
-            for (int j4 = 0; j4 < i; j4 += 4) {
-                const float4v w = *(const float4v *)&nl[16 * p + i][16 * p + j4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) l[i][j4 + r] = w[r];
-            }
-#pragma unroll
-        for (int i = 0; i < 16; ++i) v[i] = x[(16 * p + i) * TXP + li];
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int j = 0; j < 15; ++j)
-#pragma unroll
-            for (int i = j + 1; i < 16; ++i) v[i] = __builtin_fmaf(l[i][j], v[j], v[i]);
-        if (lane < 16)
-#pragma unroll
-            for (int i = 1; i < 16; ++i) x[(16 * p + i) * TXP + li] = v[i];
-        wave_sync();
-        if (p < 3) {
-            // X_q += (-L_qp) X_p for the later blocks: A[i][k] = -L[16q + i][16p + k],
-            // B[k][j] = X[16p + k][j], D[i][j] = X[16q + i][j]
-            float fa[4][4], fb[4];
-            float4v acc[4];
-#pragma unroll
-            for (int ks = 0; ks < 4; ++ks) {
-                fb[ks] = x[(16 * p + 4 * ks + lk) * TXP + li];
-#pragma unroll
-                for (int q = p + 1; q < 4; ++q) fa[ks][q] = nl[16 * q + li][16 * p + 4 * ks + lk];
-            }
-#pragma unroll
-            for (int q = p + 1; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) acc[q][r] = x[(16 * q + 4 * lk + r) * TXP + li];
-#pragma unroll
-            for (int ks = 0; ks < 4; ++ks)
-#pragma unroll
-                for (int q = p + 1; q < 4; ++q)
-                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[ks][q], fb[ks], acc[q], 0, 0, 0);
-#pragma unroll
-            for (int q = p + 1; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) x[(16 * q + 4 * lk + r) * TXP + li] = acc[q][r];
-            wave_sync();
-        }
-    }
-    if (cw + lc < n)
-#pragma unroll
-        for (int q = 0; q < 16; ++q) a[(int64_t)(kb + lr + 4 * q) * lda + cw + lc] = x[(lr + 4 * q) * TXP + lc];
+    block_phase2<P, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, tv, sh, role, lane, li, lk);
+    __syncthreads();  // B: U12, the staged -L21 and panel P's -L11 columns complete; the top block read
+    block_phase3<P, OWNER>(acc, acct, sh.pan, role, li, lk);
 }
 
 template <bool OWNER>
@@ -1817,20 +1849,43 @@ __device__ __forceinline__ void block_body(float *__restrict__ a, int lda, int n
         }
         publish_top<0>(acct, sh.pan, li, lk);
     }
-    block_panel<0, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, role, lane, li, lk);
-    block_panel<1, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, role, lane, li, lk);
-    block_panel<2, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, role, lane, li, lk);
-    block_panel<3, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sh.pan, sh.nl, role, lane, li, lk);
+    // wave 3 below the diagonal block: the workgroup's 64-column tile of A12
+    // (workgroup w takes the block TRSM's column tile w - 1; there are as many
+    // such workgroups as tiles), rows kb .. kb + 63, read into registers now
+    // (zeros from n on) and kept there until its first solve step
+    const int cc = kb + NB * (int)blockIdx.x;
+    float4v tv[NB / 4];
+    if (!OWNER && role == ROLE_TRSM) {
+        const int q4 = 4 * (lane & 15), rq = lane >> 4;
+        const bool whole = cc + NB <= n;  // uniform
+#pragma unroll
+        for (int it = 0; it < NB / 4; ++it) {
+            const float *src = a + (int64_t)(kb + 4 * it + rq) * lda;
+            if (whole) {
+                tv[it] = *(const float4v *)(src + cc + q4);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float e = src[min(cc + q4 + r, n - 1)];
+                    tv[it][r] = cc + q4 + r < n ? e : 0.0f;
+                }
+            }
+        }
+    }
+    block_panel<0, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, tv, sh, role, lane, li, lk);
+    block_panel<1, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, tv, sh, role, lane, li, lk);
+    block_panel<2, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, tv, sh, role, lane, li, lk);
+    block_panel<3, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, tv, sh, role, lane, li, lk);
+    // the last panel's -L11 columns are complete at its barrier B
+    if (!OWNER && role == ROLE_TRSM) trsm_step<3>(sh.tx, sh.nl, a, lda, n, kb, cc, lane, li, lk);
 }
 
 // full block (kb + 64 <= n), 16-byte rows, n - kb <= 4096; a workgroup of
 // four waves per 64 rows, no workgroup waiting for another (the waves'
-// parts are above block_phase2).  In the workgroups below the diagonal block
-// the four waves then form U12 for the block's rows, workgroup w taking the
-// block TRSM's column tile w - 1 (there are as many such workgroups as
-// tiles), and write it in place, so no block TRSM follows.  The diagonal
-// block stays in blk until the trailing update (or, after the last block,
-// k_take_blk) copies it in.
+// parts are above block_phase2).  Below the diagonal block the workgroups
+// also form U12 for the block's rows (wave 3) and write it in place, so no
+// block TRSM follows.  The diagonal block stays in blk until the trailing
+// update (or, after the last block, k_take_blk) copies it in.
 __global__ __launch_bounds__(256) void k_block_spec(float *__restrict__ a, int lda, int n, int kb,
                                                     float *__restrict__ blk, int *__restrict__ flag,
                                                     int32_t *__restrict__ ipiv)
@@ -1838,21 +1893,8 @@ __global__ __launch_bounds__(256) void k_block_spec(float *__restrict__ a, int l
     __shared__ BlockShared sh;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (blockIdx.x == 0) {  // the diagonal block's rows: no U12 columns
-        block_body<true>(a, lda, n, kb, blk, flag, ipiv, sh, w, lane);
-        return;
-    }
-    // this wave's A12 columns, read while the panels run
-    const int cw = kb + NB * (int)blockIdx.x + 16 * w, lc = lane & 15, lr = lane >> 4;
-    float xv[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const float e = a[(int64_t)(kb + lr + 4 * q) * lda + min(cw + lc, n - 1)];
-        xv[q] = cw + lc < n ? e : 0.0f;
-    }
-    block_body<false>(a, lda, n, kb, blk, flag, ipiv, sh, w, lane);
-    __syncthreads();  // -L11 complete in LDS, the panel phase's LDS free
-    if (cw < n) block_trsm16(a, lda, n, kb, cw, xv, sh.tx[w], sh.nl, lane);
+    if (blockIdx.x == 0) block_body<true>(a, lda, n, kb, blk, flag, ipiv, sh, w, lane);
+    else block_body<false>(a, lda, n, kb, blk, flag, ipiv, sh, w, lane);
 }
 
 // the last block's diagonal block into the matrix (no block TRSM follows)
