@@ -1768,46 +1768,80 @@ __device__ __forceinline__ void block_phase2(float *__restrict__ a, int lda, int
     }
 }
 
-// after barrier B: the K = 16 updates of the later columns (the next
-// panel's tile column first), then the holder publishes the next top block
+// after barrier B: the K = 16 updates of the later columns that the next
+// panel needs before its barrier A (its tile column; for the diagonal
+// block's holder also its top block's row), then the holder publishes the
+// next top block.  The rest keep their operands (sfa, sfb) in registers and
+// run at the start of the next panel's phase 2 (block_deferred), still
+// before any later update of the same tiles.
 template <int P, bool OWNER>
-__device__ __forceinline__ void block_phase3(float4v (&acc)[4][4], float4v (&acct)[4][4], BlockLds &sh, int role,
-                                             int li, int lk)
+__device__ __forceinline__ void block_phase3(float4v (&acc)[4][4], float4v (&acct)[4][4], float (&sfa)[PB / 4][4],
+                                             float (&sfb)[PB / 4][4], BlockLds &sh, int role, int li, int lk)
 {
     constexpr int PA = PB + 4, PBP = 64 + 16;
     if constexpr (P < 3) {
         const bool rows = role == ROLE_ROWS, diag = !OWNER && role == ROLE_DIAG;
         if (rows || diag) {
             const float *sa = rows ? sh.a : sh.at;
-            float fa[PB / 4][4], fb[PB / 4][4];
 #pragma unroll
             for (int ks = 0; ks < PB / 4; ++ks) {
 #pragma unroll
-                for (int ti = 0; ti < 4; ++ti) fa[ks][ti] = sa[(ti * 16 + li) * PA + 4 * ks + lk];
+                for (int ti = 0; ti < 4; ++ti) sfa[ks][ti] = sa[(ti * 16 + li) * PA + 4 * ks + lk];
 #pragma unroll
-                for (int tj = P + 1; tj < 4; ++tj) fb[ks][tj] = sh.b[(4 * ks + lk) * PBP + (tj - P - 1) * 16 + li];
+                for (int tj = P + 1; tj < 4; ++tj) sfb[ks][tj] = sh.b[(4 * ks + lk) * PBP + (tj - P - 1) * 16 + li];
             }
             // (a row tile past n: its -L21 rows are staged as zeros, and it
             // is never stored)
-            if (rows) {
+            if (rows && !OWNER) {
 #pragma unroll
-                for (int tj = P + 1; tj < 4; ++tj)
+                for (int ks = 0; ks < PB / 4; ++ks)
 #pragma unroll
-                    for (int ks = 0; ks < PB / 4; ++ks)
-#pragma unroll
-                        for (int ti = OWNER ? P + 1 : 0; ti < 4; ++ti)
-                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[ks][tj], fa[ks][ti], acc[ti][tj], 0, 0, 0);
-                if constexpr (OWNER) publish_top<P + 1>(acc, sh, li, lk);
+                    for (int ti = 0; ti < 4; ++ti)
+                        acc[ti][P + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(sfb[ks][P + 1], sfa[ks][ti], acc[ti][P + 1], 0, 0, 0);
             } else {
+                float4v (&dg)[4][4] = rows ? acc : acct;  // the diagonal block
 #pragma unroll
                 for (int tj = P + 1; tj < 4; ++tj)
 #pragma unroll
                     for (int ks = 0; ks < PB / 4; ++ks)
+                        dg[P + 1][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(sfb[ks][tj], sfa[ks][P + 1], dg[P + 1][tj], 0, 0, 0);
 #pragma unroll
-                        for (int ti = P + 1; ti < 4; ++ti)
-                            acct[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[ks][tj], fa[ks][ti], acct[ti][tj], 0, 0, 0);
-                publish_top<P + 1>(acct, sh, li, lk);
+                for (int ti = P + 2; ti < 4; ++ti)
+#pragma unroll
+                    for (int ks = 0; ks < PB / 4; ++ks)
+                        dg[ti][P + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(sfb[ks][P + 1], sfa[ks][ti], dg[ti][P + 1], 0, 0, 0);
+                publish_top<P + 1>(dg, sh, li, lk);
             }
+        }
+    }
+}
+
+// the updates block_phase3 of panel Q left: the non-owner rows' tile
+// columns past Q + 1; the diagonal block's tiles below and right of its
+// row and column Q + 1
+template <int Q, bool OWNER>
+__device__ __forceinline__ void block_deferred(float4v (&acc)[4][4], float4v (&acct)[4][4], const float (&sfa)[PB / 4][4],
+                                               const float (&sfb)[PB / 4][4], int role)
+{
+    if constexpr (Q + 2 < 4) {
+        const bool rows = role == ROLE_ROWS, diag = !OWNER && role == ROLE_DIAG;
+        if (rows && !OWNER) {
+#pragma unroll
+            for (int tj = Q + 2; tj < 4; ++tj)
+#pragma unroll
+                for (int ks = 0; ks < PB / 4; ++ks)
+#pragma unroll
+                    for (int ti = 0; ti < 4; ++ti)
+                        acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(sfb[ks][tj], sfa[ks][ti], acc[ti][tj], 0, 0, 0);
+        } else if (rows || diag) {
+            float4v (&dg)[4][4] = rows ? acc : acct;
+#pragma unroll
+            for (int ti = Q + 2; ti < 4; ++ti)
+#pragma unroll
+                for (int tj = Q + 2; tj < 4; ++tj)
+#pragma unroll
+                    for (int ks = 0; ks < PB / 4; ++ks)
+                        dg[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(sfb[ks][tj], sfa[ks][ti], dg[ti][tj], 0, 0, 0);
         }
     }
 }
@@ -1815,13 +1849,15 @@ __device__ __forceinline__ void block_phase3(float4v (&acc)[4][4], float4v (&acc
 template <int P, bool OWNER>
 __device__ __forceinline__ void block_panel(float *__restrict__ a, int lda, int n, int kb, int r0, float *__restrict__ blk,
                                             int *__restrict__ flag, int32_t *__restrict__ ipiv, float4v (&acc)[4][4],
-                                            float4v (&acct)[4][4], const float4v (&tv)[NB / 4], BlockShared &sh,
-                                            int role, int lane, int li, int lk)
+                                            float4v (&acct)[4][4], float (&sfa)[PB / 4][4], float (&sfb)[PB / 4][4],
+                                            const float4v (&tv)[NB / 4], BlockShared &sh, int role, int lane, int li,
+                                            int lk)
 {
     __syncthreads();  // A: panel P's top block and A12 published
+    if constexpr (P > 0) block_deferred<P - 1, OWNER>(acc, acct, sfa, sfb, role);
     block_phase2<P, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, tv, sh, role, lane, li, lk);
     __syncthreads();  // B: U12, the staged -L21 and panel P's -L11 columns complete; the top block read
-    block_phase3<P, OWNER>(acc, acct, sh.pan, role, li, lk);
+    block_phase3<P, OWNER>(acc, acct, sfa, sfb, sh.pan, role, li, lk);
 }
 
 template <bool OWNER>
@@ -1872,10 +1908,11 @@ __device__ __forceinline__ void block_body(float *__restrict__ a, int lda, int n
             }
         }
     }
-    block_panel<0, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, tv, sh, role, lane, li, lk);
-    block_panel<1, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, tv, sh, role, lane, li, lk);
-    block_panel<2, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, tv, sh, role, lane, li, lk);
-    block_panel<3, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, tv, sh, role, lane, li, lk);
+    float sfa[PB / 4][4], sfb[PB / 4][4];  // a panel's deferred update operands
+    block_panel<0, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sfa, sfb, tv, sh, role, lane, li, lk);
+    block_panel<1, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sfa, sfb, tv, sh, role, lane, li, lk);
+    block_panel<2, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sfa, sfb, tv, sh, role, lane, li, lk);
+    block_panel<3, OWNER>(a, lda, n, kb, r0, blk, flag, ipiv, acc, acct, sfa, sfb, tv, sh, role, lane, li, lk);
     // the last panel's -L11 columns are complete at its barrier B
     if (!OWNER && role == ROLE_TRSM) trsm_step<3>(sh.tx, sh.nl, a, lda, n, kb, cc, lane, li, lk);
 }
