@@ -809,8 +809,13 @@ int piro_fe_autotune(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row
         return ms / reps;
     };
     // a short pass over every variant, then the three fastest re-timed in
-    // two interleaved rounds of longer runs (near-equal variants otherwise
-    // trade places on run-to-run noise)
+    // five interleaved rounds of longer runs, each judged by its median
+    // round (near-equal variants otherwise trade places on run-to-run
+    // noise)
+    // the first ~100 launches after assembly run slower than the steady
+    // state: settle before timing anything
+    const int settle = vec_ok(d_col, d_val) || !kVariants[kDefaultVariant].vec ? kDefaultVariant : 0;
+    for (int i = 0; i < 128; ++i) launch(FE_INTERIOR, settle, p, ctx);
     std::vector<std::pair<float, int>> first;
     for (int v = 0; v < kNumVariants; ++v) {
         if (kVariants[v].vec && !vec_ok(d_col, d_val)) continue;
@@ -819,13 +824,17 @@ int piro_fe_autotune(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row
     PIRO_REQUIRE(!first.empty(), "no variant applicable");
     std::sort(first.begin(), first.end());
     const int top = (int)std::min<size_t>(3, first.size());
-    std::vector<float> tsum(top, 0.0f);
-    for (int round = 0; round < 2; ++round)
-        for (int j = 0; j < top; ++j) tsum[j] += time_variant(first[j].second, 10);
+    constexpr int kRounds = 5;
+    std::vector<std::vector<float>> tr(top);
+    for (int round = 0; round < kRounds; ++round)
+        for (int j = 0; j < top; ++j) tr[j].push_back(time_variant(first[j].second, 20));
     int best = -1;
     float best_t = 1e30f;
-    for (int j = 0; j < top; ++j)
-        if (tsum[j] / 2 < best_t) { best_t = tsum[j] / 2; best = first[j].second; }
+    for (int j = 0; j < top; ++j) {
+        std::sort(tr[j].begin(), tr[j].end());
+        const float med = tr[j][kRounds / 2];
+        if (med < best_t) { best_t = med; best = first[j].second; }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ctx->fe_variant = best;
