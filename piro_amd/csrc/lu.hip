@@ -2217,7 +2217,7 @@ struct TrsvArgs {
     int ldin;
     float *out;             // solution rows (read back by later blocks when NRW > 1)
     int ldout, nrhs;
-    uint64_t *mb;           // NRW = 1: mailbox, n x 32 words, (tag << 32) | bits of x
+    uint64_t *mb;           // NRW = 1: mailbox, 32 x n words (right-hand side major), (tag << 32) | bits of x
     uint32_t tag;           // this launch's tag (never 0; the mailbox starts zeroed)
     int *done;              // NRW > 1: per wave (<= 8): blocks finished, in solve order; zero at launch
     int *ticket;            // zero at launch
@@ -2343,7 +2343,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
         }
         float xv[NRW];
         if constexpr (NRW == 1) {
-            xv[0] = lane < js ? mb_wait(p.mb + (int64_t)(j0 + lane) * 32 + c0, p.tag, p.err) : 0.0f;
+            xv[0] = lane < js ? mb_wait(p.mb + (int64_t)c0 * p.n + j0 + lane, p.tag, p.err) : 0.0f;
         } else {
             if (s >= avail) avail = wait_done(done, s + 1, p.err);
 #pragma unroll
@@ -2400,7 +2400,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
             if (c0 + q < p.nrhs) p.out[(int64_t)row * p.ldout + c0 + q] = x[q];
     if constexpr (NRW == 1) {
         if (live)
-            __hip_atomic_store(p.mb + (int64_t)row * 32 + c0, ((uint64_t)p.tag << 32) | __float_as_uint(x[0]),
+            __hip_atomic_store(p.mb + (int64_t)c0 * p.n + row, ((uint64_t)p.tag << 32) | __float_as_uint(x[0]),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -2417,7 +2417,7 @@ __global__ __launch_bounds__(256) void k_zero_i32(int *__restrict__ p, int count
 // forward solution and the progress counters of both passes
 struct SolvePlan {
     int n = 0, nb = 0;
-    uint64_t *mb = nullptr;  // the solves' mailbox, n x 32 words, zeroed by prepare_solve
+    uint64_t *mb = nullptr;  // the solves' mailbox, 32 x n words, zeroed by prepare_solve
     int32_t *perm = nullptr;
     float *linv = nullptr, *uinv = nullptr, *y = nullptr;
     int *sync = nullptr;  // [fwd done 8][fwd ticket][bwd done 8][bwd ticket][err]
