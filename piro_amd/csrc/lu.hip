@@ -2219,19 +2219,17 @@ struct TrsvArgs {
     int ldout, nrhs;
     uint64_t *mb;           // NRW = 1: mailbox, 32 x n words (right-hand side major), (tag << 32) | bits of x
     uint32_t tag;           // this launch's tag (never 0; the mailbox starts zeroed)
-    int *done;              // NRW > 1: per wave (<= 8): blocks finished, in solve order; zero at launch
     int *ticket;            // zero at launch
     int *err;               // set when a wait exceeds its bound
 };
 
-// Two ways for a block's x to reach the blocks below it.
-//
-// One right-hand side per wave: each value travels as one 64-bit word, the
-// launch's tag in the high half.  The word validates itself, so no
-// completion flag, fence or second read is needed; each lane waits for its
-// own word.  mb_wait returns the value once the word carries the tag; past
-// the bound (or once another wait has passed it) it raises *err and
-// returns what is there.
+// A block's x reaches the blocks below it as 64-bit mailbox words, one per
+// value, the launch's tag in the high half.  The word validates itself, so
+// no completion flag, fence or second read is needed; each lane waits for
+// its own row's words.  The mailbox is right-hand side major, so a wave's 64
+// rows of one right-hand side share four cache lines.  mb_wait returns the
+// value once the word carries the tag; past the bound (or once another wait
+// has passed it) it raises *err and returns what is there.
 __device__ __forceinline__ float mb_wait(uint64_t *w, uint32_t tag, int *err)
 {
     uint64_t v;
@@ -2247,25 +2245,33 @@ __device__ __forceinline__ float mb_wait(uint64_t *w, uint32_t tag, int *err)
     return __uint_as_float((uint32_t)v);
 }
 
-// Several right-hand sides per wave: a word per value would be polled by
-// every lane for each of them (measured slower); instead each wave counts
-// finished blocks in its own word, released after the values are stored.
-// wait_done waits until *done >= need (every lane reads the same word) and
-// returns the value seen, or need after raising *err past the bound.
-__device__ __forceinline__ int wait_done(int *done, int need, int *err)
+// NRW values of one row (right-hand sides c0 .. c0 + NRW - 1, words `stride`
+// apart), polled together; only the first `valid` are waited for (the rest
+// are 0)
+template <int NRW>
+__device__ __forceinline__ void mb_wait_n(uint64_t *w, int64_t stride, int valid, uint32_t tag, int *err,
+                                          float (&out)[NRW])
 {
-    int spins = 0, d;
-    while ((d = __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
+    uint64_t v[NRW];
+    int spins = 0;
+    for (;;) {
+        bool ok = true;
+#pragma unroll
+        for (int q = 0; q < NRW; ++q)
+            if (q < valid) {
+                v[q] = __hip_atomic_load(w + q * stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = ok && (uint32_t)(v[q] >> 32) == tag;
+            }
+        if (ok) break;
         if (++spins > TRSV_SPIN_LIMIT ||
             ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            d = need;
             break;
         }
         __builtin_amdgcn_s_sleep(1);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return d;
+#pragma unroll
+    for (int q = 0; q < NRW; ++q) out[q] = q < valid ? __uint_as_float((uint32_t)v[q]) : 0.0f;
 }
 
 // row `row` of lu, columns j0 .. j0+63 (zero from js on, and for dead rows).
@@ -2296,8 +2302,8 @@ __device__ __forceinline__ void load_row(f4 (&r)[16], const float *__restrict__ 
 // One workgroup (WAVES waves) per 64-row block, blocks in ticket order (a
 // block waits only on blocks with smaller tickets, already running).
 // Right-hand sides are independent, so each wave solves its own NRW of
-// them with no barrier after the start: it waits for block j (mailbox words
-// for NRW = 1, its done word otherwise), keeps its row of the next
+// them with no barrier after the start: it waits for block j's mailbox
+// words (its row's NRW of them, polled together), keeps its row of the next
 // off-diagonal block in registers (lane = row; loaded while it waits) and
 // takes x_j by readlane (NRW = 1) or from a wave-private LDS slice (NRW > 1).
 template <bool LOWER, int NRW, int WAVES, bool VEC>
@@ -2316,7 +2322,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
     __syncthreads();
     const int c0 = g * NRW;
     if (c0 >= p.nrhs) return;
-    int *done = p.done + g;
     const bool live = lane < bs;
     const int row = i0 + lane;
     float acc[NRW];
@@ -2330,7 +2335,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
         const int j = LOWER ? 0 : p.nb - 1;
         load_row<VEC>(cur, p.lu, p.lda, row, live, i0, j * TB, min(TB, p.n - j * TB));
     }
-    int avail = 0;
     for (int s = 0; s < tk; ++s) {
         const int j = LOWER ? s : p.nb - 1 - s;
         const int j0 = j * TB, js = min(TB, p.n - j0);
@@ -2345,10 +2349,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
         if constexpr (NRW == 1) {
             xv[0] = lane < js ? mb_wait(p.mb + (int64_t)c0 * p.n + j0 + lane, p.tag, p.err) : 0.0f;
         } else {
-            if (s >= avail) avail = wait_done(done, s + 1, p.err);
+            if (lane < js) {
+                mb_wait_n<NRW>(p.mb + (int64_t)c0 * p.n + j0 + lane, p.n, min(NRW, p.nrhs - c0), p.tag, p.err, xv);
+            } else {
 #pragma unroll
-            for (int q = 0; q < NRW; ++q)
-                xv[q] = (lane < js && c0 + q < p.nrhs) ? p.out[(int64_t)(j0 + lane) * p.ldout + c0 + q] : 0.0f;
+                for (int q = 0; q < NRW; ++q) xv[q] = 0.0f;
+            }
         }
         if constexpr (NRW == 1) {
 #pragma unroll
@@ -2403,8 +2409,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
             __hip_atomic_store(p.mb + (int64_t)c0 * p.n + row, ((uint64_t)p.tag << 32) | __float_as_uint(x[0]),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(done, tk + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (live)
+#pragma unroll
+            for (int q = 0; q < NRW; ++q)
+                if (c0 + q < p.nrhs)
+                    __hip_atomic_store(p.mb + (int64_t)(c0 + q) * p.n + row,
+                                       ((uint64_t)p.tag << 32) | __float_as_uint(x[q]), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -2420,7 +2431,7 @@ struct SolvePlan {
     uint64_t *mb = nullptr;  // the solves' mailbox, 32 x n words, zeroed by prepare_solve
     int32_t *perm = nullptr;
     float *linv = nullptr, *uinv = nullptr, *y = nullptr;
-    int *sync = nullptr;  // [fwd done 8][fwd ticket][bwd done 8][bwd ticket][err]
+    int *sync = nullptr;  // [8 spare][fwd ticket][8 spare][bwd ticket][err]
     mutable uint32_t tag = 0;  // the last launch's mailbox tag
 };
 
@@ -2480,16 +2491,16 @@ void solve_f32(piro_ctx *ctx, const SolvePlan &pl, const float *lu, int lda, int
     hipStream_t s = ctx->stream;
     const int nb = pl.nb;
     const bool vec = (lda % 4 == 0) && ((uintptr_t)lu % 16 == 0);
-    int *fwd = pl.sync, *bwd = pl.sync + 9, *err = pl.sync + 18;
+    int *fwd = pl.sync, *bwd = pl.sync + 9, *err = pl.sync + 18;  // tickets, then the error word
     for (int c0 = 0; c0 < nrhs; c0 += 32) {
         const int nc = min(32, nrhs - c0);
         k_zero_i32<<<1, 256, 0, s>>>(pl.sync, 18);
         check_launch();
         launch_trsv<true>(s, TrsvArgs{lu, lda, pl.n, nb, pl.linv, pl.perm, b + c0, ldb, pl.y, 32, nc, pl.mb, ++pl.tag,
-                                      fwd, fwd + 8, err},
+                                      fwd + 8, err},
                           vec);
         launch_trsv<false>(s, TrsvArgs{lu, lda, pl.n, nb, pl.uinv, nullptr, pl.y, 32, b + c0, ldb, nc, pl.mb, ++pl.tag,
-                                       bwd, bwd + 8, err},
+                                       bwd + 8, err},
                            vec);
     }
 }
