@@ -2419,6 +2419,102 @@ __global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
     }
 }
 
+// One right-hand side per wave, two consecutive blocks in solve order per
+// workgroup (blocks 2 tk and 2 tk + 1 of the order, waves 0..3 and 4..7, a
+// right-hand side each): the second block's waves take the first block's x
+// from LDS after one workgroup barrier instead of from the mailbox, so only
+// every other hand-off goes through memory.  The second block's products
+// with all earlier blocks run while the first block is solved.  Each value
+// is formed as k_trsv<LOWER, 1, ..> forms it (the same terms in the same
+// order).  Every wave passes the three barriers, active or not (right-hand
+// sides past nrhs, or no second block when nb is odd); the waits inside are
+// bounded.
+template <bool LOWER, bool VEC>
+__global__ __launch_bounds__(512) void k_trsv_pair(TrsvArgs p)
+{
+    __shared__ float s_dinv[2][TB][TB + 1];
+    __shared__ float s_xa[4][TB];
+    __shared__ int s_tk;
+    const int t = threadIdx.x, lane = t & 63, g = t >> 6;
+    if (t == 0) s_tk = atomicAdd(p.ticket, 1);
+    __syncthreads();
+    const int sa = 2 * s_tk;  // solve-order index of the pair's first block
+    const auto blk_of = [&](int si) { return LOWER ? si : p.nb - 1 - si; };
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+        if (sa + h < p.nb)
+            for (int e = t; e < TB * TB; e += 512) s_dinv[h][e >> 6][e & 63] = p.dinv[(int64_t)blk_of(sa + h) * TB * TB + e];
+    __syncthreads();
+    const int h = g >> 2, c0 = g & 3, sh = sa + h;
+    const bool act = sh < p.nb && c0 < p.nrhs;  // uniform per wave
+    const int b = blk_of(min(sh, p.nb - 1));
+    const int i0 = b * TB, bs = min(TB, p.n - i0);
+    const bool live = act && lane < bs;
+    const int row = i0 + lane;
+    const int ja0 = blk_of(sa) * TB, jas = min(TB, p.n - ja0);  // the first block's rows
+    float acc = 0.0f;
+    f4 cur[16];
+    if (act) {
+        {
+            const int64_t src = live ? (LOWER ? p.perm[row] : row) : 0;
+            acc = live ? p.in[src * p.ldin + c0] : 0.0f;
+        }
+        const int nprev = sa + h;  // blocks before this wave's block in solve order
+        if (nprev > 0) {
+            const int j = blk_of(0);
+            load_row<VEC>(cur, p.lu, p.lda, row, live, i0, j * TB, min(TB, p.n - j * TB));
+        }
+        for (int si = 0; si < sa; ++si) {
+            const int j = blk_of(si), j0 = j * TB, js = min(TB, p.n - j0);
+            f4 nxt[16];
+            if (si + 1 < nprev) {
+                const int jn = blk_of(si + 1);
+                load_row<VEC>(nxt, p.lu, p.lda, row, live, i0, jn * TB, min(TB, p.n - jn * TB));
+            }
+            const float xv = lane < js ? mb_wait(p.mb + (int64_t)c0 * p.n + j0 + lane, p.tag, p.err) : 0.0f;
+#pragma unroll
+            for (int jj = 0; jj < TB; ++jj) {
+                const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), jj));
+                acc -= cur[jj >> 2][jj & 3] * x;
+            }
+            if (si + 1 < nprev)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) cur[q] = nxt[q];
+        }
+        if (h == 0) {
+            // x_b = D_b^-1 acc
+            float x = 0.0f;
+#pragma unroll 8
+            for (int jj = 0; jj < TB; ++jj)
+                x += s_dinv[0][lane][jj] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), jj));
+            if (live) {
+                p.out[(int64_t)row * p.ldout + c0] = x;
+                __hip_atomic_store(p.mb + (int64_t)c0 * p.n + row, ((uint64_t)p.tag << 32) | __float_as_uint(x),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            s_xa[c0][lane] = x;
+        }
+    }
+    __syncthreads();  // the first block's x in LDS
+    if (act && h == 1) {
+        const float xv = lane < jas ? s_xa[c0][lane] : 0.0f;
+#pragma unroll
+        for (int jj = 0; jj < TB; ++jj) {
+            const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), jj));
+            acc -= cur[jj >> 2][jj & 3] * x;
+        }
+        float x = 0.0f;
+#pragma unroll 8
+        for (int jj = 0; jj < TB; ++jj)
+            x += s_dinv[1][lane][jj] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), jj));
+        if (live) {
+            p.out[(int64_t)row * p.ldout + c0] = x;
+            __hip_atomic_store(p.mb + (int64_t)c0 * p.n + row, ((uint64_t)p.tag << 32) | __float_as_uint(x),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_zero_i32(int *__restrict__ p, int count)
 {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) p[i] = 0;
@@ -2471,13 +2567,15 @@ SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const in
     return pl;
 }
 
-// up to 4 right-hand sides: one per wave; up to 32: four per wave, 8 waves
+// up to 4 right-hand sides: one per wave, two blocks per workgroup; up to
+// 32: four per wave, 8 waves
 template <bool LOWER>
 void launch_trsv(hipStream_t s, const TrsvArgs &a, bool vec)
 {
     if (a.nrhs <= 4) {
-        if (vec) k_trsv<LOWER, 1, 4, true><<<a.nb, 256, 0, s>>>(a);
-        else k_trsv<LOWER, 1, 4, false><<<a.nb, 256, 0, s>>>(a);
+        const int npairs = (a.nb + 1) / 2;
+        if (vec) k_trsv_pair<LOWER, true><<<npairs, 512, 0, s>>>(a);
+        else k_trsv_pair<LOWER, false><<<npairs, 512, 0, s>>>(a);
     } else {
         if (vec) k_trsv<LOWER, 4, 8, true><<<a.nb, 512, 0, s>>>(a);
         else k_trsv<LOWER, 4, 8, false><<<a.nb, 512, 0, s>>>(a);
