@@ -44,6 +44,14 @@ constexpr int NB = 64;  // factorisation block: 4 panels, then one K = 64 update
 // 4096^2 they are faster at every size: 4.68 -> 4.41 ms; larger updates keep
 // the 128 x 128 workgroup tiles, whose operands are re-read less)
 constexpr int kUpd32Max = 4096;
+#ifndef PIRO_PERS_GRID
+#define PIRO_PERS_GRID 2048
+#endif
+#ifndef PIRO_PERS_MIN
+#define PIRO_PERS_MIN 512
+#endif
+constexpr int kPersGrid = PIRO_PERS_GRID;  // workgroups of the persistent trailing update
+constexpr int kPersMin = PIRO_PERS_MIN;    // the smallest update (rows) given to it
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -1280,6 +1288,88 @@ __global__ __launch_bounds__(256) void k_update_mfma32(float *__restrict__ a, in
                     if (row < n && col + r < c1) a[roff[ti] + col + r] = acc[ti][tj][r];
             }
         }
+}
+
+// k_update_mfma32 for an m x m update of whole 64 x 64 tiles (rows and
+// columns c0 .. c0 + m, m a multiple of 64, 16-byte rows, K = 64) by
+// workgroups that each walk tiles t, t + G, ... (in k_update_mfma32's tile
+// order): the next tile's C, -L21 and U12 are read into registers before the
+// current tile's MFMAs, which take their operands from LDS, and staged after
+// its store.  Every C entry takes the same MFMAs in the same order.  blk:
+// as in k_update_mfma32.
+__global__ __launch_bounds__(256) void k_update_pers(float *__restrict__ a, int lda, int k, int c0g, int m,
+                                                     const float *__restrict__ blk)
+{
+    constexpr int KW = NB, NT = 256, TC = 64, PA = KW + 4, PBP = TC + 16;
+    constexpr int NQA = 64 * KW / 4 / NT, NQB = KW * TC / 4 / NT;
+    __shared__ __attribute__((aligned(16))) float s_a[64 * PA];   // -L21
+    __shared__ __attribute__((aligned(16))) float s_b[KW * PBP];  // U12
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    if (blk && blockIdx.x == 0)
+        for (int e = t; e < NB * NB; e += NT) a[(int64_t)(k + e / NB) * lda + k + e % NB] = blk[e];
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32, li = lane & 15, lk = lane >> 4;
+    const int nt = m / 64, ntiles = nt * nt, G = gridDim.x;
+    float4v c[2][2], cn[2][2], va[NQA], vb[NQB];
+    auto fetch = [&](int tile) {
+        const int r0 = c0g + (tile / nt) * 64, cc = c0g + (tile % nt) * 64;
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+                cn[ti][tj] = *(const float4v *)(a + (int64_t)(r0 + wr + ti * 16 + li) * lda + cc + wc + tj * 16 + 4 * lk);
+#pragma unroll
+        for (int it = 0; it < NQA; ++it) {
+            const int e = t + NT * it, i = e / (KW / 4), kk = 4 * (e % (KW / 4));
+            va[it] = *(const float4v *)(a + (int64_t)(r0 + i) * lda + k + kk);
+        }
+#pragma unroll
+        for (int it = 0; it < NQB; ++it) {
+            const int e = t + NT * it, kk = e / (TC / 4), j = 4 * (e % (TC / 4));
+            vb[it] = *(const float4v *)(a + (int64_t)(k + kk) * lda + cc + j);
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    for (; tile < ntiles; tile += G) {
+        // stage this tile's operands (as update_stage does)
+#pragma unroll
+        for (int it = 0; it < NQA; ++it) {
+            const int e = t + NT * it, i = e / (KW / 4), kk = 4 * (e % (KW / 4));
+            *(float4v *)(s_a + i * PA + kk) = -va[it];
+        }
+#pragma unroll
+        for (int it = 0; it < NQB; ++it) {
+            const int e = t + NT * it, kk = e / (TC / 4), j = 4 * (e % (TC / 4));
+            *(float4v *)(s_b + kk * PBP + j) = vb[it];
+        }
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) c[ti][tj] = cn[ti][tj];
+        __syncthreads();
+        if (tile + G < ntiles) fetch(tile + G);  // in flight during this tile's MFMAs
+#pragma unroll
+        for (int ks = 0; ks < KW; ks += 4) {
+            float fa[2], fb[2];
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti) fa[ti] = s_a[(wr + ti * 16 + li) * PA + ks + lk];
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) fb[tj] = s_b[(ks + lk) * PBP + wc + tj * 16 + li];
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj)
+                    c[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[tj], fa[ti], c[ti][tj], 0, 0, 0);
+        }
+        const int r0 = c0g + (tile / nt) * 64, cc = c0g + (tile % nt) * 64;
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+                *(float4v *)(a + (int64_t)(r0 + wr + ti * 16 + li) * lda + cc + wc + tj * 16 + 4 * lk) = c[ti][tj];
+        __syncthreads();  // the operands read before the next tile's are staged
+    }
 }
 
 // The in-block update of a full panel's later columns [c0, c1) (c0 = k + 16,
@@ -2810,7 +2900,11 @@ void factor_spec(hipStream_t s, int n, float *a, int lda, int32_t *ipiv, const F
                 check_launch();
             }
             const bool vu = vec && kb % 4 == 0 && c0 % 4 == 0;
-            if (n - c0 <= kUpd32Max) {
+            if (whole && vu && (n - c0) % 64 == 0 && n - c0 >= kPersMin) {
+                const int nt = (n - c0) / 64;
+                k_update_pers<<<std::min(nt * nt, kPersGrid), 256, 0, s>>>(a, lda, kb, c0, n - c0, ws.blk);
+                check_launch();
+            } else if (n - c0 <= kUpd32Max) {
                 dim3 g((n - c0 + 63) / 64, (n - c0 + 63) / 64);
                 k_update_mfma32<NB><<<g, 256, 0, s>>>(a, lda, n, kb, nbk, c0, c0, n, vu, whole ? ws.blk : nullptr);
                 check_launch();
