@@ -203,16 +203,19 @@ def test_lu_refactoring_one_buffer(ctx, n):
         ref_ctx.close()
 
 
-def test_lu_solve_f32_matches_numpy(ctx):
-    n = 300
-    rng = np.random.default_rng(3)
+@pytest.mark.parametrize("n,nrhs", [(300, 1), (300, 5), (300, 32), (1000, 3), (1000, 17)])
+def test_lu_solve_f32_matches_numpy(ctx, n, nrhs):
+    # every solve form (one right-hand side per wave in block pairs; MFMA
+    # tiles for more than four), a partial last block, row exchanges
+    print(f"case lu solve n={n} nrhs={nrhs}", flush=True)
+    rng = np.random.default_rng(3 + n + nrhs)
     a = (rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)).astype(np.float32)
-    b = rng.standard_normal((n, 5)).astype(np.float32)
+    b = rng.standard_normal((n, nrhs)).astype(np.float32)
     d = dev(a, ctx)
     ipiv = ctx.lu_factor(d, n)
     db = dev(b, ctx)
     ctx.lu_solve_f32(d, n, ipiv, db)
-    x = db.host().reshape(n, 5)
+    x = db.host().reshape(n, nrhs)
     xr = np.linalg.solve(a.astype(np.float64), b.astype(np.float64))
     assert po.rel_l2(x, xr) < 1e-4
 
