@@ -137,7 +137,7 @@ def exchange_mix(n, seed):
     return a
 
 
-@pytest.mark.parametrize("n", [300, 1000, 2100, 4096])
+@pytest.mark.parametrize("n", [128, 300, 1000, 1024, 2100, 4096])
 @pytest.mark.parametrize("kind", ["random", "mixed", "dominant"])
 def test_lu_strategies_bit_identical(ctx, n, kind):
     # the speculative exchange-free panels and the pivoting panels give the
