@@ -2305,9 +2305,9 @@ struct TrsvArgs {
     const int32_t *perm;    // forward pass: b row of solution row i; else null
     const float *in;        // right-hand side rows
     int ldin;
-    float *out;             // solution rows (read back by later blocks when NRW > 1)
+    float *out;             // solution rows
     int ldout, nrhs;
-    uint64_t *mb;           // NRW = 1: mailbox, 32 x n words (right-hand side major), (tag << 32) | bits of x
+    uint64_t *mb;           // mailbox, 32 x n words (right-hand side major), (tag << 32) | bits of x
     uint32_t tag;           // this launch's tag (never 0; the mailbox starts zeroed)
     int *ticket;            // zero at launch
     int *err;               // set when a wait exceeds its bound
@@ -2335,35 +2335,6 @@ __device__ __forceinline__ float mb_wait(uint64_t *w, uint32_t tag, int *err)
     return __uint_as_float((uint32_t)v);
 }
 
-// NRW values of one row (right-hand sides c0 .. c0 + NRW - 1, words `stride`
-// apart), polled together; only the first `valid` are waited for (the rest
-// are 0)
-template <int NRW>
-__device__ __forceinline__ void mb_wait_n(uint64_t *w, int64_t stride, int valid, uint32_t tag, int *err,
-                                          float (&out)[NRW])
-{
-    uint64_t v[NRW];
-    int spins = 0;
-    for (;;) {
-        bool ok = true;
-#pragma unroll
-        for (int q = 0; q < NRW; ++q)
-            if (q < valid) {
-                v[q] = __hip_atomic_load(w + q * stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = ok && (uint32_t)(v[q] >> 32) == tag;
-            }
-        if (ok) break;
-        if (++spins > TRSV_SPIN_LIMIT ||
-            ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-#pragma unroll
-    for (int q = 0; q < NRW; ++q) out[q] = q < valid ? __uint_as_float((uint32_t)v[q]) : 0.0f;
-}
-
 // row `row` of lu, columns j0 .. j0+63 (zero from js on, and for dead rows).
 // The loads are unconditional (a dead lane reads row `any`, a live row) so
 // that all of them are in flight together; the zeros are selected after.
@@ -2389,136 +2360,19 @@ __device__ __forceinline__ void load_row(f4 (&r)[16], const float *__restrict__ 
     }
 }
 
-// One workgroup (WAVES waves) per 64-row block, blocks in ticket order (a
-// block waits only on blocks with smaller tickets, already running).
-// Right-hand sides are independent, so each wave solves its own NRW of
-// them with no barrier after the start: it waits for block j's mailbox
-// words (its row's NRW of them, polled together), keeps its row of the next
-// off-diagonal block in registers (lane = row; loaded while it waits) and
-// takes x_j by readlane (NRW = 1) or from a wave-private LDS slice (NRW > 1).
-template <bool LOWER, int NRW, int WAVES, bool VEC>
-__global__ __launch_bounds__(64 * WAVES) void k_trsv(TrsvArgs p)
-{
-    __shared__ float s_dinv[TB][TB + 1];
-    __shared__ float s_x[WAVES][TB][NRW];
-    __shared__ int s_tk;
-    const int t = threadIdx.x, lane = t & 63, g = t >> 6;
-    if (t == 0) s_tk = atomicAdd(p.ticket, 1);
-    __syncthreads();
-    const int tk = s_tk;
-    const int b = LOWER ? tk : p.nb - 1 - tk;
-    const int i0 = b * TB, bs = min(TB, p.n - i0);
-    for (int e = t; e < TB * TB; e += 64 * WAVES) s_dinv[e >> 6][e & 63] = p.dinv[(int64_t)b * TB * TB + e];
-    __syncthreads();
-    const int c0 = g * NRW;
-    if (c0 >= p.nrhs) return;
-    const bool live = lane < bs;
-    const int row = i0 + lane;
-    float acc[NRW];
-    {
-        const int64_t src = live ? (LOWER ? p.perm[row] : row) : 0;
-#pragma unroll
-        for (int q = 0; q < NRW; ++q) acc[q] = (live && c0 + q < p.nrhs) ? p.in[src * p.ldin + c0 + q] : 0.0f;
-    }
-    f4 cur[16];
-    if (tk > 0) {
-        const int j = LOWER ? 0 : p.nb - 1;
-        load_row<VEC>(cur, p.lu, p.lda, row, live, i0, j * TB, min(TB, p.n - j * TB));
-    }
-    for (int s = 0; s < tk; ++s) {
-        const int j = LOWER ? s : p.nb - 1 - s;
-        const int j0 = j * TB, js = min(TB, p.n - j0);
-        // the following block's row is in flight during this block's wait
-        // and products
-        f4 nxt[16];
-        if (s + 1 < tk) {
-            const int jn = LOWER ? s + 1 : p.nb - 2 - s;
-            load_row<VEC>(nxt, p.lu, p.lda, row, live, i0, jn * TB, min(TB, p.n - jn * TB));
-        }
-        float xv[NRW];
-        if constexpr (NRW == 1) {
-            xv[0] = lane < js ? mb_wait(p.mb + (int64_t)c0 * p.n + j0 + lane, p.tag, p.err) : 0.0f;
-        } else {
-            if (lane < js) {
-                mb_wait_n<NRW>(p.mb + (int64_t)c0 * p.n + j0 + lane, p.n, min(NRW, p.nrhs - c0), p.tag, p.err, xv);
-            } else {
-#pragma unroll
-                for (int q = 0; q < NRW; ++q) xv[q] = 0.0f;
-            }
-        }
-        if constexpr (NRW == 1) {
-#pragma unroll
-            for (int jj = 0; jj < TB; ++jj) {
-                const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv[0]), jj));
-                acc[0] -= cur[jj >> 2][jj & 3] * x;
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < NRW; ++q) s_x[g][lane][q] = xv[q];
-#pragma unroll
-            for (int jj = 0; jj < TB; ++jj) {
-                const float l = cur[jj >> 2][jj & 3];
-#pragma unroll
-                for (int q = 0; q < NRW; ++q) acc[q] -= l * s_x[g][jj][q];
-                // keep the LDS reads next to their use (hoisting them all
-                // spills): the accumulators pass through an empty asm that
-                // later memory accesses may not cross
-                if ((jj & 3) == 3)
-#pragma unroll
-                    for (int q = 0; q < NRW; ++q) __asm__ volatile("" : "+v"(acc[q])::"memory");
-            }
-        }
-        if (s + 1 < tk)
-#pragma unroll
-            for (int q = 0; q < 16; ++q) cur[q] = nxt[q];
-    }
-    // x_b = D_b^-1 acc
-    float x[NRW];
-#pragma unroll
-    for (int q = 0; q < NRW; ++q) x[q] = 0.0f;
-    if constexpr (NRW == 1) {
-#pragma unroll 8
-        for (int jj = 0; jj < TB; ++jj)
-            x[0] += s_dinv[lane][jj] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc[0]), jj));
-    } else {
-#pragma unroll
-        for (int q = 0; q < NRW; ++q) s_x[g][lane][q] = acc[q];
-#pragma unroll 8
-        for (int jj = 0; jj < TB; ++jj) {
-            const float d = s_dinv[lane][jj];
-#pragma unroll
-            for (int q = 0; q < NRW; ++q) x[q] += d * s_x[g][jj][q];
-        }
-    }
-    if (live)
-#pragma unroll
-        for (int q = 0; q < NRW; ++q)
-            if (c0 + q < p.nrhs) p.out[(int64_t)row * p.ldout + c0 + q] = x[q];
-    if constexpr (NRW == 1) {
-        if (live)
-            __hip_atomic_store(p.mb + (int64_t)c0 * p.n + row, ((uint64_t)p.tag << 32) | __float_as_uint(x[0]),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        if (live)
-#pragma unroll
-            for (int q = 0; q < NRW; ++q)
-                if (c0 + q < p.nrhs)
-                    __hip_atomic_store(p.mb + (int64_t)(c0 + q) * p.n + row,
-                                       ((uint64_t)p.tag << 32) | __float_as_uint(x[q]), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
-// One right-hand side per wave, two consecutive blocks in solve order per
-// workgroup (blocks 2 tk and 2 tk + 1 of the order, waves 0..3 and 4..7, a
-// right-hand side each): the second block's waves take the first block's x
-// from LDS after one workgroup barrier instead of from the mailbox, so only
-// every other hand-off goes through memory.  The second block's products
-// with all earlier blocks run while the first block is solved.  Each value
-// is formed as k_trsv<LOWER, 1, ..> forms it (the same terms in the same
-// order).  Every wave passes the three barriers, active or not (right-hand
-// sides past nrhs, or no second block when nb is odd); the waits inside are
-// bounded.
+// The triangular solves: one right-hand side per wave, two consecutive
+// blocks in solve order per workgroup (waves 0..3 and 4..7), four
+// right-hand sides per workgroup (ticket tk takes the pair tk / groups and
+// right-hand sides 4 (tk % groups) .. + 3), so the workgroups are taken in
+// solve order and each waits only for workgroups with smaller tickets,
+// which are already running.  The second block's waves take the first
+// block's x from LDS after one workgroup barrier instead of from the
+// mailbox, so only every other hand-off goes through memory; its products
+// with all earlier blocks run while the first block is solved.  A row's
+// terms are subtracted in ascending block and column order, unfused, then
+// multiplied by the diagonal block's inverse.  Every wave passes the three
+// barriers, active or not (right-hand sides past nrhs, or no second block
+// when nb is odd); the waits inside are bounded.
 template <bool LOWER, bool VEC>
 __global__ __launch_bounds__(512) void k_trsv_pair(TrsvArgs p)
 {
@@ -2528,14 +2382,15 @@ __global__ __launch_bounds__(512) void k_trsv_pair(TrsvArgs p)
     const int t = threadIdx.x, lane = t & 63, g = t >> 6;
     if (t == 0) s_tk = atomicAdd(p.ticket, 1);
     __syncthreads();
-    const int sa = 2 * s_tk;  // solve-order index of the pair's first block
+    const int groups = (p.nrhs + 3) / 4;
+    const int sa = 2 * (s_tk / groups);  // solve-order index of the pair's first block
     const auto blk_of = [&](int si) { return LOWER ? si : p.nb - 1 - si; };
 #pragma unroll
     for (int h = 0; h < 2; ++h)
         if (sa + h < p.nb)
             for (int e = t; e < TB * TB; e += 512) s_dinv[h][e >> 6][e & 63] = p.dinv[(int64_t)blk_of(sa + h) * TB * TB + e];
     __syncthreads();
-    const int h = g >> 2, c0 = g & 3, sh = sa + h;
+    const int h = g >> 2, q = g & 3, c0 = 4 * (s_tk % groups) + q, sh = sa + h;
     const bool act = sh < p.nb && c0 < p.nrhs;  // uniform per wave
     const int b = blk_of(min(sh, p.nb - 1));
     const int i0 = b * TB, bs = min(TB, p.n - i0);
@@ -2582,12 +2437,12 @@ __global__ __launch_bounds__(512) void k_trsv_pair(TrsvArgs p)
                 __hip_atomic_store(p.mb + (int64_t)c0 * p.n + row, ((uint64_t)p.tag << 32) | __float_as_uint(x),
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            s_xa[c0][lane] = x;
+            s_xa[q][lane] = x;
         }
     }
     __syncthreads();  // the first block's x in LDS
     if (act && h == 1) {
-        const float xv = lane < jas ? s_xa[c0][lane] : 0.0f;
+        const float xv = lane < jas ? s_xa[q][lane] : 0.0f;
 #pragma unroll
         for (int jj = 0; jj < TB; ++jj) {
             const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), jj));
@@ -2657,19 +2512,14 @@ SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const in
     return pl;
 }
 
-// up to 4 right-hand sides: one per wave, two blocks per workgroup; up to
-// 32: four per wave, 8 waves
+// up to 32 right-hand sides: a workgroup per pair of blocks and four
+// right-hand sides
 template <bool LOWER>
 void launch_trsv(hipStream_t s, const TrsvArgs &a, bool vec)
 {
-    if (a.nrhs <= 4) {
-        const int npairs = (a.nb + 1) / 2;
-        if (vec) k_trsv_pair<LOWER, true><<<npairs, 512, 0, s>>>(a);
-        else k_trsv_pair<LOWER, false><<<npairs, 512, 0, s>>>(a);
-    } else {
-        if (vec) k_trsv<LOWER, 4, 8, true><<<a.nb, 512, 0, s>>>(a);
-        else k_trsv<LOWER, 4, 8, false><<<a.nb, 512, 0, s>>>(a);
-    }
+    const int grid = (a.nb + 1) / 2 * ((a.nrhs + 3) / 4);
+    if (vec) k_trsv_pair<LOWER, true><<<grid, 512, 0, s>>>(a);
+    else k_trsv_pair<LOWER, false><<<grid, 512, 0, s>>>(a);
     check_launch();
 }
 
