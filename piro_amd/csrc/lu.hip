@@ -2365,14 +2365,17 @@ __device__ __forceinline__ void load_row(f4 (&r)[16], const float *__restrict__ 
 // right-hand sides per workgroup (ticket tk takes the pair tk / groups and
 // right-hand sides 4 (tk % groups) .. + 3), so the workgroups are taken in
 // solve order and each waits only for workgroups with smaller tickets,
-// which are already running.  The second block's waves take the first
-// block's x from LDS after one workgroup barrier instead of from the
-// mailbox, so only every other hand-off goes through memory; its products
-// with all earlier blocks run while the first block is solved.  A row's
-// terms are subtracted in ascending block and column order, unfused, then
-// multiplied by the diagonal block's inverse.  Every wave passes the three
-// barriers, active or not (right-hand sides past nrhs, or no second block
-// when nb is odd); the waits inside are bounded.
+// which are already running.  Each wave keeps its rows of the next earlier
+// block's tile in registers, loaded during the current one's wait.  The
+// second block's waves take the first block's x from LDS after one
+// workgroup barrier instead of from the mailbox, so only every other
+// hand-off goes through memory; its products with all earlier blocks run
+// while the first block is solved.  A row's terms are subtracted in
+// ascending block and column order, unfused, then multiplied by the
+// diagonal block's inverse.  Every wave passes the three barriers, active
+// or not (right-hand sides past nrhs, or no second block when nb is odd);
+// the waits inside are bounded.  For up to four right-hand sides (one
+// workgroup per pair); k_trsv_pair_lds serves more.
 template <bool LOWER, bool VEC>
 __global__ __launch_bounds__(512) void k_trsv_pair(TrsvArgs p)
 {
@@ -2460,6 +2463,127 @@ __global__ __launch_bounds__(512) void k_trsv_pair(TrsvArgs p)
     }
 }
 
+// k_trsv_pair for more than four right-hand sides, where several
+// workgroups (one per four right-hand sides) take each pair of blocks and
+// their waves' own loads of the same tiles (32 per tile) would dominate:
+// step si subtracts the products with block blk_of(si), and the workgroup
+// loads both of its blocks' 64 x 64 tiles of that block column into LDS
+// once (double-buffered; step si + 1's are in registers during step si's
+// wait and products), each wave reading its rows from there.  The terms,
+// their order and the hand-offs are k_trsv_pair's, so the bits are too.
+// Every wave passes every barrier (2 + sa + 1 of them, sa uniform), active
+// or not; the waits inside are bounded.
+template <bool LOWER, bool VEC>
+__global__ __launch_bounds__(512) void k_trsv_pair_lds(TrsvArgs p)
+{
+    constexpr int LP = TB + 4;  // tile row pitch: the 16-byte reads of 16 lanes hit distinct banks
+    __shared__ float s_dinv[2][TB][TB + 1];
+    __shared__ __attribute__((aligned(16))) float s_l[2][2][TB][LP];  // [buffer][block of the pair][row][column]
+    __shared__ float s_xa[4][TB];
+    __shared__ int s_tk;
+    const int t = threadIdx.x, lane = t & 63, g = t >> 6;
+    if (t == 0) s_tk = atomicAdd(p.ticket, 1);
+    __syncthreads();
+    const int groups = (p.nrhs + 3) / 4;
+    const int sa = 2 * (s_tk / groups);  // solve-order index of the pair's first block
+    const auto blk_of = [&](int si) { return LOWER ? si : p.nb - 1 - si; };
+    // step si's two tiles, four 16-byte pieces a thread (rows past n, and a
+    // missing second block, as zeros; columns past n as zeros)
+    f4 pre[4];
+    const auto tile_load = [&](int si) {
+        const int j0 = blk_of(si) * TB, js = min(TB, p.n - j0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = t + 512 * k, hh = e >> 10, r = (e >> 4) & 63, c = 4 * (e & 15);
+            const int rr = blk_of(min(sa + hh, p.nb - 1)) * TB + r;
+            const bool ok = sa + hh < p.nb && rr < p.n;
+            const float *src = p.lu + (int64_t)(ok ? rr : 0) * p.lda + j0;
+            if (VEC && js == TB) {
+                pre[k] = *(const f4 *)(src + c);
+                pre[k] = ok ? pre[k] : f4{0.0f, 0.0f, 0.0f, 0.0f};
+            } else {
+#pragma unroll
+                for (int e2 = 0; e2 < 4; ++e2) pre[k][e2] = src[min(c + e2, js - 1)];
+#pragma unroll
+                for (int e2 = 0; e2 < 4; ++e2) pre[k][e2] = (ok && c + e2 < js) ? pre[k][e2] : 0.0f;
+            }
+        }
+    };
+    const auto tile_store = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = t + 512 * k, hh = e >> 10, r = (e >> 4) & 63, c = 4 * (e & 15);
+            *(f4 *)&s_l[buf][hh][r][c] = pre[k];
+        }
+    };
+    tile_load(0);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+        if (sa + h < p.nb)
+            for (int e = t; e < TB * TB; e += 512) s_dinv[h][e >> 6][e & 63] = p.dinv[(int64_t)blk_of(sa + h) * TB * TB + e];
+    tile_store(0);
+    __syncthreads();  // the inverses and step 0's tiles
+    const int h = g >> 2, q = g & 3, c0 = 4 * (s_tk % groups) + q, sh = sa + h;
+    const bool act = sh < p.nb && c0 < p.nrhs;  // uniform per wave
+    const int b = blk_of(min(sh, p.nb - 1));
+    const int i0 = b * TB, bs = min(TB, p.n - i0);
+    const bool live = act && lane < bs;
+    const int row = i0 + lane;
+    const int ja0 = blk_of(sa) * TB, jas = min(TB, p.n - ja0);  // the first block's rows
+    float acc = 0.0f;
+    if (act) {
+        const int64_t src = live ? (LOWER ? p.perm[row] : row) : 0;
+        acc = live ? p.in[src * p.ldin + c0] : 0.0f;
+    }
+    // acc -= (this wave's rows of the tile in buffer buf) x, x in lanes
+    const auto product = [&](int buf, float xv) {
+        const float *lr = &s_l[buf][h][lane][0];
+#pragma unroll
+        for (int k = 0; k < TB / 4; ++k) {
+            const f4 l4 = *(const f4 *)(lr + 4 * k);
+#pragma unroll
+            for (int e2 = 0; e2 < 4; ++e2)
+                acc -= l4[e2] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), 4 * k + e2));
+        }
+    };
+    for (int si = 0; si < sa; ++si) {
+        tile_load(si + 1);  // step sa's: only the second block's tile is used
+        if (act) {
+            const int j0 = blk_of(si) * TB, js = min(TB, p.n - j0);
+            const float xv = lane < js ? mb_wait(p.mb + (int64_t)c0 * p.n + j0 + lane, p.tag, p.err) : 0.0f;
+            product(si & 1, xv);
+        }
+        tile_store((si + 1) & 1);
+        __syncthreads();  // step si + 1's tiles; step si's buffer free
+    }
+    if (act && h == 0) {
+        // x_b = D_b^-1 acc
+        float x = 0.0f;
+#pragma unroll 8
+        for (int jj = 0; jj < TB; ++jj)
+            x += s_dinv[0][lane][jj] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), jj));
+        if (live) {
+            p.out[(int64_t)row * p.ldout + c0] = x;
+            __hip_atomic_store(p.mb + (int64_t)c0 * p.n + row, ((uint64_t)p.tag << 32) | __float_as_uint(x),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_xa[q][lane] = x;
+    }
+    __syncthreads();  // the first block's x in LDS
+    if (act && h == 1) {
+        product(sa & 1, lane < jas ? s_xa[q][lane] : 0.0f);
+        float x = 0.0f;
+#pragma unroll 8
+        for (int jj = 0; jj < TB; ++jj)
+            x += s_dinv[1][lane][jj] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), jj));
+        if (live) {
+            p.out[(int64_t)row * p.ldout + c0] = x;
+            __hip_atomic_store(p.mb + (int64_t)c0 * p.n + row, ((uint64_t)p.tag << 32) | __float_as_uint(x),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_zero_i32(int *__restrict__ p, int count)
 {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) p[i] = 0;
@@ -2513,13 +2637,18 @@ SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const in
 }
 
 // up to 32 right-hand sides: a workgroup per pair of blocks and four
-// right-hand sides
+// right-hand sides, the tiles shared through LDS beyond four
 template <bool LOWER>
 void launch_trsv(hipStream_t s, const TrsvArgs &a, bool vec)
 {
     const int grid = (a.nb + 1) / 2 * ((a.nrhs + 3) / 4);
-    if (vec) k_trsv_pair<LOWER, true><<<grid, 512, 0, s>>>(a);
-    else k_trsv_pair<LOWER, false><<<grid, 512, 0, s>>>(a);
+    if (a.nrhs <= 4) {
+        if (vec) k_trsv_pair<LOWER, true><<<grid, 512, 0, s>>>(a);
+        else k_trsv_pair<LOWER, false><<<grid, 512, 0, s>>>(a);
+    } else {
+        if (vec) k_trsv_pair_lds<LOWER, true><<<grid, 512, 0, s>>>(a);
+        else k_trsv_pair_lds<LOWER, false><<<grid, 512, 0, s>>>(a);
+    }
     check_launch();
 }
 

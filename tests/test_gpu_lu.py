@@ -205,8 +205,9 @@ def test_lu_refactoring_one_buffer(ctx, n):
 
 @pytest.mark.parametrize("n,nrhs", [(300, 1), (300, 5), (300, 32), (1000, 3), (1000, 17)])
 def test_lu_solve_f32_matches_numpy(ctx, n, nrhs):
-    # every solve form (one right-hand side per wave in block pairs; MFMA
-    # tiles for more than four), a partial last block, row exchanges
+    # both solve forms (block pairs, one right-hand side per wave; up to four
+    # right-hand sides with each wave's tiles in registers, more with the
+    # tiles shared through LDS), a partial last block, row exchanges
     print(f"case lu solve n={n} nrhs={nrhs}", flush=True)
     rng = np.random.default_rng(3 + n + nrhs)
     a = (rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)).astype(np.float32)
@@ -218,6 +219,26 @@ def test_lu_solve_f32_matches_numpy(ctx, n, nrhs):
     x = db.host().reshape(n, nrhs)
     xr = np.linalg.solve(a.astype(np.float64), b.astype(np.float64))
     assert po.rel_l2(x, xr) < 1e-4
+
+
+@pytest.mark.parametrize("n", [1000, 4096])
+def test_lu_solve_forms_bit_identical(ctx, n):
+    # a right-hand side's solution does not depend on the form that solves it
+    # or on the other right-hand sides: the first four columns of an 8-column
+    # solve (tiles shared through LDS, two workgroups per pair of blocks)
+    # equal a 4-column solve (tiles in registers) bit for bit
+    print(f"case lu solve forms n={n}", flush=True)
+    rng = np.random.default_rng(11 + n)
+    a = (rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)).astype(np.float32)
+    b = rng.standard_normal((n, 8)).astype(np.float32)
+    d = dev(a, ctx)
+    ipiv = ctx.lu_factor(d, n)
+    out = []
+    for nrhs in (4, 8):
+        db = dev(np.ascontiguousarray(b[:, :nrhs]), ctx)
+        ctx.lu_solve_f32(d, n, ipiv, db)
+        out.append(db.host().reshape(n, nrhs))
+    assert out[0].tobytes() == np.ascontiguousarray(out[1][:, :4]).tobytes()
 
 
 def be_system(n_side=64, alpha=0.05, r=2.0):
