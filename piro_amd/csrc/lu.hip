@@ -2252,8 +2252,9 @@ constexpr int TRSV_SPIN_LIMIT = 1 << 22;
 
 // inverses of the diagonal blocks: blockIdx.x = 2 b + u; u = 0: L (unit
 // lower), u = 1: U (upper with its diagonal).  Output row-major 64 x 64 per
-// block, zero outside the block's bs x bs part.  Lane = row; each wave forms
-// 16 columns, 4 at a time, with the x_j broadcast through readlane.
+// block, zero outside the block's bs x bs part.  Lane = row, its row of the
+// block in registers; each wave forms 16 columns, 4 at a time, with the x_j
+// broadcast through readlane, the steps unrolled.
 __global__ __launch_bounds__(256) void k_diag_inv(const float *__restrict__ lu, int lda, int n,
                                                   float *__restrict__ linv, float *__restrict__ uinv)
 {
@@ -2269,13 +2270,19 @@ __global__ __launch_bounds__(256) void k_diag_inv(const float *__restrict__ lu, 
     __syncthreads();
     float *out = (upper ? uinv : linv) + (int64_t)b * TB * TB;
     const float dr = (lane < bs && upper) ? 1.0f / s_d[lane][lane] : 1.0f;
+    float rowv[TB];
+#pragma unroll
+    for (int j = 0; j < TB; ++j) rowv[j] = s_d[lane][j];
     for (int cg = w * 16; cg < w * 16 + 16; cg += 4) {
         float v[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = (lane == cg + q && lane < bs) ? 1.0f : 0.0f;
         if (!upper) {
-            for (int j = cg; j < bs; ++j) {
-                const float l = lane > j ? s_d[lane][j] : 0.0f;
+            // steps j = cg .. bs - 1
+#pragma unroll
+            for (int j = 0; j < TB; ++j) {
+                if (j < cg || j >= bs) continue;  // uniform
+                const float l = lane > j ? rowv[j] : 0.0f;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const float xj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[q]), j));
@@ -2283,8 +2290,12 @@ __global__ __launch_bounds__(256) void k_diag_inv(const float *__restrict__ lu, 
                 }
             }
         } else {
-            for (int j = min(cg + 3, bs - 1); j >= 0; --j) {
-                const float u = lane < j ? s_d[lane][j] : 0.0f;
+            // steps j = min(cg + 3, bs - 1) .. 0
+            const int jt = min(cg + 3, bs - 1);
+#pragma unroll
+            for (int j = TB - 1; j >= 0; --j) {
+                if (j > jt) continue;  // uniform
+                const float u = lane < j ? rowv[j] : 0.0f;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     if (lane == j) v[q] *= dr;
