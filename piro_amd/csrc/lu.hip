@@ -2250,17 +2250,18 @@ __global__ __launch_bounds__(256) void k_csr_to_dense(int n_rows, int n_cols, co
 constexpr int TB = 64;
 constexpr int TRSV_SPIN_LIMIT = 1 << 22;
 
-// inverses of the diagonal blocks: blockIdx.x = 2 b + u; u = 0: L (unit
-// lower), u = 1: U (upper with its diagonal).  Output row-major 64 x 64 per
+// inverses of the diagonal blocks: blockIdx.x = 4 (2 b + u) + part; u = 0:
+// L (unit lower), u = 1: U (upper with its diagonal); the four workgroups of
+// a block form columns 16 part .. 16 part + 15.  Output row-major 64 x 64 per
 // block, zero outside the block's bs x bs part.  Lane = row, its row of the
-// block in registers; each wave forms 16 columns, 4 at a time, with the x_j
-// broadcast through readlane, the steps unrolled.
+// block in registers; each wave forms 4 columns, with the x_j broadcast
+// through readlane, the steps unrolled.
 __global__ __launch_bounds__(256) void k_diag_inv(const float *__restrict__ lu, int lda, int n,
                                                   float *__restrict__ linv, float *__restrict__ uinv)
 {
     __shared__ float s_d[TB][TB + 1];
-    const int b = blockIdx.x >> 1;
-    const bool upper = blockIdx.x & 1;
+    const int b = blockIdx.x >> 3, part = blockIdx.x & 3;
+    const bool upper = (blockIdx.x >> 2) & 1;
     const int i0 = b * TB, bs = min(TB, n - i0);
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     for (int e = t; e < TB * TB; e += 256) {
@@ -2273,7 +2274,8 @@ __global__ __launch_bounds__(256) void k_diag_inv(const float *__restrict__ lu, 
     float rowv[TB];
 #pragma unroll
     for (int j = 0; j < TB; ++j) rowv[j] = s_d[lane][j];
-    for (int cg = w * 16; cg < w * 16 + 16; cg += 4) {
+    {
+        const int cg = 16 * part + 4 * w;
         float v[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = (lane == cg + q && lane < bs) ? 1.0f : 0.0f;
@@ -2638,7 +2640,7 @@ SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const in
     }
     PIRO_CHECK_HIP(hipMemcpyAsync(pl.perm, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
     PIRO_CHECK_HIP(hipStreamSynchronize(s));
-    k_diag_inv<<<2 * pl.nb, 256, 0, s>>>(lu, lda, n, pl.linv, pl.uinv);
+    k_diag_inv<<<8 * pl.nb, 256, 0, s>>>(lu, lda, n, pl.linv, pl.uinv);
     check_launch();
     k_zero_i32<<<1, 256, 0, s>>>(pl.sync + 18, 1);
     check_launch();
