@@ -50,6 +50,9 @@ int piro_ctx_destroy(piro_ctx *ctx)
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->lu_host) (void)hipHostFree(ctx->lu_host);
+    if (ctx->lu_piv_ev) (void)hipEventDestroy(ctx->lu_piv_ev);
+    if (ctx->lu_perm_ev) (void)hipEventDestroy(ctx->lu_perm_ev);
     delete ctx;
     PIRO_API_END
 }

@@ -2628,24 +2628,42 @@ SolvePlan prepare_solve(piro_ctx *ctx, int n, const float *lu, int lda, const in
     pl.uinv = pl.linv + blk;
     pl.perm = (int32_t *)(pl.uinv + blk);
     pl.sync = pl.perm + n;
-    // P from the exchange sequence: row i of P b is row perm[i] of b
-    std::vector<int32_t> piv(n), perm(n);
-    PIRO_CHECK_HIP(hipMemcpyAsync(piv.data(), d_ipiv, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-    PIRO_CHECK_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < n; ++i) perm[i] = i;
-    for (int i = 0; i < n; ++i) {
-        const int q = piv[i];
-        if (q < i || q >= n) throw Error(PIRO_EINVAL, "pivot index out of range");
-        std::swap(perm[i], perm[q]);
+    // P from the exchange sequence: row i of P b is row perm[i] of b.  The
+    // pivots go to pinned host memory; the host waits for that copy alone
+    // (an event), while the device forms the diagonal inverses behind it,
+    // and the permutation goes back without a wait (the solves follow it in
+    // stream order).  The buffers stay with the context: the previous
+    // call's copy back has finished before they are written again.
+    if (!ctx->lu_piv_ev) {
+        PIRO_CHECK_HIP(hipEventCreateWithFlags(&ctx->lu_piv_ev, hipEventDisableTiming));
+        PIRO_CHECK_HIP(hipEventCreateWithFlags(&ctx->lu_perm_ev, hipEventDisableTiming));
     }
-    PIRO_CHECK_HIP(hipMemcpyAsync(pl.perm, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
-    PIRO_CHECK_HIP(hipStreamSynchronize(s));
+    PIRO_CHECK_HIP(hipEventSynchronize(ctx->lu_perm_ev));
+    if (ctx->lu_host_n < n) {
+        if (ctx->lu_host) PIRO_CHECK_HIP(hipHostFree(ctx->lu_host));
+        ctx->lu_host = nullptr;
+        ctx->lu_host_n = 0;
+        PIRO_CHECK_HIP(hipHostMalloc((void **)&ctx->lu_host, (size_t)n * 2 * sizeof(int32_t), hipHostMallocDefault));
+        ctx->lu_host_n = n;
+    }
+    int32_t *piv = ctx->lu_host, *perm = ctx->lu_host + ctx->lu_host_n;
+    PIRO_CHECK_HIP(hipMemcpyAsync(piv, d_ipiv, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    PIRO_CHECK_HIP(hipEventRecord(ctx->lu_piv_ev, s));
     k_diag_inv<<<8 * pl.nb, 256, 0, s>>>(lu, lda, n, pl.linv, pl.uinv);
     check_launch();
     k_zero_i32<<<1, 256, 0, s>>>(pl.sync + 18, 1);
     check_launch();
     k_zero_i32<<<(int)std::min<int64_t>(1024, ((int64_t)n * 64 + 255) / 256), 256, 0, s>>>((int *)pl.mb, n * 64);
     check_launch();
+    PIRO_CHECK_HIP(hipEventSynchronize(ctx->lu_piv_ev));
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int i = 0; i < n; ++i) {
+        const int q = piv[i];
+        if (q < i || q >= n) throw Error(PIRO_EINVAL, "pivot index out of range");
+        std::swap(perm[i], perm[q]);
+    }
+    PIRO_CHECK_HIP(hipMemcpyAsync(pl.perm, perm, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    PIRO_CHECK_HIP(hipEventRecord(ctx->lu_perm_ev, s));
     return pl;
 }
 

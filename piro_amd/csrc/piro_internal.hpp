@@ -95,6 +95,12 @@ struct piro_ctx {
     int live_children = 0;
     // LU panels try the exchange-free columns first (same factors either way)
     int lu_speculate = 1;
+    // the LU solves' pinned host buffers (pivots, then the row permutation;
+    // 2 lu_host_n ints) and the events after the pivots' copy to the host
+    // and after the permutation's copy back
+    int32_t *lu_host = nullptr;
+    int64_t lu_host_n = 0;
+    hipEvent_t lu_piv_ev = nullptr, lu_perm_ev = nullptr;
 };
 
 namespace piro {
