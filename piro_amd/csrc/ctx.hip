@@ -51,6 +51,7 @@ int piro_ctx_destroy(piro_ctx *ctx)
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->lu_host) (void)hipHostFree(ctx->lu_host);
+    if (ctx->host_read) (void)hipHostFree(ctx->host_read);
     if (ctx->lu_piv_ev) (void)hipEventDestroy(ctx->lu_piv_ev);
     if (ctx->lu_perm_ev) (void)hipEventDestroy(ctx->lu_perm_ev);
     delete ctx;

@@ -98,7 +98,8 @@ __global__ __launch_bounds__(256) void k_copy(int64_t n, const float *__restrict
 // least N/256 + 2 doubles) and stage (JLI_STAGE doubles)
 void jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row_ptr, const int32_t *d_col,
                const float *d_val, int64_t nnz, float dt, const float *d_b, float urf, float tol, int max_iter,
-               float *d_x, float *d_work, int *iters, float *res, JliState *st, double *part, double *stage)
+               float *d_x, float *d_work, int *iters, float *res, JliState *st, double *part, double *stage,
+               JliState *h_read)
 {
     const int64_t N = (int64_t)nx * ny * nz;
     k_jli_init<<<1, 1, 0, ctx->stream>>>(st);
@@ -108,7 +109,10 @@ void jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32_t *d_row_ptr, 
     check_launch();
     const float sqrt_n = sqrtf((float)N);
     float *bufs[2] = {d_x, d_work};
-    JliState h{};
+    // the state's read-backs: into h_read (pinned) when given, else the stack
+    JliState h_local{};
+    JliState &h = h_read ? *h_read : h_local;
+    h = JliState{};
     for (int it = 0; it < max_iter; ++it) {
         const float *src = bufs[it & 1];
         float *dst = bufs[(it + 1) & 1];
@@ -188,8 +192,9 @@ extern "C" int piro_jli_solve(piro_ctx *ctx, int nx, int ny, int nz, const int32
     JliState *st = (JliState *)ctx->scratch[0].get(sizeof(JliState));
     double *part = (double *)ctx->scratch[2].get((size_t)(N / 256 + 2 + JLI_STAGE) * sizeof(double));
     double *stage = part + (N / 256 + 2);
+    static_assert(sizeof(JliState) <= kHostReadBytes, "read-back");
     jli_solve(ctx, nx, ny, nz, d_row_ptr, d_col, d_val, nnz, dt, d_b, urf, tol, max_iter, d_x, d_work, iters, res,
-              st, part, stage);
+              st, part, stage, (JliState *)host_read(ctx));
     PIRO_API_END
 }
 
@@ -208,7 +213,8 @@ extern "C" int piro_jli_solve_noalloc(piro_ctx *ctx, int nx, int ny, int nz, con
     JliState *st = (JliState *)base;
     double *part = (double *)(base + 256);
     double *stage = part + JLI_NOALLOC_PARTS;
+    // (no pinned read-back: it would be allocated at first use)
     jli_solve(ctx, nx, ny, nz, d_row_ptr, d_col, d_val, nnz, dt, d_b, urf, tol, max_iter, d_x, d_work, iters, res,
-              st, part, stage);
+              st, part, stage, nullptr);
     PIRO_API_END
 }

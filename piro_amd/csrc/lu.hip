@@ -2969,8 +2969,8 @@ int piro_lu_factor_f32(piro_ctx *ctx, int n, float *d_a, int lda, int32_t *d_ipi
         PIRO_CHECK_HIP(hipMemcpy2DAsync(keep, (size_t)n * sizeof(float), d_a, (size_t)lda * sizeof(float),
                                         (size_t)n * sizeof(float), n, hipMemcpyDeviceToDevice, s));
         factor_spec(s, n, d_a, lda, d_ipiv, ws);
-        int head[33];  // info, then the flag at int 32
-        PIRO_CHECK_HIP(hipMemcpyAsync(head, ws.info, sizeof(head), hipMemcpyDeviceToHost, s));
+        int *head = (int *)host_read(ctx);  // info, then the flag at int 32 (pinned)
+        PIRO_CHECK_HIP(hipMemcpyAsync(head, ws.info, 33 * sizeof(int), hipMemcpyDeviceToHost, s));
         PIRO_CHECK_HIP(hipStreamSynchronize(s));
         h = head[0];
         if (head[32]) {  // raised: restore and factor by the pivoting steps
@@ -3032,7 +3032,11 @@ int piro_lu_solve_refine(piro_ctx *ctx, int n, const float *d_a, int lda, const 
     solve_f32(ctx, pl, d_lu, ldlu, nrhs, r32, nrhs);
     k_f32_to_x<<<grid_for(nn, 256), 256, 0, s>>>(n, nrhs, r32, d_x, ldx);
     check_launch();
-    std::vector<double> h(2 * nrhs);
+    // the norms and the solves' error word come back together, into pinned
+    // memory, behind one wait
+    static_assert(64 * sizeof(double) + sizeof(int) <= kHostReadBytes, "read-back");
+    double *h = (double *)host_read(ctx);
+    int *herr = (int *)(h + 64);
     double worst = 0.0;
     int sw = 0;
     for (;;) {
@@ -3043,8 +3047,10 @@ int piro_lu_solve_refine(piro_ctx *ctx, int n, const float *d_a, int lda, const 
         check_launch();
         k_colnorms<<<nrhs, 256, 0, s>>>(n, nrhs, r, d_b, ldb, norms);
         check_launch();
-        PIRO_CHECK_HIP(hipMemcpyAsync(h.data(), norms, 2 * nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
-        if (solve_status(ctx, pl)) throw Error(PIRO_EINTERNAL, "triangular solve: a block wait exceeded its bound");
+        PIRO_CHECK_HIP(hipMemcpyAsync(h, norms, 2 * nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
+        PIRO_CHECK_HIP(hipMemcpyAsync(herr, pl.sync + 18, sizeof(int), hipMemcpyDeviceToHost, s));
+        PIRO_CHECK_HIP(hipStreamSynchronize(s));
+        if (*herr) throw Error(PIRO_EINTERNAL, "triangular solve: a block wait exceeded its bound");
         worst = 0.0;
         for (int c = 0; c < nrhs; ++c) {
             double rr = std::sqrt(h[2 * c]), bb = std::sqrt(h[2 * c + 1]);

@@ -100,10 +100,23 @@ struct piro_ctx {
     // and after the permutation's copy back
     int32_t *lu_host = nullptr;
     int64_t lu_host_n = 0;
+    // pinned host memory for the small read-backs of the synchronous calls
+    // (a factorisation's info and flag, the refinement's norms and error
+    // word, the JLI state), one call at a time; made at first use
+    void *host_read = nullptr;
     hipEvent_t lu_piv_ev = nullptr, lu_perm_ev = nullptr;
 };
 
 namespace piro {
+
+constexpr size_t kHostReadBytes = 1024;
+
+// the context's pinned read-back buffer (kHostReadBytes)
+inline void *host_read(piro_ctx *c)
+{
+    if (!c->host_read) PIRO_CHECK_HIP(hipHostMalloc(&c->host_read, kHostReadBytes, hipHostMallocDefault));
+    return c->host_read;
+}
 
 inline piro_ctx *ctx_of(piro_ctx *c) {
     if (!c) throw Error(PIRO_EINVAL, "null context");
