@@ -2706,10 +2706,10 @@ void solve_f32(piro_ctx *ctx, const SolvePlan &pl, const float *lu, int lda, int
 // 0 if no wait of the plan's solves ran past its bound (synchronises)
 int solve_status(piro_ctx *ctx, const SolvePlan &pl)
 {
-    int h = 0;
-    PIRO_CHECK_HIP(hipMemcpyAsync(&h, pl.sync + 18, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    int *h = (int *)host_read(ctx);  // pinned: the copy does not wait by itself
+    PIRO_CHECK_HIP(hipMemcpyAsync(h, pl.sync + 18, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     PIRO_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-    return h;
+    return *h;
 }
 
 // The factorisation's workspace: info (int 0), the speculative panels' flag
